@@ -1,0 +1,117 @@
+/* ORACLE — test infrastructure only.
+ *
+ * C ABI of the CPU restatement of NexDynIPC's contact stage. Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library, and
+ * only as the checker. The scene descriptor mirrors the product's
+ * ndipc_scene_desc (include/ndipc.h) field for field so one set of host arrays
+ * drives both.
+ */
+#ifndef NDIPC_ORACLE_H
+#define NDIPC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ndo_scene_desc {
+    int32_t n_scenes;
+    int32_t n_bodies;
+    const int32_t* scene_body_off;   /* [n_scenes+1] */
+    const int32_t* body_id;          /* [n_bodies] RigidBody::id */
+    const int32_t* vert_off;         /* [n_bodies] first vertex (in vertices) */
+    const int32_t* n_verts;          /* [n_bodies] 0: no MeshShape */
+    const int32_t* face_off;         /* [n_bodies] first face */
+    const int32_t* n_faces;          /* [n_bodies] */
+    const double* verts;             /* [total_v*3] body frame */
+    const int32_t* faces;            /* [total_f*3] body-local vertex ids */
+    const double* mass;              /* [n_bodies] */
+    const double* inertia;           /* [n_bodies*9] row-major body-frame inertia */
+    int32_t n_joints;
+    const int32_t* scene_joint_off;  /* [n_scenes+1] */
+    const int32_t* joint_type;       /* [n_joints] 0 FixedJoint, 1 HingeJoint */
+    const int32_t* joint_body;       /* [n_joints*2] scene-local world indices (b = -1 for fixed) */
+    const double* joint_data;        /* [n_joints*16] fixed: p(3) q(w,x,y,z); hinge: anchorA anchorB axisA axisB */
+    const double* joint_mu;          /* [n_joints] initial ALM stiffness */
+    int32_t n_allowed;
+    const int32_t* scene_allowed_off;/* [n_scenes+1] */
+    const int32_t* allowed_pairs;    /* [n_allowed*2] body ids */
+    const double* allowed_mu;        /* [n_allowed] friction override, < 0: none */
+} ndo_scene_desc;
+
+typedef struct ndo_params {
+    double dhat;                 /* setContactDistanceThreshold */
+    double kappa;                /* setContactStiffness (applied if > 0) */
+    double friction_mu;          /* setFrictionCoefficient */
+    int32_t enable_friction;
+    int32_t enable_ccd;
+    int32_t enable_adaptive_barrier;
+    int32_t alm_max_iters;
+    double alm_constraint_tol;
+    double alm_dual_tol;
+    double alm_contact_tol;
+    double alm_hardening_trigger;
+    double alm_hardening_ratio;
+    double ccd_safety_factor;
+} ndo_params;
+
+void* ndo_create(const ndo_scene_desc* d, int32_t scene);
+void ndo_destroy(void* h);
+int32_t ndo_configure(void* h, const ndo_params* p);
+int32_t ndo_n_bodies(void* h);
+/* state arrays are scene-local: p[n*3] q[n*4](w,x,y,z) v[n*3] w[n*3] al[n*3] aa[n*3] */
+int32_t ndo_set_state(void* h, const double* p, const double* q, const double* v,
+                      const double* w, const double* al, const double* aa);
+int32_t ndo_get_state(void* h, double* p, double* q, double* v, double* w, double* al, double* aa);
+int32_t ndo_get_barrier(void* h, double* kappa, double* dhat, int32_t* initialized);
+int32_t ndo_set_barrier(void* h, double kappa, double dhat, int32_t initialized);
+int32_t ndo_get_joint_state(void* h, double* lambda6, double* mu);
+int32_t ndo_step(void* h, double dt);
+/* step statistics of the last step: [alm_iters, newton_iters_total, n_contacts_last,
+ * candidates_fv, candidates_ee, narrow_evals, kappa_updated, toi] */
+int32_t ndo_last_stats(void* h, double* out8);
+
+/* Function-level entry points, evaluated at the current state. */
+int32_t ndo_detect_body_pairs(void* h, double inflation, int32_t* out_pairs, int32_t cap, int32_t* n);
+int32_t ndo_min_body_pair_distance(void* h, double inflation, double* out);
+/* counts[4] = vv, ev, ee, fv; lists are written if non-null (cap entries each). */
+int32_t ndo_build_candidates(void* h, double dhat, int32_t* counts, int32_t* vv, int32_t* ev,
+                             int32_t* ee, int32_t* fv, int32_t cap);
+/* top-K selection of NexDynIPC's select_top_k_indices on given scores */
+int32_t ndo_select_top_k(const double* scores, int32_t n, int32_t k, int32_t* out);
+/* narrow phase on the candidate set built at the current state; x == NULL -> detectContacts,
+ * else detectContactsAtState(x). ints[5*i] = bodyA, bodyB, type, primA, primB;
+ * dbl[7*i] = point(3), normal(3), distance. */
+int32_t ndo_narrow_phase(void* h, const double* x, int32_t cap, int32_t* n, int32_t* ints, double* dbl);
+int32_t ndo_scene_min_distance(void* h, double* out);
+/* Forms at a given x with given contacts; hess (6n x 6n dense, row-major) may be NULL. */
+int32_t ndo_barrier_eval(void* h, int32_t nc, const int32_t* ints, const double* dbl, const double* x,
+                         double dhat, double kappa, double* value, double* grad, double* hess,
+                         double* normal_forces);
+int32_t ndo_friction_eval(void* h, int32_t nc, const int32_t* ints, const double* dbl,
+                          const double* normal_forces, const double* x, double mu, double eps,
+                          double dt, double* value, double* grad, double* hess);
+int32_t ndo_friction_potential(const double* u, double N, double mu, double eps, double* value,
+                               double* grad3, double* hess9);
+int32_t ndo_barrier_potential(double d, double dhat, double kappa, double* out3);
+
+/* CCD (TimeOfImpact.cpp): poses p(3) q(4, w x y z) at t0 and t1 for bodies A and B.
+ * kind 0 = edge-vertex (a: vertex, b: edge v0 v1), 1 = edge-edge (a: e0 e1, b: e0 e1),
+ * 2 = face-vertex (a: vertex, b: face v0 v1 v2). Returns 1 on impact, 0 otherwise,
+ * -1 on a NaN interval. */
+int32_t ndo_ccd_toi(int32_t kind, const double* poseA, const double* poseB,
+                    const double* a_pts, const double* b_pts, double earliest, double tol,
+                    double* toi, int32_t* pops);
+int32_t ndo_ccd_earliest_toi(void* h, double dt, double* out);
+/* Interval root finder known-answer tests from tests/physics/test_ccd.cpp:79-125 */
+int32_t ndo_root_finder_kat(int32_t which, double* lo, double* hi);
+/* Directed-rounding primitives, exposed so tests can compare them with a device build. */
+void ndo_round_ops(const double* a, const double* b, int32_t n, double* out6);
+
+/* Newton system at x (SimulationProblem with the current contact forms): */
+int32_t ndo_objective(void* h, const double* x, double* value, double* grad, double* hess);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
