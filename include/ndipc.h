@@ -1,0 +1,179 @@
+/*
+ * ndipc — MI355X (gfx950) implementation of NexDynIPC's contact stage of the
+ * implicit-Euler Newton step. C ABI: plain pointers and sizes, int32 status
+ * (0 = ok, < 0 = error, message via ndipc_last_error), FP64 everywhere.
+ *
+ * Data layout of the host arrays passed in:
+ *   - bodies of all scenes are concatenated; scene s owns bodies
+ *     [scene_body_off[s], scene_body_off[s+1]); scene-local index = global - off;
+ *   - poses: p[3n], q[4n] as (w, x, y, z), velocities v[3n], w[3n];
+ *   - the Newton variable of a step is x[6n] = [p, theta] per body, theta the
+ *     incremental rotation vector (nexdynipc/src/Dynamics/IPCSolver.cpp:846-857).
+ *   - contacts: ints[5] = (bodyA, bodyB, type, primA, primB) with scene-local body
+ *     indices, dbl[7] = (point xyz, normal xyz, distance) — the fields of
+ *     Dynamics::ContactPair (nexdynipc/include/Dynamics/Forms/FrictionForm.h:13-22).
+ *
+ * One context per host thread; every call is ordered on the context's HIP stream.
+ */
+#ifndef NDIPC_H
+#define NDIPC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ndipc_ctx ndipc_ctx;
+
+/* Scene batch descriptor. Mirrors World + RigidBody + MeshShape + joints of the
+ * reference (nexdynipc/include/Dynamics/World.h:11-36, RigidBody.h:11-31,
+ * MeshShape.h, Joints/FixedJoint.h, Joints/HingeJoint.h) and the allowed-pair filter
+ * of IPCSolver::addAllowedContactPair (IPCSolver.cpp:275-289). */
+typedef struct ndipc_scene_desc {
+    int32_t n_scenes;
+    int32_t n_bodies;
+    const int32_t* scene_body_off;   /* [n_scenes+1] */
+    const int32_t* body_id;          /* [n_bodies] RigidBody::id */
+    const int32_t* vert_off;         /* [n_bodies] first vertex of the body's mesh */
+    const int32_t* n_verts;          /* [n_bodies] 0: body has no MeshShape */
+    const int32_t* face_off;         /* [n_bodies] first face */
+    const int32_t* n_faces;          /* [n_bodies] */
+    const double* verts;             /* [total_v*3] body-frame vertices */
+    const int32_t* faces;            /* [total_f*3] body-local vertex ids */
+    const double* mass;              /* [n_bodies] */
+    const double* inertia;           /* [n_bodies*9] body-frame inertia, row-major */
+    int32_t n_joints;
+    const int32_t* scene_joint_off;  /* [n_scenes+1] */
+    const int32_t* joint_type;       /* [n_joints] 0 FixedJoint, 1 HingeJoint */
+    const int32_t* joint_body;       /* [n_joints*2] scene-local bodies (second = -1 for fixed) */
+    const double* joint_data;        /* [n_joints*16] fixed: p(3) q(w,x,y,z); hinge: anchorA anchorB axisA axisB */
+    const double* joint_mu;          /* [n_joints] initial ALM stiffness (Joint.h:44 default 1e8) */
+    int32_t n_allowed;
+    const int32_t* scene_allowed_off;/* [n_scenes+1] */
+    const int32_t* allowed_pairs;    /* [n_allowed*2] body ids */
+    const double* allowed_mu;        /* [n_allowed] friction override, < 0: none */
+} ndipc_scene_desc;
+
+/* IPCSolver settings (nexdynipc/include/Dynamics/IPCSolver.h:38-79,94-124). */
+typedef struct ndipc_params {
+    double dhat;                 /* setContactDistanceThreshold */
+    double kappa;                /* setContactStiffness, applied if > 0 */
+    double friction_mu;          /* setFrictionCoefficient (per-scene values: ndipc_set_scene_mu) */
+    int32_t enable_friction;
+    int32_t enable_ccd;
+    int32_t enable_adaptive_barrier;
+    int32_t alm_max_iters;
+    double alm_constraint_tol;
+    double alm_dual_tol;
+    double alm_contact_tol;
+    double alm_hardening_trigger;
+    double alm_hardening_ratio;
+    double ccd_safety_factor;
+} ndipc_params;
+
+/* ---- context and data ---------------------------------------------------------- */
+int32_t ndipc_create(int32_t device, ndipc_ctx** out);
+int32_t ndipc_destroy(ndipc_ctx* ctx);
+const char* ndipc_last_error(const ndipc_ctx* ctx);
+/* Stream the context launches on (hipStream_t). */
+void* ndipc_stream(ndipc_ctx* ctx);
+
+/* Replaces World::addBody/addJoint + ContactAssembler::build's per-mesh copies
+ * (ContactAssembler.cpp:124-154): meshes, unique edges and local AABBs are uploaded
+ * once and stay resident. */
+int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* desc);
+/* Same settings for every scene (IPCSolver setters). */
+int32_t ndipc_configure(ndipc_ctx* ctx, const ndipc_params* params);
+/* Per-scene friction coefficients (setFrictionCoefficient per scene). */
+int32_t ndipc_set_scene_mu(ndipc_ctx* ctx, const double* mu);
+/* RigidBody state; any pointer may be NULL (left unchanged / not read back). */
+int32_t ndipc_set_state(ndipc_ctx* ctx, const double* p, const double* q, const double* v, const double* w,
+                        const double* a_lin, const double* a_ang);
+int32_t ndipc_get_state(ndipc_ctx* ctx, double* p, double* q, double* v, double* w, double* a_lin,
+                        double* a_ang);
+/* AdaptiveBarrier state per scene (AdaptiveBarrier.h:22-87) and the once-per-solver
+ * initialisation flag (IPCSolver.cpp:817-822). */
+int32_t ndipc_get_barrier(ndipc_ctx* ctx, double* kappa, double* dhat, int32_t* initialized);
+int32_t ndipc_set_barrier(ndipc_ctx* ctx, const double* kappa, const double* dhat, const int32_t* initialized);
+/* Joint multipliers (6 per joint) and ALM stiffness (Joint.h:113-115). */
+int32_t ndipc_get_joint_state(ndipc_ctx* ctx, double* lambda6, double* mu);
+
+/* ---- whole step ---------------------------------------------------------------- */
+/* IPCSolver::step(World&, dt) for every scene (IPCSolver.cpp:808-1231), n_steps
+ * times; dt[s] per scene. Runs entirely on the device; no host synchronisation. */
+int32_t ndipc_step(ndipc_ctx* ctx, const double* dt, int32_t n_steps);
+/* Per-scene statistics of the last step, 8 doubles: alm iterations, newton
+ * iterations, contacts after the last narrow phase, FV candidates, EE candidates,
+ * narrow-phase candidate evaluations, kappa doubled (0/1), earliest TOI. */
+int32_t ndipc_last_stats(ndipc_ctx* ctx, double* out8);
+/* Narrow-phase candidate evaluations (FV + EE, VV when the fallback runs) per scene
+ * since upload: the "contact pairs" of the throughput metric. */
+int32_t ndipc_eval_counter(ndipc_ctx* ctx, double* out);
+/* Close body pairs (over all scenes) of the last candidate build. */
+int32_t ndipc_pair_count(ndipc_ctx* ctx, int32_t* out);
+
+/* ---- function-level entry points (current state) ------------------------------ */
+/* ContactAssembler::detectBodyPairs (ContactAssembler.cpp:156-178) via a device LBVH.
+ * Pairs (i<j, scene-local) of scene `scene`, sorted by (i, j). */
+int32_t ndipc_detect_body_pairs(ndipc_ctx* ctx, int32_t scene, double inflation, int32_t* out_pairs,
+                                int32_t cap, int32_t* n_pairs);
+/* ContactAssembler::estimateMinBodyPairDistance (ContactAssembler.cpp:180-212). */
+int32_t ndipc_min_body_pair_distance(ndipc_ctx* ctx, int32_t scene, double inflation, double* out);
+/* ContactAssembler::buildCandidateSet (ContactAssembler.cpp:214-343): the candidate
+ * tuples of scene `scene` in the reference's struct layouts (VV 4, EV 5, EE 6, FV 4
+ * ints, body ids and body-local primitive ids). counts[4] = vv, ev, ee, fv. The set
+ * equals the reference's; the order is pair-major by (i, j), then by (score, index). */
+int32_t ndipc_build_candidates(ndipc_ctx* ctx, int32_t scene, double dhat, int32_t* counts, int32_t* vv,
+                               int32_t* ev, int32_t* ee, int32_t* fv, int32_t cap);
+/* select_top_k_indices (ContactAssembler.cpp:67-88) on given scores: the selected
+ * set of std::nth_element (libstdc++ introselect) sorted by (score, index). */
+int32_t ndipc_select_top_k(ndipc_ctx* ctx, const double* scores, int32_t n, int32_t k, int32_t* out);
+/* IPCSolver::detectContacts (x == NULL, IPCSolver.cpp:394-580) and
+ * detectContactsAtState (x = Newton variable of scene, IPCSolver.cpp:582-806) on the
+ * candidates of the last ndipc_build_candidates of that scene. */
+int32_t ndipc_narrow_phase(ndipc_ctx* ctx, int32_t scene, const double* x, int32_t cap, int32_t* n,
+                           int32_t* ints, double* dbl);
+/* computeSceneMinDistance(...) < 1e-9 (AdaptiveBarrier.h:130-162, IPCSolver.cpp:826-830):
+ * the only use of the scene minimum distance; exact pruning by body AABB gap. */
+int32_t ndipc_scene_min_distance_flag(ndipc_ctx* ctx, int32_t scene, int32_t* out_flag);
+/* DistanceBarrierForm::value/gradient/hessian/computeNormalForces
+ * (DistanceBarrierForm.cpp:11-87) at x for given contacts. hess: dense 6n x 6n
+ * row-major (may be NULL). */
+int32_t ndipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int32_t* ints, const double* dbl,
+                           const double* x, double dhat, double kappa, double* value, double* grad, double* hess,
+                           double* normal_forces);
+/* FrictionForm::value/gradient/hessian (FrictionForm.cpp:22-151) with lagged normal
+ * forces. */
+int32_t ndipc_friction_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int32_t* ints, const double* dbl,
+                            const double* normal_forces, const double* x, double mu, double eps, double dt,
+                            double* value, double* grad, double* hess);
+/* Interval TOI batch (TimeOfImpact.cpp:8-150): kind 0 edge-vertex, 1 edge-edge,
+ * 2 face-vertex. Per query: poseA[14], poseB[14] = p0(3) q0(4) p1(3) q1(4);
+ * a_pts[6] (vertex or edge), b_pts[9] (edge or face). out: hit (1/0, -1 on NaN),
+ * toi, pops. */
+int32_t ndipc_ccd_toi_batch(ndipc_ctx* ctx, int32_t kind, int32_t n, const double* poseA, const double* poseB,
+                            const double* a_pts, const double* b_pts, const double* earliest, double tol,
+                            int32_t* hit, double* toi, int32_t* pops);
+/* CCDSystem::computeEarliestTOI over the FV candidates of the last candidate build
+ * (CCD.h:22-82), rotation frozen, t1 = p + v dt. */
+int32_t ndipc_ccd_earliest_toi(ndipc_ctx* ctx, int32_t scene, double dt, double* out);
+/* Block-Jacobi PCG on a block-sparse SPD system: n_blocks 6x6 diagonal blocks,
+ * n_off off-diagonal blocks (row block, col block) stored once for the upper
+ * triangle. Solves A x = b; returns iterations and final relative residual. */
+int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t n_blocks, const double* diag, int32_t n_off, const int32_t* off_ij,
+                        const double* off_blocks, const double* b, double tol, int32_t max_it, double* x,
+                        int32_t* iters, double* rel_res);
+
+/* ---- diagnostics ---------------------------------------------------------------- */
+/* Average device duration (ms) and count of launches of the named kernel since the
+ * last reset, measured with HIP events on the context stream while enabled. */
+int32_t ndipc_timing_enable(ndipc_ctx* ctx, int32_t on);
+int32_t ndipc_timing_get(ndipc_ctx* ctx, const char* kernel, double* avg_ms, int32_t* count);
+/* Directed-rounding primitives of the interval CCD, evaluated on the device:
+ * out6 = add_dn, add_up, sub_dn, sub_up, mul_dn, mul_up. */
+int32_t ndipc_round_ops(ndipc_ctx* ctx, const double* a, const double* b, int32_t n, double* out6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

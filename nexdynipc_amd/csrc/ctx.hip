@@ -1,0 +1,997 @@
+// ndipc C ABI: context, scene upload, step orchestration and the function-level entry
+// points. The host side only stages data and enqueues kernels on the context stream;
+// a step (IPCSolver::step) is a fixed kernel sequence whose control decisions
+// (Newton convergence, line search, ALM break) are made on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/ndipc.h"
+#include "ndipc_internal.h"
+
+using nd::Dev;
+using nd::SceneParams;
+using nd::Work;
+
+namespace nd {
+void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, hipStream_t st);
+}
+
+struct TimingRec {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    hipEvent_t open = nullptr;
+};
+
+struct ndipc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool uploaded = false;
+    Dev d{};
+    Work w{};
+    int S = 0, N = 0, J = 0;
+    std::vector<int> body_off, nv, nf, ne, e_off, v_off, f_off, body_id, joint_off;
+    std::vector<int> edges_host;
+    long long pair_cap = 0;
+    std::vector<SceneParams> sp;
+    std::vector<void*> allocs;
+    double* d_dt = nullptr;
+    double* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    bool timing = false;
+    std::map<std::string, TimingRec> tim;
+};
+
+namespace {
+ndipc_ctx* g_timing_ctx = nullptr;  // context whose stream is being timed (one at a time)
+
+int fail(ndipc_ctx* c, const std::string& m) {
+    if (c) c->err = m;
+    return -1;
+}
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(ctx, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+T* dalloc(ndipc_ctx* ctx, size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return nullptr;
+    ctx->allocs.push_back(p);
+    return static_cast<T*>(p);
+}
+template <typename T>
+T* dupload(ndipc_ctx* ctx, const T* h, size_t n) {
+    T* p = dalloc<T>(ctx, n);
+    if (p && h && n) hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice);
+    return p;
+}
+
+void free_all(ndipc_ctx* ctx) {
+    for (void* p : ctx->allocs) hipFree(p);
+    ctx->allocs.clear();
+    ctx->uploaded = false;
+}
+
+int sync_sp_to_dev(ndipc_ctx* ctx) {
+    HIPCHK(hipMemcpyAsync(ctx->d.sp, ctx->sp.data(), sizeof(SceneParams) * ctx->S, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+int sync_sp_from_dev(ndipc_ctx* ctx) {
+    HIPCHK(hipMemcpyAsync(ctx->sp.data(), ctx->d.sp, sizeof(SceneParams) * ctx->S, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// ContactAssembler.cpp:90-120 — unique edges in face order, stored (min, max).
+std::vector<int> unique_edges(const int* faces, int nf) {
+    std::unordered_set<uint64_t> seen;
+    std::vector<int> E;
+    for (int f = 0; f < nf; ++f) {
+        const int v[3] = {faces[3 * f], faces[3 * f + 1], faces[3 * f + 2]};
+        for (int k = 0; k < 3; ++k) {
+            const int a = v[k], b = v[(k + 1) % 3];
+            const int lo = std::min(a, b), hi = std::max(a, b);
+            const uint64_t key = (uint64_t(uint32_t(lo)) << 32) | uint32_t(hi);
+            if (seen.insert(key).second) {
+                E.push_back(lo);
+                E.push_back(hi);
+            }
+        }
+    }
+    return E;
+}
+
+void default_params(SceneParams& p) {
+    p.dhat = 1e-3;
+    p.kappa = 0.0;
+    p.mu = 0.0;
+    p.ccd_safety = 0.8;
+    p.alm_ctol = 1e-4;
+    p.alm_dtol = 1e-3;
+    p.alm_ctcol = 1e-3;
+    p.alm_trig = 2.0;
+    p.alm_ratio = 2.0;
+    p.enable_friction = 0;
+    p.enable_ccd = 1;
+    p.enable_adaptive = 1;
+    p.alm_max_iters = 10;
+    p.barrier_initialized = 0;
+}
+
+int check_ctx(ndipc_ctx* ctx) {
+    if (!ctx) return -1;
+    if (!ctx->uploaded) return fail(ctx, "no scenes uploaded");
+    HIPCHK(hipSetDevice(ctx->device));
+    return 0;
+}
+
+int max_grid_pairs(ndipc_ctx* ctx) {
+    return int(std::min<long long>(ctx->pair_cap, 1 << 30));
+}
+}  // namespace
+
+// Kernel timing with HIP events on the launching stream. Events come from a pool that
+// is created once per context, so recording adds no allocation to the timed region.
+namespace {
+std::vector<hipEvent_t> g_event_pool;
+size_t g_event_next = 0;
+hipEvent_t pooled_event() {
+    if (g_event_next == g_event_pool.size()) {
+        for (int i = 0; i < 4096; ++i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) break;
+            g_event_pool.push_back(e);
+        }
+    }
+    return g_event_next < g_event_pool.size() ? g_event_pool[g_event_next++] : nullptr;
+}
+}  // namespace
+
+namespace nd {
+void timing_begin(const char* name, hipStream_t st) {
+    ndipc_ctx* c = g_timing_ctx;
+    if (!c || !c->timing || st != c->stream) return;
+    hipEvent_t e = pooled_event();
+    if (!e) return;
+    (void)hipEventRecord(e, st);
+    c->tim[name].open = e;
+}
+void timing_end(const char* name, hipStream_t st) {
+    ndipc_ctx* c = g_timing_ctx;
+    if (!c || !c->timing || st != c->stream) return;
+    TimingRec& r = c->tim[name];
+    if (!r.open) return;
+    hipEvent_t e = pooled_event();
+    if (!e) return;
+    (void)hipEventRecord(e, st);
+    r.ev.emplace_back(r.open, e);
+    r.open = nullptr;
+}
+}  // namespace nd
+
+extern "C" {
+
+int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
+    if (!out) return -1;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device) return -2;
+    if (hipSetDevice(device) != hipSuccess) return -3;
+    auto* c = new ndipc_ctx();
+    c->device = device;
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return -4;
+    }
+    *out = c;
+    return 0;
+}
+
+int32_t ndipc_destroy(ndipc_ctx* ctx) {
+    if (!ctx) return 0;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    if (g_timing_ctx == ctx) g_timing_ctx = nullptr;
+    free_all(ctx);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+const char* ndipc_last_error(const ndipc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void* ndipc_stream(ndipc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
+    if (!ctx || !dsc) return -1;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    free_all(ctx);
+    const int S = dsc->n_scenes, N = dsc->n_bodies;
+    if (S <= 0 || N < 0) return fail(ctx, "empty scene batch");
+    if (dsc->scene_body_off[0] != 0 || dsc->scene_body_off[S] != N) return fail(ctx, "scene_body_off inconsistent");
+    ctx->S = S;
+    ctx->N = N;
+    ctx->body_off.assign(dsc->scene_body_off, dsc->scene_body_off + S + 1);
+    ctx->nv.assign(dsc->n_verts, dsc->n_verts + N);
+    ctx->nf.assign(dsc->n_faces, dsc->n_faces + N);
+    ctx->v_off.assign(dsc->vert_off, dsc->vert_off + N);
+    ctx->f_off.assign(dsc->face_off, dsc->face_off + N);
+    ctx->body_id.assign(dsc->body_id, dsc->body_id + N);
+    int total_v = 0, total_f = 0;
+    for (int b = 0; b < N; ++b) {
+        total_v = std::max(total_v, ctx->v_off[b] + ctx->nv[b]);
+        total_f = std::max(total_f, ctx->f_off[b] + ctx->nf[b]);
+    }
+    // per-body scene, local AABB, max edge, unique edges (per distinct mesh)
+    std::vector<int> body_scene(N);
+    std::vector<double> lbox(6 * size_t(N), 0.0), max_edge(N, 0.0);
+    ctx->e_off.assign(N, 0);
+    ctx->ne.assign(N, 0);
+    ctx->edges_host.clear();
+    std::map<std::pair<int, int>, std::pair<int, int>> edge_cache;
+    long long pair_cap = 0;
+    for (int s = 0; s < S; ++s) {
+        const int b0 = ctx->body_off[s], b1 = ctx->body_off[s + 1];
+        if (b1 - b0 > nd::kMaxSceneBodies) return fail(ctx, "scene has more than 4096 bodies");
+        long long m = 0;
+        for (int b = b0; b < b1; ++b) {
+            body_scene[b] = s;
+            if (ctx->nv[b] > 0) ++m;
+        }
+        pair_cap += m * (m - 1) / 2;
+    }
+    for (int b = 0; b < N; ++b) {
+        const int nvb = ctx->nv[b], nfb = ctx->nf[b];
+        if (nvb == 0) continue;
+        if (nvb > nd::kMaxPrims || nfb > nd::kMaxPrims) return fail(ctx, "mesh exceeds 2048 vertices or faces");
+        const double* V = dsc->verts + 3 * size_t(ctx->v_off[b]);
+        double mn[3] = {V[0], V[1], V[2]}, mx[3] = {V[0], V[1], V[2]};
+        for (int v = 0; v < nvb; ++v)
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = std::min(mn[a], V[3 * v + a]);
+                mx[a] = std::max(mx[a], V[3 * v + a]);
+            }
+        for (int a = 0; a < 3; ++a) { lbox[6 * b + a] = mn[a]; lbox[6 * b + 3 + a] = mx[a]; }
+        const int* F = dsc->faces + 3 * size_t(ctx->f_off[b]);
+        double me = 0.0;
+        for (int f = 0; f < nfb; ++f)
+            for (int k = 0; k < 3; ++k) {
+                const int a = F[3 * f + k], c = F[3 * f + (k + 1) % 3];
+                if (a < 0 || a >= nvb || c < 0 || c >= nvb) return fail(ctx, "face index out of range");
+                const double dx = V[3 * a] - V[3 * c], dy = V[3 * a + 1] - V[3 * c + 1], dz = V[3 * a + 2] - V[3 * c + 2];
+                me = std::max(me, std::sqrt(dx * dx + dy * dy + dz * dz));
+            }
+        max_edge[b] = me;
+        const auto key = std::make_pair(ctx->f_off[b], nfb);
+        auto it = edge_cache.find(key);
+        if (it == edge_cache.end()) {
+            const std::vector<int> E = unique_edges(F, nfb);
+            if (int(E.size() / 2) > nd::kMaxPrims) return fail(ctx, "mesh exceeds 2048 edges");
+            const int off = int(ctx->edges_host.size() / 2);
+            ctx->edges_host.insert(ctx->edges_host.end(), E.begin(), E.end());
+            it = edge_cache.emplace(key, std::make_pair(off, int(E.size() / 2))).first;
+        }
+        ctx->e_off[b] = it->second.first;
+        ctx->ne[b] = it->second.second;
+    }
+    // joints
+    const int J = dsc->n_joints;
+    ctx->J = J;
+    std::vector<int> ja(std::max(J, 1), -1), jb(std::max(J, 1), -1), jt(std::max(J, 1), 0);
+    std::vector<int> joint_off(S + 1, 0);
+    std::vector<int> use_dense(S, 0);
+    for (int s = 0; s <= S; ++s) joint_off[s] = dsc->scene_joint_off ? dsc->scene_joint_off[s] : 0;
+    for (int s = 0; s < S; ++s) {
+        for (int j = joint_off[s]; j < joint_off[s + 1]; ++j) {
+            jt[j] = dsc->joint_type[j];
+            const int a = dsc->joint_body[2 * j], b = dsc->joint_body[2 * j + 1];
+            const int n = ctx->body_off[s + 1] - ctx->body_off[s];
+            if (a < 0 || a >= n) return fail(ctx, "joint body out of range");
+            ja[j] = ctx->body_off[s] + a;
+            if (jt[j] == 1) {
+                if (b < 0 || b >= n || b == a) return fail(ctx, "hinge joint needs two distinct bodies of its scene");
+                jb[j] = ctx->body_off[s] + b;
+                use_dense[s] = 1;
+            } else if (jt[j] != 0) {
+                return fail(ctx, "unsupported joint type");
+            }
+        }
+    }
+    ctx->joint_off = joint_off;
+    // allowed pairs: per-scene sorted keys
+    std::vector<int> allowed_off(S + 1, 0);
+    std::vector<unsigned long long> akeys;
+    std::vector<double> amu;
+    for (int s = 0; s < S; ++s) {
+        std::vector<std::pair<unsigned long long, double>> kv;
+        if (dsc->scene_allowed_off) {
+            for (int k = dsc->scene_allowed_off[s]; k < dsc->scene_allowed_off[s + 1]; ++k) {
+                const int a = dsc->allowed_pairs[2 * k], b = dsc->allowed_pairs[2 * k + 1];
+                const int lo = std::min(a, b), hi = std::max(a, b);
+                kv.emplace_back((unsigned long long)(uint32_t)lo << 32 | (uint32_t)hi, dsc->allowed_mu[k]);
+            }
+        }
+        std::sort(kv.begin(), kv.end(), [](auto& x, auto& y) { return x.first < y.first; });
+        // IPCSolver::addAllowedContactPair: later calls overwrite the override
+        std::vector<std::pair<unsigned long long, double>> u;
+        for (auto& e : kv) {
+            if (!u.empty() && u.back().first == e.first) {
+                if (e.second >= 0.0) u.back().second = e.second;
+            } else {
+                u.push_back(e);
+            }
+        }
+        for (auto& e : u) { akeys.push_back(e.first); amu.push_back(e.second); }
+        allowed_off[s + 1] = int(akeys.size());
+    }
+    // device scene data
+    Dev& d = ctx->d;
+    d = Dev{};
+    d.S = S;
+    d.N = N;
+    d.body_off = dupload(ctx, ctx->body_off.data(), S + 1);
+    d.body_scene = dupload(ctx, body_scene.data(), N);
+    d.body_id = dupload(ctx, ctx->body_id.data(), N);
+    d.v_off = dupload(ctx, ctx->v_off.data(), N);
+    d.nv = dupload(ctx, ctx->nv.data(), N);
+    d.f_off = dupload(ctx, ctx->f_off.data(), N);
+    d.nf = dupload(ctx, ctx->nf.data(), N);
+    d.e_off = dupload(ctx, ctx->e_off.data(), N);
+    d.ne = dupload(ctx, ctx->ne.data(), N);
+    d.verts = dupload(ctx, dsc->verts, 3 * size_t(total_v));
+    d.faces = dupload(ctx, dsc->faces, 3 * size_t(total_f));
+    d.edges = dupload(ctx, ctx->edges_host.data(), ctx->edges_host.size());
+    d.lbox = dupload(ctx, lbox.data(), lbox.size());
+    d.max_edge = dupload(ctx, max_edge.data(), N);
+    d.mass = dupload(ctx, dsc->mass, N);
+    d.inertia = dupload(ctx, dsc->inertia, 9 * size_t(N));
+    d.joint_off = dupload(ctx, joint_off.data(), S + 1);
+    d.jtype = dupload(ctx, jt.data(), jt.size());
+    d.ja = dupload(ctx, ja.data(), ja.size());
+    d.jb = dupload(ctx, jb.data(), jb.size());
+    std::vector<double> jdata(16 * size_t(std::max(J, 1)), 0.0), jmu(std::max(J, 1), 1e8);
+    if (J > 0) {
+        std::memcpy(jdata.data(), dsc->joint_data, sizeof(double) * 16 * J);
+        std::memcpy(jmu.data(), dsc->joint_mu, sizeof(double) * J);
+    }
+    d.jdata = dupload(ctx, jdata.data(), jdata.size());
+    d.jmu = dupload(ctx, jmu.data(), jmu.size());
+    d.jlambda = dalloc<double>(ctx, 6 * size_t(std::max(J, 1)));
+    d.jC = dalloc<double>(ctx, 6 * size_t(std::max(J, 1)));
+    d.allowed_off = dupload(ctx, allowed_off.data(), S + 1);
+    d.allowed_key = dupload(ctx, akeys.data(), akeys.size());
+    d.allowed_mu = dupload(ctx, amu.data(), amu.size());
+    d.p = dalloc<double>(ctx, 3 * size_t(N));
+    d.q = dalloc<double>(ctx, 4 * size_t(N));
+    d.v = dalloc<double>(ctx, 3 * size_t(N));
+    d.w = dalloc<double>(ctx, 3 * size_t(N));
+    d.al = dalloc<double>(ctx, 3 * size_t(N));
+    d.aa = dalloc<double>(ctx, 3 * size_t(N));
+    d.sp = dalloc<SceneParams>(ctx, S);
+    // per-scene prune margin for the scene-min-distance flag
+    ctx->sp.assign(S, SceneParams{});
+    for (int s = 0; s < S; ++s) {
+        default_params(ctx->sp[s]);
+        double me = 0.0;
+        for (int b = ctx->body_off[s]; b < ctx->body_off[s + 1]; ++b) me = std::max(me, max_edge[b]);
+        ctx->sp[s].prune_margin = 1e-8 + 4e-6 * me;
+        ctx->sp[s].use_dense = use_dense[s];
+    }
+    // work buffers
+    Work& w = ctx->w;
+    w = Work{};
+    const size_t PC = size_t(std::max<long long>(pair_cap, 1));
+    ctx->pair_cap = pair_cap;
+    w.wbox = dalloc<double>(ctx, 6 * size_t(N));
+    w.pose_p = dalloc<double>(ctx, 3 * size_t(N));
+    w.pose_q = dalloc<double>(ctx, 4 * size_t(N));
+    w.leaf_body = dalloc<int>(ctx, N);
+    w.node_child = dalloc<int>(ctx, 2 * size_t(N));
+    w.node_range = dalloc<int>(ctx, 2 * size_t(N));
+    w.node_box = dalloc<double>(ctx, 6 * size_t(N));
+    w.nleaves = dalloc<int>(ctx, S);
+    w.pair_cnt = dalloc<int>(ctx, S);
+    w.pair_off = dalloc<int>(ctx, S + 1);
+    w.pairs = dalloc<int>(ctx, 2 * PC);
+    w.pair_scene = dalloc<int>(ctx, PC);
+    w.n_pairs_total = dalloc<int>(ctx, 1);
+    w.prox_cnt = dalloc<int>(ctx, S);
+    w.prox_off = dalloc<int>(ctx, S + 1);
+    w.prox_pairs = dalloc<int>(ctx, 2 * PC);
+    w.prox_scene = dalloc<int>(ctx, PC);
+    w.n_prox_total = dalloc<int>(ctx, 1);
+    w.lists = dalloc<int>(ctx, PC * 6 * nd::kTopK);
+    w.lcnt = dalloc<int>(ctx, PC * 6);
+    w.hits = dalloc<unsigned long long>(ctx, PC * nd::kHitWords);
+    w.hit_fv = dalloc<int>(ctx, PC);
+    w.hit_ee = dalloc<int>(ctx, PC);
+    w.hit_off_fv = dalloc<int>(ctx, PC);
+    w.hit_off_ee = dalloc<int>(ctx, PC);
+    w.raw_cnt = dalloc<int>(ctx, S);
+    w.cint = dalloc<int>(ctx, size_t(S) * nd::kMaxRawContacts * 5);
+    w.cdbl = dalloc<double>(ctx, size_t(S) * nd::kMaxRawContacts * 7);
+    w.sort_idx = dalloc<int>(ctx, size_t(S) * 65536);
+    w.sort_key = dalloc<double>(ctx, size_t(S) * 65536);
+    const size_t V6 = 6 * size_t(N);
+    w.x = dalloc<double>(ctx, V6);
+    w.xhat = dalloc<double>(ctx, V6);
+    w.xprev = dalloc<double>(ctx, V6);
+    w.vprev = dalloc<double>(ctx, V6);
+    w.g = dalloc<double>(ctx, V6);
+    w.dx = dalloc<double>(ctx, V6);
+    w.xt = dalloc<double>(ctx, V6);
+    w.cg = dalloc<double>(ctx, V6);
+    w.r = dalloc<double>(ctx, V6);
+    w.z = dalloc<double>(ctx, V6);
+    w.pv = dalloc<double>(ctx, V6);
+    w.Ap = dalloc<double>(ctx, V6);
+    w.tmp = dalloc<double>(ctx, V6);
+    w.D = dalloc<double>(ctx, 36 * size_t(N));
+    w.Dinv = dalloc<double>(ctx, 36 * size_t(N));
+    w.nf = dalloc<double>(ctx, size_t(S) * nd::kMaxContacts);
+    w.cblk = dalloc<double>(ctx, size_t(S) * nd::kMaxContacts * 108);
+    w.cgrad = dalloc<double>(ctx, size_t(S) * nd::kMaxContacts * 18);
+    w.cactive = dalloc<int>(ctx, size_t(S) * nd::kMaxContacts);
+    w.csr_off = dalloc<int>(ctx, size_t(N) + S + 1);
+    w.csr_ent = dalloc<int>(ctx, size_t(S) * 2 * nd::kMaxContacts);
+    w.jblk = dalloc<double>(ctx, 108 * size_t(std::max(J, 1)));
+    w.jgrad = dalloc<double>(ctx, 12 * size_t(std::max(J, 1)));
+    std::vector<long long> doff(S, 0);
+    long long dtot = 0;
+    for (int s = 0; s < S; ++s) {
+        doff[s] = dtot;
+        if (use_dense[s]) {
+            const long long nd6 = 6LL * (ctx->body_off[s + 1] - ctx->body_off[s]);
+            if (nd6 > 3072) return fail(ctx, "hinge-jointed scene too large for the dense solve (> 512 bodies)");
+            dtot += nd6 * nd6;
+        }
+    }
+    w.dense = dalloc<double>(ctx, size_t(std::max<long long>(dtot, 1)));
+    w.dense_off = dupload(ctx, doff.data(), S);
+    w.flag = dalloc<int>(ctx, S);
+    ctx->d_dt = dalloc<double>(ctx, S);
+    ctx->scratch_bytes = 0;
+    ctx->d_scratch = nullptr;
+    for (void* p : ctx->allocs)
+        if (!p) return fail(ctx, "device allocation failed");
+    ctx->uploaded = true;
+    return sync_sp_to_dev(ctx);
+}
+
+int32_t ndipc_configure(ndipc_ctx* ctx, const ndipc_params* p) {
+    if (check_ctx(ctx)) return -1;
+    if (!p) return fail(ctx, "null params");
+    for (int s = 0; s < ctx->S; ++s) {
+        SceneParams& q = ctx->sp[s];
+        q.dhat = std::max(1e-9, p->dhat);
+        if (p->kappa > 0.0) q.kappa = p->kappa;
+        q.mu = p->friction_mu;
+        q.enable_friction = p->enable_friction != 0;
+        q.enable_ccd = p->enable_ccd != 0;
+        q.enable_adaptive = p->enable_adaptive_barrier != 0;
+        q.alm_max_iters = std::max(1, int(p->alm_max_iters));
+        q.alm_ctol = std::max(1e-12, p->alm_constraint_tol);
+        q.alm_dtol = std::max(1e-12, p->alm_dual_tol);
+        q.alm_ctcol = std::max(1e-12, p->alm_contact_tol);
+        q.alm_trig = std::max(1.0, p->alm_hardening_trigger);
+        q.alm_ratio = std::max(1.0, p->alm_hardening_ratio);
+        q.ccd_safety = p->ccd_safety_factor;
+        if (q.prune_margin > 4.0 * 1e-3 && q.enable_adaptive)
+            return fail(ctx, "longest face edge too long for the exact scene-min-distance pruning");
+    }
+    return sync_sp_to_dev(ctx);
+}
+
+int32_t ndipc_set_scene_mu(ndipc_ctx* ctx, const double* mu) {
+    if (check_ctx(ctx)) return -1;
+    if (sync_sp_from_dev(ctx)) return -1;
+    for (int s = 0; s < ctx->S; ++s) ctx->sp[s].mu = mu[s];
+    return sync_sp_to_dev(ctx);
+}
+
+int32_t ndipc_set_state(ndipc_ctx* ctx, const double* p, const double* q, const double* v, const double* w,
+                        const double* al, const double* aa) {
+    if (check_ctx(ctx)) return -1;
+    const size_t N = ctx->N;
+    if (p) HIPCHK(hipMemcpy(ctx->d.p, p, 3 * N * 8, hipMemcpyHostToDevice));
+    if (q) HIPCHK(hipMemcpy(ctx->d.q, q, 4 * N * 8, hipMemcpyHostToDevice));
+    if (v) HIPCHK(hipMemcpy(ctx->d.v, v, 3 * N * 8, hipMemcpyHostToDevice));
+    if (w) HIPCHK(hipMemcpy(ctx->d.w, w, 3 * N * 8, hipMemcpyHostToDevice));
+    if (al) HIPCHK(hipMemcpy(ctx->d.al, al, 3 * N * 8, hipMemcpyHostToDevice));
+    if (aa) HIPCHK(hipMemcpy(ctx->d.aa, aa, 3 * N * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int32_t ndipc_get_state(ndipc_ctx* ctx, double* p, double* q, double* v, double* w, double* al, double* aa) {
+    if (check_ctx(ctx)) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const size_t N = ctx->N;
+    if (p) HIPCHK(hipMemcpy(p, ctx->d.p, 3 * N * 8, hipMemcpyDeviceToHost));
+    if (q) HIPCHK(hipMemcpy(q, ctx->d.q, 4 * N * 8, hipMemcpyDeviceToHost));
+    if (v) HIPCHK(hipMemcpy(v, ctx->d.v, 3 * N * 8, hipMemcpyDeviceToHost));
+    if (w) HIPCHK(hipMemcpy(w, ctx->d.w, 3 * N * 8, hipMemcpyDeviceToHost));
+    if (al) HIPCHK(hipMemcpy(al, ctx->d.al, 3 * N * 8, hipMemcpyDeviceToHost));
+    if (aa) HIPCHK(hipMemcpy(aa, ctx->d.aa, 3 * N * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int32_t ndipc_get_barrier(ndipc_ctx* ctx, double* kappa, double* dhat, int32_t* init) {
+    if (check_ctx(ctx)) return -1;
+    if (sync_sp_from_dev(ctx)) return -1;
+    for (int s = 0; s < ctx->S; ++s) {
+        if (kappa) kappa[s] = ctx->sp[s].kappa;
+        if (dhat) dhat[s] = ctx->sp[s].dhat;
+        if (init) init[s] = ctx->sp[s].barrier_initialized;
+    }
+    return 0;
+}
+
+int32_t ndipc_set_barrier(ndipc_ctx* ctx, const double* kappa, const double* dhat, const int32_t* init) {
+    if (check_ctx(ctx)) return -1;
+    if (sync_sp_from_dev(ctx)) return -1;
+    for (int s = 0; s < ctx->S; ++s) {
+        if (kappa) ctx->sp[s].kappa = kappa[s];
+        if (dhat) ctx->sp[s].dhat = dhat[s];
+        if (init) ctx->sp[s].barrier_initialized = init[s];
+    }
+    return sync_sp_to_dev(ctx);
+}
+
+int32_t ndipc_get_joint_state(ndipc_ctx* ctx, double* lambda6, double* mu) {
+    if (check_ctx(ctx)) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->J == 0) return 0;
+    if (lambda6) HIPCHK(hipMemcpy(lambda6, ctx->d.jlambda, 6 * size_t(ctx->J) * 8, hipMemcpyDeviceToHost));
+    if (mu) HIPCHK(hipMemcpy(mu, ctx->d.jmu, size_t(ctx->J) * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int32_t ndipc_step(ndipc_ctx* ctx, const double* dt, int32_t n_steps) {
+    if (check_ctx(ctx)) return -1;
+    if (!dt) return fail(ctx, "null dt");
+    for (int s = 0; s < ctx->S; ++s)
+        if (!(dt[s] > 0.0)) return fail(ctx, "dt must be positive");
+    HIPCHK(hipMemcpyAsync(ctx->d_dt, dt, sizeof(double) * ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    const Dev& d = ctx->d;
+    const Work& w = ctx->w;
+    hipStream_t st = ctx->stream;
+    const int mp = max_grid_pairs(ctx);
+    int alm_max = 1;
+    for (auto& q : ctx->sp) alm_max = std::max(alm_max, q.alm_max_iters);
+    g_timing_ctx = ctx;
+    for (int k = 0; k < n_steps; ++k) {
+        nd::launch_step_begin(d, w, ctx->d_dt, st);
+        nd::launch_world_boxes(d, w, st);
+        nd::launch_lbvh_build(d, w, st);
+        nd::launch_broad_phase(d, w, 0, -1.0, true, nullptr, st);
+        nd::timing_begin("topk", st);
+        nd::launch_topk(d, w, mp, st);
+        nd::timing_end("topk", st);
+        nd::launch_barrier_init_ccd(d, w, st);
+        nd::launch_broad_phase(d, w, 1, -1.0, true, nullptr, st);
+        nd::timing_begin("scene_min_flag", st);
+        nd::launch_scene_min_flag(d, w, mp, st);
+        nd::timing_end("scene_min_flag", st);
+        nd::launch_kappa(d, w, st);
+        nd::launch_broad_phase(d, w, 2, -1.0, true, nullptr, st);
+        nd::launch_ccd_earliest(d, w, mp, st);
+        nd::launch_ccd_dt(d, w, st);
+        nd::launch_pack(d, w, st);
+        nd::launch_pose_from_state(d, w, nullptr, st);
+        nd::launch_narrow_phase(d, w, mp, false, st);
+        nd::launch_contacts_setup(d, w, st);
+        for (int a = 0; a < alm_max; ++a) {
+            nd::launch_newton(d, w, st);
+            nd::launch_pose_from_state(d, w, w.x, st);
+            nd::launch_narrow_phase(d, w, mp, true, st);
+            nd::launch_alm_post(d, w, st);
+        }
+        nd::launch_finalize(d, w, st);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int32_t ndipc_pair_count(ndipc_ctx* ctx, int32_t* out) {
+    if (check_ctx(ctx)) return -1;
+    HIPCHK(hipMemcpyAsync(out, ctx->w.n_pairs_total, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int32_t ndipc_eval_counter(ndipc_ctx* ctx, double* out) {
+    if (check_ctx(ctx)) return -1;
+    if (sync_sp_from_dev(ctx)) return -1;
+    for (int s = 0; s < ctx->S; ++s) out[s] = ctx->sp[s].evals_total;
+    return 0;
+}
+
+int32_t ndipc_last_stats(ndipc_ctx* ctx, double* out8) {
+    if (check_ctx(ctx)) return -1;
+    if (sync_sp_from_dev(ctx)) return -1;
+    for (int s = 0; s < ctx->S; ++s) {
+        const SceneParams& q = ctx->sp[s];
+        double* o = out8 + 8 * s;
+        o[0] = q.alm_iters;
+        o[1] = q.newton_iters;
+        o[2] = q.n_contacts;
+        o[3] = q.n_fv;
+        o[4] = q.n_ee;
+        o[5] = q.evals;
+        o[6] = q.kappa_flag;
+        o[7] = q.toi;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- function level
+static int prep_geometry(ndipc_ctx* ctx) {
+    nd::launch_world_boxes(ctx->d, ctx->w, ctx->stream);
+    nd::launch_lbvh_build(ctx->d, ctx->w, ctx->stream);
+    return 0;
+}
+
+int32_t ndipc_detect_body_pairs(ndipc_ctx* ctx, int32_t scene, double r, int32_t* out, int32_t cap, int32_t* n) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    prep_geometry(ctx);
+    nd::launch_broad_phase(ctx->d, ctx->w, 3, std::max(0.0, r), false, &scene, ctx->stream);
+    int cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, ctx->w.n_pairs_total, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<int> pr(2 * size_t(std::max(cnt, 1)));
+    if (cnt) HIPCHK(hipMemcpy(pr.data(), ctx->w.pairs, 8 * size_t(cnt), hipMemcpyDeviceToHost));
+    const int b0 = ctx->body_off[scene];
+    *n = cnt;
+    for (int i = 0; i < cnt && i < cap; ++i) {
+        out[2 * i] = pr[2 * i] - b0;
+        out[2 * i + 1] = pr[2 * i + 1] - b0;
+    }
+    return 0;
+}
+
+int32_t ndipc_min_body_pair_distance(ndipc_ctx* ctx, int32_t scene, double r, double* out) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    prep_geometry(ctx);
+    nd::launch_broad_phase(ctx->d, ctx->w, 1, std::max(0.0, r), false, &scene, ctx->stream);
+    if (sync_sp_from_dev(ctx)) return -1;
+    *out = ctx->sp[scene].min_pair_dist;
+    return 0;
+}
+
+int32_t ndipc_build_candidates(ndipc_ctx* ctx, int32_t scene, double dhat, int32_t* counts, int32_t* vv, int32_t* ev,
+                               int32_t* ee, int32_t* fv, int32_t cap) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    prep_geometry(ctx);
+    nd::launch_broad_phase(ctx->d, ctx->w, 0, std::max(0.0, dhat), false, &scene, ctx->stream);
+    nd::launch_topk(ctx->d, ctx->w, max_grid_pairs(ctx), ctx->stream);
+    int np = 0;
+    HIPCHK(hipMemcpyAsync(&np, ctx->w.n_pairs_total, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<int> pr(2 * size_t(std::max(np, 1))), L(size_t(std::max(np, 1)) * 6 * nd::kTopK),
+        K(size_t(std::max(np, 1)) * 6);
+    if (np) {
+        HIPCHK(hipMemcpy(pr.data(), ctx->w.pairs, 8 * size_t(np), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(L.data(), ctx->w.lists, 4 * L.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(K.data(), ctx->w.lcnt, 4 * K.size(), hipMemcpyDeviceToHost));
+    }
+    long long c[4] = {0, 0, 0, 0};
+    auto put = [&](int which, int* dst, int width, std::initializer_list<int> vals) {
+        if (dst && c[which] < cap) {
+            int k = 0;
+            for (int v : vals) dst[width * c[which] + k++] = v;
+        }
+        c[which]++;
+    };
+    for (int p = 0; p < np; ++p) {
+        const int A = pr[2 * p], B = pr[2 * p + 1];
+        const int idA = ctx->body_id[A], idB = ctx->body_id[B];
+        const int* l = &L[size_t(p) * 6 * nd::kTopK];
+        const int* k = &K[size_t(p) * 6];
+        auto eAv = [&](int i, int s) { return ctx->edges_host[2 * (ctx->e_off[A] + l[2 * nd::kTopK + i]) + s]; };
+        auto eBv = [&](int i, int s) { return ctx->edges_host[2 * (ctx->e_off[B] + l[3 * nd::kTopK + i]) + s]; };
+        for (int i = 0; i < k[0]; ++i)
+            for (int j = 0; j < k[1]; ++j) put(0, vv, 4, {idA, idB, l[i], l[nd::kTopK + j]});
+        for (int i = 0; i < k[2]; ++i)
+            for (int j = 0; j < k[1]; ++j) put(1, ev, 5, {idA, idB, eAv(i, 0), eAv(i, 1), l[nd::kTopK + j]});
+        for (int i = 0; i < k[3]; ++i)
+            for (int j = 0; j < k[0]; ++j) put(1, ev, 5, {idB, idA, eBv(i, 0), eBv(i, 1), l[j]});
+        for (int i = 0; i < k[2]; ++i)
+            for (int j = 0; j < k[3]; ++j) put(2, ee, 6, {idA, idB, eAv(i, 0), eAv(i, 1), eBv(j, 0), eBv(j, 1)});
+        for (int i = 0; i < k[4]; ++i)
+            for (int j = 0; j < k[1]; ++j) put(3, fv, 4, {idA, idB, l[4 * nd::kTopK + i], l[nd::kTopK + j]});
+        for (int i = 0; i < k[5]; ++i)
+            for (int j = 0; j < k[0]; ++j) put(3, fv, 4, {idB, idA, l[5 * nd::kTopK + i], l[j]});
+    }
+    for (int i = 0; i < 4; ++i) counts[i] = int32_t(c[i]);
+    return 0;
+}
+
+int32_t ndipc_select_top_k(ndipc_ctx* ctx, const double* scores, int32_t n, int32_t k, int32_t* out) {
+    if (!ctx) return -1;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (n < 0 || n > nd::kMaxPrims) return fail(ctx, "n must be in [0, 2048]");
+    double* ds = nullptr;
+    int* di = nullptr;
+    HIPCHK(hipMalloc(&ds, sizeof(double) * std::max(n, 1)));
+    HIPCHK(hipMalloc(&di, sizeof(int) * (std::max(n, 1) + 1)));
+    HIPCHK(hipMemcpy(ds, scores, sizeof(double) * n, hipMemcpyHostToDevice));
+    nd::launch_select_topk_scores(ds, n, k, di + 1, di, ctx->stream);
+    int m = 0;
+    HIPCHK(hipMemcpyAsync(&m, di, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (m > 0) HIPCHK(hipMemcpy(out, di + 1, 4 * size_t(m), hipMemcpyDeviceToHost));
+    hipFree(ds);
+    hipFree(di);
+    return m;
+}
+
+int32_t ndipc_narrow_phase(ndipc_ctx* ctx, int32_t scene, const double* x, int32_t cap, int32_t* n, int32_t* ints,
+                           double* dbl) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    const int b0 = ctx->body_off[scene], nb = ctx->body_off[scene + 1] - b0;
+    if (x) HIPCHK(hipMemcpy(ctx->w.xt + 6 * size_t(b0), x, 48 * size_t(nb), hipMemcpyHostToDevice));
+    // ungated pose: every body of the scene
+    if (sync_sp_from_dev(ctx)) return -1;
+    std::vector<int> saved(ctx->S);
+    for (int s = 0; s < ctx->S; ++s) { saved[s] = ctx->sp[s].alm_done; ctx->sp[s].alm_done = 0; ctx->sp[s].evals = 0; }
+    if (sync_sp_to_dev(ctx)) return -1;
+    nd::launch_pose_from_state(ctx->d, ctx->w, x ? ctx->w.xt : nullptr, ctx->stream);
+    nd::launch_narrow_phase(ctx->d, ctx->w, max_grid_pairs(ctx), false, ctx->stream);
+    if (sync_sp_from_dev(ctx)) return -1;
+    const int nc = ctx->sp[scene].n_contacts;
+    *n = nc;
+    const size_t cb = size_t(scene) * nd::kMaxRawContacts;
+    const int m = std::min(nc, cap);
+    if (m > 0) {
+        std::vector<int> ci(5 * size_t(m));
+        HIPCHK(hipMemcpy(ci.data(), ctx->w.cint + 5 * cb, 20 * size_t(m), hipMemcpyDeviceToHost));
+        if (dbl) HIPCHK(hipMemcpy(dbl, ctx->w.cdbl + 7 * cb, 56 * size_t(m), hipMemcpyDeviceToHost));
+        if (ints)
+            for (int i = 0; i < m; ++i) {
+                ints[5 * i] = ci[5 * i] - b0;
+                ints[5 * i + 1] = ci[5 * i + 1] - b0;
+                for (int k = 2; k < 5; ++k) ints[5 * i + k] = ci[5 * i + k];
+            }
+    }
+    for (int s = 0; s < ctx->S; ++s) ctx->sp[s].alm_done = saved[s];
+    return sync_sp_to_dev(ctx);
+}
+
+int32_t ndipc_scene_min_distance_flag(ndipc_ctx* ctx, int32_t scene, int32_t* out_flag) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    prep_geometry(ctx);
+    if (sync_sp_from_dev(ctx)) return -1;
+    const double r = ctx->sp[scene].prune_margin;
+    nd::launch_broad_phase(ctx->d, ctx->w, 1, r, false, &scene, ctx->stream);
+    // force the flag kernel's gate open for this scene (min gap below 2 dhat)
+    if (sync_sp_from_dev(ctx)) return -1;
+    const double saved = ctx->sp[scene].min_pair_dist;
+    ctx->sp[scene].min_pair_dist = -1.0;
+    if (sync_sp_to_dev(ctx)) return -1;
+    HIPCHK(hipMemsetAsync(ctx->w.flag, 0, sizeof(int) * ctx->S, ctx->stream));
+    nd::launch_scene_min_flag(ctx->d, ctx->w, max_grid_pairs(ctx), ctx->stream);
+    int f = 0;
+    HIPCHK(hipMemcpyAsync(&f, ctx->w.flag + scene, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *out_flag = f;
+    ctx->sp[scene].min_pair_dist = saved;
+    return sync_sp_to_dev(ctx);
+}
+
+static int upload_contacts(ndipc_ctx* ctx, int scene, int nc, const int32_t* ints, const double* dbl,
+                           const double* nf) {
+    if (nc < 0 || nc > nd::kMaxContacts) return fail(ctx, "contact count must be in [0, 2000]");
+    const int b0 = ctx->body_off[scene], nb = ctx->body_off[scene + 1] - b0;
+    std::vector<int> ci(5 * size_t(std::max(nc, 1)));
+    for (int i = 0; i < nc; ++i) {
+        if (ints[5 * i] < 0 || ints[5 * i] >= nb || ints[5 * i + 1] < 0 || ints[5 * i + 1] >= nb)
+            return fail(ctx, "contact body out of range");
+        ci[5 * i] = ints[5 * i] + b0;
+        ci[5 * i + 1] = ints[5 * i + 1] + b0;
+        for (int k = 2; k < 5; ++k) ci[5 * i + k] = ints[5 * i + k];
+    }
+    const size_t cb = size_t(scene) * nd::kMaxRawContacts;
+    if (nc) {
+        HIPCHK(hipMemcpy(ctx->w.cint + 5 * cb, ci.data(), 20 * size_t(nc), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->w.cdbl + 7 * cb, dbl, 56 * size_t(nc), hipMemcpyHostToDevice));
+        if (nf) HIPCHK(hipMemcpy(ctx->w.nf + size_t(scene) * nd::kMaxContacts, nf, 8 * size_t(nc),
+                                 hipMemcpyHostToDevice));
+    }
+    if (sync_sp_from_dev(ctx)) return -1;
+    ctx->sp[scene].n_contacts = nc;
+    return sync_sp_to_dev(ctx);
+}
+
+static int forms_eval(ndipc_ctx* ctx, int scene, int which, int nc, const int32_t* ints, const double* dbl,
+                      const double* nf, const double* x, double dhat, double kappa, double mu, double eps, double dt,
+                      double* value, double* grad, double* hess, double* nfo) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    if (upload_contacts(ctx, scene, nc, ints, dbl, nf)) return -1;
+    const int b0 = ctx->body_off[scene], nb = ctx->body_off[scene + 1] - b0;
+    const size_t nd6 = 6 * size_t(nb);
+    const size_t need = 8 * (nd6 + nd6 + 1 + (hess ? nd6 * nd6 : 0) + nd::kMaxContacts);
+    double* buf = nullptr;
+    HIPCHK(hipMalloc(&buf, need));
+    double* dx = buf;
+    double* dg = buf + nd6;
+    double* dv = buf + 2 * nd6;
+    double* dnf = dv + 1;
+    double* dh = hess ? dnf + nd::kMaxContacts : nullptr;
+    HIPCHK(hipMemcpy(dx, x, 8 * nd6, hipMemcpyHostToDevice));
+    nd::launch_forms_eval(ctx->d, ctx->w, scene, which, dhat, kappa, mu, eps, dt, dx, dv, dg, dh, nfo ? dnf : nullptr,
+                          ctx->stream);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (value) HIPCHK(hipMemcpy(value, dv, 8, hipMemcpyDeviceToHost));
+    if (grad) HIPCHK(hipMemcpy(grad, dg, 8 * nd6, hipMemcpyDeviceToHost));
+    if (hess) HIPCHK(hipMemcpy(hess, dh, 8 * nd6 * nd6, hipMemcpyDeviceToHost));
+    if (nfo && nc) HIPCHK(hipMemcpy(nfo, dnf, 8 * size_t(nc), hipMemcpyDeviceToHost));
+    hipFree(buf);
+    return 0;
+}
+
+int32_t ndipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int32_t* ints, const double* dbl,
+                           const double* x, double dhat, double kappa, double* value, double* grad, double* hess,
+                           double* normal_forces) {
+    return forms_eval(ctx, scene, 0, nc, ints, dbl, nullptr, x, dhat, kappa, 0.0, 1e-8, 0.0, value, grad, hess,
+                      normal_forces);
+}
+
+int32_t ndipc_friction_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int32_t* ints, const double* dbl,
+                            const double* nf, const double* x, double mu, double eps, double dt, double* value,
+                            double* grad, double* hess) {
+    if (!nf && nc > 0) return fail(ctx, "normal forces required");
+    return forms_eval(ctx, scene, 1, nc, ints, dbl, nf, x, 1e-3, 0.0, mu, eps, dt, value, grad, hess, nullptr);
+}
+
+int32_t ndipc_ccd_toi_batch(ndipc_ctx* ctx, int32_t kind, int32_t n, const double* pa, const double* pb,
+                            const double* a, const double* b, const double* earliest, double tol, int32_t* hit,
+                            double* toi, int32_t* pops) {
+    if (!ctx) return -1;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (kind < 0 || kind > 2) return fail(ctx, "kind must be 0, 1 or 2");
+    if (n <= 0) return 0;
+    const size_t dbls = size_t(n) * (14 + 14 + 6 + 9 + 1 + 1);
+    double* buf = nullptr;
+    int* ib = nullptr;
+    HIPCHK(hipMalloc(&buf, 8 * dbls));
+    HIPCHK(hipMalloc(&ib, 8 * size_t(n)));
+    double* dpa = buf;
+    double* dpb = dpa + 14 * size_t(n);
+    double* da = dpb + 14 * size_t(n);
+    double* db = da + 6 * size_t(n);
+    double* de = db + 9 * size_t(n);
+    double* dt = de + size_t(n);
+    HIPCHK(hipMemcpy(dpa, pa, 8 * 14 * size_t(n), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dpb, pb, 8 * 14 * size_t(n), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(da, a, 8 * 6 * size_t(n), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(db, b, 8 * 9 * size_t(n), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(de, earliest, 8 * size_t(n), hipMemcpyHostToDevice));
+    nd::launch_ccd_batch(kind, n, dpa, dpb, da, db, de, tol, ib, dt, ib + n, ctx->stream);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(hit, ib, 4 * size_t(n), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(toi, dt, 8 * size_t(n), hipMemcpyDeviceToHost));
+    if (pops) HIPCHK(hipMemcpy(pops, ib + n, 4 * size_t(n), hipMemcpyDeviceToHost));
+    hipFree(buf);
+    hipFree(ib);
+    return 0;
+}
+
+int32_t ndipc_ccd_earliest_toi(ndipc_ctx* ctx, int32_t scene, double dt, double* out) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    if (sync_sp_from_dev(ctx)) return -1;
+    for (int s = 0; s < ctx->S; ++s) {
+        ctx->sp[s].toi = 1.0;
+        ctx->sp[s].dt = dt;
+        ctx->sp[s].ccd_any = 1;
+    }
+    const int saved = ctx->sp[scene].enable_ccd;
+    ctx->sp[scene].enable_ccd = 1;
+    if (sync_sp_to_dev(ctx)) return -1;
+    nd::launch_ccd_earliest(ctx->d, ctx->w, max_grid_pairs(ctx), ctx->stream);
+    if (sync_sp_from_dev(ctx)) return -1;
+    *out = ctx->sp[scene].toi;
+    ctx->sp[scene].enable_ccd = saved;
+    return sync_sp_to_dev(ctx);
+}
+
+int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t nb, const double* diag, int32_t noff, const int32_t* offij,
+                        const double* offb, const double* b, double tol, int32_t max_it, double* x, int32_t* iters,
+                        double* rel) {
+    if (!ctx) return -1;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (nb <= 0) return fail(ctx, "no blocks");
+    const size_t nd6 = 6 * size_t(nb);
+    const size_t dbls = 36 * size_t(nb) + 36 * size_t(std::max(noff, 1)) + nd6 + nd6 + 4 * nd6 + 36 * size_t(nb) + 2;
+    double* buf = nullptr;
+    int* ib = nullptr;
+    HIPCHK(hipMalloc(&buf, 8 * dbls));
+    HIPCHK(hipMalloc(&ib, 4 * (2 * size_t(std::max(noff, 1)) + 1)));
+    double* dD = buf;
+    double* dO = dD + 36 * size_t(nb);
+    double* db = dO + 36 * size_t(std::max(noff, 1));
+    double* dx = db + nd6;
+    double* scr = dx + nd6;
+    double* drel = scr + 4 * nd6 + 36 * size_t(nb);
+    HIPCHK(hipMemcpy(dD, diag, 8 * 36 * size_t(nb), hipMemcpyHostToDevice));
+    if (noff) {
+        HIPCHK(hipMemcpy(dO, offb, 8 * 36 * size_t(noff), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ib + 1, offij, 8 * size_t(noff), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(db, b, 8 * nd6, hipMemcpyHostToDevice));
+    nd::launch_pcg(nb, dD, noff, ib + 1, dO, db, tol, max_it, dx, ib, drel, scr, ctx->stream);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(x, dx, 8 * nd6, hipMemcpyDeviceToHost));
+    if (iters) HIPCHK(hipMemcpy(iters, ib, 4, hipMemcpyDeviceToHost));
+    if (rel) HIPCHK(hipMemcpy(rel, drel, 8, hipMemcpyDeviceToHost));
+    hipFree(buf);
+    hipFree(ib);
+    return 0;
+}
+
+int32_t ndipc_timing_enable(ndipc_ctx* ctx, int32_t on) {
+    if (!ctx) return -1;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->tim.clear();
+    g_event_next = 0;  // pooled events are reused
+    ctx->timing = on != 0;
+    g_timing_ctx = ctx;
+    return 0;
+}
+
+int32_t ndipc_timing_get(ndipc_ctx* ctx, const char* kernel, double* avg_ms, int32_t* count) {
+    if (!ctx) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    auto it = ctx->tim.find(kernel);
+    if (it == ctx->tim.end() || it->second.ev.empty()) {
+        *avg_ms = 0.0;
+        *count = 0;
+        return 0;
+    }
+    double tot = 0.0;
+    for (auto& e : it->second.ev) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e.first, e.second);
+        tot += ms;
+    }
+    *count = int32_t(it->second.ev.size());
+    *avg_ms = tot / double(*count);
+    return 0;
+}
+
+int32_t ndipc_round_ops(ndipc_ctx* ctx, const double* a, const double* b, int32_t n, double* out6) {
+    if (!ctx) return -1;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (n <= 0) return 0;
+    double* buf = nullptr;
+    HIPCHK(hipMalloc(&buf, 8 * size_t(n) * 8));
+    HIPCHK(hipMemcpy(buf, a, 8 * size_t(n), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(buf + n, b, 8 * size_t(n), hipMemcpyHostToDevice));
+    nd::launch_round_ops(buf, buf + n, n, buf + 2 * size_t(n), ctx->stream);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(out6, buf + 2 * size_t(n), 8 * 6 * size_t(n), hipMemcpyDeviceToHost));
+    hipFree(buf);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1162 @@
+// Contact-stage geometry kernels for gfx950: world AABBs, per-scene LBVH broad phase,
+// tie-exact top-K candidate lists, narrow phase with order-preserving compaction, the
+// scene-minimum-distance flag and interval CCD.
+//
+// All kernels are FP64 VALU/gather work (no MFMA). Work is flattened over
+// scene x pair so one launch serves every scene of a batch; per-pair kernels run a
+// persistent grid so the device-side pair count needs no host round trip.
+#include "ndipc_internal.h"
+#include "topk_select.h"
+
+namespace nd {
+
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------- world boxes
+// ContactAssembler::build -> transform_local_aabb (ContactAssembler.cpp:22-43,145-149).
+__global__ void k_world_boxes(Dev d, Work w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.N) return;
+    if (d.nv[b] == 0) return;
+    const Box3 lb = ldbox(d.lbox, b);
+    const Box3 wb = world_box(lb.mn, lb.mx, ld3(d.p, b), ld4(d.q, b));
+    double* o = w.wbox + 6 * b;
+    o[0] = wb.mn.x; o[1] = wb.mn.y; o[2] = wb.mn.z;
+    o[3] = wb.mx.x; o[4] = wb.mx.y; o[5] = wb.mx.z;
+}
+
+void launch_world_boxes(const Dev& d, const Work& w, hipStream_t st) {
+    if (d.N == 0) return;
+    hipLaunchKernelGGL(k_world_boxes, dim3((d.N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, d, w);
+}
+
+// ------------------------------------------------------------------------- LBVH
+// Karras 2012 ("Maximizing parallelism in the construction of BVHs, octrees and k-d
+// trees") over 30-bit Morton codes of the body AABB centres, one work-group per scene.
+// Keys carry the leaf index in the low 32 bits, so they are unique. Internal-node boxes
+// are the union of the node's contiguous leaf range.
+__device__ __forceinline__ unsigned expand_bits(unsigned v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ int karras_delta(const unsigned long long* keys, int m, int i, int j) {
+    if (j < 0 || j >= m) return -1;
+    return __clzll(keys[i] ^ keys[j]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, Work w) {
+    __shared__ unsigned long long keys[kMaxSceneBodies];
+    __shared__ int leaf_tmp[kMaxSceneBodies];
+    __shared__ double red[kBlock];
+    __shared__ int redi[kBlock];
+    const int s = blockIdx.x;
+    const int t = threadIdx.x;
+    const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
+    const int nb = b1 - b0;
+    // compact the mesh bodies (scene order)
+    int m = 0;
+    for (int base = 0; base < nb; base += kBlock) {
+        const int i = base + t;
+        const int has = (i < nb && d.nv[b0 + i] > 0) ? 1 : 0;
+        int ex;
+        const int tot = wg_exclusive_scan(has, redi, &ex);
+        if (has) leaf_tmp[m + ex] = b0 + i;
+        m += tot;
+        __syncthreads();
+    }
+    if (t == 0) w.nleaves[s] = m;
+    if (m == 0) return;
+    // centroid bounds
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        double vmn = 1.7976931348623157e308, vmx = -1.7976931348623157e308;
+        for (int k = t; k < m; k += kBlock) {
+            const double* bx = w.wbox + 6 * leaf_tmp[k];
+            const double c = (bx[a] + bx[3 + a]) * 0.5;
+            vmn = dmin(vmn, c);
+            vmx = dmax(vmx, c);
+        }
+        lo[a] = wg_reduce(vmn, red, OpMin());
+        hi[a] = wg_reduce(vmx, red, OpMax());
+    }
+    int P = 1;
+    while (P < m) P <<= 1;
+    for (int k = t; k < P; k += kBlock) {
+        if (k < m) {
+            const double* bx = w.wbox + 6 * leaf_tmp[k];
+            unsigned code = 0;
+            unsigned c3[3];
+            for (int a = 0; a < 3; ++a) {
+                const double c = (bx[a] + bx[3 + a]) * 0.5;
+                const double ext = hi[a] - lo[a];
+                double u = ext > 0.0 ? (c - lo[a]) / ext : 0.0;
+                u = dmin(dmax(u, 0.0), 1.0);
+                c3[a] = (unsigned)dmin(u * 1024.0, 1023.0);
+            }
+            code = (expand_bits(c3[0]) << 2) | (expand_bits(c3[1]) << 1) | expand_bits(c3[2]);
+            keys[k] = ((unsigned long long)code << 32) | (unsigned)k;
+        } else {
+            keys[k] = ~0ull;
+        }
+    }
+    __syncthreads();
+    // bitonic sort of the keys in LDS
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = t; k < P / 2; k += kBlock) {
+                const int i = 2 * k - (k & (stride - 1));
+                const int j = i + stride;
+                const bool up = ((i & size) == 0);
+                const unsigned long long a = keys[i], b = keys[j];
+                if ((a > b) == up) { keys[i] = b; keys[j] = a; }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = t; k < m; k += kBlock) w.leaf_body[b0 + k] = leaf_tmp[keys[k] & 0xffffffffu];
+    __syncthreads();
+    // internal nodes 0 .. m-2
+    for (int i = t; i < m - 1; i += kBlock) {
+        const int dl = karras_delta(keys, m, i, i + 1) - karras_delta(keys, m, i, i - 1);
+        const int dir = dl > 0 ? 1 : -1;
+        const int dmin_ = karras_delta(keys, m, i, i - dir);
+        int lmax = 2;
+        while (karras_delta(keys, m, i, i + lmax * dir) > dmin_) lmax <<= 1;
+        int l = 0;
+        for (int tt = lmax >> 1; tt >= 1; tt >>= 1)
+            if (karras_delta(keys, m, i, i + (l + tt) * dir) > dmin_) l += tt;
+        const int j = i + l * dir;
+        const int dnode = karras_delta(keys, m, i, j);
+        int sp = 0, tt = l;
+        do {
+            tt = (tt + 1) >> 1;
+            if (karras_delta(keys, m, i, i + (sp + tt) * dir) > dnode) sp += tt;
+        } while (tt > 1);
+        const int gamma = i + sp * dir + (dir < 0 ? -1 : 0);
+        const int first = i < j ? i : j, last = i < j ? j : i;
+        w.node_child[2 * (b0 + i)] = (first == gamma) ? ~gamma : gamma;
+        w.node_child[2 * (b0 + i) + 1] = (last == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+        w.node_range[2 * (b0 + i)] = first;
+        w.node_range[2 * (b0 + i) + 1] = last;
+    }
+    __syncthreads();
+    for (int i = t; i < m - 1; i += kBlock) {
+        const int first = w.node_range[2 * (b0 + i)], last = w.node_range[2 * (b0 + i) + 1];
+        Box3 bx = ldbox(w.wbox, w.leaf_body[b0 + first]);
+        for (int k = first + 1; k <= last; ++k) {
+            const Box3 o = ldbox(w.wbox, w.leaf_body[b0 + k]);
+            bx.mn = vmin(bx.mn, o.mn);
+            bx.mx = vmax(bx.mx, o.mx);
+        }
+        double* o = w.node_box + 6 * (b0 + i);
+        o[0] = bx.mn.x; o[1] = bx.mn.y; o[2] = bx.mn.z;
+        o[3] = bx.mx.x; o[4] = bx.mx.y; o[5] = bx.mx.z;
+    }
+}
+
+void launch_lbvh_build(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_lbvh_build, dim3(d.S), dim3(kBlock), 0, st, d, w);
+}
+
+// Traversal: leaf test = AABB::intersects of both boxes inflated by r
+// (ContactAssembler.cpp:165, AABB.h:149-153,193-196), evaluated with the same FP
+// expressions. Internal nodes use the identical expression on the union box; rounding
+// is monotonic, so a rejected node cannot contain an accepted leaf.
+template <typename F>
+__device__ void lbvh_query(const Dev& d, const Work& w, int s, int i, double r, F&& on_hit) {
+    const int b0 = d.body_off[s];
+    const int m = w.nleaves[s];
+    const Box3 qa = box_inflated(ldbox(w.wbox, i), r);
+    if (m == 1) return;
+    int stack[72];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const int node = stack[--sp];
+        const int c0 = w.node_child[2 * (b0 + node)], c1 = w.node_child[2 * (b0 + node) + 1];
+        const int cs[2] = {c0, c1};
+        for (int k = 0; k < 2; ++k) {
+            const int c = cs[k];
+            if (c < 0) {
+                const int j = w.leaf_body[b0 + (~c)];
+                if (j > i && box_overlap(qa, box_inflated(ldbox(w.wbox, j), r))) on_hit(j);
+            } else {
+                if (box_overlap(qa, box_inflated(ldbox(w.node_box, b0 + c), r)) && sp < 72) stack[sp++] = c;
+            }
+        }
+    }
+}
+
+// mode 0: candidate pairs (gap <= r); mode 3: all overlapping pairs (detectBodyPairs).
+// mode 1: min gap over pairs at 2*dhat + prox pairs (gap <= prune_margin).
+// mode 2: CCD gate — any allowed pair at inflation max(1e-6, dhat) + vmax dt.
+// pass 0 counts, pass 1 emits at pair_off[s] (scan of the counts).
+__global__ void __launch_bounds__(kBlock) k_broad_phase(Dev d, Work w, int mode, int pass, double r_in,
+                                                        int only_scene) {
+    __shared__ int shi[kBlock];
+    __shared__ double shd[kBlock];
+    const int s = blockIdx.x;
+    const SceneParams& sp = d.sp[s];
+    const int t = threadIdx.x;
+    const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
+    int* cnt = (mode == 1) ? w.prox_cnt : w.pair_cnt;
+    const bool step_mode = r_in < 0.0;  // per-scene inflation of IPCSolver::step
+    const bool skip = (only_scene >= 0 && s != only_scene) || (step_mode && mode == 1 && !sp.enable_adaptive) ||
+                      (step_mode && mode == 2 && !sp.enable_ccd);
+    if (skip) {
+        if (pass == 0 && t == 0 && mode != 2) cnt[s] = 0;
+        if (mode == 1 && pass == 0 && t == 0) d.sp[s].min_pair_dist = 1.0 / 0.0;
+        if (mode == 2 && t == 0) d.sp[s].ccd_any = 0;
+        return;
+    }
+    double r = r_in;
+    if (step_mode) r = (mode == 1) ? sp.dhat * 2.0 : (mode == 2 ? sp.ccd_infl : sp.dhat);
+    r = dmax(0.0, r);
+    int* off = (mode == 1) ? w.prox_off : w.pair_off;
+    int* outp = (mode == 1) ? w.prox_pairs : w.pairs;
+    int* outs = (mode == 1) ? w.prox_scene : w.pair_scene;
+    const double margin = d.sp[s].prune_margin;
+    int running = 0;
+    double gap_min = 1.0 / 0.0;
+    int any = 0;
+    for (int base = b0; base < b1; base += kBlock) {
+        const int i = base + t;
+        int c = 0;
+        if (i < b1 && d.nv[i] > 0) {
+            lbvh_query(d, w, s, i, r, [&](int j) {
+                const double gap = box_distance(ldbox(w.wbox, i), ldbox(w.wbox, j));
+                if (mode == 0) { if (gap <= r) ++c; }
+                else if (mode == 3) ++c;
+                else if (mode == 1) { gap_min = dmin(gap_min, gap); if (gap <= margin) ++c; }
+                else if (pair_allowed(d, s, d.body_id[i], d.body_id[j])) any = 1;
+            });
+        }
+        if (mode == 2) continue;
+        int ex;
+        const int tot = wg_exclusive_scan(c, shi, &ex);
+        if (pass == 1 && c > 0) {
+            const int o = off[s] + running + ex;
+            int k = 0;
+            lbvh_query(d, w, s, i, r, [&](int j) {
+                const double gap = box_distance(ldbox(w.wbox, i), ldbox(w.wbox, j));
+                const bool keep = (mode == 0) ? (gap <= r) : (mode == 3 ? true : (gap <= margin));
+                if (keep && k < c) {
+                    // insertion into the sorted segment (j ascending)
+                    int pos = k;
+                    while (pos > 0 && outp[2 * (o + pos - 1) + 1] > j) {
+                        outp[2 * (o + pos)] = outp[2 * (o + pos - 1)];
+                        outp[2 * (o + pos) + 1] = outp[2 * (o + pos - 1) + 1];
+                        --pos;
+                    }
+                    outp[2 * (o + pos)] = i;
+                    outp[2 * (o + pos) + 1] = j;
+                    outs[o + k] = s;
+                    ++k;
+                }
+            });
+        }
+        running += tot;
+    }
+    if (mode == 2) {
+        const int a = wg_reduce(any, shi, OpOrI());
+        if (t == 0) d.sp[s].ccd_any = a;
+        return;
+    }
+    if (pass == 0 && t == 0) cnt[s] = running;
+    if (mode == 1 && pass == 0) {
+        const double g = wg_reduce(gap_min, shd, OpMin());
+        if (t == 0) d.sp[s].min_pair_dist = g;
+    }
+}
+
+// Exclusive scan of per-scene counts (one work-group).
+__global__ void __launch_bounds__(kBlock) k_scan_counts(const int* cnt, int* off, int* total, int S) {
+    __shared__ int sh[kBlock];
+    int running = 0;
+    for (int base = 0; base < S; base += kBlock) {
+        const int i = base + threadIdx.x;
+        const int v = i < S ? cnt[i] : 0;
+        int ex;
+        const int tot = wg_exclusive_scan(v, sh, &ex);
+        if (i < S) off[i] = running + ex;
+        running += tot;
+    }
+    if (threadIdx.x == 0) {
+        off[S] = running;
+        *total = running;
+    }
+}
+
+void launch_broad_phase(const Dev& d, const Work& w, int mode, double r, bool, const int* scene_mask,
+                        hipStream_t st) {
+    const int only = scene_mask ? *scene_mask : -1;
+    hipLaunchKernelGGL(k_broad_phase, dim3(d.S), dim3(kBlock), 0, st, d, w, mode, 0, r, only);
+    if (mode == 2) return;
+    int* cnt = (mode == 1) ? w.prox_cnt : w.pair_cnt;
+    int* off = (mode == 1) ? w.prox_off : w.pair_off;
+    int* tot = (mode == 1) ? w.n_prox_total : w.n_pairs_total;
+    if (only >= 0) {
+        // single-scene call: other scenes' counts are zeroed on the host side of the API
+    }
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, st, cnt, off, tot, d.S);
+    hipLaunchKernelGGL(k_broad_phase, dim3(d.S), dim3(kBlock), 0, st, d, w, mode, 1, r, only);
+}
+
+// ------------------------------------------------------------------------ top-K
+// select_top_k_indices (ContactAssembler.cpp:67-88) for the six lists of every close
+// pair (ContactAssembler.cpp:249-302). Scores are point_aabb_distance of world
+// vertices to the other body's world AABB; edges take the min over 2 endpoints, faces
+// over 3 (std::min / std::min({..}) keep the first smallest). Lists are sorted by
+// (score, index); the selected set follows std::nth_element exactly (topk_select.h).
+__device__ __forceinline__ bool key_less(double sa, int ia, double sb, int ib) {
+    return sa < sb || (sa == sb && ia < ib);
+}
+
+__device__ void wg_bitonic_pairs(double* sk, int* si, int P) {
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < P / 2; k += blockDim.x) {
+                const int i = 2 * k - (k & (stride - 1));
+                const int j = i + stride;
+                const bool up = ((i & size) == 0);
+                const bool gt = key_less(sk[j], si[j], sk[i], si[i]);
+                if (gt == up) {
+                    const double ts = sk[i]; sk[i] = sk[j]; sk[j] = ts;
+                    const int ti = si[i]; si[i] = si[j]; si[j] = ti;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Selects the top-k of s_orig[0..n) into out[0..k) ordered by (score, index).
+__device__ void wg_select_topk(const double* s_orig, double* sk, int* si, int n, int k, int* out) {
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        sk[i] = i < n ? s_orig[i] : 1.0 / 0.0;
+        si[i] = i < n ? i : 0x7fffffff;
+    }
+    __syncthreads();
+    wg_bitonic_pairs(sk, si, P);
+    const bool straddle = (k < n) && (sk[k] == sk[k - 1]);
+    if (!straddle) {
+        for (int i = threadIdx.x; i < k; i += blockDim.x) out[i] = si[i];
+        __syncthreads();
+        return;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) si[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        TopkCmp c{s_orig};
+        tk_nth_element(si, k, n, c);
+        // order the kept set by (score, index)
+        for (int a = 1; a < k; ++a) {
+            const int v = si[a];
+            int b = a - 1;
+            while (b >= 0 && key_less(s_orig[v], v, s_orig[si[b]], si[b])) {
+                si[b + 1] = si[b];
+                --b;
+            }
+            si[b + 1] = v;
+        }
+        for (int i = 0; i < k; ++i) out[i] = si[i];
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBlock) k_topk(Dev d, Work w) {
+    __shared__ double s_orig[kMaxPrims];
+    __shared__ double sk[kMaxPrims];
+    __shared__ int si[kMaxPrims];
+    __shared__ int outl[kTopK];
+    const int total = *w.n_pairs_total;
+    for (int task = blockIdx.x; task < total * 6; task += gridDim.x) {
+        const int p = task / 6, list = task % 6;
+        const int A = w.pairs[2 * p], B = w.pairs[2 * p + 1];
+        const int X = (list & 1) ? B : A;
+        const int Y = (list & 1) ? A : B;
+        const Box3 by = ldbox(w.wbox, Y);
+        const V3 px = ld3(d.p, X);
+        const Q qx = ld4(d.q, X);
+        const int kind = list >> 1;  // 0 vertices, 1 edges, 2 faces
+        const int n = kind == 0 ? d.nv[X] : (kind == 1 ? d.ne[X] : d.nf[X]);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            double sc;
+            if (kind == 0) {
+                sc = point_box_distance(world_vertex(d, X, i, px, qx), by);
+            } else if (kind == 1) {
+                const int* e = d.edges + 2 * (d.e_off[X] + i);
+                const double a = point_box_distance(world_vertex(d, X, e[0], px, qx), by);
+                const double b = point_box_distance(world_vertex(d, X, e[1], px, qx), by);
+                sc = (b < a) ? b : a;
+            } else {
+                const int* f = d.faces + 3 * (d.f_off[X] + i);
+                const double a = point_box_distance(world_vertex(d, X, f[0], px, qx), by);
+                const double b = point_box_distance(world_vertex(d, X, f[1], px, qx), by);
+                const double c = point_box_distance(world_vertex(d, X, f[2], px, qx), by);
+                sc = a;
+                if (b < sc) sc = b;
+                if (c < sc) sc = c;
+            }
+            s_orig[i] = sc;
+        }
+        __syncthreads();
+        const int k = n < kTopK ? n : kTopK;
+        wg_select_topk(s_orig, sk, si, n, k, outl);
+        for (int i = threadIdx.x; i < k; i += blockDim.x) w.lists[(p * 6 + list) * kTopK + i] = outl[i];
+        if (threadIdx.x == 0) w.lcnt[p * 6 + list] = k;
+        __syncthreads();
+    }
+}
+
+void launch_topk(const Dev& d, const Work& w, int max_pairs, hipStream_t st) {
+    int grid = max_pairs * 6;
+    if (grid > 2048) grid = 2048;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_topk, dim3(grid), dim3(kBlock), 0, st, d, w);
+}
+
+// Standalone selection on given scores (one work-group).
+__global__ void __launch_bounds__(kBlock) k_select_scores(const double* scores, int n, int k, int* out, int* out_n) {
+    __shared__ double s_orig[kMaxPrims];
+    __shared__ double sk[kMaxPrims];
+    __shared__ int si[kMaxPrims];
+    __shared__ int outl[kMaxPrims];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s_orig[i] = scores[i];
+    __syncthreads();
+    const int kk = k < n ? k : n;
+    if (kk <= 0) {
+        if (threadIdx.x == 0) *out_n = 0;
+        return;
+    }
+    wg_select_topk(s_orig, sk, si, n, kk, outl);
+    for (int i = threadIdx.x; i < kk; i += blockDim.x) out[i] = outl[i];
+    if (threadIdx.x == 0) *out_n = kk;
+}
+
+void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, hipStream_t st) {
+    hipLaunchKernelGGL(k_select_scores, dim3(1), dim3(kBlock), 0, st, scores, n, k, out, out_n);
+}
+
+// ------------------------------------------------------------------------- poses
+// Body poses of the narrow phase: step-start state (x == NULL, detectContacts) or the
+// Newton iterate (detectContactsAtState, IPCSolver.cpp:604-629).
+__global__ void k_pose(Dev d, Work w, const double* x, int gate) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.N) return;
+    if (gate && d.sp[d.body_scene[b]].alm_done) return;
+    if (!x) {
+        st3(w.pose_p, b, ld3(d.p, b));
+        st4(w.pose_q, b, ld4(d.q, b));
+    } else {
+        st3(w.pose_p, b, v3(x[6 * b], x[6 * b + 1], x[6 * b + 2]));
+        st4(w.pose_q, b, apply_theta(ld4(d.q, b), v3(x[6 * b + 3], x[6 * b + 4], x[6 * b + 5])));
+    }
+}
+
+void launch_pose_from_state(const Dev& d, const Work& w, const double* x, hipStream_t st) {
+    if (d.N == 0) return;
+    hipLaunchKernelGGL(k_pose, dim3((d.N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, d, w, x, x ? 1 : 0);
+}
+
+// ------------------------------------------------------------------ narrow phase
+// IPCSolver::detectContacts / detectContactsAtState (IPCSolver.cpp:394-806).
+// Candidate order per pair: FV faces(A) x verts(B), FV faces(B) x verts(A), then
+// EE edges(A) x edges(B) (ContactAssembler.cpp:304-341); all FV hits of a scene come
+// before its EE hits. Bits: [0,1024) FV A-B, [1024,2048) FV B-A, [2048,3072) EE.
+
+struct PairGeom {
+    V3 vA[kTopK], vB[kTopK];
+    V3 fA[kTopK][3], fB[kTopK][3];
+    V3 nA[kTopK], nB[kTopK];  // safe-normalized face normals
+    V3 eA[kTopK][2], eB[kTopK][2];
+    int lvA[kTopK], lvB[kTopK], lfA[kTopK], lfB[kTopK], leA[kTopK][2], leB[kTopK][2];
+    int k[6];
+    int A, B, s, allowed;
+    double dhat;
+};
+
+// safeNormalized (IPCSolver.cpp:92-105)
+__device__ __forceinline__ V3 safe_normalized(V3 v, V3 fb) {
+    const double n = norm(v);
+    if (n > 1e-12) return v / n;
+    const double fn = norm(fb);
+    if (fn > 1e-12) return fb / fn;
+    return v3(0.0, 1.0, 0.0);
+}
+
+// pointInTriangleBarycentric (IPCSolver.cpp:107-134)
+__device__ __forceinline__ bool point_in_triangle(V3 p, V3 a, V3 b, V3 c) {
+    const double eps = 1e-8;
+    const V3 v0 = b - a, v1 = c - a, v2 = p - a;
+    const double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
+    const double d20 = dot(v2, v0), d21 = dot(v2, v1);
+    const double den = d00 * d11 - d01 * d01;
+    if (fabs(den) < 1e-16) return false;
+    const double v = (d11 * d20 - d01 * d21) / den;
+    const double ww = (d00 * d21 - d01 * d20) / den;
+    const double u = 1.0 - v - ww;
+    return u >= -eps && v >= -eps && ww >= -eps;
+}
+
+__device__ __forceinline__ double clamp01(double x) { return x < 0.0 ? 0.0 : (1.0 < x ? 1.0 : x); }
+
+// closestPointsOnSegments (IPCSolver.cpp:32-90)
+__device__ __forceinline__ void closest_points(V3 p1, V3 q1, V3 p2, V3 q2, V3& c1, V3& c2) {
+    const double eps = 1e-12;
+    const V3 d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
+    const double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    double s, t;
+    if (a <= eps && e <= eps) { c1 = p1; c2 = p2; return; }
+    if (a <= eps) {
+        s = 0.0;
+        t = clamp01(f / e);
+    } else {
+        const double c = dot(d1, r);
+        if (e <= eps) {
+            t = 0.0;
+            s = clamp01(-c / a);
+        } else {
+            const double b = dot(d1, d2);
+            const double den = a * e - b * b;
+            s = (fabs(den) > eps) ? clamp01((b * f - c * e) / den) : 0.0;
+            t = (b * s + f) / e;
+            if (t < 0.0) { t = 0.0; s = clamp01(-c / a); }
+            else if (t > 1.0) { t = 1.0; s = clamp01((b - c) / a); }
+        }
+    }
+    c1 = p1 + d1 * s;
+    c2 = p2 + d2 * t;
+}
+
+__device__ void load_pair_geom(const Dev& d, const Work& w, int p, PairGeom& g) {
+    const int t = threadIdx.x;
+    if (t == 0) {
+        g.A = w.pairs[2 * p];
+        g.B = w.pairs[2 * p + 1];
+        g.s = w.pair_scene[p];
+        for (int l = 0; l < 6; ++l) g.k[l] = w.lcnt[p * 6 + l];
+        g.allowed = pair_allowed(d, g.s, d.body_id[g.A], d.body_id[g.B]) ? 1 : 0;
+        g.dhat = d.sp[g.s].dhat;
+    }
+    __syncthreads();
+    const int A = g.A, B = g.B;
+    const V3 pA = ld3(w.pose_p, A), pB = ld3(w.pose_p, B);
+    const Q qA = ld4(w.pose_q, A), qB = ld4(w.pose_q, B);
+    const int* L = w.lists + p * 6 * kTopK;
+    if (t < kTopK) {
+        const int i = t;
+        if (i < g.k[0]) { g.lvA[i] = L[0 * kTopK + i]; g.vA[i] = world_vertex(d, A, g.lvA[i], pA, qA); }
+        if (i < g.k[1]) { g.lvB[i] = L[1 * kTopK + i]; g.vB[i] = world_vertex(d, B, g.lvB[i], pB, qB); }
+    } else if (t < 2 * kTopK) {
+        const int i = t - kTopK;
+        if (i < g.k[2]) {
+            const int e = L[2 * kTopK + i];
+            const int* ev = d.edges + 2 * (d.e_off[A] + e);
+            g.leA[i][0] = ev[0]; g.leA[i][1] = ev[1];
+            g.eA[i][0] = world_vertex(d, A, ev[0], pA, qA);
+            g.eA[i][1] = world_vertex(d, A, ev[1], pA, qA);
+        }
+        if (i < g.k[3]) {
+            const int e = L[3 * kTopK + i];
+            const int* ev = d.edges + 2 * (d.e_off[B] + e);
+            g.leB[i][0] = ev[0]; g.leB[i][1] = ev[1];
+            g.eB[i][0] = world_vertex(d, B, ev[0], pB, qB);
+            g.eB[i][1] = world_vertex(d, B, ev[1], pB, qB);
+        }
+    } else if (t < 3 * kTopK) {
+        const int i = t - 2 * kTopK;
+        if (i < g.k[4]) {
+            const int f = L[4 * kTopK + i];
+            g.lfA[i] = f;
+            const int* fv = d.faces + 3 * (d.f_off[A] + f);
+            for (int c = 0; c < 3; ++c) g.fA[i][c] = world_vertex(d, A, fv[c], pA, qA);
+            g.nA[i] = safe_normalized(cross(g.fA[i][1] - g.fA[i][0], g.fA[i][2] - g.fA[i][0]), v3(0.0, 1.0, 0.0));
+        }
+        if (i < g.k[5]) {
+            const int f = L[5 * kTopK + i];
+            g.lfB[i] = f;
+            const int* fv = d.faces + 3 * (d.f_off[B] + f);
+            for (int c = 0; c < 3; ++c) g.fB[i][c] = world_vertex(d, B, fv[c], pB, qB);
+            g.nB[i] = safe_normalized(cross(g.fB[i][1] - g.fB[i][0], g.fB[i][2] - g.fB[i][0]), v3(0.0, 1.0, 0.0));
+        }
+    }
+    __syncthreads();
+}
+
+// FV test of face f (verts tri, normal n) against vertex p.
+__device__ __forceinline__ bool fv_test(const V3* tri, V3 n, V3 p, double dhat, double& sd, V3& proj) {
+    sd = dot(p - tri[0], n);
+    const double dist = fabs(sd);
+    if (dist >= dhat) return false;
+    proj = p - sd * n;
+    return point_in_triangle(proj, tri[0], tri[1], tri[2]);
+}
+
+__device__ __forceinline__ bool cand_hit(const PairGeom& g, int c, bool vv) {
+    if (!g.allowed) return false;
+    if (vv) {
+        const int kb = g.k[1];
+        if (c >= g.k[0] * kb) return false;
+        const V3 dd = g.vB[c % kb] - g.vA[c / kb];
+        return norm(dd) < g.dhat;
+    }
+    double sd;
+    V3 proj;
+    if (c < 1024) {
+        const int kb = g.k[1];
+        if (kb == 0 || c >= g.k[4] * kb) return false;
+        const int i = c / kb, j = c % kb;
+        return fv_test(g.fA[i], g.nA[i], g.vB[j], g.dhat, sd, proj);
+    }
+    if (c < 2048) {
+        const int cc = c - 1024;
+        const int ka = g.k[0];
+        if (ka == 0 || cc >= g.k[5] * ka) return false;
+        const int i = cc / ka, j = cc % ka;
+        return fv_test(g.fB[i], g.nB[i], g.vA[j], g.dhat, sd, proj);
+    }
+    const int cc = c - 2048;
+    const int kb = g.k[3];
+    if (kb == 0 || cc >= g.k[2] * kb) return false;
+    const int i = cc / kb, j = cc % kb;
+    V3 c0, c1;
+    closest_points(g.eA[i][0], g.eA[i][1], g.eB[j][0], g.eB[j][1], c0, c1);
+    return norm(c1 - c0) < g.dhat;
+}
+
+__global__ void __launch_bounds__(kBlock) k_np_count(Dev d, Work w, int vv, int gate) {
+    __shared__ PairGeom g;
+    __shared__ int cnt[3];
+    const int total = *w.n_pairs_total;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int p = blockIdx.x; p < total; p += gridDim.x) {
+        const int s = w.pair_scene[p];
+        const SceneParams& sp = d.sp[s];
+        if (gate && sp.alm_done) continue;
+        if (vv && !sp.vv_needed) continue;  // VV only when no FV/EE contact (IPCSolver.cpp:546)
+        if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+        load_pair_geom(d, w, p, g);
+        const int nwords = vv ? 16 : kHitWords;
+        for (int word = wave; word < nwords; word += 4) {
+            const int c = word * 64 + lane;
+            const bool h = cand_hit(g, c, vv != 0);
+            const unsigned long long m = __ballot(h);
+            if (lane == 0) {
+                w.hits[(long long)p * kHitWords + word] = m;
+                const int pc = __popcll(m);
+                atomicAdd(&cnt[(vv || c < 2048) ? 0 : 1], pc);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            w.hit_fv[p] = cnt[0];
+            w.hit_ee[p] = vv ? 0 : cnt[1];
+        }
+        __syncthreads();
+    }
+}
+
+// Per scene: offsets of FV hits then EE hits in pair order; raw count capped at 40000.
+__global__ void __launch_bounds__(kBlock) k_np_scan(Dev d, Work w, int vv, int gate) {
+    __shared__ int sh[kBlock];
+    __shared__ double shd[kBlock];
+    const int s = blockIdx.x;
+    SceneParams& sp = d.sp[s];
+    if (gate && sp.alm_done) return;
+    if (vv && !sp.vv_needed) return;
+    const int p0 = w.pair_off[s], p1 = w.pair_off[s + 1];
+    int run = 0;
+    double ev = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int base = p0; base < p1; base += kBlock) {
+            const int p = base + threadIdx.x;
+            int v = 0;
+            double e = 0.0;
+            if (p < p1) {
+                v = pass == 0 ? w.hit_fv[p] : w.hit_ee[p];
+                const int* k = w.lcnt + p * 6;
+                if (vv) e = pass == 0 ? double(k[0] * k[1]) : 0.0;
+                else e = pass == 0 ? double(k[4] * k[1] + k[5] * k[0]) : double(k[2] * k[3]);
+            }
+            int ex;
+            const int tot = wg_exclusive_scan(v, sh, &ex);
+            if (p < p1) {
+                if (pass == 0) w.hit_off_fv[p] = run + ex;
+                else w.hit_off_ee[p] = run + ex;
+            }
+            run += tot;
+            ev += wg_reduce(e, shd, OpAdd());
+        }
+    }
+    if (threadIdx.x == 0) {
+        w.raw_cnt[s] = run < kMaxRawContacts ? run : kMaxRawContacts;
+        if (!vv) sp.vv_needed = (run == 0) ? 1 : 0;
+        sp.evals += ev;
+        sp.evals_total += ev;
+    }
+}
+
+// append_contact (IPCSolver.cpp:431-469 / 651-695): orient n toward pB - pA (with
+// safeNormalized fallback), distance = max(0, d).
+__device__ void write_contact(const Dev& d, const Work& w, int s, int pos, int A, int B, int type, int pa, int pb,
+                              V3 pt, V3 nrm, double dist) {
+    const V3 pAp = ld3(w.pose_p, A), pBp = ld3(w.pose_p, B);
+    V3 n = safe_normalized(nrm, pBp - pAp);
+    if (dot(n, pBp - pAp) < 0.0) n = -n;
+    const long long c = (long long)s * kMaxRawContacts + pos;
+    int* ci = w.cint + 5 * c;
+    ci[0] = A; ci[1] = B; ci[2] = type; ci[3] = pa; ci[4] = pb;
+    double* cd = w.cdbl + 7 * c;
+    cd[0] = pt.x; cd[1] = pt.y; cd[2] = pt.z;
+    cd[3] = n.x; cd[4] = n.y; cd[5] = n.z;
+    cd[6] = dist > 0.0 ? dist : 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_np_write(Dev d, Work w, int vv, int gate) {
+    __shared__ PairGeom g;
+    __shared__ int wpre[kHitWords];
+    const int total = *w.n_pairs_total;
+    for (int p = blockIdx.x; p < total; p += gridDim.x) {
+        const int s = w.pair_scene[p];
+        if (gate && d.sp[s].alm_done) continue;
+        if (vv && (!d.sp[s].vv_needed || w.hit_fv[p] == 0)) continue;
+        if (!vv && w.hit_fv[p] + w.hit_ee[p] == 0) continue;
+        load_pair_geom(d, w, p, g);
+        const unsigned long long* H = w.hits + (long long)p * kHitWords;
+        if (threadIdx.x == 0) {
+            int run = 0;
+            for (int k = 0; k < kHitWords; ++k) {
+                if (k == 32 && !vv) run = 0;  // EE block restarts at its own offset
+                wpre[k] = run;
+                if (!(vv && k >= 16)) run += __popcll(H[k]);
+            }
+        }
+        __syncthreads();
+        const int nw = vv ? 16 : kHitWords;
+        for (int c = threadIdx.x; c < nw * 64; c += blockDim.x) {
+            const int word = c >> 6, bit = c & 63;
+            const unsigned long long m = H[word];
+            if (!((m >> bit) & 1ull)) continue;
+            const int rank = wpre[word] + __popcll(m & ((1ull << bit) - 1ull));
+            const bool is_ee = !vv && c >= 2048;
+            const int pos = (is_ee ? w.hit_off_ee[p] : w.hit_off_fv[p]) + rank;
+            if (pos >= kMaxRawContacts) continue;
+            if (vv) {
+                const int kb = g.k[1];
+                const int i = c / kb, j = c % kb;
+                const V3 a = g.vA[i], b = g.vB[j];
+                const V3 dd = b - a;
+                write_contact(d, w, s, pos, g.A, g.B, 0, g.lvA[i], g.lvB[j], 0.5 * (a + b), dd, norm(dd));
+            } else if (c < 1024) {
+                const int kb = g.k[1];
+                const int i = c / kb, j = c % kb;
+                double sd;
+                V3 proj;
+                fv_test(g.fA[i], g.nA[i], g.vB[j], g.dhat, sd, proj);
+                write_contact(d, w, s, pos, g.A, g.B, 0, g.lfA[i], g.lvB[j], 0.5 * (proj + g.vB[j]),
+                              (sd >= 0.0) ? g.nA[i] : -g.nA[i], fabs(sd));
+            } else if (c < 2048) {
+                const int cc = c - 1024;
+                const int ka = g.k[0];
+                const int i = cc / ka, j = cc % ka;
+                double sd;
+                V3 proj;
+                fv_test(g.fB[i], g.nB[i], g.vA[j], g.dhat, sd, proj);
+                write_contact(d, w, s, pos, g.B, g.A, 0, g.lfB[i], g.lvA[j], 0.5 * (proj + g.vA[j]),
+                              (sd >= 0.0) ? g.nB[i] : -g.nB[i], fabs(sd));
+            } else {
+                const int cc = c - 2048;
+                const int kb = g.k[3];
+                const int i = cc / kb, j = cc % kb;
+                V3 c0, c1;
+                closest_points(g.eA[i][0], g.eA[i][1], g.eB[j][0], g.eB[j][1], c0, c1);
+                const V3 delta = c1 - c0;
+                write_contact(d, w, s, pos, g.A, g.B, 1, g.leA[i][0], g.leB[j][0], 0.5 * (c0 + c1), delta,
+                              norm(delta));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Truncation to 2000 contacts by distance (IPCSolver.cpp:571-577): bitonic sort of
+// (distance, position) in global memory per scene, then the first 2000 are kept.
+__global__ void __launch_bounds__(1024) k_np_cap(Dev d, Work w, int gate) {
+    const int s = blockIdx.x;
+    SceneParams& sp = d.sp[s];
+    if (gate && sp.alm_done) return;
+    const int n = w.raw_cnt[s];
+    if (n <= kMaxContacts) {
+        if (threadIdx.x == 0) sp.n_contacts = n;
+        return;
+    }
+    int P = 1;
+    while (P < n) P <<= 1;
+    double* sk = w.sort_key + (long long)s * 65536;
+    int* si = w.sort_idx + (long long)s * 65536;
+    const long long cb = (long long)s * kMaxRawContacts;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        sk[i] = i < n ? w.cdbl[7 * (cb + i) + 6] : 1.0 / 0.0;
+        si[i] = i < n ? i : 0x7fffffff;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < P / 2; k += blockDim.x) {
+                const int i = 2 * k - (k & (stride - 1));
+                const int j = i + stride;
+                const bool up = ((i & size) == 0);
+                const bool gt = key_less(sk[j], si[j], sk[i], si[i]);
+                if (gt == up) {
+                    const double ts = sk[i]; sk[i] = sk[j]; sk[j] = ts;
+                    const int ti = si[i]; si[i] = si[j]; si[j] = ti;
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+    }
+    // gather the kept contacts into the sort scratch, then copy back
+    int* ti = reinterpret_cast<int*>(sk);  // reuse: 2000 * 5 ints fit in the key scratch
+    double* td = w.sort_key + (long long)s * 65536 + 8192;
+    for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
+        const int src = si[i];
+        for (int k = 0; k < 7; ++k) td[7 * i + k] = w.cdbl[7 * (cb + src) + k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
+        const int src = si[i];
+        for (int k = 0; k < 5; ++k) ti[5 * i + k] = w.cint[5 * (cb + src) + k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
+        for (int k = 0; k < 7; ++k) w.cdbl[7 * (cb + i) + k] = td[7 * i + k];
+        for (int k = 0; k < 5; ++k) w.cint[5 * (cb + i) + k] = ti[5 * i + k];
+    }
+    if (threadIdx.x == 0) sp.n_contacts = kMaxContacts;
+}
+
+void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st) {
+    int grid = max_pairs < 4096 ? max_pairs : 4096;
+    if (grid < 1) grid = 1;
+    const int gate = gate_alm ? 1 : 0;
+    timing_begin("np_count", st);
+    hipLaunchKernelGGL(k_np_count, dim3(grid), dim3(kBlock), 0, st, d, w, 0, gate);
+    timing_end("np_count", st);
+    hipLaunchKernelGGL(k_np_scan, dim3(d.S), dim3(kBlock), 0, st, d, w, 0, gate);
+    hipLaunchKernelGGL(k_np_write, dim3(grid), dim3(kBlock), 0, st, d, w, 0, gate);
+    // VV fallback for scenes without FV/EE contacts
+    hipLaunchKernelGGL(k_np_count, dim3(grid), dim3(kBlock), 0, st, d, w, 1, gate);
+    hipLaunchKernelGGL(k_np_scan, dim3(d.S), dim3(kBlock), 0, st, d, w, 1, gate);
+    hipLaunchKernelGGL(k_np_write, dim3(grid), dim3(kBlock), 0, st, d, w, 1, gate);
+    hipLaunchKernelGGL(k_np_cap, dim3(d.S), dim3(1024), 0, st, d, w, gate);
+}
+
+// ------------------------------------------------------------ scene min distance
+// computeSceneMinDistance(bodies) < 1e-9 (AdaptiveBarrier.h:89-162): vertex of body i
+// against every face of body j > i, plane distance with the barycentric test through
+// the 3x3 cofactor inverse. A pair can only pass if its AABB gap is at most
+// 1e-9 + 2e-6 * (longest face edge); pairs beyond the scene's prune margin are skipped.
+__global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w) {
+    __shared__ double wvx[kMaxPrims], wvy[kMaxPrims], wvz[kMaxPrims];
+    const int total = *w.n_prox_total;
+    for (int p = blockIdx.x; p < total; p += gridDim.x) {
+        const int s = w.prox_scene[p];
+        const SceneParams& sp = d.sp[s];
+        if (!(sp.min_pair_dist < sp.dhat * 2.0)) continue;
+        if (w.flag[s]) continue;
+        const int I = w.prox_pairs[2 * p], J = w.prox_pairs[2 * p + 1];
+        const V3 pI = ld3(d.p, I), pJ = ld3(d.p, J);
+        const Q qI = ld4(d.q, I), qJ = ld4(d.q, J);
+        const int nvJ = d.nv[J];
+        for (int k = threadIdx.x; k < nvJ; k += blockDim.x) {
+            const V3 v = world_vertex(d, J, k, pJ, qJ);
+            wvx[k] = v.x; wvy[k] = v.y; wvz[k] = v.z;
+        }
+        __syncthreads();
+        const int nfJ = d.nf[J];
+        const int* F = d.faces + 3 * d.f_off[J];
+        for (int vi = threadIdx.x; vi < d.nv[I]; vi += blockDim.x) {
+            const V3 pt = world_vertex(d, I, vi, pI, qI);
+            bool hit = false;
+            for (int f = 0; f < nfJ && !hit; ++f) {
+                const int a = F[3 * f], b = F[3 * f + 1], c = F[3 * f + 2];
+                const V3 v0 = v3(wvx[a], wvy[a], wvz[a]);
+                const V3 e1 = v3(wvx[b], wvy[b], wvz[b]) - v0;
+                const V3 e2 = v3(wvx[c], wvy[c], wvz[c]) - v0;
+                V3 n = cross(e1, e2);
+                const double area = norm(n);
+                if (area < 1e-12) continue;
+                n = normalized(n);
+                const V3 r = pt - v0;
+                const double dist = fabs(dot(r, n));
+                if (!(dist < 1e-9)) continue;
+                M3 M;
+                M.m[0] = e1.x; M.m[3] = e1.y; M.m[6] = e1.z;
+                M.m[1] = e2.x; M.m[4] = e2.y; M.m[7] = e2.z;
+                M.m[2] = n.x;  M.m[5] = n.y;  M.m[8] = n.z;
+                const V3 co = m3v(m3inverse(M), r);
+                const double b0 = 1.0 - co.x - co.y, b1 = co.x, b2 = co.y;
+                double bmn = b0, bmx = b0;
+                if (b1 < bmn) bmn = b1;
+                if (b2 < bmn) bmn = b2;
+                if (bmx < b1) bmx = b1;
+                if (bmx < b2) bmx = b2;
+                if (bmn >= -1e-6 && bmx <= 1.0 + 1e-6) hit = true;
+            }
+            if (hit) atomicOr(&w.flag[s], 1);
+        }
+        __syncthreads();
+    }
+}
+
+void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_t st) {
+    int grid = max_prox < 2048 ? max_prox : 2048;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_scene_min_flag, dim3(grid), dim3(kBlock), 0, st, d, w);
+}
+
+// ----------------------------------------------------------------------------- CCD
+// Interval CCD (TimeOfImpact.cpp:8-150, IntervalRootFinder.h:20-114,
+// RigidTrajectoryAABB.h:13-174): capped depth-first search (1000 pops), high half
+// popped first, split of the first widest dimension at the round-to-nearest median,
+// positions as the hull over the step, rotation by slerp at t.lo / t.hi.
+struct CPose {
+    V3 p0, p1;
+    Q q0, q1;
+};
+struct IV3 {
+    Iv x, y, z;
+};
+
+__device__ __forceinline__ IV3 vertex_traj(const CPose& ps, V3 lv, Iv t) {
+    const Q qlo = slerp(ps.q0, t.lo, ps.q1), qhi = slerp(ps.q0, t.hi, ps.q1);
+    const V3 a = rotate(qlo, lv), b = rotate(qhi, lv);
+    IV3 r;
+    r.x = iv(dmin(ps.p0.x, ps.p1.x), dmax(ps.p0.x, ps.p1.x)) + iv(dmin(a.x, b.x), dmax(a.x, b.x));
+    r.y = iv(dmin(ps.p0.y, ps.p1.y), dmax(ps.p0.y, ps.p1.y)) + iv(dmin(a.y, b.y), dmax(a.y, b.y));
+    r.z = iv(dmin(ps.p0.z, ps.p1.z), dmax(ps.p0.z, ps.p1.z)) + iv(dmin(a.z, b.z), dmax(a.z, b.z));
+    return r;
+}
+__device__ __forceinline__ IV3 edge_traj(const CPose& ps, V3 a, V3 b, Iv t, Iv al) {
+    const IV3 v0 = vertex_traj(ps, a, t), v1 = vertex_traj(ps, b, t);
+    const Iv A = iv(al.lo, al.hi);
+    IV3 r;
+    r.x = v0.x * (1.0 - A) + v1.x * A;
+    r.y = v0.y * (1.0 - A) + v1.y * A;
+    r.z = v0.z * (1.0 - A) + v1.z * A;
+    return r;
+}
+__device__ __forceinline__ IV3 face_traj(const CPose& ps, V3 a, V3 b, V3 c, Iv t, Iv u, Iv v) {
+    const IV3 v0 = vertex_traj(ps, a, t), v1 = vertex_traj(ps, b, t), v2 = vertex_traj(ps, c, t);
+    const Iv one = iv(1.0, 1.0);
+    const Iv wi = (one - u) - v;
+    IV3 r;
+    r.x = (wi * v0.x + u * v1.x) + v * v2.x;
+    r.y = (wi * v0.y + u * v1.y) + v * v2.y;
+    r.z = (wi * v0.z + u * v1.z) + v * v2.z;
+    return r;
+}
+
+struct CcdQuery {
+    int kind;
+    CPose A, B;
+    V3 a[2], b[3];
+};
+
+__device__ __forceinline__ IV3 ccd_f(const CcdQuery& q, const Iv* x, bool& nan) {
+    IV3 r;
+    if (q.kind == 0) {
+        const IV3 v = vertex_traj(q.A, q.a[0], x[0]);
+        const IV3 e = edge_traj(q.B, q.b[0], q.b[1], x[0], x[1]);
+        r.x = v.x - e.x; r.y = v.y - e.y; r.z = v.z - e.z;
+    } else if (q.kind == 1) {
+        const IV3 ea = edge_traj(q.A, q.a[0], q.a[1], x[0], x[1]);
+        const IV3 eb = edge_traj(q.B, q.b[0], q.b[1], x[0], x[2]);
+        r.x = ea.x - eb.x; r.y = ea.y - eb.y; r.z = ea.z - eb.z;
+    } else {
+        const IV3 v = vertex_traj(q.A, q.a[0], x[0]);
+        const IV3 f = face_traj(q.B, q.b[0], q.b[1], q.b[2], x[0], x[1], x[2]);
+        r.x = v.x - f.x; r.y = v.y - f.y; r.z = v.z - f.z;
+    }
+    if (iv_nan(r.x) || iv_nan(r.y) || iv_nan(r.z)) nan = true;
+    return r;
+}
+
+__device__ __forceinline__ bool ccd_valid(const CcdQuery& q, const Iv* x) {
+    if (q.kind != 2) return true;
+    const Iv u = x[1], v = x[2];
+    const Iv w = (iv(1.0, 1.0) - u) - v;
+    return u.lo >= 0 && u.hi <= 1 && v.lo >= 0 && v.hi <= 1 && w.lo >= 0 && w.hi <= 1;
+}
+
+constexpr int kCcdStack = 100;
+
+// Returns 1 on impact, 0 otherwise, -1 on NaN, -2 on stack overflow.
+__device__ int ccd_toi(const CcdQuery& q, double earliest, double tol, double* toi, int* pops) {
+    const int D = q.kind == 0 ? 2 : 3;
+    const double tl[3] = {tol, 1.0, 1.0};
+    Iv stk[kCcdStack][3];
+    int sp = 0;
+    bool nan = false;
+    Iv x0[3] = {iv(0.0, earliest), iv(0.0, 1.0), iv(0.0, 1.0)};
+    {
+        Iv xt[3] = {iv(0.0, tl[0]), iv(0.0, tl[1]), iv(0.0, tl[2])};
+        (void)ccd_f(q, xt, nan);  // pre-check, result unused (IntervalRootFinder.h:41-48)
+    }
+    for (int k = 0; k < 3; ++k) stk[0][k] = x0[k];
+    sp = 1;
+    Iv best[3];
+    best[0] = iv(1.0 / 0.0, 1.0 / 0.0);
+    bool found = false;
+    int it = 0;
+    for (; sp > 0 && it < 1000; ++it) {
+        Iv x[3];
+        --sp;
+        for (int k = 0; k < 3; ++k) x[k] = stk[sp][k];
+        if (x[0].lo >= best[0].lo) continue;
+        if (!ccd_valid(q, x)) continue;
+        const IV3 y = ccd_f(q, x, nan);
+        if (!(iv_zero_in(y.x) && iv_zero_in(y.y) && iv_zero_in(y.z))) continue;
+        bool within = true;
+        for (int k = 0; k < D; ++k)
+            if (iv_width(x[k]) > tl[k]) { within = false; break; }
+        if (within) {
+            if (x[0].lo < best[0].lo) {
+                for (int k = 0; k < 3; ++k) best[k] = x[k];
+                found = true;
+            }
+            continue;
+        }
+        int sd = 0;
+        double mw = iv_width(x[0]);
+        for (int k = 1; k < D; ++k) {
+            const double wk = iv_width(x[k]);
+            if (wk > mw) { mw = wk; sd = k; }
+        }
+        if (sp + 2 > kCcdStack) { *pops = it; return -2; }
+        const double mid = iv_median(x[sd]);
+        for (int k = 0; k < 3; ++k) { stk[sp][k] = x[k]; stk[sp + 1][k] = x[k]; }
+        stk[sp][sd] = iv(x[sd].lo, mid);
+        stk[sp + 1][sd] = iv(mid, x[sd].hi);
+        sp += 2;
+    }
+    *pops = it;
+    if (nan) return -1;
+    *toi = found ? best[0].lo : 1.0 / 0.0;
+    return found ? 1 : 0;
+}
+
+__device__ CPose ld_cpose(const double* ps) {
+    CPose p;
+    p.p0 = v3(ps[0], ps[1], ps[2]);
+    p.q0 = quat(ps[3], ps[4], ps[5], ps[6]);
+    p.p1 = v3(ps[7], ps[8], ps[9]);
+    p.q1 = quat(ps[10], ps[11], ps[12], ps[13]);
+    return p;
+}
+
+__global__ void k_ccd_batch(int kind, int n, const double* pa, const double* pb, const double* a, const double* b,
+                            const double* earliest, double tol, int* hit, double* toi, int* pops) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    CcdQuery q;
+    q.kind = kind;
+    q.A = ld_cpose(pa + 14 * i);
+    q.B = ld_cpose(pb + 14 * i);
+    const int na = kind == 1 ? 2 : 1, nb = kind == 2 ? 3 : 2;
+    for (int k = 0; k < na; ++k) q.a[k] = ld3(a + 6 * i, k);
+    for (int k = 0; k < nb; ++k) q.b[k] = ld3(b + 9 * i, k);
+    double t = 1.0 / 0.0;
+    int pp = 0;
+    hit[i] = ccd_toi(q, earliest[i], tol, &t, &pp);
+    toi[i] = t;
+    pops[i] = pp;
+}
+
+void launch_ccd_batch(int kind, int n, const double* pa, const double* pb, const double* a, const double* b,
+                      const double* earliest, double tol, int* hit, double* toi, int* pops, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_ccd_batch, dim3((n + 63) / 64), dim3(64), 0, st, kind, n, pa, pb, a, b, earliest, tol, hit,
+                       toi, pops);
+}
+
+// CCDSystem::computeEarliestTOI (CCD.h:22-82) over the FV candidates of every close
+// pair: vertex body moves p -> p + v dt with frozen rotation, face body likewise.
+// Each candidate starts from earliest = 1; the scene result is the minimum.
+__global__ void k_ccd_fv(Dev d, Work w, int gate_any) {
+    const int total = *w.n_pairs_total;
+    const long long ncand = (long long)total * 2048;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncand;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int p = int(c / 2048), local = int(c % 2048);
+        const int s = w.pair_scene[p];
+        SceneParams& sp = d.sp[s];
+        if (gate_any && !(sp.enable_ccd && sp.ccd_any)) continue;
+        const int A = w.pairs[2 * p], B = w.pairs[2 * p + 1];
+        const int* k = w.lcnt + p * 6;
+        int fb, vb, fl, vl;
+        if (local < 1024) {
+            if (k[1] == 0 || local >= k[4] * k[1]) continue;
+            fb = A; vb = B;
+            fl = w.lists[(p * 6 + 4) * kTopK + local / k[1]];
+            vl = w.lists[(p * 6 + 1) * kTopK + local % k[1]];
+        } else {
+            const int cc = local - 1024;
+            if (k[0] == 0 || cc >= k[5] * k[0]) continue;
+            fb = B; vb = A;
+            fl = w.lists[(p * 6 + 5) * kTopK + cc / k[0]];
+            vl = w.lists[(p * 6 + 0) * kTopK + cc % k[0]];
+        }
+        const double dt = sp.dt;
+        CcdQuery q;
+        q.kind = 2;
+        q.A.p0 = ld3(d.p, vb); q.A.q0 = ld4(d.q, vb); q.A.p1 = q.A.p0 + ld3(d.v, vb) * dt; q.A.q1 = q.A.q0;
+        q.B.p0 = ld3(d.p, fb); q.B.q0 = ld4(d.q, fb); q.B.p1 = q.B.p0 + ld3(d.v, fb) * dt; q.B.q1 = q.B.q0;
+        q.a[0] = ld3(d.verts, d.v_off[vb] + vl);
+        const int* f = d.faces + 3 * (d.f_off[fb] + fl);
+        for (int m = 0; m < 3; ++m) q.b[m] = ld3(d.verts, d.v_off[fb] + f[m]);
+        double toi = 1.0 / 0.0;
+        int pops = 0;
+        const int r = ccd_toi(q, 1.0, 1e-8, &toi, &pops);
+        if (r == 1 && toi < 1.0) {
+            unsigned long long* tp = reinterpret_cast<unsigned long long*>(&sp.toi);
+            atomicMin(tp, __double_as_longlong(toi));
+        }
+        if (r < 0) atomicOr(&sp.solve_fail, 2);
+    }
+}
+
+void launch_ccd_earliest(const Dev& d, const Work& w, int max_pairs, hipStream_t st) {
+    long long n = (long long)max_pairs * 2048;
+    long long g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_ccd_fv, dim3((unsigned)g), dim3(256), 0, st, d, w, 1);
+}
+
+__global__ void k_round_ops(const double* a, const double* b, int n, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[6 * i + 0] = add_dn(a[i], b[i]);
+    out[6 * i + 1] = add_up(a[i], b[i]);
+    out[6 * i + 2] = sub_dn(a[i], b[i]);
+    out[6 * i + 3] = sub_up(a[i], b[i]);
+    out[6 * i + 4] = mul_dn(a[i], b[i]);
+    out[6 * i + 5] = mul_up(a[i], b[i]);
+}
+
+void launch_round_ops(const double* a, const double* b, int n, double* out, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_round_ops, dim3((n + 255) / 256), dim3(256), 0, st, a, b, n, out);
+}
+
+}  // namespace nd

@@ -1,0 +1,261 @@
+// Internal device data model of the ndipc library.
+//
+// HBM layout (all FP64 unless noted; N bodies over all scenes, S scenes):
+//   per body      : mesh offsets (int), local AABB [6], mass, inertia [9], pose p[3] q[4],
+//                   velocities v[3] w[3], accelerations al[3] aa[3]
+//   mesh pools    : body-frame vertices [3 per vertex], faces [3 int], unique edges [2 int]
+//                   (shared by every body that uses the same mesh)
+//   per scene     : SceneParams (settings + per-step control flags)
+//   per step      : world AABBs [6/body], LBVH per scene, body pairs (i<j, per scene,
+//                   sorted), 6 top-K lists per close pair [6 x 32 int], contacts per
+//                   scene (up to 40000 raw, 2000 kept), Newton vectors [6/body].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dmath.h"
+
+namespace nd {
+
+constexpr int kTopK = 32;             // ContactAssembler.cpp:221-223
+constexpr int kMaxRawContacts = 40000;  // IPCSolver.cpp:413
+constexpr int kMaxContacts = 2000;    // IPCSolver.cpp:414
+constexpr int kMaxPrims = 2048;       // per-mesh vertex/edge/face limit of the LDS top-K
+constexpr int kMaxSceneBodies = 4096; // per-scene LBVH limit (LDS bitonic sort)
+constexpr int kHitWords = 48;         // 3072 candidate bits per pair (2x1024 FV + 1024 EE)
+
+struct SceneParams {
+    // settings
+    double dhat, kappa, mu, ccd_safety;
+    double alm_ctol, alm_dtol, alm_ctcol, alm_trig, alm_ratio;
+    int enable_friction, enable_ccd, enable_adaptive, alm_max_iters;
+    int barrier_initialized, use_dense, vv_needed, pad1;
+    double kappa_min;
+    // per-step control
+    double dt, adt, f_mu, min_pair_dist, toi, prune_margin, ccd_infl;
+    int needs_forms, has_barrier, has_friction, alm_done;
+    int alm_iters, newton_iters, converged, n_contacts;
+    int nf_count, kappa_flag, ccd_any, solve_fail;
+    double viol, dual, cres;
+    double evals;  // narrow-phase candidate evaluations in this step
+    double n_fv, n_ee;  // FV / EE candidates built this step
+    double evals_total; // candidate evaluations since upload
+};
+
+struct Dev {
+    int S, N;
+    // scene structure
+    const int* body_off;   // [S+1]
+    const int* body_scene; // [N]
+    const int* body_id;    // [N]
+    const int* v_off;      // [N] into verts (vertices)
+    const int* nv;         // [N]
+    const int* f_off;      // [N] into faces
+    const int* nf;         // [N]
+    const int* e_off;      // [N] into edges
+    const int* ne;         // [N]
+    const double* verts;   // pool, 3 per vertex
+    const int* faces;      // pool, 3 per face
+    const int* edges;      // pool, 2 per edge
+    const double* lbox;    // [N*6] local AABB (min, max)
+    const double* max_edge;// [N]
+    const double* mass;    // [N]
+    const double* inertia; // [N*9]
+    // joints
+    const int* joint_off;  // [S+1]
+    const int* jtype;      // [J]
+    const int* ja;         // [J] global body
+    const int* jb;         // [J] global body or -1
+    const double* jdata;   // [J*16]
+    double* jmu;           // [J]
+    double* jlambda;       // [J*6]
+    double* jC;            // [J*6] scratch
+    // allowed pairs (keys sorted within each scene)
+    const int* allowed_off;                 // [S+1]
+    const unsigned long long* allowed_key;  // pair keys (lo << 32 | hi) of body ids
+    const double* allowed_mu;
+    // state
+    double *p, *q, *v, *w, *al, *aa;
+    SceneParams* sp;
+};
+
+// Per-step work buffers.
+struct Work {
+    double* wbox;            // [N*6] world AABB at step start
+    double* pose_p;          // [N*3] pose used by the current narrow phase
+    double* pose_q;          // [N*4]
+    // LBVH (per scene segment at body_off)
+    int* leaf_body;          // [N] sorted leaf -> global body
+    int* node_child;         // [N*2] internal node children (>= 0 internal, < 0: ~leaf)
+    int* node_range;         // [N*2]
+    double* node_box;        // [N*6]
+    int* nleaves;            // [S]
+    // body pairs
+    long long* pair_cap_off; // [S+1] capacity offsets (n(n-1)/2 per scene)
+    int* pair_cnt;           // [S]
+    int* pair_off;           // [S+1] compact offsets (scan of pair_cnt)
+    int* pairs;              // [2 * pair capacity] global bodies, compact
+    int* pair_scene;         // [pair capacity]
+    int* n_pairs_total;      // [1]
+    int* prox_cnt;           // [S]
+    int* prox_off;           // [S+1]
+    int* prox_pairs;         // [2 * pair capacity]
+    int* prox_scene;         // [pair capacity]
+    int* n_prox_total;       // [1]
+    // top-K lists per pair: [pair][6][32]; order vA, vB, eA, eB, fA, fB
+    int* lists;
+    int* lcnt;               // [pair][6]
+    // narrow phase
+    unsigned long long* hits;// [pair][kHitWords]
+    int* hit_fv;             // [pair]
+    int* hit_ee;             // [pair]
+    int* hit_off_fv;         // [pair]
+    int* hit_off_ee;         // [pair]
+    int* raw_cnt;            // [S]
+    int* cint;               // [S * 40000 * 5]
+    double* cdbl;            // [S * 40000 * 7]
+    int* sort_idx;           // [S * 65536] scratch for the 2000-cap sort
+    double* sort_key;        // [S * 65536]
+    // solver vectors, 6 per body
+    double *x, *xhat, *xprev, *vprev, *g, *dx, *xt, *cg, *r, *z, *pv, *Ap, *tmp;
+    double* D;               // [N*36]
+    double* Dinv;            // [N*36]
+    double* nf;              // [S*2000] normal forces
+    double* cblk;            // [S*2000*3*36] contact blocks AA, BB, AB (scaled)
+    double* cgrad;           // [S*2000*24] barrier gA gB, friction gA gB
+    int* cactive;            // [S*2000]
+    int* csr_off;            // [N+1 + S] body -> contact entries
+    int* csr_ent;            // [S*2000*2]
+    double* jblk;            // [J*3*36] hinge blocks AA, BB, AB
+    double* jgrad;           // [J*12]
+    double* dense;           // [dense_cap] dense fallback matrices
+    long long* dense_off;    // [S]
+    int* flag;               // [S] scene-min flag scratch
+};
+
+// Launch wrappers (defined in the .hip files).
+void launch_world_boxes(const Dev& d, const Work& w, hipStream_t st);
+void launch_lbvh_build(const Dev& d, const Work& w, hipStream_t st);
+// mode 0: count + emit close pairs (inflation r, box gap <= r) into w.pairs
+// mode 1: min box gap over pairs at inflation r into sp.min_pair_dist; emit prox pairs
+//         whose gap <= sp.prune_margin into w.prox_pairs
+// mode 2: any allowed pair at inflation sp-dependent (ccd) into sp.ccd_any
+void launch_broad_phase(const Dev& d, const Work& w, int mode, double r, bool use_scene_r, const int* scene_mask,
+                        hipStream_t st);
+void launch_topk(const Dev& d, const Work& w, int max_pairs, hipStream_t st);
+void launch_pose_from_state(const Dev& d, const Work& w, const double* x, hipStream_t st);
+void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st);
+void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_t st);
+void launch_ccd_earliest(const Dev& d, const Work& w, int max_pairs, hipStream_t st);
+void launch_ccd_batch(int kind, int n, const double* pa, const double* pb, const double* a, const double* b,
+                      const double* earliest, double tol, int* hit, double* toi, int* pops, hipStream_t st);
+void launch_round_ops(const double* a, const double* b, int n, double* out, hipStream_t st);
+void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, hipStream_t st);
+
+void launch_step_begin(const Dev& d, const Work& w, const double* dt, hipStream_t st);
+void launch_kappa(const Dev& d, const Work& w, hipStream_t st);
+void launch_ccd_dt(const Dev& d, const Work& w, hipStream_t st);
+void launch_pack(const Dev& d, const Work& w, hipStream_t st);
+void launch_contacts_setup(const Dev& d, const Work& w, hipStream_t st);
+void launch_newton(const Dev& d, const Work& w, hipStream_t st);
+void launch_alm_post(const Dev& d, const Work& w, hipStream_t st);
+void launch_finalize(const Dev& d, const Work& w, hipStream_t st);
+void launch_barrier_init_ccd(const Dev& d, const Work& w, hipStream_t st);
+// which 0: DistanceBarrierForm, 1: FrictionForm, on the contacts stored for scene s
+void launch_forms_eval(const Dev& d, const Work& w, int s, int which, double dhat, double kappa, double mu,
+                       double eps, double dt, const double* x, double* value, double* grad, double* hess, double* nf,
+                       hipStream_t st);
+void launch_pcg(int nb, const double* diag, int noff, const int* offij, const double* offb, const double* b,
+                double tol, int max_it, double* x, int* iters, double* rel, double* scratch, hipStream_t st);
+
+// Kernel timing hooks (ctx.hip): kernels that bench.py rooflines are bracketed with
+// HIP events on the launching stream while timing is enabled.
+void timing_begin(const char* name, hipStream_t st);
+void timing_end(const char* name, hipStream_t st);
+
+#define ND_CHECK_LAUNCH() (void)hipGetLastError()
+
+// ---- shared device helpers
+__device__ __forceinline__ V3 ld3(const double* a, int i) { return v3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+__device__ __forceinline__ Q ld4(const double* a, int i) { return quat(a[4 * i], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3]); }
+__device__ __forceinline__ void st3(double* a, int i, V3 v) { a[3 * i] = v.x; a[3 * i + 1] = v.y; a[3 * i + 2] = v.z; }
+__device__ __forceinline__ void st4(double* a, int i, Q q) { a[4 * i] = q.w; a[4 * i + 1] = q.x; a[4 * i + 2] = q.y; a[4 * i + 3] = q.z; }
+__device__ __forceinline__ Box3 ldbox(const double* b, int i) {
+    Box3 r;
+    r.mn = v3(b[6 * i], b[6 * i + 1], b[6 * i + 2]);
+    r.mx = v3(b[6 * i + 3], b[6 * i + 4], b[6 * i + 5]);
+    return r;
+}
+__device__ __forceinline__ unsigned long long pair_key(int a, int b) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return ((unsigned long long)(unsigned)lo << 32) | (unsigned)hi;
+}
+// IPCSolver::isContactPairAllowed (IPCSolver.cpp:283-289): empty list allows all.
+__device__ __forceinline__ bool pair_allowed(const Dev& d, int s, int ida, int idb) {
+    const int lo = d.allowed_off[s], hi = d.allowed_off[s + 1];
+    if (lo == hi) return true;
+    const unsigned long long k = pair_key(ida, idb);
+    int a = lo, b = hi;
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if (d.allowed_key[m] < k) a = m + 1;
+        else b = m;
+    }
+    return a < hi && d.allowed_key[a] == k;
+}
+__device__ __forceinline__ double allowed_mu(const Dev& d, int s, int ida, int idb) {
+    const int lo = d.allowed_off[s], hi = d.allowed_off[s + 1];
+    const unsigned long long k = pair_key(ida, idb);
+    int a = lo, b = hi;
+    while (a < b) {
+        const int m = (a + b) >> 1;
+        if (d.allowed_key[m] < k) a = m + 1;
+        else b = m;
+    }
+    if (a < hi && d.allowed_key[a] == k) return d.allowed_mu[a];
+    return -1.0;
+}
+// World position of body-local vertex lv of global body b at pose (p, q).
+__device__ __forceinline__ V3 world_vertex(const Dev& d, int b, int lv, V3 p, Q q) {
+    return p + rotate(q, ld3(d.verts, d.v_off[b] + lv));
+}
+
+// Work-group reductions (blockDim.x == 256).
+template <typename T, typename Op>
+__device__ __forceinline__ T wg_reduce(T v, T* sh, Op op) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = op(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    const T r = sh[0];
+    __syncthreads();
+    return r;
+}
+struct OpAdd { __device__ double operator()(double a, double b) const { return a + b; } };
+struct OpMax { __device__ double operator()(double a, double b) const { return a < b ? b : a; } };
+struct OpMin { __device__ double operator()(double a, double b) const { return b < a ? b : a; } };
+struct OpAddI { __device__ int operator()(int a, int b) const { return a + b; } };
+struct OpOrI { __device__ int operator()(int a, int b) const { return a | b; } };
+
+// Exclusive scan over blockDim.x ints in shared memory; returns the total.
+__device__ __forceinline__ int wg_exclusive_scan(int v, int* sh, int* out_excl) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        int add = (t >= off) ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const int incl = sh[t];
+    const int total = sh[blockDim.x - 1];
+    __syncthreads();
+    *out_excl = incl - v;
+    return total;
+}
+
+}  // namespace nd

@@ -1,0 +1,1231 @@
+// Newton / ALM solver kernels: one work-group per scene runs
+// NewtonSolver::minimize (NewtonSolver.cpp:10-95) with the backtracking line search
+// (LineSearch.cpp:6-58) on the SimulationProblem objective (IPCSolver.cpp:137-262):
+// inertia (body-frame, unrotated), h^2 gravity, ALM joints, h^2 barrier and h^2 lagged
+// friction. The linear solve replaces SimplicialLDLT with a block-Jacobi PCG over 6x6
+// body blocks (dense Cholesky for scenes with hinge joints). Scenes are independent,
+// so a batch of scenes is one launch of each kernel.
+#include "ndipc_internal.h"
+
+namespace nd {
+
+constexpr int kBlk = 256;
+constexpr double kG = -9.81;  // gravity (0, -9.81, 0), IPCSolver.cpp:142
+
+// ------------------------------------------------------------------ potentials
+// BarrierPotential (Barrier.cpp:11-46)
+__device__ __forceinline__ double bar_v(double d, double dh, double k) {
+    if (d >= dh) return 0.0;
+    if (d <= 0.0) return 1.0 / 0.0;
+    const double r = d / dh;
+    return -k * (d - dh) * (d - dh) * log(r);
+}
+__device__ __forceinline__ double bar_g(double d, double dh, double k) {
+    if (d >= dh) return 0.0;
+    if (d <= 0.0) return -1.0 / 0.0;
+    const double diff = d - dh;
+    const double lr = log(d / dh);
+    return -k * (2 * diff * lr + diff * diff / d);
+}
+__device__ __forceinline__ double bar_h(double d, double dh, double k) {
+    if (d >= dh) return 0.0;
+    if (d <= 0.0) return 1.0 / 0.0;
+    const double diff = d - dh;
+    const double lr = log(d / dh);
+    return -k * (2 * lr + 4 * diff / d - (diff * diff) / (d * d));
+}
+
+struct ScnCtx {
+    int s, b0, n, nd, j0, j1, nc;
+    long long cb;  // contact base (raw contact index)
+    double h2, dhat, kappa, fmu, feps, fdt;
+    int has_barrier, has_friction, nfc;
+};
+
+__device__ __forceinline__ V3 xpos(const double* x, int b) { return v3(x[6 * b], x[6 * b + 1], x[6 * b + 2]); }
+__device__ __forceinline__ V3 xrot(const double* x, int b) { return v3(x[6 * b + 3], x[6 * b + 4], x[6 * b + 5]); }
+
+// DistanceBarrierForm::contactDistance (DistanceBarrierForm.cpp:89-98)
+__device__ __forceinline__ double contact_dist(const Work& w, long long c, const double* X) {
+    const int A = w.cint[5 * c], B = w.cint[5 * c + 1];
+    const double* cd = w.cdbl + 7 * c;
+    const V3 n = v3(cd[3], cd[4], cd[5]);
+    const double pr = dot(xpos(X, B) - xpos(X, A), n);
+    return pr > 1e-9 ? pr : 1e-9;
+}
+
+// FrictionForm::computeTangentDisplacement + Jacobians (FrictionForm.cpp:44-204)
+struct FKin {
+    V3 u;
+    double JA[3][6], JB[3][6];
+};
+__device__ void fric_kin(const Dev& d, const Work& w, long long c, const double* X, double dt, FKin& k) {
+    const int A = w.cint[5 * c], B = w.cint[5 * c + 1];
+    const double* cd = w.cdbl + 7 * c;
+    const V3 pt = v3(cd[0], cd[1], cd[2]);
+    const V3 nr = v3(cd[3], cd[4], cd[5]);
+    const V3 pA0 = ld3(d.p, A), pB0 = ld3(d.p, B);
+    const Q qA0 = ld4(d.q, A), qB0 = ld4(d.q, B);
+    const V3 pA = xpos(X, A), pB = xpos(X, B);
+    const Q qA = apply_theta(qA0, xrot(X, A)), qB = apply_theta(qB0, xrot(X, B));
+    const V3 dA = pA - pA0, dB = pB - pB0;
+    const V3 lA = rotate(qconj(qA0), pt - pA0), lB = rotate(qconj(qB0), pt - pB0);
+    const V3 rA = rotate(qA, lA), rB = rotate(qB, lB);
+    V3 wA = v3(0, 0, 0), wB = v3(0, 0, 0), vA = v3(0, 0, 0), vB = v3(0, 0, 0);
+    if (dt > 1e-12) {
+        wA = xrot(X, A) / dt;
+        wB = xrot(X, B) / dt;
+        vA = dA / dt;
+        vB = dB / dt;
+    }
+    vA = vA + cross(wA, rA);
+    vB = vB + cross(wB, rB);
+    const V3 rel = vB - vA;
+    const V3 nn = (sqnorm(nr) > 1e-16) ? normalized(nr) : v3(0.0, 1.0, 0.0);
+    const V3 tv = rel - dot(rel, nn) * nn;
+    k.u = tv * dt;
+    M3 P;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P.m[3 * i + j] = (i == j ? 1.0 : 0.0) - comp(nn, i) * comp(nn, j);
+    const M3 PA = m3mul(P, skew(rA)), PB = m3mul(P, skew(rB));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            k.JA[i][j] = -P.m[3 * i + j];
+            k.JB[i][j] = P.m[3 * i + j];
+            k.JA[i][3 + j] = PA.m[3 * i + j];
+            k.JB[i][3 + j] = -PB.m[3 * i + j];
+        }
+}
+// FrictionPotential (Friction.cpp:5-62)
+__device__ __forceinline__ double fric_v(V3 u, double N, double mu, double eps) {
+    if (N <= 0) return 0.0;
+    const double un = norm(u);
+    const double sn = sqrt(un * un + eps * eps);
+    return mu * N * (sn - eps);
+}
+__device__ __forceinline__ V3 fric_g(V3 u, double N, double mu, double eps) {
+    if (N <= 0) return v3(0, 0, 0);
+    const double un = norm(u);
+    if (un < 1e-12) return v3(0, 0, 0);
+    const double sn = sqrt(un * un + eps * eps);
+    const double scale = mu * N / sn;
+    return scale * u;
+}
+__device__ __forceinline__ void fric_h(V3 u, double N, double mu, double eps, double H[9]) {
+    for (int i = 0; i < 9; ++i) H[i] = 0.0;
+    if (N <= 0) return;
+    const double un = norm(u);
+    if (un < 1e-12) {
+        const double s = mu * N / eps;
+        H[0] = s * 1.0; H[4] = s * 1.0; H[8] = s * 1.0;
+        H[1] = H[2] = H[3] = H[5] = H[6] = H[7] = s * 0.0;
+        return;
+    }
+    const double sn = sqrt(un * un + eps * eps);
+    const double scale = mu * N / sn;
+    const double s2 = sn * sn;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) H[3 * i + j] = scale * ((i == j ? 1.0 : 0.0) - (comp(u, i) * comp(u, j)) / s2);
+}
+
+// ------------------------------------------------------------------ joints
+// FixedJoint / HingeJoint constraint and Jacobian (FixedJoint.cpp:141-173,
+// HingeJoint.cpp:36-69, JointPrimitives.h:63-108,208-245,281-324). J is 6 x 12
+// (columns: body A dofs, body B dofs).
+struct JEval {
+    double C[6];
+    double J[6][12];
+};
+__device__ void joint_eval(const Dev& d, int j, const double* X, JEval& e) {
+    const int type = d.jtype[j];
+    const int A = d.ja[j], B = d.jb[j];
+    const double* jd = d.jdata + 16 * j;
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 12; ++c) e.J[r][c] = 0.0;
+    if (type == 0) {
+        const V3 tp = v3(jd[0], jd[1], jd[2]);
+        const Q tq = quat(jd[3], jd[4], jd[5], jd[6]);
+        const V3 dp = xpos(X, A) - tp;
+        const V3 tt = rotation_vector(qmul(tq, qconj(ld4(d.q, A))));
+        const V3 e3 = xrot(X, A) - tt;
+        e.C[0] = dp.x; e.C[1] = dp.y; e.C[2] = dp.z; e.C[3] = e3.x; e.C[4] = e3.y; e.C[5] = e3.z;
+        for (int r = 0; r < 6; ++r) e.J[r][r] = 1.0;
+        return;
+    }
+    const V3 anA = v3(jd[0], jd[1], jd[2]), anB = v3(jd[3], jd[4], jd[5]);
+    const V3 axA = v3(jd[6], jd[7], jd[8]), axB = v3(jd[9], jd[10], jd[11]);
+    const V3 pA = xpos(X, A), pB = xpos(X, B);
+    const Q qA = update_rotation(ld4(d.q, A), xrot(X, A));
+    const Q qB = update_rotation(ld4(d.q, B), xrot(X, B));
+    const V3 rA = rotate(qA, anA), rB = rotate(qB, anB);
+    const V3 pv = (pA + rA) - (pB + rB);
+    const V3 nA = rotate(qA, axA), nB = rotate(qB, axB);
+    const V3 ac = cross(nA, nB);
+    e.C[0] = pv.x; e.C[1] = pv.y; e.C[2] = pv.z; e.C[3] = ac.x; e.C[4] = ac.y; e.C[5] = ac.z;
+    const M3 sA = skew(rA), sB = skew(rB);
+    for (int i = 0; i < 3; ++i) {
+        e.J[i][i] = 1.0;
+        e.J[i][6 + i] = -1.0;
+        for (int k = 0; k < 3; ++k) {
+            e.J[i][3 + k] = -sA.m[3 * i + k];
+            e.J[i][9 + k] = sB.m[3 * i + k];
+        }
+    }
+    const M3 dA = m3mul(skew(nB), skew(nA));
+    M3 mnA = skew(nA);
+    for (int k = 0; k < 9; ++k) mnA.m[k] = -mnA.m[k];
+    const M3 dB = m3mul(mnA, skew(nB));
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            e.J[3 + i][3 + k] = dA.m[3 * i + k];
+            e.J[3 + i][9 + k] = dB.m[3 * i + k];
+        }
+}
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ void load_scn(const Dev& d, int s, ScnCtx& c) {
+    const SceneParams& sp = d.sp[s];
+    c.s = s;
+    c.b0 = d.body_off[s];
+    c.n = d.body_off[s + 1] - c.b0;
+    c.nd = 6 * c.n;
+    c.j0 = d.joint_off[s];
+    c.j1 = d.joint_off[s + 1];
+    c.nc = sp.n_contacts;
+    c.cb = (long long)s * kMaxRawContacts;
+    c.h2 = sp.adt * sp.adt;
+    c.dhat = sp.dhat > 1e-9 ? sp.dhat : 1e-9;
+    c.kappa = sp.kappa > 0.0 ? sp.kappa : 0.0;
+    c.fmu = sp.f_mu;
+    c.feps = 1e-8;
+    c.fdt = sp.adt > 0.0 ? sp.adt : 0.0;
+    c.has_barrier = sp.has_barrier;
+    c.has_friction = sp.has_friction;
+    c.nfc = sp.nf_count;
+}
+
+// Per-contact element data at X: barrier b'' (unscaled), gradients, friction blocks.
+// cgrad[c][18] = gbA(3) gbB(3) gfA(6) gfB(6);  cblk[c][3*36] = h2 * friction AA BB AB;
+// cactive[c] bit0 barrier active, bit1 friction active; the barrier b'' is kept in
+// cgrad slot via a separate array (tmp) — see elem_b2.
+__device__ void eval_elements(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* b2arr) {
+    for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+        const long long c = sc.cb + i;
+        double* g = w.cgrad + 18 * ((long long)sc.s * kMaxContacts + i);
+        for (int k = 0; k < 18; ++k) g[k] = 0.0;
+        int act = 0;
+        const double* cd = w.cdbl + 7 * c;
+        const V3 n = v3(cd[3], cd[4], cd[5]);
+        b2arr[i] = 0.0;
+        if (sc.has_barrier) {
+            const double dd = contact_dist(w, c, X);
+            if (dd < sc.dhat) {
+                const double b1 = bar_g(dd, sc.dhat, sc.kappa);
+                g[0] = -b1 * n.x; g[1] = -b1 * n.y; g[2] = -b1 * n.z;
+                g[3] = b1 * n.x; g[4] = b1 * n.y; g[5] = b1 * n.z;
+                b2arr[i] = bar_h(dd, sc.dhat, sc.kappa);
+                act |= 1;
+            }
+        }
+        if (sc.has_friction && i < sc.nfc && w.nf[(long long)sc.s * kMaxContacts + i] > 0.0) {
+            const double N = w.nf[(long long)sc.s * kMaxContacts + i];
+            FKin k;
+            fric_kin(d, w, c, X, sc.fdt, k);
+            const V3 fg = fric_g(k.u, N, sc.fmu, sc.feps);
+            for (int j = 0; j < 6; ++j) {
+                g[6 + j] = k.JA[0][j] * fg.x + k.JA[1][j] * fg.y + k.JA[2][j] * fg.z;
+                g[12 + j] = k.JB[0][j] * fg.x + k.JB[1][j] * fg.y + k.JB[2][j] * fg.z;
+            }
+            double H[9];
+            fric_h(k.u, N, sc.fmu, sc.feps, H);
+            double* blk = w.cblk + 108 * ((long long)sc.s * kMaxContacts + i);
+            // (X^T H) Y for (A,A), (B,B), (A,B)
+            for (int which = 0; which < 3; ++which) {
+                double (*Xm)[6] = (which == 1) ? k.JB : k.JA;
+                double (*Ym)[6] = (which == 0) ? k.JA : k.JB;
+                double T[6][3];
+                for (int r = 0; r < 6; ++r)
+                    for (int cc = 0; cc < 3; ++cc)
+                        T[r][cc] = Xm[0][r] * H[0 * 3 + cc] + Xm[1][r] * H[1 * 3 + cc] + Xm[2][r] * H[2 * 3 + cc];
+                for (int r = 0; r < 6; ++r)
+                    for (int cc = 0; cc < 6; ++cc)
+                        blk[which * 36 + r * 6 + cc] = (T[r][0] * Ym[0][cc] + T[r][1] * Ym[1][cc] + T[r][2] * Ym[2][cc]) * sc.h2;
+            }
+            act |= 2;
+        }
+        w.cactive[(long long)sc.s * kMaxContacts + i] = act;
+    }
+    // hinge joint element blocks (mu J^T J) and gradients J^T (mu C + lambda)
+    for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
+        JEval e;
+        joint_eval(d, j, X, e);
+        const double mu = d.jmu[j];
+        double f[6];
+        for (int r = 0; r < 6; ++r) f[r] = mu * e.C[r] + d.jlambda[6 * j + r];
+        double* jg = w.jgrad + 12 * j;
+        for (int c = 0; c < 12; ++c) {
+            double acc = 0.0;
+            for (int r = 0; r < 6; ++r) acc = acc + e.J[r][c] * f[r];
+            jg[c] = acc;
+        }
+        double* jb = w.jblk + 108 * j;
+        for (int which = 0; which < 3; ++which) {
+            const int oa = (which == 1) ? 6 : 0, ob = (which == 0) ? 0 : 6;
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) {
+                    double acc = 0.0;
+                    for (int k = 0; k < 6; ++k) acc = acc + e.J[k][oa + r] * e.J[k][ob + c];
+                    jb[which * 36 + r * 6 + c] = acc * mu;
+                }
+        }
+        for (int r = 0; r < 6; ++r) d.jC[6 * j + r] = e.C[r];
+    }
+    __syncthreads();
+}
+
+// Gradient of the Newton objective (IPCSolver.cpp:167-190) per body, gathered in
+// contact order through the body CSR. Returns max |g|.
+__device__ double objective_grad(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* G,
+                                 double* cgout, double* red) {
+    double gmax = 0.0;
+    const double* XH = w.xhat;
+    const int* off = w.csr_off + sc.b0 + sc.s;
+    const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int b = sc.b0 + li;
+        const double m = d.mass[b];
+        const double* I = d.inertia + 9 * b;
+        double g[6];
+        const V3 dp = xpos(X, b) - xpos(XH, b);
+        const V3 gp = m * dp;
+        const V3 dt = xrot(X, b) - xrot(XH, b);
+        g[0] = 0.0 + gp.x; g[1] = 0.0 + gp.y; g[2] = 0.0 + gp.z;
+        g[3] = 0.0 + (I[0] * dt.x + I[1] * dt.y + I[2] * dt.z);
+        g[4] = 0.0 + (I[3] * dt.x + I[4] * dt.y + I[5] * dt.z);
+        g[5] = 0.0 + (I[6] * dt.x + I[7] * dt.y + I[8] * dt.z);
+        const double fy = (-m) * kG;
+        g[0] = g[0] + sc.h2 * ((-m) * 0.0);
+        g[1] = g[1] + sc.h2 * fy;
+        g[2] = g[2] + sc.h2 * ((-m) * 0.0);
+        g[3] = g[3] + sc.h2 * 0.0; g[4] = g[4] + sc.h2 * 0.0; g[5] = g[5] + sc.h2 * 0.0;
+        // joints (ConstraintForm accumulates each joint's J^T f)
+        double gj[6] = {0, 0, 0, 0, 0, 0};
+        for (int j = sc.j0; j < sc.j1; ++j) {
+            const double* jg = w.jgrad + 12 * j;
+            if (d.ja[j] == b) for (int k = 0; k < 6; ++k) gj[k] += jg[k];
+            if (d.jb[j] == b) for (int k = 0; k < 6; ++k) gj[k] += jg[6 + k];
+        }
+        for (int k = 0; k < 6; ++k) g[k] = g[k] + gj[k];
+        double gb[6] = {0, 0, 0, 0, 0, 0}, gf[6] = {0, 0, 0, 0, 0, 0};
+        for (int e = off[li]; e < off[li + 1]; ++e) {
+            const int c = ent[e] >> 1, side = ent[e] & 1;
+            const double* cg = w.cgrad + 18 * ((long long)sc.s * kMaxContacts + c);
+            for (int k = 0; k < 3; ++k) gb[k] += cg[side * 3 + k];
+            for (int k = 0; k < 6; ++k) gf[k] += cg[6 + side * 6 + k];
+        }
+        for (int k = 0; k < 6; ++k) {
+            g[k] = g[k] + sc.h2 * gb[k];
+            g[k] = g[k] + sc.h2 * gf[k];
+            G[6 * li + k] = g[k];
+            if (cgout) cgout[6 * li + k] = (0.0 + sc.h2 * gb[k]) + sc.h2 * gf[k];
+            gmax = dmax(gmax, fabs(g[k]));
+        }
+    }
+    return wg_reduce(gmax, red, OpMax());
+}
+
+// Objective value (IPCSolver.cpp:154-165) and contact value (IPCSolver.cpp:223-235).
+__device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* red,
+                                double* out_val, double* out_cval) {
+    const double* XH = w.xhat;
+    double vin = 0.0, vg = 0.0;
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int b = sc.b0 + li;
+        const double m = d.mass[b];
+        const double* I = d.inertia + 9 * b;
+        const V3 dp = xpos(X, b) - xpos(XH, b);
+        vin += 0.5 * m * sqnorm(dp);
+        const V3 dt = xrot(X, b) - xrot(XH, b);
+        const V3 r = 0.5 * dt;
+        const double s0 = r.x * I[0] + r.y * I[3] + r.z * I[6];
+        const double s1 = r.x * I[1] + r.y * I[4] + r.z * I[7];
+        const double s2 = r.x * I[2] + r.y * I[5] + r.z * I[8];
+        vin += s0 * dt.x + s1 * dt.y + s2 * dt.z;
+        vg -= m * (0.0 * X[6 * b] + kG * X[6 * b + 1] + 0.0 * X[6 * b + 2]);
+    }
+    double vj = 0.0;
+    for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
+        JEval e;
+        joint_eval(d, j, X, e);
+        double t1 = 0.0, sq = 0.0;
+        for (int k = 0; k < 6; ++k) t1 += d.jlambda[6 * j + k] * e.C[k];
+        for (int k = 0; k < 6; ++k) sq += e.C[k] * e.C[k];
+        vj += t1 + 0.5 * d.jmu[j] * sq;
+    }
+    double vb = 0.0, vf = 0.0;
+    for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+        const long long c = sc.cb + i;
+        if (sc.has_barrier) vb += bar_v(contact_dist(w, c, X), sc.dhat, sc.kappa);
+        if (sc.has_friction && i < sc.nfc) {
+            const double N = w.nf[(long long)sc.s * kMaxContacts + i];
+            if (N > 0.0) {
+                FKin k;
+                fric_kin(d, w, c, X, sc.fdt, k);
+                vf += fric_v(k.u, N, sc.fmu, sc.feps);
+            }
+        }
+    }
+    vin = wg_reduce(vin, red, OpAdd());
+    vg = wg_reduce(vg, red, OpAdd());
+    vj = wg_reduce(vj, red, OpAdd());
+    vb = wg_reduce(vb, red, OpAdd());
+    vf = wg_reduce(vf, red, OpAdd());
+    double val = vin;
+    val += sc.h2 * vg;
+    val += vj;
+    double cval = 0.0;
+    if (sc.has_barrier) { val += sc.h2 * vb; cval += sc.h2 * vb; }
+    if (sc.has_friction) { val += sc.h2 * vf; cval += sc.h2 * vf; }
+    *out_val = val;
+    *out_cval = cval;
+}
+
+// Diagonal 6x6 blocks: inertia + joints + h2 (barrier, then friction) contributions.
+__device__ void assemble_diag(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr) {
+    const int* off = w.csr_off + sc.b0 + sc.s;
+    const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int b = sc.b0 + li;
+        double* D = w.D + 36 * b;
+        for (int k = 0; k < 36; ++k) D[k] = 0.0;
+        const double m = d.mass[b];
+        D[0] = m; D[7] = m; D[14] = m;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) D[(3 + r) * 6 + 3 + c] = d.inertia[9 * b + 3 * r + c];
+        for (int j = sc.j0; j < sc.j1; ++j) {
+            const double* jb = w.jblk + 108 * j;
+            if (d.ja[j] == b) for (int k = 0; k < 36; ++k) D[k] = D[k] + jb[k];
+            if (d.jb[j] == b) for (int k = 0; k < 36; ++k) D[k] = D[k] + jb[36 + k];
+        }
+        for (int e = off[li]; e < off[li + 1]; ++e) {
+            const int c = ent[e] >> 1;
+            const long long cc = sc.cb + c;
+            const long long ce = (long long)sc.s * kMaxContacts + c;
+            if (!(w.cactive[ce] & 1)) continue;
+            const double* cd = w.cdbl + 7 * cc;
+            const double n[3] = {cd[3], cd[4], cd[5]};
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q) D[r * 6 + q] = D[r * 6 + q] + (b2arr[c] * (n[r] * n[q])) * sc.h2;
+        }
+        for (int e = off[li]; e < off[li + 1]; ++e) {
+            const int c = ent[e] >> 1, side = ent[e] & 1;
+            const long long cc = sc.cb + c;
+            const long long ce = (long long)sc.s * kMaxContacts + c;
+            if (!(w.cactive[ce] & 2)) continue;
+            const double* blk = w.cblk + 108 * ce + side * 36;
+            for (int k = 0; k < 36; ++k) D[k] = D[k] + blk[k];
+        }
+    }
+}
+
+// y = H x over the scene (6 per body).
+__device__ void spmv(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* x, double* y) {
+    const int* off = w.csr_off + sc.b0 + sc.s;
+    const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int b = sc.b0 + li;
+        const double* D = w.D + 36 * b;
+        double acc[6];
+        for (int r = 0; r < 6; ++r) {
+            double s = 0.0;
+            for (int c = 0; c < 6; ++c) s += D[r * 6 + c] * x[6 * li + c];
+            acc[r] = s;
+        }
+        for (int e = off[li]; e < off[li + 1]; ++e) {
+            const int c = ent[e] >> 1, side = ent[e] & 1;
+            const long long cc = sc.cb + c;
+            const long long ce = (long long)sc.s * kMaxContacts + c;
+            const int act = w.cactive[ce];
+            if (!act) continue;
+            const int other = (side == 0 ? w.cint[5 * cc + 1] : w.cint[5 * cc]) - sc.b0;
+            if (act & 1) {
+                const double* cd = w.cdbl + 7 * cc;
+                const double n[3] = {cd[3], cd[4], cd[5]};
+                for (int r = 0; r < 3; ++r) {
+                    double s = 0.0;
+                    for (int q = 0; q < 3; ++q) s += (-(b2arr[c] * (n[r] * n[q])) * sc.h2) * x[6 * other + q];
+                    acc[r] += s;
+                }
+            }
+            if (act & 2) {
+                const double* blk = w.cblk + 108 * ce + 72;  // AB
+                for (int r = 0; r < 6; ++r) {
+                    double s = 0.0;
+                    for (int q = 0; q < 6; ++q)
+                        s += (side == 0 ? blk[r * 6 + q] : blk[q * 6 + r]) * x[6 * other + q];
+                    acc[r] += s;
+                }
+            }
+        }
+        for (int j = sc.j0; j < sc.j1; ++j) {
+            if (d.jtype[j] != 1) continue;
+            const double* jb = w.jblk + 108 * j + 72;
+            int other = -1, side = 0;
+            if (d.ja[j] == b) { other = d.jb[j]; side = 0; }
+            else if (d.jb[j] == b) { other = d.ja[j]; side = 1; }
+            if (other < 0) continue;
+            const int lo = other - sc.b0;
+            for (int r = 0; r < 6; ++r) {
+                double s = 0.0;
+                for (int q = 0; q < 6; ++q) s += (side == 0 ? jb[r * 6 + q] : jb[q * 6 + r]) * x[6 * lo + q];
+                acc[r] += s;
+            }
+        }
+        for (int r = 0; r < 6; ++r) y[6 * li + r] = acc[r];
+    }
+}
+
+// 6x6 SPD inverse via Cholesky; returns false if not positive definite.
+__device__ bool inv6(const double* A, double* out) {
+    double L[36];
+    for (int k = 0; k < 36; ++k) L[k] = 0.0;
+    for (int j = 0; j < 6; ++j) {
+        double s = A[j * 6 + j];
+        for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
+        if (!(s > 0.0)) return false;
+        const double dj = sqrt(s);
+        L[j * 6 + j] = dj;
+        for (int i = j + 1; i < 6; ++i) {
+            double t = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) t -= L[i * 6 + k] * L[j * 6 + k];
+            L[i * 6 + j] = t / dj;
+        }
+    }
+    // inverse = L^-T L^-1 column by column
+    for (int c = 0; c < 6; ++c) {
+        double y[6];
+        for (int i = 0; i < 6; ++i) {
+            double t = (i == c) ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) t -= L[i * 6 + k] * y[k];
+            y[i] = t / L[i * 6 + i];
+        }
+        for (int i = 5; i >= 0; --i) {
+            double t = y[i];
+            for (int k = i + 1; k < 6; ++k) t -= L[k * 6 + i] * out[k * 6 + c];
+            out[i * 6 + c] = t / L[i * 6 + i];
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ void bj_apply(const Work& w, const ScnCtx& sc, const double* r, double* z) {
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const double* M = w.Dinv + 36 * (sc.b0 + li);
+        for (int a = 0; a < 6; ++a) {
+            double s = 0.0;
+            for (int c = 0; c < 6; ++c) s += M[a * 6 + c] * r[6 * li + c];
+            z[6 * li + a] = s;
+        }
+    }
+}
+
+__device__ double wg_dot(const double* a, const double* b, int n, double* red) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += a[i] * b[i];
+    return wg_reduce(s, red, OpAdd());
+}
+__device__ double wg_absmax(const double* a, int n, double* red) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s = dmax(s, fabs(a[i]));
+    return wg_reduce(s, red, OpMax());
+}
+
+// Block-Jacobi PCG for H dx = rhs. Returns false when not converged.
+__device__ bool pcg_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* rhs,
+                          double* xo, double* red, int* ok_sh) {
+    const int nd = sc.nd;
+    double* r = w.r + 6 * sc.b0;
+    double* z = w.z + 6 * sc.b0;
+    double* p = w.pv + 6 * sc.b0;
+    double* Ap = w.Ap + 6 * sc.b0;
+    if (threadIdx.x == 0) *ok_sh = 1;
+    __syncthreads();
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x)
+        if (!inv6(w.D + 36 * (sc.b0 + li), w.Dinv + 36 * (sc.b0 + li))) atomicAnd(ok_sh, 0);
+    __syncthreads();
+    if (!*ok_sh) return false;
+    bj_apply(w, sc, rhs, xo);
+    __syncthreads();
+    spmv(d, w, sc, b2arr, xo, Ap);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) r[i] = rhs[i] - Ap[i];
+    __syncthreads();
+    const double bn = wg_absmax(rhs, nd, red);
+    const double tol = 1e-14 * bn;
+    double rn = wg_absmax(r, nd, red);
+    if (rn <= tol) return true;
+    bj_apply(w, sc, r, z);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) p[i] = z[i];
+    __syncthreads();
+    double rz = wg_dot(r, z, nd, red);
+    for (int it = 0; it < 400; ++it) {
+        spmv(d, w, sc, b2arr, p, Ap);
+        __syncthreads();
+        const double pAp = wg_dot(p, Ap, nd, red);
+        if (!(pAp > 0.0)) return false;
+        const double alpha = rz / pAp;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+            xo[i] += alpha * p[i];
+            r[i] -= alpha * Ap[i];
+        }
+        __syncthreads();
+        rn = wg_absmax(r, nd, red);
+        if (rn <= tol) return true;
+        bj_apply(w, sc, r, z);
+        __syncthreads();
+        const double rz2 = wg_dot(r, z, nd, red);
+        const double beta = rz2 / rz;
+        rz = rz2;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) p[i] = z[i] + beta * p[i];
+        __syncthreads();
+    }
+    return rn <= 1e-10 * bn;
+}
+
+// Dense LDL^T fallback (scenes with hinge joints): assemble, factor, solve.
+__device__ bool dense_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* rhs,
+                            double* xo, double* red, int* ok_sh) {
+    const int nd = sc.nd;
+    double* M = w.dense + w.dense_off[sc.s];
+    for (long long i = threadIdx.x; i < (long long)nd * nd; i += blockDim.x) M[i] = 0.0;
+    __syncthreads();
+    // diagonal blocks
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const double* D = w.D + 36 * (sc.b0 + li);
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) M[(long long)(6 * li + r) * nd + 6 * li + c] = D[r * 6 + c];
+    }
+    __syncthreads();
+    // off-diagonal blocks (contacts, hinge joints): one thread adds all, in order
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < sc.nc; ++i) {
+            const long long cc = sc.cb + i;
+            const long long ce = (long long)sc.s * kMaxContacts + i;
+            const int act = w.cactive[ce];
+            if (!act) continue;
+            const int a = w.cint[5 * cc] - sc.b0, b = w.cint[5 * cc + 1] - sc.b0;
+            if (act & 1) {
+                const double* cd = w.cdbl + 7 * cc;
+                const double n[3] = {cd[3], cd[4], cd[5]};
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) {
+                        const double v = -(b2arr[i] * (n[r] * n[q])) * sc.h2;
+                        M[(long long)(6 * a + r) * nd + 6 * b + q] += v;
+                        M[(long long)(6 * b + r) * nd + 6 * a + q] += v;
+                    }
+            }
+            if (act & 2) {
+                const double* blk = w.cblk + 108 * ce + 72;
+                for (int r = 0; r < 6; ++r)
+                    for (int q = 0; q < 6; ++q) {
+                        M[(long long)(6 * a + r) * nd + 6 * b + q] += blk[r * 6 + q];
+                        M[(long long)(6 * b + q) * nd + 6 * a + r] += blk[r * 6 + q];
+                    }
+            }
+        }
+        for (int j = sc.j0; j < sc.j1; ++j) {
+            if (d.jtype[j] != 1) continue;
+            const int a = d.ja[j] - sc.b0, b = d.jb[j] - sc.b0;
+            const double* jb = w.jblk + 108 * j + 72;
+            for (int r = 0; r < 6; ++r)
+                for (int q = 0; q < 6; ++q) {
+                    M[(long long)(6 * a + r) * nd + 6 * b + q] += jb[r * 6 + q];
+                    M[(long long)(6 * b + q) * nd + 6 * a + r] += jb[r * 6 + q];
+                }
+        }
+    }
+    __syncthreads();
+    // right-looking LDL^T, lower triangle, D on the diagonal
+    if (threadIdx.x == 0) *ok_sh = 1;
+    __syncthreads();
+    for (int k = 0; k < nd; ++k) {
+        const double dk = M[(long long)k * nd + k];
+        if (!(dk != 0.0) || !isfinite(dk)) {
+            if (threadIdx.x == 0) *ok_sh = 0;
+            __syncthreads();
+            return false;
+        }
+        // column k below the diagonal: l_ik = a_ik / d_k; trailing update a_ij -= l_ik d_k l_jk
+        for (int i = k + 1 + threadIdx.x; i < nd; i += blockDim.x) M[(long long)i * nd + k] /= dk;
+        __syncthreads();
+        const int m = nd - k - 1;
+        for (long long t = threadIdx.x; t < (long long)m * m; t += blockDim.x) {
+            const int i = k + 1 + int(t / m), j = k + 1 + int(t % m);
+            if (j > i) continue;
+            M[(long long)i * nd + j] -= M[(long long)i * nd + k] * dk * M[(long long)j * nd + k];
+        }
+        __syncthreads();
+    }
+    // solve L D L^T x = rhs (single lane; nd is small on this path)
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < nd; ++i) {
+            double s = rhs[i];
+            for (int k = 0; k < i; ++k) s -= M[(long long)i * nd + k] * xo[k];
+            xo[i] = s;
+        }
+        for (int i = 0; i < nd; ++i) xo[i] /= M[(long long)i * nd + i];
+        for (int i = nd - 1; i >= 0; --i) {
+            double s = xo[i];
+            for (int k = i + 1; k < nd; ++k) s -= M[(long long)k * nd + i] * xo[k];
+            xo[i] = s;
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// ------------------------------------------------------------------ Newton kernel
+__global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
+    __shared__ double red[kBlk];
+    __shared__ int ok_sh;
+    const int s = blockIdx.x;
+    SceneParams& sp = d.sp[s];
+    if (sp.alm_done) return;
+    ScnCtx sc;
+    load_scn(d, s, sc);
+    double* X = w.x + 6 * sc.b0;
+    double* G = w.g + 6 * sc.b0;
+    double* DX = w.dx + 6 * sc.b0;
+    double* XT = w.xt + 6 * sc.b0;
+    double* CG = w.cg + 6 * sc.b0;
+    double* NG = w.tmp + 6 * sc.b0;
+    double* b2arr = w.sort_key + (long long)s * 65536;  // per-contact b'' scratch (<= 2000)
+    int conv = 0;
+    int it = 0;
+    const double* Xg = w.x;
+    const double* XTg = w.xt;
+    for (; it < 20; ++it) {
+        eval_elements(d, w, sc, Xg, b2arr);
+        const double ginf = objective_grad(d, w, sc, Xg, G, CG, red);
+        __syncthreads();
+        if (ginf < 1e-6) { conv = 1; break; }
+        assemble_diag(d, w, sc, b2arr);
+        for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) NG[i] = -G[i];
+        __syncthreads();
+        const bool ok = sp.use_dense ? dense_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh)
+                                     : pcg_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh);
+        __syncthreads();
+        if (!ok) {
+            if (threadIdx.x == 0) atomicOr(&sp.solve_fail, 1);
+            break;
+        }
+        // line search (LineSearch.cpp:6-58), alpha0 = min(1, maxStep) = 1
+        double fx, fc;
+        objective_value(d, w, sc, Xg, red, &fx, &fc);
+        const double descent = wg_dot(G, DX, sc.nd, red);
+        const double cdesc = wg_dot(CG, DX, sc.nd, red);
+        if (descent > 0) break;
+        double alpha = 1.0;
+        bool accepted = false;
+        for (int ls = 0; ls < 20; ++ls) {
+            for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) XT[i] = X[i] + alpha * DX[i];
+            __syncthreads();
+            double fn, fcn;
+            objective_value(d, w, sc, XTg, red, &fn, &fcn);
+            const bool cok = (cdesc <= 0.0) ? (fcn <= fc + 1e-4 * alpha * cdesc) : (fcn <= fc);
+            if (fn <= fx + 1e-4 * alpha * descent && cok) { accepted = true; break; }
+            alpha *= 0.5;
+        }
+        if (!accepted) break;
+        for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) X[i] = X[i] + alpha * DX[i];
+        __syncthreads();
+    }
+    __syncthreads();
+    // ALM bookkeeping (IPCSolver.cpp:1084-1110): violation, dual residual, hardening,
+    // lambda += mu C with the hardened mu.
+    double viol = 0.0, dual = 0.0;
+    for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
+        JEval e;
+        joint_eval(d, j, Xg, e);
+        double inf = 0.0;
+        for (int k = 0; k < 6; ++k) inf = dmax(inf, fabs(e.C[k]));
+        viol = dmax(viol, inf);
+        const double dc = d.jmu[j] * inf;
+        dual = isfinite(dc) ? dmax(dual, dmin(dc, 1e12)) : dmax(dual, 1e12);
+        if (inf > sp.alm_trig * sp.alm_ctol) {
+            const double nm = d.jmu[j] * sp.alm_ratio;
+            d.jmu[j] = isfinite(nm) ? dmin(nm, 1e12) : 1e12;
+        }
+        for (int k = 0; k < 6; ++k) d.jlambda[6 * j + k] += d.jmu[j] * e.C[k];
+    }
+    viol = wg_reduce(viol, red, OpMax());
+    dual = wg_reduce(dual, red, OpMax());
+    if (threadIdx.x == 0) {
+        sp.converged = conv;
+        sp.newton_iters += it + 1;
+        sp.viol = viol;
+        sp.dual = dual;
+    }
+}
+
+void launch_newton(const Dev& d, const Work& w, hipStream_t st) {
+    timing_begin("newton", st);
+    hipLaunchKernelGGL(k_newton, dim3(d.S), dim3(kBlk), 0, st, d, w);
+    timing_end("newton", st);
+}
+
+// ------------------------------------------------------------------ contacts setup
+// Body -> contact CSR in contact order (each contact once under A, once under B).
+__device__ void build_csr(const Work& w, const ScnCtx& sc) {
+    int* off = w.csr_off + sc.b0 + sc.s;
+    int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int b = sc.b0 + li;
+        int c = 0;
+        for (int i = 0; i < sc.nc; ++i) {
+            const int* ci = w.cint + 5 * (sc.cb + i);
+            c += (ci[0] == b) + (ci[1] == b);
+        }
+        off[li + 1] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        off[0] = 0;
+        for (int li = 0; li < sc.n; ++li) off[li + 1] += off[li];
+    }
+    __syncthreads();
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int b = sc.b0 + li;
+        int k = off[li];
+        for (int i = 0; i < sc.nc; ++i) {
+            const int* ci = w.cint + 5 * (sc.cb + i);
+            if (ci[0] == b) ent[k++] = (i << 1) | 0;
+            if (ci[1] == b) ent[k++] = (i << 1) | 1;
+        }
+    }
+    __syncthreads();
+}
+
+// Contact forms of the step (IPCSolver.cpp:872-921): barrier on the step-start
+// contacts, friction with mu_eff = mean of per-pair overrides and no normal forces yet.
+__global__ void __launch_bounds__(kBlk) k_contacts_setup(Dev d, Work w) {
+    const int s = blockIdx.x;
+    SceneParams& sp = d.sp[s];
+    const bool needs = sp.enable_adaptive || (sp.enable_friction && sp.mu > 0.0);
+    if (threadIdx.x == 0) {
+        if (!needs) sp.n_contacts = 0;
+        sp.needs_forms = needs ? 1 : 0;
+        sp.has_barrier = (sp.enable_adaptive && needs) ? 1 : 0;
+        sp.has_friction = (sp.enable_friction && sp.mu > 0.0) ? 1 : 0;
+        double emu = sp.mu;
+        const bool any_override = d.allowed_off[s + 1] > d.allowed_off[s];
+        if (sp.has_friction && sp.n_contacts > 0 && any_override) {
+            bool has = false;
+            for (int k = d.allowed_off[s]; k < d.allowed_off[s + 1]; ++k) has = has || d.allowed_mu[k] >= 0.0;
+            if (has) {
+                double sum = 0.0;
+                int cnt = 0;
+                const long long cb = (long long)s * kMaxRawContacts;
+                for (int i = 0; i < sp.n_contacts; ++i) {
+                    const int A = w.cint[5 * (cb + i)], B = w.cint[5 * (cb + i) + 1];
+                    const double m = allowed_mu(d, s, d.body_id[A], d.body_id[B]);
+                    sum += (m >= 0.0) ? m : sp.mu;
+                    ++cnt;
+                }
+                if (cnt > 0) emu = sum / double(cnt);
+            }
+        }
+        sp.f_mu = emu;
+        sp.nf_count = 0;
+        sp.alm_done = 0;
+        sp.alm_iters = 0;
+        sp.newton_iters = 0;
+    }
+    __syncthreads();
+    ScnCtx sc;
+    load_scn(d, s, sc);
+    build_csr(w, sc);
+}
+
+void launch_contacts_setup(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_contacts_setup, dim3(d.S), dim3(kBlk), 0, st, d, w);
+}
+
+// After the narrow phase at x_new (IPCSolver.cpp:1112-1165): lagged normal forces from
+// the new contacts, contact residual, ALM break test.
+__global__ void __launch_bounds__(kBlk) k_alm_post(Dev d, Work w) {
+    __shared__ double red[kBlk];
+    const int s = blockIdx.x;
+    SceneParams& sp = d.sp[s];
+    if (sp.alm_done) return;
+    ScnCtx sc;
+    load_scn(d, s, sc);
+    const double* X = w.x;  // global body indexing
+    double cres = 0.0;
+    if (sp.needs_forms) {
+        for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+            const double dd = contact_dist(w, sc.cb + i, X);
+            w.nf[(long long)s * kMaxContacts + i] =
+                (dd < sc.dhat && dd > 0.0) ? fabs(bar_g(dd, sc.dhat, sc.kappa)) : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && sp.has_friction) sp.nf_count = sc.nc;
+        __syncthreads();
+        load_scn(d, s, sc);
+        build_csr(w, sc);
+        // contact gradient without h^2 (barrier + friction)
+        double* b2arr = w.sort_key + (long long)s * 65536;
+        ScnCtx sb = sc;
+        sb.j1 = sb.j0;  // no joints here
+        sb.has_barrier = 1;  // barrier form exists whenever contact forms are needed
+        eval_elements(d, w, sb, X, b2arr);
+        const int* off = w.csr_off + sc.b0 + sc.s;
+        const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+        double cm = 0.0;
+        for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+            double gb[6] = {0, 0, 0, 0, 0, 0}, gf[6] = {0, 0, 0, 0, 0, 0};
+            for (int e = off[li]; e < off[li + 1]; ++e) {
+                const int c = ent[e] >> 1, side = ent[e] & 1;
+                const double* cg = w.cgrad + 18 * ((long long)sc.s * kMaxContacts + c);
+                for (int k = 0; k < 3; ++k) gb[k] += cg[side * 3 + k];
+                for (int k = 0; k < 6; ++k) gf[k] += cg[6 + side * 6 + k];
+            }
+            for (int k = 0; k < 6; ++k) {
+                const double v = (0.0 + gb[k]) + (sp.has_friction ? gf[k] : 0.0);
+                cm = isnan(v) ? 1.0 / 0.0 : dmax(cm, fabs(v));
+            }
+        }
+        cres = wg_reduce(cm, red, OpMax());
+        if (!isfinite(cres)) cres = 1e12;
+    }
+    if (threadIdx.x == 0) {
+        sp.cres = cres;
+        sp.alm_iters += 1;
+        const bool brk = sp.converged && sp.viol <= sp.alm_ctol && sp.dual <= sp.alm_dtol && cres <= sp.alm_ctcol;
+        if (brk || sp.alm_iters >= sp.alm_max_iters) sp.alm_done = 1;
+    }
+}
+
+void launch_alm_post(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_alm_post, dim3(d.S), dim3(kBlk), 0, st, d, w);
+}
+
+// ------------------------------------------------------------------ step prologue
+__global__ void k_step_begin(Dev d, Work w, const double* dt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.S) return;
+    SceneParams& sp = d.sp[s];
+    sp.dt = dt[s];
+    sp.evals = 0.0;
+    sp.kappa_flag = 0;
+    sp.toi = 1.0;
+    sp.solve_fail = 0;
+    sp.alm_done = 0;
+    sp.vv_needed = 0;
+    w.flag[s] = 0;
+}
+
+void launch_step_begin(const Dev& d, const Work& w, const double* dt, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_begin, dim3((d.S + 255) / 256), dim3(256), 0, st, d, w, dt);
+}
+
+// IPCSolver::initializeAdaptiveBarrier + AdaptiveBarrier::initialize (IPCSolver.cpp:
+// 327-353, AdaptiveBarrier.h:27-31) once per scene, then the max-speed CCD inflation
+// (IPCSolver.cpp:360-365).
+__global__ void __launch_bounds__(kBlk) k_barrier_init(Dev d, Work w) {
+    __shared__ double red[kBlk];
+    const int s = blockIdx.x;
+    SceneParams& sp = d.sp[s];
+    const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
+    if (sp.enable_adaptive && !sp.barrier_initialized && b1 > b0) {
+        if (threadIdx.x == 0) {
+            double tm = 0.0;
+            const double inf = 1.0 / 0.0;
+            V3 mn = v3(inf, inf, inf), mx = v3(-inf, -inf, -inf);
+            for (int b = b0; b < b1; ++b) {
+                tm += d.mass[b];
+                mn = vmin(mn, ld3(d.p, b));
+                mx = vmax(mx, ld3(d.p, b));
+            }
+            const double avg = tm / double(b1 - b0);
+            double diag = norm(mx - mn);
+            if (diag < 1e-6) diag = 1.0;
+            sp.kappa_min = 1e11 * avg / (diag * diag);
+            sp.kappa = sp.kappa_min;
+            sp.dhat = 1e-3;
+            sp.barrier_initialized = 1;
+        }
+    }
+    __syncthreads();
+    double vm = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) vm = dmax(vm, norm(ld3(d.v, b)));
+    vm = wg_reduce(vm, red, OpMax());
+    if (threadIdx.x == 0) {
+        const double dh = sp.dhat;
+        sp.ccd_infl = dmax(1e-6, dh) + vm * sp.dt;
+    }
+}
+
+// AdaptiveBarrier::updateStiffness (AdaptiveBarrier.h:33-37) when the broad-phase gap is
+// below 2 dhat and some vertex-face plane distance is below 1e-9.
+__global__ void k_kappa_update(Dev d, Work w) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.S) return;
+    SceneParams& sp = d.sp[s];
+    if (!sp.enable_adaptive) return;
+    if (!(sp.min_pair_dist < sp.dhat * 2.0)) return;
+    if (w.flag[s] && sp.kappa < 1e20) {
+        sp.kappa = dmin(sp.kappa * 2.0, 1e20);
+        sp.kappa_flag = 1;
+    }
+}
+
+void launch_kappa(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_kappa_update, dim3((d.S + 255) / 256), dim3(256), 0, st, d, w);
+}
+
+// IPCSolver::computeMaxStep result (IPCSolver.cpp:377-391).
+__global__ void k_ccd_dt(Dev d, Work w) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.S) return;
+    SceneParams& sp = d.sp[s];
+    double nfv = 0.0, nee = 0.0;
+    for (int p = w.pair_off[s]; p < w.pair_off[s + 1]; ++p) {
+        const int* k = w.lcnt + 6 * p;
+        nfv += double(k[4] * k[1] + k[5] * k[0]);
+        nee += double(k[2] * k[3]);
+    }
+    sp.n_fv = nfv;
+    sp.n_ee = nee;
+    double adt = sp.dt;
+    if (sp.enable_ccd && d.body_off[s + 1] > d.body_off[s] && sp.ccd_any && nfv > 0.0 && sp.toi < 1.0)
+        adt = sp.toi * sp.ccd_safety * sp.dt;
+    sp.adt = adt;
+}
+
+void launch_ccd_dt(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_ccd_dt, dim3((d.S + 255) / 256), dim3(256), 0, st, d, w);
+}
+
+// Pack x, v, a and the implicit-Euler prediction x_hat = x + h v
+// (IPCSolver.cpp:839-861, ImplicitEulerIntegrator.cpp:5-7).
+__global__ void k_pack(Dev d, Work w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.N) return;
+    const double h = d.sp[d.body_scene[b]].adt;
+    const double xv[6] = {d.p[3 * b], d.p[3 * b + 1], d.p[3 * b + 2], 0.0, 0.0, 0.0};
+    const double vv[6] = {d.v[3 * b], d.v[3 * b + 1], d.v[3 * b + 2], d.w[3 * b], d.w[3 * b + 1], d.w[3 * b + 2]};
+    for (int k = 0; k < 6; ++k) {
+        w.xprev[6 * b + k] = xv[k];
+        w.vprev[6 * b + k] = vv[k];
+        const double xh = xv[k] + h * vv[k];
+        w.xhat[6 * b + k] = xh;
+        w.x[6 * b + k] = xh;
+    }
+}
+
+void launch_pack(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack, dim3((d.N + 255) / 256), dim3(256), 0, st, d, w);
+}
+
+// ImplicitEuler::update_quantities + pose write-back (IPCSolver.cpp:1168-1195).
+__global__ void k_finalize(Dev d, Work w) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.N) return;
+    const double h = d.sp[d.body_scene[b]].adt;
+    double vn[6], an[6];
+    for (int k = 0; k < 6; ++k) {
+        vn[k] = (w.x[6 * b + k] - w.xprev[6 * b + k]) / h;
+        an[k] = (vn[k] - w.vprev[6 * b + k]) / h;
+    }
+    st3(d.al, b, v3(an[0], an[1], an[2]));
+    st3(d.aa, b, v3(an[3], an[4], an[5]));
+    st3(d.v, b, v3(vn[0], vn[1], vn[2]));
+    st3(d.w, b, v3(vn[3], vn[4], vn[5]));
+    st3(d.p, b, v3(w.x[6 * b], w.x[6 * b + 1], w.x[6 * b + 2]));
+    const V3 th = v3(w.x[6 * b + 3], w.x[6 * b + 4], w.x[6 * b + 5]);
+    st4(d.q, b, apply_theta(ld4(d.q, b), th));
+}
+
+void launch_finalize(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize, dim3((d.N + 255) / 256), dim3(256), 0, st, d, w);
+}
+
+void launch_barrier_init_ccd(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_barrier_init, dim3(d.S), dim3(kBlk), 0, st, d, w);
+}
+
+// ------------------------------------------------------------------ form evaluation API
+// Dense evaluation of DistanceBarrierForm / FrictionForm for given contacts (one scene),
+// using the same per-contact element code as the solver.
+__global__ void __launch_bounds__(kBlk) k_forms_eval(Dev d, Work w, int s, int which, double dhat, double kappa,
+                                                     double mu, double eps, double dt, const double* x, double* value,
+                                                     double* grad, double* hess, double* nfo) {
+    __shared__ double red[kBlk];
+    ScnCtx sc;
+    load_scn(d, s, sc);
+    sc.h2 = 1.0;
+    sc.dhat = dhat > 1e-9 ? dhat : 1e-9;
+    sc.kappa = kappa > 0.0 ? kappa : 0.0;
+    sc.fmu = mu;
+    sc.feps = eps;
+    sc.fdt = dt > 0.0 ? dt : 0.0;
+    sc.has_barrier = which == 0;
+    sc.has_friction = which == 1;
+    sc.nfc = sc.nc;
+    sc.j1 = sc.j0;
+    const double* X = x - 6 * (long long)sc.b0;  // contacts carry global body indices
+    double* b2arr = w.sort_key + (long long)s * 65536;
+    build_csr(w, sc);
+    eval_elements(d, w, sc, X, b2arr);
+    double v = 0.0;
+    for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+        const long long c = sc.cb + i;
+        if (which == 0) {
+            const double dd = contact_dist(w, c, X);
+            v += bar_v(dd, sc.dhat, sc.kappa);
+            if (nfo) nfo[i] = (dd < sc.dhat && dd > 0.0) ? fabs(bar_g(dd, sc.dhat, sc.kappa)) : 0.0;
+        } else {
+            const double N = w.nf[(long long)s * kMaxContacts + i];
+            if (N > 0.0) {
+                FKin k;
+                fric_kin(d, w, c, X, sc.fdt, k);
+                v += fric_v(k.u, N, sc.fmu, sc.feps);
+            }
+        }
+    }
+    v = wg_reduce(v, red, OpAdd());
+    if (threadIdx.x == 0) *value = v;
+    const int* off = w.csr_off + sc.b0 + sc.s;
+    const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    const int nd = sc.nd;
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        double g[6] = {0, 0, 0, 0, 0, 0};
+        for (int e = off[li]; e < off[li + 1]; ++e) {
+            const int c = ent[e] >> 1, side = ent[e] & 1;
+            const double* cg = w.cgrad + 18 * ((long long)sc.s * kMaxContacts + c);
+            if (which == 0) for (int k = 0; k < 3; ++k) g[k] += cg[side * 3 + k];
+            else for (int k = 0; k < 6; ++k) g[k] += cg[6 + side * 6 + k];
+        }
+        for (int k = 0; k < 6; ++k) grad[6 * li + k] = g[k];
+    }
+    if (hess) {
+        __syncthreads();
+        // sum in contact order per entry (one lane: dense test path)
+        if (threadIdx.x == 0) {
+            for (long long i = 0; i < (long long)nd * nd; ++i) hess[i] = 0.0;
+            for (int i = 0; i < sc.nc; ++i) {
+                const long long cc = sc.cb + i;
+            const long long ce = (long long)sc.s * kMaxContacts + i;
+                const int act = w.cactive[ce];
+                const int a = w.cint[5 * cc] - sc.b0, b = w.cint[5 * cc + 1] - sc.b0;
+                if (which == 0 && (act & 1)) {
+                    const double* cd = w.cdbl + 7 * cc;
+                    const double n[3] = {cd[3], cd[4], cd[5]};
+                    for (int r = 0; r < 3; ++r)
+                        for (int q = 0; q < 3; ++q) {
+                            const double blk = b2arr[i] * (n[r] * n[q]);
+                            hess[(long long)(6 * a + r) * nd + 6 * a + q] += blk;
+                            hess[(long long)(6 * b + r) * nd + 6 * b + q] += blk;
+                            hess[(long long)(6 * a + r) * nd + 6 * b + q] += -blk;
+                            hess[(long long)(6 * b + r) * nd + 6 * a + q] += -blk;
+                        }
+                }
+                if (which == 1 && (act & 2)) {
+                    const double* blk = w.cblk + 108 * ce;
+                    for (int r = 0; r < 6; ++r)
+                        for (int q = 0; q < 6; ++q) {
+                            hess[(long long)(6 * a + r) * nd + 6 * a + q] += blk[r * 6 + q];
+                            hess[(long long)(6 * b + r) * nd + 6 * b + q] += blk[36 + r * 6 + q];
+                            hess[(long long)(6 * a + r) * nd + 6 * b + q] += blk[72 + r * 6 + q];
+                            hess[(long long)(6 * b + q) * nd + 6 * a + r] += blk[72 + r * 6 + q];
+                        }
+                }
+            }
+        }
+    }
+}
+
+void launch_forms_eval(const Dev& d, const Work& w, int s, int which, double dhat, double kappa, double mu,
+                       double eps, double dt, const double* x, double* value, double* grad, double* hess, double* nf,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_forms_eval, dim3(1), dim3(kBlk), 0, st, d, w, s, which, dhat, kappa, mu, eps, dt, x, value,
+                       grad, hess, nf);
+}
+
+// ------------------------------------------------------------------ standalone PCG
+// ndipc_pcg_solve: block-Jacobi PCG over caller-provided 6x6 blocks (one work-group).
+__global__ void __launch_bounds__(kBlk) k_pcg(int nb, const double* diag, int noff, const int* offij,
+                                              const double* offb, const double* b, double tol, int max_it, double* x,
+                                              int* iters, double* rel, double* scr) {
+    __shared__ double red[kBlk];
+    const int nd = 6 * nb;
+    double* r = scr;
+    double* z = scr + nd;
+    double* p = scr + 2 * nd;
+    double* Ap = scr + 3 * nd;
+    double* Dinv = scr + 4 * nd;
+    auto mv = [&](const double* v, double* y) {
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            for (int a = 0; a < 6; ++a) {
+                double s = 0.0;
+                for (int c = 0; c < 6; ++c) s += diag[36 * i + a * 6 + c] * v[6 * i + c];
+                for (int o = 0; o < noff; ++o) {
+                    const int bi = offij[2 * o], bj = offij[2 * o + 1];
+                    const double* B = offb + 36 * o;
+                    if (bi == i) for (int c = 0; c < 6; ++c) s += B[a * 6 + c] * v[6 * bj + c];
+                    if (bj == i) for (int c = 0; c < 6; ++c) s += B[c * 6 + a] * v[6 * bi + c];
+                }
+                y[6 * i + a] = s;
+            }
+        }
+        __syncthreads();
+    };
+    auto prec = [&](const double* v, double* y) {
+        for (int i = threadIdx.x; i < nb; i += blockDim.x)
+            for (int a = 0; a < 6; ++a) {
+                double s = 0.0;
+                for (int c = 0; c < 6; ++c) s += Dinv[36 * i + a * 6 + c] * v[6 * i + c];
+                y[6 * i + a] = s;
+            }
+        __syncthreads();
+    };
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) inv6(diag + 36 * i, Dinv + 36 * i);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) { x[i] = 0.0; r[i] = b[i]; }
+    __syncthreads();
+    prec(r, z);
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) p[i] = z[i];
+    __syncthreads();
+    double rz = wg_dot(r, z, nd, red);
+    const double bn = sqrt(wg_dot(b, b, nd, red));
+    double rn = bn;
+    int it = 0;
+    for (; it < max_it && rn > tol * bn; ++it) {
+        mv(p, Ap);
+        const double pAp = wg_dot(p, Ap, nd, red);
+        const double alpha = rz / pAp;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; }
+        __syncthreads();
+        rn = sqrt(wg_dot(r, r, nd, red));
+        prec(r, z);
+        const double rz2 = wg_dot(r, z, nd, red);
+        const double beta = rz2 / rz;
+        rz = rz2;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) p[i] = z[i] + beta * p[i];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *iters = it;
+        *rel = bn > 0.0 ? rn / bn : 0.0;
+    }
+}
+
+void launch_pcg(int nb, const double* diag, int noff, const int* offij, const double* offb, const double* b,
+                double tol, int max_it, double* x, int* iters, double* rel, double* scratch, hipStream_t st) {
+    hipLaunchKernelGGL(k_pcg, dim3(1), dim3(kBlk), 0, st, nb, diag, noff, offij, offb, b, tol, max_it, x, iters, rel,
+                       scratch);
+}
+
+}  // namespace nd

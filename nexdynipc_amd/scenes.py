@@ -14,11 +14,13 @@ Configurations (sizes, seeds and draw order are part of the benchmark definition
   c1  8 unit cubes in a 2x2x2 grid (1.1 m spacing, +-5 deg random tilt) 0.5 m above
       a static 10x1x10 m ground box; dt 1e-2, dhat 1e-3, mu 0.3.
   c2  512 icospheres (subdivision 3, 642 verts / 1280 faces), r ~ U[0.1, 0.3],
-      non-overlapping in a 4 x 8 (y) x 4 m region above an open-top container made of
-      5 static boxes; v ~ N(0, 0.5) per axis; dt 5e-3, dhat 1e-3, mu 0.4.
+      non-overlapping in a 4 x 8 (y) x 4 m region (y in [2.5, 10.5]) above an open-top
+      container of 5 static boxes (4.4 x 0.2 x 4.4 floor, 2 m walls, 1 cm gaps between
+      parts); v ~ N(0, 0.5) per axis; dt 5e-3, dhat 1e-3, mu 0.4.
   c3  256 scenes, n ~ round(exp(U[ln 4, ln 128])) bodies each, shapes uniform over
       {unit cube, icosphere subdivision 2 (r 0.5), 2x1x0.5 box}, dropped from
-      U[0.2, 2] m onto a static ground box; dt 1e-2, mu ~ U[0.1, 0.6] per scene.
+      U[0.2, 2] m onto a static ground box (0.7 m thick, so no body's vertices sit on
+      its faces when the centres align); dt 1e-2, mu ~ U[0.1, 0.6] per scene.
   c4  16-link hinge chain (0.4x0.1x0.1 links, link 0 pinned) + 48 unit cubes above a
       static 20 deg ramp box; dt 1e-2, mu 0.3.
   c5  128 scenes of 32 unit cubes (grid jitter +-0.05 m, v ~ N(0, 1)) thrown at a
@@ -298,17 +300,18 @@ def _box_body(key, sx, sy, sz, p, q=_IDENT, static=False, v=None, w=None):
                     np.zeros(3) if w is None else np.asarray(w, dtype=np.float64), static)
 
 
-_ICO_CACHE = {}
+_ICO_UNIT = {}
 
 
 def _ico_body(subdiv, radius, p, q=_IDENT, v=None):
-    key = (subdiv, float(radius))
-    if key not in _ICO_CACHE:
-        V, F = icosphere_mesh(subdiv, radius)
+    """Icosphere of the given radius: the unit mesh scaled by r (mass ~ r^3, inertia ~ r^5)."""
+    if subdiv not in _ICO_UNIT:
+        V, F = icosphere_mesh(subdiv, 1.0)
         m, I = polyhedron_mass_properties(V, F)
-        _ICO_CACHE[key] = (V, F, m, I)
-    V, F, m, I = _ICO_CACHE[key]
-    return BodySpec(f"ico{subdiv}_{radius!r}", V, F, m, I, np.asarray(p, dtype=np.float64),
+        _ICO_UNIT[subdiv] = (V, F, m, I)
+    V1, F, m1, I1 = _ICO_UNIT[subdiv]
+    r = float(radius)
+    return BodySpec(f"ico{subdiv}_{r!r}", V1 * r, F, m1 * r ** 3, I1 * r ** 5, np.asarray(p, dtype=np.float64),
                     np.asarray(q, dtype=np.float64), np.zeros(3) if v is None else np.asarray(v, dtype=np.float64))
 
 
@@ -324,15 +327,22 @@ def config1(scene=0) -> SceneSpec:
     return SceneSpec(bodies, dt=1e-2, dhat=1e-3, mu=0.3, name="c1")
 
 
-def container_boxes(half=2.0, wall_h=3.0, t=0.2):
-    """Open-top container as 5 static boxes: floor plus four walls around [-half, half]^2."""
+def container_boxes(half=2.0, wall_h=2.0, t=0.2, gap=0.01):
+    """Open-top container as 5 static boxes: a floor and four walls around [-half, half]^2.
+
+    The parts are separated by `gap` (> 2 dhat) so no two static parts touch or overlap:
+    touching or interpenetrating static boxes produce contacts whose normal can be
+    orthogonal to the centre offset, which the reference's centre-distance barrier
+    clamps to d = 1e-9 (DistanceBarrierForm.cpp:89-98)."""
     L = 2 * half + 2 * t
     return [
         _box_body(f"floor{L}", L, t, L, [0.0, -0.5 * t, 0.0], static=True),
-        _box_body(f"wallx{wall_h}", t, wall_h, L, [half + 0.5 * t, 0.5 * wall_h, 0.0], static=True),
-        _box_body(f"wallx{wall_h}", t, wall_h, L, [-half - 0.5 * t, 0.5 * wall_h, 0.0], static=True),
-        _box_body(f"wallz{wall_h}", L, wall_h, t, [0.0, 0.5 * wall_h, half + 0.5 * t], static=True),
-        _box_body(f"wallz{wall_h}", L, wall_h, t, [0.0, 0.5 * wall_h, -half - 0.5 * t], static=True),
+        _box_body(f"wallx{wall_h}", t, wall_h, 2 * half - 2 * gap, [half + 0.5 * t, gap + 0.5 * wall_h, 0.0],
+                  static=True),
+        _box_body(f"wallx{wall_h}", t, wall_h, 2 * half - 2 * gap, [-half - 0.5 * t, gap + 0.5 * wall_h, 0.0],
+                  static=True),
+        _box_body(f"wallz{wall_h}", L, wall_h, t, [0.0, gap + 0.5 * wall_h, half + 0.5 * t + gap], static=True),
+        _box_body(f"wallz{wall_h}", L, wall_h, t, [0.0, gap + 0.5 * wall_h, -half - 0.5 * t - gap], static=True),
     ]
 
 
@@ -349,7 +359,7 @@ def config2(scene=0, n_spheres=512, subdiv=3) -> SceneSpec:
             raise RuntimeError("could not place spheres")
         r = rng.uniform(0.1, 0.3)
         c = np.array([rng.uniform(-2 + r + margin, 2 - r - margin),
-                      rng.uniform(r + margin, 8 - r),
+                      rng.uniform(2.5 + r, 10.5 - r),
                       rng.uniform(-2 + r + margin, 2 - r - margin)])
         if centres:
             C = np.asarray(centres)
@@ -375,7 +385,7 @@ def config3(n_scenes=256, bodies_override: Optional[int] = None) -> List[SceneSp
         side = int(math.ceil(math.sqrt(n)))
         spacing = 2.5
         extent = side * spacing + 2.0
-        bodies = [_box_body(f"ground{extent}", extent, 1.0, extent, [0.0, -0.5, 0.0], static=True)]
+        bodies = [_box_body(f"ground{extent}", extent, 0.7, extent, [0.0, -0.35, 0.0], static=True)]
         for k in range(n):
             shape = int(rng.integers(0, 3))
             h = rng.uniform(0.2, 2.0)

@@ -21,6 +21,7 @@
 #include "ndo_math.h"
 
 #include <array>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -500,6 +501,11 @@ struct Solver {
     // integrator (ImplicitEuler)
     vector<double> x_prev, v_prev, a_prev, x_hat;
     double idt = 0;
+
+    // CPU-baseline sampling of computeSceneMinDistance: only every k-th vertex of the
+    // outer loop is evaluated and the measured time is scaled by k (bench only).
+    int sample_stride = 1;
+    double t_scene_min = 0, t_step = 0;
 
     // statistics of the last step
     double st_alm = 0, st_newton = 0, st_contacts = 0, st_fv = 0, st_ee = 0, st_evals = 0, st_kappa = 0, st_toi = 1;
@@ -1120,10 +1126,12 @@ struct Solver {
     // computeMinDistance / computeSceneMinDistance (AdaptiveBarrier.h:89-162)
     double scene_min_distance() const {
         double md = std::numeric_limits<double>::infinity();
+        long long vcount = 0;
         for (int i = 0; i < n(); ++i) {
             if (bodies[i].mesh < 0) continue;
             const Mesh& MA = meshes[bodies[i].mesh];
             for (size_t vi = 0; vi < MA.V.size(); ++vi) {
+                if ((vcount++ % sample_stride) != 0) continue;
                 const V3 p = bodies[i].p + rotate(bodies[i].q, MA.V[vi]);
                 for (int j = i + 1; j < n(); ++j) {
                     if (bodies[j].mesh < 0) continue;
@@ -1189,7 +1197,10 @@ struct Solver {
             double bmd = min_body_pair_distance(dh * 2.0);
             if (!std::isfinite(bmd)) bmd = std::numeric_limits<double>::infinity();
             if (bmd < dh * 2.0) {
+                const auto t0 = std::chrono::steady_clock::now();
                 const double md = scene_min_distance();
+                t_scene_min += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() *
+                               double(sample_stride);
                 if (md < std::numeric_limits<double>::infinity()) {
                     if (md < 1e-9 && kappa < 1e20) { kappa = std::min(kappa * 2.0, 1e20); st_kappa = 1; }
                 }
@@ -1665,8 +1676,21 @@ int32_t ndo_get_joint_state(void* h, double* lambda6, double* mu) {
     return 0;
 }
 
+int32_t ndo_set_sampling(void* h, int32_t stride) {
+    S(h).sample_stride = std::max(1, int(stride));
+    return 0;
+}
+
+int32_t ndo_timing(void* h, double* out2) {
+    out2[0] = S(h).t_step;
+    out2[1] = S(h).t_scene_min;
+    return 0;
+}
+
 int32_t ndo_step(void* h, double dt) {
+    const auto t0 = std::chrono::steady_clock::now();
     S(h).step(dt);
+    S(h).t_step += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     S(h).assembler_build();
     return 0;
 }

@@ -67,6 +67,11 @@ int32_t ndo_get_barrier(void* h, double* kappa, double* dhat, int32_t* initializ
 int32_t ndo_set_barrier(void* h, double kappa, double dhat, int32_t initialized);
 int32_t ndo_get_joint_state(void* h, double* lambda6, double* mu);
 int32_t ndo_step(void* h, double dt);
+/* CPU-baseline sampling: evaluate every `stride`-th vertex of computeSceneMinDistance's
+ * outer loop and scale its measured time by `stride` (results are then not exact). */
+int32_t ndo_set_sampling(void* h, int32_t stride);
+/* Accumulated wall time: [all ndo_step calls, scaled computeSceneMinDistance]. */
+int32_t ndo_timing(void* h, double* out2);
 /* step statistics of the last step: [alm_iters, newton_iters_total, n_contacts_last,
  * candidates_fv, candidates_ee, narrow_evals, kappa_updated, toi] */
 int32_t ndo_last_stats(void* h, double* out8);

@@ -97,6 +97,10 @@ def lib():
         L.ndo_round_ops.argtypes = [f64p, f64p, C.c_int32, f64p]
         L.ndo_objective.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
         L.ndo_scene_min_distance.argtypes = [C.c_void_p, f64p]
+        L.ndo_set_sampling.restype = C.c_int32
+        L.ndo_set_sampling.argtypes = [C.c_void_p, C.c_int32]
+        L.ndo_timing.restype = C.c_int32
+        L.ndo_timing.argtypes = [C.c_void_p, f64p]
         _lib = L
     return _lib
 
@@ -193,6 +197,14 @@ class OracleScene:
 
     def step(self, dt):
         self.L.ndo_step(self.h, dt)
+
+    def set_sampling(self, stride):
+        self.L.ndo_set_sampling(self.h, int(stride))
+
+    def timing(self):
+        o = np.zeros(2)
+        self.L.ndo_timing(self.h, ptr(o, f64p))
+        return o
 
     def stats(self):
         o = np.zeros(8)

@@ -1,0 +1,276 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same inputs.
+
+Bars (from the task definition of this path):
+  - candidate pair sets bit-exact after sorting;
+  - barrier energy / gradient / Hessian within 1e-10 relative;
+  - time of impact never above the oracle's and within 1e-6 of it;
+  - per-frame body poses within 1e-8 m and 1e-8 rad.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+from nexdynipc_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def rows_sorted(a):
+    a = np.asarray(a)
+    if len(a) == 0:
+        return a
+    return a[np.lexsort(a.T[::-1])]
+
+
+def contact_sort(ints, dbl):
+    key = np.concatenate([ints.astype(np.float64), dbl], axis=1)
+    order = np.lexsort(key.T[::-1])
+    return ints[order], dbl[order]
+
+
+def quat_angle(q1, q2):
+    d = np.abs(np.sum(q1 * q2, axis=1)).clip(0, 1)
+    return 2 * np.arccos(d)
+
+
+def perturbed_c2_scene(n=40, seed=0):
+    """A small icosphere scene whose spheres interpenetrate: many zero scores (ties)."""
+    sp = scenes.config2(n_spheres=n, subdiv=3)
+    rng = np.random.default_rng(seed)
+    sc = scenes.pack([sp])
+    p = sc.p.copy()
+    # pull spheres together so AABBs overlap and vertices fall inside other boxes
+    p[5:] = p[5:] * 0.25 + rng.normal(0, 0.02, size=p[5:].shape)
+    p[5:, 1] = np.abs(p[5:, 1]) * 0.3
+    sc.p[:] = p
+    return sc
+
+
+def test_round_ops_bit_exact(engine_factory):
+    sc = scenes.make_config("c1")
+    e = engine_factory(sc)
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=4000) * np.exp(rng.uniform(-20, 20, size=4000))
+    b = rng.normal(size=4000) * np.exp(rng.uniform(-20, 20, size=4000))
+    a[:50] = -b[:50]  # exact cancellations
+    ref = O.round_ops(a, b)
+    got = e.round_ops(a, b)
+    assert np.array_equal(ref.view(np.int64), got.view(np.int64))
+
+
+def test_select_top_k_ties(engine_factory):
+    sc = scenes.make_config("c1")
+    e = engine_factory(sc)
+    rng = np.random.default_rng(2)
+    for trial in range(300):
+        n = int(rng.integers(1, 2049))
+        k = int(rng.integers(1, 40))
+        s = rng.integers(0, 4, size=n).astype(np.float64) * 0.25
+        s[rng.random(n) < 0.5] = 0.0
+        ref = O.select_top_k(s, min(k, n))
+        got = e.select_top_k(s, k)
+        assert sorted(ref.tolist()) == sorted(got.tolist()), (trial, n, k)
+
+
+@pytest.mark.parametrize("name", ["c1", "tie"])
+def test_body_pairs_and_candidates(engine_factory, name):
+    sc = scenes.make_config("c1") if name == "c1" else perturbed_c2_scene()
+    if name == "c1":
+        sc.p[1:, 1] -= 1.0  # cubes penetrate the ground
+    e = engine_factory(sc)
+    o = O.OracleScene(sc)
+    for r in (0.0, 1e-3, 2e-3, 0.05):
+        np.testing.assert_array_equal(rows_sorted(o.body_pairs(r)), e.body_pairs(0, r))
+        assert o.min_body_pair_distance(r) == e.min_body_pair_distance(0, r)
+    ro = o.candidates(1e-3)
+    rg = e.candidates(0, 1e-3)
+    assert sum(len(x) for x in ro) > 0
+    for a, b in zip(ro, rg):
+        np.testing.assert_array_equal(rows_sorted(a), rows_sorted(b))
+
+
+@pytest.mark.parametrize("name", ["c1", "tie", "micro"])
+def test_narrow_phase(engine_factory, name):
+    if name == "c1":
+        sc = scenes.make_config("c1")
+        sc.p[1:, 1] -= 1.0
+    elif name == "tie":
+        sc = perturbed_c2_scene()
+    else:
+        sc = scenes.pack([scenes.micro_contact_scene(0)])
+    e = engine_factory(sc)
+    o = O.OracleScene(sc)
+    o.candidates(float(sc.dhat[0]))
+    e.candidates(0, float(sc.dhat[0]))
+    for x in (None, "perturb"):
+        xs = None
+        if x is not None:
+            rng = np.random.default_rng(3)
+            xs = np.zeros(6 * o.n)
+            xs[0::6] = sc.p[:, 0]
+            xs[1::6] = sc.p[:, 1]
+            xs[2::6] = sc.p[:, 2]
+            xs += rng.normal(0, 1e-4, size=xs.shape)
+        oi, od = o.narrow_phase(xs)
+        gi, gd = e.narrow_phase(0, xs)
+        assert len(oi) == len(gi)
+        oi, od = contact_sort(oi, od)
+        gi, gd = contact_sort(gi, gd)
+        np.testing.assert_array_equal(oi, gi)
+        np.testing.assert_allclose(gd, od, rtol=0, atol=1e-14)
+
+
+def test_barrier_friction_forms(engine_factory):
+    sc = scenes.pack([scenes.micro_contact_scene(1)])
+    e = engine_factory(sc)
+    o = O.OracleScene(sc)
+    o.candidates(1e-3)
+    ints, dbl = o.narrow_phase(None)
+    assert len(ints) > 0
+    n = o.n
+    rng = np.random.default_rng(4)
+    x = np.zeros(6 * n)
+    for k in range(3):
+        x[k::6] = sc.p[:, k]
+    x[3::6] = rng.normal(0, 1e-4, size=n)
+    kappa = 1e9
+    vo, go, Ho, nfo = o.barrier_eval(ints, dbl, x, 1e-3, kappa)
+    vg, gg, Hg, nfg = e.barrier_eval(0, ints, dbl, x, 1e-3, kappa)
+    assert vo > 0
+    assert abs(vg - vo) <= 1e-10 * abs(vo)
+    assert np.max(np.abs(gg - go)) <= 1e-10 * np.max(np.abs(go))
+    assert np.max(np.abs(Hg - Ho)) <= 1e-10 * np.max(np.abs(Ho))
+    np.testing.assert_allclose(nfg, nfo, rtol=1e-10, atol=0)
+    nf = np.where(nfo > 0, nfo, 1.0)
+    x2 = x.copy()
+    x2[0::6] += rng.normal(0, 1e-5, size=n)
+    fo = o.friction_eval(ints, dbl, nf, x2, 0.5, 1e-8, 1e-3)
+    fg = e.friction_eval(0, ints, dbl, nf, x2, 0.5, 1e-8, 1e-3)
+    assert abs(fg[0] - fo[0]) <= 1e-10 * abs(fo[0])
+    assert np.max(np.abs(fg[1] - fo[1])) <= 1e-10 * np.max(np.abs(fo[1]))
+    assert np.max(np.abs(fg[2] - fo[2])) <= 1e-10 * np.max(np.abs(fo[2]))
+
+
+def test_scene_min_distance_flag(engine_factory):
+    sc = scenes.make_config("c1")
+    # cube 1 resting exactly on the ground top face: vertex-plane distance 0
+    sc.q[1] = [1.0, 0.0, 0.0, 0.0]
+    sc.p[1] = [-0.55, 0.5, -0.55]
+    e = engine_factory(sc)
+    o = O.OracleScene(sc)
+    ref = o.scene_min_distance() < 1e-9
+    assert e.scene_min_distance_flag(0) == int(ref)
+    assert ref
+    sc2 = scenes.make_config("c1")
+    e2 = engine_factory(sc2)
+    o2 = O.OracleScene(sc2)
+    assert e2.scene_min_distance_flag(0) == int(o2.scene_min_distance() < 1e-9)
+
+
+def _pose(p0, p1, q0=(1, 0, 0, 0), q1=(1, 0, 0, 0)):
+    return np.array(list(p0) + list(q0) + list(p1) + list(q1), dtype=np.float64)
+
+
+def test_ccd_edge_vertex_kat_and_random(engine_factory):
+    sc = scenes.make_config("c1")
+    e = engine_factory(sc)
+    # tests/physics/test_ccd.cpp:258-292
+    pa = _pose((0, 0, 0), (0, 0, 0))
+    pb = _pose((2, 0, 0), (0, 0, 0))
+    a = np.array([[0, 0, 0]], dtype=np.float64)
+    b = np.array([[0, -1, 0, 0, 1, 0]], dtype=np.float64)
+    hit, toi, pops = e.ccd_toi_batch(0, pa[None], pb[None], a, b, 1.0, tol=1e-6)
+    r, toi_o, pops_o = O.ccd_toi(0, pa, pb, a[0], b[0], 1.0, 1e-6)
+    assert hit[0] == 1 and r == 1 and 0 <= toi[0] <= 1
+    assert toi[0] == toi_o and pops[0] == pops_o
+    rng = np.random.default_rng(5)
+    for kind in (0, 1, 2):
+        n = 64
+        PA, PB, AP, BP = [], [], [], []
+        for i in range(n):
+            ax = rng.normal(size=3)
+            ang = rng.uniform(0, 0.3) if i % 2 else 0.0
+            q1 = scenes.quat_from_axis_angle(ax, ang)
+            PA.append(_pose(rng.normal(0, 0.1, 3), rng.normal(0, 0.1, 3), (1, 0, 0, 0), q1))
+            PB.append(_pose(rng.normal(0, 0.1, 3) + [0.3, 0, 0], rng.normal(0, 0.1, 3)))
+            AP.append(rng.normal(0, 0.2, 6))
+            BP.append(rng.normal(0, 0.2, 9))
+        PA, PB, AP, BP = map(np.array, (PA, PB, AP, BP))
+        hit, toi, pops = e.ccd_toi_batch(kind, PA, PB, AP[:, : (6 if kind == 1 else 3)],
+                                         BP[:, : (9 if kind == 2 else 6)], 1.0, tol=1e-8)
+        for i in range(n):
+            r, t, p = O.ccd_toi(kind, PA[i], PB[i], AP[i], BP[i], 1.0, 1e-8)
+            assert hit[i] == r, (kind, i)
+            if r == 1:
+                assert toi[i] <= t + 1e-15 and abs(toi[i] - t) <= 1e-6, (kind, i, toi[i], t)
+
+
+def _run_pair(sc, steps, params=None, every=1):
+    e_params = None
+    if params is not None:
+        from nexdynipc_amd._lib import make_params
+        e_params = make_params(**params)
+    return e_params
+
+
+@pytest.mark.parametrize("name,steps", [("c1", 200), ("micro", 60)])
+def test_step_trajectory(engine_factory, name, steps):
+    if name == "c1":
+        sc = scenes.make_config("c1")
+    else:
+        sc = scenes.pack([scenes.micro_contact_scene(2, n_cubes=4)])
+    e = engine_factory(sc)
+    o = O.OracleScene(sc)
+    dt = float(sc.dt[0])
+    worst_p = worst_q = 0.0
+    for k in range(steps):
+        o.step(dt)
+        e.step(1)
+        po, qo = o.get_state()[:2]
+        pg, qg = e.get_state()[:2]
+        worst_p = max(worst_p, float(np.max(np.abs(pg - po))))
+        worst_q = max(worst_q, float(np.max(quat_angle(qg, qo))))
+        assert worst_p <= 1e-8 and worst_q <= 1e-8, (k, worst_p, worst_q)
+    ko, _, _ = o.barrier()
+    kg, _, _ = e.barrier()
+    assert abs(kg[0] - ko) <= 1e-12 * abs(ko)
+
+
+def test_batched_scenes_match_per_scene_oracle(engine_factory):
+    specs = scenes.config3(n_scenes=6)
+    sc = scenes.pack(specs)
+    e = engine_factory(sc)
+    steps = 10
+    e.step(steps)
+    pg, qg = e.get_state()[:2]
+    for s in range(sc.n_scenes):
+        o = O.OracleScene(sc, s)
+        for _ in range(steps):
+            o.step(float(sc.dt[s]))
+        po, qo = o.get_state()[:2]
+        sl = sc.scene_slice(s)
+        assert np.max(np.abs(pg[sl] - po)) <= 1e-8
+        assert np.max(quat_angle(qg[sl], qo)) <= 1e-8
+
+
+def test_pcg_matches_dense_solve(engine_factory):
+    sc = scenes.make_config("c1")
+    e = engine_factory(sc)
+    rng = np.random.default_rng(6)
+    nb = 20
+    diag = []
+    for i in range(nb):
+        A = rng.normal(size=(6, 6))
+        diag.append(A @ A.T + 6 * np.eye(6))
+    offij = [(i, i + 1) for i in range(nb - 1)]
+    offb = [0.1 * rng.normal(size=(6, 6)) for _ in offij]
+    H = np.zeros((6 * nb, 6 * nb))
+    for i, D in enumerate(diag):
+        H[6 * i:6 * i + 6, 6 * i:6 * i + 6] = D
+    for (i, j), B in zip(offij, offb):
+        H[6 * i:6 * i + 6, 6 * j:6 * j + 6] = B
+        H[6 * j:6 * j + 6, 6 * i:6 * i + 6] = B.T
+    b = rng.normal(size=6 * nb)
+    x, it, rel = e.pcg_solve(np.array(diag), np.array(offij), np.array(offb), b, tol=1e-14, max_it=500)
+    xr = np.linalg.solve(H, b)
+    assert np.max(np.abs(x - xr)) <= 1e-11 * np.max(np.abs(xr))
