@@ -29,8 +29,12 @@ def contact_sort(ints, dbl):
 
 
 def quat_angle(q1, q2):
-    d = np.abs(np.sum(q1 * q2, axis=1)).clip(0, 1)
-    return 2 * np.arccos(d)
+    """Rotation angle of q1^-1 q2 (well conditioned for tiny angles)."""
+    w1, v1 = q1[:, 0], q1[:, 1:]
+    w2, v2 = q2[:, 0], q2[:, 1:]
+    w = w1 * w2 + np.sum(v1 * v2, axis=1)
+    v = w1[:, None] * v2 - w2[:, None] * v1 - np.cross(v1, v2)
+    return 2 * np.arctan2(np.linalg.norm(v, axis=1), np.abs(w))
 
 
 def perturbed_c2_scene(n=40, seed=0):
@@ -153,9 +157,12 @@ def test_barrier_friction_forms(engine_factory):
 
 def test_scene_min_distance_flag(engine_factory):
     sc = scenes.make_config("c1")
-    # cube 1 resting exactly on the ground top face: vertex-plane distance 0
+    # cube 1 resting exactly on cube 2's top face: vertices of body 1 lie on a face of
+    # body 2 > 1 (computeSceneMinDistance tests vertices of i against faces of j > i)
     sc.q[1] = [1.0, 0.0, 0.0, 0.0]
-    sc.p[1] = [-0.55, 0.5, -0.55]
+    sc.q[2] = [1.0, 0.0, 0.0, 0.0]
+    sc.p[1] = [3.0, 2.0, 3.0]
+    sc.p[2] = [3.0, 1.0, 3.0]
     e = engine_factory(sc)
     o = O.OracleScene(sc)
     ref = o.scene_min_distance() < 1e-9
@@ -203,14 +210,6 @@ def test_ccd_edge_vertex_kat_and_random(engine_factory):
             assert hit[i] == r, (kind, i)
             if r == 1:
                 assert toi[i] <= t + 1e-15 and abs(toi[i] - t) <= 1e-6, (kind, i, toi[i], t)
-
-
-def _run_pair(sc, steps, params=None, every=1):
-    e_params = None
-    if params is not None:
-        from nexdynipc_amd._lib import make_params
-        e_params = make_params(**params)
-    return e_params
 
 
 @pytest.mark.parametrize("name,steps", [("c1", 200), ("micro", 60)])
