@@ -139,6 +139,9 @@ def main():
         "config": {"workload": workload, "bodies": int(sc.n_bodies), "scenes": int(sc.n_scenes),
                    "verts": int(sc.total_verts()), "dt": float(sc.dt[0]), "dhat": float(sc.dhat[0])},
         "contact_pairs_per_s": pairs_evaluated / T,
+        "last_step": {"alm_iters": float(np.mean(stats[:, 0])), "newton_iters": float(np.mean(stats[:, 1])),
+                      "contacts": float(np.sum(stats[:, 2])), "fv_candidates": n_fv, "ee_candidates": n_ee,
+                      "close_pairs": int(n_pairs)},
         "kernel_avg_ms": {k: v[0] for k, v in kern.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -293,7 +293,6 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     ctx->J = J;
     std::vector<int> ja(std::max(J, 1), -1), jb(std::max(J, 1), -1), jt(std::max(J, 1), 0);
     std::vector<int> joint_off(S + 1, 0);
-    std::vector<int> use_dense(S, 0);
     for (int s = 0; s <= S; ++s) joint_off[s] = dsc->scene_joint_off ? dsc->scene_joint_off[s] : 0;
     for (int s = 0; s < S; ++s) {
         for (int j = joint_off[s]; j < joint_off[s + 1]; ++j) {
@@ -305,7 +304,6 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
             if (jt[j] == 1) {
                 if (b < 0 || b >= n || b == a) return fail(ctx, "hinge joint needs two distinct bodies of its scene");
                 jb[j] = ctx->body_off[s] + b;
-                use_dense[s] = 1;
             } else if (jt[j] != 0) {
                 return fail(ctx, "unsupported joint type");
             }
@@ -388,8 +386,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
         default_params(ctx->sp[s]);
         double me = 0.0;
         for (int b = ctx->body_off[s]; b < ctx->body_off[s + 1]; ++b) me = std::max(me, max_edge[b]);
-        ctx->sp[s].prune_margin = 1e-8 + 4e-6 * me;
-        ctx->sp[s].use_dense = use_dense[s];
+        ctx->sp[s].prune_margin = 1e-8 + 1e-5 * me;
     }
     // work buffers
     Work& w = ctx->w;
@@ -443,26 +440,17 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     w.D = dalloc<double>(ctx, 36 * size_t(N));
     w.Dinv = dalloc<double>(ctx, 36 * size_t(N));
     w.nf = dalloc<double>(ctx, size_t(S) * nd::kMaxContacts);
-    w.cblk = dalloc<double>(ctx, size_t(S) * nd::kMaxContacts * 108);
+    w.cblk = dalloc<double>(ctx, size_t(S) * nd::kMaxContacts * 144);
     w.cgrad = dalloc<double>(ctx, size_t(S) * nd::kMaxContacts * 18);
     w.cactive = dalloc<int>(ctx, size_t(S) * nd::kMaxContacts);
     w.csr_off = dalloc<int>(ctx, size_t(N) + S + 1);
     w.csr_ent = dalloc<int>(ctx, size_t(S) * 2 * nd::kMaxContacts);
     w.jblk = dalloc<double>(ctx, 108 * size_t(std::max(J, 1)));
     w.jgrad = dalloc<double>(ctx, 12 * size_t(std::max(J, 1)));
-    std::vector<long long> doff(S, 0);
-    long long dtot = 0;
-    for (int s = 0; s < S; ++s) {
-        doff[s] = dtot;
-        if (use_dense[s]) {
-            const long long nd6 = 6LL * (ctx->body_off[s + 1] - ctx->body_off[s]);
-            if (nd6 > 3072) return fail(ctx, "hinge-jointed scene too large for the dense solve (> 512 bodies)");
-            dtot += nd6 * nd6;
-        }
-    }
-    w.dense = dalloc<double>(ctx, size_t(std::max<long long>(dtot, 1)));
-    w.dense_off = dupload(ctx, doff.data(), S);
+    w.dense = dalloc<double>(ctx, size_t(S) * nd::kDenseRows * nd::kDenseRows);
     w.flag = dalloc<int>(ctx, S);
+    w.bcnt = dalloc<int>(ctx, 2 * size_t(N));
+    w.boff = dalloc<int>(ctx, 2 * size_t(N));
     ctx->d_dt = dalloc<double>(ctx, S);
     ctx->scratch_bytes = 0;
     ctx->d_scratch = nullptr;
@@ -577,17 +565,15 @@ int32_t ndipc_step(ndipc_ctx* ctx, const double* dt, int32_t n_steps) {
         nd::launch_step_begin(d, w, ctx->d_dt, st);
         nd::launch_world_boxes(d, w, st);
         nd::launch_lbvh_build(d, w, st);
-        nd::launch_broad_phase(d, w, 0, -1.0, true, nullptr, st);
+        nd::launch_barrier_init_ccd(d, w, st);
+        nd::launch_broad_phase_step(d, w, st);
         nd::timing_begin("topk", st);
         nd::launch_topk(d, w, mp, st);
         nd::timing_end("topk", st);
-        nd::launch_barrier_init_ccd(d, w, st);
-        nd::launch_broad_phase(d, w, 1, -1.0, true, nullptr, st);
         nd::timing_begin("scene_min_flag", st);
         nd::launch_scene_min_flag(d, w, mp, st);
         nd::timing_end("scene_min_flag", st);
         nd::launch_kappa(d, w, st);
-        nd::launch_broad_phase(d, w, 2, -1.0, true, nullptr, st);
         nd::launch_ccd_earliest(d, w, mp, st);
         nd::launch_ccd_dt(d, w, st);
         nd::launch_pack(d, w, st);
@@ -788,7 +774,7 @@ int32_t ndipc_scene_min_distance_flag(ndipc_ctx* ctx, int32_t scene, int32_t* ou
     ctx->sp[scene].min_pair_dist = -1.0;
     if (sync_sp_to_dev(ctx)) return -1;
     HIPCHK(hipMemsetAsync(ctx->w.flag, 0, sizeof(int) * ctx->S, ctx->stream));
-    nd::launch_scene_min_flag(ctx->d, ctx->w, max_grid_pairs(ctx), ctx->stream);
+    nd::launch_scene_min_flag(ctx->d, ctx->w, max_grid_pairs(ctx), ctx->stream, 1);
     int f = 0;
     HIPCHK(hipMemcpyAsync(&f, ctx->w.flag + scene, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -191,6 +191,120 @@ __device__ void lbvh_query(const Dev& d, const Work& w, int s, int i, double r, 
     }
 }
 
+// Fused broad phase of IPCSolver::step: the three detectBodyPairs queries of a step
+// (candidates at the pre-initialisation dhat, ContactAssembler.cpp:220; stiffness
+// check at 2 dhat, IPCSolver.cpp:825; CCD gate at max(1e-6, dhat) + v_max dt,
+// IPCSolver.cpp:365) share one traversal per body with the largest inflation; each hit
+// is then classified with the exact per-query expression. One thread per body over all
+// scenes; pass 0 counts, pass 1 emits at scanned offsets (pairs stay sorted by (i, j)).
+__device__ __forceinline__ void bp_sort_insert(int* outp, int o, int k, int i, int j) {
+    int pos = k;
+    while (pos > 0 && outp[2 * (o + pos - 1) + 1] > j) {
+        outp[2 * (o + pos)] = outp[2 * (o + pos - 1)];
+        outp[2 * (o + pos) + 1] = outp[2 * (o + pos - 1) + 1];
+        --pos;
+    }
+    outp[2 * (o + pos)] = i;
+    outp[2 * (o + pos) + 1] = j;
+}
+
+__global__ void k_bp_step(Dev d, Work w, int pass) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N) return;
+    const int s = d.body_scene[i];
+    SceneParams& sp = d.sp[s];
+    int nc = 0, np = 0;
+    if (d.nv[i] > 0) {
+        const double r0 = dmax(0.0, sp.dhat_cand);
+        const double r1 = dmax(0.0, sp.dhat * 2.0);
+        const double r2 = dmax(0.0, sp.ccd_infl);
+        const bool do1 = sp.enable_adaptive != 0, do2 = sp.enable_ccd != 0;
+        double rmax = r0;
+        if (do1) rmax = dmax(rmax, r1);
+        if (do2) rmax = dmax(rmax, r2);
+        const Box3 bi = ldbox(w.wbox, i);
+        const Box3 q0 = box_inflated(bi, r0), q1 = box_inflated(bi, r1), q2 = box_inflated(bi, r2);
+        double gmin = 1.0 / 0.0;
+        int any = 0;
+        const int oc = pass ? w.boff[i] : 0, op = pass ? w.boff[d.N + i] : 0;
+        const double margin = sp.prune_margin;
+        lbvh_query(d, w, s, i, rmax, [&](int j) {
+            const Box3 bj = ldbox(w.wbox, j);
+            const double gap = box_distance(bi, bj);
+            if (box_overlap(q0, box_inflated(bj, r0)) && gap <= r0) {
+                if (pass) {
+                    bp_sort_insert(w.pairs, w.pair_off[s] + oc, nc, i, j);
+                    w.pair_scene[w.pair_off[s] + oc + nc] = s;
+                }
+                ++nc;
+            }
+            if (do1 && box_overlap(q1, box_inflated(bj, r1))) {
+                gmin = dmin(gmin, gap);
+                if (gap <= margin) {
+                    if (pass) {
+                        bp_sort_insert(w.prox_pairs, w.prox_off[s] + op, np, i, j);
+                        w.prox_scene[w.prox_off[s] + op + np] = s;
+                    }
+                    ++np;
+                }
+            }
+            if (do2 && !any && box_overlap(q2, box_inflated(bj, r2)) && pair_allowed(d, s, d.body_id[i], d.body_id[j]))
+                any = 1;
+        });
+        if (!pass) {
+            if (do1 && gmin < 1.0 / 0.0)
+                atomicMin(reinterpret_cast<unsigned long long*>(&sp.min_pair_dist), __double_as_longlong(gmin));
+            if (any) atomicOr(&sp.ccd_any, 1);
+        }
+    }
+    if (!pass) {
+        w.bcnt[i] = nc;
+        w.bcnt[d.N + i] = np;
+    }
+}
+
+// Exclusive scans of the per-body counts (candidates, prox) -> per-body offsets within
+// their scene, per-scene offsets and totals. One work-group.
+__global__ void __launch_bounds__(kBlock) k_bp_scan(Dev d, Work w) {
+    __shared__ int sh[kBlock];
+    for (int list = 0; list < 2; ++list) {
+        const int* cnt = w.bcnt + list * d.N;
+        int* boff = w.boff + list * d.N;
+        int* soff = list == 0 ? w.pair_off : w.prox_off;
+        int* scnt = list == 0 ? w.pair_cnt : w.prox_cnt;
+        int* tot = list == 0 ? w.n_pairs_total : w.n_prox_total;
+        int run = 0;
+        for (int s = 0; s < d.S; ++s) {
+            const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
+            if (threadIdx.x == 0) soff[s] = run;
+            int local = 0;
+            for (int base = b0; base < b1; base += kBlock) {
+                const int b = base + threadIdx.x;
+                const int v = b < b1 ? cnt[b] : 0;
+                int ex;
+                const int t = wg_exclusive_scan(v, sh, &ex);
+                if (b < b1) boff[b] = local + ex;
+                local += t;
+            }
+            if (threadIdx.x == 0) scnt[s] = local;
+            run += local;
+        }
+        if (threadIdx.x == 0) {
+            soff[d.S] = run;
+            *tot = run;
+        }
+        __syncthreads();
+    }
+}
+
+void launch_broad_phase_step(const Dev& d, const Work& w, hipStream_t st) {
+    if (d.N == 0) return;
+    const dim3 grid((d.N + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_bp_step, grid, dim3(kBlock), 0, st, d, w, 0);
+    hipLaunchKernelGGL(k_bp_scan, dim3(1), dim3(kBlock), 0, st, d, w);
+    hipLaunchKernelGGL(k_bp_step, grid, dim3(kBlock), 0, st, d, w, 1);
+}
+
 // mode 0: candidate pairs (gap <= r); mode 3: all overlapping pairs (detectBodyPairs).
 // mode 1: min gap over pairs at 2*dhat + prox pairs (gap <= prune_margin).
 // mode 2: CCD gate — any allowed pair at inflation max(1e-6, dhat) + vmax dt.
@@ -351,19 +465,25 @@ __device__ void wg_select_topk(const double* s_orig, double* sk, int* si, int n,
         return;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) si[i] = i;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        sk[i] = s_orig[i];
+        si[i] = i;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        TopkCmp c{s_orig};
-        tk_nth_element(si, k, n, c);
+        KVStore st{sk, si};
+        tk_nth_element(st, k, n);
         // order the kept set by (score, index)
         for (int a = 1; a < k; ++a) {
+            const double vk = sk[a];
             const int v = si[a];
             int b = a - 1;
-            while (b >= 0 && key_less(s_orig[v], v, s_orig[si[b]], si[b])) {
+            while (b >= 0 && key_less(vk, v, sk[b], si[b])) {
+                sk[b + 1] = sk[b];
                 si[b + 1] = si[b];
                 --b;
             }
+            sk[b + 1] = vk;
             si[b + 1] = v;
         }
         for (int i = 0; i < k; ++i) out[i] = si[i];
@@ -865,18 +985,26 @@ void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_a
 // against every face of body j > i, plane distance with the barycentric test through
 // the 3x3 cofactor inverse. A pair can only pass if its AABB gap is at most
 // 1e-9 + 2e-6 * (longest face edge); pairs beyond the scene's prune margin are skipped.
-__global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w) {
+// Within a pair, a face of J can only pass for a vertex of I if the face's box,
+// inflated by 1e-8 + 1e-5 * (its longest edge), meets I's world AABB, and a vertex of I
+// only if it lies in J's box inflated by the same bound for J's longest edge (the
+// projection lies within the triangle enlarged by the 1e-6 barycentric tolerance).
+// Both filters are conservative, so the flag is exact.
+__global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int ungated) {
     __shared__ double wvx[kMaxPrims], wvy[kMaxPrims], wvz[kMaxPrims];
+    __shared__ int act[kMaxPrims];
+    __shared__ int nact;
     const int total = *w.n_prox_total;
     for (int p = blockIdx.x; p < total; p += gridDim.x) {
         const int s = w.prox_scene[p];
         const SceneParams& sp = d.sp[s];
-        if (!(sp.min_pair_dist < sp.dhat * 2.0)) continue;
+        if (!ungated && !(sp.min_pair_dist < sp.dhat * 2.0)) continue;
         if (w.flag[s]) continue;
         const int I = w.prox_pairs[2 * p], J = w.prox_pairs[2 * p + 1];
         const V3 pI = ld3(d.p, I), pJ = ld3(d.p, J);
         const Q qI = ld4(d.q, I), qJ = ld4(d.q, J);
         const int nvJ = d.nv[J];
+        if (threadIdx.x == 0) nact = 0;
         for (int k = threadIdx.x; k < nvJ; k += blockDim.x) {
             const V3 v = world_vertex(d, J, k, pJ, qJ);
             wvx[k] = v.x; wvy[k] = v.y; wvz[k] = v.z;
@@ -884,10 +1012,27 @@ __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w) {
         __syncthreads();
         const int nfJ = d.nf[J];
         const int* F = d.faces + 3 * d.f_off[J];
-        for (int vi = threadIdx.x; vi < d.nv[I]; vi += blockDim.x) {
+        const Box3 bI = box_inflated(ldbox(w.wbox, I), 1e-9);
+        for (int f = threadIdx.x; f < nfJ; f += blockDim.x) {
+            const int a = F[3 * f], b = F[3 * f + 1], c = F[3 * f + 2];
+            const V3 A = v3(wvx[a], wvy[a], wvz[a]), B = v3(wvx[b], wvy[b], wvz[b]), C = v3(wvx[c], wvy[c], wvz[c]);
+            const double L = dmax(norm(B - A), dmax(norm(C - B), norm(A - C)));
+            Box3 fb;
+            fb.mn = vmin(A, vmin(B, C));
+            fb.mx = vmax(A, vmax(B, C));
+            if (box_overlap(box_inflated(fb, 1e-8 + 1e-5 * L), bI)) act[atomicAdd(&nact, 1)] = f;
+        }
+        __syncthreads();
+        const int na = nact;
+        const Box3 bJ = box_inflated(ldbox(w.wbox, J), 1e-8 + 1e-5 * d.max_edge[J] + 1e-9);
+        for (int vi = threadIdx.x; vi < d.nv[I] && na > 0; vi += blockDim.x) {
             const V3 pt = world_vertex(d, I, vi, pI, qI);
+            if (!(pt.x >= bJ.mn.x && pt.x <= bJ.mx.x && pt.y >= bJ.mn.y && pt.y <= bJ.mx.y && pt.z >= bJ.mn.z &&
+                  pt.z <= bJ.mx.z))
+                continue;
             bool hit = false;
-            for (int f = 0; f < nfJ && !hit; ++f) {
+            for (int fi = 0; fi < na && !hit; ++fi) {
+                const int f = act[fi];
                 const int a = F[3 * f], b = F[3 * f + 1], c = F[3 * f + 2];
                 const V3 v0 = v3(wvx[a], wvy[a], wvz[a]);
                 const V3 e1 = v3(wvx[b], wvy[b], wvz[b]) - v0;
@@ -918,10 +1063,10 @@ __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w) {
     }
 }
 
-void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_t st) {
+void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_t st, int ungated) {
     int grid = max_prox < 2048 ? max_prox : 2048;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_scene_min_flag, dim3(grid), dim3(kBlock), 0, st, d, w);
+    hipLaunchKernelGGL(k_scene_min_flag, dim3(grid), dim3(kBlock), 0, st, d, w, ungated);
 }
 
 // ----------------------------------------------------------------------------- CCD

@@ -23,16 +23,18 @@ constexpr int kMaxContacts = 2000;    // IPCSolver.cpp:414
 constexpr int kMaxPrims = 2048;       // per-mesh vertex/edge/face limit of the LDS top-K
 constexpr int kMaxSceneBodies = 4096; // per-scene LBVH limit (LDS bitonic sort)
 constexpr int kHitWords = 48;         // 3072 candidate bits per pair (2x1024 FV + 1024 EE)
+constexpr int kDenseBodies = 64;      // largest coupled component factored directly
+constexpr int kDenseRows = 6 * kDenseBodies;
 
 struct SceneParams {
     // settings
     double dhat, kappa, mu, ccd_safety;
     double alm_ctol, alm_dtol, alm_ctcol, alm_trig, alm_ratio;
     int enable_friction, enable_ccd, enable_adaptive, alm_max_iters;
-    int barrier_initialized, use_dense, vv_needed, pad1;
+    int barrier_initialized, pad0, vv_needed, pad1;
     double kappa_min;
     // per-step control
-    double dt, adt, f_mu, min_pair_dist, toi, prune_margin, ccd_infl;
+    double dt, adt, f_mu, min_pair_dist, toi, prune_margin, ccd_infl, dhat_cand;
     int needs_forms, has_barrier, has_friction, alm_done;
     int alm_iters, newton_iters, converged, n_contacts;
     int nf_count, kappa_flag, ccd_any, solve_fail;
@@ -128,9 +130,10 @@ struct Work {
     int* csr_ent;            // [S*2000*2]
     double* jblk;            // [J*3*36] hinge blocks AA, BB, AB
     double* jgrad;           // [J*12]
-    double* dense;           // [dense_cap] dense fallback matrices
-    long long* dense_off;    // [S]
+    double* dense;           // [S][kDenseRows^2] component LDL^T factors
     int* flag;               // [S] scene-min flag scratch
+    int* bcnt;               // [2N] per-body pair counts (candidates, prox)
+    int* boff;               // [2N] per-body offsets within the scene
 };
 
 // Launch wrappers (defined in the .hip files).
@@ -142,10 +145,11 @@ void launch_lbvh_build(const Dev& d, const Work& w, hipStream_t st);
 // mode 2: any allowed pair at inflation sp-dependent (ccd) into sp.ccd_any
 void launch_broad_phase(const Dev& d, const Work& w, int mode, double r, bool use_scene_r, const int* scene_mask,
                         hipStream_t st);
+void launch_broad_phase_step(const Dev& d, const Work& w, hipStream_t st);
 void launch_topk(const Dev& d, const Work& w, int max_pairs, hipStream_t st);
 void launch_pose_from_state(const Dev& d, const Work& w, const double* x, hipStream_t st);
 void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st);
-void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_t st);
+void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_t st, int ungated = 0);
 void launch_ccd_earliest(const Dev& d, const Work& w, int max_pairs, hipStream_t st);
 void launch_ccd_batch(int kind, int n, const double* pa, const double* pb, const double* a, const double* b,
                       const double* earliest, double tol, int* hit, double* toi, int* pops, hipStream_t st);

@@ -2,9 +2,9 @@
 // NewtonSolver::minimize (NewtonSolver.cpp:10-95) with the backtracking line search
 // (LineSearch.cpp:6-58) on the SimulationProblem objective (IPCSolver.cpp:137-262):
 // inertia (body-frame, unrotated), h^2 gravity, ALM joints, h^2 barrier and h^2 lagged
-// friction. The linear solve replaces SimplicialLDLT with a block-Jacobi PCG over 6x6
-// body blocks (dense Cholesky for scenes with hinge joints). Scenes are independent,
-// so a batch of scenes is one launch of each kernel.
+// friction. The linear solve replaces SimplicialLDLT with an LDL^T per connected
+// component of the coupling graph (block-Jacobi PCG for components above 64 bodies).
+// Scenes are independent, so a batch of scenes is one launch of each kernel.
 #include "ndipc_internal.h"
 
 namespace nd {
@@ -205,9 +205,9 @@ __device__ __forceinline__ void load_scn(const Dev& d, int s, ScnCtx& c) {
 }
 
 // Per-contact element data at X: barrier b'' (unscaled), gradients, friction blocks.
-// cgrad[c][18] = gbA(3) gbB(3) gfA(6) gfB(6);  cblk[c][3*36] = h2 * friction AA BB AB;
-// cactive[c] bit0 barrier active, bit1 friction active; the barrier b'' is kept in
-// cgrad slot via a separate array (tmp) — see elem_b2.
+// cgrad[c][18] = gbA(3) gbB(3) gfA(6) gfB(6);  cblk[c][4*36] = h2 * friction AA BB AB BA;
+// cactive[c] bit0 barrier active, bit1 friction active; the unscaled barrier b'' of
+// contact c is written to b2arr[c].
 __device__ void eval_elements(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* b2arr) {
     for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
         const long long c = sc.cb + i;
@@ -238,11 +238,11 @@ __device__ void eval_elements(const Dev& d, const Work& w, const ScnCtx& sc, con
             }
             double H[9];
             fric_h(k.u, N, sc.fmu, sc.feps, H);
-            double* blk = w.cblk + 108 * ((long long)sc.s * kMaxContacts + i);
-            // (X^T H) Y for (A,A), (B,B), (A,B)
-            for (int which = 0; which < 3; ++which) {
-                double (*Xm)[6] = (which == 1) ? k.JB : k.JA;
-                double (*Ym)[6] = (which == 0) ? k.JA : k.JB;
+            double* blk = w.cblk + 144 * ((long long)sc.s * kMaxContacts + i);
+            // (X^T H) Y for (A,A), (B,B), (A,B), (B,A)   (FrictionForm.cpp:137-140)
+            for (int which = 0; which < 4; ++which) {
+                double (*Xm)[6] = (which == 1 || which == 3) ? k.JB : k.JA;
+                double (*Ym)[6] = (which == 0 || which == 3) ? k.JA : k.JB;
                 double T[6][3];
                 for (int r = 0; r < 6; ++r)
                     for (int cc = 0; cc < 3; ++cc)
@@ -422,7 +422,7 @@ __device__ void assemble_diag(const Dev& d, const Work& w, const ScnCtx& sc, con
             const long long cc = sc.cb + c;
             const long long ce = (long long)sc.s * kMaxContacts + c;
             if (!(w.cactive[ce] & 2)) continue;
-            const double* blk = w.cblk + 108 * ce + side * 36;
+            const double* blk = w.cblk + 144 * ce + side * 36;
             for (int k = 0; k < 36; ++k) D[k] = D[k] + blk[k];
         }
     }
@@ -458,11 +458,10 @@ __device__ void spmv(const Dev& d, const Work& w, const ScnCtx& sc, const double
                 }
             }
             if (act & 2) {
-                const double* blk = w.cblk + 108 * ce + 72;  // AB
+                const double* blk = w.cblk + 144 * ce + (side == 0 ? 72 : 108);  // AB or BA
                 for (int r = 0; r < 6; ++r) {
                     double s = 0.0;
-                    for (int q = 0; q < 6; ++q)
-                        s += (side == 0 ? blk[r * 6 + q] : blk[q * 6 + r]) * x[6 * other + q];
+                    for (int q = 0; q < 6; ++q) s += blk[r * 6 + q] * x[6 * other + q];
                     acc[r] += s;
                 }
             }
@@ -593,102 +592,219 @@ __device__ bool pcg_solve(const Dev& d, const Work& w, const ScnCtx& sc, const d
     return rn <= 1e-10 * bn;
 }
 
-// Dense LDL^T fallback (scenes with hinge joints): assemble, factor, solve.
-__device__ bool dense_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* rhs,
-                            double* xo, double* red, int* ok_sh) {
-    const int nd = sc.nd;
-    double* M = w.dense + w.dense_off[sc.s];
-    for (long long i = threadIdx.x; i < (long long)nd * nd; i += blockDim.x) M[i] = 0.0;
-    __syncthreads();
-    // diagonal blocks
-    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
-        const double* D = w.D + 36 * (sc.b0 + li);
-        for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 6; ++c) M[(long long)(6 * li + r) * nd + 6 * li + c] = D[r * 6 + c];
+// ------------------------------------------------------------------ direct solve
+// The Newton matrix is block diagonal up to the bodies coupled by active contacts and
+// hinge joints. Bodies are grouped into connected components (union-find over active
+// elements); every component is factored by LDL^T in natural (global body) order with
+// the operation order of a row-oriented skyline LDL^T:
+//   L_ij = (A_ij - sum_k (L_ik D_k) L_jk) / D_j,   D_i = A_ii - sum_k (L_ik L_ik) D_k,
+// k ascending. Entries of other components are exact zeros in that factorization, so a
+// component-wise factorization equals the factorization of the whole matrix. Singletons
+// use the same recurrences on their 6x6 block (one lane each); larger components are
+// factored column by column with the rows of a column in parallel (left-looking, same
+// per-entry order). Components above kDenseBodies bodies fall back to PCG.
+struct CompLds {
+    int par[kMaxSceneBodies + 1];  // union-find parent, then component offsets
+    int cid[kMaxSceneBodies];      // component of each body
+    int list[kMaxSceneBodies];     // bodies grouped by component, ascending
+    int ncomp, maxsize;
+};
+
+__device__ __forceinline__ int uf_find(int* par, int x) {
+    while (par[x] != x) {
+        par[x] = par[par[x]];
+        x = par[x];
     }
-    __syncthreads();
-    // off-diagonal blocks (contacts, hinge joints): one thread adds all, in order
+    return x;
+}
+
+__device__ void build_components(const Dev& d, const Work& w, const ScnCtx& sc, CompLds& L) {
     if (threadIdx.x == 0) {
+        const int n = sc.n;
+        for (int i = 0; i < n; ++i) L.par[i] = i;
+        auto unite = [&](int a, int b) {
+            a = uf_find(L.par, a);
+            b = uf_find(L.par, b);
+            if (a != b) { if (a < b) L.par[b] = a; else L.par[a] = b; }
+        };
         for (int i = 0; i < sc.nc; ++i) {
-            const long long cc = sc.cb + i;
-            const long long ce = (long long)sc.s * kMaxContacts + i;
-            const int act = w.cactive[ce];
-            if (!act) continue;
-            const int a = w.cint[5 * cc] - sc.b0, b = w.cint[5 * cc + 1] - sc.b0;
-            if (act & 1) {
-                const double* cd = w.cdbl + 7 * cc;
-                const double n[3] = {cd[3], cd[4], cd[5]};
-                for (int r = 0; r < 3; ++r)
-                    for (int q = 0; q < 3; ++q) {
-                        const double v = -(b2arr[i] * (n[r] * n[q])) * sc.h2;
-                        M[(long long)(6 * a + r) * nd + 6 * b + q] += v;
-                        M[(long long)(6 * b + r) * nd + 6 * a + q] += v;
-                    }
-            }
-            if (act & 2) {
-                const double* blk = w.cblk + 108 * ce + 72;
-                for (int r = 0; r < 6; ++r)
-                    for (int q = 0; q < 6; ++q) {
-                        M[(long long)(6 * a + r) * nd + 6 * b + q] += blk[r * 6 + q];
-                        M[(long long)(6 * b + q) * nd + 6 * a + r] += blk[r * 6 + q];
-                    }
-            }
+            if (!w.cactive[(long long)sc.s * kMaxContacts + i]) continue;
+            const int* ci = w.cint + 5 * (sc.cb + i);
+            unite(ci[0] - sc.b0, ci[1] - sc.b0);
         }
-        for (int j = sc.j0; j < sc.j1; ++j) {
-            if (d.jtype[j] != 1) continue;
-            const int a = d.ja[j] - sc.b0, b = d.jb[j] - sc.b0;
-            const double* jb = w.jblk + 108 * j + 72;
-            for (int r = 0; r < 6; ++r)
-                for (int q = 0; q < 6; ++q) {
-                    M[(long long)(6 * a + r) * nd + 6 * b + q] += jb[r * 6 + q];
-                    M[(long long)(6 * b + q) * nd + 6 * a + r] += jb[r * 6 + q];
-                }
+        for (int j = sc.j0; j < sc.j1; ++j)
+            if (d.jtype[j] == 1) unite(d.ja[j] - sc.b0, d.jb[j] - sc.b0);
+        for (int i = 0; i < n; ++i) L.list[i] = -1;
+        int nc = 0;
+        for (int i = 0; i < n; ++i) {
+            const int r = uf_find(L.par, i);
+            if (L.list[r] < 0) L.list[r] = nc++;
+            L.cid[i] = L.list[r];
         }
+        // component offsets (par reused), then members in ascending body order
+        for (int c = 0; c <= nc; ++c) L.par[c] = 0;
+        for (int i = 0; i < n; ++i) L.par[L.cid[i] + 1]++;
+        int mx = 0;
+        for (int c = 0; c < nc; ++c) {
+            mx = L.par[c + 1] > mx ? L.par[c + 1] : mx;
+            L.par[c + 1] += L.par[c];
+        }
+        for (int i = 0; i < n; ++i) L.list[L.par[L.cid[i]]++] = i;
+        for (int c = nc; c >= 1; --c) L.par[c] = L.par[c - 1];
+        L.par[0] = 0;
+        L.ncomp = nc;
+        L.maxsize = mx;
     }
     __syncthreads();
-    // right-looking LDL^T, lower triangle, D on the diagonal
+}
+
+// 6x6 LDL^T (skyline recurrences) and solve of one body block; false on a zero pivot.
+__device__ bool ldlt6_solve(const double* A, const double* b, double* x) {
+    double L[36];
+    for (int i = 0; i < 6; ++i) {
+        for (int j = 0; j < i; ++j) {
+            double s = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[k * 6 + k] * L[j * 6 + k];
+            L[i * 6 + j] = s / L[j * 6 + j];
+        }
+        double dd = A[i * 6 + i];
+        for (int k = 0; k < i; ++k) dd -= L[i * 6 + k] * L[i * 6 + k] * L[k * 6 + k];
+        if (dd == 0.0 || !isfinite(dd)) return false;
+        L[i * 6 + i] = dd;
+    }
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * x[k];
+        x[i] = s;
+    }
+    for (int i = 0; i < 6; ++i) x[i] /= L[i * 6 + i];
+    for (int i = 5; i >= 0; --i)
+        for (int k = 0; k < i; ++k) x[k] -= L[i * 6 + k] * x[i];
+    return true;
+}
+
+// Returns 0 solved, 1 zero pivot, 2 a component exceeds kDenseBodies.
+__device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* rhs,
+                               double* xo, CompLds& L, int* ok_sh) {
+    build_components(d, w, sc, L);
+    if (L.maxsize > kDenseBodies) return 2;
     if (threadIdx.x == 0) *ok_sh = 1;
     __syncthreads();
-    for (int k = 0; k < nd; ++k) {
-        const double dk = M[(long long)k * nd + k];
-        if (!(dk != 0.0) || !isfinite(dk)) {
-            if (threadIdx.x == 0) *ok_sh = 0;
-            __syncthreads();
-            return false;
-        }
-        // column k below the diagonal: l_ik = a_ik / d_k; trailing update a_ij -= l_ik d_k l_jk
-        for (int i = k + 1 + threadIdx.x; i < nd; i += blockDim.x) M[(long long)i * nd + k] /= dk;
-        __syncthreads();
-        const int m = nd - k - 1;
-        for (long long t = threadIdx.x; t < (long long)m * m; t += blockDim.x) {
-            const int i = k + 1 + int(t / m), j = k + 1 + int(t % m);
-            if (j > i) continue;
-            M[(long long)i * nd + j] -= M[(long long)i * nd + k] * dk * M[(long long)j * nd + k];
-        }
-        __syncthreads();
-    }
-    // solve L D L^T x = rhs (single lane; nd is small on this path)
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < nd; ++i) {
-            double s = rhs[i];
-            for (int k = 0; k < i; ++k) s -= M[(long long)i * nd + k] * xo[k];
-            xo[i] = s;
-        }
-        for (int i = 0; i < nd; ++i) xo[i] /= M[(long long)i * nd + i];
-        for (int i = nd - 1; i >= 0; --i) {
-            double s = xo[i];
-            for (int k = i + 1; k < nd; ++k) s -= M[(long long)k * nd + i] * xo[k];
-            xo[i] = s;
-        }
+    // singletons
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int c = L.cid[li];
+        if (L.par[c + 1] - L.par[c] != 1) continue;
+        if (!ldlt6_solve(w.D + 36 * (sc.b0 + li), rhs + 6 * li, xo + 6 * li)) atomicAnd(ok_sh, 0);
     }
     __syncthreads();
-    return true;
+    if (!*ok_sh) return 1;
+    double* M = w.dense + (long long)sc.s * kDenseRows * kDenseRows;
+    for (int c = 0; c < L.ncomp; ++c) {
+        const int c0 = L.par[c], c1 = L.par[c + 1];
+        const int mb = c1 - c0;
+        if (mb < 2) continue;
+        const int m = 6 * mb;
+        // local row of body b: 6 * (index of b in the component's ascending list)
+        auto lpos = [&](int lb) {
+            int lo = c0, hi = c1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (L.list[mid] < lb) lo = mid + 1;
+                else hi = mid;
+            }
+            return lo - c0;
+        };
+        for (int t = threadIdx.x; t < m * m; t += blockDim.x) M[t] = 0.0;
+        __syncthreads();
+        for (int t = threadIdx.x; t < mb; t += blockDim.x) {
+            const double* D = w.D + 36 * (sc.b0 + L.list[c0 + t]);
+            for (int r = 0; r < 6; ++r)
+                for (int q = 0; q < 6; ++q) M[(6 * t + r) * m + 6 * t + q] = D[r * 6 + q];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // lower-triangle off-diagonal blocks: hinges, then barrier, then friction
+            for (int j = sc.j0; j < sc.j1; ++j) {
+                if (d.jtype[j] != 1) continue;
+                const int a = d.ja[j] - sc.b0, b = d.jb[j] - sc.b0;
+                if (L.cid[a] != c) continue;
+                const int pa = lpos(a), pb = lpos(b);
+                const double* jb = w.jblk + 108 * j + 72;  // rows a, cols b
+                for (int r = 0; r < 6; ++r)
+                    for (int q = 0; q < 6; ++q) {
+                        if (pa > pb) M[(6 * pa + r) * m + 6 * pb + q] += jb[r * 6 + q];
+                        else M[(6 * pb + r) * m + 6 * pa + q] += jb[q * 6 + r];
+                    }
+            }
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int i = 0; i < sc.nc; ++i) {
+                    const long long cc = sc.cb + i, ce = (long long)sc.s * kMaxContacts + i;
+                    const int act = w.cactive[ce];
+                    if (!(act & (pass == 0 ? 1 : 2))) continue;
+                    const int a = w.cint[5 * cc] - sc.b0, b = w.cint[5 * cc + 1] - sc.b0;
+                    if (L.cid[a] != c) continue;
+                    const int pa = lpos(a), pb = lpos(b);
+                    const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
+                    if (pass == 0) {
+                        const double* cd = w.cdbl + 7 * cc;
+                        const double n[3] = {cd[3], cd[4], cd[5]};
+                        for (int r = 0; r < 3; ++r)
+                            for (int q = 0; q < 3; ++q)
+                                M[(6 * hi + r) * m + 6 * lo + q] += (-(b2arr[i] * (n[r] * n[q]))) * sc.h2;
+                    } else {
+                        const double* blk = w.cblk + 144 * ce + (pa > pb ? 72 : 108);
+                        for (int r = 0; r < 6; ++r)
+                            for (int q = 0; q < 6; ++q) M[(6 * hi + r) * m + 6 * lo + q] += blk[r * 6 + q];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // left-looking LDL^T, column by column
+        for (int j = 0; j < m; ++j) {
+            if (threadIdx.x == 0) {
+                double dj = M[j * m + j];
+                for (int k = 0; k < j; ++k) dj -= M[j * m + k] * M[j * m + k] * M[k * m + k];
+                if (dj == 0.0 || !isfinite(dj)) *ok_sh = 0;
+                M[j * m + j] = dj;
+            }
+            __syncthreads();
+            if (!*ok_sh) return 1;
+            const double Dj = M[j * m + j];
+            for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) {
+                double s = M[i * m + j];
+                for (int k = 0; k < j; ++k) s -= M[i * m + k] * M[k * m + k] * M[j * m + k];
+                M[i * m + j] = s / Dj;
+            }
+            __syncthreads();
+        }
+        // solve: gather rhs, forward (column order), diagonal, backward
+        double* xs = w.r + 6 * sc.b0;  // scratch of this scene, m <= 6 n
+        for (int t = threadIdx.x; t < m; t += blockDim.x) xs[t] = rhs[6 * L.list[c0 + t / 6] + t % 6];
+        __syncthreads();
+        for (int k = 0; k < m; ++k) {
+            const double xk = xs[k];
+            for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) xs[i] -= M[i * m + k] * xk;
+            __syncthreads();
+        }
+        for (int t = threadIdx.x; t < m; t += blockDim.x) xs[t] /= M[t * m + t];
+        __syncthreads();
+        for (int i = m - 1; i >= 0; --i) {
+            const double xi = xs[i];
+            for (int k = threadIdx.x; k < i; k += blockDim.x) xs[k] -= M[i * m + k] * xi;
+            __syncthreads();
+        }
+        for (int t = threadIdx.x; t < m; t += blockDim.x) xo[6 * L.list[c0 + t / 6] + t % 6] = xs[t];
+        __syncthreads();
+    }
+    return 0;
 }
 
 // ------------------------------------------------------------------ Newton kernel
 __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     __shared__ double red[kBlk];
     __shared__ int ok_sh;
+    __shared__ CompLds comp;
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
     if (sp.alm_done) return;
@@ -713,8 +829,8 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         assemble_diag(d, w, sc, b2arr);
         for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) NG[i] = -G[i];
         __syncthreads();
-        const bool ok = sp.use_dense ? dense_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh)
-                                     : pcg_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh);
+        const int cs = component_solve(d, w, sc, b2arr, NG, DX, comp, &ok_sh);
+        const bool ok = cs == 0 ? true : cs == 2 ? pcg_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh) : false;
         __syncthreads();
         if (!ok) {
             if (threadIdx.x == 0) atomicOr(&sp.solve_fail, 1);
@@ -917,6 +1033,9 @@ __global__ void k_step_begin(Dev d, Work w, const double* dt) {
     if (s >= d.S) return;
     SceneParams& sp = d.sp[s];
     sp.dt = dt[s];
+    sp.dhat_cand = sp.dhat;  // candidates use the barrier's dhat before its first initialisation
+    sp.min_pair_dist = 1.0 / 0.0;
+    sp.ccd_any = 0;
     sp.evals = 0.0;
     sp.kappa_flag = 0;
     sp.toi = 1.0;
@@ -1133,13 +1252,13 @@ __global__ void __launch_bounds__(kBlk) k_forms_eval(Dev d, Work w, int s, int w
                         }
                 }
                 if (which == 1 && (act & 2)) {
-                    const double* blk = w.cblk + 108 * ce;
+                    const double* blk = w.cblk + 144 * ce;
                     for (int r = 0; r < 6; ++r)
                         for (int q = 0; q < 6; ++q) {
                             hess[(long long)(6 * a + r) * nd + 6 * a + q] += blk[r * 6 + q];
                             hess[(long long)(6 * b + r) * nd + 6 * b + q] += blk[36 + r * 6 + q];
                             hess[(long long)(6 * a + r) * nd + 6 * b + q] += blk[72 + r * 6 + q];
-                            hess[(long long)(6 * b + q) * nd + 6 * a + r] += blk[72 + r * 6 + q];
+                            hess[(long long)(6 * b + r) * nd + 6 * a + q] += blk[108 + r * 6 + q];
                         }
                 }
             }

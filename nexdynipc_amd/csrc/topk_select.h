@@ -7,115 +7,144 @@
 // libstdc++, so the device reproduces libstdc++'s published introselect (Musser's
 // introspective selection: median-of-three pivot moved to the front, unguarded Hoare
 // partition, insertion sort below 4 elements, heap selection once the 2*floor(log2 n)
-// depth budget is spent) step for step on the index array. It runs on one lane over
-// an LDS-resident list and is only entered when a tie straddles the boundary; the
-// common case is decided by a parallel sort (see candidates.hip).
+// depth budget is spent) step for step. It runs on one lane over an LDS-resident list
+// and is only entered when a tie straddles the boundary; the common case is decided by
+// a parallel sort (see geom.hip).
+//
+// The algorithm is written once over an element store: IdxStore permutes indices and
+// compares their scores (the reference's formulation), KVStore moves (score, index)
+// pairs so that a comparison is a single load. Both produce the same permutation.
 #pragma once
 #include "dmath.h"
 
 namespace nd {
 
-struct TopkCmp {
+struct IdxStore {
+    int* a;
     const double* s;
-    ND_HD bool operator()(int a, int b) const { return s[a] < s[b]; }
+    typedef int V;
+    ND_HD V get(int i) const { return a[i]; }
+    ND_HD void set(int i, V v) const { a[i] = v; }
+    ND_HD bool less(V x, V y) const { return s[x] < s[y]; }
 };
 
-ND_HD void tk_swap(int* a, int i, int j) {
-    const int t = a[i];
-    a[i] = a[j];
-    a[j] = t;
+struct KVPair {
+    double k;
+    int i;
+};
+struct KVStore {
+    double* k;
+    int* idx;
+    typedef KVPair V;
+    ND_HD V get(int i) const { KVPair p; p.k = k[i]; p.i = idx[i]; return p; }
+    ND_HD void set(int i, V v) const { k[i] = v.k; idx[i] = v.i; }
+    ND_HD bool less(V x, V y) const { return x.k < y.k; }
+};
+
+template <typename St>
+ND_HD void tk_swap(const St& a, int i, int j) {
+    const typename St::V t = a.get(i);
+    a.set(i, a.get(j));
+    a.set(j, t);
 }
 
-ND_HD void tk_push_heap(int* a, int first, int hole, int top, int value, const TopkCmp& c) {
+template <typename St>
+ND_HD void tk_push_heap(const St& a, int first, int hole, int top, typename St::V value) {
     int parent = (hole - 1) / 2;
-    while (hole > top && c(a[first + parent], value)) {
-        a[first + hole] = a[first + parent];
+    while (hole > top && a.less(a.get(first + parent), value)) {
+        a.set(first + hole, a.get(first + parent));
         hole = parent;
         parent = (hole - 1) / 2;
     }
-    a[first + hole] = value;
+    a.set(first + hole, value);
 }
 
-ND_HD void tk_adjust_heap(int* a, int first, int hole, int len, int value, const TopkCmp& c) {
+template <typename St>
+ND_HD void tk_adjust_heap(const St& a, int first, int hole, int len, typename St::V value) {
     const int top = hole;
     int second = hole;
     while (second < (len - 1) / 2) {
         second = 2 * (second + 1);
-        if (c(a[first + second], a[first + second - 1])) second--;
-        a[first + hole] = a[first + second];
+        if (a.less(a.get(first + second), a.get(first + second - 1))) second--;
+        a.set(first + hole, a.get(first + second));
         hole = second;
     }
     if ((len & 1) == 0 && second == (len - 2) / 2) {
         second = 2 * (second + 1);
-        a[first + hole] = a[first + second - 1];
+        a.set(first + hole, a.get(first + second - 1));
         hole = second - 1;
     }
-    tk_push_heap(a, first, hole, top, value, c);
+    tk_push_heap(a, first, hole, top, value);
 }
 
-ND_HD void tk_make_heap(int* a, int first, int last, const TopkCmp& c) {
+template <typename St>
+ND_HD void tk_make_heap(const St& a, int first, int last) {
     const int len = last - first;
     if (len < 2) return;
     int parent = (len - 2) / 2;
     while (true) {
-        const int v = a[first + parent];
-        tk_adjust_heap(a, first, parent, len, v, c);
+        tk_adjust_heap(a, first, parent, len, a.get(first + parent));
         if (parent == 0) return;
         parent--;
     }
 }
 
-ND_HD void tk_heap_select(int* a, int first, int middle, int last, const TopkCmp& c) {
-    tk_make_heap(a, first, middle, c);
+template <typename St>
+ND_HD void tk_heap_select(const St& a, int first, int middle, int last) {
+    tk_make_heap(a, first, middle);
     for (int i = middle; i < last; ++i) {
-        if (c(a[i], a[first])) {
-            const int v = a[i];
-            a[i] = a[first];
-            tk_adjust_heap(a, first, 0, middle - first, v, c);
+        if (a.less(a.get(i), a.get(first))) {
+            const typename St::V v = a.get(i);
+            a.set(i, a.get(first));
+            tk_adjust_heap(a, first, 0, middle - first, v);
         }
     }
 }
 
-ND_HD void tk_move_median_to_first(int* a, int result, int x, int y, int z, const TopkCmp& c) {
-    if (c(a[x], a[y])) {
-        if (c(a[y], a[z])) tk_swap(a, result, y);
-        else if (c(a[x], a[z])) tk_swap(a, result, z);
+template <typename St>
+ND_HD void tk_move_median_to_first(const St& a, int result, int x, int y, int z) {
+    if (a.less(a.get(x), a.get(y))) {
+        if (a.less(a.get(y), a.get(z))) tk_swap(a, result, y);
+        else if (a.less(a.get(x), a.get(z))) tk_swap(a, result, z);
         else tk_swap(a, result, x);
-    } else if (c(a[x], a[z])) {
+    } else if (a.less(a.get(x), a.get(z))) {
         tk_swap(a, result, x);
-    } else if (c(a[y], a[z])) {
+    } else if (a.less(a.get(y), a.get(z))) {
         tk_swap(a, result, z);
     } else {
         tk_swap(a, result, y);
     }
 }
 
-ND_HD int tk_unguarded_partition(int* a, int first, int last, int pivot, const TopkCmp& c) {
+template <typename St>
+ND_HD int tk_unguarded_partition(const St& a, int first, int last, int pivot) {
+    const typename St::V pv = a.get(pivot);  // the pivot slot is not moved by this loop
     while (true) {
-        while (c(a[first], a[pivot])) ++first;
+        while (a.less(a.get(first), pv)) ++first;
         --last;
-        while (c(a[pivot], a[last])) --last;
+        while (a.less(pv, a.get(last))) --last;
         if (!(first < last)) return first;
         tk_swap(a, first, last);
         ++first;
     }
 }
 
-ND_HD void tk_insertion_sort(int* a, int first, int last, const TopkCmp& c) {
+template <typename St>
+ND_HD void tk_insertion_sort(const St& a, int first, int last) {
     if (first == last) return;
     for (int i = first + 1; i != last; ++i) {
-        const int v = a[i];
-        if (c(v, a[first])) {
-            for (int k = i; k > first; --k) a[k] = a[k - 1];
-            a[first] = v;
+        const typename St::V v = a.get(i);
+        if (a.less(v, a.get(first))) {
+            for (int k = i; k > first; --k) a.set(k, a.get(k - 1));
+            a.set(first, v);
         } else {
             int hole = i, next = i - 1;
-            while (c(v, a[next])) {
-                a[hole] = a[next];
+            while (a.less(v, a.get(next))) {
+                a.set(hole, a.get(next));
                 hole = next;
                 --next;
             }
-            a[hole] = v;
+            a.set(hole, v);
         }
     }
 }
@@ -126,25 +155,26 @@ ND_HD int tk_floor_log2(int n) {
     return r;
 }
 
-// nth_element(a, a + nth, a + n) with comparator score[x] < score[y].
-ND_HD void tk_nth_element(int* a, int nth, int n, const TopkCmp& c) {
+// nth_element(a, a + nth, a + n) with the store's comparator.
+template <typename St>
+ND_HD void tk_nth_element(const St& a, int nth, int n) {
     if (n == 0 || nth == n) return;
     int first = 0, last = n;
     int depth = tk_floor_log2(n) * 2;
     while (last - first > 3) {
         if (depth == 0) {
-            tk_heap_select(a, first, nth + 1, last, c);
+            tk_heap_select(a, first, nth + 1, last);
             tk_swap(a, first, nth);
             return;
         }
         --depth;
         const int mid = first + (last - first) / 2;
-        tk_move_median_to_first(a, first, first + 1, mid, last - 1, c);
-        const int cut = tk_unguarded_partition(a, first + 1, last, first, c);
+        tk_move_median_to_first(a, first, first + 1, mid, last - 1);
+        const int cut = tk_unguarded_partition(a, first + 1, last, first);
         if (cut <= nth) first = cut;
         else last = cut;
     }
-    tk_insertion_sort(a, first, last, c);
+    tk_insertion_sort(a, first, last);
 }
 
 }  // namespace nd

@@ -21,7 +21,8 @@ Configurations (sizes, seeds and draw order are part of the benchmark definition
       {unit cube, icosphere subdivision 2 (r 0.5), 2x1x0.5 box}, dropped from
       U[0.2, 2] m onto a static ground box (0.7 m thick, so no body's vertices sit on
       its faces when the centres align); dt 1e-2, mu ~ U[0.1, 0.6] per scene.
-  c4  16-link hinge chain (0.4x0.1x0.1 links, link 0 pinned) + 48 unit cubes above a
+  c4  16-link hinge chain (0.4x0.1x0.1 links 5 cm apart, hinges mid-gap, link 0 pinned)
+      + 48 unit cubes above a
       static 20 deg ramp box; dt 1e-2, mu 0.3.
   c5  128 scenes of 32 unit cubes (grid jitter +-0.05 m, v ~ N(0, 1)) thrown at a
       static wall box and floor box; dt 1e-2, mu ~ U[0.1, 0.8] per scene.
@@ -409,11 +410,11 @@ def config4(scene=0):
     bodies = [_box_body("ramp", 12.0, 0.5, 6.0, [0.0, -0.25, 0.0], qr, static=True)]
     base = len(bodies)
     for k in range(16):
-        p = [-4.0 + 0.4 * k, 2.0, -1.5]
+        p = [-4.0 + 0.45 * k, 2.0, -1.5]  # 5 cm gaps: adjacent links never touch
         bodies.append(_box_body("link", 0.4, 0.1, 0.1, p, static=False))
     hinges = []
     for k in range(15):
-        hinges.append((base + k, base + k + 1, np.array([0.2, 0, 0]), np.array([-0.2, 0, 0]),
+        hinges.append((base + k, base + k + 1, np.array([0.225, 0, 0]), np.array([-0.225, 0, 0]),
                        np.array([0, 0, 1.0]), np.array([0, 0, 1.0])))
     # link 0 pinned with a FixedJoint (a world-anchored hinge is mis-indexed by the
     # reference: body_id_to_idx[-1] default-inserts 0, IPCSolver.cpp:972-973).

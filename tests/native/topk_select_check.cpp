@@ -24,15 +24,17 @@ int main() {
         }
         if (trial % 7 == 0) std::sort(s.begin(), s.end());
         if (trial % 11 == 0) std::sort(s.rbegin(), s.rend());
-        std::vector<int> a(n), b(n);
+        std::vector<int> a(n), b(n), c(n);
+        std::vector<double> ck(s);
         std::iota(a.begin(), a.end(), 0);
         std::iota(b.begin(), b.end(), 0);
+        std::iota(c.begin(), c.end(), 0);
         if (k < n) {
             std::nth_element(a.begin(), a.begin() + k, a.end(), [&](int x, int y) { return s[x] < s[y]; });
-            nd::TopkCmp c{s.data()};
-            nd::tk_nth_element(b.data(), k, n, c);
+            nd::tk_nth_element(nd::IdxStore{b.data(), s.data()}, k, n);
+            nd::tk_nth_element(nd::KVStore{ck.data(), c.data()}, k, n);
         }
-        if (a != b) {
+        if (a != b || a != c) {
             std::printf("MISMATCH trial %d n %d k %d\n", trial, n, k);
             return 1;
         }

@@ -212,10 +212,10 @@ def test_ccd_edge_vertex_kat_and_random(engine_factory):
                 assert toi[i] <= t + 1e-15 and abs(toi[i] - t) <= 1e-6, (kind, i, toi[i], t)
 
 
-@pytest.mark.parametrize("name,steps", [("c1", 200), ("micro", 60)])
+@pytest.mark.parametrize("name,steps", [("c1", 200), ("micro", 60), ("c4", 60)])
 def test_step_trajectory(engine_factory, name, steps):
-    if name == "c1":
-        sc = scenes.make_config("c1")
+    if name in ("c1", "c4"):
+        sc = scenes.make_config(name)
     else:
         sc = scenes.pack([scenes.micro_contact_scene(2, n_cubes=4)])
     e = engine_factory(sc)
@@ -235,8 +235,9 @@ def test_step_trajectory(engine_factory, name, steps):
     assert abs(kg[0] - ko) <= 1e-12 * abs(ko)
 
 
-def test_batched_scenes_match_per_scene_oracle(engine_factory):
-    specs = scenes.config3(n_scenes=6)
+@pytest.mark.parametrize("cfg", ["c3", "c5"])
+def test_batched_scenes_match_per_scene_oracle(engine_factory, cfg):
+    specs = scenes.config3(n_scenes=6) if cfg == "c3" else scenes.config5(n_scenes=4)
     sc = scenes.pack(specs)
     e = engine_factory(sc)
     steps = 10
