@@ -172,6 +172,12 @@ int32_t ndipc_timing_get(ndipc_ctx* ctx, const char* kernel, double* avg_ms, int
 /* Directed-rounding primitives of the interval CCD, evaluated on the device:
  * out6 = add_dn, add_up, sub_dn, sub_up, mul_dn, mul_up. */
 int32_t ndipc_round_ops(ndipc_ctx* ctx, const double* a, const double* b, int32_t n, double* out6);
+/* Elementary functions of the path evaluated on the device, element-wise: fn 0 log(x)
+ * (std::log in Barrier.cpp:11-46), 1 sin(x), 2 cos(x) (Eigen AngleAxis and slerp used by
+ * IPCSolver.cpp:616-626 and the CCD trajectories), 3 atan2(x, y) (Eigen
+ * quaternion -> AngleAxis, FixedJoint.cpp), 4 acos(x) (Eigen slerp). y may be NULL
+ * unless fn == 3. */
+int32_t ndipc_elemfn(ndipc_ctx* ctx, int32_t fn, const double* x, const double* y, int32_t n, double* out);
 
 #ifdef __cplusplus
 }

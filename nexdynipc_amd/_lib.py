@@ -47,7 +47,7 @@ EXPORTS = [
     "ndipc_get_joint_state", "ndipc_step", "ndipc_last_stats", "ndipc_detect_body_pairs",
     "ndipc_min_body_pair_distance", "ndipc_build_candidates", "ndipc_select_top_k", "ndipc_narrow_phase",
     "ndipc_scene_min_distance_flag", "ndipc_barrier_eval", "ndipc_friction_eval", "ndipc_ccd_toi_batch",
-    "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops",
+    "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn",
 ]
 
 
@@ -104,6 +104,7 @@ def library():
         "ndipc_timing_enable": ([vp, C.c_int32], C.c_int32),
         "ndipc_timing_get": ([vp, C.c_char_p, f64p, i32p], C.c_int32),
         "ndipc_round_ops": ([vp, f64p, f64p, C.c_int32, f64p], C.c_int32),
+        "ndipc_elemfn": ([vp, C.c_int32, f64p, f64p, C.c_int32, f64p], C.c_int32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -143,6 +144,9 @@ def make_params(dhat=1e-3, mu=0.0, friction=True, ccd=True, adaptive=True, alm_m
     p.alm_hardening_ratio = 2.0
     p.ccd_safety_factor = 0.8
     return p
+
+
+ELEMFN = ("log", "sin", "cos", "atan2", "acos")
 
 
 class NdipcError(RuntimeError):
@@ -364,4 +368,13 @@ class Engine:
         b = np.ascontiguousarray(b, dtype=np.float64)
         out = np.zeros((len(a), 6))
         self._ck(self.L.ndipc_round_ops(self.ctx, ptr(a, f64p), ptr(b, f64p), len(a), ptr(out, f64p)), "round_ops")
+        return out
+
+    def elemfn(self, fn, x, y=None):
+        """fn in ("log", "sin", "cos", "atan2", "acos"); atan2 takes (x, y) = (y_arg, x_arg) of C."""
+        code = ELEMFN.index(fn)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        yy = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float64)
+        out = np.zeros(len(x))
+        self._ck(self.L.ndipc_elemfn(self.ctx, code, ptr(x, f64p), ptr(yy, f64p), len(x), ptr(out, f64p)), "elemfn")
         return out

@@ -421,8 +421,8 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     w.raw_cnt = dalloc<int>(ctx, S);
     w.cint = dalloc<int>(ctx, size_t(S) * nd::kMaxRawContacts * 5);
     w.cdbl = dalloc<double>(ctx, size_t(S) * nd::kMaxRawContacts * 7);
-    w.sort_idx = dalloc<int>(ctx, size_t(S) * 65536);
-    w.sort_key = dalloc<double>(ctx, size_t(S) * 65536);
+    w.sort_idx = dalloc<int>(ctx, size_t(S) * nd::kSceneScratch);
+    w.sort_key = dalloc<double>(ctx, size_t(S) * nd::kSceneScratch);
     const size_t V6 = 6 * size_t(N);
     w.x = dalloc<double>(ctx, V6);
     w.xhat = dalloc<double>(ctx, V6);
@@ -976,6 +976,23 @@ int32_t ndipc_round_ops(ndipc_ctx* ctx, const double* a, const double* b, int32_
     nd::launch_round_ops(buf, buf + n, n, buf + 2 * size_t(n), ctx->stream);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out6, buf + 2 * size_t(n), 8 * 6 * size_t(n), hipMemcpyDeviceToHost));
+    hipFree(buf);
+    return 0;
+}
+
+int32_t ndipc_elemfn(ndipc_ctx* ctx, int32_t fn, const double* x, const double* y, int32_t n, double* out) {
+    if (!ctx) return -1;
+    if (fn < 0 || fn > 4) return fail(ctx, "elemfn: unknown function");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (n <= 0) return 0;
+    double* buf = nullptr;
+    HIPCHK(hipMalloc(&buf, 3 * 8 * size_t(n)));
+    HIPCHK(hipMemcpy(buf, x, 8 * size_t(n), hipMemcpyHostToDevice));
+    if (y) HIPCHK(hipMemcpy(buf + n, y, 8 * size_t(n), hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(buf + n, 0, 8 * size_t(n)));
+    nd::launch_elemfn(fn, buf, buf + n, n, buf + 2 * size_t(n), ctx->stream);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(out, buf + 2 * size_t(n), 8 * size_t(n), hipMemcpyDeviceToHost));
     hipFree(buf);
     return 0;
 }

@@ -7,9 +7,12 @@
 //   dot = (a0*b0 + a1*b1) + a2*b2; cross = (ay bz - az by, az bx - ax bz, ax by - ay bx);
 //   q*v = v + w*uv + qv x uv with uv = 2 (qv x v); Hamilton product;
 //   AngleAxis -> quaternion (cos, sin of the half angle); normalized = each / sqrt(|.|^2).
+// Elementary functions come from elemfn.h, which rounds identically on every machine.
 #pragma once
 #include <math.h>
 #include <stdint.h>
+
+#include "elemfn.h"
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -69,8 +72,9 @@ ND_HD Q qnormalized(Q q) {
 }
 ND_HD Q from_angle_axis(double angle, V3 axis) {
     const double ha = 0.5 * angle;
-    const double s = sin(ha);
-    return quat(cos(ha), s * axis.x, s * axis.y, s * axis.z);
+    double s, c;
+    ef::sincos(ha, &s, &c);
+    return quat(c, s * axis.x, s * axis.y, s * axis.z);
 }
 // Pose of a body at Newton variable theta: normalize(AngleAxis(|t|, t/|t|) * q) for
 // |t| > 1e-10 (IPCSolver.cpp:616-626, FrictionForm.cpp:206-223).
@@ -89,7 +93,7 @@ ND_HD Q update_rotation(Q q, V3 th) {
 ND_HD V3 rotation_vector(Q q) {
     const double n = norm(qvec(q));
     if (n != 0.0) {
-        const double ang = 2.0 * atan2(n, fabs(q.w));
+        const double ang = 2.0 * ef::atan2(n, fabs(q.w));
         const V3 axis = (q.w < 0.0) ? (-qvec(q)) / n : qvec(q) / n;
         return ang * axis;
     }
@@ -104,10 +108,10 @@ ND_HD Q slerp(Q a, double t, Q b) {
         s0 = 1.0 - t;
         s1 = t;
     } else {
-        const double th = acos(ad);
-        const double st = sin(th);
-        s0 = sin((1.0 - t) * th) / st;
-        s1 = sin(t * th) / st;
+        const double th = ef::acos(ad);
+        const double st = ef::sin(th);
+        s0 = ef::sin((1.0 - t) * th) / st;
+        s1 = ef::sin(t * th) / st;
     }
     if (d < 0.0) s1 = -s1;
     return quat(s0 * a.w + s1 * b.w, s0 * a.x + s1 * b.x, s0 * a.y + s1 * b.y, s0 * a.z + s1 * b.z);

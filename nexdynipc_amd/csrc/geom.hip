@@ -920,8 +920,8 @@ __global__ void __launch_bounds__(1024) k_np_cap(Dev d, Work w, int gate) {
     }
     int P = 1;
     while (P < n) P <<= 1;
-    double* sk = w.sort_key + (long long)s * 65536;
-    int* si = w.sort_idx + (long long)s * 65536;
+    double* sk = w.sort_key + (long long)s * kSceneScratch;
+    int* si = w.sort_idx + (long long)s * kSceneScratch;
     const long long cb = (long long)s * kMaxRawContacts;
     for (int i = threadIdx.x; i < P; i += blockDim.x) {
         sk[i] = i < n ? w.cdbl[7 * (cb + i) + 6] : 1.0 / 0.0;
@@ -946,7 +946,7 @@ __global__ void __launch_bounds__(1024) k_np_cap(Dev d, Work w, int gate) {
     }
     // gather the kept contacts into the sort scratch, then copy back
     int* ti = reinterpret_cast<int*>(sk);  // reuse: 2000 * 5 ints fit in the key scratch
-    double* td = w.sort_key + (long long)s * 65536 + 8192;
+    double* td = w.sort_key + (long long)s * kSceneScratch + 8192;
     for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
         const int src = si[i];
         for (int k = 0; k < 7; ++k) td[7 * i + k] = w.cdbl[7 * (cb + src) + k];
@@ -1302,6 +1302,25 @@ __global__ void k_round_ops(const double* a, const double* b, int n, double* out
 void launch_round_ops(const double* a, const double* b, int n, double* out, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_round_ops, dim3((n + 255) / 256), dim3(256), 0, st, a, b, n, out);
+}
+
+__global__ void k_elemfn(int fn, const double* x, const double* y, int n, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r;
+    switch (fn) {
+        case 0: r = ef::log(x[i]); break;
+        case 1: r = ef::sin(x[i]); break;
+        case 2: r = ef::cos(x[i]); break;
+        case 3: r = ef::atan2(x[i], y[i]); break;
+        default: r = ef::acos(x[i]); break;
+    }
+    out[i] = r;
+}
+
+void launch_elemfn(int fn, const double* x, const double* y, int n, double* out, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_elemfn, dim3((n + 255) / 256), dim3(256), 0, st, fn, x, y, n, out);
 }
 
 }  // namespace nd

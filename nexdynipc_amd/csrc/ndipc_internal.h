@@ -23,6 +23,7 @@ constexpr int kMaxContacts = 2000;    // IPCSolver.cpp:414
 constexpr int kMaxPrims = 2048;       // per-mesh vertex/edge/face limit of the LDS top-K
 constexpr int kMaxSceneBodies = 4096; // per-scene LBVH limit (LDS bitonic sort)
 constexpr int kHitWords = 48;         // 3072 candidate bits per pair (2x1024 FV + 1024 EE)
+constexpr int kSceneScratch = 65536;  // per-scene scratch doubles (sort keys, barrier b'')
 constexpr int kDenseBodies = 64;      // largest coupled component factored directly
 constexpr int kDenseRows = 6 * kDenseBodies;
 
@@ -116,8 +117,8 @@ struct Work {
     int* raw_cnt;            // [S]
     int* cint;               // [S * 40000 * 5]
     double* cdbl;            // [S * 40000 * 7]
-    int* sort_idx;           // [S * 65536] scratch for the 2000-cap sort
-    double* sort_key;        // [S * 65536]
+    int* sort_idx;           // [S * kSceneScratch] scratch for the 2000-cap sort
+    double* sort_key;        // [S * kSceneScratch] sort keys / Newton scratch
     // solver vectors, 6 per body
     double *x, *xhat, *xprev, *vprev, *g, *dx, *xt, *cg, *r, *z, *pv, *Ap, *tmp;
     double* D;               // [N*36]
@@ -154,6 +155,7 @@ void launch_ccd_earliest(const Dev& d, const Work& w, int max_pairs, hipStream_t
 void launch_ccd_batch(int kind, int n, const double* pa, const double* pb, const double* a, const double* b,
                       const double* earliest, double tol, int* hit, double* toi, int* pops, hipStream_t st);
 void launch_round_ops(const double* a, const double* b, int n, double* out, hipStream_t st);
+void launch_elemfn(int fn, const double* x, const double* y, int n, double* out, hipStream_t st);
 void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, hipStream_t st);
 
 void launch_step_begin(const Dev& d, const Work& w, const double* dt, hipStream_t st);

@@ -18,20 +18,20 @@ __device__ __forceinline__ double bar_v(double d, double dh, double k) {
     if (d >= dh) return 0.0;
     if (d <= 0.0) return 1.0 / 0.0;
     const double r = d / dh;
-    return -k * (d - dh) * (d - dh) * log(r);
+    return -k * (d - dh) * (d - dh) * ef::log(r);
 }
 __device__ __forceinline__ double bar_g(double d, double dh, double k) {
     if (d >= dh) return 0.0;
     if (d <= 0.0) return -1.0 / 0.0;
     const double diff = d - dh;
-    const double lr = log(d / dh);
+    const double lr = ef::log(d / dh);
     return -k * (2 * diff * lr + diff * diff / d);
 }
 __device__ __forceinline__ double bar_h(double d, double dh, double k) {
     if (d >= dh) return 0.0;
     if (d <= 0.0) return 1.0 / 0.0;
     const double diff = d - dh;
-    const double lr = log(d / dh);
+    const double lr = ef::log(d / dh);
     return -k * (2 * lr + 4 * diff / d - (diff * diff) / (d * d));
 }
 
@@ -335,51 +335,116 @@ __device__ double objective_grad(const Dev& d, const Work& w, const ScnCtx& sc, 
 }
 
 // Objective value (IPCSolver.cpp:154-165) and contact value (IPCSolver.cpp:223-235).
+// ------------------------------------------------------------------ ordered sums
+// The reference accumulates the objective and the line-search slopes in plain loops.
+// A parallel tree sum rounds differently, and the Armijo test and the ALM stopping test
+// compare these values in the last bits, so they are summed left to right here: the 64
+// lanes of a wavefront compute 64 consecutive terms, then every lane adds them in lane
+// order, reading lane k's term with v_readlane (no memory round trip). Independent
+// accumulators run on different wavefronts of the work-group at the same time.
+static_assert(kBlk == 256, "ordered sums assume four wavefronts per work-group");
+
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Objective value (IPCSolver.cpp:144-160): inertia, h^2 gravity, joints, h^2 barrier,
+// h^2 friction, each accumulator in the reference's loop order. Skipped elements add
+// +0.0, which leaves a sum started at +0.0 unchanged.
 __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* red,
                                 double* out_val, double* out_cval) {
     const double* XH = w.xhat;
-    double vin = 0.0, vg = 0.0;
-    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
-        const int b = sc.b0 + li;
-        const double m = d.mass[b];
-        const double* I = d.inertia + 9 * b;
-        const V3 dp = xpos(X, b) - xpos(XH, b);
-        vin += 0.5 * m * sqnorm(dp);
-        const V3 dt = xrot(X, b) - xrot(XH, b);
-        const V3 r = 0.5 * dt;
-        const double s0 = r.x * I[0] + r.y * I[3] + r.z * I[6];
-        const double s1 = r.x * I[1] + r.y * I[4] + r.z * I[7];
-        const double s2 = r.x * I[2] + r.y * I[5] + r.z * I[8];
-        vin += s0 * dt.x + s1 * dt.y + s2 * dt.z;
-        vg -= m * (0.0 * X[6 * b] + kG * X[6 * b + 1] + 0.0 * X[6 * b + 2]);
-    }
-    double vj = 0.0;
-    for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
-        JEval e;
-        joint_eval(d, j, X, e);
-        double t1 = 0.0, sq = 0.0;
-        for (int k = 0; k < 6; ++k) t1 += d.jlambda[6 * j + k] * e.C[k];
-        for (int k = 0; k < 6; ++k) sq += e.C[k] * e.C[k];
-        vj += t1 + 0.5 * d.jmu[j] * sq;
-    }
-    double vb = 0.0, vf = 0.0;
-    for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
-        const long long c = sc.cb + i;
-        if (sc.has_barrier) vb += bar_v(contact_dist(w, c, X), sc.dhat, sc.kappa);
-        if (sc.has_friction && i < sc.nfc) {
-            const double N = w.nf[(long long)sc.s * kMaxContacts + i];
-            if (N > 0.0) {
-                FKin k;
-                fric_kin(d, w, c, X, sc.fdt, k);
-                vf += fric_v(k.u, N, sc.fmu, sc.feps);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave == 0) {  // InertiaForm::value: val += m|dp|^2/2; val += dt . (I dt/2)
+        double acc = 0.0;
+        for (int base = 0; base < sc.n; base += 64) {
+            const int li = base + lane;
+            double t0 = 0.0, t1 = 0.0;
+            if (li < sc.n) {
+                const int b = sc.b0 + li;
+                const double m = d.mass[b];
+                const double* I = d.inertia + 9 * b;
+                const V3 dp = xpos(X, b) - xpos(XH, b);
+                const V3 dt = xrot(X, b) - xrot(XH, b);
+                const V3 r = 0.5 * dt;
+                const double s0 = r.x * I[0] + r.y * I[3] + r.z * I[6];
+                const double s1 = r.x * I[1] + r.y * I[4] + r.z * I[7];
+                const double s2 = r.x * I[2] + r.y * I[5] + r.z * I[8];
+                t0 = 0.5 * m * sqnorm(dp);
+                t1 = s0 * dt.x + s1 * dt.y + s2 * dt.z;
+            }
+            const int cnt = min(64, sc.n - base);
+            for (int k = 0; k < cnt; ++k) {
+                acc += lane_value(t0, k);
+                acc += lane_value(t1, k);
             }
         }
+        if (lane == 0) red[0] = acc;
+    } else if (wave == 1) {  // GravityForm::value: gv -= m (g . x)  (x - y == x + (-y))
+        double acc = 0.0;
+        for (int base = 0; base < sc.n; base += 64) {
+            const int li = base + lane;
+            double t = 0.0;
+            if (li < sc.n) {
+                const int b = sc.b0 + li;
+                t = -(d.mass[b] * (0.0 * X[6 * b] + kG * X[6 * b + 1] + 0.0 * X[6 * b + 2]));
+            }
+            const int cnt = min(64, sc.n - base);
+            for (int k = 0; k < cnt; ++k) acc += lane_value(t, k);
+        }
+        if (lane == 0) red[1] = acc;
+    } else if (wave == 2) {  // ConstraintForm::value: sum of lambda.C + mu/2 |C|^2
+        double acc = 0.0;
+        const int nj = sc.j1 - sc.j0;
+        for (int base = 0; base < nj; base += 64) {
+            double t = 0.0;
+            if (base + lane < nj) {
+                const int j = sc.j0 + base + lane;
+                JEval e;
+                joint_eval(d, j, X, e);
+                double t1 = 0.0, sq = 0.0;
+                for (int k = 0; k < 6; ++k) t1 += d.jlambda[6 * j + k] * e.C[k];
+                for (int k = 0; k < 6; ++k) sq += e.C[k] * e.C[k];
+                t = t1 + 0.5 * d.jmu[j] * sq;
+            }
+            const int cnt = min(64, nj - base);
+            for (int k = 0; k < cnt; ++k) acc += lane_value(t, k);
+        }
+        if (lane == 0) red[2] = acc;
+    } else {  // DistanceBarrierForm::value and FrictionForm::value, two chains
+        double ab = 0.0, af = 0.0;
+        for (int base = 0; base < sc.nc; base += 64) {
+            const int i = base + lane;
+            double tb = 0.0, tf = 0.0;
+            if (i < sc.nc) {
+                const long long c = sc.cb + i;
+                if (sc.has_barrier) tb = bar_v(contact_dist(w, c, X), sc.dhat, sc.kappa);
+                if (sc.has_friction && i < sc.nfc) {
+                    const double N = w.nf[(long long)sc.s * kMaxContacts + i];
+                    if (N > 0.0) {
+                        FKin k;
+                        fric_kin(d, w, c, X, sc.fdt, k);
+                        tf = fric_v(k.u, N, sc.fmu, sc.feps);
+                    }
+                }
+            }
+            const int cnt = min(64, sc.nc - base);
+            for (int k = 0; k < cnt; ++k) {
+                ab += lane_value(tb, k);
+                af += lane_value(tf, k);
+            }
+        }
+        if (lane == 0) {
+            red[3] = ab;
+            red[4] = af;
+        }
     }
-    vin = wg_reduce(vin, red, OpAdd());
-    vg = wg_reduce(vg, red, OpAdd());
-    vj = wg_reduce(vj, red, OpAdd());
-    vb = wg_reduce(vb, red, OpAdd());
-    vf = wg_reduce(vf, red, OpAdd());
+    __syncthreads();
+    const double vin = red[0], vg = red[1], vj = red[2], vb = red[3], vf = red[4];
+    __syncthreads();
     double val = vin;
     val += sc.h2 * vg;
     val += vj;
@@ -388,6 +453,27 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
     if (sc.has_friction) { val += sc.h2 * vf; cval += sc.h2 * vf; }
     *out_val = val;
     *out_cval = cval;
+}
+
+// (sum_i a0[i] b[i], sum_i a1[i] b[i]) left to right, on wavefronts 0 and 1.
+__device__ void serial_dot2(const double* a0, const double* a1, const double* b, int n, double* red, double* o0,
+                            double* o1) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave < 2) {
+        const double* a = wave ? a1 : a0;
+        double acc = 0.0;
+        for (int base = 0; base < n; base += 64) {
+            const int i = base + lane;
+            const double t = i < n ? a[i] * b[i] : 0.0;
+            const int cnt = min(64, n - base);
+            for (int k = 0; k < cnt; ++k) acc += lane_value(t, k);
+        }
+        if (lane == 0) red[wave] = acc;
+    }
+    __syncthreads();
+    *o0 = red[0];
+    *o1 = red[1];
+    __syncthreads();
 }
 
 // Diagonal 6x6 blocks: inertia + joints + h2 (barrier, then friction) contributions.
@@ -816,7 +902,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     double* XT = w.xt + 6 * sc.b0;
     double* CG = w.cg + 6 * sc.b0;
     double* NG = w.tmp + 6 * sc.b0;
-    double* b2arr = w.sort_key + (long long)s * 65536;  // per-contact b'' scratch (<= 2000)
+    double* b2arr = w.sort_key + (long long)s * kSceneScratch;  // per-contact b'' scratch (<= 2000)
     int conv = 0;
     int it = 0;
     const double* Xg = w.x;
@@ -839,8 +925,8 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         // line search (LineSearch.cpp:6-58), alpha0 = min(1, maxStep) = 1
         double fx, fc;
         objective_value(d, w, sc, Xg, red, &fx, &fc);
-        const double descent = wg_dot(G, DX, sc.nd, red);
-        const double cdesc = wg_dot(CG, DX, sc.nd, red);
+        double descent, cdesc;
+        serial_dot2(G, CG, DX, sc.nd, red, &descent, &cdesc);
         if (descent > 0) break;
         double alpha = 1.0;
         bool accepted = false;
@@ -879,7 +965,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     dual = wg_reduce(dual, red, OpMax());
     if (threadIdx.x == 0) {
         sp.converged = conv;
-        sp.newton_iters += it + 1;
+        sp.newton_iters += (it < 20 ? it : 19) + 1;  // NewtonSolver reports the last index
         sp.viol = viol;
         sp.dual = dual;
     }
@@ -991,7 +1077,7 @@ __global__ void __launch_bounds__(kBlk) k_alm_post(Dev d, Work w) {
         load_scn(d, s, sc);
         build_csr(w, sc);
         // contact gradient without h^2 (barrier + friction)
-        double* b2arr = w.sort_key + (long long)s * 65536;
+        double* b2arr = w.sort_key + (long long)s * kSceneScratch;
         ScnCtx sb = sc;
         sb.j1 = sb.j0;  // no joints here
         sb.has_barrier = 1;  // barrier form exists whenever contact forms are needed
@@ -1195,7 +1281,7 @@ __global__ void __launch_bounds__(kBlk) k_forms_eval(Dev d, Work w, int s, int w
     sc.nfc = sc.nc;
     sc.j1 = sc.j0;
     const double* X = x - 6 * (long long)sc.b0;  // contacts carry global body indices
-    double* b2arr = w.sort_key + (long long)s * 65536;
+    double* b2arr = w.sort_key + (long long)s * kSceneScratch;
     build_csr(w, sc);
     eval_elements(d, w, sc, X, b2arr);
     double v = 0.0;

@@ -342,20 +342,20 @@ static double barrier_value(double d, double dh, double k) {
     if (d >= dh) return 0.0;
     if (d <= 0.0) return std::numeric_limits<double>::infinity();
     const double r = d / dh;
-    return -k * (d - dh) * (d - dh) * std::log(r);
+    return -k * (d - dh) * (d - dh) * nd::ef::log(r);
 }
 static double barrier_gradient(double d, double dh, double k) {
     if (d >= dh) return 0.0;
     if (d <= 0.0) return -std::numeric_limits<double>::infinity();
     const double diff = d - dh;
-    const double lr = std::log(d / dh);
+    const double lr = nd::ef::log(d / dh);
     return -k * (2 * diff * lr + diff * diff / d);
 }
 static double barrier_hessian(double d, double dh, double k) {
     if (d >= dh) return 0.0;
     if (d <= 0.0) return std::numeric_limits<double>::infinity();
     const double diff = d - dh;
-    const double lr = std::log(d / dh);
+    const double lr = nd::ef::log(d / dh);
     return -k * (2 * lr + 4 * diff / d - (diff * diff) / (d * d));
 }
 
@@ -1841,6 +1841,22 @@ void ndo_round_ops(const double* a, const double* b, int32_t n, double* out6) {
         out6[6 * i + 4] = mul_dn(a[i], b[i]);
         out6[6 * i + 5] = mul_up(a[i], b[i]);
     }
+}
+
+// Elementary functions of the path, element-wise: fn 0 log(x), 1 sin(x), 2 cos(x),
+// 3 atan2(x, y), 4 acos(x).
+int32_t ndo_elemfn(int32_t fn, const double* x, const double* y, int32_t n, double* out) {
+    for (int i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = nd::ef::log(x[i]); break;
+            case 1: out[i] = nd::ef::sin(x[i]); break;
+            case 2: out[i] = nd::ef::cos(x[i]); break;
+            case 3: out[i] = nd::ef::atan2(x[i], y[i]); break;
+            case 4: out[i] = nd::ef::acos(x[i]); break;
+            default: return -1;
+        }
+    }
+    return 0;
 }
 
 int32_t ndo_objective(void* h, const double* x, double* value, double* grad, double* hess) {

@@ -112,6 +112,7 @@ int32_t ndo_ccd_earliest_toi(void* h, double dt, double* out);
 int32_t ndo_root_finder_kat(int32_t which, double* lo, double* hi);
 /* Directed-rounding primitives, exposed so tests can compare them with a device build. */
 void ndo_round_ops(const double* a, const double* b, int32_t n, double* out6);
+int32_t ndo_elemfn(int32_t fn, const double* x, const double* y, int32_t n, double* out);
 
 /* Newton system at x (SimulationProblem with the current contact forms): */
 int32_t ndo_objective(void* h, const double* x, double* value, double* grad, double* hess);

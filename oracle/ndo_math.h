@@ -17,6 +17,11 @@
 // real reference and is recorded in DESIGN.md.
 #pragma once
 #include <cmath>
+
+// log, sin, cos, atan2 and acos come from the library's elemfn.h: built from correctly
+// rounded IEEE operations only, they give the same bits here as on the GPU, whereas two
+// libms may differ in the last bit. They are pinned against the C library (<= 2 ulp).
+#include "elemfn.h"
 #include <limits>
 #include <algorithm>
 
@@ -80,8 +85,9 @@ inline Q qnormalized(const Q& q) {
 }
 inline Q from_angle_axis(double angle, const V3& axis) {
     const double ha = 0.5 * angle;
-    const double s = std::sin(ha);
-    return Q(std::cos(ha), s * axis.x, s * axis.y, s * axis.z);
+    double s, c;
+    nd::ef::sincos(ha, &s, &c);
+    return Q(c, s * axis.x, s * axis.y, s * axis.z);
 }
 // Incremental rotation used throughout IPCSolver / FrictionForm / JointPrimitives:
 // q(theta) = normalize(AngleAxis(|theta|, theta/|theta|) * q_ref) if |theta| > 1e-10.
@@ -105,7 +111,7 @@ inline Q update_rotation(const Q& q_ref, const V3& theta) {
 inline V3 to_rotation_vector(const Q& q) {
     double n = norm(q.vec());
     if (n != 0.0) {
-        const double angle = 2.0 * std::atan2(n, std::abs(q.w));
+        const double angle = 2.0 * nd::ef::atan2(n, std::abs(q.w));
         const V3 axis = (q.w < 0.0) ? (-q.vec()) / n : q.vec() / n;
         return angle * axis;
     }
@@ -120,10 +126,10 @@ inline Q slerp(const Q& a, double t, const Q& b) {
         scale0 = 1.0 - t;
         scale1 = t;
     } else {
-        const double theta = std::acos(absD);
-        const double sinTheta = std::sin(theta);
-        scale0 = std::sin((1.0 - t) * theta) / sinTheta;
-        scale1 = std::sin(t * theta) / sinTheta;
+        const double theta = nd::ef::acos(absD);
+        const double sinTheta = nd::ef::sin(theta);
+        scale0 = nd::ef::sin((1.0 - t) * theta) / sinTheta;
+        scale1 = nd::ef::sin(t * theta) / sinTheta;
     }
     if (d < 0.0) scale1 = -scale1;
     return Q(scale0 * a.w + scale1 * b.w, scale0 * a.x + scale1 * b.x,

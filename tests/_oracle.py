@@ -95,6 +95,8 @@ def lib():
         L.ndo_select_top_k.argtypes = [f64p, C.c_int32, C.c_int32, i32p]
         L.ndo_round_ops.restype = None
         L.ndo_round_ops.argtypes = [f64p, f64p, C.c_int32, f64p]
+        L.ndo_elemfn.restype = C.c_int32
+        L.ndo_elemfn.argtypes = [C.c_int32, f64p, f64p, C.c_int32, f64p]
         L.ndo_objective.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
         L.ndo_scene_min_distance.argtypes = [C.c_void_p, f64p]
         L.ndo_set_sampling.restype = C.c_int32
@@ -333,4 +335,17 @@ def round_ops(a, b):
     b = np.ascontiguousarray(b, dtype=np.float64)
     out = np.zeros((len(a), 6))
     L.ndo_round_ops(ptr(a, f64p), ptr(b, f64p), len(a), ptr(out, f64p))
+    return out
+
+
+ELEMFN = ("log", "sin", "cos", "atan2", "acos")
+
+
+def elemfn(fn, x, y=None):
+    """The oracle's elementary functions; atan2 is called as atan2(x, y)."""
+    L = lib()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    yy = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float64)
+    out = np.zeros(len(x))
+    assert L.ndo_elemfn(ELEMFN.index(fn), ptr(x, f64p), ptr(yy, f64p), len(x), ptr(out, f64p)) == 0
     return out

@@ -62,6 +62,25 @@ def test_round_ops_bit_exact(engine_factory):
     assert np.array_equal(ref.view(np.int64), got.view(np.int64))
 
 
+def test_elemfn_bit_exact(engine_factory):
+    """log / sin / cos / atan2 / acos on the device give the oracle's bits."""
+    e = engine_factory(scenes.make_config("c1"))
+    rng = np.random.default_rng(21)
+    xs = {
+        "log": np.concatenate([rng.uniform(1e-9, 1.0, 4096), np.exp(rng.uniform(-700, 700, 4096)),
+                               [5e-324, 1e-310, 1.0, 0.0, -1.0, np.inf]]),
+        "sin": np.concatenate([rng.uniform(-4, 4, 4096), rng.uniform(-1e-3, 1e-3, 1024), [0.0, -0.0, 1e6]]),
+        "cos": np.concatenate([rng.uniform(-4, 4, 4096), rng.uniform(-1e-3, 1e-3, 1024), [0.0, 1e6]]),
+        "acos": np.concatenate([rng.uniform(-1, 1, 4096), 1.0 - np.abs(rng.normal(0, 1e-6, 1024))]),
+        "atan2": rng.normal(size=4096),
+    }
+    for fn, x in xs.items():
+        y = rng.normal(size=len(x)) if fn == "atan2" else None
+        got = e.elemfn(fn, x, y)
+        ref = O.elemfn(fn, x, y)
+        assert np.array_equal(got.view(np.int64), ref.view(np.int64)), fn
+
+
 def test_select_top_k_ties(engine_factory):
     sc = scenes.make_config("c1")
     e = engine_factory(sc)
@@ -230,6 +249,10 @@ def test_step_trajectory(engine_factory, name, steps):
         worst_p = max(worst_p, float(np.max(np.abs(pg - po))))
         worst_q = max(worst_q, float(np.max(quat_angle(qg, qo))))
         assert worst_p <= 1e-8 and worst_q <= 1e-8, (k, worst_p, worst_q)
+        # every coupled component here is factored directly, every sum runs in the
+        # oracle's order: the trajectory is bit-identical, iteration counts included
+        assert np.array_equal(pg, po) and np.array_equal(qg, qo), (k, worst_p, worst_q)
+        assert np.array_equal(e.stats()[0][:3], np.asarray(o.stats())[:3]), k
     ko, _, _ = o.barrier()
     kg, _, _ = e.barrier()
     assert abs(kg[0] - ko) <= 1e-12 * abs(ko)
