@@ -196,10 +196,17 @@ __device__ void lbvh_query(const Dev& d, const Work& w, int s, int i, double r, 
 // check at 2 dhat, IPCSolver.cpp:825; CCD gate at max(1e-6, dhat) + v_max dt,
 // IPCSolver.cpp:365) share one traversal per body with the largest inflation; each hit
 // is then classified with the exact per-query expression. One thread per body over all
-// scenes; pass 0 counts, pass 1 emits at scanned offsets (pairs stay sorted by (i, j)).
+// scenes; pass 0 counts, pass 1 emits at scanned offsets.
+//
+// Pair order is the reference's: BVH::query walks its tree depth first with a stack,
+// pushing the left child before the right one, and its builder never reorders the
+// primitives (the SAH split index always lies strictly inside the range), so leaves are
+// visited in descending primitive order. detectBodyPairs therefore lists (i, j) by i
+// ascending, then j descending (BroadPhase.cpp, BVH.cpp query/build). The contact list,
+// and with it every per-body sum of the Newton objective, inherits this order.
 __device__ __forceinline__ void bp_sort_insert(int* outp, int o, int k, int i, int j) {
     int pos = k;
-    while (pos > 0 && outp[2 * (o + pos - 1) + 1] > j) {
+    while (pos > 0 && outp[2 * (o + pos - 1) + 1] < j) {
         outp[2 * (o + pos)] = outp[2 * (o + pos - 1)];
         outp[2 * (o + pos) + 1] = outp[2 * (o + pos - 1) + 1];
         --pos;
@@ -359,9 +366,9 @@ __global__ void __launch_bounds__(kBlock) k_broad_phase(Dev d, Work w, int mode,
                 const double gap = box_distance(ldbox(w.wbox, i), ldbox(w.wbox, j));
                 const bool keep = (mode == 0) ? (gap <= r) : (mode == 3 ? true : (gap <= margin));
                 if (keep && k < c) {
-                    // insertion into the sorted segment (j ascending)
+                    // insertion into the segment in the reference's order (j descending)
                     int pos = k;
-                    while (pos > 0 && outp[2 * (o + pos - 1) + 1] > j) {
+                    while (pos > 0 && outp[2 * (o + pos - 1) + 1] < j) {
                         outp[2 * (o + pos)] = outp[2 * (o + pos - 1)];
                         outp[2 * (o + pos) + 1] = outp[2 * (o + pos - 1) + 1];
                         --pos;
@@ -424,8 +431,10 @@ void launch_broad_phase(const Dev& d, const Work& w, int mode, double r, bool, c
 // select_top_k_indices (ContactAssembler.cpp:67-88) for the six lists of every close
 // pair (ContactAssembler.cpp:249-302). Scores are point_aabb_distance of world
 // vertices to the other body's world AABB; edges take the min over 2 endpoints, faces
-// over 3 (std::min / std::min({..}) keep the first smallest). Lists are sorted by
-// (score, index); the selected set follows std::nth_element exactly (topk_select.h).
+// over 3 (std::min / std::min({..}) keep the first smallest). The kept set and its
+// order are those of std::nth_element followed by std::sort (topk_select.h): a parallel
+// sort decides every list without equal scores among the kept ones, a single lane
+// replays the standard library's algorithms on the others.
 __device__ __forceinline__ bool key_less(double sa, int ia, double sb, int ib) {
     return sa < sb || (sa == sb && ia < ib);
 }
@@ -458,13 +467,17 @@ __device__ void wg_select_topk(const double* s_orig, double* sk, int* si, int n,
     }
     __syncthreads();
     wg_bitonic_pairs(sk, si, P);
-    const bool straddle = (k < n) && (sk[k] == sk[k - 1]);
-    if (!straddle) {
+    // Equal scores among sorted positions 0..k (position k: a tie straddling the cut)
+    // make both the kept set and its order depend on the standard library's algorithms;
+    // otherwise the kept set is unique and sorted order is the reference's order.
+    const int last = k < n ? k : n - 1;
+    int tie = 0;
+    for (int p = 1 + threadIdx.x; p <= last; p += blockDim.x) tie |= (sk[p] == sk[p - 1]);
+    if (!__syncthreads_or(tie)) {
         for (int i = threadIdx.x; i < k; i += blockDim.x) out[i] = si[i];
         __syncthreads();
         return;
     }
-    __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         sk[i] = s_orig[i];
         si[i] = i;
@@ -472,20 +485,8 @@ __device__ void wg_select_topk(const double* s_orig, double* sk, int* si, int n,
     __syncthreads();
     if (threadIdx.x == 0) {
         KVStore st{sk, si};
-        tk_nth_element(st, k, n);
-        // order the kept set by (score, index)
-        for (int a = 1; a < k; ++a) {
-            const double vk = sk[a];
-            const int v = si[a];
-            int b = a - 1;
-            while (b >= 0 && key_less(vk, v, sk[b], si[b])) {
-                sk[b + 1] = sk[b];
-                si[b + 1] = si[b];
-                --b;
-            }
-            sk[b + 1] = vk;
-            si[b + 1] = v;
-        }
+        if (k < n) tk_nth_element(st, k, n);
+        tk_sort(st, k);
         for (int i = 0; i < k; ++i) out[i] = si[i];
     }
     __syncthreads();

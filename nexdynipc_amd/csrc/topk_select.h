@@ -2,14 +2,15 @@
 //
 // NexDynIPC keeps the K nearest primitives per body pair with
 //   std::nth_element(idx, idx + k, idx + n, score(a) < score(b)); idx.resize(k);
-// (ContactAssembler.cpp:67-88, idx = iota). When scores tie at the K boundary the kept
-// SET depends on the standard library's introselect. The oracle is built with GCC's
-// libstdc++, so the device reproduces libstdc++'s published introselect (Musser's
-// introspective selection: median-of-three pivot moved to the front, unguarded Hoare
-// partition, insertion sort below 4 elements, heap selection once the 2*floor(log2 n)
-// depth budget is spent) step for step. It runs on one lane over an LDS-resident list
-// and is only entered when a tie straddles the boundary; the common case is decided by
-// a parallel sort (see geom.hip).
+// then orders them with std::sort (ContactAssembler.cpp:67-88, idx = iota). When
+// scores tie at the K boundary the kept SET depends on the standard library's
+// introselect, and when kept scores tie their ORDER depends on its introsort. The
+// oracle is built with GCC's libstdc++, so the device reproduces libstdc++'s published
+// algorithms (Musser's introspective selection and sorting: median-of-three pivot moved
+// to the front, unguarded Hoare partition, insertion sort on small ranges, heap
+// selection / heap sort once the 2*floor(log2 n) depth budget is spent) step for step.
+// They run on one lane over an LDS-resident list and only for lists with equal kept
+// scores; the common case is decided by a parallel sort (see geom.hip).
 //
 // The algorithm is written once over an element store: IdxStore permutes indices and
 // compares their scores (the reference's formulation), KVStore moves (score, index)
@@ -153,6 +154,71 @@ ND_HD int tk_floor_log2(int n) {
     int r = 0;
     while (n > 1) { n >>= 1; ++r; }
     return r;
+}
+
+template <typename St>
+ND_HD void tk_unguarded_linear_insert(const St& a, int last) {
+    const typename St::V v = a.get(last);
+    int next = last - 1;
+    while (a.less(v, a.get(next))) {
+        a.set(last, a.get(next));
+        last = next;
+        --next;
+    }
+    a.set(last, v);
+}
+
+template <typename St>
+ND_HD void tk_sort_heap(const St& a, int first, int last) {
+    while (last - first > 1) {
+        --last;
+        const typename St::V v = a.get(last);
+        a.set(last, a.get(first));
+        tk_adjust_heap(a, first, 0, last - first, v);
+    }
+}
+
+// sort(a, a + n) with the store's comparator: introsort as in libstdc++ (median-of-three
+// quicksort down to ranges of 16, heap sort once the 2*floor(log2 n) depth budget is
+// spent, then insertion sort over the first 16 and unguarded insertion of the rest).
+// Partitioned ranges are independent, so an explicit stack replaces the recursion; the
+// depths on it strictly decrease, so it never holds more than 2*log2(n) + 1 entries.
+template <typename St>
+ND_HD void tk_sort(const St& a, int n) {
+    if (n < 2) return;
+    int sf[64], sl[64], sd[64];
+    int sp = 0;
+    sf[0] = 0;
+    sl[0] = n;
+    sd[0] = 2 * tk_floor_log2(n);
+    sp = 1;
+    while (sp > 0) {
+        --sp;
+        const int first = sf[sp];
+        int last = sl[sp], depth = sd[sp];
+        while (last - first > 16) {
+            if (depth == 0) {
+                tk_heap_select(a, first, last, last);
+                tk_sort_heap(a, first, last);
+                break;
+            }
+            --depth;
+            const int mid = first + (last - first) / 2;
+            tk_move_median_to_first(a, first, first + 1, mid, last - 1);
+            const int cut = tk_unguarded_partition(a, first + 1, last, first);
+            sf[sp] = cut;
+            sl[sp] = last;
+            sd[sp] = depth;
+            ++sp;
+            last = cut;
+        }
+    }
+    if (n > 16) {
+        tk_insertion_sort(a, 0, 16);
+        for (int i = 16; i < n; ++i) tk_unguarded_linear_insert(a, i);
+    } else {
+        tk_insertion_sort(a, 0, n);
+    }
 }
 
 // nth_element(a, a + nth, a + n) with the store's comparator.

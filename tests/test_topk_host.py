@@ -1,6 +1,6 @@
 """The device top-K restatement (nexdynipc_amd/csrc/topk_select.h), compiled for the
-host, reproduces the standard library's std::nth_element permutation exactly on
-tie-heavy inputs (the oracle's select_top_k_indices calls std::nth_element)."""
+host, reproduces the standard library's std::nth_element and std::sort permutations
+exactly on tie-heavy inputs (the oracle's select_top_k_indices calls both)."""
 import os
 import subprocess
 
