@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--batch", action="store_true", help="256-scene batch (config 3) instead of config 2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=2000)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 20 device copies of 256 MiB (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs on one GPU "
@@ -63,6 +65,15 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe:
+        a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(20):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_per_copy": a.numel() * 8, "copies": 20}))
+        return
 
     from nexdynipc_amd import scenes
     from nexdynipc_amd._lib import Engine

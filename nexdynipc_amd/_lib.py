@@ -47,7 +47,7 @@ EXPORTS = [
     "ndipc_get_joint_state", "ndipc_step", "ndipc_last_stats", "ndipc_detect_body_pairs",
     "ndipc_min_body_pair_distance", "ndipc_build_candidates", "ndipc_select_top_k", "ndipc_narrow_phase",
     "ndipc_scene_min_distance_flag", "ndipc_barrier_eval", "ndipc_friction_eval", "ndipc_ccd_toi_batch",
-    "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn",
+    "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters",
 ]
 
 
@@ -105,6 +105,7 @@ def library():
         "ndipc_timing_get": ([vp, C.c_char_p, f64p, i32p], C.c_int32),
         "ndipc_round_ops": ([vp, f64p, f64p, C.c_int32, f64p], C.c_int32),
         "ndipc_elemfn": ([vp, C.c_int32, f64p, f64p, C.c_int32, f64p], C.c_int32),
+        "ndipc_solver_counters": ([vp, f64p], C.c_int32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -232,6 +233,13 @@ class Engine:
     def stats(self):
         o = np.zeros((self.sc.n_scenes, 8))
         self._ck(self.L.ndipc_last_stats(self.ctx, ptr(o, f64p)), "last_stats")
+        return o
+
+    def solver_counters(self):
+        """Per scene: newton iterations, line-search trials, direct solves, PCG solves,
+        PCG iterations, largest coupled component (bodies), for the last step."""
+        o = np.zeros((self.sc.n_scenes, 6))
+        self._ck(self.L.ndipc_solver_counters(self.ctx, ptr(o, f64p)), "solver_counters")
         return o
 
     def eval_counter(self):

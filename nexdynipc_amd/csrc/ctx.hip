@@ -624,6 +624,22 @@ int32_t ndipc_last_stats(ndipc_ctx* ctx, double* out8) {
     return 0;
 }
 
+int32_t ndipc_solver_counters(ndipc_ctx* ctx, double* out6) {
+    if (check_ctx(ctx)) return -1;
+    if (sync_sp_from_dev(ctx)) return -1;
+    for (int s = 0; s < ctx->S; ++s) {
+        const SceneParams& q = ctx->sp[s];
+        double* o = out6 + 6 * s;
+        o[0] = q.newton_iters;
+        o[1] = q.ls_trials;
+        o[2] = q.solves_direct;
+        o[3] = q.solves_pcg;
+        o[4] = q.pcg_iters;
+        o[5] = q.comp_max;
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------- function level
 static int prep_geometry(ndipc_ctx* ctx) {
     nd::launch_world_boxes(ctx->d, ctx->w, ctx->stream);

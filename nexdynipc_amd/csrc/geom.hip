@@ -271,44 +271,36 @@ __global__ void k_bp_step(Dev d, Work w, int pass) {
 }
 
 // Exclusive scans of the per-body counts (candidates, prox) -> per-body offsets within
-// their scene, per-scene offsets and totals. One work-group.
+// their scene and per-scene totals; one work-group per (scene, list). The per-scene
+// offsets are then a scan over scenes (k_scan_counts).
 __global__ void __launch_bounds__(kBlock) k_bp_scan(Dev d, Work w) {
     __shared__ int sh[kBlock];
-    for (int list = 0; list < 2; ++list) {
-        const int* cnt = w.bcnt + list * d.N;
-        int* boff = w.boff + list * d.N;
-        int* soff = list == 0 ? w.pair_off : w.prox_off;
-        int* scnt = list == 0 ? w.pair_cnt : w.prox_cnt;
-        int* tot = list == 0 ? w.n_pairs_total : w.n_prox_total;
-        int run = 0;
-        for (int s = 0; s < d.S; ++s) {
-            const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
-            if (threadIdx.x == 0) soff[s] = run;
-            int local = 0;
-            for (int base = b0; base < b1; base += kBlock) {
-                const int b = base + threadIdx.x;
-                const int v = b < b1 ? cnt[b] : 0;
-                int ex;
-                const int t = wg_exclusive_scan(v, sh, &ex);
-                if (b < b1) boff[b] = local + ex;
-                local += t;
-            }
-            if (threadIdx.x == 0) scnt[s] = local;
-            run += local;
-        }
-        if (threadIdx.x == 0) {
-            soff[d.S] = run;
-            *tot = run;
-        }
-        __syncthreads();
+    const int s = blockIdx.x, list = blockIdx.y;
+    const int* cnt = w.bcnt + list * d.N;
+    int* boff = w.boff + list * d.N;
+    int* scnt = list == 0 ? w.pair_cnt : w.prox_cnt;
+    const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
+    int local = 0;
+    for (int base = b0; base < b1; base += kBlock) {
+        const int b = base + threadIdx.x;
+        const int v = b < b1 ? cnt[b] : 0;
+        int ex;
+        const int t = wg_exclusive_scan(v, sh, &ex);
+        if (b < b1) boff[b] = local + ex;
+        local += t;
     }
+    if (threadIdx.x == 0) scnt[s] = local;
 }
+
+__global__ void __launch_bounds__(kBlock) k_scan_counts(const int* cnt, int* off, int* total, int S);
 
 void launch_broad_phase_step(const Dev& d, const Work& w, hipStream_t st) {
     if (d.N == 0) return;
     const dim3 grid((d.N + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_bp_step, grid, dim3(kBlock), 0, st, d, w, 0);
-    hipLaunchKernelGGL(k_bp_scan, dim3(1), dim3(kBlock), 0, st, d, w);
+    hipLaunchKernelGGL(k_bp_scan, dim3(d.S, 2), dim3(kBlock), 0, st, d, w);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, st, w.pair_cnt, w.pair_off, w.n_pairs_total, d.S);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, st, w.prox_cnt, w.prox_off, w.n_prox_total, d.S);
     hipLaunchKernelGGL(k_bp_step, grid, dim3(kBlock), 0, st, d, w, 1);
 }
 

@@ -43,6 +43,9 @@ struct SceneParams {
     double evals;  // narrow-phase candidate evaluations in this step
     double n_fv, n_ee;  // FV / EE candidates built this step
     double evals_total; // candidate evaluations since upload
+    // Newton work counters of the last step
+    int ls_trials, solves_direct, solves_pcg, pcg_iters;
+    int comp_max, pad2, pad3, pad4;
 };
 
 struct Dev {

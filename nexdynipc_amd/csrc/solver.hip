@@ -10,6 +10,7 @@
 namespace nd {
 
 constexpr int kBlk = 256;
+constexpr int kTermLds = 8192;  // doubles of LDS for the ordered sums of the Newton kernel
 constexpr double kG = -9.81;  // gravity (0, -9.81, 0), IPCSolver.cpp:142
 
 // ------------------------------------------------------------------ potentials
@@ -338,109 +339,99 @@ __device__ double objective_grad(const Dev& d, const Work& w, const ScnCtx& sc, 
 // ------------------------------------------------------------------ ordered sums
 // The reference accumulates the objective and the line-search slopes in plain loops.
 // A parallel tree sum rounds differently, and the Armijo test and the ALM stopping test
-// compare these values in the last bits, so they are summed left to right here: the 64
-// lanes of a wavefront compute 64 consecutive terms, then every lane adds them in lane
-// order, reading lane k's term with v_readlane (no memory round trip). Independent
-// accumulators run on different wavefronts of the work-group at the same time.
+// compare these values in the last bits, so they are summed left to right here: all
+// lanes compute a window of terms into LDS, then one lane per accumulator adds them in
+// order (LDS reads pipelined ahead of the dependent adds), one wavefront per
+// accumulator so that independent accumulators run side by side.
 static_assert(kBlk == 256, "ordered sums assume four wavefronts per work-group");
 
-__device__ __forceinline__ double lane_value(double v, int k) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, k);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+__device__ __forceinline__ double lds_serial_sum(double acc, const double* t, int n) {
+    int k = 0;
+    for (; k + 8 <= n; k += 8) {
+        const double a0 = t[k], a1 = t[k + 1], a2 = t[k + 2], a3 = t[k + 3];
+        const double a4 = t[k + 4], a5 = t[k + 5], a6 = t[k + 6], a7 = t[k + 7];
+        acc += a0; acc += a1; acc += a2; acc += a3;
+        acc += a4; acc += a5; acc += a6; acc += a7;
+    }
+    for (; k < n; ++k) acc += t[k];
+    return acc;
 }
 
 // Objective value (IPCSolver.cpp:144-160): inertia, h^2 gravity, joints, h^2 barrier,
 // h^2 friction, each accumulator in the reference's loop order. Skipped elements add
-// +0.0, which leaves a sum started at +0.0 unchanged.
+// +0.0, which leaves a sum started at +0.0 unchanged. T: kTermLds doubles of LDS.
 __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* red,
-                                double* out_val, double* out_cval) {
+                                double* out_val, double* out_cval, double* T) {
+    constexpr int WB = 1024;  // bodies / joints / contacts per window
+    constexpr int OI = 0, OG = 2 * WB, OJ = 3 * WB, OB = 4 * WB, OF = 5 * WB;
+    static_assert(6 * WB <= kTermLds, "term windows exceed the LDS buffer");
     const double* XH = w.xhat;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (wave == 0) {  // InertiaForm::value: val += m|dp|^2/2; val += dt . (I dt/2)
-        double acc = 0.0;
-        for (int base = 0; base < sc.n; base += 64) {
-            const int li = base + lane;
-            double t0 = 0.0, t1 = 0.0;
+    const int nj = sc.j1 - sc.j0;
+    int rounds = (sc.n + WB - 1) / WB;
+    rounds = max(rounds, (nj + WB - 1) / WB);
+    rounds = max(rounds, (sc.nc + WB - 1) / WB);
+    double acc0 = 0.0, acc1 = 0.0;  // this lane's accumulators (lanes 0, 64, 128, 192)
+    for (int r = 0; r < rounds; ++r) {
+        const int b_lo = r * WB;
+        for (int t = threadIdx.x; t < WB; t += blockDim.x) {
+            const int li = b_lo + t;
             if (li < sc.n) {
                 const int b = sc.b0 + li;
                 const double m = d.mass[b];
                 const double* I = d.inertia + 9 * b;
                 const V3 dp = xpos(X, b) - xpos(XH, b);
                 const V3 dt = xrot(X, b) - xrot(XH, b);
-                const V3 r = 0.5 * dt;
-                const double s0 = r.x * I[0] + r.y * I[3] + r.z * I[6];
-                const double s1 = r.x * I[1] + r.y * I[4] + r.z * I[7];
-                const double s2 = r.x * I[2] + r.y * I[5] + r.z * I[8];
-                t0 = 0.5 * m * sqnorm(dp);
-                t1 = s0 * dt.x + s1 * dt.y + s2 * dt.z;
+                const V3 rr = 0.5 * dt;
+                const double s0 = rr.x * I[0] + rr.y * I[3] + rr.z * I[6];
+                const double s1 = rr.x * I[1] + rr.y * I[4] + rr.z * I[7];
+                const double s2 = rr.x * I[2] + rr.y * I[5] + rr.z * I[8];
+                T[OI + 2 * t] = 0.5 * m * sqnorm(dp);               // InertiaForm::value
+                T[OI + 2 * t + 1] = s0 * dt.x + s1 * dt.y + s2 * dt.z;
+                // GravityForm::value: gv -= m (g . x), and x - y == x + (-y)
+                T[OG + t] = -(m * (0.0 * X[6 * b] + kG * X[6 * b + 1] + 0.0 * X[6 * b + 2]));
             }
-            const int cnt = min(64, sc.n - base);
-            for (int k = 0; k < cnt; ++k) {
-                acc += lane_value(t0, k);
-                acc += lane_value(t1, k);
-            }
-        }
-        if (lane == 0) red[0] = acc;
-    } else if (wave == 1) {  // GravityForm::value: gv -= m (g . x)  (x - y == x + (-y))
-        double acc = 0.0;
-        for (int base = 0; base < sc.n; base += 64) {
-            const int li = base + lane;
-            double t = 0.0;
-            if (li < sc.n) {
-                const int b = sc.b0 + li;
-                t = -(d.mass[b] * (0.0 * X[6 * b] + kG * X[6 * b + 1] + 0.0 * X[6 * b + 2]));
-            }
-            const int cnt = min(64, sc.n - base);
-            for (int k = 0; k < cnt; ++k) acc += lane_value(t, k);
-        }
-        if (lane == 0) red[1] = acc;
-    } else if (wave == 2) {  // ConstraintForm::value: sum of lambda.C + mu/2 |C|^2
-        double acc = 0.0;
-        const int nj = sc.j1 - sc.j0;
-        for (int base = 0; base < nj; base += 64) {
-            double t = 0.0;
-            if (base + lane < nj) {
-                const int j = sc.j0 + base + lane;
+            if (li < nj) {  // ConstraintForm::value: lambda.C + mu/2 |C|^2 per joint
+                const int j = sc.j0 + li;
                 JEval e;
                 joint_eval(d, j, X, e);
                 double t1 = 0.0, sq = 0.0;
                 for (int k = 0; k < 6; ++k) t1 += d.jlambda[6 * j + k] * e.C[k];
                 for (int k = 0; k < 6; ++k) sq += e.C[k] * e.C[k];
-                t = t1 + 0.5 * d.jmu[j] * sq;
+                T[OJ + t] = t1 + 0.5 * d.jmu[j] * sq;
             }
-            const int cnt = min(64, nj - base);
-            for (int k = 0; k < cnt; ++k) acc += lane_value(t, k);
-        }
-        if (lane == 0) red[2] = acc;
-    } else {  // DistanceBarrierForm::value and FrictionForm::value, two chains
-        double ab = 0.0, af = 0.0;
-        for (int base = 0; base < sc.nc; base += 64) {
-            const int i = base + lane;
-            double tb = 0.0, tf = 0.0;
-            if (i < sc.nc) {
-                const long long c = sc.cb + i;
-                if (sc.has_barrier) tb = bar_v(contact_dist(w, c, X), sc.dhat, sc.kappa);
-                if (sc.has_friction && i < sc.nfc) {
-                    const double N = w.nf[(long long)sc.s * kMaxContacts + i];
+            if (li < sc.nc) {  // DistanceBarrierForm::value, FrictionForm::value
+                const long long c = sc.cb + li;
+                T[OB + t] = sc.has_barrier ? bar_v(contact_dist(w, c, X), sc.dhat, sc.kappa) : 0.0;
+                double fv = 0.0;
+                if (sc.has_friction && li < sc.nfc) {
+                    const double N = w.nf[(long long)sc.s * kMaxContacts + li];
                     if (N > 0.0) {
                         FKin k;
                         fric_kin(d, w, c, X, sc.fdt, k);
-                        tf = fric_v(k.u, N, sc.fmu, sc.feps);
+                        fv = fric_v(k.u, N, sc.fmu, sc.feps);
                     }
                 }
-            }
-            const int cnt = min(64, sc.nc - base);
-            for (int k = 0; k < cnt; ++k) {
-                ab += lane_value(tb, k);
-                af += lane_value(tf, k);
+                T[OF + t] = fv;
             }
         }
-        if (lane == 0) {
-            red[3] = ab;
-            red[4] = af;
+        __syncthreads();
+        const int nb = min(WB, max(0, sc.n - b_lo));
+        if (threadIdx.x == 0) acc0 = lds_serial_sum(acc0, T + OI, 2 * nb);
+        else if (threadIdx.x == 64) acc0 = lds_serial_sum(acc0, T + OG, nb);
+        else if (threadIdx.x == 128) acc0 = lds_serial_sum(acc0, T + OJ, min(WB, max(0, nj - b_lo)));
+        else if (threadIdx.x == 192) {
+            const int ncw = min(WB, max(0, sc.nc - b_lo));
+            acc0 = lds_serial_sum(acc0, T + OB, ncw);
+            acc1 = lds_serial_sum(acc1, T + OF, ncw);
         }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) red[0] = acc0;
+    if (threadIdx.x == 64) red[1] = acc0;
+    if (threadIdx.x == 128) red[2] = acc0;
+    if (threadIdx.x == 192) {
+        red[3] = acc0;
+        red[4] = acc1;
     }
     __syncthreads();
     const double vin = red[0], vg = red[1], vj = red[2], vb = red[3], vf = red[4];
@@ -455,21 +446,24 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
     *out_cval = cval;
 }
 
-// (sum_i a0[i] b[i], sum_i a1[i] b[i]) left to right, on wavefronts 0 and 1.
+// (sum_i a0[i] b[i], sum_i a1[i] b[i]) left to right; T: kTermLds doubles of LDS.
 __device__ void serial_dot2(const double* a0, const double* a1, const double* b, int n, double* red, double* o0,
-                            double* o1) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (wave < 2) {
-        const double* a = wave ? a1 : a0;
-        double acc = 0.0;
-        for (int base = 0; base < n; base += 64) {
-            const int i = base + lane;
-            const double t = i < n ? a[i] * b[i] : 0.0;
-            const int cnt = min(64, n - base);
-            for (int k = 0; k < cnt; ++k) acc += lane_value(t, k);
+                            double* o1, double* T) {
+    constexpr int W = kTermLds / 2;
+    double acc = 0.0;
+    for (int lo = 0; lo < n; lo += W) {
+        const int cnt = min(W, n - lo);
+        for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+            T[t] = a0[lo + t] * b[lo + t];
+            T[W + t] = a1[lo + t] * b[lo + t];
         }
-        if (lane == 0) red[wave] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) acc = lds_serial_sum(acc, T, cnt);
+        else if (threadIdx.x == 64) acc = lds_serial_sum(acc, T + W, cnt);
+        __syncthreads();
     }
+    if (threadIdx.x == 0) red[0] = acc;
+    if (threadIdx.x == 64) red[1] = acc;
     __syncthreads();
     *o0 = red[0];
     *o1 = red[1];
@@ -482,16 +476,23 @@ __device__ void assemble_diag(const Dev& d, const Work& w, const ScnCtx& sc, con
     const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
     for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
         const int b = sc.b0 + li;
-        double* D = w.D + 36 * b;
+        // accumulated in registers (constant indices), in the reference's triplet order:
+        // inertia, joints, barrier contacts, friction contacts
+        double D[36];
+#pragma unroll
         for (int k = 0; k < 36; ++k) D[k] = 0.0;
         const double m = d.mass[b];
         D[0] = m; D[7] = m; D[14] = m;
+#pragma unroll
         for (int r = 0; r < 3; ++r)
+#pragma unroll
             for (int c = 0; c < 3; ++c) D[(3 + r) * 6 + 3 + c] = d.inertia[9 * b + 3 * r + c];
         for (int j = sc.j0; j < sc.j1; ++j) {
-            const double* jb = w.jblk + 108 * j;
-            if (d.ja[j] == b) for (int k = 0; k < 36; ++k) D[k] = D[k] + jb[k];
-            if (d.jb[j] == b) for (int k = 0; k < 36; ++k) D[k] = D[k] + jb[36 + k];
+            const int side = d.ja[j] == b ? 0 : (d.jb[j] == b ? 1 : -1);
+            if (side < 0) continue;
+            const double* jb = w.jblk + 108 * j + 36 * side;
+#pragma unroll
+            for (int k = 0; k < 36; ++k) D[k] = D[k] + jb[k];
         }
         for (int e = off[li]; e < off[li + 1]; ++e) {
             const int c = ent[e] >> 1;
@@ -500,17 +501,23 @@ __device__ void assemble_diag(const Dev& d, const Work& w, const ScnCtx& sc, con
             if (!(w.cactive[ce] & 1)) continue;
             const double* cd = w.cdbl + 7 * cc;
             const double n[3] = {cd[3], cd[4], cd[5]};
+            const double b2 = b2arr[c];
+#pragma unroll
             for (int r = 0; r < 3; ++r)
-                for (int q = 0; q < 3; ++q) D[r * 6 + q] = D[r * 6 + q] + (b2arr[c] * (n[r] * n[q])) * sc.h2;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) D[r * 6 + q] = D[r * 6 + q] + (b2 * (n[r] * n[q])) * sc.h2;
         }
         for (int e = off[li]; e < off[li + 1]; ++e) {
             const int c = ent[e] >> 1, side = ent[e] & 1;
-            const long long cc = sc.cb + c;
             const long long ce = (long long)sc.s * kMaxContacts + c;
             if (!(w.cactive[ce] & 2)) continue;
             const double* blk = w.cblk + 144 * ce + side * 36;
+#pragma unroll
             for (int k = 0; k < 36; ++k) D[k] = D[k] + blk[k];
         }
+        double* Dout = w.D + 36 * b;
+#pragma unroll
+        for (int k = 0; k < 36; ++k) Dout[k] = D[k];
     }
 }
 
@@ -627,7 +634,7 @@ __device__ double wg_absmax(const double* a, int n, double* red) {
 
 // Block-Jacobi PCG for H dx = rhs. Returns false when not converged.
 __device__ bool pcg_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* rhs,
-                          double* xo, double* red, int* ok_sh) {
+                          double* xo, double* red, int* ok_sh, int* iters = nullptr) {
     const int nd = sc.nd;
     double* r = w.r + 6 * sc.b0;
     double* z = w.z + 6 * sc.b0;
@@ -655,6 +662,7 @@ __device__ bool pcg_solve(const Dev& d, const Work& w, const ScnCtx& sc, const d
     __syncthreads();
     double rz = wg_dot(r, z, nd, red);
     for (int it = 0; it < 400; ++it) {
+        if (iters) *iters = it + 1;
         spmv(d, w, sc, b2arr, p, Ap);
         __syncthreads();
         const double pAp = wg_dot(p, Ap, nd, red);
@@ -690,57 +698,107 @@ __device__ bool pcg_solve(const Dev& d, const Work& w, const ScnCtx& sc, const d
 // factored column by column with the rows of a column in parallel (left-looking, same
 // per-entry order). Components above kDenseBodies bodies fall back to PCG.
 struct CompLds {
-    int par[kMaxSceneBodies + 1];  // union-find parent, then component offsets
-    int cid[kMaxSceneBodies];      // component of each body
-    int list[kMaxSceneBodies];     // bodies grouped by component, ascending
-    int ncomp, maxsize;
+    int lab[kMaxSceneBodies];   // component label of each body: its smallest member
+    int cnt[kMaxSceneBodies];   // component size, indexed by label
+    int off[kMaxSceneBodies];   // offset of a multi-body component in list, by label
+    int list[kMaxSceneBodies];  // members of the multi-body components, ascending in each
+    int tmp[kMaxSceneBodies];   // coupled bodies, ascending
+    int scan[kBlk];
+    int maxsize, changed, ncoupled, pad;
+};
+union NewtonLds {
+    CompLds comp;
+    double terms[kTermLds];
 };
 
-__device__ __forceinline__ int uf_find(int* par, int x) {
-    while (par[x] != x) {
-        par[x] = par[par[x]];
-        x = par[x];
+// Calls f(a, b) (scene-local body indices) for every coupling element in parallel:
+// active contacts and hinge joints.
+template <typename F>
+__device__ __forceinline__ void for_each_coupling(const Dev& d, const Work& w, const ScnCtx& sc, F&& f) {
+    for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+        if (!w.cactive[(long long)sc.s * kMaxContacts + i]) continue;
+        const int* ci = w.cint + 5 * (sc.cb + i);
+        f(ci[0] - sc.b0, ci[1] - sc.b0);
     }
-    return x;
+    for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x)
+        if (d.jtype[j] == 1) f(d.ja[j] - sc.b0, d.jb[j] - sc.b0);
 }
 
+// Connected components of the coupling graph, all lanes: min-label propagation with
+// root hooking and pointer jumping, then the members of every multi-body component
+// listed in ascending order. With no coupling element every body is a singleton.
 __device__ void build_components(const Dev& d, const Work& w, const ScnCtx& sc, CompLds& L) {
-    if (threadIdx.x == 0) {
-        const int n = sc.n;
-        for (int i = 0; i < n; ++i) L.par[i] = i;
-        auto unite = [&](int a, int b) {
-            a = uf_find(L.par, a);
-            b = uf_find(L.par, b);
-            if (a != b) { if (a < b) L.par[b] = a; else L.par[a] = b; }
-        };
-        for (int i = 0; i < sc.nc; ++i) {
-            if (!w.cactive[(long long)sc.s * kMaxContacts + i]) continue;
-            const int* ci = w.cint + 5 * (sc.cb + i);
-            unite(ci[0] - sc.b0, ci[1] - sc.b0);
-        }
-        for (int j = sc.j0; j < sc.j1; ++j)
-            if (d.jtype[j] == 1) unite(d.ja[j] - sc.b0, d.jb[j] - sc.b0);
-        for (int i = 0; i < n; ++i) L.list[i] = -1;
-        int nc = 0;
-        for (int i = 0; i < n; ++i) {
-            const int r = uf_find(L.par, i);
-            if (L.list[r] < 0) L.list[r] = nc++;
-            L.cid[i] = L.list[r];
-        }
-        // component offsets (par reused), then members in ascending body order
-        for (int c = 0; c <= nc; ++c) L.par[c] = 0;
-        for (int i = 0; i < n; ++i) L.par[L.cid[i] + 1]++;
-        int mx = 0;
-        for (int c = 0; c < nc; ++c) {
-            mx = L.par[c + 1] > mx ? L.par[c + 1] : mx;
-            L.par[c + 1] += L.par[c];
-        }
-        for (int i = 0; i < n; ++i) L.list[L.par[L.cid[i]]++] = i;
-        for (int c = nc; c >= 1; --c) L.par[c] = L.par[c - 1];
-        L.par[0] = 0;
-        L.ncomp = nc;
-        L.maxsize = mx;
+    const int n = sc.n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        L.lab[i] = i;
+        L.cnt[i] = 0;
     }
+    if (threadIdx.x == 0) {
+        L.maxsize = 1;
+        L.ncoupled = 0;
+    }
+    int any = 0;
+    for_each_coupling(d, w, sc, [&](int, int) { any = 1; });
+    if (!__syncthreads_or(any)) return;
+    while (true) {
+        if (threadIdx.x == 0) L.changed = 0;
+        __syncthreads();
+        for_each_coupling(d, w, sc, [&](int a, int b) {
+            const int la = L.lab[a], lb = L.lab[b];
+            if (la != lb) {
+                const int m = la < lb ? la : lb;
+                atomicMin(&L.lab[a], m);
+                atomicMin(&L.lab[b], m);
+                atomicMin(&L.lab[la], m);
+                atomicMin(&L.lab[lb], m);
+                L.changed = 1;
+            }
+        });
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            int l = L.lab[i];
+            while (L.lab[l] < l) l = L.lab[l];
+            L.lab[i] = l;
+        }
+        __syncthreads();
+        const int ch = L.changed;
+        __syncthreads();
+        if (!ch) break;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&L.cnt[L.lab[i]], 1);
+    __syncthreads();
+    int mx = 1;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) mx = max(mx, L.cnt[i]);
+    atomicMax(&L.maxsize, mx);
+    // offsets of the multi-body components (by root, ascending) and the coupled bodies in
+    // ascending order; then one lane places each coupled body after the earlier members
+    // of its component (the off[] of a root serves as its cursor meanwhile)
+    int run = 0, runc = 0;
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        const bool in = i < n;
+        const int root = in ? L.lab[i] : 0;
+        const int v = (in && root == i && L.cnt[i] > 1) ? L.cnt[i] : 0;
+        int ex;
+        const int t = wg_exclusive_scan(v, L.scan, &ex);
+        if (v) L.off[i] = run + ex;
+        run += t;
+        const int c = (in && L.cnt[root] > 1) ? 1 : 0;
+        const int tc = wg_exclusive_scan(c, L.scan, &ex);
+        if (c) L.tmp[runc + ex] = i;
+        runc += tc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        L.ncoupled = runc;
+        for (int p = 0; p < runc; ++p) {
+            const int i = L.tmp[p];
+            L.list[L.off[L.lab[i]]++] = i;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (L.lab[i] == i && L.cnt[i] > 1) L.off[i] -= L.cnt[i];
     __syncthreads();
 }
 
@@ -776,19 +834,18 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
     if (L.maxsize > kDenseBodies) return 2;
     if (threadIdx.x == 0) *ok_sh = 1;
     __syncthreads();
-    // singletons
+    // singletons (a body without coupling elements has cnt 0)
     for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
-        const int c = L.cid[li];
-        if (L.par[c + 1] - L.par[c] != 1) continue;
+        if (L.cnt[L.lab[li]] > 1) continue;
         if (!ldlt6_solve(w.D + 36 * (sc.b0 + li), rhs + 6 * li, xo + 6 * li)) atomicAnd(ok_sh, 0);
     }
     __syncthreads();
     if (!*ok_sh) return 1;
     double* M = w.dense + (long long)sc.s * kDenseRows * kDenseRows;
-    for (int c = 0; c < L.ncomp; ++c) {
-        const int c0 = L.par[c], c1 = L.par[c + 1];
-        const int mb = c1 - c0;
-        if (mb < 2) continue;
+    for (int c0 = 0; c0 < L.ncoupled;) {
+        const int c = L.lab[L.list[c0]];  // the component's label (its first member)
+        const int mb = L.cnt[c];
+        const int c1 = c0 + mb;
         const int m = 6 * mb;
         // local row of body b: 6 * (index of b in the component's ascending list)
         auto lpos = [&](int lb) {
@@ -813,7 +870,7 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
             for (int j = sc.j0; j < sc.j1; ++j) {
                 if (d.jtype[j] != 1) continue;
                 const int a = d.ja[j] - sc.b0, b = d.jb[j] - sc.b0;
-                if (L.cid[a] != c) continue;
+                if (L.lab[a] != c) continue;
                 const int pa = lpos(a), pb = lpos(b);
                 const double* jb = w.jblk + 108 * j + 72;  // rows a, cols b
                 for (int r = 0; r < 6; ++r)
@@ -828,7 +885,7 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
                     const int act = w.cactive[ce];
                     if (!(act & (pass == 0 ? 1 : 2))) continue;
                     const int a = w.cint[5 * cc] - sc.b0, b = w.cint[5 * cc + 1] - sc.b0;
-                    if (L.cid[a] != c) continue;
+                    if (L.lab[a] != c) continue;
                     const int pa = lpos(a), pb = lpos(b);
                     const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
                     if (pass == 0) {
@@ -882,6 +939,7 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
         }
         for (int t = threadIdx.x; t < m; t += blockDim.x) xo[6 * L.list[c0 + t / 6] + t % 6] = xs[t];
         __syncthreads();
+        c0 = c1;
     }
     return 0;
 }
@@ -890,7 +948,8 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
 __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     __shared__ double red[kBlk];
     __shared__ int ok_sh;
-    __shared__ CompLds comp;
+    __shared__ NewtonLds nl;
+    CompLds& comp = nl.comp;
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
     if (sp.alm_done) return;
@@ -916,7 +975,17 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) NG[i] = -G[i];
         __syncthreads();
         const int cs = component_solve(d, w, sc, b2arr, NG, DX, comp, &ok_sh);
-        const bool ok = cs == 0 ? true : cs == 2 ? pcg_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh) : false;
+        int pit = 0;
+        const bool ok = cs == 0 ? true : cs == 2 ? pcg_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh, &pit) : false;
+        if (threadIdx.x == 0) {
+            if (cs == 2) {
+                sp.solves_pcg += 1;
+                sp.pcg_iters += pit;
+            } else {
+                sp.solves_direct += 1;
+            }
+            sp.comp_max = max(sp.comp_max, comp.maxsize);
+        }
         __syncthreads();
         if (!ok) {
             if (threadIdx.x == 0) atomicOr(&sp.solve_fail, 1);
@@ -924,9 +993,9 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         }
         // line search (LineSearch.cpp:6-58), alpha0 = min(1, maxStep) = 1
         double fx, fc;
-        objective_value(d, w, sc, Xg, red, &fx, &fc);
+        objective_value(d, w, sc, Xg, red, &fx, &fc, nl.terms);
         double descent, cdesc;
-        serial_dot2(G, CG, DX, sc.nd, red, &descent, &cdesc);
+        serial_dot2(G, CG, DX, sc.nd, red, &descent, &cdesc, nl.terms);
         if (descent > 0) break;
         double alpha = 1.0;
         bool accepted = false;
@@ -934,7 +1003,8 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
             for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) XT[i] = X[i] + alpha * DX[i];
             __syncthreads();
             double fn, fcn;
-            objective_value(d, w, sc, XTg, red, &fn, &fcn);
+            objective_value(d, w, sc, XTg, red, &fn, &fcn, nl.terms);
+            if (threadIdx.x == 0) sp.ls_trials += 1;
             const bool cok = (cdesc <= 0.0) ? (fcn <= fc + 1e-4 * alpha * cdesc) : (fcn <= fc);
             if (fn <= fx + 1e-4 * alpha * descent && cok) { accepted = true; break; }
             alpha *= 0.5;
@@ -1128,6 +1198,11 @@ __global__ void k_step_begin(Dev d, Work w, const double* dt) {
     sp.solve_fail = 0;
     sp.alm_done = 0;
     sp.vv_needed = 0;
+    sp.ls_trials = 0;
+    sp.solves_direct = 0;
+    sp.solves_pcg = 0;
+    sp.pcg_iters = 0;
+    sp.comp_max = 0;
     w.flag[s] = 0;
 }
 
