@@ -450,7 +450,82 @@ __device__ void wg_bitonic_pairs(double* sk, int* si, int P) {
 }
 
 // Selects the top-k of s_orig[0..n) into out[0..k) ordered by (score, index).
-__device__ void wg_select_topk(const double* s_orig, double* sk, int* si, int n, int k, int* out) {
+// nth_element over the (score, index) pairs in LDS by the whole work-group, giving the
+// permutation of tk_nth_element (libstdc++'s introselect). The Hoare partition of a
+// pass is computed in parallel: with L_1 < L_2 < ... the positions whose score is not
+// below the pivot and R_1 > R_2 > ... those not above it (both in the array as the
+// pass starts), the scanners' k-th stops are exactly L_k and R_k for as long as
+// L_k < R_k, so the pass swaps (L_k, R_k) for k = 1..m, m the length of that prefix,
+// and returns min(L_{m+1}, R_m). Median selection, the final insertion sort and the
+// heap-select fallback run on one lane as in the serial restatement.
+__device__ void wg_nth_element_kv(double* sk, int* si, int n, int nth, int* lpos, int* rpos, int* sh) {
+    if (n == 0 || nth == n) return;
+    int first = 0, last = n;
+    int depth = tk_floor_log2(n) * 2;
+    while (last - first > 3) {
+        if (depth == 0) {
+            if (threadIdx.x == 0) {
+                KVStore st{sk, si};
+                tk_heap_select(st, first, nth + 1, last);
+                tk_swap(st, first, nth);
+            }
+            __syncthreads();
+            return;
+        }
+        --depth;
+        if (threadIdx.x == 0) {
+            KVStore st{sk, si};
+            tk_move_median_to_first(st, first, first + 1, first + (last - first) / 2, last - 1);
+        }
+        __syncthreads();
+        const double pv = sk[first];
+        const int a = first + 1, len = last - a;
+        const int per = (len + blockDim.x - 1) / blockDim.x;
+        const int i0 = a + threadIdx.x * per, i1 = min(last, i0 + per);
+        int cl = 0, cr = 0;
+        for (int i = i0; i < i1; ++i) {
+            cl += !(sk[i] < pv);
+            cr += !(pv < sk[i]);
+        }
+        int ex;
+        const int tot = wg_scan_shfl(cl | (cr << 16), sh, &ex);
+        const int nL = tot & 0xffff, nR = tot >> 16;
+        int kl = ex & 0xffff, kr = ex >> 16;
+        for (int i = i0; i < i1; ++i) {
+            if (!(sk[i] < pv)) lpos[kl++] = i;
+            if (!(pv < sk[i])) rpos[nR - 1 - kr++] = i;
+        }
+        const int mm = min(nL, nR);
+        if (threadIdx.x == 0) sh[16] = mm;
+        __syncthreads();
+        for (int q = threadIdx.x; q < mm; q += blockDim.x)
+            if (lpos[q] >= rpos[q]) atomicMin(&sh[16], q);
+        __syncthreads();
+        const int m = sh[16];
+        int cut = m < nL ? lpos[m] : last;
+        if (m >= 1 && rpos[m - 1] < cut) cut = rpos[m - 1];
+        for (int q = threadIdx.x; q < m; q += blockDim.x) {
+            const int x = lpos[q], y = rpos[q];
+            const double tk = sk[x];
+            const int ti = si[x];
+            sk[x] = sk[y];
+            si[x] = si[y];
+            sk[y] = tk;
+            si[y] = ti;
+        }
+        __syncthreads();
+        if (cut <= nth) first = cut;
+        else last = cut;
+    }
+    if (threadIdx.x == 0) {
+        KVStore st{sk, si};
+        tk_insertion_sort(st, first, last);
+    }
+    __syncthreads();
+}
+
+__device__ void wg_select_topk(const double* s_orig, double* sk, int* si, int n, int k, int* out, int* lpos,
+                               int* rpos, int* sh) {
     int P = 1;
     while (P < n) P <<= 1;
     for (int i = threadIdx.x; i < P; i += blockDim.x) {
@@ -475,9 +550,9 @@ __device__ void wg_select_topk(const double* s_orig, double* sk, int* si, int n,
         si[i] = i;
     }
     __syncthreads();
+    if (k < n) wg_nth_element_kv(sk, si, n, k, lpos, rpos, sh);
     if (threadIdx.x == 0) {
         KVStore st{sk, si};
-        if (k < n) tk_nth_element(st, k, n);
         tk_sort(st, k);
         for (int i = 0; i < k; ++i) out[i] = si[i];
     }
@@ -489,6 +564,7 @@ __global__ void __launch_bounds__(kBlock) k_topk(Dev d, Work w) {
     __shared__ double sk[kMaxPrims];
     __shared__ int si[kMaxPrims];
     __shared__ int outl[kTopK];
+    __shared__ int lpos[kMaxPrims], rpos[kMaxPrims], shs[32];
     const int total = *w.n_pairs_total;
     for (int task = blockIdx.x; task < total * 6; task += gridDim.x) {
         const int p = task / 6, list = task % 6;
@@ -522,7 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_topk(Dev d, Work w) {
         }
         __syncthreads();
         const int k = n < kTopK ? n : kTopK;
-        wg_select_topk(s_orig, sk, si, n, k, outl);
+        wg_select_topk(s_orig, sk, si, n, k, outl, lpos, rpos, shs);
         for (int i = threadIdx.x; i < k; i += blockDim.x) w.lists[(p * 6 + list) * kTopK + i] = outl[i];
         if (threadIdx.x == 0) w.lcnt[p * 6 + list] = k;
         __syncthreads();
@@ -542,6 +618,7 @@ __global__ void __launch_bounds__(kBlock) k_select_scores(const double* scores, 
     __shared__ double sk[kMaxPrims];
     __shared__ int si[kMaxPrims];
     __shared__ int outl[kMaxPrims];
+    __shared__ int lpos[kMaxPrims], rpos[kMaxPrims], shs[32];
     for (int i = threadIdx.x; i < n; i += blockDim.x) s_orig[i] = scores[i];
     __syncthreads();
     const int kk = k < n ? k : n;
@@ -549,7 +626,7 @@ __global__ void __launch_bounds__(kBlock) k_select_scores(const double* scores, 
         if (threadIdx.x == 0) *out_n = 0;
         return;
     }
-    wg_select_topk(s_orig, sk, si, n, kk, outl);
+    wg_select_topk(s_orig, sk, si, n, kk, outl, lpos, rpos, shs);
     for (int i = threadIdx.x; i < kk; i += blockDim.x) out[i] = outl[i];
     if (threadIdx.x == 0) *out_n = kk;
 }

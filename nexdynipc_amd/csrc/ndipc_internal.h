@@ -250,6 +250,29 @@ struct OpAddI { __device__ int operator()(int a, int b) const { return a + b; } 
 struct OpOrI { __device__ int operator()(int a, int b) const { return a | b; } };
 
 // Exclusive scan over blockDim.x ints in shared memory; returns the total.
+// Exclusive scan over the work-group with wavefront shuffles: two barriers. blockDim.x
+// must be a multiple of 64 and at most 64 * 64; sh holds one int per wavefront.
+__device__ __forceinline__ int wg_scan_shfl(int v, int* sh, int* out_excl) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) sh[wave] = incl;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int k = 0; k < nw; ++k) {
+        const int c = sh[k];
+        if (k < wave) before += c;
+        total += c;
+    }
+    __syncthreads();
+    *out_excl = before + incl - v;
+    return total;
+}
+
 __device__ __forceinline__ int wg_exclusive_scan(int v, int* sh, int* out_excl) {
     const int t = threadIdx.x;
     sh[t] = v;

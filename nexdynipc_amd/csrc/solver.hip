@@ -345,13 +345,26 @@ __device__ double objective_grad(const Dev& d, const Work& w, const ScnCtx& sc, 
 // accumulator so that independent accumulators run side by side.
 static_assert(kBlk == 256, "ordered sums assume four wavefronts per work-group");
 
+// acc + t[0] + t[1] + ... + t[n-1]; the reads of the next 8 terms are in flight while
+// the current 8 are added.
 __device__ __forceinline__ double lds_serial_sum(double acc, const double* t, int n) {
     int k = 0;
-    for (; k + 8 <= n; k += 8) {
-        const double a0 = t[k], a1 = t[k + 1], a2 = t[k + 2], a3 = t[k + 3];
-        const double a4 = t[k + 4], a5 = t[k + 5], a6 = t[k + 6], a7 = t[k + 7];
-        acc += a0; acc += a1; acc += a2; acc += a3;
-        acc += a4; acc += a5; acc += a6; acc += a7;
+    if (n >= 8) {
+        double a[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[q] = t[q];
+        for (; k + 16 <= n; k += 8) {
+            double b[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) b[q] = t[k + 8 + q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc += a[q];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) a[q] = b[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += a[q];
+        k += 8;
     }
     for (; k < n; ++k) acc += t[k];
     return acc;

@@ -221,6 +221,50 @@ ND_HD void tk_sort(const St& a, int n) {
     }
 }
 
+// The Hoare partition of tk_unguarded_partition(a, lo, hi, pivot) restated through the
+// positions it swaps: L = ascending positions in [lo, hi) whose element is not less than
+// the pivot, R = descending positions whose element is not greater (both taken before
+// any swap). The k-th swap exchanges L_k and R_k for as long as L_k < R_k; the returned
+// cut is min(L_{m+1}, R_m) for m swaps. Same permutation and cut as the scanning loop;
+// the work-group version in geom.hip computes L and R with a parallel scan.
+template <typename St>
+ND_HD int tk_partition_lr(const St& a, int lo, int hi, int pivot, int* lpos, int* rpos) {
+    const typename St::V pv = a.get(pivot);
+    int nL = 0, nR = 0;
+    for (int i = lo; i < hi; ++i)
+        if (!a.less(a.get(i), pv)) lpos[nL++] = i;
+    for (int i = hi - 1; i >= lo; --i)
+        if (!a.less(pv, a.get(i))) rpos[nR++] = i;
+    int m = 0;
+    while (m < nL && m < nR && lpos[m] < rpos[m]) ++m;
+    int cut = m < nL ? lpos[m] : hi;
+    if (m >= 1 && rpos[m - 1] < cut) cut = rpos[m - 1];
+    for (int q = 0; q < m; ++q) tk_swap(a, lpos[q], rpos[q]);
+    return cut;
+}
+
+// tk_nth_element with every partition done by tk_partition_lr.
+template <typename St>
+ND_HD void tk_nth_element_lr(const St& a, int nth, int n, int* lpos, int* rpos) {
+    if (n == 0 || nth == n) return;
+    int first = 0, last = n;
+    int depth = tk_floor_log2(n) * 2;
+    while (last - first > 3) {
+        if (depth == 0) {
+            tk_heap_select(a, first, nth + 1, last);
+            tk_swap(a, first, nth);
+            return;
+        }
+        --depth;
+        const int mid = first + (last - first) / 2;
+        tk_move_median_to_first(a, first, first + 1, mid, last - 1);
+        const int cut = tk_partition_lr(a, first + 1, last, first, lpos, rpos);
+        if (cut <= nth) first = cut;
+        else last = cut;
+    }
+    tk_insertion_sort(a, first, last);
+}
+
 // nth_element(a, a + nth, a + n) with the store's comparator.
 template <typename St>
 ND_HD void tk_nth_element(const St& a, int nth, int n) {

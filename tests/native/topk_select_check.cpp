@@ -26,16 +26,22 @@ int main() {
         }
         if (trial % 7 == 0) std::sort(s.begin(), s.end());
         if (trial % 11 == 0) std::sort(s.rbegin(), s.rend());
-        std::vector<int> a(n), b(n), c(n);
-        std::vector<double> ck(s);
+        std::vector<int> a(n), b(n), c(n), e(n), lp(n), rp(n);
+        std::vector<double> ck(s), ek(s);
         std::iota(a.begin(), a.end(), 0);
         std::iota(b.begin(), b.end(), 0);
         std::iota(c.begin(), c.end(), 0);
+        std::iota(e.begin(), e.end(), 0);
         auto cmp = [&](int x, int y) { return s[x] < s[y]; };
         if (k < n) {
             std::nth_element(a.begin(), a.begin() + k, a.end(), cmp);
             nd::tk_nth_element(nd::IdxStore{b.data(), s.data()}, k, n);
             nd::tk_nth_element(nd::KVStore{ck.data(), c.data()}, k, n);
+            nd::tk_nth_element_lr(nd::KVStore{ek.data(), e.data()}, k, n, lp.data(), rp.data());
+        }
+        if (a != e) {
+            std::printf("MISMATCH partition-by-positions trial %d n %d k %d\n", trial, n, k);
+            return 1;
         }
         if (a != b || a != c) {
             std::printf("MISMATCH nth_element trial %d n %d k %d\n", trial, n, k);
