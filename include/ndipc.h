@@ -114,19 +114,21 @@ int32_t ndipc_pair_count(ndipc_ctx* ctx, int32_t* out);
 
 /* ---- function-level entry points (current state) ------------------------------ */
 /* ContactAssembler::detectBodyPairs (ContactAssembler.cpp:156-178) via a device LBVH.
- * Pairs (i<j, scene-local) of scene `scene`, sorted by (i, j). */
+ * Pairs (i<j, scene-local) of scene `scene` in the reference's order: i ascending,
+ * then j descending (the order of BVH::query's stack walk). */
 int32_t ndipc_detect_body_pairs(ndipc_ctx* ctx, int32_t scene, double inflation, int32_t* out_pairs,
                                 int32_t cap, int32_t* n_pairs);
 /* ContactAssembler::estimateMinBodyPairDistance (ContactAssembler.cpp:180-212). */
 int32_t ndipc_min_body_pair_distance(ndipc_ctx* ctx, int32_t scene, double inflation, double* out);
 /* ContactAssembler::buildCandidateSet (ContactAssembler.cpp:214-343): the candidate
  * tuples of scene `scene` in the reference's struct layouts (VV 4, EV 5, EE 6, FV 4
- * ints, body ids and body-local primitive ids). counts[4] = vv, ev, ee, fv. The set
- * equals the reference's; the order is pair-major by (i, j), then by (score, index). */
+ * ints, body ids and body-local primitive ids). counts[4] = vv, ev, ee, fv. Same
+ * tuples in the same order as the reference built with libstdc++. */
 int32_t ndipc_build_candidates(ndipc_ctx* ctx, int32_t scene, double dhat, int32_t* counts, int32_t* vv,
                                int32_t* ev, int32_t* ee, int32_t* fv, int32_t cap);
-/* select_top_k_indices (ContactAssembler.cpp:67-88) on given scores: the selected
- * set of std::nth_element (libstdc++ introselect) sorted by (score, index). */
+/* select_top_k_indices (ContactAssembler.cpp:67-88) on given scores: the indices kept
+ * by std::nth_element, in the order std::sort leaves them (libstdc++'s introselect and
+ * introsort, ties included). n <= 2048. */
 int32_t ndipc_select_top_k(ndipc_ctx* ctx, const double* scores, int32_t n, int32_t k, int32_t* out);
 /* IPCSolver::detectContacts (x == NULL, IPCSolver.cpp:394-580) and
  * detectContactsAtState (x = Newton variable of scene, IPCSolver.cpp:582-806) on the

@@ -1270,6 +1270,19 @@ __device__ int ccd_toi(const CcdQuery& q, double earliest, double tol, double* t
     return found ? 1 : 0;
 }
 
+// Outcome of ccd_toi for a face-vertex query. The root box has u = v = [0, 1], so the
+// barycentric weight w = (1 - u) - v = [-1, 1] fails the domain check (lower bound
+// exactly -1 under outward rounding) and the search ends at its first pop without an
+// impact (TimeOfImpact.cpp:97-150 with IntervalRootFinder.h:20-114). What remains is
+// the NaN screen of the pre-check evaluation, which decides between 0 and -1. Same
+// result as the general search, without its explicit stack (no scratch traffic).
+__device__ int ccd_toi_face_vertex(const CcdQuery& q, double tol) {
+    bool nan = false;
+    const Iv xt[3] = {iv(0.0, tol), iv(0.0, 1.0), iv(0.0, 1.0)};
+    (void)ccd_f(q, xt, nan);
+    return nan ? -1 : 0;
+}
+
 __device__ CPose ld_cpose(const double* ps) {
     CPose p;
     p.p0 = v3(ps[0], ps[1], ps[2]);
@@ -1339,14 +1352,9 @@ __global__ void k_ccd_fv(Dev d, Work w, int gate_any) {
         q.a[0] = ld3(d.verts, d.v_off[vb] + vl);
         const int* f = d.faces + 3 * (d.f_off[fb] + fl);
         for (int m = 0; m < 3; ++m) q.b[m] = ld3(d.verts, d.v_off[fb] + f[m]);
-        double toi = 1.0 / 0.0;
-        int pops = 0;
-        const int r = ccd_toi(q, 1.0, 1e-8, &toi, &pops);
-        if (r == 1 && toi < 1.0) {
-            unsigned long long* tp = reinterpret_cast<unsigned long long*>(&sp.toi);
-            atomicMin(tp, __double_as_longlong(toi));
-        }
-        if (r < 0) atomicOr(&sp.solve_fail, 2);
+        // face-vertex queries never report an impact (ccd_toi_face_vertex); a NaN in
+        // the pre-check flags the step
+        if (ccd_toi_face_vertex(q, 1e-8) < 0) atomicOr(&sp.solve_fail, 2);
     }
 }
 
