@@ -1,13 +1,17 @@
 """Benchmark: IPCSolver steps/s and narrow-phase contact pairs/s on one MI355X.
 
-Workload (default): 512 rigid icospheres (subdivision 3, 642 verts / 1280 faces,
-r ~ U[0.1, 0.3] m, density 1000) falling in an open-top container of 5 static boxes,
-dt 5e-3, dhat 1e-3, mu 0.4, FP64 (scenes.config2). The scene is uploaded once; every
-step runs entirely on the device (candidate build, stiffness control, CCD gate, narrow
-phase, Newton/ALM with PCG) with state resident in HBM.
+Workload (default, the metric's single-scene configuration): 512 rigid icospheres
+(subdivision 3, 642 verts / 1280 faces, r ~ U[0.1, 0.3] m, density 1000) falling in an
+open-top container of 5 static boxes, dt 5e-3, dhat 1e-3, mu 0.4, FP64
+(scenes.config2). The scene is uploaded once; every step runs entirely on the device
+(candidate build, stiffness control, CCD gate, narrow phase, Newton/ALM) with state
+resident in HBM. `value` is steps/s of that scene.
 
-Prints one JSON line. With --batch the 256-scene batch (scenes.config3) is measured
-instead. --gpus above 1 is not supported: this build runs on one GPU.
+The same JSON line carries the 256-scene batch (scenes.config3): aggregate scene-steps/s
+of one batched engine, and the speedup over running the same scenes one by one through
+the plain single-scene path (a bounded sample of scenes, steps/s summed over scenes).
+
+Prints one JSON line. --gpus above 1 is not supported: this build runs on one GPU.
 """
 import argparse
 import json
@@ -23,16 +27,18 @@ sys.path.insert(0, ROOT)
 METRIC = "steps/s & contact pairs/s: plain vs 256-scene batch; adjoint grads/s (FD-checked)"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+TIMED = ("newton", "scene_min_flag", "topk", "np_count", "ccd_fv", "bp_step", "lbvh_build", "alm_post")
 
-# Algorithmic cost of one narrow-phase candidate (reference formulas, sqrt = div = 1):
-# FV plane test 34 flops (the barycentric test, 53 more, only runs after it passes);
-# EE segment-segment distance 67 flops.
+# Narrow phase, per candidate (reference formulas, sqrt = div = 1 flop): the FV plane
+# test is 34 flops (the barycentric test, 53 more, only runs after it passes); the EE
+# segment-segment distance is 67 flops.
 FLOPS_FV, FLOPS_EE = 34.0, 67.0
-# Algorithmic HBM bytes per close pair per narrow-phase pass: six 32-entry index
-# lists (768 B), the gathered body-frame vertices of the selected primitives
-# (32 + 32 + 2*32 + 2*32 + 3*32 + 3*32 = 384 vertices * 24 B), edge/face index
-# records (2 * 32 * 8 + 2 * 32 * 12 B), two poses (2 * 56 B) and 48 hit words (384 B).
-BYTES_PER_PAIR = 768 + 384 * 24 + 2 * 32 * 8 + 2 * 32 * 12 + 112 + 384
+# Newton kernel, compulsory HBM bytes of one Newton iteration / one line-search trial
+# (IPCSolver.cpp:137-262 forms at x): per body x, x_hat (6 doubles each), mass and
+# body-frame inertia (10 doubles), the step dx (6 doubles) = 224 B; per contact its
+# record (5 ints + 7 doubles) and lagged normal force = 84 B; per joint its data,
+# multipliers and stiffness (16 + 6 + 1 doubles) = 184 B.
+NEWTON_B_BODY, NEWTON_B_CONTACT, NEWTON_B_JOINT = 224.0, 84.0, 184.0
 
 
 def cpu_baseline(sc, stride, steps=1):
@@ -48,12 +54,83 @@ def cpu_baseline(sc, stride, steps=1):
     return steps / est, t_step
 
 
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_c2.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get("k_" + kernel)
+    if not k:
+        return None
+    return 1024.0 * (k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"])
+
+
+def run_plain(Engine, torch, sc, steps, warmup):
+    e = Engine(0)
+    e.upload(sc)
+    e.step(warmup)
+    torch.cuda.synchronize()
+    ev0 = np.array(e.eval_counter())
+    c0 = e.solver_counters()
+    e.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.stats()  # synchronises the context stream
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    ev1 = np.array(e.eval_counter())
+    c1 = e.solver_counters()
+    stats = e.stats()
+    kern = {name: e.timing_get(name) for name in TIMED}
+    e.timing_enable(False)
+    pairs = e.pair_count()
+    e.close()
+    return dict(T=T, evals=float(np.sum(ev1 - ev0)), stats=stats, kern=kern, pairs=pairs,
+                newton_iters=float(np.sum(c1[:, 6] - c0[:, 6])), trials=float(np.sum(c1[:, 7] - c0[:, 7])))
+
+
+def roofline_for(name, r, sc):
+    """Roofline of kernel `name` over the timed region: algorithmic work / total time."""
+    avg_ms, cnt = r["kern"][name]
+    t = avg_ms * cnt * 1e-3
+    if t <= 0:
+        return None
+    stats = r["stats"]
+    if name == "np_count":
+        n_fv, n_ee = float(np.sum(stats[:, 3])), float(np.sum(stats[:, 4]))
+        f_fv = n_fv / max(n_fv + n_ee, 1.0)
+        flops = r["evals"] * (FLOPS_FV * f_fv + FLOPS_EE * (1.0 - f_fv))
+        achieved = flops / t / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS}
+    elif name == "newton":
+        contacts = float(np.sum(stats[:, 2]))
+        per = NEWTON_B_BODY * sc.n_bodies + NEWTON_B_CONTACT * contacts + NEWTON_B_JOINT * sc.n_joints
+        bytes_ = (r["newton_iters"] + r["trials"]) * per
+        achieved = bytes_ / t / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS}
+    else:
+        return None
+    roof["traffic"] = load_traffic(name)
+    roof["kernel"] = "k_" + name
+    roof["avg_launch_ms"] = avg_ms
+    roof["launches"] = cnt
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", action="store_true", help="256-scene batch (config 3) instead of config 2")
+    ap.add_argument("--batch", action="store_true", help="measure only the 256-scene batch (config 3)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch / one-by-one comparison")
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--one-by-one", type=int, default=32, help="scenes sampled for the one-by-one comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=2000)
     ap.add_argument("--copy-probe", action="store_true",
@@ -78,57 +155,51 @@ def main():
     from nexdynipc_amd import scenes
     from nexdynipc_amd._lib import Engine
 
-    if args.batch:
-        sc = scenes.pack(scenes.config3())
-        workload = "c3: 256 scenes, log-U[4,128] bodies each (cube / icosphere s2 / 2x1x0.5 box), ground box"
-    else:
-        sc = scenes.make_config("c2")
-        workload = "c2: 512 icospheres (subdiv 3) + 5-box open container, single scene"
-    e = Engine(0)
-    e.upload(sc)
-    e.step(args.warmup)
-    torch.cuda.synchronize()
-    e.stats()
-    ev0 = np.array(e.eval_counter())
-    e.timing_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e.step(args.steps)
-    e.stats()  # synchronises the context stream
-    torch.cuda.synchronize()
-    T = time.perf_counter() - t0
-    ev1 = np.array(e.eval_counter())
-    stats = e.stats()
-    pairs_evaluated = float(np.sum(ev1 - ev0))
-    steps_per_s = args.steps * sc.n_scenes / T if args.batch else args.steps / T
-    kern = {}
-    for name in ("np_count", "newton", "topk", "scene_min_flag"):
-        avg, cnt = e.timing_get(name)
-        kern[name] = (avg, cnt)
-    e.timing_enable(False)
+    batch = None
+    if not args.no_batch:
+        specs = scenes.config3()
+        scb = scenes.pack(specs)
+        rb = run_plain(Engine, torch, scb, args.batch_steps, args.warmup)
+        batch_rate = args.batch_steps * scb.n_scenes / rb["T"]
+        # the same scenes one by one through the single-scene path: a seeded sample of
+        # scenes, each on its own engine, scene-steps/s = steps / (sum of wall times)
+        rng = np.random.default_rng(7)
+        pick = rng.choice(len(specs), size=min(args.one_by_one, len(specs)), replace=False)
+        t_sum, st_sum = 0.0, 0
+        for i in pick:
+            r1 = run_plain(Engine, torch, scenes.pack([specs[int(i)]]), args.batch_steps // 5, args.warmup)
+            t_sum += r1["T"]
+            st_sum += args.batch_steps // 5
+        one_rate = st_sum / t_sum
+        batch = {"workload": "c3: 256 scenes, log-U[4,128] bodies each (cube / icosphere s2 / 2x1x0.5 box), "
+                             "ground box, dt 1e-2", "steps": args.batch_steps,
+                 "scene_steps_per_s": batch_rate, "ms_per_step": 1e3 * rb["T"] / args.batch_steps,
+                 "contact_pairs_per_s": rb["evals"] / rb["T"],
+                 "one_by_one_scene_steps_per_s": one_rate,
+                 "one_by_one_sample": f"{len(pick)} of {len(specs)} scenes, {args.batch_steps // 5} steps each",
+                 "speedup_vs_one_by_one": batch_rate / one_rate}
+        if args.batch:
+            out = {"metric": METRIC, "value": batch_rate, "unit": "scene-steps/s", "n_gpus": 1,
+                   "steps": args.batch_steps, "warmup": args.warmup, "ms_per_step": batch["ms_per_step"],
+                   "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                   "data": "synthetic", "config": {"workload": batch["workload"], "scenes": int(scb.n_scenes),
+                                                    "bodies": int(scb.n_bodies)},
+                   "batch": batch, "roofline": roofline_for("newton", rb, scb), "cpu_baseline": None}
+            print(json.dumps(out))
+            return
 
-    # roofline of the narrow-phase candidate kernel
-    n_fv = float(np.sum(stats[:, 3]))
-    n_ee = float(np.sum(stats[:, 4]))
-    avg_ms, cnt = kern["np_count"]
-    n_pairs = e.pair_count()
-    flops = FLOPS_FV * n_fv + FLOPS_EE * n_ee
-    bytes_ = BYTES_PER_PAIR * n_pairs
-    roof = None
-    if avg_ms > 0:
-        t = avg_ms * 1e-3
-        fr_f = flops / t / (FP64_PEAK_TFLOPS * 1e12)
-        fr_b = bytes_ / t / (HBM_PEAK_GBS * 1e9)
-        if fr_f >= fr_b:
-            roof = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": fr_f, "traffic": None}
-        else:
-            roof = {"bound": "hbm", "achieved": bytes_ / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": fr_b, "traffic": None}
-        roof["kernel"] = "k_np_count"
-        roof["avg_launch_ms"] = avg_ms
+    sc = scenes.make_config("c2")
+    workload = "c2: 512 icospheres (subdiv 3) + 5-box open container, single scene"
+    r = run_plain(Engine, torch, sc, args.steps, args.warmup)
+    T = r["T"]
+    stats = r["stats"]
+    totals = {k: v[0] * v[1] for k, v in r["kern"].items()}
+    dominant = max(totals, key=totals.get)
+    roof = roofline_for(dominant, r, sc)
+    if roof is not None:
+        roof["share_of_timed_kernels"] = totals[dominant] / max(sum(totals.values()), 1e-30)
     cpu = None
-    if not args.no_cpu_baseline and not args.batch:
+    if not args.no_cpu_baseline:
         v, wall = cpu_baseline(sc, args.cpu_stride)
         cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
                "sample": f"1 step of c2 by the C++ oracle on 1 host core; computeSceneMinDistance's vertex loop "
@@ -136,7 +207,7 @@ def main():
                          f"(measured wall {wall:.1f} s)"}
     out = {
         "metric": METRIC,
-        "value": steps_per_s,
+        "value": args.steps / T,
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -149,16 +220,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "bodies": int(sc.n_bodies), "scenes": int(sc.n_scenes),
                    "verts": int(sc.total_verts()), "dt": float(sc.dt[0]), "dhat": float(sc.dhat[0])},
-        "contact_pairs_per_s": pairs_evaluated / T,
+        "contact_pairs_per_s": r["evals"] / T,
         "last_step": {"alm_iters": float(np.mean(stats[:, 0])), "newton_iters": float(np.mean(stats[:, 1])),
-                      "contacts": float(np.sum(stats[:, 2])), "fv_candidates": n_fv, "ee_candidates": n_ee,
-                      "close_pairs": int(n_pairs)},
-        "kernel_avg_ms": {k: v[0] for k, v in kern.items()},
+                      "contacts": float(np.sum(stats[:, 2])), "fv_candidates": float(np.sum(stats[:, 3])),
+                      "ee_candidates": float(np.sum(stats[:, 4])), "close_pairs": int(r["pairs"])},
+        "newton_iterations": r["newton_iters"],
+        "line_search_trials": r["trials"],
+        "kernel_total_ms": totals,
         "roofline": roof,
+        "roofline_narrow_phase": roofline_for("np_count", r, sc),
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    e.close()
 
 
 if __name__ == "__main__":

@@ -167,10 +167,11 @@ int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t n_blocks, const double* diag, in
                         int32_t* iters, double* rel_res);
 
 /* ---- diagnostics ---------------------------------------------------------------- */
-/* Newton work of the last step, 6 per scene: Newton iterations, line-search trials,
+/* Newton work, 8 per scene: of the last step Newton iterations, line-search trials,
  * direct (component LDL^T) solves, PCG solves, PCG iterations, largest coupled
- * component in bodies. No reference counterpart (NewtonSolver.cpp:10-95 only logs). */
-int32_t ndipc_solver_counters(ndipc_ctx* ctx, double* out6);
+ * component in bodies; since upload Newton iterations and line-search trials.
+ * No reference counterpart (NewtonSolver.cpp:10-95 only logs). */
+int32_t ndipc_solver_counters(ndipc_ctx* ctx, double* out8);
 /* Average device duration (ms) and count of launches of the named kernel since the
  * last reset, measured with HIP events on the context stream while enabled. */
 int32_t ndipc_timing_enable(ndipc_ctx* ctx, int32_t on);

@@ -237,8 +237,9 @@ class Engine:
 
     def solver_counters(self):
         """Per scene: newton iterations, line-search trials, direct solves, PCG solves,
-        PCG iterations, largest coupled component (bodies), for the last step."""
-        o = np.zeros((self.sc.n_scenes, 6))
+        PCG iterations, largest coupled component (bodies) of the last step; Newton
+        iterations and line-search trials since upload."""
+        o = np.zeros((self.sc.n_scenes, 8))
         self._ck(self.L.ndipc_solver_counters(self.ctx, ptr(o, f64p)), "solver_counters")
         return o
 

@@ -624,12 +624,14 @@ int32_t ndipc_last_stats(ndipc_ctx* ctx, double* out8) {
     return 0;
 }
 
-int32_t ndipc_solver_counters(ndipc_ctx* ctx, double* out6) {
+int32_t ndipc_solver_counters(ndipc_ctx* ctx, double* out8) {
     if (check_ctx(ctx)) return -1;
     if (sync_sp_from_dev(ctx)) return -1;
     for (int s = 0; s < ctx->S; ++s) {
         const SceneParams& q = ctx->sp[s];
-        double* o = out6 + 6 * s;
+        double* o = out8 + 8 * s;
+        o[6] = q.newton_total;
+        o[7] = q.trials_total;
         o[0] = q.newton_iters;
         o[1] = q.ls_trials;
         o[2] = q.solves_direct;

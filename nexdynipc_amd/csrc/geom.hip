@@ -159,7 +159,9 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, Work w) {
 }
 
 void launch_lbvh_build(const Dev& d, const Work& w, hipStream_t st) {
+    timing_begin("lbvh_build", st);
     hipLaunchKernelGGL(k_lbvh_build, dim3(d.S), dim3(kBlock), 0, st, d, w);
+    timing_end("lbvh_build", st);
 }
 
 // Traversal: leaf test = AABB::intersects of both boxes inflated by r
@@ -297,11 +299,15 @@ __global__ void __launch_bounds__(kBlock) k_scan_counts(const int* cnt, int* off
 void launch_broad_phase_step(const Dev& d, const Work& w, hipStream_t st) {
     if (d.N == 0) return;
     const dim3 grid((d.N + kBlock - 1) / kBlock);
+    timing_begin("bp_step", st);
     hipLaunchKernelGGL(k_bp_step, grid, dim3(kBlock), 0, st, d, w, 0);
+    timing_end("bp_step", st);
     hipLaunchKernelGGL(k_bp_scan, dim3(d.S, 2), dim3(kBlock), 0, st, d, w);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, st, w.pair_cnt, w.pair_off, w.n_pairs_total, d.S);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, st, w.prox_cnt, w.prox_off, w.n_prox_total, d.S);
+    timing_begin("bp_step", st);
     hipLaunchKernelGGL(k_bp_step, grid, dim3(kBlock), 0, st, d, w, 1);
+    timing_end("bp_step", st);
 }
 
 // mode 0: candidate pairs (gap <= r); mode 3: all overlapping pairs (detectBodyPairs).
@@ -1363,7 +1369,9 @@ void launch_ccd_earliest(const Dev& d, const Work& w, int max_pairs, hipStream_t
     long long g = (n + 255) / 256;
     if (g > 4096) g = 4096;
     if (g < 1) g = 1;
+    timing_begin("ccd_fv", st);
     hipLaunchKernelGGL(k_ccd_fv, dim3((unsigned)g), dim3(256), 0, st, d, w, 1);
+    timing_end("ccd_fv", st);
 }
 
 __global__ void k_round_ops(const double* a, const double* b, int n, double* out) {

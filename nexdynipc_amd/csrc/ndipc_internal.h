@@ -46,6 +46,7 @@ struct SceneParams {
     // Newton work counters of the last step
     int ls_trials, solves_direct, solves_pcg, pcg_iters;
     int comp_max, pad2, pad3, pad4;
+    double newton_total, trials_total;  // since upload
 };
 
 struct Dev {

@@ -991,6 +991,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         int pit = 0;
         const bool ok = cs == 0 ? true : cs == 2 ? pcg_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh, &pit) : false;
         if (threadIdx.x == 0) {
+            sp.newton_total += 1.0;
             if (cs == 2) {
                 sp.solves_pcg += 1;
                 sp.pcg_iters += pit;
@@ -1017,7 +1018,10 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
             __syncthreads();
             double fn, fcn;
             objective_value(d, w, sc, XTg, red, &fn, &fcn, nl.terms);
-            if (threadIdx.x == 0) sp.ls_trials += 1;
+            if (threadIdx.x == 0) {
+                sp.ls_trials += 1;
+                sp.trials_total += 1.0;
+            }
             const bool cok = (cdesc <= 0.0) ? (fcn <= fc + 1e-4 * alpha * cdesc) : (fcn <= fc);
             if (fn <= fx + 1e-4 * alpha * descent && cok) { accepted = true; break; }
             alpha *= 0.5;
@@ -1193,7 +1197,9 @@ __global__ void __launch_bounds__(kBlk) k_alm_post(Dev d, Work w) {
 }
 
 void launch_alm_post(const Dev& d, const Work& w, hipStream_t st) {
+    timing_begin("alm_post", st);
     hipLaunchKernelGGL(k_alm_post, dim3(d.S), dim3(kBlk), 0, st, d, w);
+    timing_end("alm_post", st);
 }
 
 // ------------------------------------------------------------------ step prologue
