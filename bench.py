@@ -170,14 +170,14 @@ def main():
             r1 = run_plain(Engine, torch, scenes.pack([specs[int(i)]]), args.batch_steps // 5, args.warmup)
             t_sum += r1["T"]
             st_sum += args.batch_steps // 5
-        one_rate = st_sum / t_sum
+        one_rate = st_sum / t_sum if t_sum > 0 else None
         batch = {"workload": "c3: 256 scenes, log-U[4,128] bodies each (cube / icosphere s2 / 2x1x0.5 box), "
                              "ground box, dt 1e-2", "steps": args.batch_steps,
                  "scene_steps_per_s": batch_rate, "ms_per_step": 1e3 * rb["T"] / args.batch_steps,
                  "contact_pairs_per_s": rb["evals"] / rb["T"],
                  "one_by_one_scene_steps_per_s": one_rate,
                  "one_by_one_sample": f"{len(pick)} of {len(specs)} scenes, {args.batch_steps // 5} steps each",
-                 "speedup_vs_one_by_one": batch_rate / one_rate}
+                 "speedup_vs_one_by_one": batch_rate / one_rate if one_rate else None}
         if args.batch:
             out = {"metric": METRIC, "value": batch_rate, "unit": "scene-steps/s", "n_gpus": 1,
                    "steps": args.batch_steps, "warmup": args.warmup, "ms_per_step": batch["ms_per_step"],
