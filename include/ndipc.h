@@ -69,6 +69,9 @@ typedef struct ndipc_params {
     double alm_hardening_trigger;
     double alm_hardening_ratio;
     double ccd_safety_factor;
+    double newton_tol;           /* NewtonSolver gradient tolerance (default 1e-6; <= 0: default) */
+    int32_t newton_max_iters;    /* NewtonSolver iteration cap (default 20; <= 0: default) */
+    int32_t reserved;
 } ndipc_params;
 
 /* ---- context and data ---------------------------------------------------------- */
@@ -97,6 +100,10 @@ int32_t ndipc_get_barrier(ndipc_ctx* ctx, double* kappa, double* dhat, int32_t* 
 int32_t ndipc_set_barrier(ndipc_ctx* ctx, const double* kappa, const double* dhat, const int32_t* initialized);
 /* Joint multipliers (6 per joint) and ALM stiffness (Joint.h:113-115). */
 int32_t ndipc_get_joint_state(ndipc_ctx* ctx, double* lambda6, double* mu);
+int32_t ndipc_set_joint_state(ndipc_ctx* ctx, const double* lambda6, const double* mu);
+/* RigidBody::setMass for every body (RigidBody.h:17-31); the body-frame inertia is
+ * left as uploaded. */
+int32_t ndipc_set_mass(ndipc_ctx* ctx, const double* mass);
 
 /* ---- whole step ---------------------------------------------------------------- */
 /* IPCSolver::step(World&, dt) for every scene (IPCSolver.cpp:808-1231), n_steps
@@ -165,6 +172,22 @@ int32_t ndipc_ccd_earliest_toi(ndipc_ctx* ctx, int32_t scene, double dt, double*
 int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t n_blocks, const double* diag, int32_t n_off, const int32_t* off_ij,
                         const double* off_blocks, const double* b, double tol, int32_t max_it, double* x,
                         int32_t* iters, double* rel_res);
+
+/* ---- adjoint (no reference counterpart: the reference has no differentiation) ------ */
+/* n_steps of ndipc_step, saving the full state (poses, velocities, accelerations, joint
+ * multipliers and stiffnesses, barrier state) every checkpoint_every steps for the
+ * reverse sweep. Leaves the scenes at the end state like ndipc_step. */
+int32_t ndipc_adjoint_forward(ndipc_ctx* ctx, const double* dt, int32_t n_steps, int32_t checkpoint_every);
+/* Reverse sweep of the last ndipc_adjoint_forward. In: the cotangent of the end state
+ * (dL/dp[3n], dL/dq[4n], dL/dv[3n], dL/dw[3n]; NULL = 0). Out (NULL = not wanted): dL/d
+ * of the start state p, q, v, w, of every body mass, and of every scene's friction
+ * coefficient. Each step is differentiated with the implicit-function theorem at its
+ * converged Newton point (the step's Hessian solved on the device), its contact set,
+ * lagged normal forces, barrier stiffness and joint multipliers held fixed; segments
+ * between checkpoints are recomputed forward before they are swept back. */
+int32_t ndipc_adjoint_backward(ndipc_ctx* ctx, const double* dl_dp, const double* dl_dq, const double* dl_dv,
+                               const double* dl_dw, double* g_p0, double* g_q0, double* g_v0, double* g_w0,
+                               double* g_mass, double* g_mu);
 
 /* ---- diagnostics ---------------------------------------------------------------- */
 /* Newton work, 8 per scene: of the last step Newton iterations, line-search trials,

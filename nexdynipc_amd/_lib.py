@@ -38,6 +38,7 @@ class Params(C.Structure):
         ("alm_max_iters", C.c_int32), ("alm_constraint_tol", C.c_double), ("alm_dual_tol", C.c_double),
         ("alm_contact_tol", C.c_double), ("alm_hardening_trigger", C.c_double),
         ("alm_hardening_ratio", C.c_double), ("ccd_safety_factor", C.c_double),
+        ("newton_tol", C.c_double), ("newton_max_iters", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -47,7 +48,8 @@ EXPORTS = [
     "ndipc_get_joint_state", "ndipc_step", "ndipc_last_stats", "ndipc_detect_body_pairs",
     "ndipc_min_body_pair_distance", "ndipc_build_candidates", "ndipc_select_top_k", "ndipc_narrow_phase",
     "ndipc_scene_min_distance_flag", "ndipc_barrier_eval", "ndipc_friction_eval", "ndipc_ccd_toi_batch",
-    "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters",
+    "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters", "ndipc_set_joint_state",
+    "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward",
 ]
 
 
@@ -106,6 +108,10 @@ def library():
         "ndipc_round_ops": ([vp, f64p, f64p, C.c_int32, f64p], C.c_int32),
         "ndipc_elemfn": ([vp, C.c_int32, f64p, f64p, C.c_int32, f64p], C.c_int32),
         "ndipc_solver_counters": ([vp, f64p], C.c_int32),
+        "ndipc_set_joint_state": ([vp, f64p, f64p], C.c_int32),
+        "ndipc_set_mass": ([vp, f64p], C.c_int32),
+        "ndipc_adjoint_forward": ([vp, f64p, C.c_int32, C.c_int32], C.c_int32),
+        "ndipc_adjoint_backward": ([vp] + [f64p] * 10, C.c_int32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -129,7 +135,8 @@ def make_desc(sc):
     return d
 
 
-def make_params(dhat=1e-3, mu=0.0, friction=True, ccd=True, adaptive=True, alm_max_iters=10, kappa=0.0):
+def make_params(dhat=1e-3, mu=0.0, friction=True, ccd=True, adaptive=True, alm_max_iters=10, kappa=0.0,
+                newton_tol=1e-6, newton_max_iters=20):
     p = Params()
     p.dhat = dhat
     p.kappa = kappa
@@ -144,6 +151,8 @@ def make_params(dhat=1e-3, mu=0.0, friction=True, ccd=True, adaptive=True, alm_m
     p.alm_hardening_trigger = 2.0
     p.alm_hardening_ratio = 2.0
     p.ccd_safety_factor = 0.8
+    p.newton_tol = newton_tol
+    p.newton_max_iters = newton_max_iters
     return p
 
 
@@ -224,6 +233,36 @@ class Engine:
         lam, mu = np.zeros(6 * max(J, 1)), np.zeros(max(J, 1))
         self._ck(self.L.ndipc_get_joint_state(self.ctx, ptr(lam, f64p), ptr(mu, f64p)), "get_joint_state")
         return lam[: 6 * J].reshape(-1, 6), mu[:J]
+
+    def set_joint_state(self, lam=None, mu=None):
+        lam = None if lam is None else np.ascontiguousarray(lam, dtype=np.float64)
+        mu = None if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        self._ck(self.L.ndipc_set_joint_state(self.ctx, ptr(lam, f64p), ptr(mu, f64p)), "set_joint_state")
+
+    def set_mass(self, mass):
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        assert m.shape == (self.sc.n_bodies,)
+        self._ck(self.L.ndipc_set_mass(self.ctx, ptr(m, f64p)), "set_mass")
+
+    def set_mu(self, mu):
+        mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (self.sc.n_scenes,)))
+        self._ck(self.L.ndipc_set_scene_mu(self.ctx, ptr(mu, f64p)), "set_scene_mu")
+
+    def adjoint_forward(self, n_steps, every=10, dt=None):
+        dt = self.sc.dt if dt is None else np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.sc.n_scenes,))
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        self._ck(self.L.ndipc_adjoint_forward(self.ctx, ptr(dt, f64p), int(n_steps), int(every)), "adjoint_forward")
+
+    def adjoint_backward(self, dl_dp=None, dl_dq=None, dl_dv=None, dl_dw=None):
+        """Gradients of a loss of the end state with respect to the start state, the
+        masses and the per-scene friction coefficients (dict of arrays)."""
+        n, S = self.sc.n_bodies, self.sc.n_scenes
+        ins = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (dl_dp, dl_dq, dl_dv, dl_dw)]
+        out = {"p0": np.zeros((n, 3)), "q0": np.zeros((n, 4)), "v0": np.zeros((n, 3)), "w0": np.zeros((n, 3)),
+               "mass": np.zeros(n), "mu": np.zeros(S)}
+        args = [ptr(a, f64p) for a in ins] + [ptr(out[k], f64p) for k in ("p0", "q0", "v0", "w0", "mass", "mu")]
+        self._ck(self.L.ndipc_adjoint_backward(self.ctx, *args), "adjoint_backward")
+        return out
 
     def step(self, n_steps=1, dt=None):
         dt = self.sc.dt if dt is None else np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.sc.n_scenes,))

@@ -46,6 +46,15 @@ struct ndipc_ctx {
     size_t scratch_bytes = 0;
     bool timing = false;
     std::map<std::string, TimingRec> tim;
+    double* mass_dev = nullptr;  // writable view of d.mass
+    // adjoint: checkpoints every `adj_every` steps of the last recorded forward run
+    nd::Adj adj{};
+    std::vector<void*> adj_allocs;
+    int adj_every = 0, adj_steps = 0, adj_nck = 0;
+    std::vector<double> adj_dt;
+    double* ck_state = nullptr;  // [nck][19N] p q v w al aa
+    double* ck_joint = nullptr;  // [nck][7J] lambda(6J) mu(J)
+    SceneParams* ck_sp = nullptr;  // [nck][S]
 };
 
 namespace {
@@ -77,7 +86,17 @@ T* dupload(ndipc_ctx* ctx, const T* h, size_t n) {
     return p;
 }
 
+void free_adjoint(ndipc_ctx* ctx) {
+    for (void* p : ctx->adj_allocs) hipFree(p);
+    ctx->adj_allocs.clear();
+    ctx->adj = nd::Adj{};
+    ctx->adj_every = ctx->adj_steps = ctx->adj_nck = 0;
+    ctx->ck_state = ctx->ck_joint = nullptr;
+    ctx->ck_sp = nullptr;
+}
+
 void free_all(ndipc_ctx* ctx) {
+    free_adjoint(ctx);
     for (void* p : ctx->allocs) hipFree(p);
     ctx->allocs.clear();
     ctx->uploaded = false;
@@ -130,6 +149,8 @@ void default_params(SceneParams& p) {
     p.enable_adaptive = 1;
     p.alm_max_iters = 10;
     p.barrier_initialized = 0;
+    p.newton_tol = 1e-6;
+    p.newton_max_iters = 20;
 }
 
 int check_ctx(ndipc_ctx* ctx) {
@@ -355,7 +376,8 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     d.edges = dupload(ctx, ctx->edges_host.data(), ctx->edges_host.size());
     d.lbox = dupload(ctx, lbox.data(), lbox.size());
     d.max_edge = dupload(ctx, max_edge.data(), N);
-    d.mass = dupload(ctx, dsc->mass, N);
+    ctx->mass_dev = dupload(ctx, dsc->mass, N);
+    d.mass = ctx->mass_dev;
     d.inertia = dupload(ctx, dsc->inertia, 9 * size_t(N));
     d.joint_off = dupload(ctx, joint_off.data(), S + 1);
     d.jtype = dupload(ctx, jt.data(), jt.size());
@@ -478,6 +500,8 @@ int32_t ndipc_configure(ndipc_ctx* ctx, const ndipc_params* p) {
         q.alm_trig = std::max(1.0, p->alm_hardening_trigger);
         q.alm_ratio = std::max(1.0, p->alm_hardening_ratio);
         q.ccd_safety = p->ccd_safety_factor;
+        q.newton_tol = p->newton_tol > 0.0 ? p->newton_tol : 1e-6;
+        q.newton_max_iters = p->newton_max_iters > 0 ? int(p->newton_max_iters) : 20;
         if (q.prune_margin > 4.0 * 1e-3 && q.enable_adaptive)
             return fail(ctx, "longest face edge too long for the exact scene-min-distance pruning");
     }
@@ -489,6 +513,23 @@ int32_t ndipc_set_scene_mu(ndipc_ctx* ctx, const double* mu) {
     if (sync_sp_from_dev(ctx)) return -1;
     for (int s = 0; s < ctx->S; ++s) ctx->sp[s].mu = mu[s];
     return sync_sp_to_dev(ctx);
+}
+
+int32_t ndipc_set_joint_state(ndipc_ctx* ctx, const double* lambda6, const double* mu) {
+    if (check_ctx(ctx)) return -1;
+    if (ctx->J == 0) return 0;
+    if (lambda6) HIPCHK(hipMemcpy(ctx->d.jlambda, lambda6, 8 * 6 * size_t(ctx->J), hipMemcpyHostToDevice));
+    if (mu) HIPCHK(hipMemcpy(ctx->d.jmu, mu, 8 * size_t(ctx->J), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int32_t ndipc_set_mass(ndipc_ctx* ctx, const double* mass) {
+    if (check_ctx(ctx)) return -1;
+    if (!mass) return fail(ctx, "null mass");
+    for (int b = 0; b < ctx->N; ++b)
+        if (!(mass[b] > 0.0)) return fail(ctx, "masses must be positive");
+    HIPCHK(hipMemcpy(ctx->mass_dev, mass, 8 * size_t(ctx->N), hipMemcpyHostToDevice));
+    return 0;
 }
 
 int32_t ndipc_set_state(ndipc_ctx* ctx, const double* p, const double* q, const double* v, const double* w,
@@ -548,20 +589,26 @@ int32_t ndipc_get_joint_state(ndipc_ctx* ctx, double* lambda6, double* mu) {
     return 0;
 }
 
-int32_t ndipc_step(ndipc_ctx* ctx, const double* dt, int32_t n_steps) {
-    if (check_ctx(ctx)) return -1;
-    if (!dt) return fail(ctx, "null dt");
-    for (int s = 0; s < ctx->S; ++s)
-        if (!(dt[s] > 0.0)) return fail(ctx, "dt must be positive");
-    HIPCHK(hipMemcpyAsync(ctx->d_dt, dt, sizeof(double) * ctx->S, hipMemcpyHostToDevice, ctx->stream));
+// The step sequence of IPCSolver::step for every scene, n_steps times. With `rec`, the
+// data the adjoint needs is recorded into slot k of step k: the step-start state, the
+// objective of every Newton solve (the last one stays), and the converged variable.
+static int run_steps(ndipc_ctx* ctx, int n_steps, const nd::Adj* rec) {
     const Dev& d = ctx->d;
     const Work& w = ctx->w;
     hipStream_t st = ctx->stream;
     const int mp = max_grid_pairs(ctx);
+    const size_t N = size_t(ctx->N);
     int alm_max = 1;
     for (auto& q : ctx->sp) alm_max = std::max(alm_max, q.alm_max_iters);
     g_timing_ctx = ctx;
     for (int k = 0; k < n_steps; ++k) {
+        if (rec) {
+            double* dst = rec->s_state + size_t(k) * 13 * N;
+            HIPCHK(hipMemcpyAsync(dst, d.p, 8 * 3 * N, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(dst + 3 * N, d.q, 8 * 4 * N, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(dst + 7 * N, d.v, 8 * 3 * N, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(dst + 10 * N, d.w, 8 * 3 * N, hipMemcpyDeviceToDevice, st));
+        }
         nd::launch_step_begin(d, w, ctx->d_dt, st);
         nd::launch_world_boxes(d, w, st);
         nd::launch_lbvh_build(d, w, st);
@@ -581,7 +628,9 @@ int32_t ndipc_step(ndipc_ctx* ctx, const double* dt, int32_t n_steps) {
         nd::launch_narrow_phase(d, w, mp, false, st);
         nd::launch_contacts_setup(d, w, st);
         for (int a = 0; a < alm_max; ++a) {
+            if (rec) nd::launch_adj_snapshot(d, w, *rec, k * rec->A + a, st);
             nd::launch_newton(d, w, st);
+            if (rec) nd::launch_adj_snapshot_post(d, w, *rec, k * rec->A + a, st);
             nd::launch_pose_from_state(d, w, w.x, st);
             nd::launch_narrow_phase(d, w, mp, true, st);
             nd::launch_alm_post(d, w, st);
@@ -590,6 +639,20 @@ int32_t ndipc_step(ndipc_ctx* ctx, const double* dt, int32_t n_steps) {
     }
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+static int upload_dt(ndipc_ctx* ctx, const double* dt) {
+    if (!dt) return fail(ctx, "null dt");
+    for (int s = 0; s < ctx->S; ++s)
+        if (!(dt[s] > 0.0)) return fail(ctx, "dt must be positive");
+    HIPCHK(hipMemcpyAsync(ctx->d_dt, dt, sizeof(double) * ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+
+int32_t ndipc_step(ndipc_ctx* ctx, const double* dt, int32_t n_steps) {
+    if (check_ctx(ctx)) return -1;
+    if (upload_dt(ctx, dt)) return -1;
+    return run_steps(ctx, n_steps, nullptr);
 }
 
 int32_t ndipc_pair_count(ndipc_ctx* ctx, int32_t* out) {
@@ -1012,6 +1075,182 @@ int32_t ndipc_elemfn(ndipc_ctx* ctx, int32_t fn, const double* x, const double* 
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, buf + 2 * size_t(n), 8 * size_t(n), hipMemcpyDeviceToHost));
     hipFree(buf);
+    return 0;
+}
+
+// ---------------------------------------------------------------- adjoint
+}  // extern "C"
+
+namespace {
+template <typename T>
+T* aalloc(ndipc_ctx* ctx, size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return nullptr;
+    ctx->adj_allocs.push_back(p);
+    return static_cast<T*>(p);
+}
+
+int save_checkpoint(ndipc_ctx* ctx, int c) {
+    const size_t N = size_t(ctx->N), J = size_t(std::max(ctx->J, 1));
+    hipStream_t st = ctx->stream;
+    double* s = ctx->ck_state + size_t(c) * 19 * N;
+    const Dev& d = ctx->d;
+    double* src[6] = {d.p, d.q, d.v, d.w, d.al, d.aa};
+    const size_t len[6] = {3 * N, 4 * N, 3 * N, 3 * N, 3 * N, 3 * N};
+    for (int k = 0; k < 6; ++k) {
+        const hipError_t e = hipMemcpyAsync(s, src[k], 8 * len[k], hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) {
+            char buf[256];
+            snprintf(buf, sizeof buf, "checkpoint copy %d: dst %p src %p bytes %zu base %p: %s", k, (void*)s,
+                     (void*)src[k], 8 * len[k], (void*)ctx->ck_state, hipGetErrorString(e));
+            return fail(ctx, buf);
+        }
+        s += len[k];
+    }
+    double* jst = ctx->ck_joint + size_t(c) * 7 * J;
+    HIPCHK(hipMemcpyAsync(jst, d.jlambda, 8 * 6 * J, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(jst + 6 * J, d.jmu, 8 * J, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->ck_sp + size_t(c) * ctx->S, d.sp, sizeof(SceneParams) * ctx->S,
+                          hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+int load_checkpoint(ndipc_ctx* ctx, int c) {
+    const size_t N = size_t(ctx->N), J = size_t(std::max(ctx->J, 1));
+    hipStream_t st = ctx->stream;
+    const double* s = ctx->ck_state + size_t(c) * 19 * N;
+    const Dev& d = ctx->d;
+    double* dst[6] = {d.p, d.q, d.v, d.w, d.al, d.aa};
+    const size_t len[6] = {3 * N, 4 * N, 3 * N, 3 * N, 3 * N, 3 * N};
+    for (int k = 0; k < 6; ++k) {
+        HIPCHK(hipMemcpyAsync(dst[k], s, 8 * len[k], hipMemcpyDeviceToDevice, st));
+        s += len[k];
+    }
+    const double* jst = ctx->ck_joint + size_t(c) * 7 * J;
+    HIPCHK(hipMemcpyAsync(d.jlambda, jst, 8 * 6 * J, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.jmu, jst + 6 * J, 8 * J, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.sp, ctx->ck_sp + size_t(c) * ctx->S, sizeof(SceneParams) * ctx->S,
+                          hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return sync_sp_from_dev(ctx);
+}
+}  // namespace
+
+extern "C" {
+
+int32_t ndipc_adjoint_forward(ndipc_ctx* ctx, const double* dt, int32_t n_steps, int32_t checkpoint_every) {
+    if (check_ctx(ctx)) return -1;
+    if (n_steps < 1 || checkpoint_every < 1) return fail(ctx, "n_steps and checkpoint_every must be positive");
+    if (upload_dt(ctx, dt)) return -1;
+    free_adjoint(ctx);
+    const size_t N = size_t(ctx->N), S = size_t(ctx->S), J = size_t(std::max(ctx->J, 1));
+    const int L = std::min(checkpoint_every, n_steps);
+    const int nseg = (n_steps + checkpoint_every - 1) / checkpoint_every;
+    ctx->adj_every = checkpoint_every;
+    ctx->adj_steps = n_steps;
+    ctx->adj_nck = nseg + 1;
+    ctx->adj_dt.assign(dt, dt + S);
+    ctx->ck_state = aalloc<double>(ctx, size_t(ctx->adj_nck) * 19 * N);
+    ctx->ck_joint = aalloc<double>(ctx, size_t(ctx->adj_nck) * 7 * J);
+    ctx->ck_sp = aalloc<SceneParams>(ctx, size_t(ctx->adj_nck) * S);
+    int alm_max = 1;
+    for (auto& q : ctx->sp) alm_max = std::max(alm_max, q.alm_max_iters);
+    nd::Adj& a = ctx->adj;
+    a.L = L;
+    a.A = alm_max;
+    const size_t LA = size_t(L) * alm_max;
+    a.s_state = aalloc<double>(ctx, size_t(L) * 13 * N);
+    a.s_x = aalloc<double>(ctx, LA * 6 * N);
+    a.s_sp = aalloc<SceneParams>(ctx, LA * S);
+    a.s_cint = aalloc<int>(ctx, LA * S * nd::kMaxContacts * 5);
+    a.s_cdbl = aalloc<double>(ctx, LA * S * nd::kMaxContacts * 7);
+    a.s_nf = aalloc<double>(ctx, LA * S * nd::kMaxContacts);
+    a.s_jl = aalloc<double>(ctx, LA * 6 * J);
+    a.s_jm = aalloc<double>(ctx, LA * J);
+    a.s_jm_post = aalloc<double>(ctx, LA * J);
+    a.ap = aalloc<double>(ctx, 3 * N); a.aq = aalloc<double>(ctx, 4 * N);
+    a.av = aalloc<double>(ctx, 3 * N); a.aw = aalloc<double>(ctx, 3 * N);
+    a.bp = aalloc<double>(ctx, 3 * N); a.bq = aalloc<double>(ctx, 4 * N);
+    a.bv = aalloc<double>(ctx, 3 * N); a.bw = aalloc<double>(ctx, 3 * N);
+    a.lbar = aalloc<double>(ctx, 6 * J);
+    a.g_mass = aalloc<double>(ctx, N);
+    a.g_mu = aalloc<double>(ctx, S);
+    a.xbar = aalloc<double>(ctx, 6 * N);
+    a.eta = aalloc<double>(ctx, 6 * N);
+    a.res = aalloc<double>(ctx, 6 * N);
+    a.tmp = aalloc<double>(ctx, 6 * N);
+    a.kry = aalloc<double>(ctx, 25 * 6 * N);
+    a.cvjp = aalloc<double>(ctx, S * nd::kMaxContacts * 15);
+    a.dcor_c = aalloc<double>(ctx, S * nd::kMaxContacts * 144);
+    a.dcor_j = aalloc<double>(ctx, 144 * J);
+    a.ye_c = aalloc<double>(ctx, S * nd::kMaxContacts * 12);
+    a.ye_j = aalloc<double>(ctx, 12 * J);
+    a.jvjp = aalloc<double>(ctx, 8 * J);
+    a.jxbar = aalloc<double>(ctx, 12 * J);
+    for (void* p : ctx->adj_allocs)
+        if (!p) {
+            free_adjoint(ctx);
+            return fail(ctx, "adjoint allocation failed");
+        }
+    for (int seg = 0; seg < nseg; ++seg) {
+        if (save_checkpoint(ctx, seg)) return -1;
+        const int len = std::min(checkpoint_every, n_steps - seg * checkpoint_every);
+        if (run_steps(ctx, len, nullptr)) return -1;
+    }
+    if (save_checkpoint(ctx, nseg)) return -1;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int32_t ndipc_adjoint_backward(ndipc_ctx* ctx, const double* dl_dp, const double* dl_dq, const double* dl_dv,
+                               const double* dl_dw, double* g_p0, double* g_q0, double* g_v0, double* g_w0,
+                               double* g_mass, double* g_mu) {
+    if (check_ctx(ctx)) return -1;
+    if (ctx->adj_nck == 0) return fail(ctx, "ndipc_adjoint_forward has not run");
+    const size_t N = size_t(ctx->N), S = size_t(ctx->S);
+    hipStream_t st = ctx->stream;
+    nd::Adj a = ctx->adj;
+    const double* in[4] = {dl_dp, dl_dq, dl_dv, dl_dw};
+    double* dst[4] = {a.ap, a.aq, a.av, a.aw};
+    const size_t len[4] = {3 * N, 4 * N, 3 * N, 3 * N};
+    for (int k = 0; k < 4; ++k) {
+        if (in[k]) HIPCHK(hipMemcpyAsync(dst[k], in[k], 8 * len[k], hipMemcpyHostToDevice, st));
+        else HIPCHK(hipMemsetAsync(dst[k], 0, 8 * len[k], st));
+    }
+    HIPCHK(hipMemsetAsync(a.g_mass, 0, 8 * N, st));
+    HIPCHK(hipMemsetAsync(a.g_mu, 0, 8 * S, st));
+    HIPCHK(hipMemsetAsync(a.lbar, 0, 8 * 6 * size_t(std::max(ctx->J, 1)), st));
+    if (upload_dt(ctx, ctx->adj_dt.data())) return -1;
+    const int E = ctx->adj_every, nseg = ctx->adj_nck - 1;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        if (load_checkpoint(ctx, seg)) return -1;
+        const int len = std::min(E, ctx->adj_steps - seg * E);
+        if (run_steps(ctx, len, &a)) return -1;
+        for (int t = len - 1; t >= 0; --t) {
+            nd::launch_adj_step(ctx->d, ctx->w, a, t, st);
+            std::swap(a.ap, a.bp);
+            std::swap(a.aq, a.bq);
+            std::swap(a.av, a.bv);
+            std::swap(a.aw, a.bw);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    double* outp[4] = {g_p0, g_q0, g_v0, g_w0};
+    const double* srcp[4] = {a.ap, a.aq, a.av, a.aw};
+    for (int k = 0; k < 4; ++k)
+        if (outp[k]) HIPCHK(hipMemcpyAsync(outp[k], srcp[k], 8 * len[k], hipMemcpyDeviceToHost, st));
+    if (g_mass) HIPCHK(hipMemcpyAsync(g_mass, a.g_mass, 8 * N, hipMemcpyDeviceToHost, st));
+    if (g_mu) HIPCHK(hipMemcpyAsync(g_mu, a.g_mu, 8 * S, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (sync_sp_from_dev(ctx)) return -1;
+    int bad = -1;
+    for (int s = 0; s < ctx->S; ++s)
+        if (ctx->sp[s].solve_fail & 4) bad = s;
+    // the adjoint sweep leaves the scene in its recorded state; return to the end state
+    if (load_checkpoint(ctx, nseg)) return -1;
+    if (bad >= 0) return fail(ctx, "adjoint: the transposed solve did not converge in scene " + std::to_string(bad));
     return 0;
 }
 

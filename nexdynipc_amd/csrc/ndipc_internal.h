@@ -47,6 +47,8 @@ struct SceneParams {
     int ls_trials, solves_direct, solves_pcg, pcg_iters;
     int comp_max, pad2, pad3, pad4;
     double newton_total, trials_total;  // since upload
+    double newton_tol;                  // NewtonSolver tolerance (NewtonSolver.cpp:10-95: 1e-6)
+    int newton_max_iters, pad5;         // NewtonSolver iteration cap (20)
 };
 
 struct Dev {
@@ -161,6 +163,44 @@ void launch_ccd_batch(int kind, int n, const double* pa, const double* pb, const
 void launch_round_ops(const double* a, const double* b, int n, double* out, hipStream_t st);
 void launch_elemfn(int fn, const double* x, const double* y, int n, double* out, hipStream_t st);
 void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, hipStream_t st);
+
+// Adjoint of a step (implicit-function theorem at every converged Newton point of the
+// step's ALM loop). Slot k*A + a holds ALM iteration a of step k of a checkpoint
+// segment: the objective of its Newton solve (scene parameters, contacts, lagged normal
+// forces, joint multipliers and stiffnesses), the converged variable and the hardened
+// stiffness used in the multiplier update; s_state holds the step-start state.
+struct Adj {
+    int L, A;               // steps per segment, ALM iterations per step
+    double* s_state;        // [L][13N] p(3N) q(4N) v(3N) w(3N)
+    double* s_x;            // [L*A][6N]
+    SceneParams* s_sp;      // [L*A][S] (alm_done = 1: the scene did not solve in that slot)
+    int* s_cint;            // [L*A][S*kMaxContacts*5]
+    double* s_cdbl;         // [L*A][S*kMaxContacts*7]
+    double* s_nf;           // [L*A][S*kMaxContacts]
+    double* s_jl;           // [L*A][6J] multipliers before the solve
+    double* s_jm;           // [L*A][J] stiffness of the solve
+    double* s_jm_post;      // [L*A][J] stiffness of the multiplier update after it
+    double *ap, *aq, *av, *aw;  // cotangent of the state after the step
+    double *bp, *bq, *bv, *bw;  // cotangent of the state before the step
+    double* lbar;           // [6J] cotangent of the joint multipliers (updated in place)
+    double* g_mass;         // [N]
+    double* g_mu;           // [S]
+    double* xbar;           // [6N] cotangent of the converged variable of a solve
+    double* eta;            // [6N] A^-T xbar
+    double* res;            // [6N] refinement residual / correction
+    double* tmp;            // [6N]
+    double* kry;            // [25][6N] Krylov basis of the transposed solve
+    double* cvjp;           // [S*kMaxContacts][15]
+    double* dcor_c;         // [S*kMaxContacts][144] exact minus assembled Jacobian, transposed
+    double* dcor_j;         // [J][144]
+    double* ye_c;           // [S*kMaxContacts][12]
+    double* ye_j;           // [J][12]
+    double* jvjp;           // [J][8] cotangent of the reference rotations
+    double* jxbar;          // [J][12] cotangent of x from the multiplier update
+};
+void launch_adj_snapshot(const Dev& d, const Work& w, const Adj& a, int slot, hipStream_t st);
+void launch_adj_snapshot_post(const Dev& d, const Work& w, const Adj& a, int slot, hipStream_t st);
+void launch_adj_step(const Dev& d, const Work& w, const Adj& a, int step, hipStream_t st);
 
 void launch_step_begin(const Dev& d, const Work& w, const double* dt, hipStream_t st);
 void launch_kappa(const Dev& d, const Work& w, hipStream_t st);

@@ -6,6 +6,7 @@
 // component of the coupling graph (block-Jacobi PCG for components above 64 bodies).
 // Scenes are independent, so a batch of scenes is one launch of each kernel.
 #include "ndipc_internal.h"
+#include "dual.h"
 
 namespace nd {
 
@@ -979,11 +980,12 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     int it = 0;
     const double* Xg = w.x;
     const double* XTg = w.xt;
-    for (; it < 20; ++it) {
+    const int maxit = sp.newton_max_iters;
+    for (; it < maxit; ++it) {
         eval_elements(d, w, sc, Xg, b2arr);
         const double ginf = objective_grad(d, w, sc, Xg, G, CG, red);
         __syncthreads();
-        if (ginf < 1e-6) { conv = 1; break; }
+        if (ginf < sp.newton_tol) { conv = 1; break; }
         assemble_diag(d, w, sc, b2arr);
         for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) NG[i] = -G[i];
         __syncthreads();
@@ -1052,7 +1054,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     dual = wg_reduce(dual, red, OpMax());
     if (threadIdx.x == 0) {
         sp.converged = conv;
-        sp.newton_iters += (it < 20 ? it : 19) + 1;  // NewtonSolver reports the last index
+        sp.newton_iters += (it < maxit ? it : maxit - 1) + 1;  // NewtonSolver reports the last index
         sp.viol = viol;
         sp.dual = dual;
     }
@@ -1525,6 +1527,541 @@ void launch_pcg(int nb, const double* diag, int noff, const int* offij, const do
                 double tol, int max_it, double* x, int* iters, double* rel, double* scratch, hipStream_t st) {
     hipLaunchKernelGGL(k_pcg, dim3(1), dim3(kBlk), 0, st, nb, diag, noff, offij, offb, b, tol, max_it, x, iters, rel,
                        scratch);
+}
+
+// ------------------------------------------------------------------ adjoint
+// Recording: before each Newton launch the objective of the solve (scenes whose ALM loop
+// has finished mark the slot unused), after it the converged variable and the hardened
+// joint stiffness the multiplier update used.
+__global__ void __launch_bounds__(kBlk) k_adj_snapshot(Dev d, Work w, Adj a, int slot) {
+    const int s = blockIdx.x;
+    const SceneParams& sp = d.sp[s];
+    const long long S = d.S;
+    const int J = d.joint_off[d.S];
+    if (sp.alm_done) {
+        if (threadIdx.x == 0) a.s_sp[slot * S + s].alm_done = 1;
+        return;
+    }
+    if (threadIdx.x == 0) a.s_sp[slot * S + s] = sp;
+    const int nc = sp.n_contacts;
+    const long long cb = (long long)s * kMaxRawContacts;
+    const long long so = ((long long)slot * S + s) * kMaxContacts;
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        for (int k = 0; k < 5; ++k) a.s_cint[5 * (so + i) + k] = w.cint[5 * (cb + i) + k];
+        for (int k = 0; k < 7; ++k) a.s_cdbl[7 * (so + i) + k] = w.cdbl[7 * (cb + i) + k];
+        a.s_nf[so + i] = w.nf[(long long)s * kMaxContacts + i];
+    }
+    for (int j = d.joint_off[s] + threadIdx.x; j < d.joint_off[s + 1]; j += blockDim.x) {
+        for (int k = 0; k < 6; ++k) a.s_jl[(long long)slot * 6 * J + 6 * j + k] = d.jlambda[6 * j + k];
+        a.s_jm[(long long)slot * J + j] = d.jmu[j];
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_adj_snapshot_post(Dev d, Work w, Adj a, int slot) {
+    const int s = blockIdx.x;
+    const long long S = d.S;
+    const int J = d.joint_off[d.S];
+    if (a.s_sp[slot * S + s].alm_done) return;
+    const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
+    for (int i = 6 * b0 + threadIdx.x; i < 6 * b1; i += blockDim.x) a.s_x[(long long)slot * 6 * d.N + i] = w.x[i];
+    for (int j = d.joint_off[s] + threadIdx.x; j < d.joint_off[s + 1]; j += blockDim.x)
+        a.s_jm_post[(long long)slot * J + j] = d.jmu[j];
+}
+
+void launch_adj_snapshot(const Dev& d, const Work& w, const Adj& a, int slot, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_snapshot, dim3(d.S), dim3(kBlk), 0, st, d, w, a, slot);
+}
+void launch_adj_snapshot_post(const Dev& d, const Work& w, const Adj& a, int slot, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_snapshot_post, dim3(d.S), dim3(kBlk), 0, st, d, w, a, slot);
+}
+
+// y = A^T v for the scene, A = dg/dx the exact Jacobian of the Newton gradient at the
+// converged point: the assembled Hessian H (symmetric) plus per-element corrections
+// (A_e^T - H_e) for the elements whose Hessian is not the derivative of their
+// gradient (hinge joints, friction, clamped barrier distances).
+__device__ void adj_matvec_t(const Dev& d, const Work& w, const Adj& a, const ScnCtx& sc, const double* b2arr,
+                             const double* v, double* y) {
+    const double* vg = v - 6 * (long long)sc.b0;  // global-indexed view
+    spmv(d, w, sc, b2arr, v, y);
+    for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+        const long long ce = (long long)sc.s * kMaxContacts + i;
+        const long long cc = sc.cb + i;
+        const int A = w.cint[5 * cc], B = w.cint[5 * cc + 1];
+        const double* D = a.dcor_c + 144 * ce;
+        double e[12];
+        for (int k = 0; k < 6; ++k) {
+            e[k] = vg[6 * A + k];
+            e[6 + k] = vg[6 * B + k];
+        }
+        for (int r = 0; r < 12; ++r) {
+            double acc = 0.0;
+            for (int c = 0; c < 12; ++c) acc += D[12 * r + c] * e[c];
+            a.ye_c[12 * ce + r] = acc;
+        }
+    }
+    for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
+        if (d.jtype[j] != 1) continue;
+        const double* D = a.dcor_j + 144 * (long long)j;
+        double e[12];
+        for (int k = 0; k < 6; ++k) {
+            e[k] = vg[6 * d.ja[j] + k];
+            e[6 + k] = vg[6 * d.jb[j] + k];
+        }
+        for (int r = 0; r < 12; ++r) {
+            double acc = 0.0;
+            for (int c = 0; c < 12; ++c) acc += D[12 * r + c] * e[c];
+            a.ye_j[12 * (long long)j + r] = acc;
+        }
+    }
+    __syncthreads();
+    const int* off = w.csr_off + sc.b0 + sc.s;
+    const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int b = sc.b0 + li;
+        for (int e = off[li]; e < off[li + 1]; ++e) {
+            const int c = ent[e] >> 1, side = ent[e] & 1;
+            const double* ye = a.ye_c + 12 * ((long long)sc.s * kMaxContacts + c) + 6 * side;
+            for (int k = 0; k < 6; ++k) y[6 * li + k] += ye[k];
+        }
+        for (int j = sc.j0; j < sc.j1; ++j) {
+            if (d.jtype[j] != 1) continue;
+            if (d.ja[j] == b) for (int k = 0; k < 6; ++k) y[6 * li + k] += a.ye_j[12 * (long long)j + k];
+            if (d.jb[j] == b) for (int k = 0; k < 6; ++k) y[6 * li + k] += a.ye_j[12 * (long long)j + 6 + k];
+        }
+    }
+    __syncthreads();
+}
+
+// One ALM solve of the reverse sweep: eta = A^-T xbar, A the exact Jacobian of the
+// Newton gradient at the converged point, by restarted GMRES with right preconditioning
+// by the assembled Hessian's direct solve (A and H differ by the hinge, friction and
+// clamped-distance corrections only, so few iterations are needed). Returns false if
+// the relative residual does not fall below 1e-12.
+constexpr int kGmresM = 24;
+__device__ bool adj_solve_gmres(const Dev& d, const Work& w, const Adj& a, const ScnCtx& sc, const double* b2arr,
+                                double* red, int* ok_sh, NewtonLds& nl) {
+    __shared__ double Hh[(kGmresM + 1) * kGmresM], cs[kGmresM], sn[kGmresM], gg[kGmresM + 1], yy[kGmresM];
+    const long long o = 6 * (long long)sc.b0;
+    const long long stride = 6 * (long long)d.N;
+    const double* xb = a.xbar + o;
+    double* eta = a.eta + o;
+    double* r = a.res + o;
+    double* z = a.tmp + o;
+    auto V = [&](int j) { return a.kry + j * stride + o; };
+    const int nd = sc.nd;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) eta[i] = 0.0;
+    __syncthreads();
+    const double bn = sqrt(wg_dot(xb, xb, nd, red));
+    if (!(bn > 0.0)) return true;
+    const double tol = 1e-13 * bn;
+    auto precond = [&](const double* in, double* out) -> bool {
+        const int c = component_solve(d, w, sc, b2arr, in, out, nl.comp, ok_sh);
+        const bool ok = c == 0 ? true : c == 2 ? pcg_solve(d, w, sc, b2arr, in, out, red, ok_sh) : false;
+        __syncthreads();
+        return ok;
+    };
+    double rn = bn;
+    for (int restart = 0; restart < 8; ++restart) {
+        adj_matvec_t(d, w, a, sc, b2arr, eta, z);
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) r[i] = xb[i] - z[i];
+        __syncthreads();
+        const double beta = sqrt(wg_dot(r, r, nd, red));
+        rn = beta;
+        if (beta <= tol) return true;
+        double* v0 = V(0);
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) v0[i] = r[i] / beta;
+        if (threadIdx.x == 0) {
+            gg[0] = beta;
+            for (int k = 1; k <= kGmresM; ++k) gg[k] = 0.0;
+        }
+        __syncthreads();
+        int jend = 0;
+        for (int j = 0; j < kGmresM; ++j) {
+            if (!precond(V(j), z)) return false;
+            double* wv = V(j + 1);
+            adj_matvec_t(d, w, a, sc, b2arr, z, wv);
+            for (int i = 0; i <= j; ++i) {
+                const double hij = wg_dot(wv, V(i), nd, red);
+                const double* vi = V(i);
+                for (int t = threadIdx.x; t < nd; t += blockDim.x) wv[t] -= hij * vi[t];
+                if (threadIdx.x == 0) Hh[i * kGmresM + j] = hij;
+                __syncthreads();
+            }
+            const double hn = sqrt(wg_dot(wv, wv, nd, red));
+            if (threadIdx.x == 0) Hh[(j + 1) * kGmresM + j] = hn;
+            if (hn > 0.0)
+                for (int t = threadIdx.x; t < nd; t += blockDim.x) wv[t] /= hn;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                for (int i = 0; i < j; ++i) {  // previous rotations
+                    const double h0 = Hh[i * kGmresM + j], h1 = Hh[(i + 1) * kGmresM + j];
+                    Hh[i * kGmresM + j] = cs[i] * h0 + sn[i] * h1;
+                    Hh[(i + 1) * kGmresM + j] = -sn[i] * h0 + cs[i] * h1;
+                }
+                const double h0 = Hh[j * kGmresM + j], h1 = Hh[(j + 1) * kGmresM + j];
+                const double den = sqrt(h0 * h0 + h1 * h1);
+                cs[j] = den > 0.0 ? h0 / den : 1.0;
+                sn[j] = den > 0.0 ? h1 / den : 0.0;
+                Hh[j * kGmresM + j] = den;
+                Hh[(j + 1) * kGmresM + j] = 0.0;
+                gg[j + 1] = -sn[j] * gg[j];
+                gg[j] = cs[j] * gg[j];
+            }
+            __syncthreads();
+            jend = j + 1;
+            if (fabs(gg[j + 1]) <= tol || !(hn > 0.0)) break;
+        }
+        if (threadIdx.x == 0) {
+            for (int i = jend - 1; i >= 0; --i) {
+                double s_ = gg[i];
+                for (int k = i + 1; k < jend; ++k) s_ -= Hh[i * kGmresM + k] * yy[k];
+                yy[i] = s_ / Hh[i * kGmresM + i];
+            }
+        }
+        __syncthreads();
+        // eta += H^-1 (V y)
+        for (int t = threadIdx.x; t < nd; t += blockDim.x) {
+            double acc = 0.0;
+            for (int k = 0; k < jend; ++k) acc += yy[k] * V(k)[t];
+            r[t] = acc;
+        }
+        __syncthreads();
+        if (!precond(r, z)) return false;
+        for (int t = threadIdx.x; t < nd; t += blockDim.x) eta[t] += z[t];
+        __syncthreads();
+    }
+    adj_matvec_t(d, w, a, sc, b2arr, eta, z);
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) r[i] = xb[i] - z[i];
+    __syncthreads();
+    rn = sqrt(wg_dot(r, r, nd, red));
+    return rn <= 1e-12 * bn;
+}
+
+// Reverse of step `step` of the recorded segment, every scene. ALM iteration a solved
+// g_a(x; lambda_{a-1}, z) = 0 (z: step-start pose and velocity, masses, friction
+// coefficient) and then set lambda_a = lambda_{a-1} + mu_a C(x*_a). Going back over the
+// iterations: xbar_a gets the write-back's cotangent (last iteration) and the multiplier
+// update's mu_a (dC/dx)^T lbar; eta = A_a^-T xbar_a; z and lambda_{a-1} receive -eta^T
+// dg_a/dz and -eta^T dg_a/dlambda. The contact set, lagged normal forces, barrier
+// stiffness and the hardening decisions are the solves' data and are held fixed.
+__global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int step) {
+    __shared__ double red[kBlk];
+    __shared__ int ok_sh;
+    __shared__ NewtonLds nl;
+    const int s = blockIdx.x;
+    const long long S = d.S;
+    const int N = d.N, J = d.joint_off[d.S];
+    const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
+    const double* stt = a.s_state + (long long)step * 13 * N;
+    const double* sp_ = stt;
+    const double* sq_ = stt + 3 * (long long)N;
+    const double* sv_ = stt + 7 * (long long)N;
+    int alast = -1;
+    for (int k = a.A - 1; k >= 0; --k)
+        if (!a.s_sp[((long long)step * a.A + k) * S + s].alm_done) { alast = k; break; }
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+        for (int k = 0; k < 3; ++k) d.p[3 * b + k] = sp_[3 * b + k];
+        for (int k = 0; k < 4; ++k) d.q[4 * b + k] = sq_[4 * b + k];
+    }
+    if (alast < 0) return;  // nothing recorded for this scene
+    const double* xfin = a.s_x + ((long long)step * a.A + alast) * 6 * N;
+    const double h = a.s_sp[((long long)step * a.A + alast) * S + s].adt;
+    // write-back: p' = x_p, v' = (x_p - p)/h, w' = x_th/h, q' = apply_theta(q, x_th)
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+        du::Dn<7> th[3], qq[4];
+        for (int k = 0; k < 3; ++k) th[k] = du::var<7>(xfin[6 * b + 3 + k], k);
+        for (int k = 0; k < 4; ++k) qq[k] = du::var<7>(sq_[4 * b + k], 3 + k);
+        const du::QT<du::Dn<7>> qn =
+            du::rot_update(du::mkq(qq[0], qq[1], qq[2], qq[3]), du::mk3(th[0], th[1], th[2]), true);
+        const du::Dn<7> qo[4] = {qn.w, qn.x, qn.y, qn.z};
+        double c[7];
+        for (int k = 0; k < 7; ++k) {
+            double acc = 0.0;
+            for (int r = 0; r < 4; ++r) acc += a.aq[4 * b + r] * qo[r].d[k];
+            c[k] = acc;
+        }
+        for (int k = 0; k < 3; ++k) {
+            a.bp[3 * b + k] = -a.av[3 * b + k] / h;
+            a.bv[3 * b + k] = 0.0;
+            a.bw[3 * b + k] = 0.0;
+            a.tmp[6 * b + k] = a.ap[3 * b + k] + a.av[3 * b + k] / h;  // final xbar (kept in tmp)
+            a.tmp[6 * b + 3 + k] = a.aw[3 * b + k] / h + c[k];
+        }
+        for (int k = 0; k < 4; ++k) a.bq[4 * b + k] = c[3 + k];
+    }
+    __syncthreads();
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
+        for (int k = 0; k < 6; ++k) a.res[6 * b + k] = a.tmp[6 * b + k];  // res holds the final xbar
+    __syncthreads();
+    for (int it = alast; it >= 0; --it) {
+        const long long slot = (long long)step * a.A + it;
+        if (a.s_sp[slot * S + s].alm_done) continue;
+        const double* sx = a.s_x + slot * 6 * N;
+        // restore the objective of this solve
+        if (threadIdx.x == 0) d.sp[s] = a.s_sp[slot * S + s];
+        __syncthreads();
+        SceneParams& sp = d.sp[s];
+        for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
+            for (int k = 0; k < 6; ++k) {
+                w.x[6 * b + k] = sx[6 * b + k];
+                a.xbar[6 * b + k] = it == alast ? a.res[6 * b + k] : 0.0;
+            }
+        {
+            const int nc = sp.n_contacts;
+            const long long cb = (long long)s * kMaxRawContacts;
+            const long long so = (slot * S + s) * kMaxContacts;
+            for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+                for (int k = 0; k < 5; ++k) w.cint[5 * (cb + i) + k] = a.s_cint[5 * (so + i) + k];
+                for (int k = 0; k < 7; ++k) w.cdbl[7 * (cb + i) + k] = a.s_cdbl[7 * (so + i) + k];
+                w.nf[(long long)s * kMaxContacts + i] = a.s_nf[so + i];
+            }
+            for (int j = d.joint_off[s] + threadIdx.x; j < d.joint_off[s + 1]; j += blockDim.x) {
+                for (int k = 0; k < 6; ++k) d.jlambda[6 * j + k] = a.s_jl[slot * 6 * J + 6 * j + k];
+                d.jmu[j] = a.s_jm[slot * J + j];
+            }
+        }
+        __syncthreads();
+        ScnCtx sc;
+        load_scn(d, s, sc);
+        // multiplier update lambda_a = lambda_{a-1} + mu_a C(x*_a): lbar flows to
+        // lambda_{a-1} unchanged and to x*_a and the reference rotations
+        for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
+            const int A = d.ja[j], B = d.jb[j], ty = d.jtype[j];
+            const double mpost = a.s_jm_post[slot * J + j];
+            const double* lb = a.lbar + 6 * (long long)j;
+            typedef du::Dn<12> D12;
+            D12 xa[6], xbv[6];
+            for (int k = 0; k < 6; ++k) {
+                xa[k] = du::var<12>(w.x[6 * A + k], k);
+                xbv[k] = du::var<12>(B >= 0 ? w.x[6 * B + k] : 0.0, 6 + k);
+            }
+            const du::QT<D12> qa12 = du::liftq<D12>(d.q + 4 * A);
+            const du::QT<D12> qb12 = du::liftq<D12>(d.q + 4 * (B >= 0 ? B : A));
+            D12 g12[12], C12[6];
+            du::joint_grad_t<D12>(ty, d.jdata + 16 * j, xa, xbv, qa12, qb12, d.jmu[j], d.jlambda + 6 * j, g12, C12);
+            double* xo = a.jxbar + 12 * (long long)j;
+            for (int k = 0; k < 12; ++k) {
+                double acc = 0.0;
+                for (int r = 0; r < 6; ++r) acc += lb[r] * C12[r].d[k];
+                xo[k] = mpost * acc;
+            }
+            typedef du::Dn<8> D8;
+            D8 xa8[6], xb8[6];
+            for (int k = 0; k < 6; ++k) {
+                xa8[k] = du::cst<8>(w.x[6 * A + k]);
+                xb8[k] = du::cst<8>(B >= 0 ? w.x[6 * B + k] : 0.0);
+            }
+            const du::QT<D8> qa8 = du::mkq(du::var<8>(d.q[4 * A], 0), du::var<8>(d.q[4 * A + 1], 1),
+                                           du::var<8>(d.q[4 * A + 2], 2), du::var<8>(d.q[4 * A + 3], 3));
+            const int Bq = B >= 0 ? B : A;
+            const du::QT<D8> qb8 = du::mkq(du::var<8>(d.q[4 * Bq], 4), du::var<8>(d.q[4 * Bq + 1], 5),
+                                           du::var<8>(d.q[4 * Bq + 2], 6), du::var<8>(d.q[4 * Bq + 3], 7));
+            D8 g8[12], C8[6];
+            du::joint_grad_t<D8>(ty, d.jdata + 16 * j, xa8, xb8, qa8, qb8, d.jmu[j], d.jlambda + 6 * j, g8, C8);
+            double* qo = a.jvjp + 8 * (long long)j;
+            for (int k = 0; k < 8; ++k) {
+                double acc = 0.0;
+                for (int r = 0; r < 6; ++r) acc += lb[r] * C8[r].d[k];
+                qo[k] = mpost * acc;
+            }
+            if (ty == 0)
+                for (int k = 4; k < 8; ++k) qo[k] = 0.0;  // fixed joint: qb8 aliases qa8
+            // exact Jacobian correction of a hinge (transposed): (dg/dx)^T - H_j
+            if (ty == 1) {
+                const double* jb = w.jblk + 108 * (long long)j;  // computed below by eval_elements
+                (void)jb;
+                double* Dj = a.dcor_j + 144 * (long long)j;
+                for (int r = 0; r < 12; ++r)
+                    for (int c = 0; c < 12; ++c) Dj[12 * r + c] = g12[c].d[r];
+            }
+        }
+        __syncthreads();
+        for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+            const int b = b0 + li;
+            for (int j = sc.j0; j < sc.j1; ++j) {
+                if (d.ja[j] == b) for (int k = 0; k < 6; ++k) a.xbar[6 * b + k] += a.jxbar[12 * (long long)j + k];
+                if (d.jb[j] == b) for (int k = 0; k < 6; ++k) a.xbar[6 * b + k] += a.jxbar[12 * (long long)j + 6 + k];
+                if (d.ja[j] == b) for (int k = 0; k < 4; ++k) a.bq[4 * b + k] += a.jvjp[8 * (long long)j + k];
+                if (d.jb[j] == b) for (int k = 0; k < 4; ++k) a.bq[4 * b + k] += a.jvjp[8 * (long long)j + 4 + k];
+            }
+        }
+        __syncthreads();
+        // Hessian of the solve at x*, then the hinge and contact corrections
+        build_csr(w, sc);
+        double* b2arr = w.sort_key + (long long)s * kSceneScratch;
+        eval_elements(d, w, sc, w.x, b2arr);
+        assemble_diag(d, w, sc, b2arr);
+        for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
+            if (d.jtype[j] != 1) continue;
+            const double* jb = w.jblk + 108 * (long long)j;
+            double* Dj = a.dcor_j + 144 * (long long)j;
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) {
+                    Dj[12 * r + c] -= jb[6 * r + c];                  // AA
+                    Dj[12 * (6 + r) + 6 + c] -= jb[36 + 6 * r + c];   // BB
+                    Dj[12 * r + 6 + c] -= jb[72 + 6 * r + c];         // AB
+                    Dj[12 * (6 + r) + c] -= jb[72 + 6 * c + r];       // BA = AB^T
+                }
+        }
+        for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+            const long long ce = (long long)s * kMaxContacts + i;
+            const long long cc = sc.cb + i;
+            double* Dc = a.dcor_c + 144 * ce;
+            for (int k = 0; k < 144; ++k) Dc[k] = 0.0;
+            const int act = w.cactive[ce];
+            const int A = w.cint[5 * cc], B = w.cint[5 * cc + 1];
+            const double* cd = w.cdbl + 7 * cc;
+            if (act & 1) {
+                // d = max(1e-9, (pB - pA).n): at the clamp the distance does not move
+                const double pr = (w.x[6 * B] - w.x[6 * A]) * cd[3] + (w.x[6 * B + 1] - w.x[6 * A + 1]) * cd[4] +
+                                  (w.x[6 * B + 2] - w.x[6 * A + 2]) * cd[5];
+                if (!(pr > 1e-9)) {
+                    const double b2 = b2arr[i] * sc.h2;
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) {
+                            const double v = b2 * (cd[3 + r] * cd[3 + c]);
+                            Dc[12 * r + c] -= v;
+                            Dc[12 * (6 + r) + 6 + c] -= v;
+                            Dc[12 * r + 6 + c] += v;
+                            Dc[12 * (6 + r) + c] += v;
+                        }
+                }
+            }
+            if (act & 2) {
+                typedef du::Dn<12> D12;
+                D12 xa[6], xbv[6];
+                for (int k = 0; k < 6; ++k) {
+                    xa[k] = du::var<12>(w.x[6 * A + k], k);
+                    xbv[k] = du::var<12>(w.x[6 * B + k], 6 + k);
+                }
+                D12 g[12];
+                du::friction_grad_t<D12>(xa, xbv, du::lift3<D12>(d.p[3 * A], d.p[3 * A + 1], d.p[3 * A + 2]),
+                                         du::liftq<D12>(d.q + 4 * A),
+                                         du::lift3<D12>(d.p[3 * B], d.p[3 * B + 1], d.p[3 * B + 2]),
+                                         du::liftq<D12>(d.q + 4 * B), cd, cd + 3, w.nf[ce], du::cst<12>(sc.fmu),
+                                         sc.feps, sc.fdt, g);
+                const double* blk = w.cblk + 144 * ce;
+                for (int r = 0; r < 12; ++r)
+                    for (int c = 0; c < 12; ++c) {
+                        const int br = r / 6, bc = c / 6;
+                        const int q = br == 0 ? (bc == 0 ? 0 : 72) : (bc == 0 ? 108 : 36);
+                        Dc[12 * r + c] += g[c].d[r] * sc.h2 - blk[q + 6 * (r % 6) + (c % 6)];
+                    }
+            }
+        }
+        __syncthreads();
+        const bool ok = adj_solve_gmres(d, w, a, sc, b2arr, red, &ok_sh, nl);
+        if (!ok && threadIdx.x == 0) atomicOr(&sp.solve_fail, 4);
+        const double* ETA = a.eta + 6 * (long long)b0;
+        // friction: lagged poses of both bodies and mu
+        for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+            const long long ce = (long long)s * kMaxContacts + i;
+            double* out = a.cvjp + 15 * ce;
+            for (int k = 0; k < 15; ++k) out[k] = 0.0;
+            if (!(w.cactive[ce] & 2)) continue;
+            const long long cc = sc.cb + i;
+            const int A = w.cint[5 * cc], B = w.cint[5 * cc + 1];
+            const double* cd = w.cdbl + 7 * cc;
+            typedef du::Dn<15> D;
+            D xa[6], xbv[6];
+            for (int k = 0; k < 6; ++k) {
+                xa[k] = du::cst<15>(w.x[6 * A + k]);
+                xbv[k] = du::cst<15>(w.x[6 * B + k]);
+            }
+            const du::V3T<D> pA0 = du::mk3(du::var<15>(d.p[3 * A], 0), du::var<15>(d.p[3 * A + 1], 1),
+                                           du::var<15>(d.p[3 * A + 2], 2));
+            const du::QT<D> qA0 = du::mkq(du::var<15>(d.q[4 * A], 3), du::var<15>(d.q[4 * A + 1], 4),
+                                          du::var<15>(d.q[4 * A + 2], 5), du::var<15>(d.q[4 * A + 3], 6));
+            const du::V3T<D> pB0 = du::mk3(du::var<15>(d.p[3 * B], 7), du::var<15>(d.p[3 * B + 1], 8),
+                                           du::var<15>(d.p[3 * B + 2], 9));
+            const du::QT<D> qB0 = du::mkq(du::var<15>(d.q[4 * B], 10), du::var<15>(d.q[4 * B + 1], 11),
+                                          du::var<15>(d.q[4 * B + 2], 12), du::var<15>(d.q[4 * B + 3], 13));
+            D g[12];
+            du::friction_grad_t<D>(xa, xbv, pA0, qA0, pB0, qB0, cd, cd + 3, w.nf[ce], du::var<15>(sc.fmu, 14),
+                                   sc.feps, sc.fdt, g);
+            const double* lA = ETA + 6 * (A - b0);
+            const double* lB = ETA + 6 * (B - b0);
+            for (int k = 0; k < 15; ++k) {
+                double acc = 0.0;
+                for (int j = 0; j < 6; ++j) acc += lA[j] * g[j].d[k] + lB[j] * g[6 + j].d[k];
+                out[k] = -(acc * sc.h2);
+            }
+        }
+        // joints: reference rotations, and lambda_{a-1} through g = J^T (mu C + lambda)
+        for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
+            const int A = d.ja[j], B = d.jb[j], ty = d.jtype[j];
+            const double* lA = ETA + 6 * (A - b0);
+            const double* lB = B >= 0 ? ETA + 6 * (B - b0) : lA;
+            typedef du::Dn<8> D8;
+            D8 xa8[6], xb8[6];
+            for (int k = 0; k < 6; ++k) {
+                xa8[k] = du::cst<8>(w.x[6 * A + k]);
+                xb8[k] = du::cst<8>(B >= 0 ? w.x[6 * B + k] : 0.0);
+            }
+            const int Bq = B >= 0 ? B : A;
+            const du::QT<D8> qa8 = du::mkq(du::var<8>(d.q[4 * A], 0), du::var<8>(d.q[4 * A + 1], 1),
+                                           du::var<8>(d.q[4 * A + 2], 2), du::var<8>(d.q[4 * A + 3], 3));
+            const du::QT<D8> qb8 = du::mkq(du::var<8>(d.q[4 * Bq], 4), du::var<8>(d.q[4 * Bq + 1], 5),
+                                           du::var<8>(d.q[4 * Bq + 2], 6), du::var<8>(d.q[4 * Bq + 3], 7));
+            D8 g8[12], C8[6];
+            du::joint_grad_t<D8>(ty, d.jdata + 16 * j, xa8, xb8, qa8, qb8, d.jmu[j], d.jlambda + 6 * j, g8, C8);
+            double* qo = a.jvjp + 8 * (long long)j;
+            for (int k = 0; k < 8; ++k) {
+                double acc = 0.0;
+                for (int r = 0; r < 6; ++r) acc += lA[r] * g8[r].d[k] + (ty == 1 ? lB[r] * g8[6 + r].d[k] : 0.0);
+                qo[k] = ty == 0 && k >= 4 ? 0.0 : -acc;
+            }
+            // dg/dlambda = J^T: lbar_{a-1} = lbar_a - J eta (J the joint Jacobian at x*)
+            JEval e;
+            joint_eval(d, j, w.x, e);
+            for (int r = 0; r < 6; ++r) {
+                double acc = 0.0;
+                for (int c = 0; c < 6; ++c) acc += e.J[r][c] * lA[c];
+                if (ty == 1)
+                    for (int c = 0; c < 6; ++c) acc += e.J[r][6 + c] * lB[c];
+                a.lbar[6 * (long long)j + r] -= acc;
+            }
+        }
+        __syncthreads();
+        const int* off = w.csr_off + sc.b0 + sc.s;
+        const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+        for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+            const int b = b0 + li;
+            const double m = d.mass[b];
+            const double* I = d.inertia + 9 * b;
+            const double* lp = ETA + 6 * li;
+            const double gvec[3] = {0.0, kG, 0.0};
+            double gm = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double xh = sp_[3 * b + k] + h * sv_[3 * b + k];
+                gm -= lp[k] * ((sx[6 * b + k] - xh) - sc.h2 * gvec[k]);
+                a.bp[3 * b + k] += m * lp[k];
+                a.bv[3 * b + k] += h * (m * lp[k]);
+                a.bw[3 * b + k] += h * (I[k] * lp[3] + I[3 + k] * lp[4] + I[6 + k] * lp[5]);
+            }
+            a.g_mass[b] += gm;
+            for (int e = off[li]; e < off[li + 1]; ++e) {
+                const int c = ent[e] >> 1, side = ent[e] & 1;
+                const double* cv = a.cvjp + 15 * ((long long)s * kMaxContacts + c) + 7 * side;
+                for (int k = 0; k < 3; ++k) a.bp[3 * b + k] += cv[k];
+                for (int k = 0; k < 4; ++k) a.bq[4 * b + k] += cv[3 + k];
+            }
+            for (int j = sc.j0; j < sc.j1; ++j) {
+                if (d.ja[j] == b) for (int k = 0; k < 4; ++k) a.bq[4 * b + k] += a.jvjp[8 * (long long)j + k];
+                if (d.jb[j] == b) for (int k = 0; k < 4; ++k) a.bq[4 * b + k] += a.jvjp[8 * (long long)j + 4 + k];
+            }
+        }
+        if (threadIdx.x == 0) {
+            double gmu = 0.0;
+            for (int i = 0; i < sc.nc; ++i) gmu += a.cvjp[15 * ((long long)s * kMaxContacts + i) + 14];
+            a.g_mu[s] += gmu;
+        }
+        __syncthreads();
+    }
+}
+
+void launch_adj_step(const Dev& d, const Work& w, const Adj& a, int step, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_step, dim3(d.S), dim3(kBlk), 0, st, d, w, a, step);
 }
 
 }  // namespace nd
