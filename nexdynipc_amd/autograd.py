@@ -46,12 +46,21 @@ class Simulation:
             self.e.set_joint_state(self.lam0, self.jmu0)
         self.e.set_barrier(*self.barrier0)
 
-    def run(self, v0, w0, mass, mu):
-        """Forward only (numpy in, numpy out): final positions and orientations."""
+    def run(self, v0, w0, mass, mu, with_path=False):
+        """Forward only (numpy in, numpy out): final positions and orientations. With
+        `with_path`, also the discrete path of the run: the ALM and Newton iteration
+        counts of every scene in every step ([n_steps, S, 2])."""
         self.reset(v0, w0, mass, mu)
-        self.e.step(self.n_steps)
+        if not with_path:
+            self.e.step(self.n_steps)
+            p, q = self.e.get_state()[:2]
+            return p, q
+        path = np.zeros((self.n_steps, self.sc.n_scenes, 2))
+        for k in range(self.n_steps):
+            self.e.step(1)
+            path[k] = self.e.stats()[:, :2]
         p, q = self.e.get_state()[:2]
-        return p, q
+        return p, q, path
 
 
 def _np(t):

@@ -1183,12 +1183,18 @@ int32_t ndipc_adjoint_forward(ndipc_ctx* ctx, const double* dt, int32_t n_steps,
     a.tmp = aalloc<double>(ctx, 6 * N);
     a.kry = aalloc<double>(ctx, 25 * 6 * N);
     a.cvjp = aalloc<double>(ctx, S * nd::kMaxContacts * 15);
-    a.dcor_c = aalloc<double>(ctx, S * nd::kMaxContacts * 144);
-    a.dcor_j = aalloc<double>(ctx, 144 * J);
+    a.ejac_c = aalloc<double>(ctx, S * nd::kMaxContacts * 144);
+    a.ejac_j = aalloc<double>(ctx, 144 * J);
     a.ye_c = aalloc<double>(ctx, S * nd::kMaxContacts * 12);
     a.ye_j = aalloc<double>(ctx, 12 * J);
+    a.yea_c = aalloc<double>(ctx, S * nd::kMaxContacts * 12);
+    a.yea_j = aalloc<double>(ctx, 12 * J);
+    a.dbase = aalloc<double>(ctx, 36 * N);
     a.jvjp = aalloc<double>(ctx, 8 * J);
     a.jxbar = aalloc<double>(ctx, 12 * J);
+    a.dense = aalloc<double>(ctx, S * 2 * nd::kDenseRows * nd::kDenseRows);
+    a.fail = aalloc<int>(ctx, S);
+    a.fail_info = aalloc<double>(ctx, 5 * S);
     for (void* p : ctx->adj_allocs)
         if (!p) {
             free_adjoint(ctx);
@@ -1222,6 +1228,7 @@ int32_t ndipc_adjoint_backward(ndipc_ctx* ctx, const double* dl_dp, const double
     HIPCHK(hipMemsetAsync(a.g_mass, 0, 8 * N, st));
     HIPCHK(hipMemsetAsync(a.g_mu, 0, 8 * S, st));
     HIPCHK(hipMemsetAsync(a.lbar, 0, 8 * 6 * size_t(std::max(ctx->J, 1)), st));
+    HIPCHK(hipMemsetAsync(a.fail, 0, 4 * S, st));
     if (upload_dt(ctx, ctx->adj_dt.data())) return -1;
     const int E = ctx->adj_every, nseg = ctx->adj_nck - 1;
     for (int seg = nseg - 1; seg >= 0; --seg) {
@@ -1229,7 +1236,7 @@ int32_t ndipc_adjoint_backward(ndipc_ctx* ctx, const double* dl_dp, const double
         const int len = std::min(E, ctx->adj_steps - seg * E);
         if (run_steps(ctx, len, &a)) return -1;
         for (int t = len - 1; t >= 0; --t) {
-            nd::launch_adj_step(ctx->d, ctx->w, a, t, st);
+            nd::launch_adj_step(ctx->d, ctx->w, a, t, seg * E + t, st);
             std::swap(a.ap, a.bp);
             std::swap(a.aq, a.bq);
             std::swap(a.av, a.bv);
@@ -1243,14 +1250,23 @@ int32_t ndipc_adjoint_backward(ndipc_ctx* ctx, const double* dl_dp, const double
         if (outp[k]) HIPCHK(hipMemcpyAsync(outp[k], srcp[k], 8 * len[k], hipMemcpyDeviceToHost, st));
     if (g_mass) HIPCHK(hipMemcpyAsync(g_mass, a.g_mass, 8 * N, hipMemcpyDeviceToHost, st));
     if (g_mu) HIPCHK(hipMemcpyAsync(g_mu, a.g_mu, 8 * S, hipMemcpyDeviceToHost, st));
+    std::vector<int> fl(S);
+    std::vector<double> fi(5 * S);
+    HIPCHK(hipMemcpyAsync(fl.data(), a.fail, 4 * S, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(fi.data(), a.fail_info, 8 * 5 * S, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    if (sync_sp_from_dev(ctx)) return -1;
-    int bad = -1;
-    for (int s = 0; s < ctx->S; ++s)
-        if (ctx->sp[s].solve_fail & 4) bad = s;
     // the adjoint sweep leaves the scene in its recorded state; return to the end state
     if (load_checkpoint(ctx, nseg)) return -1;
-    if (bad >= 0) return fail(ctx, "adjoint: the transposed solve did not converge in scene " + std::to_string(bad));
+    for (size_t s = 0; s < S; ++s)
+        if (fl[s]) {
+            const double* f = fi.data() + 5 * s;
+            char buf[256];
+            snprintf(buf, sizeof buf,
+                     "adjoint: the transposed solve did not converge in scene %zu (step %d, ALM iteration %d, "
+                     "direct solve code %d residual %.3e, GMRES residual %.3e)",
+                     s, int(f[0]), int(f[1]), int(f[2]), f[3], f[4]);
+            return fail(ctx, buf);
+        }
     return 0;
 }
 

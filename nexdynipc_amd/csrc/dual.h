@@ -111,8 +111,14 @@ template <int N>
 ND_DU Dn<N> operator/(double a, const Dn<N>& b) { return cst<N>(a) / b; }
 
 ND_DU double val(double x) { return x; }
+ND_DU double with_val(double, double v) { return v; }
 template <int N>
 ND_DU double val(const Dn<N>& x) { return x.v; }
+template <int N>
+ND_DU Dn<N> with_val(Dn<N> x, double v) {
+    x.v = v;
+    return x;
+}
 
 ND_DU double dsqrt(double x) { return sqrt(x); }
 template <int N>
@@ -221,7 +227,14 @@ template <typename T>
 ND_DU QT<T> rot_update(QT<T> q, V3T<T> th, bool strict) {
     const T ang = dsqrt(dot(th, th));
     const bool skip = strict ? !(val(ang) > 1e-10) : (val(ang) < 1e-10);
-    if (skip) return q;
+    if (skip) {
+        // below the threshold the update leaves q as it is; the derivative taken is that
+        // of the rotation exp(th/2) q which it stands for (at th = 0: (0, th/2) q), the
+        // limit that finite differences with |th| above the threshold see
+        const QT<T> lin = qmul(mkq<T>(lift<T>(1.0), th.x * 0.5, th.y * 0.5, th.z * 0.5), q);
+        return mkq<T>(with_val(lin.w, val(q.w)), with_val(lin.x, val(q.x)), with_val(lin.y, val(q.y)),
+                      with_val(lin.z, val(q.z)));
+    }
     const V3T<T> axis = dvs(th, ang);
     T s, c;
     dsincos(ang * 0.5, &s, &c);

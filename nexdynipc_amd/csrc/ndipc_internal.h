@@ -191,16 +191,23 @@ struct Adj {
     double* tmp;            // [6N]
     double* kry;            // [25][6N] Krylov basis of the transposed solve
     double* cvjp;           // [S*kMaxContacts][15]
-    double* dcor_c;         // [S*kMaxContacts][144] exact minus assembled Jacobian, transposed
-    double* dcor_j;         // [J][144]
-    double* ye_c;           // [S*kMaxContacts][12]
+    double* ejac_c;         // [S*kMaxContacts][144] exact friction Jacobian (h^2), transposed
+    double* ejac_j;         // [J][144] exact hinge Jacobian, transposed
+    double* ye_c;           // [S*kMaxContacts][12] element products of a matrix-vector product
     double* ye_j;           // [J][12]
+    double* yea_c;          // [S*kMaxContacts][12] the same with absolute values
+    double* yea_j;          // [J][12]
+    double* dbase;          // [N][36] inertia + fixed-joint + barrier diagonal blocks
     double* jvjp;           // [J][8] cotangent of the reference rotations
     double* jxbar;          // [J][12] cotangent of x from the multiplier update
+    double* dense;          // [S][2][kDenseRows^2] a component's A^T and its LU factors
+    int* fail;              // [S] 1: a transposed solve did not converge
+    double* fail_info;      // [S][5] first failure: step, ALM iteration, direct-solve code,
+                            // its relative residual, GMRES relative residual
 };
 void launch_adj_snapshot(const Dev& d, const Work& w, const Adj& a, int slot, hipStream_t st);
 void launch_adj_snapshot_post(const Dev& d, const Work& w, const Adj& a, int slot, hipStream_t st);
-void launch_adj_step(const Dev& d, const Work& w, const Adj& a, int step, hipStream_t st);
+void launch_adj_step(const Dev& d, const Work& w, const Adj& a, int step, int gstep, hipStream_t st);
 
 void launch_step_begin(const Dev& d, const Work& w, const double* dt, hipStream_t st);
 void launch_kappa(const Dev& d, const Work& w, hipStream_t st);

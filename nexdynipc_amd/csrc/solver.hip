@@ -1575,42 +1575,82 @@ void launch_adj_snapshot_post(const Dev& d, const Work& w, const Adj& a, int slo
     hipLaunchKernelGGL(k_adj_snapshot_post, dim3(d.S), dim3(kBlk), 0, st, d, w, a, slot);
 }
 
+// Diagonal blocks of the parts of A = dg/dx that equal the assembled Hessian: inertia,
+// fixed joints (C is linear in x) and the barrier (its b'' n n^T; zero where the
+// distance sits at its clamp, see k_adj_step). Hinge and friction elements enter A
+// through their own exact Jacobians instead.
+__device__ void assemble_diag_base(const Dev& d, const Work& w, const Adj& a, const ScnCtx& sc,
+                                   const double* b2arr) {
+    const int* off = w.csr_off + sc.b0 + sc.s;
+    const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int b = sc.b0 + li;
+        double D[36];
+        for (int k = 0; k < 36; ++k) D[k] = 0.0;
+        const double m = d.mass[b];
+        D[0] = m; D[7] = m; D[14] = m;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) D[(3 + r) * 6 + 3 + c] = d.inertia[9 * b + 3 * r + c];
+        for (int j = sc.j0; j < sc.j1; ++j) {
+            if (d.jtype[j] != 0 || d.ja[j] != b) continue;
+            const double* jb = w.jblk + 108 * j;
+            for (int k = 0; k < 36; ++k) D[k] += jb[k];
+        }
+        for (int e = off[li]; e < off[li + 1]; ++e) {
+            const int c = ent[e] >> 1;
+            if (!(w.cactive[(long long)sc.s * kMaxContacts + c] & 1)) continue;
+            const double* cd = w.cdbl + 7 * (sc.cb + c);
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q) D[r * 6 + q] += (b2arr[c] * (cd[3 + r] * cd[3 + q])) * sc.h2;
+        }
+        for (int k = 0; k < 36; ++k) a.dbase[36 * (long long)b + k] = D[k];
+    }
+}
+
 // y = A^T v for the scene, A = dg/dx the exact Jacobian of the Newton gradient at the
-// converged point: the assembled Hessian H (symmetric) plus per-element corrections
-// (A_e^T - H_e) for the elements whose Hessian is not the derivative of their
-// gradient (hinge joints, friction, clamped barrier distances).
+// converged point: the base diagonal blocks, the barrier's off-diagonal blocks and the
+// exact (transposed) Jacobians of the hinge and friction elements. With ya != nullptr
+// also ya = |A^T| |v| (entrywise), the scale of the componentwise backward error.
 __device__ void adj_matvec_t(const Dev& d, const Work& w, const Adj& a, const ScnCtx& sc, const double* b2arr,
-                             const double* v, double* y) {
+                             const double* v, double* y, double* ya) {
     const double* vg = v - 6 * (long long)sc.b0;  // global-indexed view
-    spmv(d, w, sc, b2arr, v, y);
     for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
         const long long ce = (long long)sc.s * kMaxContacts + i;
+        if (!(w.cactive[ce] & 2)) continue;
         const long long cc = sc.cb + i;
         const int A = w.cint[5 * cc], B = w.cint[5 * cc + 1];
-        const double* D = a.dcor_c + 144 * ce;
+        const double* E = a.ejac_c + 144 * ce;
         double e[12];
         for (int k = 0; k < 6; ++k) {
             e[k] = vg[6 * A + k];
             e[6 + k] = vg[6 * B + k];
         }
         for (int r = 0; r < 12; ++r) {
-            double acc = 0.0;
-            for (int c = 0; c < 12; ++c) acc += D[12 * r + c] * e[c];
+            double acc = 0.0, aa = 0.0;
+            for (int c = 0; c < 12; ++c) {
+                acc += E[12 * r + c] * e[c];
+                aa += fabs(E[12 * r + c]) * fabs(e[c]);
+            }
             a.ye_c[12 * ce + r] = acc;
+            a.yea_c[12 * ce + r] = aa;
         }
     }
     for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
         if (d.jtype[j] != 1) continue;
-        const double* D = a.dcor_j + 144 * (long long)j;
+        const double* E = a.ejac_j + 144 * (long long)j;
         double e[12];
         for (int k = 0; k < 6; ++k) {
             e[k] = vg[6 * d.ja[j] + k];
             e[6 + k] = vg[6 * d.jb[j] + k];
         }
         for (int r = 0; r < 12; ++r) {
-            double acc = 0.0;
-            for (int c = 0; c < 12; ++c) acc += D[12 * r + c] * e[c];
+            double acc = 0.0, aa = 0.0;
+            for (int c = 0; c < 12; ++c) {
+                acc += E[12 * r + c] * e[c];
+                aa += fabs(E[12 * r + c]) * fabs(e[c]);
+            }
             a.ye_j[12 * (long long)j + r] = acc;
+            a.yea_j[12 * (long long)j + r] = aa;
         }
     }
     __syncthreads();
@@ -1618,28 +1658,85 @@ __device__ void adj_matvec_t(const Dev& d, const Work& w, const Adj& a, const Sc
     const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
     for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
         const int b = sc.b0 + li;
+        const double* D = a.dbase + 36 * (long long)b;
+        double acc[6], aa[6];
+        for (int r = 0; r < 6; ++r) {
+            double s_ = 0.0, t_ = 0.0;
+            for (int c = 0; c < 6; ++c) {
+                s_ += D[r * 6 + c] * v[6 * li + c];
+                t_ += fabs(D[r * 6 + c]) * fabs(v[6 * li + c]);
+            }
+            acc[r] = s_;
+            aa[r] = t_;
+        }
         for (int e = off[li]; e < off[li + 1]; ++e) {
             const int c = ent[e] >> 1, side = ent[e] & 1;
-            const double* ye = a.ye_c + 12 * ((long long)sc.s * kMaxContacts + c) + 6 * side;
-            for (int k = 0; k < 6; ++k) y[6 * li + k] += ye[k];
+            const long long cc = sc.cb + c;
+            const long long ce = (long long)sc.s * kMaxContacts + c;
+            const int act = w.cactive[ce];
+            if (act & 1) {
+                const int other = (side == 0 ? w.cint[5 * cc + 1] : w.cint[5 * cc]) - sc.b0;
+                const double* cd = w.cdbl + 7 * cc;
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) {
+                        const double hv = (-(b2arr[c] * (cd[3 + r] * cd[3 + q]))) * sc.h2;
+                        acc[r] += hv * v[6 * other + q];
+                        aa[r] += fabs(hv) * fabs(v[6 * other + q]);
+                    }
+            }
+            if (act & 2)
+                for (int k = 0; k < 6; ++k) {
+                    acc[k] += a.ye_c[12 * ce + 6 * side + k];
+                    aa[k] += a.yea_c[12 * ce + 6 * side + k];
+                }
         }
         for (int j = sc.j0; j < sc.j1; ++j) {
             if (d.jtype[j] != 1) continue;
-            if (d.ja[j] == b) for (int k = 0; k < 6; ++k) y[6 * li + k] += a.ye_j[12 * (long long)j + k];
-            if (d.jb[j] == b) for (int k = 0; k < 6; ++k) y[6 * li + k] += a.ye_j[12 * (long long)j + 6 + k];
+            const int side = d.ja[j] == b ? 0 : (d.jb[j] == b ? 1 : -1);
+            if (side < 0) continue;
+            for (int k = 0; k < 6; ++k) {
+                acc[k] += a.ye_j[12 * (long long)j + 6 * side + k];
+                aa[k] += a.yea_j[12 * (long long)j + 6 * side + k];
+            }
+        }
+        for (int r = 0; r < 6; ++r) {
+            y[6 * li + r] = acc[r];
+            if (ya) ya[6 * li + r] = aa[r];
         }
     }
     __syncthreads();
 }
 
+// Componentwise backward error of eta as a solution of A^T eta = xbar:
+// max_i |xbar - A^T eta|_i / (|A^T| |eta| + |xbar|)_i (Oettli-Prager). r, ra: scratch.
+__device__ double adj_backward_error(const Dev& d, const Work& w, const Adj& a, const ScnCtx& sc,
+                                     const double* b2arr, const double* xb, const double* eta, double* r, double* ra,
+                                     double* red) {
+    adj_matvec_t(d, w, a, sc, b2arr, eta, r, ra);
+    double om = 0.0;
+    for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) {
+        const double den = ra[i] + fabs(xb[i]);
+        const double num = fabs(xb[i] - r[i]);
+        const double q = den > 0.0 ? num / den : (num > 0.0 ? 1.0 / 0.0 : 0.0);
+        om = isnan(q) ? 1.0 / 0.0 : dmax(om, q);
+    }
+    return wg_reduce(om, red, OpMax());
+}
+
+// Acceptance bound of a transposed solve: componentwise backward error (Oettli-Prager).
+// A normwise relative residual is no test here: fixed joints (mu = 1e8 and hardened up
+// to 1e12) and stiction blocks make A's rows differ in scale by up to 1e20.
+constexpr double kAdjBackwardTol = 1e-12;
+
 // One ALM solve of the reverse sweep: eta = A^-T xbar, A the exact Jacobian of the
 // Newton gradient at the converged point, by restarted GMRES with right preconditioning
-// by the assembled Hessian's direct solve (A and H differ by the hinge, friction and
-// clamped-distance corrections only, so few iterations are needed). Returns false if
-// the relative residual does not fall below 1e-12.
+// by the assembled Hessian's direct solve (A and H differ in the second-order hinge and
+// friction terms only, so few iterations are needed). The fallback of the direct solve
+// for components above kDenseBodies bodies. Returns whether the componentwise backward
+// error (in *rres; -1 if the preconditioner failed) is within kAdjBackwardTol.
 constexpr int kGmresM = 24;
 __device__ bool adj_solve_gmres(const Dev& d, const Work& w, const Adj& a, const ScnCtx& sc, const double* b2arr,
-                                double* red, int* ok_sh, NewtonLds& nl) {
+                                double* red, int* ok_sh, NewtonLds& nl, double* rres) {
     __shared__ double Hh[(kGmresM + 1) * kGmresM], cs[kGmresM], sn[kGmresM], gg[kGmresM + 1], yy[kGmresM];
     const long long o = 6 * (long long)sc.b0;
     const long long stride = 6 * (long long)d.N;
@@ -1652,6 +1749,7 @@ __device__ bool adj_solve_gmres(const Dev& d, const Work& w, const Adj& a, const
     for (int i = threadIdx.x; i < nd; i += blockDim.x) eta[i] = 0.0;
     __syncthreads();
     const double bn = sqrt(wg_dot(xb, xb, nd, red));
+    *rres = 0.0;
     if (!(bn > 0.0)) return true;
     const double tol = 1e-13 * bn;
     auto precond = [&](const double* in, double* out) -> bool {
@@ -1662,12 +1760,17 @@ __device__ bool adj_solve_gmres(const Dev& d, const Work& w, const Adj& a, const
     };
     double rn = bn;
     for (int restart = 0; restart < 8; ++restart) {
-        adj_matvec_t(d, w, a, sc, b2arr, eta, z);
+        adj_matvec_t(d, w, a, sc, b2arr, eta, z, nullptr);
         for (int i = threadIdx.x; i < nd; i += blockDim.x) r[i] = xb[i] - z[i];
         __syncthreads();
         const double beta = sqrt(wg_dot(r, r, nd, red));
         rn = beta;
-        if (beta <= tol) return true;
+        *rres = rn / bn;
+        if (beta <= tol) {
+            const double om = adj_backward_error(d, w, a, sc, b2arr, xb, eta, r, V(1), red);
+            *rres = om;
+            return om <= kAdjBackwardTol;
+        }
         double* v0 = V(0);
         for (int i = threadIdx.x; i < nd; i += blockDim.x) v0[i] = r[i] / beta;
         if (threadIdx.x == 0) {
@@ -1677,9 +1780,12 @@ __device__ bool adj_solve_gmres(const Dev& d, const Work& w, const Adj& a, const
         __syncthreads();
         int jend = 0;
         for (int j = 0; j < kGmresM; ++j) {
-            if (!precond(V(j), z)) return false;
+            if (!precond(V(j), z)) {
+                *rres = -1.0;
+                return false;
+            }
             double* wv = V(j + 1);
-            adj_matvec_t(d, w, a, sc, b2arr, z, wv);
+            adj_matvec_t(d, w, a, sc, b2arr, z, wv, nullptr);
             for (int i = 0; i <= j; ++i) {
                 const double hij = wg_dot(wv, V(i), nd, red);
                 const double* vi = V(i);
@@ -1726,15 +1832,156 @@ __device__ bool adj_solve_gmres(const Dev& d, const Work& w, const Adj& a, const
             r[t] = acc;
         }
         __syncthreads();
-        if (!precond(r, z)) return false;
+        if (!precond(r, z)) {
+            *rres = -1.0;
+            return false;
+        }
         for (int t = threadIdx.x; t < nd; t += blockDim.x) eta[t] += z[t];
         __syncthreads();
     }
-    adj_matvec_t(d, w, a, sc, b2arr, eta, z);
-    for (int i = threadIdx.x; i < nd; i += blockDim.x) r[i] = xb[i] - z[i];
+    const double om = adj_backward_error(d, w, a, sc, b2arr, xb, eta, r, V(1), red);
+    *rres = om;
+    return om <= kAdjBackwardTol;
+}
+
+// One ALM solve of the reverse sweep by a direct method. A^T is block diagonal over the
+// coupled components of H (hinges and active contacts couple bodies), so each
+// component's A^T is assembled densely (base diagonal blocks, then the barrier's
+// off-diagonal blocks and the exact hinge / friction Jacobians in element order) and
+// factored by LU without pivoting (right-looking, the trailing update in parallel; A
+// differs from the symmetric positive definite H only in second-order hinge and friction
+// terms, so the pivots stay those of a positive definite matrix). Two refinement steps
+// against the unfactored copy follow. Singletons are their base 6x6 blocks. Returns 0
+// when solved; 1 when a component exceeds kDenseBodies, 2 / 3 when a singleton /
+// component pivot vanishes, 4 when the componentwise backward error (*rres) exceeds
+// kAdjBackwardTol. The caller then runs GMRES.
+__device__ int adj_solve_direct(const Dev& d, const Work& w, const Adj& a, const ScnCtx& sc, const double* b2arr,
+                                double* red, int* ok_sh, CompLds& L, double* rres) {
+    const long long o = 6 * (long long)sc.b0;
+    const double* xb = a.xbar + o;
+    double* eta = a.eta + o;
+    double* xs = a.res + o;  // component right-hand side, then its residual
+    double* ys = a.tmp + o;  // component solution of one triangular pass
+    const int nd = sc.nd;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) eta[i] = 0.0;
     __syncthreads();
-    rn = sqrt(wg_dot(r, r, nd, red));
-    return rn <= 1e-12 * bn;
+    const double bn = sqrt(wg_dot(xb, xb, nd, red));
+    *rres = 0.0;
+    if (!(bn > 0.0)) return 0;
+    build_components(d, w, sc, L);
+    if (L.maxsize > kDenseBodies) return 1;
+    if (threadIdx.x == 0) *ok_sh = 1;
+    __syncthreads();
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        if (L.cnt[L.lab[li]] > 1) continue;
+        if (!ldlt6_solve(a.dbase + 36 * (sc.b0 + li), xb + 6 * li, eta + 6 * li)) atomicAnd(ok_sh, 0);
+    }
+    __syncthreads();
+    if (!*ok_sh) return 2;
+    double* Mo = a.dense + (long long)sc.s * 2 * kDenseRows * kDenseRows;
+    double* Mf = Mo + kDenseRows * kDenseRows;
+    for (int c0 = 0; c0 < L.ncoupled;) {
+        const int c = L.lab[L.list[c0]];
+        const int mb = L.cnt[c];
+        const int c1 = c0 + mb;
+        const int m = 6 * mb;
+        auto lpos = [&](int lb) {
+            int lo = c0, hi = c1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (L.list[mid] < lb) lo = mid + 1;
+                else hi = mid;
+            }
+            return lo - c0;
+        };
+        for (int t = threadIdx.x; t < m * m; t += blockDim.x) Mo[t] = 0.0;
+        __syncthreads();
+        for (int t = threadIdx.x; t < 36 * mb; t += blockDim.x) {
+            const int bi = t / 36, e = t % 36;
+            Mo[(6 * bi + e / 6) * m + 6 * bi + e % 6] = a.dbase[36 * (sc.b0 + L.list[c0 + bi]) + e];
+        }
+        __syncthreads();
+        // one element at a time, its 12x12 contribution to A^T (rows/cols [A | B]) by 144 lanes
+        auto add_element = [&](int A, int B, auto&& val) {
+            const int pa = lpos(A), pb = lpos(B);
+            if (threadIdx.x < 144) {
+                const int r = threadIdx.x / 12, q = threadIdx.x % 12;
+                const int pr = r < 6 ? pa : pb, pq = q < 6 ? pa : pb;
+                Mo[(6 * pr + r % 6) * m + 6 * pq + q % 6] += val(r, q);
+            }
+            __syncthreads();
+        };
+        for (int j = sc.j0; j < sc.j1; ++j) {
+            if (d.jtype[j] != 1) continue;
+            const int A = d.ja[j] - sc.b0, B = d.jb[j] - sc.b0;
+            if (L.lab[A] != c) continue;
+            const double* E = a.ejac_j + 144 * (long long)j;
+            add_element(A, B, [&](int r, int q) { return E[12 * r + q]; });
+        }
+        for (int i = 0; i < sc.nc; ++i) {
+            const long long cc = sc.cb + i, ce = (long long)sc.s * kMaxContacts + i;
+            const int act = w.cactive[ce];
+            if (!act) continue;
+            const int A = w.cint[5 * cc] - sc.b0, B = w.cint[5 * cc + 1] - sc.b0;
+            if (L.lab[A] != c) continue;
+            const double* cd = w.cdbl + 7 * cc;
+            const double* E = a.ejac_c + 144 * ce;
+            const double bb = b2arr[i];
+            add_element(A, B, [&](int r, int q) {
+                double v = (act & 2) ? E[12 * r + q] : 0.0;
+                const bool off = (r < 6) != (q < 6);
+                if (off && (act & 1) && r % 6 < 3 && q % 6 < 3)
+                    v += (-(bb * (cd[3 + r % 6] * cd[3 + q % 6]))) * sc.h2;
+                return v;
+            });
+        }
+        for (int t = threadIdx.x; t < m * m; t += blockDim.x) Mf[t] = Mo[t];
+        __syncthreads();
+        for (int k = 0; k < m; ++k) {
+            const double piv = Mf[k * m + k];
+            if (piv == 0.0 || !isfinite(piv)) return 3;
+            for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) Mf[i * m + k] /= piv;
+            __syncthreads();
+            const int r = m - k - 1;
+            for (int t = threadIdx.x; t < r * r; t += blockDim.x) {
+                const int i = k + 1 + t / r, j = k + 1 + t % r;
+                Mf[i * m + j] -= Mf[i * m + k] * Mf[k * m + j];
+            }
+            __syncthreads();
+        }
+        for (int t = threadIdx.x; t < m; t += blockDim.x) {
+            xs[t] = xb[6 * L.list[c0 + t / 6] + t % 6];
+            ys[t] = 0.0;
+        }
+        __syncthreads();
+        for (int pass = 0; pass < 3; ++pass) {
+            if (pass > 0) {  // residual of the component against the unfactored copy
+                for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                    double acc = xb[6 * L.list[c0 + i / 6] + i % 6];
+                    for (int j = 0; j < m; ++j) acc -= Mo[i * m + j] * ys[j];
+                    xs[i] = acc;
+                }
+                __syncthreads();
+            }
+            for (int k = 0; k < m; ++k) {  // unit lower
+                const double xk = xs[k];
+                for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) xs[i] -= Mf[i * m + k] * xk;
+                __syncthreads();
+            }
+            for (int k = m - 1; k >= 0; --k) {  // upper, solution added to ys
+                const double xk = xs[k] / Mf[k * m + k];
+                for (int i = threadIdx.x; i < k; i += blockDim.x) xs[i] -= Mf[i * m + k] * xk;
+                if (threadIdx.x == 0) ys[k] += xk;
+                __syncthreads();
+            }
+        }
+        for (int t = threadIdx.x; t < m; t += blockDim.x) eta[6 * L.list[c0 + t / 6] + t % 6] = ys[t];
+        __syncthreads();
+        c0 = c1;
+    }
+    const double om = adj_backward_error(d, w, a, sc, b2arr, xb, eta, xs, ys, red);
+    *rres = om;
+    return om <= kAdjBackwardTol ? 0 : 4;
 }
 
 // Reverse of step `step` of the recorded segment, every scene. ALM iteration a solved
@@ -1744,7 +1991,7 @@ __device__ bool adj_solve_gmres(const Dev& d, const Work& w, const Adj& a, const
 // update's mu_a (dC/dx)^T lbar; eta = A_a^-T xbar_a; z and lambda_{a-1} receive -eta^T
 // dg_a/dz and -eta^T dg_a/dlambda. The contact set, lagged normal forces, barrier
 // stiffness and the hardening decisions are the solves' data and are held fixed.
-__global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int step) {
+__global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int step, int gstep) {
     __shared__ double red[kBlk];
     __shared__ int ok_sh;
     __shared__ NewtonLds nl;
@@ -1866,13 +2113,11 @@ __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int ste
             }
             if (ty == 0)
                 for (int k = 4; k < 8; ++k) qo[k] = 0.0;  // fixed joint: qb8 aliases qa8
-            // exact Jacobian correction of a hinge (transposed): (dg/dx)^T - H_j
+            // exact Jacobian of a hinge's gradient, transposed: E[r][c] = dg_c / dx_r
             if (ty == 1) {
-                const double* jb = w.jblk + 108 * (long long)j;  // computed below by eval_elements
-                (void)jb;
-                double* Dj = a.dcor_j + 144 * (long long)j;
+                double* E = a.ejac_j + 144 * (long long)j;
                 for (int r = 0; r < 12; ++r)
-                    for (int c = 0; c < 12; ++c) Dj[12 * r + c] = g12[c].d[r];
+                    for (int c = 0; c < 12; ++c) E[12 * r + c] = g12[c].d[r];
             }
         }
         __syncthreads();
@@ -1890,43 +2135,30 @@ __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int ste
         build_csr(w, sc);
         double* b2arr = w.sort_key + (long long)s * kSceneScratch;
         eval_elements(d, w, sc, w.x, b2arr);
-        assemble_diag(d, w, sc, b2arr);
-        for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
-            if (d.jtype[j] != 1) continue;
-            const double* jb = w.jblk + 108 * (long long)j;
-            double* Dj = a.dcor_j + 144 * (long long)j;
-            for (int r = 0; r < 6; ++r)
-                for (int c = 0; c < 6; ++c) {
-                    Dj[12 * r + c] -= jb[6 * r + c];                  // AA
-                    Dj[12 * (6 + r) + 6 + c] -= jb[36 + 6 * r + c];   // BB
-                    Dj[12 * r + 6 + c] -= jb[72 + 6 * r + c];         // AB
-                    Dj[12 * (6 + r) + c] -= jb[72 + 6 * c + r];       // BA = AB^T
-                }
+        // d = max(1e-9, (pB - pA).n): at the clamp the distance, and so the barrier
+        // gradient, does not move with x. Its Hessian term (b'' at 1e-9, enormous) is not
+        // part of the exact Jacobian; it is left out of the assembly rather than added and
+        // subtracted, which would cancel every smaller term of the block.
+        for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+            const long long ce = (long long)s * kMaxContacts + i;
+            if (!(w.cactive[ce] & 1)) continue;
+            const long long cc = sc.cb + i;
+            const int A = w.cint[5 * cc], B = w.cint[5 * cc + 1];
+            const double* cd = w.cdbl + 7 * cc;
+            const double pr = (w.x[6 * B] - w.x[6 * A]) * cd[3] + (w.x[6 * B + 1] - w.x[6 * A + 1]) * cd[4] +
+                              (w.x[6 * B + 2] - w.x[6 * A + 2]) * cd[5];
+            if (!(pr > 1e-9)) b2arr[i] = 0.0;
         }
+        __syncthreads();
+        assemble_diag(d, w, sc, b2arr);  // H, for the GMRES fallback's preconditioner
+        assemble_diag_base(d, w, a, sc, b2arr);
         for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
             const long long ce = (long long)s * kMaxContacts + i;
             const long long cc = sc.cb + i;
-            double* Dc = a.dcor_c + 144 * ce;
-            for (int k = 0; k < 144; ++k) Dc[k] = 0.0;
+            double* Dc = a.ejac_c + 144 * ce;
             const int act = w.cactive[ce];
             const int A = w.cint[5 * cc], B = w.cint[5 * cc + 1];
             const double* cd = w.cdbl + 7 * cc;
-            if (act & 1) {
-                // d = max(1e-9, (pB - pA).n): at the clamp the distance does not move
-                const double pr = (w.x[6 * B] - w.x[6 * A]) * cd[3] + (w.x[6 * B + 1] - w.x[6 * A + 1]) * cd[4] +
-                                  (w.x[6 * B + 2] - w.x[6 * A + 2]) * cd[5];
-                if (!(pr > 1e-9)) {
-                    const double b2 = b2arr[i] * sc.h2;
-                    for (int r = 0; r < 3; ++r)
-                        for (int c = 0; c < 3; ++c) {
-                            const double v = b2 * (cd[3 + r] * cd[3 + c]);
-                            Dc[12 * r + c] -= v;
-                            Dc[12 * (6 + r) + 6 + c] -= v;
-                            Dc[12 * r + 6 + c] += v;
-                            Dc[12 * (6 + r) + c] += v;
-                        }
-                }
-            }
             if (act & 2) {
                 typedef du::Dn<12> D12;
                 D12 xa[6], xbv[6];
@@ -1940,18 +2172,25 @@ __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int ste
                                          du::lift3<D12>(d.p[3 * B], d.p[3 * B + 1], d.p[3 * B + 2]),
                                          du::liftq<D12>(d.q + 4 * B), cd, cd + 3, w.nf[ce], du::cst<12>(sc.fmu),
                                          sc.feps, sc.fdt, g);
-                const double* blk = w.cblk + 144 * ce;
+                // exact Jacobian of the friction gradient (scaled by h^2), transposed
                 for (int r = 0; r < 12; ++r)
-                    for (int c = 0; c < 12; ++c) {
-                        const int br = r / 6, bc = c / 6;
-                        const int q = br == 0 ? (bc == 0 ? 0 : 72) : (bc == 0 ? 108 : 36);
-                        Dc[12 * r + c] += g[c].d[r] * sc.h2 - blk[q + 6 * (r % 6) + (c % 6)];
-                    }
+                    for (int c = 0; c < 12; ++c) Dc[12 * r + c] = g[c].d[r] * sc.h2;
             }
         }
         __syncthreads();
-        const bool ok = adj_solve_gmres(d, w, a, sc, b2arr, red, &ok_sh, nl);
-        if (!ok && threadIdx.x == 0) atomicOr(&sp.solve_fail, 4);
+        double rdir = 0.0, rgm = -2.0;
+        const int dcode = adj_solve_direct(d, w, a, sc, b2arr, red, &ok_sh, nl.comp, &rdir);
+        bool ok = dcode == 0;
+        if (!ok) ok = adj_solve_gmres(d, w, a, sc, b2arr, red, &ok_sh, nl, &rgm);
+        if (!ok && threadIdx.x == 0 && a.fail[s] == 0) {
+            a.fail[s] = 1;
+            double* fi = a.fail_info + 5 * (long long)s;
+            fi[0] = gstep;
+            fi[1] = it;
+            fi[2] = dcode;
+            fi[3] = rdir;
+            fi[4] = rgm;
+        }
         const double* ETA = a.eta + 6 * (long long)b0;
         // friction: lagged poses of both bodies and mu
         for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
@@ -2060,8 +2299,8 @@ __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int ste
     }
 }
 
-void launch_adj_step(const Dev& d, const Work& w, const Adj& a, int step, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_step, dim3(d.S), dim3(kBlk), 0, st, d, w, a, step);
+void launch_adj_step(const Dev& d, const Work& w, const Adj& a, int step, int gstep, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_step, dim3(d.S), dim3(kBlk), 0, st, d, w, a, step, gstep);
 }
 
 }  // namespace nd
