@@ -1170,6 +1170,8 @@ int32_t ndipc_adjoint_forward(ndipc_ctx* ctx, const double* dt, int32_t n_steps,
     a.s_jl = aalloc<double>(ctx, LA * 6 * J);
     a.s_jm = aalloc<double>(ctx, LA * J);
     a.s_jm_post = aalloc<double>(ctx, LA * J);
+    a.s_nit = aalloc<int>(ctx, LA * S);
+    a.carry = aalloc<double>(ctx, 6 * N);
     a.ap = aalloc<double>(ctx, 3 * N); a.aq = aalloc<double>(ctx, 4 * N);
     a.av = aalloc<double>(ctx, 3 * N); a.aw = aalloc<double>(ctx, 3 * N);
     a.bp = aalloc<double>(ctx, 3 * N); a.bq = aalloc<double>(ctx, 4 * N);
@@ -1229,6 +1231,7 @@ int32_t ndipc_adjoint_backward(ndipc_ctx* ctx, const double* dl_dp, const double
     HIPCHK(hipMemsetAsync(a.g_mu, 0, 8 * S, st));
     HIPCHK(hipMemsetAsync(a.lbar, 0, 8 * 6 * size_t(std::max(ctx->J, 1)), st));
     HIPCHK(hipMemsetAsync(a.fail, 0, 4 * S, st));
+    HIPCHK(hipMemsetAsync(a.carry, 0, 8 * 6 * N, st));
     if (upload_dt(ctx, ctx->adj_dt.data())) return -1;
     const int E = ctx->adj_every, nseg = ctx->adj_nck - 1;
     for (int seg = nseg - 1; seg >= 0; --seg) {

@@ -180,6 +180,8 @@ struct Adj {
     double* s_jl;           // [L*A][6J] multipliers before the solve
     double* s_jm;           // [L*A][J] stiffness of the solve
     double* s_jm_post;      // [L*A][J] stiffness of the multiplier update after it
+    int* s_nit;             // [L*A][S] Newton iterations of the solve (0: no step accepted)
+    double* carry;          // [6N] cotangent passed to the previous solve's x*
     double *ap, *aq, *av, *aw;  // cotangent of the state after the step
     double *bp, *bq, *bv, *bw;  // cotangent of the state before the step
     double* lbar;           // [6J] cotangent of the joint multipliers (updated in place)

@@ -1563,6 +1563,8 @@ __global__ void __launch_bounds__(kBlk) k_adj_snapshot_post(Dev d, Work w, Adj a
     const int J = d.joint_off[d.S];
     if (a.s_sp[slot * S + s].alm_done) return;
     const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
+    if (threadIdx.x == 0)  // Newton iterations of this solve (the step's count before it + this + 1)
+        a.s_nit[slot * S + s] = d.sp[s].newton_iters - a.s_sp[slot * S + s].newton_iters - 1;
     for (int i = 6 * b0 + threadIdx.x; i < 6 * b1; i += blockDim.x) a.s_x[(long long)slot * 6 * d.N + i] = w.x[i];
     for (int j = d.joint_off[s] + threadIdx.x; j < d.joint_off[s + 1]; j += blockDim.x)
         a.s_jm_post[(long long)slot * J + j] = d.jmu[j];
@@ -2051,7 +2053,9 @@ __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int ste
         for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
             for (int k = 0; k < 6; ++k) {
                 w.x[6 * b + k] = sx[6 * b + k];
-                a.xbar[6 * b + k] = it == alast ? a.res[6 * b + k] : 0.0;
+                const double cv = a.carry[6 * b + k];  // from the next solve, if it took no step
+                a.carry[6 * b + k] = 0.0;
+                a.xbar[6 * b + k] = it == alast ? a.res[6 * b + k] : cv;
             }
         {
             const int nc = sp.n_contacts;
@@ -2131,6 +2135,23 @@ __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int ste
             }
         }
         __syncthreads();
+        // a solve whose Newton loop accepted no step returned its starting point: the
+        // previous solve's x*, or x_hat = [p + h v, h w] for the first
+        if (a.s_nit[slot * S + s] == 0) {
+            for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
+                for (int k = 0; k < 6; ++k) {
+                    const double v = a.xbar[6 * b + k];
+                    if (it > 0) a.carry[6 * b + k] = v;
+                    else if (k < 3) {
+                        a.bp[3 * b + k] += v;
+                        a.bv[3 * b + k] += h * v;
+                    } else {
+                        a.bw[3 * b + k - 3] += h * v;
+                    }
+                }
+            __syncthreads();
+            continue;
+        }
         // Hessian of the solve at x*, then the hinge and contact corrections
         build_csr(w, sc);
         double* b2arr = w.sort_key + (long long)s * kSceneScratch;

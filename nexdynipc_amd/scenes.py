@@ -465,6 +465,21 @@ def micro_contact_scene(seed=0, n_cubes=4, size=6e-4, gap=2e-4, dhat=1e-3, mu=0.
     return SceneSpec(bodies, dt=1e-3, dhat=dhat, mu=mu, name=f"micro{seed}")
 
 
+def hinge_chain_scene(n_links=3, joint_mu=1e4, seed=5) -> Scenes:
+    """A free chain of 0.4 x 0.1 x 0.1 m links joined by hinges about z, with seeded
+    initial velocities N(0, 0.3) and angular velocities N(0, 1), no static body. The
+    hinges' ALM stiffness is joint_mu (the reference's default, 1e8, makes the Newton
+    loop fail at every ALM iteration once links spin; 1e4 keeps every solve converged)."""
+    rng = np.random.default_rng(SEED_BASE + 8000 + seed)
+    bodies = [_box_body("link", 0.4, 0.1, 0.1, [0.45 * k, 1.0, 0.0], v=rng.normal(0.0, 0.3, 3),
+                        w=rng.normal(0.0, 1.0, 3)) for k in range(n_links)]
+    hinges = [(k, k + 1, np.array([0.225, 0, 0]), np.array([-0.225, 0, 0]), np.array([0, 0, 1.0]),
+               np.array([0, 0, 1.0])) for k in range(n_links - 1)]
+    sc = pack([SceneSpec(bodies, dt=1e-2, dhat=1e-3, mu=0.3, hinges=hinges, name="chain")])
+    sc.joint_mu[:] = joint_mu
+    return sc
+
+
 def make_config(name: str, **kw) -> Scenes:
     if name == "c1":
         return pack([config1()])

@@ -477,6 +477,8 @@ struct Solver {
     bool enable_friction = false;
     double friction_coeff = 0.0;
     int alm_max_iters = 10;
+    double newton_tol = 1e-6;     // NewtonSolver (NewtonSolver.cpp:10-95)
+    int newton_max_iters = 20;
     double alm_constraint_tol = 1e-4, alm_dual_tol = 1e-3, alm_contact_tol = 1e-3;
     double alm_hardening_trigger = 2.0, alm_hardening_ratio = 2.0;
     std::unordered_set<uint64_t> allowed;
@@ -501,6 +503,28 @@ struct Solver {
     // integrator (ImplicitEuler)
     vector<double> x_prev, v_prev, a_prev, x_hat;
     double idt = 0;
+
+    // Recording for the CPU adjoint (adjoint_sweep): every Newton solve of every step
+    // with the data its objective reads, the converged x, and the joint stiffness after
+    // the hardening that follows it.
+    struct AdjSolve {
+        vector<double> x_star;
+        vector<Joint> joints;  // multipliers before the solve, stiffness of the solve
+        vector<double> mu_post;
+        vector<Contact> b_contacts, f_contacts;
+        vector<double> f_normal;
+        double b_dhat = 0, b_kappa = 0, f_mu = 0, f_eps = 0, f_dt = 0;
+        bool has_barrier = false, has_friction = false;
+        bool converged = false;
+        int newton_iters = 0;
+    };
+    struct AdjStep {
+        vector<Body> bodies;  // step-start state
+        vector<double> x_hat;
+        double adt = 0;
+        vector<AdjSolve> solves;
+    };
+    vector<AdjStep>* adj_rec = nullptr;
 
     // CPU-baseline sampling of computeSceneMinDistance: only every k-th vertex of the
     // outer loop is evaluated and the measured time is scaled by k (bench only).
@@ -1089,12 +1113,12 @@ struct Solver {
         vector<double> g, cg, dx, ng;
         SymAccum H;
         SkylineLDLT ldlt;
-        for (int it = 0; it < 20; ++it) {
+        for (int it = 0; it < newton_max_iters; ++it) {
             iters = it;
             obj_grad(x, g);
             double ginf = 0.0;
             for (double v : g) ginf = std::max(ginf, std::abs(v));
-            if (ginf < 1e-6) return true;
+            if (ginf < newton_tol) return true;
             obj_hess(x, H);
             if (!ldlt.factor(H)) return false;
             ng.resize(g.size());
@@ -1223,6 +1247,13 @@ struct Solver {
         x_hat.assign(nd, 0.0);
         for (int i = 0; i < nd; ++i) x_hat[i] = x_prev[i] + adt * v_prev[i];  // ImplicitEulerIntegrator.cpp:5-7
         vector<double> x_new = x_hat;
+        if (adj_rec) {
+            AdjStep st;
+            st.bodies = bodies;
+            st.x_hat = x_hat;
+            st.adt = adt;
+            adj_rec->push_back(std::move(st));
+        }
 
         vector<Contact> active;
         const bool needs = enable_adaptive_barrier || (enable_friction && friction_coeff > 0.0);
@@ -1267,6 +1298,16 @@ struct Solver {
                     j.qA_ref = bodies[j.a].q;
                 }
             }
+            AdjSolve rec_solve;
+            if (adj_rec) {
+                rec_solve.joints = joints;
+                rec_solve.b_contacts = b_contacts;
+                rec_solve.f_contacts = f_contacts;
+                rec_solve.f_normal = f_normal;
+                rec_solve.b_dhat = b_dhat; rec_solve.b_kappa = b_kappa;
+                rec_solve.f_mu = f_mu; rec_solve.f_eps = f_eps; rec_solve.f_dt = f_dt;
+                rec_solve.has_barrier = has_barrier; rec_solve.has_friction = has_friction;
+            }
             int nit = 0;
             const bool conv = newton(x_new, nit);
             st_newton += nit + 1;
@@ -1283,6 +1324,13 @@ struct Solver {
                     const double nm = j.mu * alm_hardening_ratio;
                     j.mu = std::isfinite(nm) ? std::min(nm, 1e12) : 1e12;
                 }
+            }
+            if (adj_rec) {
+                rec_solve.x_star = x_new;
+                rec_solve.converged = conv;
+                rec_solve.newton_iters = nit;
+                for (auto& j : joints) rec_solve.mu_post.push_back(j.mu);
+                adj_rec->back().solves.push_back(std::move(rec_solve));
             }
             for (auto& j : joints) {  // ConstraintForm::updateLambdas -> Joint::updateLambda
                 double C[6];
@@ -1326,14 +1374,248 @@ struct Solver {
             b.v = V3(vn[0], vn[1], vn[2]);
             b.w = V3(vn[3], vn[4], vn[5]);
             b.p = seg3(x_new, i * 6);
-            const V3 th = seg3(x_new, i * 6 + 3);
-            const double ang = norm(th);
-            if (ang > 1e-10) {
-                const V3 axis = th / ang;
-                b.q = qnormalized(mul(from_angle_axis(ang, axis), b.q));
-            }
+            b.q = write_back_q(b.q, seg3(x_new, i * 6 + 3));
         }
         has_barrier = has_friction = false;
+    }
+    // IPCSolver.cpp:1187-1191
+    static Q write_back_q(const Q& q, const V3& th) {
+        const double ang = norm(th);
+        if (ang > 1e-10) {
+            const V3 axis = th / ang;
+            return qnormalized(mul(from_angle_axis(ang, axis), q));
+        }
+        return q;
+    }
+
+    // ---------------- CPU adjoint
+    // The adjoint of recorded steps, with every derivative taken by central differences
+    // of this file's own functions (objective gradient, joint constraint, quaternion
+    // write-back): an FP64 check of the device adjoint that shares none of its
+    // derivative code. Per step, in reverse: the write-back's cotangent gives x-bar of
+    // the last solve; each solve a (last to first) adds the multiplier update's
+    // mu_a (dC/dx)^T lambda-bar, solves A^T eta = x-bar with A = dg/dx at x*_a (dense,
+    // by differences, LU with partial pivoting), then sends -eta^T dg/dz to every input z
+    // of g_a: x_hat (step-start p, v, w), masses, step-start poses (friction frame, joint
+    // reference rotations), multipliers and the friction coefficient. A solve whose
+    // Newton loop accepted no step (NewtonSolver returned at its first iteration, with or
+    // without convergence) is the identity on its starting point, so its x-bar passes to
+    // the previous solve's x* (or to x_hat). Contact sets, normals and normal forces,
+    // barrier stiffness and the hardening are held fixed, as on the device.
+    void adj_load(const AdjStep& st, const AdjSolve& so) {
+        bodies = st.bodies;
+        x_hat = st.x_hat;
+        idt = st.adt;
+        joints = so.joints;
+        b_contacts = so.b_contacts;
+        f_contacts = so.f_contacts;
+        f_normal = so.f_normal;
+        b_dhat = so.b_dhat; b_kappa = so.b_kappa;
+        f_mu = so.f_mu; f_eps = so.f_eps; f_dt = so.f_dt;
+        has_barrier = so.has_barrier; has_friction = so.has_friction;
+    }
+    // step-start pose of body i, which the friction frame and the joint references read
+    void set_start_pose(int i, const V3& p, const Q& q) {
+        bodies[i].p = p;
+        bodies[i].q = q;
+        for (auto& j : joints) {
+            if (j.a == i) { j.pA_ref = p; j.qA_ref = q; }
+            if (j.b == i) { j.pB_ref = p; j.qB_ref = q; }
+        }
+    }
+    double eta_dot_grad(const vector<double>& x, const vector<double>& eta) const {
+        vector<double> g;
+        obj_grad(x, g);
+        double s = 0.0;
+        for (size_t i = 0; i < g.size(); ++i) s += eta[i] * g[i];
+        return s;
+    }
+    // x = M^-1 b, M dense row-major n x n (LU with partial pivoting); false if singular
+    static bool lu_solve(vector<double> M, vector<double>& b, int n) {
+        vector<int> piv(n);
+        for (int k = 0; k < n; ++k) {
+            int p = k;
+            for (int i = k + 1; i < n; ++i)
+                if (std::abs(M[size_t(i) * n + k]) > std::abs(M[size_t(p) * n + k])) p = i;
+            if (!(std::abs(M[size_t(p) * n + k]) > 0.0)) return false;
+            if (p != k) {
+                for (int c = 0; c < n; ++c) std::swap(M[size_t(k) * n + c], M[size_t(p) * n + c]);
+                std::swap(b[k], b[p]);
+            }
+            const double d = M[size_t(k) * n + k];
+            for (int i = k + 1; i < n; ++i) {
+                const double f = M[size_t(i) * n + k] / d;
+                if (f == 0.0) continue;
+                M[size_t(i) * n + k] = f;
+                for (int c = k + 1; c < n; ++c) M[size_t(i) * n + c] -= f * M[size_t(k) * n + c];
+                b[i] -= f * b[k];
+            }
+        }
+        for (int k = n - 1; k >= 0; --k) {
+            double s = b[k];
+            for (int c = k + 1; c < n; ++c) s -= M[size_t(k) * n + c] * b[c];
+            b[k] = s / M[size_t(k) * n + k];
+        }
+        return true;
+    }
+    // central difference of f at z with step h*max(1, |z|); f reads z by reference
+    template <typename F>
+    static double cdiff(double& z, double rel, F&& f) {
+        const double z0 = z, h = rel * std::max(1.0, std::abs(z0));
+        z = z0 + h;
+        const double fp = f();
+        z = z0 - h;
+        const double fm = f();
+        z = z0;
+        return (fp - fm) / (2.0 * h);
+    }
+    // Cotangents in: P (3N), Qb (4N), V (3N), W (3N) of the end state. Out: the same of
+    // the start state, gm (N) and gmu. Returns false if a transposed solve is singular.
+    bool adjoint_sweep(vector<AdjStep>& rec, vector<double>& P, vector<double>& Qb, vector<double>& V,
+                       vector<double>& W, vector<double>& gm, double& gmu) {
+        const int N = n(), nd = 6 * N;
+        const double rel = 1e-7;
+        const int J = int(joints.size());
+        vector<double> lbar(6 * size_t(J), 0.0);
+        gm.assign(N, 0.0);
+        gmu = 0.0;
+        for (int k = int(rec.size()) - 1; k >= 0; --k) {
+            AdjStep& st = rec[k];
+            const double h = st.adt;
+            const int last = int(st.solves.size()) - 1;
+            vector<double> Pn(3 * N, 0.0), Qn(4 * N, 0.0), Vn(3 * N, 0.0), Wn(3 * N, 0.0), xbar(nd, 0.0);
+            vector<double> carry(nd, 0.0);  // cotangent of the previous solve's x* (its warm start)
+            const vector<double>& xf = st.solves[last].x_star;
+            for (int i = 0; i < N; ++i) {
+                for (int c = 0; c < 3; ++c) {
+                    xbar[6 * i + c] = P[3 * i + c] + V[3 * i + c] / h;
+                    Pn[3 * i + c] -= V[3 * i + c] / h;
+                    xbar[6 * i + 3 + c] = W[3 * i + c] / h;
+                }
+                // q' = write_back_q(q, theta)
+                double in[7] = {xf[6 * i + 3], xf[6 * i + 4], xf[6 * i + 5], st.bodies[i].q.w, st.bodies[i].q.x,
+                                st.bodies[i].q.y, st.bodies[i].q.z};
+                for (int t = 0; t < 7; ++t) {
+                    double dq[4];
+                    for (int r = 0; r < 4; ++r) {
+                        dq[r] = cdiff(in[t], rel, [&] {
+                            const Q o = write_back_q(Q(in[3], in[4], in[5], in[6]), V3(in[0], in[1], in[2]));
+                            const double oo[4] = {o.w, o.x, o.y, o.z};
+                            return oo[r];
+                        });
+                    }
+                    double acc = 0.0;
+                    for (int r = 0; r < 4; ++r) acc += Qb[4 * i + r] * dq[r];
+                    if (t < 3) xbar[6 * i + 3 + t] += acc;
+                    else Qn[4 * i + t - 3] += acc;
+                }
+            }
+            for (int a = last; a >= 0; --a) {
+                AdjSolve& so = st.solves[a];
+                adj_load(st, so);
+
+                vector<double> xs = so.x_star;
+                if (a != last) xbar = carry;
+                carry.assign(nd, 0.0);
+                // multiplier update lambda_a = lambda_{a-1} + mu_post C(x*_a)
+                for (int j = 0; j < J; ++j) {
+                    bool any = false;
+                    for (int r = 0; r < 6; ++r) any = any || lbar[6 * j + r] != 0.0;
+                    if (!any) continue;
+                    Joint& jt = joints[j];
+                    const double mp = so.mu_post[j];
+                    auto lc = [&] {
+                        double C[6];
+                        joint_C(jt, xs, C);
+                        double t = 0.0;
+                        for (int r = 0; r < 6; ++r) t += lbar[6 * j + r] * C[r];
+                        return t;
+                    };
+                    for (int g : {jt.gA, jt.gB}) {
+                        if (g < 0) continue;
+                        for (int c = 0; c < 6; ++c) xbar[g + c] += mp * cdiff(xs[g + c], rel, lc);
+                    }
+                    for (int side = 0; side < 2; ++side) {
+                        const int bi = side == 0 ? jt.a : jt.b;
+                        if (bi < 0) continue;
+                        Q& qr = side == 0 ? jt.qA_ref : jt.qB_ref;
+                        double* qc[4] = {&qr.w, &qr.x, &qr.y, &qr.z};
+                        for (int r = 0; r < 4; ++r) Qn[4 * bi + r] += mp * cdiff(*qc[r], rel, lc);
+                    }
+                }
+                // a solve that accepted no Newton step returned its starting point: the
+                // previous solve's x*, or x_hat for the first
+                if (so.newton_iters == 0) {
+                    if (a == 0) {
+                        for (int i = 0; i < nd; ++i) {
+                            const int b = i / 6, c = i % 6;
+                            if (c < 3) {
+                                Pn[3 * b + c] += xbar[i];
+                                Vn[3 * b + c] += h * xbar[i];
+                            } else {
+                                Wn[3 * b + c - 3] += h * xbar[i];
+                            }
+                        }
+                    } else {
+                        carry = xbar;
+                    }
+                    continue;
+                }
+                // A = dg/dx at x*, by columns; solve A^T eta = xbar
+                vector<double> At(size_t(nd) * nd);  // row c of At = column c of A
+                for (int c = 0; c < nd; ++c) {
+                    const double x0 = xs[c], hh = rel * std::max(1.0, std::abs(x0));
+                    vector<double> gp, gmi;
+                    xs[c] = x0 + hh;
+                    obj_grad(xs, gp);
+                    xs[c] = x0 - hh;
+                    obj_grad(xs, gmi);
+                    xs[c] = x0;
+                    for (int r = 0; r < nd; ++r) At[size_t(c) * nd + r] = (gp[r] - gmi[r]) / (2.0 * hh);
+                }
+                vector<double> eta = xbar;
+                if (!lu_solve(At, eta, nd)) return false;
+                auto s_eta = [&] { return eta_dot_grad(xs, eta); };
+                for (int i = 0; i < nd; ++i) {
+                    const double xh = -cdiff(x_hat[i], rel, s_eta);
+                    const int b = i / 6, c = i % 6;
+                    if (c < 3) {
+                        Pn[3 * b + c] += xh;
+                        Vn[3 * b + c] += h * xh;
+                    } else {
+                        Wn[3 * b + c - 3] += h * xh;
+                    }
+                }
+                for (int b = 0; b < N; ++b) gm[b] -= cdiff(bodies[b].mass, rel, s_eta);
+                // step-start poses: bodies in a friction contact or a joint
+                vector<char> touched(N, 0);
+                if (has_friction)
+                    for (auto& c : f_contacts) touched[c.A] = touched[c.B] = 1;
+                for (auto& jt : joints) {
+                    if (jt.a >= 0) touched[jt.a] = 1;
+                    if (jt.b >= 0) touched[jt.b] = 1;
+                }
+                for (int b = 0; b < N; ++b) {
+                    if (!touched[b]) continue;
+                    double pq[7] = {bodies[b].p.x, bodies[b].p.y, bodies[b].p.z, bodies[b].q.w, bodies[b].q.x,
+                                    bodies[b].q.y, bodies[b].q.z};
+                    for (int t = 0; t < 7; ++t) {
+                        const double d = cdiff(pq[t], rel, [&] {
+                            set_start_pose(b, V3(pq[0], pq[1], pq[2]), Q(pq[3], pq[4], pq[5], pq[6]));
+                            return s_eta();
+                        });
+                        set_start_pose(b, V3(pq[0], pq[1], pq[2]), Q(pq[3], pq[4], pq[5], pq[6]));
+                        if (t < 3) Pn[3 * b + t] -= d;
+                        else Qn[4 * b + t - 3] -= d;
+                    }
+                }
+                for (int j = 0; j < J; ++j)
+                    for (int r = 0; r < 6; ++r) lbar[6 * j + r] -= cdiff(joints[j].lambda[r], rel, s_eta);
+                if (has_friction) gmu -= cdiff(f_mu, rel, s_eta);
+            }
+            P = Pn; Qb = Qn; V = Vn; W = Wn;
+        }
+        return true;
     }
 };
 
@@ -1692,6 +1974,59 @@ int32_t ndo_step(void* h, double dt) {
     S(h).step(dt);
     S(h).t_step += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     S(h).assembler_build();
+    return 0;
+}
+
+int32_t ndo_set_newton(void* h, double tol, int32_t max_iters) {
+    Solver& s = S(h);
+    s.newton_tol = tol;
+    s.newton_max_iters = std::max(1, int(max_iters));
+    return 0;
+}
+
+int32_t ndo_set_mass(void* h, const double* mass) {
+    Solver& s = S(h);
+    for (int i = 0; i < s.n(); ++i) s.bodies[i].mass = mass[i];
+    return 0;
+}
+
+int32_t ndo_adjoint(void* h, double dt, int32_t n_steps, const double* dl_dp, const double* dl_dq,
+                    const double* dl_dv, const double* dl_dw, double* g_p0, double* g_q0, double* g_v0,
+                    double* g_w0, double* g_mass, double* g_mu) {
+    Solver& s = S(h);
+    const int N = s.n();
+    std::vector<Solver::AdjStep> rec;
+    s.adj_rec = &rec;
+    for (int k = 0; k < n_steps; ++k) ndo_step(h, dt);
+    s.adj_rec = nullptr;
+    // the sweep overwrites the scene's forms; keep the end state to restore it
+    const std::vector<Body> end_bodies = s.bodies;
+    const std::vector<Joint> end_joints = s.joints;
+    std::vector<double> P(3 * N, 0.0), Qb(4 * N, 0.0), V(3 * N, 0.0), W(3 * N, 0.0), gm;
+    for (int i = 0; i < 3 * N; ++i) {
+        if (dl_dp) P[i] = dl_dp[i];
+        if (dl_dv) V[i] = dl_dv[i];
+        if (dl_dw) W[i] = dl_dw[i];
+    }
+    for (int i = 0; i < 4 * N; ++i)
+        if (dl_dq) Qb[i] = dl_dq[i];
+    double gmu = 0.0;
+    const bool ok = s.adjoint_sweep(rec, P, Qb, V, W, gm, gmu);
+    s.bodies = end_bodies;
+    s.joints = end_joints;
+    s.has_barrier = s.has_friction = false;
+    s.assembler_build();
+    if (!ok) return -1;
+    for (int i = 0; i < 3 * N; ++i) {
+        if (g_p0) g_p0[i] = P[i];
+        if (g_v0) g_v0[i] = V[i];
+        if (g_w0) g_w0[i] = W[i];
+    }
+    for (int i = 0; i < 4 * N; ++i)
+        if (g_q0) g_q0[i] = Qb[i];
+    for (int i = 0; i < N; ++i)
+        if (g_mass) g_mass[i] = gm[i];
+    if (g_mu) *g_mu = gmu;
     return 0;
 }
 

@@ -114,6 +114,20 @@ int32_t ndo_root_finder_kat(int32_t which, double* lo, double* hi);
 void ndo_round_ops(const double* a, const double* b, int32_t n, double* out6);
 int32_t ndo_elemfn(int32_t fn, const double* x, const double* y, int32_t n, double* out);
 
+/* NewtonSolver tolerance on |g|_inf and iteration cap (the reference's 1e-6 and 20). */
+int32_t ndo_set_newton(void* h, double tol, int32_t max_iters);
+/* Body masses (RigidBody::mass), scene-local order. */
+int32_t ndo_set_mass(void* h, const double* mass);
+/* CPU adjoint: runs n_steps steps of dt from the current state, recording every Newton
+ * solve, then sweeps back (implicit-function theorem at every solve, every derivative by
+ * central differences of the restatement's own functions, dense LU). dl_d* are the
+ * cotangents of the end state (NULL: zero); outputs are the gradients with respect to
+ * the start state, the masses and the friction coefficient. Leaves the scene at the end
+ * state. Returns -1 if a transposed solve is singular. */
+int32_t ndo_adjoint(void* h, double dt, int32_t n_steps, const double* dl_dp, const double* dl_dq,
+                    const double* dl_dv, const double* dl_dw, double* g_p0, double* g_q0, double* g_v0,
+                    double* g_w0, double* g_mass, double* g_mu);
+
 /* Newton system at x (SimulationProblem with the current contact forms): */
 int32_t ndo_objective(void* h, const double* x, double* value, double* grad, double* hess);
 

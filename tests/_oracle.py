@@ -103,6 +103,12 @@ def lib():
         L.ndo_set_sampling.argtypes = [C.c_void_p, C.c_int32]
         L.ndo_timing.restype = C.c_int32
         L.ndo_timing.argtypes = [C.c_void_p, f64p]
+        L.ndo_set_newton.restype = C.c_int32
+        L.ndo_set_newton.argtypes = [C.c_void_p, C.c_double, C.c_int32]
+        L.ndo_set_mass.restype = C.c_int32
+        L.ndo_set_mass.argtypes = [C.c_void_p, f64p]
+        L.ndo_adjoint.restype = C.c_int32
+        L.ndo_adjoint.argtypes = [C.c_void_p, C.c_double, C.c_int32] + [f64p] * 10
         _lib = L
     return _lib
 
@@ -212,6 +218,31 @@ class OracleScene:
         o = np.zeros(8)
         self.L.ndo_last_stats(self.h, ptr(o, f64p))
         return o
+
+    def set_newton(self, tol=1e-6, max_iters=20):
+        self.L.ndo_set_newton(self.h, float(tol), int(max_iters))
+
+    def set_mass(self, mass):
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        assert m.shape == (self.n,)
+        self.L.ndo_set_mass(self.h, ptr(m, f64p))
+
+    def set_mu(self, mu):
+        self.params.friction_mu = float(mu)
+        self.L.ndo_configure(self.h, C.byref(self.params))
+
+    def adjoint(self, dt, n_steps, dl_dp=None, dl_dq=None, dl_dv=None, dl_dw=None):
+        """CPU adjoint of n_steps steps from the current state (ndo_adjoint): gradients
+        of a loss of the end state with respect to the start state, masses and mu."""
+        n = self.n
+        ins = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (dl_dp, dl_dq, dl_dv, dl_dw)]
+        out = {"p0": np.zeros((n, 3)), "q0": np.zeros((n, 4)), "v0": np.zeros((n, 3)), "w0": np.zeros((n, 3)),
+               "mass": np.zeros(n), "mu": np.zeros(1)}
+        rc = self.L.ndo_adjoint(self.h, float(dt), int(n_steps), *[ptr(a, f64p) for a in ins],
+                                *[ptr(out[k], f64p) for k in ("p0", "q0", "v0", "w0", "mass", "mu")])
+        if rc != 0:
+            raise RuntimeError("ndo_adjoint: a transposed solve is singular")
+        return out
 
     def body_pairs(self, r):
         cap = self.n * self.n

@@ -1,0 +1,77 @@
+"""The oracle's FP64 CPU adjoint (ndo_adjoint) against central finite differences of the
+oracle's own forward steps. The CPU adjoint is the FP64 reference the device adjoint is
+checked against (tests/test_gpu_adjoint.py); here it is pinned by differences on the CPU.
+
+Central differences use h = 1e-6, or the first of 1e-5, 1e-4 whose +h and -h runs keep the
+discrete path (ALM and Newton iteration counts of every step); components with no such
+step are not compared.
+"""
+import numpy as np
+
+import _oracle as O
+from nexdynipc_amd import scenes
+
+
+def _run(sc, n_steps, v0=None, w0=None, mass=None):
+    o = O.OracleScene(sc, 0)
+    o.set_state(sc.p, sc.q, sc.v if v0 is None else v0, sc.w if w0 is None else w0)
+    if mass is not None:
+        o.set_mass(mass)
+    path = []
+    for _ in range(n_steps):
+        o.step(float(sc.dt[0]))
+        path.append(tuple(o.stats()[:2]))
+    p, q = o.get_state()[:2]
+    return p, q, path
+
+
+def _check(sc, n_steps, t, cq, comps, bar):
+    p, q, path0 = _run(sc, n_steps)
+    o = O.OracleScene(sc, 0)
+    g = o.adjoint(float(sc.dt[0]), n_steps, dl_dp=2.0 * (p - t), dl_dq=cq)
+    pe, qe = o.get_state()[:2]
+    assert np.array_equal(pe, p) and np.array_equal(qe, q)  # the adjoint run is the forward run
+    adj, fd = [], []
+    for key, idx in comps:
+        for h in (1e-6, 1e-5, 1e-4):
+            vals, same = [], True
+            for sgn in (1.0, -1.0):
+                kw = {"v0": sc.v.copy(), "w0": sc.w.copy(), "mass": sc.mass.copy()}
+                step = h * kw[key][idx] if key == "mass" else h
+                kw[key][idx] += sgn * step
+                pp, qq, path = _run(sc, n_steps, **kw)
+                vals.append(float(((pp - t) ** 2).sum() + (cq * qq).sum()))
+                same = same and path == path0
+            if same:
+                break
+        if same:
+            adj.append(float(g[key][idx]))
+            fd.append((vals[0] - vals[1]) / (2.0 * step))
+    adj, fd = np.array(adj), np.array(fd)
+    assert len(adj) >= len(comps) // 2
+    err = np.linalg.norm(adj - fd) / np.linalg.norm(fd)
+    assert err <= bar, (err, list(zip(adj, fd)))
+
+
+def test_cpu_adjoint_hinge_chain():
+    """Free 3-link hinge chain, 3 steps, every velocity / angular velocity / mass. The bar
+    is 5e-5: the solves stop at the reference's Newton tolerance (1e-6), and differences
+    see the derivative of that remainder (1.5e-5 at 5 steps)."""
+    sc = scenes.hinge_chain_scene()
+    rng = np.random.default_rng(21)
+    t = sc.p + rng.normal(0.0, 0.1, size=sc.p.shape)
+    cq = rng.normal(0.0, 0.1, size=(sc.n_bodies, 4))
+    comps = [(k, (b, c)) for k in ("v0", "w0") for b in range(sc.n_bodies) for c in range(3)]
+    comps += [("mass", (b,)) for b in range(sc.n_bodies)]
+    _check(sc, 3, t, cq, comps, 5e-5)
+
+
+def test_cpu_adjoint_c4_free_cubes():
+    """C4 (chain, pinned link, 48 cubes on a ramp), 2 steps: cubes' velocities and
+    angular velocities, within 1e-5."""
+    spec, targets = scenes.config4()
+    sc = scenes.pack([spec])
+    rng = np.random.default_rng(22)
+    cq = rng.normal(0.0, 1e-2, size=(sc.n_bodies, 4))
+    comps = [("v0", (b, c)) for b in (20, 41) for c in range(3)] + [("w0", (33, c)) for c in range(3)]
+    _check(sc, 2, np.asarray(targets), cq, comps, 1e-5)
