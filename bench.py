@@ -9,7 +9,11 @@ resident in HBM. `value` is steps/s of that scene.
 
 The same JSON line carries the 256-scene batch (scenes.config3): aggregate scene-steps/s
 of one batched engine, and the speedup over running the same scenes one by one through
-the plain single-scene path (a bounded sample of scenes, steps/s summed over scenes).
+the plain single-scene path (a bounded sample of scenes, steps/s summed over scenes),
+and the adjoint: gradients/s of the differentiation workload (scenes.config4, 100 steps,
+checkpoint every 10) and of the 128-scene batch (scenes.config5, 50 steps), the
+backward's cost relative to a plain forward of the same rollout, and a central-difference
+check (h = 1e-6) of a few components.
 
 Prints one JSON line. --gpus above 1 is not supported: this build runs on one GPU.
 """
@@ -92,6 +96,60 @@ def run_plain(Engine, torch, sc, steps, warmup):
                 newton_iters=float(np.sum(c1[:, 6] - c0[:, 6])), trials=float(np.sum(c1[:, 7] - c0[:, 7])))
 
 
+def run_adjoint(Engine, sc, targets, n_steps, every):
+    """Forward with checkpoints + reverse sweep of an n_steps rollout, against the plain
+    forward of the same rollout (loss: squared distance of the final positions to the
+    targets). Second repetition timed (the first allocates)."""
+    from nexdynipc_amd.autograd import Simulation
+
+    sim = Simulation(sc, n_steps, every, engine=Engine(0))
+    for _ in range(2):
+        sim.reset(sc.v, sc.w, sc.mass, sc.mu)
+        sim.e.stats()
+        t0 = time.perf_counter()
+        sim.e.step(n_steps)
+        sim.e.stats()
+        t_plain = time.perf_counter() - t0
+        sim.reset(sc.v, sc.w, sc.mass, sc.mu)
+        sim.e.stats()
+        t0 = time.perf_counter()
+        sim.e.adjoint_forward(n_steps, every)
+        sim.e.stats()
+        t_fwd = time.perf_counter() - t0
+        p = sim.e.get_state()[0]
+        t0 = time.perf_counter()
+        g = sim.e.adjoint_backward(dl_dp=2.0 * (p - targets))
+        t_bwd = time.perf_counter() - t0
+    sim.e.close()
+    return dict(plain_s=t_plain, forward_s=t_fwd, backward_s=t_bwd, grads=g,
+                gradients_per_s=sc.n_scenes / (t_fwd + t_bwd),
+                backward_over_forward=t_bwd / t_plain, total_over_forward=(t_fwd + t_bwd) / t_plain)
+
+
+def adjoint_fd_check(Engine, sc, targets, n_steps, comps, h=1e-6):
+    """Central differences of the loss for a few initial-velocity components (device
+    forward runs) against the adjoint: largest relative difference."""
+    from nexdynipc_amd.autograd import Simulation
+
+    sim = Simulation(sc, n_steps, n_steps, engine=Engine(0))
+    sim.reset(sc.v, sc.w, sc.mass, sc.mu)
+    sim.e.adjoint_forward(n_steps, n_steps)
+    p = sim.e.get_state()[0]
+    g = sim.e.adjoint_backward(dl_dp=2.0 * (p - targets))["v0"]
+    worst = 0.0
+    for b, c in comps:
+        vals = []
+        for sgn in (1.0, -1.0):
+            v = sc.v.copy()
+            v[b, c] += sgn * h
+            pp = sim.run(v, sc.w, sc.mass, sc.mu)[0]
+            vals.append(float(((pp - targets) ** 2).sum()))
+        fd = (vals[0] - vals[1]) / (2.0 * h)
+        worst = max(worst, abs(g[b, c] - fd) / abs(fd))
+    sim.e.close()
+    return worst
+
+
 def roofline_for(name, r, sc):
     """Roofline of kernel `name` over the timed region: algorithmic work / total time."""
     avg_ms, cnt = r["kern"][name]
@@ -132,6 +190,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--one-by-one", type=int, default=32, help="scenes sampled for the one-by-one comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs (c4, c5)")
+    ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--cpu-stride", type=int, default=2000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 20 device copies of 256 MiB (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -188,6 +248,31 @@ def main():
             print(json.dumps(out))
             return
 
+    adjoint = None
+    if not args.no_adjoint:
+        spec4, t4 = scenes.config4()
+        sc4 = scenes.pack([spec4])
+        a4 = run_adjoint(Engine, sc4, np.asarray(t4), args.adjoint_steps, 10)
+        fd_worst = adjoint_fd_check(Engine, sc4, np.asarray(t4), 10, [(20, 0), (33, 2), (47, 1), (58, 0)])
+        sc5 = scenes.pack(scenes.config5(128))
+        p5 = sc5.p + 0.0
+        a5 = run_adjoint(Engine, sc5, p5, 50, 10)
+        adjoint = {
+            "c4": {"workload": "c4: 16-link hinge chain (link 0 pinned) + 48 cubes on a 20 deg ramp, "
+                               f"{args.adjoint_steps} steps, checkpoint every 10; loss sum |p - target|^2",
+                   "gradients_per_s": a4["gradients_per_s"], "plain_forward_s": a4["plain_s"],
+                   "recorded_forward_s": a4["forward_s"], "backward_s": a4["backward_s"],
+                   "backward_over_forward": a4["backward_over_forward"],
+                   "total_over_forward": a4["total_over_forward"],
+                   "fd_check": {"h": 1e-6, "steps": 10, "components": 4, "max_rel_err": fd_worst}},
+            "c5": {"workload": "c5: 128 scenes x 32 cubes thrown at a wall and a floor, 50 steps, one batched "
+                               "forward + adjoint; loss sum |p - p0|^2",
+                   "scene_gradients_per_s": a5["gradients_per_s"], "plain_forward_s": a5["plain_s"],
+                   "recorded_forward_s": a5["forward_s"], "backward_s": a5["backward_s"],
+                   "backward_over_forward": a5["backward_over_forward"],
+                   "total_over_forward": a5["total_over_forward"]},
+        }
+
     sc = scenes.make_config("c2")
     workload = "c2: 512 icospheres (subdiv 3) + 5-box open container, single scene"
     r = run_plain(Engine, torch, sc, args.steps, args.warmup)
@@ -230,6 +315,7 @@ def main():
         "roofline": roof,
         "roofline_narrow_phase": roofline_for("np_count", r, sc),
         "batch": batch,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
