@@ -26,6 +26,7 @@ constexpr int kHitWords = 48;         // 3072 candidate bits per pair (2x1024 FV
 constexpr int kSceneScratch = 65536;  // per-scene scratch doubles (sort keys, barrier b'')
 constexpr int kDenseBodies = 64;      // largest coupled component factored directly
 constexpr int kDenseRows = 6 * kDenseBodies;
+constexpr int kLsBatch = 8;           // line-search trials evaluated together
 
 struct SceneParams {
     // settings
@@ -127,6 +128,7 @@ struct Work {
     double* sort_key;        // [S * kSceneScratch] sort keys / Newton scratch
     // solver vectors, 6 per body
     double *x, *xhat, *xprev, *vprev, *g, *dx, *xt, *cg, *r, *z, *pv, *Ap, *tmp;
+    double* xtb;             // [kLsBatch][6N] line-search trial points
     double* D;               // [N*36]
     double* Dinv;            // [N*36]
     double* nf;              // [S*2000] normal forces

@@ -460,6 +460,98 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
     *out_cval = cval;
 }
 
+// Objective and contact values at K trial points XT_k = X + alpha_k DX (k < K <=
+// kLsBatch, XT_k at XTB + k * stride, global body indexing): the line search's next K
+// step lengths evaluated together. Every value is summed exactly as in objective_value
+// (same terms, same order, from +0.0); the 5 K accumulators run on lanes 0..5K-1 of
+// the first wavefront, side by side, so K trials cost about one trial's serial chain.
+__device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx& sc, const double* XTB,
+                                      long long stride, int K, double* red, double* vals, double* cvals,
+                                      double* T) {
+    constexpr int WB = kTermLds / (6 * kLsBatch);  // items per window and trial
+    constexpr int TS = 6 * WB;                      // doubles per trial window
+    constexpr int OI = 0, OG = 2 * WB, OJ = 3 * WB, OB = 4 * WB, OF = 5 * WB;
+    const double* XH = w.xhat;
+    const int nj = sc.j1 - sc.j0;
+    int rounds = (sc.n + WB - 1) / WB;
+    rounds = max(rounds, (nj + WB - 1) / WB);
+    rounds = max(rounds, (sc.nc + WB - 1) / WB);
+    double acc = 0.0;  // lane L < 5K: accumulator L % 5 of trial L / 5
+    const int lk = threadIdx.x / 5, lc = threadIdx.x % 5;
+    for (int r = 0; r < rounds; ++r) {
+        const int b_lo = r * WB;
+        for (int t = threadIdx.x; t < K * WB; t += blockDim.x) {
+            const int k = t / WB, tt = t % WB;
+            const double* X = XTB + k * stride;
+            double* Tk = T + k * TS;
+            const int li = b_lo + tt;
+            if (li < sc.n) {
+                const int b = sc.b0 + li;
+                const double m = d.mass[b];
+                const double* I = d.inertia + 9 * b;
+                const V3 dp = xpos(X, b) - xpos(XH, b);
+                const V3 dt = xrot(X, b) - xrot(XH, b);
+                const V3 rr = 0.5 * dt;
+                const double s0 = rr.x * I[0] + rr.y * I[3] + rr.z * I[6];
+                const double s1 = rr.x * I[1] + rr.y * I[4] + rr.z * I[7];
+                const double s2 = rr.x * I[2] + rr.y * I[5] + rr.z * I[8];
+                Tk[OI + 2 * tt] = 0.5 * m * sqnorm(dp);
+                Tk[OI + 2 * tt + 1] = s0 * dt.x + s1 * dt.y + s2 * dt.z;
+                Tk[OG + tt] = -(m * (0.0 * X[6 * b] + kG * X[6 * b + 1] + 0.0 * X[6 * b + 2]));
+            }
+            if (li < nj) {
+                const int j = sc.j0 + li;
+                JEval e;
+                joint_eval(d, j, X, e);
+                double t1 = 0.0, sq = 0.0;
+                for (int q = 0; q < 6; ++q) t1 += d.jlambda[6 * j + q] * e.C[q];
+                for (int q = 0; q < 6; ++q) sq += e.C[q] * e.C[q];
+                Tk[OJ + tt] = t1 + 0.5 * d.jmu[j] * sq;
+            }
+            if (li < sc.nc) {
+                const long long c = sc.cb + li;
+                Tk[OB + tt] = sc.has_barrier ? bar_v(contact_dist(w, c, X), sc.dhat, sc.kappa) : 0.0;
+                double fv = 0.0;
+                if (sc.has_friction && li < sc.nfc) {
+                    const double N = w.nf[(long long)sc.s * kMaxContacts + li];
+                    if (N > 0.0) {
+                        FKin kk;
+                        fric_kin(d, w, c, X, sc.fdt, kk);
+                        fv = fric_v(kk.u, N, sc.fmu, sc.feps);
+                    }
+                }
+                Tk[OF + tt] = fv;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 5 * K) {
+            const double* Tk = T + lk * TS;
+            const int nb = min(WB, max(0, sc.n - b_lo));
+            const int ncw = min(WB, max(0, sc.nc - b_lo));
+            const int njw = min(WB, max(0, nj - b_lo));
+            const double* base = lc == 0 ? Tk + OI : lc == 1 ? Tk + OG : lc == 2 ? Tk + OJ : lc == 3 ? Tk + OB : Tk + OF;
+            const int cnt = lc == 0 ? 2 * nb : lc == 1 ? nb : lc == 2 ? njw : ncw;
+            acc = lds_serial_sum(acc, base, cnt);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5 * K) red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        const double vin = red[5 * k], vg = red[5 * k + 1], vj = red[5 * k + 2], vb = red[5 * k + 3],
+                     vf = red[5 * k + 4];
+        double val = vin;
+        val += sc.h2 * vg;
+        val += vj;
+        double cval = 0.0;
+        if (sc.has_barrier) { val += sc.h2 * vb; cval += sc.h2 * vb; }
+        if (sc.has_friction) { val += sc.h2 * vf; cval += sc.h2 * vf; }
+        vals[k] = val;
+        cvals[k] = cval;
+    }
+    __syncthreads();
+}
+
 // (sum_i a0[i] b[i], sum_i a1[i] b[i]) left to right; T: kTermLds doubles of LDS.
 __device__ void serial_dot2(const double* a0, const double* a1, const double* b, int n, double* red, double* o0,
                             double* o1, double* T) {
@@ -709,8 +801,8 @@ __device__ bool pcg_solve(const Dev& d, const Work& w, const ScnCtx& sc, const d
 // k ascending. Entries of other components are exact zeros in that factorization, so a
 // component-wise factorization equals the factorization of the whole matrix. Singletons
 // use the same recurrences on their 6x6 block (one lane each); larger components are
-// factored column by column with the rows of a column in parallel (left-looking, same
-// per-entry order). Components above kDenseBodies bodies fall back to PCG.
+// factored right-looking with the trailing update in parallel (same per-entry order).
+// Components above kDenseBodies bodies fall back to PCG.
 struct CompLds {
     int lab[kMaxSceneBodies];   // component label of each body: its smallest member
     int cnt[kMaxSceneBodies];   // component size, indexed by label
@@ -917,40 +1009,102 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
             }
         }
         __syncthreads();
-        // left-looking LDL^T, column by column
-        for (int j = 0; j < m; ++j) {
-            if (threadIdx.x == 0) {
-                double dj = M[j * m + j];
-                for (int k = 0; k < j; ++k) dj -= M[j * m + k] * M[j * m + k] * M[k * m + k];
-                if (dj == 0.0 || !isfinite(dj)) *ok_sh = 0;
-                M[j * m + j] = dj;
-            }
-            __syncthreads();
-            if (!*ok_sh) return 1;
-            const double Dj = M[j * m + j];
-            for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) {
-                double s = M[i * m + j];
-                for (int k = 0; k < j; ++k) s -= M[i * m + k] * M[k * m + k] * M[j * m + k];
-                M[i * m + j] = s / Dj;
-            }
-            __syncthreads();
-        }
-        // solve: gather rhs, forward (column order), diagonal, backward
-        double* xs = w.r + 6 * sc.b0;  // scratch of this scene, m <= 6 n
-        for (int t = threadIdx.x; t < m; t += blockDim.x) xs[t] = rhs[6 * L.list[c0 + t / 6] + t % 6];
-        __syncthreads();
+        // right-looking LDL^T: at step k, D_k = A_kk, L_ik = A_ik / D_k, then the trailing
+        // update A_ij -= (L_ik D_k) L_jk (A_jj -= (L_jk L_jk) D_k) of the lower triangle.
+        // Every entry receives the same subtractions in the same (k ascending) order as
+        // in the row-oriented recurrences above, so the factor is bitwise theirs. L_ik is
+        // mirrored to the upper triangle (never read by the factorization) so that the
+        // forward substitution reads rows.
         for (int k = 0; k < m; ++k) {
-            const double xk = xs[k];
-            for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) xs[i] -= M[i * m + k] * xk;
+            const double Dk = M[k * m + k];
+            if (Dk == 0.0 || !isfinite(Dk)) return 1;
+            for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) {
+                const double l = M[i * m + k] / Dk;
+                M[i * m + k] = l;
+                M[k * m + i] = l;
+            }
+            __syncthreads();
+            const int r = m - k - 1;
+            for (int t = threadIdx.x; t < r * r; t += blockDim.x) {
+                const int i = k + 1 + t / r, j = k + 1 + t % r;
+                if (j > i) continue;
+                const double lik = M[i * m + k], ljk = M[j * m + k];
+                if (i == j) M[i * m + i] -= (lik * lik) * Dk;
+                else M[i * m + j] -= (lik * Dk) * ljk;
+            }
             __syncthreads();
         }
-        for (int t = threadIdx.x; t < m; t += blockDim.x) xs[t] /= M[t * m + t];
+        // substitutions by the first wavefront, rows lane + 64 q in registers: forward
+        // x_i -= L_ik x_k (k ascending), x_i /= D_i, backward x_k -= L_ik x_i (i
+        // descending), the per-row operation order of the column-oriented substitutions
+        double* xs = w.r + 6 * sc.b0;  // scratch of this scene, m <= 6 n
+        if (threadIdx.x < 64) {
+            constexpr int QR = kDenseRows / 64;
+            const int lane = threadIdx.x;
+            double xr[QR];
+#pragma unroll
+            for (int q = 0; q < QR; ++q) {
+                const int i = lane + 64 * q;
+                xr[q] = i < m ? rhs[6 * L.list[c0 + i / 6] + i % 6] : 0.0;
+            }
+            auto bcast = [&](int k) {
+                double v = 0.0;
+#pragma unroll
+                for (int q = 0; q < QR; ++q)
+                    if ((k >> 6) == q) v = xr[q];
+                return __shfl(v, k & 63);
+            };
+            double cur[QR], nxt[QR];
+#pragma unroll
+            for (int q = 0; q < QR; ++q) {
+                const int i = lane + 64 * q;
+                cur[q] = (i > 0 && i < m) ? M[i] : 0.0;  // row 0 of the mirror: L_i0
+            }
+            for (int k = 0; k < m; ++k) {
+#pragma unroll
+                for (int q = 0; q < QR; ++q) {
+                    const int i = lane + 64 * q;
+                    nxt[q] = (k + 1 < m && i > k + 1 && i < m) ? M[(k + 1) * m + i] : 0.0;
+                }
+                const double xk = bcast(k);
+#pragma unroll
+                for (int q = 0; q < QR; ++q) {
+                    const int i = lane + 64 * q;
+                    if (i > k && i < m) xr[q] -= cur[q] * xk;
+                    cur[q] = nxt[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < QR; ++q) {
+                const int i = lane + 64 * q;
+                if (i < m) xr[q] /= M[i * m + i];
+            }
+#pragma unroll
+            for (int q = 0; q < QR; ++q) {
+                const int k = lane + 64 * q;
+                cur[q] = (k < m - 1) ? M[(m - 1) * m + k] : 0.0;  // row m-1 of L
+            }
+            for (int i = m - 1; i >= 0; --i) {
+#pragma unroll
+                for (int q = 0; q < QR; ++q) {
+                    const int k = lane + 64 * q;
+                    nxt[q] = (i >= 1 && k < i - 1) ? M[(i - 1) * m + k] : 0.0;
+                }
+                const double xi = bcast(i);
+#pragma unroll
+                for (int q = 0; q < QR; ++q) {
+                    const int k = lane + 64 * q;
+                    if (k < i) xr[q] -= cur[q] * xi;
+                    cur[q] = nxt[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < QR; ++q) {
+                const int i = lane + 64 * q;
+                if (i < m) xs[i] = xr[q];
+            }
+        }
         __syncthreads();
-        for (int i = m - 1; i >= 0; --i) {
-            const double xi = xs[i];
-            for (int k = threadIdx.x; k < i; k += blockDim.x) xs[k] -= M[i * m + k] * xi;
-            __syncthreads();
-        }
         for (int t = threadIdx.x; t < m; t += blockDim.x) xo[6 * L.list[c0 + t / 6] + t % 6] = xs[t];
         __syncthreads();
         c0 = c1;
@@ -972,14 +1126,14 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     double* X = w.x + 6 * sc.b0;
     double* G = w.g + 6 * sc.b0;
     double* DX = w.dx + 6 * sc.b0;
-    double* XT = w.xt + 6 * sc.b0;
+    const long long xstride = 6 * (long long)d.N;
+    double* XTB = w.xtb + 6 * sc.b0;  // kLsBatch trial points, xstride apart
     double* CG = w.cg + 6 * sc.b0;
     double* NG = w.tmp + 6 * sc.b0;
     double* b2arr = w.sort_key + (long long)s * kSceneScratch;  // per-contact b'' scratch (<= 2000)
     int conv = 0;
     int it = 0;
     const double* Xg = w.x;
-    const double* XTg = w.xt;
     const int maxit = sp.newton_max_iters;
     for (; it < maxit; ++it) {
         eval_elements(d, w, sc, Xg, b2arr);
@@ -1013,20 +1167,36 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         double descent, cdesc;
         serial_dot2(G, CG, DX, sc.nd, red, &descent, &cdesc, nl.terms);
         if (descent > 0) break;
+        // The trials alpha = 1, 1/2, 1/4, ... (at most 20) are evaluated kLsBatch at a
+        // time; the first that passes is taken, and only the trials up to it count.
         double alpha = 1.0;
         bool accepted = false;
-        for (int ls = 0; ls < 20; ++ls) {
-            for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) XT[i] = X[i] + alpha * DX[i];
-            __syncthreads();
-            double fn, fcn;
-            objective_value(d, w, sc, XTg, red, &fn, &fcn, nl.terms);
-            if (threadIdx.x == 0) {
-                sp.ls_trials += 1;
-                sp.trials_total += 1.0;
+        for (int ls = 0; ls < 20 && !accepted;) {
+            const int K = min(kLsBatch, 20 - ls);
+            for (int t = threadIdx.x; t < K * sc.nd; t += blockDim.x) {
+                const int k = t / sc.nd, i = t % sc.nd;
+                double ak = alpha;
+                for (int q = 0; q < k; ++q) ak *= 0.5;
+                XTB[k * xstride + i] = X[i] + ak * DX[i];
             }
-            const bool cok = (cdesc <= 0.0) ? (fcn <= fc + 1e-4 * alpha * cdesc) : (fcn <= fc);
-            if (fn <= fx + 1e-4 * alpha * descent && cok) { accepted = true; break; }
-            alpha *= 0.5;
+            __syncthreads();
+            double fn[kLsBatch], fcn[kLsBatch];
+            objective_value_batch(d, w, sc, w.xtb, xstride, K, red, fn, fcn, nl.terms);
+            int k = 0;
+            for (; k < K; ++k) {
+                const bool cok = (cdesc <= 0.0) ? (fcn[k] <= fc + 1e-4 * alpha * cdesc) : (fcn[k] <= fc);
+                if (fn[k] <= fx + 1e-4 * alpha * descent && cok) {
+                    accepted = true;
+                    break;
+                }
+                alpha *= 0.5;
+            }
+            const int used = accepted ? k + 1 : K;
+            if (threadIdx.x == 0) {
+                sp.ls_trials += used;
+                sp.trials_total += double(used);
+            }
+            ls += used;
         }
         if (!accepted) break;
         for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) X[i] = X[i] + alpha * DX[i];
