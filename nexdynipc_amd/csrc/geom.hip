@@ -1066,10 +1066,17 @@ void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_a
 // only if it lies in J's box inflated by the same bound for J's longest edge (the
 // projection lies within the triangle enlarged by the 1e-6 barycentric tolerance).
 // Both filters are conservative, so the flag is exact.
+// The per-face quantities (unit normal, the inverse of [e1 e2 n]) are computed once per
+// face, in chunks of kSmfChunk faces, and every (vertex, face) candidate of a pair is one
+// work item: the same expressions as the per-vertex loop, evaluated once.
+constexpr int kSmfChunk = 256;
 __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int ungated) {
-    __shared__ double wvx[kMaxPrims], wvy[kMaxPrims], wvz[kMaxPrims];
+    __shared__ double wvx[kMaxPrims], wvy[kMaxPrims], wvz[kMaxPrims];  // J's vertices, world
+    __shared__ double ivx[kMaxPrims], ivy[kMaxPrims], ivz[kMaxPrims];  // I's candidate vertices
+    __shared__ double fd[kSmfChunk * 15];                              // v0(3) n(3) inverse(9)
+    __shared__ int fok[kSmfChunk];
     __shared__ int act[kMaxPrims];
-    __shared__ int nact;
+    __shared__ int nact, nvi;
     const int total = *w.n_prox_total;
     for (int p = blockIdx.x; p < total; p += gridDim.x) {
         const int s = w.prox_scene[p];
@@ -1080,7 +1087,10 @@ __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int un
         const V3 pI = ld3(d.p, I), pJ = ld3(d.p, J);
         const Q qI = ld4(d.q, I), qJ = ld4(d.q, J);
         const int nvJ = d.nv[J];
-        if (threadIdx.x == 0) nact = 0;
+        if (threadIdx.x == 0) {
+            nact = 0;
+            nvi = 0;
+        }
         for (int k = threadIdx.x; k < nvJ; k += blockDim.x) {
             const V3 v = world_vertex(d, J, k, pJ, qJ);
             wvx[k] = v.x; wvy[k] = v.y; wvz[k] = v.z;
@@ -1098,33 +1108,51 @@ __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int un
             fb.mx = vmax(A, vmax(B, C));
             if (box_overlap(box_inflated(fb, 1e-8 + 1e-5 * L), bI)) act[atomicAdd(&nact, 1)] = f;
         }
-        __syncthreads();
-        const int na = nact;
         const Box3 bJ = box_inflated(ldbox(w.wbox, J), 1e-8 + 1e-5 * d.max_edge[J] + 1e-9);
-        for (int vi = threadIdx.x; vi < d.nv[I] && na > 0; vi += blockDim.x) {
+        for (int vi = threadIdx.x; vi < d.nv[I]; vi += blockDim.x) {
             const V3 pt = world_vertex(d, I, vi, pI, qI);
             if (!(pt.x >= bJ.mn.x && pt.x <= bJ.mx.x && pt.y >= bJ.mn.y && pt.y <= bJ.mx.y && pt.z >= bJ.mn.z &&
                   pt.z <= bJ.mx.z))
                 continue;
-            bool hit = false;
-            for (int fi = 0; fi < na && !hit; ++fi) {
-                const int f = act[fi];
+            const int k = atomicAdd(&nvi, 1);
+            ivx[k] = pt.x; ivy[k] = pt.y; ivz[k] = pt.z;
+        }
+        __syncthreads();
+        const int na = nact, nv = nvi;
+        for (int c0 = 0; c0 < na && nv > 0; c0 += kSmfChunk) {
+            const int nc = min(kSmfChunk, na - c0);
+            for (int fi = threadIdx.x; fi < nc; fi += blockDim.x) {
+                const int f = act[c0 + fi];
                 const int a = F[3 * f], b = F[3 * f + 1], c = F[3 * f + 2];
                 const V3 v0 = v3(wvx[a], wvy[a], wvz[a]);
                 const V3 e1 = v3(wvx[b], wvy[b], wvz[b]) - v0;
                 const V3 e2 = v3(wvx[c], wvy[c], wvz[c]) - v0;
                 V3 n = cross(e1, e2);
                 const double area = norm(n);
-                if (area < 1e-12) continue;
+                fok[fi] = area < 1e-12 ? 0 : 1;
                 n = normalized(n);
-                const V3 r = pt - v0;
-                const double dist = fabs(dot(r, n));
-                if (!(dist < 1e-9)) continue;
                 M3 M;
                 M.m[0] = e1.x; M.m[3] = e1.y; M.m[6] = e1.z;
                 M.m[1] = e2.x; M.m[4] = e2.y; M.m[7] = e2.z;
                 M.m[2] = n.x;  M.m[5] = n.y;  M.m[8] = n.z;
-                const V3 co = m3v(m3inverse(M), r);
+                const M3 Mi = m3inverse(M);
+                double* o = fd + 15 * fi;
+                o[0] = v0.x; o[1] = v0.y; o[2] = v0.z;
+                o[3] = n.x;  o[4] = n.y;  o[5] = n.z;
+                for (int k = 0; k < 9; ++k) o[6 + k] = Mi.m[k];
+            }
+            __syncthreads();
+            bool hit = false;
+            for (int t = threadIdx.x; t < nv * nc && !hit; t += blockDim.x) {
+                const int vi = t / nc, fi = t % nc;
+                if (!fok[fi]) continue;
+                const double* o = fd + 15 * fi;
+                const V3 r = v3(ivx[vi], ivy[vi], ivz[vi]) - v3(o[0], o[1], o[2]);
+                const double dist = fabs(dot(r, v3(o[3], o[4], o[5])));
+                if (!(dist < 1e-9)) continue;
+                M3 Mi;
+                for (int k = 0; k < 9; ++k) Mi.m[k] = o[6 + k];
+                const V3 co = m3v(Mi, r);
                 const double b0 = 1.0 - co.x - co.y, b1 = co.x, b2 = co.y;
                 double bmn = b0, bmx = b0;
                 if (b1 < bmn) bmn = b1;
@@ -1133,7 +1161,10 @@ __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int un
                 if (bmx < b2) bmx = b2;
                 if (bmn >= -1e-6 && bmx <= 1.0 + 1e-6) hit = true;
             }
-            if (hit) atomicOr(&w.flag[s], 1);
+            if (__syncthreads_or(hit)) {
+                if (threadIdx.x == 0) atomicOr(&w.flag[s], 1);
+                break;
+            }
         }
         __syncthreads();
     }
