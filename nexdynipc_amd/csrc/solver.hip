@@ -342,32 +342,45 @@ __device__ double objective_grad(const Dev& d, const Work& w, const ScnCtx& sc, 
 // A parallel tree sum rounds differently, and the Armijo test and the ALM stopping test
 // compare these values in the last bits, so they are summed left to right here: all
 // lanes compute a window of terms into LDS, then one lane per accumulator adds them in
-// order (LDS reads pipelined ahead of the dependent adds), one wavefront per
-// accumulator so that independent accumulators run side by side.
+// order (LDS reads pipelined ahead of the dependent adds); independent accumulators sit
+// on neighbouring lanes of one wavefront and advance in the same instructions.
 static_assert(kBlk == 256, "ordered sums assume four wavefronts per work-group");
 
-// acc + t[0] + t[1] + ... + t[n-1]; the reads of the next 8 terms are in flight while
-// the current 8 are added.
+// acc + t[0] + t[1] + ... + t[n-1], t 16-byte aligned; the 16-byte reads of the next 16
+// terms are in flight while the current 16 are added.
 __device__ __forceinline__ double lds_serial_sum(double acc, const double* t, int n) {
+    const double2* t2 = reinterpret_cast<const double2*>(t);
+    const int n2 = n >> 1;
     int k = 0;
-    if (n >= 8) {
-        double a[8];
+    if (n2 >= 8) {
+        double2 a[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a[q] = t[q];
-        for (; k + 16 <= n; k += 8) {
-            double b[8];
+        for (int q = 0; q < 8; ++q) a[q] = t2[q];
+        for (; k + 16 <= n2; k += 8) {
+            double2 b[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) b[q] = t[k + 8 + q];
+            for (int q = 0; q < 8; ++q) b[q] = t2[k + 8 + q];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) acc += a[q];
+            for (int q = 0; q < 8; ++q) {
+                acc += a[q].x;
+                acc += a[q].y;
+            }
 #pragma unroll
             for (int q = 0; q < 8; ++q) a[q] = b[q];
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc += a[q];
+        for (int q = 0; q < 8; ++q) {
+            acc += a[q].x;
+            acc += a[q].y;
+        }
         k += 8;
     }
-    for (; k < n; ++k) acc += t[k];
+    for (; k < n2; ++k) {
+        const double2 v = t2[k];
+        acc += v.x;
+        acc += v.y;
+    }
+    if (n & 1) acc += t[n - 1];
     return acc;
 }
 
@@ -384,7 +397,7 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
     int rounds = (sc.n + WB - 1) / WB;
     rounds = max(rounds, (nj + WB - 1) / WB);
     rounds = max(rounds, (sc.nc + WB - 1) / WB);
-    double acc0 = 0.0, acc1 = 0.0;  // this lane's accumulators (lanes 0, 64, 128, 192)
+    double acc0 = 0.0;  // lane L < 5: accumulator L (inertia, gravity, joints, barrier, friction)
     for (int r = 0; r < rounds; ++r) {
         const int b_lo = r * WB;
         for (int t = threadIdx.x; t < WB; t += blockDim.x) {
@@ -429,24 +442,17 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
             }
         }
         __syncthreads();
-        const int nb = min(WB, max(0, sc.n - b_lo));
-        if (threadIdx.x == 0) acc0 = lds_serial_sum(acc0, T + OI, 2 * nb);
-        else if (threadIdx.x == 64) acc0 = lds_serial_sum(acc0, T + OG, nb);
-        else if (threadIdx.x == 128) acc0 = lds_serial_sum(acc0, T + OJ, min(WB, max(0, nj - b_lo)));
-        else if (threadIdx.x == 192) {
+        if (threadIdx.x < 5) {  // lanes 0..4 of the first wavefront, side by side
+            const int nb = min(WB, max(0, sc.n - b_lo));
             const int ncw = min(WB, max(0, sc.nc - b_lo));
-            acc0 = lds_serial_sum(acc0, T + OB, ncw);
-            acc1 = lds_serial_sum(acc1, T + OF, ncw);
+            const int lc = threadIdx.x;
+            const double* base = lc == 0 ? T + OI : lc == 1 ? T + OG : lc == 2 ? T + OJ : lc == 3 ? T + OB : T + OF;
+            const int cnt = lc == 0 ? 2 * nb : lc == 1 ? nb : lc == 2 ? min(WB, max(0, nj - b_lo)) : ncw;
+            acc0 = lds_serial_sum(acc0, base, cnt);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) red[0] = acc0;
-    if (threadIdx.x == 64) red[1] = acc0;
-    if (threadIdx.x == 128) red[2] = acc0;
-    if (threadIdx.x == 192) {
-        red[3] = acc0;
-        red[4] = acc1;
-    }
+    if (threadIdx.x < 5) red[threadIdx.x] = acc0;
     __syncthreads();
     const double vin = red[0], vg = red[1], vj = red[2], vb = red[3], vf = red[4];
     __syncthreads();
@@ -470,6 +476,7 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
                                       double* T) {
     constexpr int WB = kTermLds / (6 * kLsBatch);  // items per window and trial
     constexpr int TS = 6 * WB;                      // doubles per trial window
+    static_assert(WB % 2 == 0, "windows start 16-byte aligned");
     constexpr int OI = 0, OG = 2 * WB, OJ = 3 * WB, OB = 4 * WB, OF = 5 * WB;
     const double* XH = w.xhat;
     const int nj = sc.j1 - sc.j0;
@@ -564,12 +571,10 @@ __device__ void serial_dot2(const double* a0, const double* a1, const double* b,
             T[W + t] = a1[lo + t] * b[lo + t];
         }
         __syncthreads();
-        if (threadIdx.x == 0) acc = lds_serial_sum(acc, T, cnt);
-        else if (threadIdx.x == 64) acc = lds_serial_sum(acc, T + W, cnt);
+        if (threadIdx.x < 2) acc = lds_serial_sum(acc, T + threadIdx.x * W, cnt);
         __syncthreads();
     }
-    if (threadIdx.x == 0) red[0] = acc;
-    if (threadIdx.x == 64) red[1] = acc;
+    if (threadIdx.x < 2) red[threadIdx.x] = acc;
     __syncthreads();
     *o0 = red[0];
     *o1 = red[1];
@@ -812,7 +817,7 @@ struct CompLds {
     int scan[kBlk];
     int maxsize, changed, ncoupled, pad;
 };
-union NewtonLds {
+union alignas(16) NewtonLds {  // terms: 16-byte reads in lds_serial_sum
     CompLds comp;
     double terms[kTermLds];
 };
