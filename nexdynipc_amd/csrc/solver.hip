@@ -1256,10 +1256,17 @@ __device__ void build_csr(const Work& w, const ScnCtx& sc) {
         off[li + 1] = c;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        off[0] = 0;
-        for (int li = 0; li < sc.n; ++li) off[li + 1] += off[li];
+    __shared__ int csr_sh[kBlk / 64];
+    int run = 0;
+    for (int base = 0; base < sc.n; base += blockDim.x) {  // counts -> inclusive prefix sums
+        const int li = base + threadIdx.x;
+        const int v = li < sc.n ? off[li + 1] : 0;
+        int ex;
+        const int tot = wg_scan_shfl(v, csr_sh, &ex);
+        if (li < sc.n) off[li + 1] = run + ex + v;
+        run += tot;
     }
+    if (threadIdx.x == 0) off[0] = 0;
     __syncthreads();
     for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
         const int b = sc.b0 + li;
