@@ -325,21 +325,10 @@ __device__ __forceinline__ int wg_scan_shfl(int v, int* sh, int* out_excl) {
     return total;
 }
 
+// Exclusive scan over the work-group (any multiple of 64 threads); sh: at least
+// blockDim.x / 64 ints of LDS. Returns the total.
 __device__ __forceinline__ int wg_exclusive_scan(int v, int* sh, int* out_excl) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-        int add = (t >= off) ? sh[t - off] : 0;
-        __syncthreads();
-        sh[t] += add;
-        __syncthreads();
-    }
-    const int incl = sh[t];
-    const int total = sh[blockDim.x - 1];
-    __syncthreads();
-    *out_excl = incl - v;
-    return total;
+    return wg_scan_shfl(v, sh, out_excl);
 }
 
 }  // namespace nd

@@ -487,17 +487,20 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
     const int lk = threadIdx.x / 5, lc = threadIdx.x % 5;
     for (int r = 0; r < rounds; ++r) {
         const int b_lo = r * WB;
-        for (int t = threadIdx.x; t < K * WB; t += blockDim.x) {
-            const int k = t / WB, tt = t % WB;
-            const double* X = XTB + k * stride;
-            double* Tk = T + k * TS;
+        // inertia and gravity: one lane per body, its K trial points in registers
+        for (int tt = threadIdx.x; tt < WB; tt += blockDim.x) {
             const int li = b_lo + tt;
-            if (li < sc.n) {
-                const int b = sc.b0 + li;
-                const double m = d.mass[b];
-                const double* I = d.inertia + 9 * b;
-                const V3 dp = xpos(X, b) - xpos(XH, b);
-                const V3 dt = xrot(X, b) - xrot(XH, b);
+            if (li >= sc.n) break;
+            const int b = sc.b0 + li;
+            const double m = d.mass[b];
+            double I[9], xh[6];
+            for (int q = 0; q < 9; ++q) I[q] = d.inertia[9 * b + q];
+            for (int q = 0; q < 6; ++q) xh[q] = XH[6 * b + q];
+            for (int k = 0; k < K; ++k) {
+                const double* X = XTB + k * stride;
+                double* Tk = T + k * TS;
+                const V3 dp = xpos(X, b) - v3(xh[0], xh[1], xh[2]);
+                const V3 dt = xrot(X, b) - v3(xh[3], xh[4], xh[5]);
                 const V3 rr = 0.5 * dt;
                 const double s0 = rr.x * I[0] + rr.y * I[3] + rr.z * I[6];
                 const double s1 = rr.x * I[1] + rr.y * I[4] + rr.z * I[7];
@@ -506,6 +509,12 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
                 Tk[OI + 2 * tt + 1] = s0 * dt.x + s1 * dt.y + s2 * dt.z;
                 Tk[OG + tt] = -(m * (0.0 * X[6 * b] + kG * X[6 * b + 1] + 0.0 * X[6 * b + 2]));
             }
+        }
+        for (int t = threadIdx.x; t < K * WB; t += blockDim.x) {
+            const int k = t / WB, tt = t % WB;
+            const double* X = XTB + k * stride;
+            double* Tk = T + k * TS;
+            const int li = b_lo + tt;
             if (li < nj) {
                 const int j = sc.j0 + li;
                 JEval e;
@@ -1178,11 +1187,13 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         bool accepted = false;
         for (int ls = 0; ls < 20 && !accepted;) {
             const int K = min(kLsBatch, 20 - ls);
-            for (int t = threadIdx.x; t < K * sc.nd; t += blockDim.x) {
-                const int k = t / sc.nd, i = t % sc.nd;
+            for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) {
+                const double xi = X[i], di = DX[i];
                 double ak = alpha;
-                for (int q = 0; q < k; ++q) ak *= 0.5;
-                XTB[k * xstride + i] = X[i] + ak * DX[i];
+                for (int k = 0; k < K; ++k) {
+                    XTB[k * xstride + i] = xi + ak * di;
+                    ak *= 0.5;
+                }
             }
             __syncthreads();
             double fn[kLsBatch], fcn[kLsBatch];
