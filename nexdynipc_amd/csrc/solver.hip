@@ -467,13 +467,14 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
 }
 
 // Objective and contact values at K trial points XT_k = X + alpha_k DX (k < K <=
-// kLsBatch, XT_k at XTB + k * stride, global body indexing): the line search's next K
-// step lengths evaluated together. Every value is summed exactly as in objective_value
-// (same terms, same order, from +0.0); the 5 K accumulators run on lanes 0..5K-1 of
-// the first wavefront, side by side, so K trials cost about one trial's serial chain.
+// kLsBatch, alpha_k = alpha / 2^k, XT_k at XTB + k * stride; X0, DX0, XTB global body
+// indexing): the line search's next K step lengths evaluated together. Every value is
+// summed exactly as in objective_value (same terms, same order, from +0.0); the 5 K
+// accumulators run on lanes 0..5K-1 of the first wavefront, side by side, so K trials
+// cost about one trial's serial chain.
 __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx& sc, const double* XTB,
-                                      long long stride, int K, double* red, double* vals, double* cvals,
-                                      double* T) {
+                                      long long stride, int K, const double* X0, const double* DX0, double alpha,
+                                      double* red, double* vals, double* cvals, double* T) {
     constexpr int WB = kTermLds / (6 * kLsBatch);  // items per window and trial
     constexpr int TS = 6 * WB;                      // doubles per trial window
     static_assert(WB % 2 == 0, "windows start 16-byte aligned");
@@ -487,27 +488,38 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
     const int lk = threadIdx.x / 5, lc = threadIdx.x % 5;
     for (int r = 0; r < rounds; ++r) {
         const int b_lo = r * WB;
-        // inertia and gravity: one lane per body, its K trial points in registers
+        // inertia and gravity: one lane per body, its K trial points formed in registers
+        // (x + alpha_k dx, the expression that filled XTB)
         for (int tt = threadIdx.x; tt < WB; tt += blockDim.x) {
             const int li = b_lo + tt;
             if (li >= sc.n) break;
             const int b = sc.b0 + li;
             const double m = d.mass[b];
-            double I[9], xh[6];
+            double I[9], xh[6], x0[6], dx[6];
             for (int q = 0; q < 9; ++q) I[q] = d.inertia[9 * b + q];
-            for (int q = 0; q < 6; ++q) xh[q] = XH[6 * b + q];
-            for (int k = 0; k < K; ++k) {
-                const double* X = XTB + k * stride;
-                double* Tk = T + k * TS;
-                const V3 dp = xpos(X, b) - v3(xh[0], xh[1], xh[2]);
-                const V3 dt = xrot(X, b) - v3(xh[3], xh[4], xh[5]);
-                const V3 rr = 0.5 * dt;
-                const double s0 = rr.x * I[0] + rr.y * I[3] + rr.z * I[6];
-                const double s1 = rr.x * I[1] + rr.y * I[4] + rr.z * I[7];
-                const double s2 = rr.x * I[2] + rr.y * I[5] + rr.z * I[8];
-                Tk[OI + 2 * tt] = 0.5 * m * sqnorm(dp);
-                Tk[OI + 2 * tt + 1] = s0 * dt.x + s1 * dt.y + s2 * dt.z;
-                Tk[OG + tt] = -(m * (0.0 * X[6 * b] + kG * X[6 * b + 1] + 0.0 * X[6 * b + 2]));
+            for (int q = 0; q < 6; ++q) {
+                xh[q] = XH[6 * b + q];
+                x0[q] = X0[6 * b + q];
+                dx[q] = DX0[6 * b + q];
+            }
+            double ak = alpha;
+#pragma unroll
+            for (int k = 0; k < kLsBatch; ++k) {
+                if (k < K) {
+                    double X[6];
+                    for (int q = 0; q < 6; ++q) X[q] = x0[q] + ak * dx[q];
+                    double* Tk = T + k * TS;
+                    const V3 dp = v3(X[0], X[1], X[2]) - v3(xh[0], xh[1], xh[2]);
+                    const V3 dt = v3(X[3], X[4], X[5]) - v3(xh[3], xh[4], xh[5]);
+                    const V3 rr = 0.5 * dt;
+                    const double s0 = rr.x * I[0] + rr.y * I[3] + rr.z * I[6];
+                    const double s1 = rr.x * I[1] + rr.y * I[4] + rr.z * I[7];
+                    const double s2 = rr.x * I[2] + rr.y * I[5] + rr.z * I[8];
+                    Tk[OI + 2 * tt] = 0.5 * m * sqnorm(dp);
+                    Tk[OI + 2 * tt + 1] = s0 * dt.x + s1 * dt.y + s2 * dt.z;
+                    Tk[OG + tt] = -(m * (0.0 * X[0] + kG * X[1] + 0.0 * X[2]));
+                }
+                ak *= 0.5;
             }
         }
         for (int t = threadIdx.x; t < K * WB; t += blockDim.x) {
@@ -985,44 +997,62 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
                 for (int q = 0; q < 6; ++q) M[(6 * t + r) * m + 6 * t + q] = D[r * 6 + q];
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            // lower-triangle off-diagonal blocks: hinges, then barrier, then friction
-            for (int j = sc.j0; j < sc.j1; ++j) {
-                if (d.jtype[j] != 1) continue;
-                const int a = d.ja[j] - sc.b0, b = d.jb[j] - sc.b0;
-                if (L.lab[a] != c) continue;
-                const int pa = lpos(a), pb = lpos(b);
-                const double* jb = w.jblk + 108 * j + 72;  // rows a, cols b
-                for (int r = 0; r < 6; ++r)
-                    for (int q = 0; q < 6; ++q) {
-                        if (pa > pb) M[(6 * pa + r) * m + 6 * pb + q] += jb[r * 6 + q];
-                        else M[(6 * pb + r) * m + 6 * pa + q] += jb[q * 6 + r];
-                    }
-            }
-            for (int pass = 0; pass < 2; ++pass) {
-                for (int i = 0; i < sc.nc; ++i) {
-                    const long long cc = sc.cb + i, ce = (long long)sc.s * kMaxContacts + i;
-                    const int act = w.cactive[ce];
-                    if (!(act & (pass == 0 ? 1 : 2))) continue;
-                    const int a = w.cint[5 * cc] - sc.b0, b = w.cint[5 * cc + 1] - sc.b0;
-                    if (L.lab[a] != c) continue;
-                    const int pa = lpos(a), pb = lpos(b);
-                    const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
-                    if (pass == 0) {
-                        const double* cd = w.cdbl + 7 * cc;
-                        const double n[3] = {cd[3], cd[4], cd[5]};
-                        for (int r = 0; r < 3; ++r)
-                            for (int q = 0; q < 3; ++q)
-                                M[(6 * hi + r) * m + 6 * lo + q] += (-(b2arr[i] * (n[r] * n[q]))) * sc.h2;
+        // lower-triangle off-diagonal blocks: hinges, then barrier, then friction contacts,
+        // each in element order. The component's elements are compacted in that order by
+        // work-group scans; then one element at a time adds its block entries in parallel.
+        int* el = w.sort_idx + (long long)sc.s * kSceneScratch;
+        int ne = 0;
+        for (int kind = 0; kind < 3; ++kind) {
+            const int cnt = kind == 0 ? sc.j1 - sc.j0 : sc.nc;
+            for (int base = 0; base < cnt; base += blockDim.x) {
+                const int i = base + threadIdx.x;
+                int f = 0;
+                if (i < cnt) {
+                    if (kind == 0) {
+                        const int j = sc.j0 + i;
+                        f = d.jtype[j] == 1 && L.lab[d.ja[j] - sc.b0] == c;
                     } else {
-                        const double* blk = w.cblk + 144 * ce + (pa > pb ? 72 : 108);
-                        for (int r = 0; r < 6; ++r)
-                            for (int q = 0; q < 6; ++q) M[(6 * hi + r) * m + 6 * lo + q] += blk[r * 6 + q];
+                        const int act = w.cactive[(long long)sc.s * kMaxContacts + i];
+                        f = (act & (kind == 1 ? 1 : 2)) && L.lab[w.cint[5 * (sc.cb + i)] - sc.b0] == c;
                     }
                 }
+                int ex;
+                const int tot = wg_exclusive_scan(f, L.scan, &ex);
+                if (f) el[ne + ex] = (kind << 24) | i;
+                ne += tot;
             }
         }
         __syncthreads();
+        for (int e = 0; e < ne; ++e) {
+            const int code = el[e], kind = code >> 24, i = code & 0xffffff;
+            const int t = threadIdx.x;
+            if (kind == 0) {
+                const int j = sc.j0 + i;
+                const int pa = lpos(d.ja[j] - sc.b0), pb = lpos(d.jb[j] - sc.b0);
+                const double* jb = w.jblk + 108 * j + 72;  // rows a, cols b
+                if (t < 36) {
+                    const int r = t / 6, q = t % 6;
+                    if (pa > pb) M[(6 * pa + r) * m + 6 * pb + q] += jb[r * 6 + q];
+                    else M[(6 * pb + r) * m + 6 * pa + q] += jb[q * 6 + r];
+                }
+            } else {
+                const long long cc = sc.cb + i, ce = (long long)sc.s * kMaxContacts + i;
+                const int pa = lpos(w.cint[5 * cc] - sc.b0), pb = lpos(w.cint[5 * cc + 1] - sc.b0);
+                const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
+                if (kind == 1) {
+                    if (t < 9) {
+                        const int r = t / 3, q = t % 3;
+                        const double* cd = w.cdbl + 7 * cc;
+                        M[(6 * hi + r) * m + 6 * lo + q] += (-(b2arr[i] * (cd[3 + r] * cd[3 + q]))) * sc.h2;
+                    }
+                } else if (t < 36) {
+                    const int r = t / 6, q = t % 6;
+                    const double* blk = w.cblk + 144 * ce + (pa > pb ? 72 : 108);
+                    M[(6 * hi + r) * m + 6 * lo + q] += blk[r * 6 + q];
+                }
+            }
+            __syncthreads();
+        }
         // right-looking LDL^T: at step k, D_k = A_kk, L_ik = A_ik / D_k, then the trailing
         // update A_ij -= (L_ik D_k) L_jk (A_jj -= (L_jk L_jk) D_k) of the lower triangle.
         // Every entry receives the same subtractions in the same (k ascending) order as
@@ -1197,7 +1227,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
             }
             __syncthreads();
             double fn[kLsBatch], fcn[kLsBatch];
-            objective_value_batch(d, w, sc, w.xtb, xstride, K, red, fn, fcn, nl.terms);
+            objective_value_batch(d, w, sc, w.xtb, xstride, K, w.x, w.dx, alpha, red, fn, fcn, nl.terms);
             int k = 0;
             for (; k < K; ++k) {
                 const bool cok = (cdesc <= 0.0) ? (fcn[k] <= fc + 1e-4 * alpha * cdesc) : (fcn[k] <= fc);
