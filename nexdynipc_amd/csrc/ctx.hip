@@ -453,7 +453,6 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     w.g = dalloc<double>(ctx, V6);
     w.dx = dalloc<double>(ctx, V6);
     w.xt = dalloc<double>(ctx, V6);
-    w.xtb = dalloc<double>(ctx, size_t(nd::kLsBatch) * V6);
     w.cg = dalloc<double>(ctx, V6);
     w.r = dalloc<double>(ctx, V6);
     w.z = dalloc<double>(ctx, V6);

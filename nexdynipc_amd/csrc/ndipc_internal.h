@@ -128,7 +128,6 @@ struct Work {
     double* sort_key;        // [S * kSceneScratch] sort keys / Newton scratch
     // solver vectors, 6 per body
     double *x, *xhat, *xprev, *vprev, *g, *dx, *xt, *cg, *r, *z, *pv, *Ap, *tmp;
-    double* xtb;             // [kLsBatch][6N] line-search trial points
     double* D;               // [N*36]
     double* Dinv;            // [N*36]
     double* nf;              // [S*2000] normal forces

@@ -47,13 +47,17 @@ struct ScnCtx {
 __device__ __forceinline__ V3 xpos(const double* x, int b) { return v3(x[6 * b], x[6 * b + 1], x[6 * b + 2]); }
 __device__ __forceinline__ V3 xrot(const double* x, int b) { return v3(x[6 * b + 3], x[6 * b + 4], x[6 * b + 5]); }
 
-// DistanceBarrierForm::contactDistance (DistanceBarrierForm.cpp:89-98)
-__device__ __forceinline__ double contact_dist(const Work& w, long long c, const double* X) {
-    const int A = w.cint[5 * c], B = w.cint[5 * c + 1];
+// DistanceBarrierForm::contactDistance (DistanceBarrierForm.cpp:89-98); xa, xb: the
+// Newton variables of the contact's two bodies.
+__device__ __forceinline__ double contact_dist_x(const Work& w, long long c, const double* xa, const double* xb) {
     const double* cd = w.cdbl + 7 * c;
     const V3 n = v3(cd[3], cd[4], cd[5]);
-    const double pr = dot(xpos(X, B) - xpos(X, A), n);
+    const double pr = dot(v3(xb[0], xb[1], xb[2]) - v3(xa[0], xa[1], xa[2]), n);
     return pr > 1e-9 ? pr : 1e-9;
+}
+__device__ __forceinline__ double contact_dist(const Work& w, long long c, const double* X) {
+    const int A = w.cint[5 * c], B = w.cint[5 * c + 1];
+    return contact_dist_x(w, c, X + 6 * A, X + 6 * B);
 }
 
 // FrictionForm::computeTangentDisplacement + Jacobians (FrictionForm.cpp:44-204)
@@ -97,6 +101,35 @@ __device__ void fric_kin(const Dev& d, const Work& w, long long c, const double*
             k.JA[i][3 + j] = PA.m[3 * i + j];
             k.JB[i][3 + j] = -PB.m[3 * i + j];
         }
+}
+// fric_kin's tangential displacement u alone (the friction value needs no Jacobian),
+// from the Newton variables xa, xb of the two bodies: the same expressions.
+__device__ V3 fric_u_x(const Dev& d, const Work& w, long long c, const double* xa, const double* xb, double dt) {
+    const int A = w.cint[5 * c], B = w.cint[5 * c + 1];
+    const double* cd = w.cdbl + 7 * c;
+    const V3 pt = v3(cd[0], cd[1], cd[2]);
+    const V3 nr = v3(cd[3], cd[4], cd[5]);
+    const V3 pA0 = ld3(d.p, A), pB0 = ld3(d.p, B);
+    const Q qA0 = ld4(d.q, A), qB0 = ld4(d.q, B);
+    const V3 pA = v3(xa[0], xa[1], xa[2]), pB = v3(xb[0], xb[1], xb[2]);
+    const V3 thA = v3(xa[3], xa[4], xa[5]), thB = v3(xb[3], xb[4], xb[5]);
+    const Q qA = apply_theta(qA0, thA), qB = apply_theta(qB0, thB);
+    const V3 dA = pA - pA0, dB = pB - pB0;
+    const V3 lA = rotate(qconj(qA0), pt - pA0), lB = rotate(qconj(qB0), pt - pB0);
+    const V3 rA = rotate(qA, lA), rB = rotate(qB, lB);
+    V3 wA = v3(0, 0, 0), wB = v3(0, 0, 0), vA = v3(0, 0, 0), vB = v3(0, 0, 0);
+    if (dt > 1e-12) {
+        wA = thA / dt;
+        wB = thB / dt;
+        vA = dA / dt;
+        vB = dB / dt;
+    }
+    vA = vA + cross(wA, rA);
+    vB = vB + cross(wB, rB);
+    const V3 rel = vB - vA;
+    const V3 nn = (sqnorm(nr) > 1e-16) ? normalized(nr) : v3(0.0, 1.0, 0.0);
+    const V3 tv = rel - dot(rel, nn) * nn;
+    return tv * dt;
 }
 // FrictionPotential (Friction.cpp:5-62)
 __device__ __forceinline__ double fric_v(V3 u, double N, double mu, double eps) {
@@ -182,6 +215,38 @@ __device__ void joint_eval(const Dev& d, int j, const double* X, JEval& e) {
             e.J[3 + i][3 + k] = dA.m[3 * i + k];
             e.J[3 + i][9 + k] = dB.m[3 * i + k];
         }
+}
+
+// The constraint alone (the value and the ALM updates need no Jacobian): the same
+// expressions as joint_eval's C. xa, xb: the Newton variables of bodies A and B (xb
+// unused by a fixed joint).
+__device__ void joint_C_x(const Dev& d, int j, const double* xa, const double* xb, double* C) {
+    const int type = d.jtype[j];
+    const int A = d.ja[j], B = d.jb[j];
+    const double* jd = d.jdata + 16 * j;
+    if (type == 0) {
+        const V3 tp = v3(jd[0], jd[1], jd[2]);
+        const Q tq = quat(jd[3], jd[4], jd[5], jd[6]);
+        const V3 dp = v3(xa[0], xa[1], xa[2]) - tp;
+        const V3 tt = rotation_vector(qmul(tq, qconj(ld4(d.q, A))));
+        const V3 e3 = v3(xa[3], xa[4], xa[5]) - tt;
+        C[0] = dp.x; C[1] = dp.y; C[2] = dp.z; C[3] = e3.x; C[4] = e3.y; C[5] = e3.z;
+        return;
+    }
+    const V3 anA = v3(jd[0], jd[1], jd[2]), anB = v3(jd[3], jd[4], jd[5]);
+    const V3 axA = v3(jd[6], jd[7], jd[8]), axB = v3(jd[9], jd[10], jd[11]);
+    const V3 pA = v3(xa[0], xa[1], xa[2]), pB = v3(xb[0], xb[1], xb[2]);
+    const Q qA = update_rotation(ld4(d.q, A), v3(xa[3], xa[4], xa[5]));
+    const Q qB = update_rotation(ld4(d.q, B), v3(xb[3], xb[4], xb[5]));
+    const V3 rA = rotate(qA, anA), rB = rotate(qB, anB);
+    const V3 pv = (pA + rA) - (pB + rB);
+    const V3 nA = rotate(qA, axA), nB = rotate(qB, axB);
+    const V3 ac = cross(nA, nB);
+    C[0] = pv.x; C[1] = pv.y; C[2] = pv.z; C[3] = ac.x; C[4] = ac.y; C[5] = ac.z;
+}
+__device__ void joint_C(const Dev& d, int j, const double* X, double* C) {
+    const int A = d.ja[j], B = d.jb[j];
+    joint_C_x(d, j, X + 6 * A, B >= 0 ? X + 6 * B : nullptr, C);
 }
 
 // ------------------------------------------------------------------ helpers
@@ -419,11 +484,11 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
             }
             if (li < nj) {  // ConstraintForm::value: lambda.C + mu/2 |C|^2 per joint
                 const int j = sc.j0 + li;
-                JEval e;
-                joint_eval(d, j, X, e);
+                double C[6];
+                joint_C(d, j, X, C);
                 double t1 = 0.0, sq = 0.0;
-                for (int k = 0; k < 6; ++k) t1 += d.jlambda[6 * j + k] * e.C[k];
-                for (int k = 0; k < 6; ++k) sq += e.C[k] * e.C[k];
+                for (int k = 0; k < 6; ++k) t1 += d.jlambda[6 * j + k] * C[k];
+                for (int k = 0; k < 6; ++k) sq += C[k] * C[k];
                 T[OJ + t] = t1 + 0.5 * d.jmu[j] * sq;
             }
             if (li < sc.nc) {  // DistanceBarrierForm::value, FrictionForm::value
@@ -466,15 +531,16 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
     *out_cval = cval;
 }
 
-// Objective and contact values at K trial points XT_k = X + alpha_k DX (k < K <=
-// kLsBatch, alpha_k = alpha / 2^k, XT_k at XTB + k * stride; X0, DX0, XTB global body
-// indexing): the line search's next K step lengths evaluated together. Every value is
+// Objective and contact values at K trial points X0 + alpha_k DX0 (k < K <= kLsBatch,
+// alpha_k = alpha / 2^k; X0, DX0 global body indexing): the line search's next K step
+// lengths evaluated together. Each element's lane forms its bodies' trial points in
+// registers with the same expression as the line search's x + alpha dx. Every value is
 // summed exactly as in objective_value (same terms, same order, from +0.0); the 5 K
 // accumulators run on lanes 0..5K-1 of the first wavefront, side by side, so K trials
 // cost about one trial's serial chain.
-__device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx& sc, const double* XTB,
-                                      long long stride, int K, const double* X0, const double* DX0, double alpha,
-                                      double* red, double* vals, double* cvals, double* T) {
+__device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx& sc, int K, const double* X0,
+                                      const double* DX0, double alpha, double* red, double* vals, double* cvals,
+                                      double* T) {
     constexpr int WB = kTermLds / (6 * kLsBatch);  // items per window and trial
     constexpr int TS = 6 * WB;                      // doubles per trial window
     static_assert(WB % 2 == 0, "windows start 16-byte aligned");
@@ -522,31 +588,42 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
                 ak *= 0.5;
             }
         }
+        // joints and contacts: one lane per (trial, element), the trial point of the
+        // element's bodies formed in registers
         for (int t = threadIdx.x; t < K * WB; t += blockDim.x) {
             const int k = t / WB, tt = t % WB;
-            const double* X = XTB + k * stride;
-            double* Tk = T + k * TS;
             const int li = b_lo + tt;
+            if (li >= nj && li >= sc.nc) continue;
+            double ak = alpha;
+            for (int q = 0; q < k; ++q) ak *= 0.5;
+            double* Tk = T + k * TS;
             if (li < nj) {
                 const int j = sc.j0 + li;
-                JEval e;
-                joint_eval(d, j, X, e);
+                const int A = d.ja[j], B = d.jb[j];
+                double xa[6], xb[6], C[6];
+                for (int q = 0; q < 6; ++q) {
+                    xa[q] = X0[6 * A + q] + ak * DX0[6 * A + q];
+                    xb[q] = B >= 0 ? X0[6 * B + q] + ak * DX0[6 * B + q] : 0.0;
+                }
+                joint_C_x(d, j, xa, xb, C);
                 double t1 = 0.0, sq = 0.0;
-                for (int q = 0; q < 6; ++q) t1 += d.jlambda[6 * j + q] * e.C[q];
-                for (int q = 0; q < 6; ++q) sq += e.C[q] * e.C[q];
+                for (int q = 0; q < 6; ++q) t1 += d.jlambda[6 * j + q] * C[q];
+                for (int q = 0; q < 6; ++q) sq += C[q] * C[q];
                 Tk[OJ + tt] = t1 + 0.5 * d.jmu[j] * sq;
             }
             if (li < sc.nc) {
                 const long long c = sc.cb + li;
-                Tk[OB + tt] = sc.has_barrier ? bar_v(contact_dist(w, c, X), sc.dhat, sc.kappa) : 0.0;
+                const int A = w.cint[5 * c], B = w.cint[5 * c + 1];
+                double xa[6], xb[6];
+                for (int q = 0; q < 6; ++q) {
+                    xa[q] = X0[6 * A + q] + ak * DX0[6 * A + q];
+                    xb[q] = X0[6 * B + q] + ak * DX0[6 * B + q];
+                }
+                Tk[OB + tt] = sc.has_barrier ? bar_v(contact_dist_x(w, c, xa, xb), sc.dhat, sc.kappa) : 0.0;
                 double fv = 0.0;
                 if (sc.has_friction && li < sc.nfc) {
                     const double N = w.nf[(long long)sc.s * kMaxContacts + li];
-                    if (N > 0.0) {
-                        FKin kk;
-                        fric_kin(d, w, c, X, sc.fdt, kk);
-                        fv = fric_v(kk.u, N, sc.fmu, sc.feps);
-                    }
+                    if (N > 0.0) fv = fric_v(fric_u_x(d, w, c, xa, xb, sc.fdt), N, sc.fmu, sc.feps);
                 }
                 Tk[OF + tt] = fv;
             }
@@ -1170,8 +1247,6 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     double* X = w.x + 6 * sc.b0;
     double* G = w.g + 6 * sc.b0;
     double* DX = w.dx + 6 * sc.b0;
-    const long long xstride = 6 * (long long)d.N;
-    double* XTB = w.xtb + 6 * sc.b0;  // kLsBatch trial points, xstride apart
     double* CG = w.cg + 6 * sc.b0;
     double* NG = w.tmp + 6 * sc.b0;
     double* b2arr = w.sort_key + (long long)s * kSceneScratch;  // per-contact b'' scratch (<= 2000)
@@ -1217,17 +1292,8 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         bool accepted = false;
         for (int ls = 0; ls < 20 && !accepted;) {
             const int K = min(kLsBatch, 20 - ls);
-            for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) {
-                const double xi = X[i], di = DX[i];
-                double ak = alpha;
-                for (int k = 0; k < K; ++k) {
-                    XTB[k * xstride + i] = xi + ak * di;
-                    ak *= 0.5;
-                }
-            }
-            __syncthreads();
             double fn[kLsBatch], fcn[kLsBatch];
-            objective_value_batch(d, w, sc, w.xtb, xstride, K, w.x, w.dx, alpha, red, fn, fcn, nl.terms);
+            objective_value_batch(d, w, sc, K, w.x, w.dx, alpha, red, fn, fcn, nl.terms);
             int k = 0;
             for (; k < K; ++k) {
                 const bool cok = (cdesc <= 0.0) ? (fcn[k] <= fc + 1e-4 * alpha * cdesc) : (fcn[k] <= fc);
@@ -1253,10 +1319,10 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     // lambda += mu C with the hardened mu.
     double viol = 0.0, dual = 0.0;
     for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
-        JEval e;
-        joint_eval(d, j, Xg, e);
+        double C[6];
+        joint_C(d, j, Xg, C);
         double inf = 0.0;
-        for (int k = 0; k < 6; ++k) inf = dmax(inf, fabs(e.C[k]));
+        for (int k = 0; k < 6; ++k) inf = dmax(inf, fabs(C[k]));
         viol = dmax(viol, inf);
         const double dc = d.jmu[j] * inf;
         dual = isfinite(dc) ? dmax(dual, dmin(dc, 1e12)) : dmax(dual, 1e12);
@@ -1264,7 +1330,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
             const double nm = d.jmu[j] * sp.alm_ratio;
             d.jmu[j] = isfinite(nm) ? dmin(nm, 1e12) : 1e12;
         }
-        for (int k = 0; k < 6; ++k) d.jlambda[6 * j + k] += d.jmu[j] * e.C[k];
+        for (int k = 0; k < 6; ++k) d.jlambda[6 * j + k] += d.jmu[j] * C[k];
     }
     viol = wg_reduce(viol, red, OpMax());
     dual = wg_reduce(dual, red, OpMax());
