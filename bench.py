@@ -71,14 +71,19 @@ def load_traffic(kernel):
     return 1024.0 * (k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"])
 
 
-def run_plain(Engine, torch, sc, steps, warmup):
+def run_plain(Engine, torch, sc, steps, warmup, timing=False, graph=True):
+    """steps timed steps after warmup. With `graph` each step replays the captured step
+    graph, else (or with `timing`, where HIP events bracket each kernel) the kernels are
+    enqueued one by one."""
     e = Engine(0)
     e.upload(sc)
+    e.use_graph(graph)
     e.step(warmup)
     torch.cuda.synchronize()
     ev0 = np.array(e.eval_counter())
     c0 = e.solver_counters()
-    e.timing_enable(True)
+    if timing:
+        e.timing_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e.step(steps)
@@ -88,8 +93,9 @@ def run_plain(Engine, torch, sc, steps, warmup):
     ev1 = np.array(e.eval_counter())
     c1 = e.solver_counters()
     stats = e.stats()
-    kern = {name: e.timing_get(name) for name in TIMED}
-    e.timing_enable(False)
+    kern = {name: e.timing_get(name) for name in TIMED} if timing else None
+    if timing:
+        e.timing_enable(False)
     pairs = e.pair_count()
     e.close()
     return dict(T=T, evals=float(np.sum(ev1 - ev0)), stats=stats, kern=kern, pairs=pairs,
@@ -225,26 +231,36 @@ def main():
         # scenes, each on its own engine, scene-steps/s = steps / (sum of wall times)
         rng = np.random.default_rng(7)
         pick = rng.choice(len(specs), size=min(args.one_by_one, len(specs)), replace=False)
-        t_sum, st_sum = 0.0, 0
-        for i in pick:
-            r1 = run_plain(Engine, torch, scenes.pack([specs[int(i)]]), args.batch_steps // 5, args.warmup)
-            t_sum += r1["T"]
-            st_sum += args.batch_steps // 5
-        one_rate = st_sum / t_sum if t_sum > 0 else None
+        one = {}
+        for graph in (False, True):
+            t_sum, st_sum = 0.0, 0
+            for i in pick:
+                r1 = run_plain(Engine, torch, scenes.pack([specs[int(i)]]), args.batch_steps // 5, args.warmup,
+                               graph=graph)
+                t_sum += r1["T"]
+                st_sum += args.batch_steps // 5
+            one[graph] = st_sum / t_sum if t_sum > 0 else None
+        one_rate, one_rate_graph = one[False], one[True]
         batch = {"workload": "c3: 256 scenes, log-U[4,128] bodies each (cube / icosphere s2 / 2x1x0.5 box), "
                              "ground box, dt 1e-2", "steps": args.batch_steps,
                  "scene_steps_per_s": batch_rate, "ms_per_step": 1e3 * rb["T"] / args.batch_steps,
                  "contact_pairs_per_s": rb["evals"] / rb["T"],
                  "one_by_one_scene_steps_per_s": one_rate,
-                 "one_by_one_sample": f"{len(pick)} of {len(specs)} scenes, {args.batch_steps // 5} steps each",
-                 "speedup_vs_one_by_one": batch_rate / one_rate if one_rate else None}
+                 "one_by_one_sample": f"{len(pick)} of {len(specs)} scenes, {args.batch_steps // 5} steps each, "
+                                      "each scene on its own context through the single-scene path",
+                 "speedup_vs_one_by_one": batch_rate / one_rate if one_rate else None,
+                 "one_by_one_graph_scene_steps_per_s": one_rate_graph,
+                 "speedup_vs_one_by_one_graph": batch_rate / one_rate_graph if one_rate_graph else None,
+                 "note": "batch: one launch per stage for all scenes, step replayed from a HIP graph; "
+                         "one_by_one: the plain single-scene call, kernels enqueued per step; "
+                         "one_by_one_graph: the single-scene call with the step graph as well"}
         if args.batch:
             out = {"metric": METRIC, "value": batch_rate, "unit": "scene-steps/s", "n_gpus": 1,
                    "steps": args.batch_steps, "warmup": args.warmup, "ms_per_step": batch["ms_per_step"],
                    "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
                    "data": "synthetic", "config": {"workload": batch["workload"], "scenes": int(scb.n_scenes),
                                                     "bodies": int(scb.n_bodies)},
-                   "batch": batch, "roofline": roofline_for("newton", rb, scb), "cpu_baseline": None}
+                   "batch": batch, "roofline": None, "cpu_baseline": None}
             print(json.dumps(out))
             return
 
@@ -278,9 +294,11 @@ def main():
     r = run_plain(Engine, torch, sc, args.steps, args.warmup)
     T = r["T"]
     stats = r["stats"]
-    totals = {k: v[0] * v[1] for k, v in r["kern"].items()}
+    # the same steps again, kernels enqueued one by one between HIP events
+    rt = run_plain(Engine, torch, sc, args.steps, args.warmup, timing=True)
+    totals = {k: v[0] * v[1] for k, v in rt["kern"].items()}
     dominant = max(totals, key=totals.get)
-    roof = roofline_for(dominant, r, sc)
+    roof = roofline_for(dominant, rt, sc)
     if roof is not None:
         roof["share_of_timed_kernels"] = totals[dominant] / max(sum(totals.values()), 1e-30)
     cpu = None
@@ -312,8 +330,9 @@ def main():
         "newton_iterations": r["newton_iters"],
         "line_search_trials": r["trials"],
         "kernel_total_ms": totals,
+        "ms_per_step_kernels_one_by_one": 1e3 * rt["T"] / args.steps,
         "roofline": roof,
-        "roofline_narrow_phase": roofline_for("np_count", r, sc),
+        "roofline_narrow_phase": roofline_for("np_count", rt, sc),
         "batch": batch,
         "adjoint": adjoint,
         "cpu_baseline": cpu,

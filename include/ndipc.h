@@ -201,8 +201,13 @@ int32_t ndipc_adjoint_backward(ndipc_ctx* ctx, const double* dl_dp, const double
  * No reference counterpart (NewtonSolver.cpp:10-95 only logs). */
 int32_t ndipc_solver_counters(ndipc_ctx* ctx, double* out8);
 /* Average device duration (ms) and count of launches of the named kernel since the
- * last reset, measured with HIP events on the context stream while enabled. */
+ * last reset, measured with HIP events on the context stream while enabled (steps then
+ * enqueue their kernels one by one instead of replaying the step graph). */
 int32_t ndipc_timing_enable(ndipc_ctx* ctx, int32_t on);
+/* ndipc_step replays one whole step captured as a HIP graph (default, on = 1) or
+ * enqueues the step's kernels one by one (on = 0); both run the same kernels with the
+ * same arguments. No reference counterpart (host launch strategy). */
+int32_t ndipc_use_graph(ndipc_ctx* ctx, int32_t on);
 int32_t ndipc_timing_get(ndipc_ctx* ctx, const char* kernel, double* avg_ms, int32_t* count);
 /* Directed-rounding primitives of the interval CCD, evaluated on the device:
  * out6 = add_dn, add_up, sub_dn, sub_up, mul_dn, mul_up. */

@@ -49,7 +49,7 @@ EXPORTS = [
     "ndipc_min_body_pair_distance", "ndipc_build_candidates", "ndipc_select_top_k", "ndipc_narrow_phase",
     "ndipc_scene_min_distance_flag", "ndipc_barrier_eval", "ndipc_friction_eval", "ndipc_ccd_toi_batch",
     "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters", "ndipc_set_joint_state",
-    "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward",
+    "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
 ]
 
 
@@ -104,6 +104,7 @@ def library():
         "ndipc_pcg_solve": ([vp, C.c_int32, f64p, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_int32, f64p, i32p,
                              f64p], C.c_int32),
         "ndipc_timing_enable": ([vp, C.c_int32], C.c_int32),
+        "ndipc_use_graph": ([vp, C.c_int32], C.c_int32),
         "ndipc_timing_get": ([vp, C.c_char_p, f64p, i32p], C.c_int32),
         "ndipc_round_ops": ([vp, f64p, f64p, C.c_int32, f64p], C.c_int32),
         "ndipc_elemfn": ([vp, C.c_int32, f64p, f64p, C.c_int32, f64p], C.c_int32),
@@ -401,6 +402,10 @@ class Engine:
         self._ck(self.L.ndipc_pcg_solve(self.ctx, nb, ptr(d, f64p), len(oij), ptr(oij, i32p), ptr(ob, f64p),
                                         ptr(bb, f64p), tol, max_it, ptr(x, f64p), C.byref(it), C.byref(rel)), "pcg")
         return x, it.value, rel.value
+
+    def use_graph(self, on=True):
+        """Replay the captured step graph (default) or enqueue kernels one by one."""
+        self._ck(self.L.ndipc_use_graph(self.ctx, 1 if on else 0), "use_graph")
 
     def timing_enable(self, on=True):
         self._ck(self.L.ndipc_timing_enable(self.ctx, 1 if on else 0), "timing_enable")

@@ -55,6 +55,10 @@ struct ndipc_ctx {
     double* ck_state = nullptr;  // [nck][19N] p q v w al aa
     double* ck_joint = nullptr;  // [nck][7J] lambda(6J) mu(J)
     SceneParams* ck_sp = nullptr;  // [nck][S]
+    // one whole step captured as a HIP graph (kernel nodes only), replayed per step
+    bool use_graph = true;
+    hipGraphExec_t step_exec = nullptr;
+    int graph_alm = -1;  // ALM iterations the captured step enqueues
 };
 
 namespace {
@@ -95,7 +99,14 @@ void free_adjoint(ndipc_ctx* ctx) {
     ctx->ck_sp = nullptr;
 }
 
+void drop_graph(ndipc_ctx* ctx) {
+    if (ctx->step_exec) (void)hipGraphExecDestroy(ctx->step_exec);
+    ctx->step_exec = nullptr;
+    ctx->graph_alm = -1;
+}
+
 void free_all(ndipc_ctx* ctx) {
+    drop_graph(ctx);
     free_adjoint(ctx);
     for (void* p : ctx->allocs) hipFree(p);
     ctx->allocs.clear();
@@ -589,53 +600,95 @@ int32_t ndipc_get_joint_state(ndipc_ctx* ctx, double* lambda6, double* mu) {
     return 0;
 }
 
-// The step sequence of IPCSolver::step for every scene, n_steps times. With `rec`, the
-// data the adjoint needs is recorded into slot k of step k: the step-start state, the
-// objective of every Newton solve (the last one stays), and the converged variable.
-static int run_steps(ndipc_ctx* ctx, int n_steps, const nd::Adj* rec) {
+// The kernel sequence of one IPCSolver::step for every scene. With `rec`, the data
+// the adjoint needs is recorded into slot k of the segment: the step-start state, the
+// objective of every Newton solve, the converged variable and the solve's iteration
+// count. Kernels only (no copies, fills or host work), except the recording copies.
+static int enqueue_step(ndipc_ctx* ctx, const nd::Adj* rec, int k, int alm_max) {
     const Dev& d = ctx->d;
     const Work& w = ctx->w;
     hipStream_t st = ctx->stream;
     const int mp = max_grid_pairs(ctx);
     const size_t N = size_t(ctx->N);
+    if (rec) {
+        double* dst = rec->s_state + size_t(k) * 13 * N;
+        HIPCHK(hipMemcpyAsync(dst, d.p, 8 * 3 * N, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(dst + 3 * N, d.q, 8 * 4 * N, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(dst + 7 * N, d.v, 8 * 3 * N, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(dst + 10 * N, d.w, 8 * 3 * N, hipMemcpyDeviceToDevice, st));
+    }
+    nd::launch_step_begin(d, w, ctx->d_dt, st);
+    nd::launch_world_boxes(d, w, st);
+    nd::launch_lbvh_build(d, w, st);
+    nd::launch_barrier_init_ccd(d, w, st);
+    nd::launch_broad_phase_step(d, w, st);
+    nd::timing_begin("topk", st);
+    nd::launch_topk(d, w, mp, st);
+    nd::timing_end("topk", st);
+    nd::timing_begin("scene_min_flag", st);
+    nd::launch_scene_min_flag(d, w, mp, st);
+    nd::timing_end("scene_min_flag", st);
+    nd::launch_kappa(d, w, st);
+    nd::launch_ccd_earliest(d, w, mp, st);
+    nd::launch_ccd_dt(d, w, st);
+    nd::launch_pack(d, w, st);
+    nd::launch_pose_from_state(d, w, nullptr, st);
+    nd::launch_narrow_phase(d, w, mp, false, st);
+    nd::launch_contacts_setup(d, w, st);
+    for (int a = 0; a < alm_max; ++a) {
+        if (rec) nd::launch_adj_snapshot(d, w, *rec, k * rec->A + a, st);
+        nd::launch_newton(d, w, st);
+        if (rec) nd::launch_adj_snapshot_post(d, w, *rec, k * rec->A + a, st);
+        nd::launch_pose_from_state(d, w, w.x, st);
+        nd::launch_narrow_phase(d, w, mp, true, st);
+        nd::launch_alm_post(d, w, st);
+    }
+    nd::launch_finalize(d, w, st);
+    return 0;
+}
+
+// Captures one step (no recording, no timing) into a graph. The step's kernels take only
+// device pointers and sizes fixed at upload, and read dt and every per-step value from
+// device memory, so one capture serves every later step until the next upload or a
+// change of the ALM iteration count.
+static int capture_step(ndipc_ctx* ctx, int alm_max) {
+    drop_graph(ctx);
+    hipStream_t st = ctx->stream;
+    ndipc_ctx* saved = g_timing_ctx;
+    g_timing_ctx = nullptr;  // no event records inside the graph
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step(ctx, nullptr, 0, alm_max);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    g_timing_ctx = saved;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(ctx, std::string("step capture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&ctx->step_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        ctx->step_exec = nullptr;
+        return fail(ctx, std::string("step graph: ") + hipGetErrorString(ei));
+    }
+    ctx->graph_alm = alm_max;
+    return 0;
+}
+
+// The step sequence of IPCSolver::step for every scene, n_steps times: the captured
+// graph replayed per step, or the kernels enqueued one by one when recording for the
+// adjoint, when kernel timing is on, or when graphs are switched off.
+static int run_steps(ndipc_ctx* ctx, int n_steps, const nd::Adj* rec) {
     int alm_max = 1;
     for (auto& q : ctx->sp) alm_max = std::max(alm_max, q.alm_max_iters);
+    const bool graph = !rec && ctx->use_graph && !ctx->timing;
+    if (graph && (!ctx->step_exec || ctx->graph_alm != alm_max))
+        if (capture_step(ctx, alm_max)) return -1;
     g_timing_ctx = ctx;
     for (int k = 0; k < n_steps; ++k) {
-        if (rec) {
-            double* dst = rec->s_state + size_t(k) * 13 * N;
-            HIPCHK(hipMemcpyAsync(dst, d.p, 8 * 3 * N, hipMemcpyDeviceToDevice, st));
-            HIPCHK(hipMemcpyAsync(dst + 3 * N, d.q, 8 * 4 * N, hipMemcpyDeviceToDevice, st));
-            HIPCHK(hipMemcpyAsync(dst + 7 * N, d.v, 8 * 3 * N, hipMemcpyDeviceToDevice, st));
-            HIPCHK(hipMemcpyAsync(dst + 10 * N, d.w, 8 * 3 * N, hipMemcpyDeviceToDevice, st));
-        }
-        nd::launch_step_begin(d, w, ctx->d_dt, st);
-        nd::launch_world_boxes(d, w, st);
-        nd::launch_lbvh_build(d, w, st);
-        nd::launch_barrier_init_ccd(d, w, st);
-        nd::launch_broad_phase_step(d, w, st);
-        nd::timing_begin("topk", st);
-        nd::launch_topk(d, w, mp, st);
-        nd::timing_end("topk", st);
-        nd::timing_begin("scene_min_flag", st);
-        nd::launch_scene_min_flag(d, w, mp, st);
-        nd::timing_end("scene_min_flag", st);
-        nd::launch_kappa(d, w, st);
-        nd::launch_ccd_earliest(d, w, mp, st);
-        nd::launch_ccd_dt(d, w, st);
-        nd::launch_pack(d, w, st);
-        nd::launch_pose_from_state(d, w, nullptr, st);
-        nd::launch_narrow_phase(d, w, mp, false, st);
-        nd::launch_contacts_setup(d, w, st);
-        for (int a = 0; a < alm_max; ++a) {
-            if (rec) nd::launch_adj_snapshot(d, w, *rec, k * rec->A + a, st);
-            nd::launch_newton(d, w, st);
-            if (rec) nd::launch_adj_snapshot_post(d, w, *rec, k * rec->A + a, st);
-            nd::launch_pose_from_state(d, w, w.x, st);
-            nd::launch_narrow_phase(d, w, mp, true, st);
-            nd::launch_alm_post(d, w, st);
-        }
-        nd::launch_finalize(d, w, st);
+        if (graph) HIPCHK(hipGraphLaunch(ctx->step_exec, ctx->stream));
+        else if (enqueue_step(ctx, rec, k, alm_max)) return -1;
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -1013,6 +1066,12 @@ int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t nb, const double* diag, int32_t 
     if (rel) HIPCHK(hipMemcpy(rel, drel, 8, hipMemcpyDeviceToHost));
     hipFree(buf);
     hipFree(ib);
+    return 0;
+}
+
+int32_t ndipc_use_graph(ndipc_ctx* ctx, int32_t on) {
+    if (!ctx) return -1;
+    ctx->use_graph = on != 0;
     return 0;
 }
 

@@ -284,3 +284,28 @@ def test_pcg_matches_dense_solve(engine_factory):
     x, it, rel = e.pcg_solve(np.array(diag), np.array(offij), np.array(offb), b, tol=1e-14, max_it=500)
     xr = np.linalg.solve(H, b)
     assert np.max(np.abs(x - xr)) <= 1e-11 * np.max(np.abs(xr))
+
+
+@pytest.mark.parametrize("name", ["c1", "micro", "c3x8"])
+def test_step_graph_replays_match_eager_steps(engine_factory, name):
+    """ndipc_step replays a captured step graph; the same steps enqueued kernel by kernel
+    give the same bits. Replayed in chunks with eager host work (state reads, a state
+    write-back, statistics) between the replays."""
+    if name == "c1":
+        sc = scenes.make_config("c1")
+    elif name == "micro":
+        sc = scenes.pack([scenes.micro_contact_scene(seed=1, n_cubes=4)])
+    else:
+        sc = scenes.pack(scenes.config3(n_scenes=8))
+    eg, ee = engine_factory(sc), engine_factory(sc)
+    eg.use_graph(True)
+    ee.use_graph(False)
+    for chunk in (1, 3, 1, 5):
+        eg.step(chunk)
+        ee.step(chunk)
+        sg, se = eg.get_state(), ee.get_state()
+        for a, b in zip(sg, se):
+            assert np.array_equal(a, b)
+        assert np.array_equal(eg.stats(), ee.stats())
+        eg.set_state(*sg)  # a host round trip between replays
+        ee.set_state(*se)
