@@ -71,12 +71,13 @@ def load_traffic(kernel):
     return 1024.0 * (k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"])
 
 
-def run_plain(Engine, torch, sc, steps, warmup, timing=False, graph=True):
+def run_plain(Engine, torch, sc, steps, warmup, timing=False, graph=True, autotune=False):
     """steps timed steps after warmup. With `graph` each step replays the captured step
     graph, else (or with `timing`, where HIP events bracket each kernel) the kernels are
     enqueued one by one."""
     e = Engine(0)
     e.upload(sc)
+    tune = e.autotune(trial_steps=2) if autotune else None
     e.use_graph(graph)
     e.step(warmup)
     torch.cuda.synchronize()
@@ -99,7 +100,8 @@ def run_plain(Engine, torch, sc, steps, warmup, timing=False, graph=True):
     pairs = e.pair_count()
     e.close()
     return dict(T=T, evals=float(np.sum(ev1 - ev0)), stats=stats, kern=kern, pairs=pairs,
-                newton_iters=float(np.sum(c1[:, 6] - c0[:, 6])), trials=float(np.sum(c1[:, 7] - c0[:, 7])))
+                newton_iters=float(np.sum(c1[:, 6] - c0[:, 6])), trials=float(np.sum(c1[:, 7] - c0[:, 7])),
+                tune=None if tune is None else {"grids": [int(v) for v in tune[0]], "ms_per_step": tune[1]})
 
 
 def run_adjoint(Engine, sc, targets, n_steps, every):
@@ -225,7 +227,7 @@ def main():
     if not args.no_batch:
         specs = scenes.config3()
         scb = scenes.pack(specs)
-        rb = run_plain(Engine, torch, scb, args.batch_steps, args.warmup)
+        rb = run_plain(Engine, torch, scb, args.batch_steps, args.warmup, autotune=True)
         batch_rate = args.batch_steps * scb.n_scenes / rb["T"]
         # the same scenes one by one through the single-scene path: a seeded sample of
         # scenes, each on its own engine, scene-steps/s = steps / (sum of wall times)
@@ -249,6 +251,7 @@ def main():
                  "one_by_one_sample": f"{len(pick)} of {len(specs)} scenes, {args.batch_steps // 5} steps each, "
                                       "each scene on its own context through the single-scene path",
                  "speedup_vs_one_by_one": batch_rate / one_rate if one_rate else None,
+                 "autotune": rb["tune"],
                  "one_by_one_graph_scene_steps_per_s": one_rate_graph,
                  "speedup_vs_one_by_one_graph": batch_rate / one_rate_graph if one_rate_graph else None,
                  "note": "batch: one launch per stage for all scenes, step replayed from a HIP graph; "
@@ -291,7 +294,7 @@ def main():
 
     sc = scenes.make_config("c2")
     workload = "c2: 512 icospheres (subdiv 3) + 5-box open container, single scene"
-    r = run_plain(Engine, torch, sc, args.steps, args.warmup)
+    r = run_plain(Engine, torch, sc, args.steps, args.warmup, autotune=True)
     T = r["T"]
     stats = r["stats"]
     # the same steps again, kernels enqueued one by one between HIP events
@@ -331,6 +334,7 @@ def main():
         "line_search_trials": r["trials"],
         "kernel_total_ms": totals,
         "ms_per_step_kernels_one_by_one": 1e3 * rt["T"] / args.steps,
+        "autotune": r["tune"],
         "roofline": roof,
         "roofline_narrow_phase": roofline_for("np_count", rt, sc),
         "batch": batch,

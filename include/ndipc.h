@@ -208,6 +208,13 @@ int32_t ndipc_timing_enable(ndipc_ctx* ctx, int32_t on);
  * enqueues the step's kernels one by one (on = 0); both run the same kernels with the
  * same arguments. No reference counterpart (host launch strategy). */
 int32_t ndipc_use_graph(ndipc_ctx* ctx, int32_t on);
+/* Autotuner of the launch shapes (work-group counts of the persistent top-K, narrow-phase,
+ * scene-min-distance and CCD kernels) for the uploaded batch: from the current state,
+ * each candidate runs trial_steps steps of dt (timed with HIP events), one knob at a
+ * time; the state is restored and the fastest shapes are kept (out4: the chosen counts,
+ * out_ms: ms per step with them). Results of a step do not depend on the shapes. No
+ * reference counterpart. */
+int32_t ndipc_autotune(ndipc_ctx* ctx, const double* dt, int32_t trial_steps, int32_t* out4, double* out_ms);
 int32_t ndipc_timing_get(ndipc_ctx* ctx, const char* kernel, double* avg_ms, int32_t* count);
 /* Directed-rounding primitives of the interval CCD, evaluated on the device:
  * out6 = add_dn, add_up, sub_dn, sub_up, mul_dn, mul_up. */

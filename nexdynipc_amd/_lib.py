@@ -50,6 +50,7 @@ EXPORTS = [
     "ndipc_scene_min_distance_flag", "ndipc_barrier_eval", "ndipc_friction_eval", "ndipc_ccd_toi_batch",
     "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters", "ndipc_set_joint_state",
     "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
+    "ndipc_autotune",
 ]
 
 
@@ -105,6 +106,7 @@ def library():
                              f64p], C.c_int32),
         "ndipc_timing_enable": ([vp, C.c_int32], C.c_int32),
         "ndipc_use_graph": ([vp, C.c_int32], C.c_int32),
+        "ndipc_autotune": ([vp, f64p, C.c_int32, i32p, f64p], C.c_int32),
         "ndipc_timing_get": ([vp, C.c_char_p, f64p, i32p], C.c_int32),
         "ndipc_round_ops": ([vp, f64p, f64p, C.c_int32, f64p], C.c_int32),
         "ndipc_elemfn": ([vp, C.c_int32, f64p, f64p, C.c_int32, f64p], C.c_int32),
@@ -402,6 +404,17 @@ class Engine:
         self._ck(self.L.ndipc_pcg_solve(self.ctx, nb, ptr(d, f64p), len(oij), ptr(oij, i32p), ptr(ob, f64p),
                                         ptr(bb, f64p), tol, max_it, ptr(x, f64p), C.byref(it), C.byref(rel)), "pcg")
         return x, it.value, rel.value
+
+    def autotune(self, trial_steps=3, dt=None):
+        """Chooses the persistent kernels' work-group counts for this batch (state is
+        restored); returns (counts[topk, narrow phase, scene-min flag, CCD], ms/step)."""
+        dt = self.sc.dt if dt is None else np.broadcast_to(np.asarray(dt, dtype=np.float64), (self.sc.n_scenes,))
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        out = np.zeros(4, dtype=np.int32)
+        ms = C.c_double()
+        self._ck(self.L.ndipc_autotune(self.ctx, ptr(dt, f64p), int(trial_steps), ptr(out, i32p), C.byref(ms)),
+                 "autotune")
+        return out, ms.value
 
     def use_graph(self, on=True):
         """Replay the captured step graph (default) or enqueue kernels one by one."""

@@ -1069,6 +1069,84 @@ int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t nb, const double* diag, int32_t 
     return 0;
 }
 
+}  // extern "C"
+namespace {
+int snapshot_state(ndipc_ctx* ctx, double* s, double* jst, SceneParams* spb);
+int restore_state(ndipc_ctx* ctx, const double* s, const double* jst, const SceneParams* spb);
+}  // namespace
+extern "C" {
+
+// Launch shapes of the persistent kernels (work-groups of k_topk, the narrow phase,
+// k_scene_min_flag and k_ccd_fv): from one saved state, each candidate runs trial_steps
+// steps between HIP events (kernels enqueued one by one), best of two runs; one knob at
+// a time, the others at their best so far. The state is restored afterwards. Results
+// do not depend on these shapes, only the speed does.
+int32_t ndipc_autotune(ndipc_ctx* ctx, const double* dt, int32_t trial_steps, int32_t* out4, double* out_ms) {
+    if (check_ctx(ctx)) return -1;
+    if (trial_steps < 1) return fail(ctx, "trial_steps must be positive");
+    if (upload_dt(ctx, dt)) return -1;
+    const size_t N = size_t(ctx->N), J = size_t(std::max(ctx->J, 1));
+    double *sst = nullptr, *sjt = nullptr;
+    SceneParams* ssp = nullptr;
+    if (hipMalloc(&sst, 8 * 19 * std::max<size_t>(N, 1)) != hipSuccess ||
+        hipMalloc(&sjt, 8 * 7 * J) != hipSuccess || hipMalloc(&ssp, sizeof(SceneParams) * ctx->S) != hipSuccess) {
+        (void)hipFree(sst); (void)hipFree(sjt); (void)hipFree(ssp);
+        return fail(ctx, "autotune: allocation failed");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = snapshot_state(ctx, sst, sjt, ssp);
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(ctx, "autotune: event creation failed");
+    const bool graph = ctx->use_graph, timing = ctx->timing;
+    ctx->use_graph = false;
+    ctx->timing = false;
+    int* knob[4] = {&ctx->w.grid_topk, &ctx->w.grid_np, &ctx->w.grid_smf, &ctx->w.grid_ccd};
+    const int cand[4][5] = {{256, 512, 1024, 2048, 4096},
+                            {256, 512, 1024, 2048, 4096},
+                            {256, 512, 1024, 2048, 4096},
+                            {512, 1024, 2048, 4096, 8192}};
+    const int defaults[4] = {2048, 4096, 2048, 4096};
+    double best_ms = 1e300;
+    for (int k = 0; k < 4 && !rc; ++k) {
+        int best = defaults[k];
+        double kbest = 1e300;
+        for (int c = 0; c < 5 && !rc; ++c) {
+            *knob[k] = cand[k][c];
+            double t = 1e300;
+            for (int rep = 0; rep < 2 && !rc; ++rep) {
+                if ((rc = restore_state(ctx, sst, sjt, ssp))) break;
+                if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = fail(ctx, "autotune: event"); break; }
+                if ((rc = run_steps(ctx, trial_steps, nullptr))) break;
+                float ms = 0.f;
+                if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    rc = fail(ctx, "autotune: event timing");
+                    break;
+                }
+                t = std::min(t, double(ms) / trial_steps);
+            }
+            if (t < kbest) {
+                kbest = t;
+                best = cand[k][c];
+            }
+        }
+        *knob[k] = best;
+        best_ms = kbest;
+    }
+    ctx->use_graph = graph;
+    ctx->timing = timing;
+    if (!rc) rc = restore_state(ctx, sst, sjt, ssp);
+    drop_graph(ctx);  // recaptured with the chosen shapes
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(sst); (void)hipFree(sjt); (void)hipFree(ssp);
+    if (rc) return rc;
+    for (int k = 0; k < 4; ++k)
+        if (out4) out4[k] = *knob[k];
+    if (out_ms) *out_ms = best_ms;
+    return 0;
+}
+
 int32_t ndipc_use_graph(ndipc_ctx* ctx, int32_t on) {
     if (!ctx) return -1;
     ctx->use_graph = on != 0;
@@ -1149,6 +1227,40 @@ T* aalloc(ndipc_ctx* ctx, size_t n) {
     if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return nullptr;
     ctx->adj_allocs.push_back(p);
     return static_cast<T*>(p);
+}
+
+// Saves the full state (poses, velocities, accelerations, joint multipliers and
+// stiffnesses, scene parameters) to s[19N], jst[7J], spb[S]; restore_state reverses it.
+int snapshot_state(ndipc_ctx* ctx, double* s, double* jst, SceneParams* spb) {
+    const size_t N = size_t(ctx->N), J = size_t(std::max(ctx->J, 1));
+    hipStream_t st = ctx->stream;
+    const Dev& d = ctx->d;
+    double* src[6] = {d.p, d.q, d.v, d.w, d.al, d.aa};
+    const size_t len[6] = {3 * N, 4 * N, 3 * N, 3 * N, 3 * N, 3 * N};
+    for (int k = 0; k < 6; ++k) {
+        HIPCHK(hipMemcpyAsync(s, src[k], 8 * len[k], hipMemcpyDeviceToDevice, st));
+        s += len[k];
+    }
+    HIPCHK(hipMemcpyAsync(jst, d.jlambda, 8 * 6 * J, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(jst + 6 * J, d.jmu, 8 * J, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(spb, d.sp, sizeof(SceneParams) * ctx->S, hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+int restore_state(ndipc_ctx* ctx, const double* s, const double* jst, const SceneParams* spb) {
+    const size_t N = size_t(ctx->N), J = size_t(std::max(ctx->J, 1));
+    hipStream_t st = ctx->stream;
+    const Dev& d = ctx->d;
+    double* dst[6] = {d.p, d.q, d.v, d.w, d.al, d.aa};
+    const size_t len[6] = {3 * N, 4 * N, 3 * N, 3 * N, 3 * N, 3 * N};
+    for (int k = 0; k < 6; ++k) {
+        HIPCHK(hipMemcpyAsync(dst[k], s, 8 * len[k], hipMemcpyDeviceToDevice, st));
+        s += len[k];
+    }
+    HIPCHK(hipMemcpyAsync(d.jlambda, jst, 8 * 6 * J, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.jmu, jst + 6 * J, 8 * J, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.sp, spb, sizeof(SceneParams) * ctx->S, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return sync_sp_from_dev(ctx);
 }
 
 int save_checkpoint(ndipc_ctx* ctx, int c) {

@@ -612,8 +612,9 @@ __global__ void __launch_bounds__(kBlock) k_topk(Dev d, Work w) {
 }
 
 void launch_topk(const Dev& d, const Work& w, int max_pairs, hipStream_t st) {
+    const int cap = w.grid_topk > 0 ? w.grid_topk : 2048;
     int grid = max_pairs * 6;
-    if (grid > 2048) grid = 2048;
+    if (grid > cap) grid = cap;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(k_topk, dim3(grid), dim3(kBlock), 0, st, d, w);
 }
@@ -1041,7 +1042,8 @@ __global__ void __launch_bounds__(1024) k_np_cap(Dev d, Work w, int gate) {
 }
 
 void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st) {
-    int grid = max_pairs < 4096 ? max_pairs : 4096;
+    const int cap = w.grid_np > 0 ? w.grid_np : 4096;
+    int grid = max_pairs < cap ? max_pairs : cap;
     if (grid < 1) grid = 1;
     const int gate = gate_alm ? 1 : 0;
     timing_begin("np_count", st);
@@ -1171,7 +1173,8 @@ __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int un
 }
 
 void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_t st, int ungated) {
-    int grid = max_prox < 2048 ? max_prox : 2048;
+    const int cap = w.grid_smf > 0 ? w.grid_smf : 2048;
+    int grid = max_prox < cap ? max_prox : cap;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(k_scene_min_flag, dim3(grid), dim3(kBlock), 0, st, d, w, ungated);
 }
@@ -1398,7 +1401,8 @@ __global__ void k_ccd_fv(Dev d, Work w, int gate_any) {
 void launch_ccd_earliest(const Dev& d, const Work& w, int max_pairs, hipStream_t st) {
     long long n = (long long)max_pairs * 2048;
     long long g = (n + 255) / 256;
-    if (g > 4096) g = 4096;
+    const long long cap = w.grid_ccd > 0 ? w.grid_ccd : 4096;
+    if (g > cap) g = cap;
     if (g < 1) g = 1;
     timing_begin("ccd_fv", st);
     hipLaunchKernelGGL(k_ccd_fv, dim3((unsigned)g), dim3(256), 0, st, d, w, 1);

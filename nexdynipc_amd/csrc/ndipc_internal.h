@@ -91,6 +91,9 @@ struct Dev {
 
 // Per-step work buffers.
 struct Work {
+    // launch shape of the persistent kernels (work-groups; 0: the built-in default),
+    // chosen by ndipc_autotune; results do not depend on them
+    int grid_topk, grid_np, grid_smf, grid_ccd;
     double* wbox;            // [N*6] world AABB at step start
     double* pose_p;          // [N*3] pose used by the current narrow phase
     double* pose_q;          // [N*4]

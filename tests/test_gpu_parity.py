@@ -309,3 +309,23 @@ def test_step_graph_replays_match_eager_steps(engine_factory, name):
         assert np.array_equal(eg.stats(), ee.stats())
         eg.set_state(*sg)  # a host round trip between replays
         ee.set_state(*se)
+
+
+def test_autotune_restores_state_and_keeps_results(engine_factory):
+    """ndipc_autotune times launch shapes from the current state and restores it; the
+    chosen shapes change speed only: the next steps give the untuned engine's bits."""
+    sc = scenes.pack(scenes.config3(n_scenes=8))
+    a, b = engine_factory(sc), engine_factory(sc)
+    a.step(2)
+    b.step(2)
+    s0 = a.get_state()
+    counts, ms = a.autotune(trial_steps=2)
+    assert counts.shape == (4,) and np.all(counts > 0) and ms > 0.0
+    for x, y in zip(s0, a.get_state()):
+        assert np.array_equal(x, y)
+    a.step(4)
+    b.step(4)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.eval_counter(), b.eval_counter())
+    assert np.array_equal(a.solver_counters()[:, 6:], b.solver_counters()[:, 6:])
