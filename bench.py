@@ -195,8 +195,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", action="store_true", help="measure only the 256-scene batch (config 3)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batch / one-by-one comparison")
-    ap.add_argument("--batch-steps", type=int, default=50)
-    ap.add_argument("--one-by-one", type=int, default=32, help="scenes sampled for the one-by-one comparison")
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--one-by-one", type=int, default=24, help="scenes sampled for the one-by-one comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs (c4, c5)")
     ap.add_argument("--adjoint-steps", type=int, default=100)
@@ -237,10 +237,10 @@ def main():
         for graph in (False, True):
             t_sum, st_sum = 0.0, 0
             for i in pick:
-                r1 = run_plain(Engine, torch, scenes.pack([specs[int(i)]]), args.batch_steps // 5, args.warmup,
+                r1 = run_plain(Engine, torch, scenes.pack([specs[int(i)]]), args.batch_steps, args.warmup,
                                graph=graph)
                 t_sum += r1["T"]
-                st_sum += args.batch_steps // 5
+                st_sum += args.batch_steps
             one[graph] = st_sum / t_sum if t_sum > 0 else None
         one_rate, one_rate_graph = one[False], one[True]
         batch = {"workload": "c3: 256 scenes, log-U[4,128] bodies each (cube / icosphere s2 / 2x1x0.5 box), "
@@ -248,8 +248,9 @@ def main():
                  "scene_steps_per_s": batch_rate, "ms_per_step": 1e3 * rb["T"] / args.batch_steps,
                  "contact_pairs_per_s": rb["evals"] / rb["T"],
                  "one_by_one_scene_steps_per_s": one_rate,
-                 "one_by_one_sample": f"{len(pick)} of {len(specs)} scenes, {args.batch_steps // 5} steps each, "
-                                      "each scene on its own context through the single-scene path",
+                 "one_by_one_sample": f"{len(pick)} of {len(specs)} scenes, the same {args.batch_steps} steps "
+                                      "after the same warm-up, each scene on its own context through the "
+                                      "single-scene path",
                  "speedup_vs_one_by_one": batch_rate / one_rate if one_rate else None,
                  "autotune": rb["tune"],
                  "one_by_one_graph_scene_steps_per_s": one_rate_graph,
