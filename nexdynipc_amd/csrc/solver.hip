@@ -1037,6 +1037,7 @@ __device__ bool ldlt6_solve(const double* A, const double* b, double* x) {
 }
 
 // Returns 0 solved, 1 zero pivot, 2 a component exceeds kDenseBodies.
+constexpr int kSmallRows = 24;  // components of up to 4 bodies are factored in LDS
 __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* rhs,
                                double* xo, CompLds& L, int* ok_sh) {
     build_components(d, w, sc, L);
@@ -1050,12 +1051,15 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
     }
     __syncthreads();
     if (!*ok_sh) return 1;
-    double* M = w.dense + (long long)sc.s * kDenseRows * kDenseRows;
+    double* Mg = w.dense + (long long)sc.s * kDenseRows * kDenseRows;
+    __shared__ double Msmall[kSmallRows * kSmallRows];  // components of up to 4 bodies, in LDS
     for (int c0 = 0; c0 < L.ncoupled;) {
         const int c = L.lab[L.list[c0]];  // the component's label (its first member)
         const int mb = L.cnt[c];
         const int c1 = c0 + mb;
         const int m = 6 * mb;
+        const bool small = m <= kSmallRows;
+        double* M = small ? Msmall : Mg;
         // local row of body b: 6 * (index of b in the component's ascending list)
         auto lpos = [&](int lb) {
             int lo = c0, hi = c1;
