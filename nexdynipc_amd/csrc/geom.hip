@@ -1316,7 +1316,7 @@ __device__ int ccd_toi(const CcdQuery& q, double earliest, double tol, double* t
 // impact (TimeOfImpact.cpp:97-150 with IntervalRootFinder.h:20-114). What remains is
 // the NaN screen of the pre-check evaluation, which decides between 0 and -1. Same
 // result as the general search, without its explicit stack (no scratch traffic).
-__device__ int ccd_toi_face_vertex(const CcdQuery& q, double tol) {
+__device__ __forceinline__ int ccd_toi_face_vertex(const CcdQuery& q, double tol) {
     bool nan = false;
     const Iv xt[3] = {iv(0.0, tol), iv(0.0, 1.0), iv(0.0, 1.0)};
     (void)ccd_f(q, xt, nan);
