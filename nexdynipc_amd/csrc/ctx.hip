@@ -1101,11 +1101,11 @@ int32_t ndipc_autotune(ndipc_ctx* ctx, const double* dt, int32_t trial_steps, in
     ctx->use_graph = false;
     ctx->timing = false;
     int* knob[4] = {&ctx->w.grid_topk, &ctx->w.grid_np, &ctx->w.grid_smf, &ctx->w.grid_ccd};
-    const int cand[4][5] = {{256, 512, 1024, 2048, 4096},
+    const int cand[4][5] = {{512, 1024, 2048, 4096, 8192},
                             {256, 512, 1024, 2048, 4096},
                             {256, 512, 1024, 2048, 4096},
                             {512, 1024, 2048, 4096, 8192}};
-    const int defaults[4] = {2048, 4096, 2048, 4096};
+    const int defaults[4] = {4096, 4096, 2048, 4096};
     double best_ms = 1e300;
     for (int k = 0; k < 4 && !rc; ++k) {
         int best = defaults[k];

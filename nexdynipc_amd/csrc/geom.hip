@@ -429,148 +429,246 @@ void launch_broad_phase(const Dev& d, const Work& w, int mode, double r, bool, c
 // select_top_k_indices (ContactAssembler.cpp:67-88) for the six lists of every close
 // pair (ContactAssembler.cpp:249-302). Scores are point_aabb_distance of world
 // vertices to the other body's world AABB; edges take the min over 2 endpoints, faces
-// over 3 (std::min / std::min({..}) keep the first smallest). The kept set and its
-// order are those of std::nth_element followed by std::sort (topk_select.h): a parallel
-// sort decides every list without equal scores among the kept ones, a single lane
-// replays the standard library's algorithms on the others.
+// over 3 (std::min / std::min({..}) return the smallest value). The kept set and its
+// order are those of std::nth_element followed by std::sort (libstdc++): with equal
+// scores among the kept ones (most lists: an edge or face score is the score of a
+// shared vertex, and vertices inside the other box all score 0) both depend on the
+// library's algorithms, so every list replays them. One wave (a 64-lane work-group)
+// owns a list, held in LDS as (score, 16-bit index) pairs, and runs the steps of
+// tk_nth_element_lr and tk_sort_blocks (topk_select.h, checked against libstdc++ on
+// the host): each Hoare partition is computed from the positions it swaps with a wave
+// scan, median-of-three and the heap fallbacks run on lane 0, and the final insertion
+// sorts are per-block rank sorts with one lane per element.
 __device__ __forceinline__ bool key_less(double sa, int ia, double sb, int ib) {
     return sa < sb || (sa == sb && ia < ib);
 }
 
-__device__ void wg_bitonic_pairs(double* sk, int* si, int P) {
-    for (int size = 2; size <= P; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int k = threadIdx.x; k < P / 2; k += blockDim.x) {
-                const int i = 2 * k - (k & (stride - 1));
-                const int j = i + stride;
-                const bool up = ((i & size) == 0);
-                const bool gt = key_less(sk[j], si[j], sk[i], si[i]);
-                if (gt == up) {
-                    const double ts = sk[i]; sk[i] = sk[j]; sk[j] = ts;
-                    const int ti = si[i]; si[i] = si[j]; si[j] = ti;
-                }
-            }
-            __syncthreads();
-        }
+constexpr int kTkLanes = 64;
+constexpr int kTkPer = kMaxPrims / kTkLanes;  // list entries per lane at most
+
+struct TkLds {
+    double key[kMaxPrims];
+    unsigned short idx[kMaxPrims];
+    unsigned short lpos[kMaxPrims], rpos[kMaxPrims];
+    unsigned char start[kMaxPrims];
+    int stk[3 * 64];
+};
+
+// Exclusive prefix sum over the wave's 64 lanes, and the total.
+__device__ __forceinline__ int tk_wave_excl_sum(int v, int* total) {
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < kTkLanes; o <<= 1) {
+        const int y = __shfl_up(x, o, kTkLanes);
+        if ((int)threadIdx.x >= o) x += y;
     }
+    *total = __shfl(x, kTkLanes - 1, kTkLanes);
+    return x - v;
 }
 
-// Selects the top-k of s_orig[0..n) into out[0..k) ordered by (score, index).
-// nth_element over the (score, index) pairs in LDS by the whole work-group, giving the
-// permutation of tk_nth_element (libstdc++'s introselect). The Hoare partition of a
-// pass is computed in parallel: with L_1 < L_2 < ... the positions whose score is not
-// below the pivot and R_1 > R_2 > ... those not above it (both in the array as the
-// pass starts), the scanners' k-th stops are exactly L_k and R_k for as long as
-// L_k < R_k, so the pass swaps (L_k, R_k) for k = 1..m, m the length of that prefix,
-// and returns min(L_{m+1}, R_m). Median selection, the final insertion sort and the
-// heap-select fallback run on one lane as in the serial restatement.
-__device__ void wg_nth_element_kv(double* sk, int* si, int n, int nth, int* lpos, int* rpos, int* sh) {
+// tk_partition_lr over [lo, hi) with the pivot at `pivot`: L (ascending positions not
+// below the pivot) and R (descending positions not above it) by a wave scan of per-lane
+// counts over contiguous chunks; m = the first q with L_q >= R_q (L ascends and R
+// descends, so the test is monotone in q: a 64-way search in two rounds); the m swaps
+// touch 2m distinct positions and run in parallel. Returns the cut (wave-uniform).
+__device__ int tk_wave_partition(const KV16Store& a, int lo, int hi, int pivot, unsigned short* lpos,
+                                 unsigned short* rpos) {
+    const int lane = threadIdx.x;
+    const double pv = a.k[pivot];
+    const int len = hi - lo;
+    const int per = (len + kTkLanes - 1) / kTkLanes;
+    const int i0 = lo + lane * per, i1 = min(hi, i0 + per);
+    int cl = 0, cr = 0;
+    for (int i = i0; i < i1; ++i) {
+        const double v = a.k[i];
+        cl += !(v < pv);
+        cr += !(pv < v);
+    }
+    int tot;
+    const int ex = tk_wave_excl_sum(cl | (cr << 16), &tot);
+    const int nL = tot & 0xffff, nR = tot >> 16;
+    int kl = ex & 0xffff, kr = ex >> 16;
+    for (int i = i0; i < i1; ++i) {
+        const double v = a.k[i];
+        if (!(v < pv)) lpos[kl++] = (unsigned short)i;
+        if (!(pv < v)) rpos[nR - 1 - kr++] = (unsigned short)i;
+    }
+    __syncthreads();
+    // m by tk_first_crossing's two rounds of 64 probes
+    const int mm = min(nL, nR);
+    const int step = (mm + kTkLanes - 1) / kTkLanes;  // at most 32: mm < kMaxPrims
+    const int nq = step ? (mm + step - 1) / step : 0;  // probes below mm
+    int m;
+    {
+        const int q = lane * step;
+        const unsigned long long b = __ballot(lane < nq && lpos[q] >= rpos[q]);
+        const int f = b ? __ffsll((unsigned long long)b) - 1 : nq;
+        if (f == 0) {
+            m = 0;
+        } else {  // m in ((f - 1) * step, min(f * step, mm)]
+            const int qa = (f - 1) * step + 1, qb = min(f * step, mm);
+            const int q2 = qa + lane;
+            const unsigned long long b2 = __ballot(q2 < qb && lpos[q2] >= rpos[q2]);
+            m = b2 ? qa + __ffsll((unsigned long long)b2) - 1 : qb;
+        }
+    }
+    int cut = m < nL ? (int)lpos[m] : hi;
+    if (m >= 1 && (int)rpos[m - 1] < cut) cut = rpos[m - 1];
+    for (int q = lane; q < m; q += kTkLanes) {
+        const int x = lpos[q], y = rpos[q];
+        const double tk = a.k[x];
+        const unsigned short ti = a.idx[x];
+        a.k[x] = a.k[y];
+        a.idx[x] = a.idx[y];
+        a.k[y] = tk;
+        a.idx[y] = ti;
+    }
+    __syncthreads();
+    return cut;
+}
+
+// tk_block_rank_sort of [0, n), n <= 64 * PER: lane-parallel, every element's
+// destination computed (and the element held in registers) before any element moves.
+template <int PER>
+__device__ void tk_wave_block_rank_sort(const KV16Store& a, int n, const unsigned char* start) {
+    const int lane = threadIdx.x;
+    int dst[PER];
+    double kv[PER];
+    unsigned short iv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = lane + j * kTkLanes;
+        if (i < n) {
+            int s = i, e = i + 1;
+            while (!start[s]) --s;
+            while (e < n && !start[e]) ++e;
+            const double v = a.k[i];
+            int r = s;
+            for (int t = s; t < e; ++t) {
+                const double u = a.k[t];
+                r += (u < v) || (t < i && !(v < u));
+            }
+            dst[j] = r;
+            kv[j] = v;
+            iv[j] = a.idx[i];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = lane + j * kTkLanes;
+        if (i < n) {
+            a.k[dst[j]] = kv[j];
+            a.idx[dst[j]] = iv[j];
+        }
+    }
+    __syncthreads();
+}
+
+// tk_nth_element_lr(a, nth, n) by the wave.
+__device__ void tk_wave_nth_element(const KV16Store& a, int nth, int n, TkLds& L) {
     if (n == 0 || nth == n) return;
+    const int lane = threadIdx.x;
     int first = 0, last = n;
     int depth = tk_floor_log2(n) * 2;
     while (last - first > 3) {
         if (depth == 0) {
-            if (threadIdx.x == 0) {
-                KVStore st{sk, si};
-                tk_heap_select(st, first, nth + 1, last);
-                tk_swap(st, first, nth);
+            if (lane == 0) {
+                tk_heap_select(a, first, nth + 1, last);
+                tk_swap(a, first, nth);
             }
             __syncthreads();
             return;
         }
         --depth;
-        if (threadIdx.x == 0) {
-            KVStore st{sk, si};
-            tk_move_median_to_first(st, first, first + 1, first + (last - first) / 2, last - 1);
-        }
+        if (lane == 0) tk_move_median_to_first(a, first, first + 1, first + (last - first) / 2, last - 1);
         __syncthreads();
-        const double pv = sk[first];
-        const int a = first + 1, len = last - a;
-        const int per = (len + blockDim.x - 1) / blockDim.x;
-        const int i0 = a + threadIdx.x * per, i1 = min(last, i0 + per);
-        int cl = 0, cr = 0;
-        for (int i = i0; i < i1; ++i) {
-            cl += !(sk[i] < pv);
-            cr += !(pv < sk[i]);
-        }
-        int ex;
-        const int tot = wg_scan_shfl(cl | (cr << 16), sh, &ex);
-        const int nL = tot & 0xffff, nR = tot >> 16;
-        int kl = ex & 0xffff, kr = ex >> 16;
-        for (int i = i0; i < i1; ++i) {
-            if (!(sk[i] < pv)) lpos[kl++] = i;
-            if (!(pv < sk[i])) rpos[nR - 1 - kr++] = i;
-        }
-        const int mm = min(nL, nR);
-        if (threadIdx.x == 0) sh[16] = mm;
-        __syncthreads();
-        for (int q = threadIdx.x; q < mm; q += blockDim.x)
-            if (lpos[q] >= rpos[q]) atomicMin(&sh[16], q);
-        __syncthreads();
-        const int m = sh[16];
-        int cut = m < nL ? lpos[m] : last;
-        if (m >= 1 && rpos[m - 1] < cut) cut = rpos[m - 1];
-        for (int q = threadIdx.x; q < m; q += blockDim.x) {
-            const int x = lpos[q], y = rpos[q];
-            const double tk = sk[x];
-            const int ti = si[x];
-            sk[x] = sk[y];
-            si[x] = si[y];
-            sk[y] = tk;
-            si[y] = ti;
-        }
-        __syncthreads();
+        const int cut = tk_wave_partition(a, first + 1, last, first, L.lpos, L.rpos);
         if (cut <= nth) first = cut;
         else last = cut;
     }
-    if (threadIdx.x == 0) {
-        KVStore st{sk, si};
-        tk_insertion_sort(st, first, last);
+    // the final insertion sort of at most 3 elements: a one-block rank sort
+    const int len = last - first;
+    double v = 0.0;
+    unsigned short vi = 0;
+    int r = 0;
+    if (lane < len) {
+        v = a.k[first + lane];
+        vi = a.idx[first + lane];
+        for (int t = 0; t < len; ++t) {
+            const double u = a.k[first + t];
+            r += (u < v) || (t < lane && !(v < u));
+        }
+    }
+    __syncthreads();
+    if (lane < len) {
+        a.k[first + r] = v;
+        a.idx[first + r] = vi;
     }
     __syncthreads();
 }
 
-__device__ void wg_select_topk(const double* s_orig, double* sk, int* si, int n, int k, int* out, int* lpos,
-                               int* rpos, int* sh) {
-    int P = 1;
-    while (P < n) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) {
-        sk[i] = i < n ? s_orig[i] : 1.0 / 0.0;
-        si[i] = i < n ? i : 0x7fffffff;
+// tk_sort_blocks(a, n) by the wave, n <= 64 * PER; lane 0 keeps the range stack in LDS.
+template <int PER>
+__device__ void tk_wave_sort(const KV16Store& a, int n, TkLds& L) {
+    if (n < 2) return;
+    const int lane = threadIdx.x;
+    int* sf = L.stk;
+    int* sl = L.stk + 64;
+    int* sd = L.stk + 128;
+    for (int i = lane; i < n; i += kTkLanes) L.start[i] = 0;
+    if (lane == 0) {
+        sf[0] = 0;
+        sl[0] = n;
+        sd[0] = 2 * tk_floor_log2(n);
     }
-    __syncthreads();
-    wg_bitonic_pairs(sk, si, P);
-    // Equal scores among sorted positions 0..k (position k: a tie straddling the cut)
-    // make both the kept set and its order depend on the standard library's algorithms;
-    // otherwise the kept set is unique and sorted order is the reference's order.
-    const int last = k < n ? k : n - 1;
-    int tie = 0;
-    for (int p = 1 + threadIdx.x; p <= last; p += blockDim.x) tie |= (sk[p] == sk[p - 1]);
-    if (!__syncthreads_or(tie)) {
-        for (int i = threadIdx.x; i < k; i += blockDim.x) out[i] = si[i];
+    int sp = 1;
+    while (sp > 0) {
         __syncthreads();
-        return;
-    }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        sk[i] = s_orig[i];
-        si[i] = i;
+        --sp;
+        const int first = sf[sp];
+        int last = sl[sp], depth = sd[sp];
+        bool heap = false;
+        while (last - first > 16) {
+            if (depth == 0) {
+                if (lane == 0) {
+                    tk_heap_select(a, first, last, last);
+                    tk_sort_heap(a, first, last);
+                }
+                for (int i = first + lane; i < last; i += kTkLanes) L.start[i] = 1;
+                heap = true;
+                break;
+            }
+            --depth;
+            if (lane == 0) tk_move_median_to_first(a, first, first + 1, first + (last - first) / 2, last - 1);
+            __syncthreads();
+            const int cut = tk_wave_partition(a, first + 1, last, first, L.lpos, L.rpos);
+            if (lane == 0) {
+                sf[sp] = cut;
+                sl[sp] = last;
+                sd[sp] = depth;
+            }
+            ++sp;
+            last = cut;
+        }
+        if (!heap && lane == 0 && first < n) L.start[first] = 1;
     }
     __syncthreads();
-    if (k < n) wg_nth_element_kv(sk, si, n, k, lpos, rpos, sh);
-    if (threadIdx.x == 0) {
-        KVStore st{sk, si};
-        tk_sort(st, k);
-        for (int i = 0; i < k; ++i) out[i] = si[i];
-    }
-    __syncthreads();
+    tk_wave_block_rank_sort<PER>(a, n, L.start);
 }
 
-__global__ void __launch_bounds__(kBlock) k_topk(Dev d, Work w) {
-    __shared__ double s_orig[kMaxPrims];
-    __shared__ double sk[kMaxPrims];
-    __shared__ int si[kMaxPrims];
-    __shared__ int outl[kTopK];
-    __shared__ int lpos[kMaxPrims], rpos[kMaxPrims], shs[32];
+// The list's scores are in L.key[0..n) with L.idx[i] = i; leaves the kept k indices
+// (k <= 64 * PER), in the reference's order, in L.idx[0..k).
+template <int PER>
+__device__ void tk_wave_select(TkLds& L, int n, int k) {
+    const KV16Store a{L.key, L.idx};
+    if (k < n) tk_wave_nth_element(a, k, n, L);
+    tk_wave_sort<PER>(a, k, L);
+}
+
+__global__ void __launch_bounds__(kTkLanes) k_topk(Dev d, Work w) {
+    __shared__ TkLds L;
+    const int lane = threadIdx.x;
     const int total = *w.n_pairs_total;
     for (int task = blockIdx.x; task < total * 6; task += gridDim.x) {
         const int p = task / 6, list = task % 6;
@@ -582,7 +680,7 @@ __global__ void __launch_bounds__(kBlock) k_topk(Dev d, Work w) {
         const Q qx = ld4(d.q, X);
         const int kind = list >> 1;  // 0 vertices, 1 edges, 2 faces
         const int n = kind == 0 ? d.nv[X] : (kind == 1 ? d.ne[X] : d.nf[X]);
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        for (int i = lane; i < n; i += kTkLanes) {
             double sc;
             if (kind == 0) {
                 sc = point_box_distance(world_vertex(d, X, i, px, qx), by);
@@ -600,46 +698,48 @@ __global__ void __launch_bounds__(kBlock) k_topk(Dev d, Work w) {
                 if (b < sc) sc = b;
                 if (c < sc) sc = c;
             }
-            s_orig[i] = sc;
+            L.key[i] = sc;
+            L.idx[i] = (unsigned short)i;
         }
         __syncthreads();
         const int k = n < kTopK ? n : kTopK;
-        wg_select_topk(s_orig, sk, si, n, k, outl, lpos, rpos, shs);
-        for (int i = threadIdx.x; i < k; i += blockDim.x) w.lists[(p * 6 + list) * kTopK + i] = outl[i];
-        if (threadIdx.x == 0) w.lcnt[p * 6 + list] = k;
+        tk_wave_select<(kTopK + kTkLanes - 1) / kTkLanes>(L, n, k);
+        for (int i = lane; i < k; i += kTkLanes) w.lists[(p * 6 + list) * kTopK + i] = L.idx[i];
+        if (lane == 0) w.lcnt[p * 6 + list] = k;
         __syncthreads();
     }
 }
 
 void launch_topk(const Dev& d, const Work& w, int max_pairs, hipStream_t st) {
-    const int cap = w.grid_topk > 0 ? w.grid_topk : 2048;
+    const int cap = w.grid_topk > 0 ? w.grid_topk : 4096;
     int grid = max_pairs * 6;
     if (grid > cap) grid = cap;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_topk, dim3(grid), dim3(kBlock), 0, st, d, w);
+    hipLaunchKernelGGL(k_topk, dim3(grid), dim3(kTkLanes), 0, st, d, w);
 }
 
-// Standalone selection on given scores (one work-group).
-__global__ void __launch_bounds__(kBlock) k_select_scores(const double* scores, int n, int k, int* out, int* out_n) {
-    __shared__ double s_orig[kMaxPrims];
-    __shared__ double sk[kMaxPrims];
-    __shared__ int si[kMaxPrims];
-    __shared__ int outl[kMaxPrims];
-    __shared__ int lpos[kMaxPrims], rpos[kMaxPrims], shs[32];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s_orig[i] = scores[i];
+// Standalone selection on given scores (one wave).
+__global__ void __launch_bounds__(kTkLanes) k_select_scores(const double* scores, int n, int k, int* out,
+                                                           int* out_n) {
+    __shared__ TkLds L;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < n; i += kTkLanes) {
+        L.key[i] = scores[i];
+        L.idx[i] = (unsigned short)i;
+    }
     __syncthreads();
     const int kk = k < n ? k : n;
     if (kk <= 0) {
-        if (threadIdx.x == 0) *out_n = 0;
+        if (lane == 0) *out_n = 0;
         return;
     }
-    wg_select_topk(s_orig, sk, si, n, kk, outl, lpos, rpos, shs);
-    for (int i = threadIdx.x; i < kk; i += blockDim.x) out[i] = outl[i];
-    if (threadIdx.x == 0) *out_n = kk;
+    tk_wave_select<kTkPer>(L, n, kk);
+    for (int i = lane; i < kk; i += kTkLanes) out[i] = L.idx[i];
+    if (lane == 0) *out_n = kk;
 }
 
 void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, hipStream_t st) {
-    hipLaunchKernelGGL(k_select_scores, dim3(1), dim3(kBlock), 0, st, scores, n, k, out, out_n);
+    hipLaunchKernelGGL(k_select_scores, dim3(1), dim3(kTkLanes), 0, st, scores, n, k, out, out_n);
 }
 
 // ------------------------------------------------------------------------- poses

@@ -9,12 +9,13 @@
 // algorithms (Musser's introspective selection and sorting: median-of-three pivot moved
 // to the front, unguarded Hoare partition, insertion sort on small ranges, heap
 // selection / heap sort once the 2*floor(log2 n) depth budget is spent) step for step.
-// They run on one lane over an LDS-resident list and only for lists with equal kept
-// scores; the common case is decided by a parallel sort (see geom.hip).
 //
 // The algorithm is written once over an element store: IdxStore permutes indices and
-// compares their scores (the reference's formulation), KVStore moves (score, index)
-// pairs so that a comparison is a single load. Both produce the same permutation.
+// compares their scores (the reference's formulation), KVStore / KV16Store move
+// (score, index) pairs so that a comparison is a single load. All produce the same
+// permutation. tk_nth_element_lr and tk_sort_blocks restate the two algorithms in the
+// form the device runs them in parallel (one wave per list, geom.hip): partitions
+// through the positions they swap, final insertion sorts as per-block rank sorts.
 #pragma once
 #include "dmath.h"
 
@@ -27,6 +28,7 @@ struct IdxStore {
     ND_HD V get(int i) const { return a[i]; }
     ND_HD void set(int i, V v) const { a[i] = v; }
     ND_HD bool less(V x, V y) const { return s[x] < s[y]; }
+    ND_HD IdxStore offset(int o) const { return IdxStore{a + o, s}; }
 };
 
 struct KVPair {
@@ -40,6 +42,19 @@ struct KVStore {
     ND_HD V get(int i) const { KVPair p; p.k = k[i]; p.i = idx[i]; return p; }
     ND_HD void set(int i, V v) const { k[i] = v.k; idx[i] = v.i; }
     ND_HD bool less(V x, V y) const { return x.k < y.k; }
+    ND_HD KVStore offset(int o) const { return KVStore{k + o, idx + o}; }
+};
+
+// (score, index) pairs with 16-bit indices (lists of at most 65536 primitives): the
+// device's LDS-resident list.
+struct KV16Store {
+    double* k;
+    unsigned short* idx;
+    typedef KVPair V;
+    ND_HD V get(int i) const { KVPair p; p.k = k[i]; p.i = idx[i]; return p; }
+    ND_HD void set(int i, V v) const { k[i] = v.k; idx[i] = (unsigned short)v.i; }
+    ND_HD bool less(V x, V y) const { return x.k < y.k; }
+    ND_HD KV16Store offset(int o) const { return KV16Store{k + o, idx + o}; }
 };
 
 template <typename St>
@@ -243,7 +258,58 @@ ND_HD int tk_partition_lr(const St& a, int lo, int hi, int pivot, int* lpos, int
     return cut;
 }
 
-// tk_nth_element with every partition done by tk_partition_lr.
+// Stable sort of every block of [0, n): blocks begin at the positions p with start[p]
+// set (start[0] is set). Element i of block [s, e) moves to
+//   s + #{j in [s, e): a_j < a_i} + #{j in [s, i): a_j equal to a_i}.
+// libstdc++'s insertion sorts move an element left past strictly greater ones only, so
+// they are stable sorts; the introsort loop leaves [0, n) as blocks in which every
+// element is no greater than any element of a later block (Hoare partition around a
+// pivot kept at the block's first position), so its final insertion sort over [0, n)
+// moves no element out of its block and is this per-block stable sort. tmp holds n
+// elements.
+template <typename St>
+ND_HD void tk_block_rank_sort(const St& a, int n, const unsigned char* start, typename St::V* tmp) {
+    for (int i = 0; i < n; ++i) {
+        int s = i, e = i + 1;
+        while (!start[s]) --s;
+        while (e < n && !start[e]) ++e;
+        const typename St::V v = a.get(i);
+        int r = s;
+        for (int j = s; j < e; ++j) {
+            const typename St::V u = a.get(j);
+            r += a.less(u, v) || (j < i && !a.less(v, u));
+        }
+        tmp[r] = v;
+    }
+    for (int i = 0; i < n; ++i) a.set(i, tmp[i]);
+}
+
+// The first q in [0, mm) with lpos[q] >= rpos[q], or mm if there is none, for an
+// ascending lpos and a descending rpos (the test is monotone in q), found as the device
+// finds it with W = 64 lanes: probes at q = j * step (j < nq, step = ceil(mm / W),
+// nq = ceil(mm / step)) bracket it to ((f - 1) * step, min(f * step, mm)], f the first
+// probe that holds (nq if none), and a second round of probes inside the bracket
+// (fewer than step <= W positions) finds it.
+template <typename T>
+ND_HD int tk_first_crossing(const T* lpos, const T* rpos, int mm) {
+    const int W = 64;
+    const int step = (mm + W - 1) / W;
+    const int nq = step ? (mm + step - 1) / step : 0;
+    int f = nq;
+    for (int j = 0; j < nq; ++j)
+        if (lpos[j * step] >= rpos[j * step]) {
+            f = j;
+            break;
+        }
+    if (f == 0) return 0;
+    const int qa = (f - 1) * step + 1, qb = f * step < mm ? f * step : mm;
+    for (int j = 0; j < W; ++j)
+        if (qa + j < qb && lpos[qa + j] >= rpos[qa + j]) return qa + j;
+    return qb;
+}
+
+// tk_nth_element with every partition done by tk_partition_lr and the final insertion
+// sort of the last (at most 3-element) range done as a one-block rank sort.
 template <typename St>
 ND_HD void tk_nth_element_lr(const St& a, int nth, int n, int* lpos, int* rpos) {
     if (n == 0 || nth == n) return;
@@ -262,7 +328,52 @@ ND_HD void tk_nth_element_lr(const St& a, int nth, int n, int* lpos, int* rpos) 
         if (cut <= nth) first = cut;
         else last = cut;
     }
-    tk_insertion_sort(a, first, last);
+    typename St::V tmp[3];
+    unsigned char start[3] = {1, 0, 0};
+    const St sub = a.offset(first);
+    tk_block_rank_sort(sub, last - first, start, tmp);
+}
+
+// tk_sort restated with the device's steps: partitions by tk_partition_lr, the final
+// ranges of the introsort loop (at most 16 elements) marked as blocks, every element of
+// a heap-sorted range a block of its own, and the final insertion sort done by
+// tk_block_rank_sort. Same permutation as tk_sort. start and tmp hold n entries.
+template <typename St>
+ND_HD void tk_sort_blocks(const St& a, int n, int* lpos, int* rpos, unsigned char* start,
+                          typename St::V* tmp) {
+    if (n < 2) return;
+    for (int i = 0; i < n; ++i) start[i] = 0;
+    int sf[64], sl[64], sd[64];
+    int sp = 1;
+    sf[0] = 0;
+    sl[0] = n;
+    sd[0] = 2 * tk_floor_log2(n);
+    while (sp > 0) {
+        --sp;
+        const int first = sf[sp];
+        int last = sl[sp], depth = sd[sp];
+        bool heap = false;
+        while (last - first > 16) {
+            if (depth == 0) {
+                tk_heap_select(a, first, last, last);
+                tk_sort_heap(a, first, last);
+                for (int i = first; i < last; ++i) start[i] = 1;
+                heap = true;
+                break;
+            }
+            --depth;
+            const int mid = first + (last - first) / 2;
+            tk_move_median_to_first(a, first, first + 1, mid, last - 1);
+            const int cut = tk_partition_lr(a, first + 1, last, first, lpos, rpos);
+            sf[sp] = cut;
+            sl[sp] = last;
+            sd[sp] = depth;
+            ++sp;
+            last = cut;
+        }
+        if (!heap && first < n) start[first] = 1;
+    }
+    tk_block_rank_sort(a, n, start, tmp);
 }
 
 // nth_element(a, a + nth, a + n) with the store's comparator.

@@ -6,6 +6,8 @@ Bars (from the task definition of this path):
   - time of impact never above the oracle's and within 1e-6 of it;
   - per-frame body poses within 1e-8 m and 1e-8 rad.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -81,6 +83,27 @@ def test_select_top_k_ties(engine_factory):
         got = e.select_top_k(s, k)
         # same kept set in the same order (std::nth_element then std::sort)
         assert ref.tolist() == got.tolist(), (trial, n, k)
+
+
+def test_select_top_k_large_k_and_adversarial(engine_factory):
+    """Kept counts up to the whole list (the device sort's rank step over many blocks),
+    distinct and few-level scores, and McIlroy adversary inputs that drive libstdc++'s
+    introselect and introsort into their heap fallbacks (tests/golden)."""
+    e = engine_factory(scenes.make_config("c1"))
+    rng = np.random.default_rng(5)
+    for trial in range(120):
+        n = int(rng.integers(1, 2049))
+        k = int(rng.integers(1, n + 1))
+        if trial % 3 == 0:
+            s = rng.permutation(n).astype(np.float64)
+        else:
+            s = rng.integers(0, 1 + trial % 9, size=n).astype(np.float64)
+        assert O.select_top_k(s, k).tolist() == e.select_top_k(s, k).tolist(), (trial, n, k)
+    adv = np.load(os.path.join(os.path.dirname(__file__), "golden", "topk_adversarial.npz"))
+    for name in sorted(adv.files):
+        s = adv[name].astype(np.float64)
+        k = int(name.split("_k")[1])
+        assert O.select_top_k(s, k).tolist() == e.select_top_k(s, k).tolist(), name
 
 
 @pytest.mark.parametrize("name", ["c1", "tie"])

@@ -12,6 +12,8 @@ Reference tests restated (file:line in W-Moorer/NexDynIPC):
 """
 import math
 
+import os
+
 import numpy as np
 import pytest
 
@@ -186,6 +188,20 @@ def test_select_top_k_without_ties_is_a_full_sort():
         s = rng.permutation(n).astype(np.float64) * 0.5
         got = O.select_top_k(s, 32)
         assert got.tolist() == np.argsort(s, kind="stable")[:32].tolist()
+
+
+def test_select_top_k_adversarial_inputs_keep_the_smallest():
+    """On the adversary inputs (tests/golden/topk_adversarial.npz: libstdc++'s heap
+    fallbacks taken) the kept scores are still the k smallest, in ascending order, with
+    distinct indices."""
+    adv = np.load(os.path.join(os.path.dirname(__file__), "golden", "topk_adversarial.npz"))
+    assert len(adv.files) == 68
+    for name in adv.files:
+        s = adv[name].astype(np.float64)
+        k = int(name.split("_k")[1])
+        got = O.select_top_k(s, k)
+        assert len(set(got.tolist())) == k
+        assert np.array_equal(s[got], np.sort(s)[:k]), name
 
 
 def test_scene_min_distance_brute_force_small():
