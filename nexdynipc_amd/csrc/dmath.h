@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "elemfn.h"
+#include "vecsum.h"
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>

@@ -483,12 +483,11 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
                 T[OG + t] = -(m * (0.0 * X[6 * b] + kG * X[6 * b + 1] + 0.0 * X[6 * b + 2]));
             }
             if (li < nj) {  // ConstraintForm::value: lambda.C + mu/2 |C|^2 per joint
-                const int j = sc.j0 + li;
+                const int j = sc.j0 + li;  // Joint.cpp:5-17, Eigen's order (vecsum.h)
                 double C[6];
                 joint_C(d, j, X, C);
-                double t1 = 0.0, sq = 0.0;
-                for (int k = 0; k < 6; ++k) t1 += d.jlambda[6 * j + k] * C[k];
-                for (int k = 0; k < 6; ++k) sq += C[k] * C[k];
+                const double t1 = eigen_dot6(d.jlambda + 6 * j, C);
+                const double sq = eigen_sqnorm6(C);
                 T[OJ + t] = t1 + 0.5 * d.jmu[j] * sq;
             }
             if (li < sc.nc) {  // DistanceBarrierForm::value, FrictionForm::value
@@ -606,9 +605,8 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
                     xb[q] = B >= 0 ? X0[6 * B + q] + ak * DX0[6 * B + q] : 0.0;
                 }
                 joint_C_x(d, j, xa, xb, C);
-                double t1 = 0.0, sq = 0.0;
-                for (int q = 0; q < 6; ++q) t1 += d.jlambda[6 * j + q] * C[q];
-                for (int q = 0; q < 6; ++q) sq += C[q] * C[q];
+                const double t1 = eigen_dot6(d.jlambda + 6 * j, C);
+                const double sq = eigen_sqnorm6(C);
                 Tk[OJ + tt] = t1 + 0.5 * d.jmu[j] * sq;
             }
             if (li < sc.nc) {
@@ -658,24 +656,44 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
 }
 
 // (sum_i a0[i] b[i], sum_i a1[i] b[i]) left to right; T: kTermLds doubles of LDS.
+// (a0.dot(b), a1.dot(b)) for dynamic-size vectors, in Eigen's order (vecsum.h): the
+// four packet-lane chains of each product sum run on lanes 0..7 of the first wavefront
+// (products stored per chain, contiguous, so each lane streams its own run), then the
+// chains combine with the leftover terms. A chain starts from -0.0, the exact identity
+// of IEEE addition, so its first add reproduces Eigen's start at t_r.
 __device__ void serial_dot2(const double* a0, const double* a1, const double* b, int n, double* red, double* o0,
                             double* o1, double* T) {
-    constexpr int W = kTermLds / 2;
-    double acc = 0.0;
-    for (int lo = 0; lo < n; lo += W) {
-        const int cnt = min(W, n - lo);
+    constexpr int Q = kTermLds / 8;  // terms per chain and window (a multiple of 2)
+    static_assert(Q % 2 == 0, "chain windows start 16-byte aligned");
+    const int n4 = n >= 4 ? n - n % 4 : 0;
+    double acc = -0.0;  // lane l < 8: chain l % 4 of product l / 4
+    for (int lo = 0; lo < n4; lo += 4 * Q) {
+        const int cnt = min(4 * Q, n4 - lo);  // a multiple of 4
         for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-            T[t] = a0[lo + t] * b[lo + t];
-            T[W + t] = a1[lo + t] * b[lo + t];
+            const int i = lo + t, r = t & 3, m = t >> 2;
+            T[r * Q + m] = a0[i] * b[i];
+            T[(4 + r) * Q + m] = a1[i] * b[i];
         }
         __syncthreads();
-        if (threadIdx.x < 2) acc = lds_serial_sum(acc, T + threadIdx.x * W, cnt);
+        if (threadIdx.x < 8) acc = lds_serial_sum(acc, T + threadIdx.x * Q, cnt >> 2);
         __syncthreads();
     }
-    if (threadIdx.x < 2) red[threadIdx.x] = acc;
+    if (threadIdx.x < 8) red[threadIdx.x] = acc;
     __syncthreads();
-    *o0 = red[0];
-    *o1 = red[1];
+    double v[2];
+    for (int k = 0; k < 2; ++k) {
+        const double* a = k ? a1 : a0;
+        auto t = [&](int i) { return a[i] * b[i]; };
+        if (n < 4) {
+            v[k] = eigen_vec_sum(n, t);
+        } else {
+            const int n2 = n - n % 2;
+            v[k] = eigen_sum_finish(red[4 * k], red[4 * k + 1], red[4 * k + 2], red[4 * k + 3], n,
+                                    n2 > n4 ? t(n4) : 0.0, n2 > n4 ? t(n4 + 1) : 0.0, (n % 2) ? t(n - 1) : 0.0);
+        }
+    }
+    *o0 = v[0];
+    *o1 = v[1];
     __syncthreads();
 }
 

@@ -965,12 +965,12 @@ struct Solver {
         }
     }
     double joint_value(const Joint& j, const vector<double>& x) const {
+        // Joint.cpp:5-17: lambda.dot(C) and C.squaredNorm() over dim() = 6 entries are
+        // Eigen reductions of dynamic-size vectors (vecsum.h)
         double C[6];
         joint_C(j, x, C);
-        double t1 = 0.0;
-        for (int i = 0; i < 6; ++i) t1 += j.lambda[i] * C[i];
-        double sq = 0.0;
-        for (int i = 0; i < 6; ++i) sq += C[i] * C[i];
+        const double t1 = nd::eigen_dot6(j.lambda, C);
+        const double sq = nd::eigen_sqnorm6(C);
         return t1 + 0.5 * j.mu * sq;
     }
     void joint_grad(const Joint& j, const vector<double>& x, vector<double>& g) const {
@@ -1090,11 +1090,10 @@ struct Solver {
                        const vector<double>& cg, double alpha0) const {
         double alpha = alpha0;
         const double fx = obj_value(x);
-        double descent = 0.0;
-        for (size_t i = 0; i < g.size(); ++i) descent += g[i] * dir[i];
+        // grad.dot(dir), contact_grad->dot(dir): Eigen dot products of VectorXd (vecsum.h)
+        const double descent = nd::eigen_vec_sum(int(g.size()), [&](int i) { return g[i] * dir[i]; });
         const double fc = contact_value(x);
-        double cdesc = 0.0;
-        for (size_t i = 0; i < cg.size(); ++i) cdesc += cg[i] * dir[i];
+        const double cdesc = nd::eigen_vec_sum(int(cg.size()), [&](int i) { return cg[i] * dir[i]; });
         if (descent > 0) return 0.0;
         vector<double> xn(x.size());
         for (int it = 0; it < 20; ++it) {

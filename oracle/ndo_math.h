@@ -22,6 +22,7 @@
 // rounded IEEE operations only, they give the same bits here as on the GPU, whereas two
 // libms may differ in the last bit. They are pinned against the C library (<= 2 ulp).
 #include "elemfn.h"
+#include "vecsum.h"
 #include <limits>
 #include <algorithm>
 
