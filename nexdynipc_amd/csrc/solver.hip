@@ -587,43 +587,66 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
                 ak *= 0.5;
             }
         }
-        // joints and contacts: one lane per (trial, element), the trial point of the
-        // element's bodies formed in registers
-        for (int t = threadIdx.x; t < K * WB; t += blockDim.x) {
-            const int k = t / WB, tt = t % WB;
+        // joints and contacts: the window's ne elements spread over the work-group, each
+        // element on G lanes (G = min(K, work-group size / ne)); lane g of an element
+        // evaluates trials g, g + G, ... with the element's data loaded once and the
+        // trial points of its bodies formed in registers (alpha_k by k halvings, as the
+        // line search forms it)
+        const int ne = min(WB, max(0, max(nj, sc.nc) - b_lo));
+        if (ne > 0) {
+            const int G = min(K, max(1, (int)blockDim.x / ne));
+            const int tt = threadIdx.x % ne, g = threadIdx.x / ne;
             const int li = b_lo + tt;
-            if (li >= nj && li >= sc.nc) continue;
-            double ak = alpha;
-            for (int q = 0; q < k; ++q) ak *= 0.5;
-            double* Tk = T + k * TS;
-            if (li < nj) {
+            if (g < G && li < nj) {
                 const int j = sc.j0 + li;
                 const int A = d.ja[j], B = d.jb[j];
-                double xa[6], xb[6], C[6];
+                double x0a[6], dxa[6], x0b[6], dxb[6];
                 for (int q = 0; q < 6; ++q) {
-                    xa[q] = X0[6 * A + q] + ak * DX0[6 * A + q];
-                    xb[q] = B >= 0 ? X0[6 * B + q] + ak * DX0[6 * B + q] : 0.0;
+                    x0a[q] = X0[6 * A + q];
+                    dxa[q] = DX0[6 * A + q];
+                    x0b[q] = B >= 0 ? X0[6 * B + q] : 0.0;
+                    dxb[q] = B >= 0 ? DX0[6 * B + q] : 0.0;
                 }
-                joint_C_x(d, j, xa, xb, C);
-                const double t1 = eigen_dot6(d.jlambda + 6 * j, C);
-                const double sq = eigen_sqnorm6(C);
-                Tk[OJ + tt] = t1 + 0.5 * d.jmu[j] * sq;
+                for (int k = g; k < K; k += G) {
+                    double ak = alpha;
+                    for (int q = 0; q < k; ++q) ak *= 0.5;
+                    double xa[6], xb[6], C[6];
+                    for (int q = 0; q < 6; ++q) {
+                        xa[q] = x0a[q] + ak * dxa[q];
+                        xb[q] = B >= 0 ? x0b[q] + ak * dxb[q] : 0.0;
+                    }
+                    joint_C_x(d, j, xa, xb, C);
+                    const double t1 = eigen_dot6(d.jlambda + 6 * j, C);
+                    const double sq = eigen_sqnorm6(C);
+                    T[k * TS + OJ + tt] = t1 + 0.5 * d.jmu[j] * sq;
+                }
             }
-            if (li < sc.nc) {
+            if (g < G && li < sc.nc) {
                 const long long c = sc.cb + li;
                 const int A = w.cint[5 * c], B = w.cint[5 * c + 1];
-                double xa[6], xb[6];
+                double x0a[6], dxa[6], x0b[6], dxb[6];
                 for (int q = 0; q < 6; ++q) {
-                    xa[q] = X0[6 * A + q] + ak * DX0[6 * A + q];
-                    xb[q] = X0[6 * B + q] + ak * DX0[6 * B + q];
+                    x0a[q] = X0[6 * A + q];
+                    dxa[q] = DX0[6 * A + q];
+                    x0b[q] = X0[6 * B + q];
+                    dxb[q] = DX0[6 * B + q];
                 }
-                Tk[OB + tt] = sc.has_barrier ? bar_v(contact_dist_x(w, c, xa, xb), sc.dhat, sc.kappa) : 0.0;
-                double fv = 0.0;
-                if (sc.has_friction && li < sc.nfc) {
-                    const double N = w.nf[(long long)sc.s * kMaxContacts + li];
-                    if (N > 0.0) fv = fric_v(fric_u_x(d, w, c, xa, xb, sc.fdt), N, sc.fmu, sc.feps);
+                const bool fr = sc.has_friction && li < sc.nfc;
+                const double N = fr ? w.nf[(long long)sc.s * kMaxContacts + li] : 0.0;
+                for (int k = g; k < K; k += G) {
+                    double ak = alpha;
+                    for (int q = 0; q < k; ++q) ak *= 0.5;
+                    double xa[6], xb[6];
+                    for (int q = 0; q < 6; ++q) {
+                        xa[q] = x0a[q] + ak * dxa[q];
+                        xb[q] = x0b[q] + ak * dxb[q];
+                    }
+                    double* Tk = T + k * TS;
+                    Tk[OB + tt] = sc.has_barrier ? bar_v(contact_dist_x(w, c, xa, xb), sc.dhat, sc.kappa) : 0.0;
+                    double fv = 0.0;
+                    if (fr && N > 0.0) fv = fric_v(fric_u_x(d, w, c, xa, xb, sc.fdt), N, sc.fmu, sc.feps);
+                    Tk[OF + tt] = fv;
                 }
-                Tk[OF + tt] = fv;
             }
         }
         __syncthreads();
@@ -1055,7 +1078,132 @@ __device__ bool ldlt6_solve(const double* A, const double* b, double* x) {
 }
 
 // Returns 0 solved, 1 zero pivot, 2 a component exceeds kDenseBodies.
-constexpr int kSmallRows = 24;  // components of up to 4 bodies are factored in LDS
+constexpr int kSmallRows = 24;  // components of up to 4 bodies: one wavefront each
+constexpr int kSmallElems = kSceneScratch / 4;  // element list of each wavefront
+
+// Orders the wavefront's LDS accesses (no work-group barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The component solve below for one component of at most 4 bodies (m <= kSmallRows),
+// by the calling wavefront alone: the same assembly (diagonal blocks, then the
+// elements one at a time in the same order), the same right-looking LDL^T and the same
+// substitutions, entry for entry and operation for operation, with wavefront
+// synchronisation in place of work-group barriers; the element list is compacted
+// with ballots. M: the wavefront's m x m LDS tile; el: its element list. Returns false
+// on a zero or non-finite pivot.
+__device__ bool small_component_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr,
+                                      const double* rhs, double* xo, const CompLds& L, int c0, int c1, double* M,
+                                      int* el) {
+    const int lane = threadIdx.x & 63;
+    const int c = L.lab[L.list[c0]];
+    const int mb = c1 - c0, m = 6 * mb;
+    auto lpos = [&](int lb) {
+        int lo = c0, hi = c1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (L.list[mid] < lb) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo - c0;
+    };
+    for (int t = lane; t < m * m; t += 64) M[t] = 0.0;
+    wave_sync();
+    for (int t = lane; t < 36 * mb; t += 64) {
+        const int tb = t / 36, r = (t % 36) / 6, q = t % 6;
+        M[(6 * tb + r) * m + 6 * tb + q] = w.D[36 * (sc.b0 + L.list[c0 + tb]) + r * 6 + q];
+    }
+    int ne = 0;
+    for (int kind = 0; kind < 3; ++kind) {
+        const int cnt = kind == 0 ? sc.j1 - sc.j0 : sc.nc;
+        for (int base = 0; base < cnt; base += 64) {
+            const int i = base + lane;
+            bool f = false;
+            if (i < cnt) {
+                if (kind == 0) {
+                    const int j = sc.j0 + i;
+                    f = d.jtype[j] == 1 && L.lab[d.ja[j] - sc.b0] == c;
+                } else {
+                    const int act = w.cactive[(long long)sc.s * kMaxContacts + i];
+                    f = (act & (kind == 1 ? 1 : 2)) && L.lab[w.cint[5 * (sc.cb + i)] - sc.b0] == c;
+                }
+            }
+            const unsigned long long bal = __ballot(f);
+            if (f) el[ne + __popcll(bal & ((1ull << lane) - 1ull))] = (kind << 24) | i;
+            ne += __popcll(bal);
+        }
+    }
+    wave_sync();
+    for (int e = 0; e < ne; ++e) {
+        const int code = el[e], kind = code >> 24, i = code & 0xffffff;
+        const int t = lane;
+        if (kind == 0) {
+            const int j = sc.j0 + i;
+            const int pa = lpos(d.ja[j] - sc.b0), pb = lpos(d.jb[j] - sc.b0);
+            const double* jb = w.jblk + 108 * j + 72;  // rows a, cols b
+            if (t < 36) {
+                const int r = t / 6, q = t % 6;
+                if (pa > pb) M[(6 * pa + r) * m + 6 * pb + q] += jb[r * 6 + q];
+                else M[(6 * pb + r) * m + 6 * pa + q] += jb[q * 6 + r];
+            }
+        } else {
+            const long long cc = sc.cb + i, ce = (long long)sc.s * kMaxContacts + i;
+            const int pa = lpos(w.cint[5 * cc] - sc.b0), pb = lpos(w.cint[5 * cc + 1] - sc.b0);
+            const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
+            if (kind == 1) {
+                if (t < 9) {
+                    const int r = t / 3, q = t % 3;
+                    const double* cd = w.cdbl + 7 * cc;
+                    M[(6 * hi + r) * m + 6 * lo + q] += (-(b2arr[i] * (cd[3 + r] * cd[3 + q]))) * sc.h2;
+                }
+            } else if (t < 36) {
+                const int r = t / 6, q = t % 6;
+                const double* blk = w.cblk + 144 * ce + (pa > pb ? 72 : 108);
+                M[(6 * hi + r) * m + 6 * lo + q] += blk[r * 6 + q];
+            }
+        }
+        wave_sync();
+    }
+    for (int k = 0; k < m; ++k) {
+        const double Dk = M[k * m + k];
+        if (Dk == 0.0 || !isfinite(Dk)) return false;
+        const int i = k + 1 + lane;
+        if (i < m) {
+            const double l = M[i * m + k] / Dk;
+            M[i * m + k] = l;
+            M[k * m + i] = l;
+        }
+        wave_sync();
+        const int r = m - k - 1;
+        for (int t = lane; t < r * r; t += 64) {
+            const int ii = k + 1 + t / r, j = k + 1 + t % r;
+            if (j > ii) continue;
+            const double lik = M[ii * m + k], ljk = M[j * m + k];
+            if (ii == j) M[ii * m + ii] -= (lik * lik) * Dk;
+            else M[ii * m + j] -= (lik * Dk) * ljk;
+        }
+        wave_sync();
+    }
+    // row `lane` of x in a register: forward x_i -= L_ik x_k (k ascending), x_i /= D_i,
+    // backward x_k -= L_ik x_i (i descending)
+    double x = lane < m ? rhs[6 * L.list[c0 + lane / 6] + lane % 6] : 0.0;
+    for (int k = 0; k < m; ++k) {
+        const double xk = __shfl(x, k);
+        if (lane > k && lane < m) x -= M[k * m + lane] * xk;
+    }
+    if (lane < m) x /= M[lane * m + lane];
+    for (int i = m - 1; i >= 0; --i) {
+        const double xi = __shfl(x, i);
+        if (lane < i) x -= M[i * m + lane] * xi;
+    }
+    if (lane < m) xo[6 * L.list[c0 + lane / 6] + lane % 6] = x;
+    wave_sync();
+    return true;
+}
+
 __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* rhs,
                                double* xo, CompLds& L, int* ok_sh) {
     build_components(d, w, sc, L);
@@ -1067,17 +1215,38 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
         if (L.cnt[L.lab[li]] > 1) continue;
         if (!ldlt6_solve(w.D + 36 * (sc.b0 + li), rhs + 6 * li, xo + 6 * li)) atomicAnd(ok_sh, 0);
     }
+    // components of up to 4 bodies: wavefront v takes the small components numbered
+    // v, v + 4, ... in list order
+    __shared__ double Msmall[kBlk / 64][kSmallRows * kSmallRows];
+    {
+        const int wv = threadIdx.x >> 6;
+        int* el = w.sort_idx + (long long)sc.s * kSceneScratch + wv * kSmallElems;
+        int nsmall = 0;
+        bool wave_ok = true;
+        for (int c0 = 0; c0 < L.ncoupled && wave_ok;) {
+            const int mb = L.cnt[L.lab[L.list[c0]]];
+            if (6 * mb <= kSmallRows) {
+                if ((nsmall & (kBlk / 64 - 1)) == wv)
+                    wave_ok = small_component_solve(d, w, sc, b2arr, rhs, xo, L, c0, c0 + mb, Msmall[wv], el);
+                ++nsmall;
+            }
+            c0 += mb;
+        }
+        if (!wave_ok && (threadIdx.x & 63) == 0) atomicAnd(ok_sh, 0);
+    }
     __syncthreads();
     if (!*ok_sh) return 1;
     double* Mg = w.dense + (long long)sc.s * kDenseRows * kDenseRows;
-    __shared__ double Msmall[kSmallRows * kSmallRows];  // components of up to 4 bodies, in LDS
     for (int c0 = 0; c0 < L.ncoupled;) {
         const int c = L.lab[L.list[c0]];  // the component's label (its first member)
         const int mb = L.cnt[c];
         const int c1 = c0 + mb;
         const int m = 6 * mb;
-        const bool small = m <= kSmallRows;
-        double* M = small ? Msmall : Mg;
+        if (m <= kSmallRows) {
+            c0 = c1;
+            continue;
+        }
+        double* M = Mg;
         // local row of body b: 6 * (index of b in the component's ascending list)
         auto lpos = [&](int lb) {
             int lo = c0, hi = c1;
