@@ -222,6 +222,7 @@ int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= device) return -2;
     if (hipSetDevice(device) != hipSuccess) return -3;
+    nd::geom_configure();
     auto* c = new ndipc_ctx();
     c->device = device;
     if (hipStreamCreate(&c->stream) != hipSuccess) {
@@ -373,6 +374,8 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     d = Dev{};
     d.S = S;
     d.N = N;
+    d.max_nb = 0;
+    for (int s = 0; s < S; ++s) d.max_nb = std::max(d.max_nb, ctx->body_off[s + 1] - ctx->body_off[s]);
     d.body_off = dupload(ctx, ctx->body_off.data(), S + 1);
     d.body_scene = dupload(ctx, body_scene.data(), N);
     d.body_id = dupload(ctx, ctx->body_id.data(), N);

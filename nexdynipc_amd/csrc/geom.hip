@@ -144,17 +144,45 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, Work w) {
         w.node_range[2 * (b0 + i) + 1] = last;
     }
     __syncthreads();
+    // Node boxes: the union of the node's leaf range [first, last], from a sparse table
+    // of power-of-two range unions in the scene's scratch: T_0[k] = box of sorted leaf
+    // k, T_l[k] = T_{l-1}[k] u T_{l-1}[k + 2^(l-1)], built level by level; a range is then
+    // covered by steps of 2^l from `first` plus one entry ending at `last` (l the largest
+    // level stored not above log2 of its length). Min and max are exact, so the box does
+    // not depend on how the range is covered.
+    double* tab = w.sort_key + (long long)s * kSceneScratch;
+    int L = 0;
+    while ((2 << L) <= m && 6 * m * (L + 2) <= kSceneScratch) ++L;  // levels 0..L
+    for (int k = t; k < m; k += kBlock) {
+        const double* bx = w.wbox + 6 * w.leaf_body[b0 + k];
+        for (int q = 0; q < 6; ++q) tab[6 * k + q] = bx[q];
+    }
+    __syncthreads();
+    for (int l = 1; l <= L; ++l) {
+        const double* pv = tab + 6 * m * (l - 1);
+        double* cur = tab + 6 * m * l;
+        const int h = 1 << (l - 1);
+        for (int k = t; k + 2 * h <= m; k += kBlock) {
+            for (int q = 0; q < 3; ++q) cur[6 * k + q] = dmin(pv[6 * k + q], pv[6 * (k + h) + q]);
+            for (int q = 3; q < 6; ++q) cur[6 * k + q] = dmax(pv[6 * k + q], pv[6 * (k + h) + q]);
+        }
+        __syncthreads();
+    }
     for (int i = t; i < m - 1; i += kBlock) {
         const int first = w.node_range[2 * (b0 + i)], last = w.node_range[2 * (b0 + i) + 1];
-        Box3 bx = ldbox(w.wbox, w.leaf_body[b0 + first]);
-        for (int k = first + 1; k <= last; ++k) {
-            const Box3 o = ldbox(w.wbox, w.leaf_body[b0 + k]);
-            bx.mn = vmin(bx.mn, o.mn);
-            bx.mx = vmax(bx.mx, o.mx);
+        const int len = last - first + 1;
+        int l = 0;
+        while (l < L && (2 << l) <= len) ++l;
+        const int S = 1 << l;
+        const double* tl = tab + 6 * m * l;
+        double bx[6];
+        for (int q = 0; q < 6; ++q) bx[q] = tl[6 * (last - S + 1) + q];
+        for (int p = first; p + S - 1 < last; p += S) {
+            for (int q = 0; q < 3; ++q) bx[q] = dmin(bx[q], tl[6 * p + q]);
+            for (int q = 3; q < 6; ++q) bx[q] = dmax(bx[q], tl[6 * p + q]);
         }
         double* o = w.node_box + 6 * (b0 + i);
-        o[0] = bx.mn.x; o[1] = bx.mn.y; o[2] = bx.mn.z;
-        o[3] = bx.mx.x; o[4] = bx.mx.y; o[5] = bx.mx.z;
+        for (int q = 0; q < 6; ++q) o[q] = bx[q];
     }
 }
 
@@ -217,10 +245,104 @@ __device__ __forceinline__ void bp_sort_insert(int* outp, int o, int k, int i, i
     outp[2 * (o + pos) + 1] = j;
 }
 
-__global__ void k_bp_step(Dev d, Work w, int pass) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.N) return;
-    const int s = d.body_scene[i];
+// The scene's tree staged in LDS for the traversals of k_bp_step: internal nodes
+// 0..m-2 (children, union boxes) and the m leaves in sorted order (body index, body id,
+// world box), 112 bytes per leaf. Reads give the same values as the tree in HBM.
+struct BvhLds {
+    int* child;     // [2 (m - 1)]
+    double* nbox;   // [6 (m - 1)]
+    int* lbody;     // [m]
+    int* lbid;      // [m] scene-local id of the leaf's body
+    double* lbox;   // [6 m]
+    unsigned short* stk;  // traversal stacks, kBvhStackBytes
+    unsigned short* hit;  // per-lane hit buffers, kBvhHitBytes
+};
+constexpr int kBvhLdsPerLeaf = 2 * 4 + 6 * 8 + 4 + 4 + 6 * 8;
+constexpr int kBvhStackBytes = 72 * kBlock * 2;  // 72 entries of 2 bytes per lane
+constexpr int kBvhHits = 32;                     // buffered hits per lane
+constexpr int kBvhHitBytes = kBvhHits * kBlock * 2;
+
+__device__ __forceinline__ Box3 box_at(const double* b, int i) {
+    Box3 r;
+    r.mn = v3(b[6 * i], b[6 * i + 1], b[6 * i + 2]);
+    r.mx = v3(b[6 * i + 3], b[6 * i + 4], b[6 * i + 5]);
+    return r;
+}
+
+// lbvh_query over the LDS copy: the same traversal and tests. The node stack of each
+// lane lives in LDS as well (16-bit node indices, entry e of lane t at stk[e * kBlock
+// + t]): a stack in registers with a dynamic index would be placed in scratch memory.
+// Accepted leaves are buffered per lane and handed to on_leaf(leaf) when the buffer is
+// full and after the traversal, so the lanes of a wavefront process their hits in the
+// same few iterations instead of one divergent call per hit. Callers must not depend
+// on the order of the calls (pass 0 counts and takes minima, pass 1 inserts sorted).
+constexpr int kBvhStack = 72;
+template <typename F>
+__device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, double r, F&& on_leaf) {
+    const Box3 qa = box_inflated(bi, r);
+    if (m == 1) return;
+    unsigned short* st = T.stk + threadIdx.x;
+    unsigned short* hb = T.hit + threadIdx.x;
+    int nh = 0;
+    int sp = 1;
+    st[0] = 0;
+    while (sp > 0) {
+        const int node = st[kBlock * --sp];
+        const int cs[2] = {T.child[2 * node], T.child[2 * node + 1]};
+        for (int k = 0; k < 2; ++k) {
+            const int c = cs[k];
+            if (c < 0) {
+                if (T.lbody[~c] > i && box_overlap(qa, box_inflated(box_at(T.lbox, ~c), r))) {
+                    hb[kBlock * nh++] = (unsigned short)(~c);
+                    if (nh == kBvhHits) {
+                        for (int h = 0; h < nh; ++h) on_leaf(hb[kBlock * h]);
+                        nh = 0;
+                    }
+                }
+            } else {
+                if (box_overlap(qa, box_inflated(box_at(T.nbox, c), r)) && sp < kBvhStack)
+                    st[kBlock * sp++] = (unsigned short)c;
+            }
+        }
+    }
+    for (int h = 0; h < nh; ++h) on_leaf(hb[kBlock * h]);
+}
+
+// Grid (scene, chunk of kBlock bodies). With `lds` the work-group first copies its
+// scene's tree into dynamic LDS (kBvhLdsPerLeaf bytes per leaf, sized for the largest
+// scene at launch); otherwise the traversal reads the tree in HBM.
+template <bool lds>
+__global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass) {
+    extern __shared__ __align__(16) unsigned char bp_lds[];
+    const int s = blockIdx.x;
+    const int b0 = d.body_off[s], nb = d.body_off[s + 1] - b0;
+    const int li = blockIdx.y * blockDim.x + threadIdx.x;
+    if (blockIdx.y * blockDim.x >= nb) return;  // the whole work-group: no body of this chunk
+    const int m = w.nleaves[s];
+    BvhLds T{};
+    if (lds) {
+        T.lbox = reinterpret_cast<double*>(bp_lds);
+        T.nbox = T.lbox + 6 * m;
+        T.child = reinterpret_cast<int*>(T.nbox + 6 * (m > 1 ? m - 1 : 0));
+        T.lbody = T.child + 2 * (m > 1 ? m - 1 : 0);
+        T.lbid = T.lbody + m;
+        T.stk = reinterpret_cast<unsigned short*>(bp_lds + size_t(d.max_nb) * kBvhLdsPerLeaf);
+        T.hit = T.stk + kBvhStackBytes / 2;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            const int j = w.leaf_body[b0 + k];
+            T.lbody[k] = j;
+            T.lbid[k] = d.body_id[j];
+            for (int q = 0; q < 6; ++q) T.lbox[6 * k + q] = w.wbox[6 * j + q];
+        }
+        for (int k = threadIdx.x; k < m - 1; k += blockDim.x) {
+            T.child[2 * k] = w.node_child[2 * (b0 + k)];
+            T.child[2 * k + 1] = w.node_child[2 * (b0 + k) + 1];
+            for (int q = 0; q < 6; ++q) T.nbox[6 * k + q] = w.node_box[6 * (b0 + k) + q];
+        }
+        __syncthreads();
+    }
+    if (li >= nb) return;
+    const int i = b0 + li;
     SceneParams& sp = d.sp[s];
     int nc = 0, np = 0;
     if (d.nv[i] > 0) {
@@ -237,8 +359,8 @@ __global__ void k_bp_step(Dev d, Work w, int pass) {
         int any = 0;
         const int oc = pass ? w.boff[i] : 0, op = pass ? w.boff[d.N + i] : 0;
         const double margin = sp.prune_margin;
-        lbvh_query(d, w, s, i, rmax, [&](int j) {
-            const Box3 bj = ldbox(w.wbox, j);
+        const int idi = d.body_id[i];
+        auto hit = [&](int j, int idj, const Box3& bj) {
             const double gap = box_distance(bi, bj);
             if (box_overlap(q0, box_inflated(bj, r0)) && gap <= r0) {
                 if (pass) {
@@ -257,9 +379,12 @@ __global__ void k_bp_step(Dev d, Work w, int pass) {
                     ++np;
                 }
             }
-            if (do2 && !any && box_overlap(q2, box_inflated(bj, r2)) && pair_allowed(d, s, d.body_id[i], d.body_id[j]))
-                any = 1;
-        });
+            if (do2 && !any && box_overlap(q2, box_inflated(bj, r2)) && pair_allowed(d, s, idi, idj)) any = 1;
+        };
+        if constexpr (lds)
+            lbvh_query_lds(T, m, i, bi, rmax, [&](int leaf) { hit(T.lbody[leaf], T.lbid[leaf], box_at(T.lbox, leaf)); });
+        else
+            lbvh_query(d, w, s, i, rmax, [&](int j) { hit(j, d.body_id[j], ldbox(w.wbox, j)); });
         if (!pass) {
             if (do1 && gmin < 1.0 / 0.0)
                 atomicMin(reinterpret_cast<unsigned long long*>(&sp.min_pair_dist), __double_as_longlong(gmin));
@@ -296,17 +421,44 @@ __global__ void __launch_bounds__(kBlock) k_bp_scan(Dev d, Work w) {
 
 __global__ void __launch_bounds__(kBlock) k_scan_counts(const int* cnt, int* off, int* total, int S);
 
+// Dynamic LDS that k_bp_step may request: raised from the 64 KiB default by
+// geom_configure (called at context creation, outside any stream capture).
+static size_t g_bp_lds_cap = 65536;
+
+int geom_configure() {
+    // NDIPC_BP_NO_LDS=1 keeps every traversal on the tree in HBM (the path of scenes too
+    // large for LDS), so that tests can run it on small scenes
+    const char* e = getenv("NDIPC_BP_NO_LDS");
+    if (e && e[0] == '1') {
+        g_bp_lds_cap = 0;
+        return 0;
+    }
+    const size_t want = 131072;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_bp_step<true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) == hipSuccess)
+        g_bp_lds_cap = want;
+    else
+        (void)hipGetLastError();
+    return 0;
+}
+
 void launch_broad_phase_step(const Dev& d, const Work& w, hipStream_t st) {
     if (d.N == 0) return;
-    const dim3 grid((d.N + kBlock - 1) / kBlock);
+    const dim3 grid(d.S, (d.max_nb + kBlock - 1) / kBlock);
+    // the tree of the largest scene in LDS, with the lanes' traversal stacks and hit
+    // buffers, when all fit (694 bodies with 128 KiB); otherwise the tree in HBM
+    const size_t bytes = size_t(d.max_nb) * kBvhLdsPerLeaf + kBvhStackBytes + kBvhHitBytes;
+    const int lds = bytes <= g_bp_lds_cap ? 1 : 0;
     timing_begin("bp_step", st);
-    hipLaunchKernelGGL(k_bp_step, grid, dim3(kBlock), 0, st, d, w, 0);
+    if (lds) hipLaunchKernelGGL(k_bp_step<true>, grid, dim3(kBlock), bytes, st, d, w, 0);
+    else hipLaunchKernelGGL(k_bp_step<false>, grid, dim3(kBlock), 0, st, d, w, 0);
     timing_end("bp_step", st);
     hipLaunchKernelGGL(k_bp_scan, dim3(d.S, 2), dim3(kBlock), 0, st, d, w);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, st, w.pair_cnt, w.pair_off, w.n_pairs_total, d.S);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, st, w.prox_cnt, w.prox_off, w.n_prox_total, d.S);
     timing_begin("bp_step", st);
-    hipLaunchKernelGGL(k_bp_step, grid, dim3(kBlock), 0, st, d, w, 1);
+    if (lds) hipLaunchKernelGGL(k_bp_step<true>, grid, dim3(kBlock), bytes, st, d, w, 1);
+    else hipLaunchKernelGGL(k_bp_step<false>, grid, dim3(kBlock), 0, st, d, w, 1);
     timing_end("bp_step", st);
 }
 

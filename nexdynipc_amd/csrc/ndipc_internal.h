@@ -54,6 +54,7 @@ struct SceneParams {
 
 struct Dev {
     int S, N;
+    int max_nb;            // largest scene body count (host-side launch shapes)
     // scene structure
     const int* body_off;   // [S+1]
     const int* body_scene; // [N]
@@ -157,6 +158,7 @@ void launch_lbvh_build(const Dev& d, const Work& w, hipStream_t st);
 void launch_broad_phase(const Dev& d, const Work& w, int mode, double r, bool use_scene_r, const int* scene_mask,
                         hipStream_t st);
 void launch_broad_phase_step(const Dev& d, const Work& w, hipStream_t st);
+int geom_configure();  // kernel attributes, once per context creation (no stream capture)
 void launch_topk(const Dev& d, const Work& w, int max_pairs, hipStream_t st);
 void launch_pose_from_state(const Dev& d, const Work& w, const double* x, hipStream_t st);
 void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st);

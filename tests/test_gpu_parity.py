@@ -334,6 +334,26 @@ def test_step_graph_replays_match_eager_steps(engine_factory, name):
         ee.set_state(*se)
 
 
+@pytest.mark.parametrize("name", ["tie", "c3x8"])
+def test_broad_phase_tree_in_hbm_matches_tree_in_lds(engine_factory, name):
+    """The step's broad phase traverses a copy of each scene's tree in LDS when the
+    largest scene fits (694 bodies), else the tree in HBM. With NDIPC_BP_NO_LDS=1 at
+    context creation the HBM path runs on these small scenes: same steps, same bits."""
+    sc = perturbed_c2_scene() if name == "tie" else scenes.pack(scenes.config3(n_scenes=8))
+    a = engine_factory(sc)
+    a.step(6)
+    ref = a.get_state(), a.stats()
+    os.environ["NDIPC_BP_NO_LDS"] = "1"
+    try:
+        b = engine_factory(sc)
+    finally:
+        del os.environ["NDIPC_BP_NO_LDS"]
+    b.step(6)
+    for x, y in zip(ref[0], b.get_state()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(ref[1], b.stats())
+
+
 def test_autotune_restores_state_and_keeps_results(engine_factory):
     """ndipc_autotune times launch shapes from the current state and restores it; the
     chosen shapes change speed only: the next steps give the untuned engine's bits."""
