@@ -437,6 +437,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     w.node_range = dalloc<int>(ctx, 2 * size_t(N));
     w.node_box = dalloc<double>(ctx, 6 * size_t(N));
     w.nleaves = dalloc<int>(ctx, S);
+    w.bvh_age = dalloc<int>(ctx, S);  // zero: the first call builds
     w.pair_cnt = dalloc<int>(ctx, S);
     w.pair_off = dalloc<int>(ctx, S + 1);
     w.pairs = dalloc<int>(ctx, 2 * PC);

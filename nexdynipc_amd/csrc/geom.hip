@@ -48,6 +48,15 @@ __device__ __forceinline__ int karras_delta(const unsigned long long* keys, int 
     return __clzll(keys[i] ^ keys[j]);
 }
 
+// Every query of the tree returns the set of leaves whose boxes overlap the query box,
+// and every consumer orders its pairs itself (j descending per i), so results do not
+// depend on the tree's shape: the Morton sort and radix tree are rebuilt every
+// kBvhRebuild-th call of a scene, and the calls in between only refit the node boxes to
+// the current body boxes over the kept topology.
+constexpr int kBvhRebuild = 16;
+
+__device__ void lbvh_node_boxes(const Work& w, int s, int b0, int m);
+
 __global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, Work w) {
     __shared__ unsigned long long keys[kMaxSceneBodies];
     __shared__ int leaf_tmp[kMaxSceneBodies];
@@ -57,6 +66,14 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, Work w) {
     const int t = threadIdx.x;
     const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
     const int nb = b1 - b0;
+    const int age = w.bvh_age[s];
+    __syncthreads();
+    if (t == 0) w.bvh_age[s] = age + 1 == kBvhRebuild ? 0 : age + 1;
+    if (age != 0) {  // refit
+        const int m = w.nleaves[s];
+        if (m > 1) lbvh_node_boxes(w, s, b0, m);
+        return;
+    }
     // compact the mesh bodies (scene order)
     int m = 0;
     for (int base = 0; base < nb; base += kBlock) {
@@ -144,12 +161,17 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, Work w) {
         w.node_range[2 * (b0 + i) + 1] = last;
     }
     __syncthreads();
-    // Node boxes: the union of the node's leaf range [first, last], from a sparse table
-    // of power-of-two range unions in the scene's scratch: T_0[k] = box of sorted leaf
-    // k, T_l[k] = T_{l-1}[k] u T_{l-1}[k + 2^(l-1)], built level by level; a range is then
-    // covered by steps of 2^l from `first` plus one entry ending at `last` (l the largest
-    // level stored not above log2 of its length). Min and max are exact, so the box does
-    // not depend on how the range is covered.
+    lbvh_node_boxes(w, s, b0, m);
+}
+
+// Node boxes: the union of the node's leaf range [first, last], from a sparse table of
+// power-of-two range unions in the scene's scratch: T_0[k] = box of sorted leaf k,
+// T_l[k] = T_{l-1}[k] u T_{l-1}[k + 2^(l-1)], built level by level; a range is then
+// covered by steps of 2^l from `first` plus one entry ending at `last` (l the largest
+// level stored not above log2 of its length). Min and max are exact, so the box does
+// not depend on how the range is covered. Called by the whole work-group, m > 1.
+__device__ void lbvh_node_boxes(const Work& w, int s, int b0, int m) {
+    const int t = threadIdx.x;
     double* tab = w.sort_key + (long long)s * kSceneScratch;
     int L = 0;
     while ((2 << L) <= m && 6 * m * (L + 2) <= kSceneScratch) ++L;  // levels 0..L

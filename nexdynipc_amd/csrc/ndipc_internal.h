@@ -104,6 +104,7 @@ struct Work {
     int* node_range;         // [N*2]
     double* node_box;        // [N*6]
     int* nleaves;            // [S]
+    int* bvh_age;            // [S] LBVH builds and refits since the last full build
     // body pairs
     long long* pair_cap_off; // [S+1] capacity offsets (n(n-1)/2 per scene)
     int* pair_cnt;           // [S]
