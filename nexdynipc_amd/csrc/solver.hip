@@ -1445,6 +1445,12 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     int it = 0;
     const double* Xg = w.x;
     const int maxit = sp.newton_max_iters;
+    // The accepted trial point of a line search is bitwise the next iterate (both are
+    // x + alpha dx with the same alpha), and objective_value_batch sums its terms as
+    // objective_value does: the next iteration takes f(x) and the contact value from
+    // the accepted trial instead of evaluating them again.
+    double fx_next = 0.0, fc_next = 0.0;
+    bool have_f = false;
     for (; it < maxit; ++it) {
         eval_elements(d, w, sc, Xg, b2arr);
         const double ginf = objective_grad(d, w, sc, Xg, G, CG, red);
@@ -1472,8 +1478,8 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
             break;
         }
         // line search (LineSearch.cpp:6-58), alpha0 = min(1, maxStep) = 1
-        double fx, fc;
-        objective_value(d, w, sc, Xg, red, &fx, &fc, nl.terms);
+        double fx = fx_next, fc = fc_next;
+        if (!have_f) objective_value(d, w, sc, Xg, red, &fx, &fc, nl.terms);
         double descent, cdesc;
         serial_dot2(G, CG, DX, sc.nd, red, &descent, &cdesc, nl.terms);
         if (descent > 0) break;
@@ -1490,6 +1496,9 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
                 const bool cok = (cdesc <= 0.0) ? (fcn[k] <= fc + 1e-4 * alpha * cdesc) : (fcn[k] <= fc);
                 if (fn[k] <= fx + 1e-4 * alpha * descent && cok) {
                     accepted = true;
+                    fx_next = fn[k];
+                    fc_next = fcn[k];
+                    have_f = true;
                     break;
                 }
                 alpha *= 0.5;
@@ -1544,12 +1553,20 @@ void launch_newton(const Dev& d, const Work& w, hipStream_t st) {
 __device__ void build_csr(const Work& w, const ScnCtx& sc) {
     int* off = w.csr_off + sc.b0 + sc.s;
     int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    // the contacts' body pairs in LDS: every lane then scans them in the same order, so
+    // each read is a broadcast
+    __shared__ int2 cab[kMaxContacts];
+    for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+        const int* ci = w.cint + 5 * (sc.cb + i);
+        cab[i] = make_int2(ci[0], ci[1]);
+    }
+    __syncthreads();
     for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
         const int b = sc.b0 + li;
         int c = 0;
         for (int i = 0; i < sc.nc; ++i) {
-            const int* ci = w.cint + 5 * (sc.cb + i);
-            c += (ci[0] == b) + (ci[1] == b);
+            const int2 ab = cab[i];
+            c += (ab.x == b) + (ab.y == b);
         }
         off[li + 1] = c;
     }
@@ -1570,9 +1587,9 @@ __device__ void build_csr(const Work& w, const ScnCtx& sc) {
         const int b = sc.b0 + li;
         int k = off[li];
         for (int i = 0; i < sc.nc; ++i) {
-            const int* ci = w.cint + 5 * (sc.cb + i);
-            if (ci[0] == b) ent[k++] = (i << 1) | 0;
-            if (ci[1] == b) ent[k++] = (i << 1) | 1;
+            const int2 ab = cab[i];
+            if (ab.x == b) ent[k++] = (i << 1) | 0;
+            if (ab.y == b) ent[k++] = (i << 1) | 1;
         }
     }
     __syncthreads();
