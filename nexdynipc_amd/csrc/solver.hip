@@ -540,22 +540,31 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
 __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx& sc, int K, const double* X0,
                                       const double* DX0, double alpha, double* red, double* vals, double* cvals,
                                       double* T) {
-    constexpr int WB = kTermLds / (6 * kLsBatch);  // items per window and trial
-    constexpr int TS = 6 * WB;                      // doubles per trial window
-    static_assert(WB % 2 == 0, "windows start 16-byte aligned");
-    constexpr int OI = 0, OG = 2 * WB, OJ = 3 * WB, OB = 4 * WB, OF = 5 * WB;
+    // Each trial owns TS doubles of LDS per round: 3 WBb body terms (two inertia terms
+    // and one gravity term per body) and 3 WBe element terms (joint, barrier, friction),
+    // the two widths chosen per scene so that the fewest rounds cover the bodies and
+    // the elements (widths even: every window starts 16-byte aligned; at most one
+    // element window per work-group of lanes).
+    constexpr int TS = kTermLds / kLsBatch;
     const double* XH = w.xhat;
     const int nj = sc.j1 - sc.j0;
-    int rounds = (sc.n + WB - 1) / WB;
-    rounds = max(rounds, (nj + WB - 1) / WB);
-    rounds = max(rounds, (sc.nc + WB - 1) / WB);
+    const int nel = max(nj, sc.nc);
+    int rounds = 1, WBb = 0, WBe = 0;
+    for (;; ++rounds) {
+        WBb = (sc.n + rounds - 1) / rounds;
+        WBe = (nel + rounds - 1) / rounds;
+        WBb += WBb & 1;
+        WBe += WBe & 1;
+        if (3 * (WBb + WBe) <= TS && WBe <= kBlk) break;  // an element per lane at most
+    }
+    const int OI = 0, OG = 2 * WBb, OJ = 3 * WBb, OB = 3 * WBb + WBe, OF = 3 * WBb + 2 * WBe;
     double acc = 0.0;  // lane L < 5K: accumulator L % 5 of trial L / 5
     const int lk = threadIdx.x / 5, lc = threadIdx.x % 5;
     for (int r = 0; r < rounds; ++r) {
-        const int b_lo = r * WB;
+        const int b_lo = r * WBb, e_lo = r * WBe;
         // inertia and gravity: one lane per body, its K trial points formed in registers
         // (x + alpha_k dx, the expression that filled XTB)
-        for (int tt = threadIdx.x; tt < WB; tt += blockDim.x) {
+        for (int tt = threadIdx.x; tt < WBb; tt += blockDim.x) {
             const int li = b_lo + tt;
             if (li >= sc.n) break;
             const int b = sc.b0 + li;
@@ -592,11 +601,11 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
         // evaluates trials g, g + G, ... with the element's data loaded once and the
         // trial points of its bodies formed in registers (alpha_k by k halvings, as the
         // line search forms it)
-        const int ne = min(WB, max(0, max(nj, sc.nc) - b_lo));
+        const int ne = min(WBe, max(0, nel - e_lo));
         if (ne > 0) {
             const int G = min(K, max(1, (int)blockDim.x / ne));
             const int tt = threadIdx.x % ne, g = threadIdx.x / ne;
-            const int li = b_lo + tt;
+            const int li = e_lo + tt;
             if (g < G && li < nj) {
                 const int j = sc.j0 + li;
                 const int A = d.ja[j], B = d.jb[j];
@@ -652,9 +661,9 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
         __syncthreads();
         if (threadIdx.x < 5 * K) {
             const double* Tk = T + lk * TS;
-            const int nb = min(WB, max(0, sc.n - b_lo));
-            const int ncw = min(WB, max(0, sc.nc - b_lo));
-            const int njw = min(WB, max(0, nj - b_lo));
+            const int nb = min(WBb, max(0, sc.n - b_lo));
+            const int ncw = min(WBe, max(0, sc.nc - e_lo));
+            const int njw = min(WBe, max(0, nj - e_lo));
             const double* base = lc == 0 ? Tk + OI : lc == 1 ? Tk + OG : lc == 2 ? Tk + OJ : lc == 3 ? Tk + OB : Tk + OF;
             const int cnt = lc == 0 ? 2 * nb : lc == 1 ? nb : lc == 2 ? njw : ncw;
             acc = lds_serial_sum(acc, base, cnt);
