@@ -1062,27 +1062,45 @@ __device__ void build_components(const Dev& d, const Work& w, const ScnCtx& sc, 
 }
 
 // 6x6 LDL^T (skyline recurrences) and solve of one body block; false on a zero pivot.
+// The 6x6 block and the right-hand side are read once into registers and x written
+// once: x may alias neither in the compiler's view, so working on x in place would turn
+// every substitution step into a global-memory round trip.
 __device__ bool ldlt6_solve(const double* A, const double* b, double* x) {
-    double L[36];
+    double a[36], L[36], y[6];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) a[i] = A[i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) y[i] = b[i];
+#pragma unroll
     for (int i = 0; i < 6; ++i) {
+#pragma unroll
         for (int j = 0; j < i; ++j) {
-            double s = A[i * 6 + j];
+            double s = a[i * 6 + j];
+#pragma unroll
             for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[k * 6 + k] * L[j * 6 + k];
             L[i * 6 + j] = s / L[j * 6 + j];
         }
-        double dd = A[i * 6 + i];
+        double dd = a[i * 6 + i];
+#pragma unroll
         for (int k = 0; k < i; ++k) dd -= L[i * 6 + k] * L[i * 6 + k] * L[k * 6 + k];
         if (dd == 0.0 || !isfinite(dd)) return false;
         L[i * 6 + i] = dd;
     }
+#pragma unroll
     for (int i = 0; i < 6; ++i) {
-        double s = b[i];
-        for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * x[k];
-        x[i] = s;
+        double s = y[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * y[k];
+        y[i] = s;
     }
-    for (int i = 0; i < 6; ++i) x[i] /= L[i * 6 + i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) y[i] /= L[i * 6 + i];
+#pragma unroll
     for (int i = 5; i >= 0; --i)
-        for (int k = 0; k < i; ++k) x[k] -= L[i * 6 + k] * x[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) y[k] -= L[i * 6 + k] * y[i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) x[i] = y[i];
     return true;
 }
 
