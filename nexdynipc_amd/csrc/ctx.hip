@@ -397,6 +397,23 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     d.jtype = dupload(ctx, jt.data(), jt.size());
     d.ja = dupload(ctx, ja.data(), ja.size());
     d.jb = dupload(ctx, jb.data(), jb.size());
+    {
+        // body -> joint incidence in joint order, for the per-body sums over joints
+        std::vector<int> off(size_t(N) + 1, 0), ent;
+        for (int j = 0; j < J; ++j) {
+            if (ja[j] >= 0) ++off[ja[j] + 1];
+            if (jb[j] >= 0) ++off[jb[j] + 1];
+        }
+        for (int b = 0; b < N; ++b) off[b + 1] += off[b];
+        ent.assign(std::max<size_t>(off[N], 1), 0);
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int j = 0; j < J; ++j) {
+            if (ja[j] >= 0) ent[fill[ja[j]]++] = (j << 1) | 0;
+            if (jb[j] >= 0) ent[fill[jb[j]]++] = (j << 1) | 1;
+        }
+        d.bj_off = dupload(ctx, off.data(), off.size());
+        d.bj_ent = dupload(ctx, ent.data(), ent.size());
+    }
     std::vector<double> jdata(16 * size_t(std::max(J, 1)), 0.0), jmu(std::max(J, 1), 1e8);
     if (J > 0) {
         std::memcpy(jdata.data(), dsc->joint_data, sizeof(double) * 16 * J);

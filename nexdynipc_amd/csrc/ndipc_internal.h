@@ -77,6 +77,8 @@ struct Dev {
     const int* jtype;      // [J]
     const int* ja;         // [J] global body
     const int* jb;         // [J] global body or -1
+    const int* bj_off;     // [N+1] joints of each body: bj_ent[bj_off[b] .. bj_off[b+1])
+    const int* bj_ent;     // [2J] (j << 1) | side, j ascending (side 0: body a, 1: body b)
     const double* jdata;   // [J*16]
     double* jmu;           // [J]
     double* jlambda;       // [J*6]

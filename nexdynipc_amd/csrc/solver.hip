@@ -377,10 +377,9 @@ __device__ double objective_grad(const Dev& d, const Work& w, const ScnCtx& sc, 
         g[3] = g[3] + sc.h2 * 0.0; g[4] = g[4] + sc.h2 * 0.0; g[5] = g[5] + sc.h2 * 0.0;
         // joints (ConstraintForm accumulates each joint's J^T f)
         double gj[6] = {0, 0, 0, 0, 0, 0};
-        for (int j = sc.j0; j < sc.j1; ++j) {
-            const double* jg = w.jgrad + 12 * j;
-            if (d.ja[j] == b) for (int k = 0; k < 6; ++k) gj[k] += jg[k];
-            if (d.jb[j] == b) for (int k = 0; k < 6; ++k) gj[k] += jg[6 + k];
+        for (int e = d.bj_off[b]; e < d.bj_off[b + 1]; ++e) {  // the body's joints, j ascending
+            const double* jg = w.jgrad + 12 * (d.bj_ent[e] >> 1) + 6 * (d.bj_ent[e] & 1);
+            for (int k = 0; k < 6; ++k) gj[k] += jg[k];
         }
         for (int k = 0; k < 6; ++k) g[k] = g[k] + gj[k];
         double gb[6] = {0, 0, 0, 0, 0, 0}, gf[6] = {0, 0, 0, 0, 0, 0};
@@ -746,10 +745,8 @@ __device__ void assemble_diag(const Dev& d, const Work& w, const ScnCtx& sc, con
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) D[(3 + r) * 6 + 3 + c] = d.inertia[9 * b + 3 * r + c];
-        for (int j = sc.j0; j < sc.j1; ++j) {
-            const int side = d.ja[j] == b ? 0 : (d.jb[j] == b ? 1 : -1);
-            if (side < 0) continue;
-            const double* jb = w.jblk + 108 * j + 36 * side;
+        for (int e = d.bj_off[b]; e < d.bj_off[b + 1]; ++e) {  // the body's joints, j ascending
+            const double* jb = w.jblk + 108 * (d.bj_ent[e] >> 1) + 36 * (d.bj_ent[e] & 1);
 #pragma unroll
             for (int k = 0; k < 36; ++k) D[k] = D[k] + jb[k];
         }
