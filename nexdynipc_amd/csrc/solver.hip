@@ -1140,27 +1140,55 @@ __device__ bool small_component_solve(const Dev& d, const Work& w, const ScnCtx&
         const int tb = t / 36, r = (t % 36) / 6, q = t % 6;
         M[(6 * tb + r) * m + 6 * tb + q] = w.D[36 * (sc.b0 + L.list[c0 + tb]) + r * 6 + q];
     }
+    // The component's elements: hinges (kind 0), active barrier (1) and friction (2)
+    // contacts whose first body is a member, gathered from the members' incidence lists
+    // (body -> joints, body -> contacts) and ordered by (kind, index) with a rank count:
+    // the order of a scan over all elements of the scene.
+    (void)c;
     int ne = 0;
-    for (int kind = 0; kind < 3; ++kind) {
-        const int cnt = kind == 0 ? sc.j1 - sc.j0 : sc.nc;
-        for (int base = 0; base < cnt; base += 64) {
-            const int i = base + lane;
+    const int* coff = w.csr_off + sc.b0 + sc.s;
+    const int* cent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    auto append = [&](bool f, int code) {
+        const unsigned long long bal = __ballot(f);
+        if (f) el[ne + __popcll(bal & ((1ull << lane) - 1ull))] = code;
+        ne += __popcll(bal);
+    };
+    for (int t = 0; t < mb; ++t) {
+        const int lb = L.list[c0 + t], b = sc.b0 + lb;
+        const int jl = d.bj_off[b], jh = d.bj_off[b + 1];
+        for (int base = jl; base < jh; base += 64) {
+            const int e = base + lane;
             bool f = false;
-            if (i < cnt) {
-                if (kind == 0) {
-                    const int j = sc.j0 + i;
-                    f = d.jtype[j] == 1 && L.lab[d.ja[j] - sc.b0] == c;
-                } else {
-                    const int act = w.cactive[(long long)sc.s * kMaxContacts + i];
-                    f = (act & (kind == 1 ? 1 : 2)) && L.lab[w.cint[5 * (sc.cb + i)] - sc.b0] == c;
-                }
+            int code = 0;
+            if (e < jh) {
+                const int je = d.bj_ent[e], j = je >> 1;
+                f = (je & 1) == 0 && d.jtype[j] == 1;
+                code = j - sc.j0;
             }
-            const unsigned long long bal = __ballot(f);
-            if (f) el[ne + __popcll(bal & ((1ull << lane) - 1ull))] = (kind << 24) | i;
-            ne += __popcll(bal);
+            append(f, code);
+        }
+        const int el0 = coff[lb], eh = coff[lb + 1];
+        for (int base = el0; base < eh; base += 64) {
+            const int e = base + lane;
+            int act = 0, i = 0;
+            if (e < eh) {
+                const int ce = cent[e];
+                i = ce >> 1;
+                if ((ce & 1) == 0) act = w.cactive[(long long)sc.s * kMaxContacts + i];
+            }
+            append((act & 1) != 0, (1 << 24) | i);
+            append((act & 2) != 0, (2 << 24) | i);
         }
     }
     wave_sync();
+    for (int e = lane; e < ne; e += 64) {
+        const int code = el[e];
+        int r = 0;
+        for (int q = 0; q < ne; ++q) r += el[q] < code;
+        el[ne + r] = code;
+    }
+    wave_sync();
+    el += ne;
     for (int e = 0; e < ne; ++e) {
         const int code = el[e], kind = code >> 24, i = code & 0xffffff;
         const int t = lane;
