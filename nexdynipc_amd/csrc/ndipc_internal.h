@@ -26,7 +26,8 @@ constexpr int kHitWords = 48;         // 3072 candidate bits per pair (2x1024 FV
 constexpr int kSceneScratch = 65536;  // per-scene scratch doubles (sort keys, barrier b'')
 constexpr int kDenseBodies = 64;      // largest coupled component factored directly
 constexpr int kDenseRows = 6 * kDenseBodies;
-constexpr int kLsBatch = 8;           // line-search trials evaluated together
+constexpr int kLsBatch = 20;          // line-search trials evaluated together, at most
+constexpr int kLsBatchLarge = 8;      // ... for scenes whose 20 trials need several LDS rounds
 
 struct SceneParams {
     // settings

@@ -534,8 +534,8 @@ __device__ void objective_value(const Dev& d, const Work& w, const ScnCtx& sc, c
 // lengths evaluated together. Each element's lane forms its bodies' trial points in
 // registers with the same expression as the line search's x + alpha dx. Every value is
 // summed exactly as in objective_value (same terms, same order, from +0.0); the 5 K
-// accumulators run on lanes 0..5K-1 of the first wavefront, side by side, so K trials
-// cost about one trial's serial chain.
+// accumulators run on lanes 0..5K-1 (the first two wavefronts), side by side, so K
+// trials cost about one trial's serial chain. vals, cvals: K entries each, in LDS.
 __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx& sc, int K, const double* X0,
                                       const double* DX0, double alpha, double* red, double* vals, double* cvals,
                                       double* T) {
@@ -544,7 +544,7 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
     // the two widths chosen per scene so that the fewest rounds cover the bodies and
     // the elements (widths even: every window starts 16-byte aligned; at most one
     // element window per work-group of lanes).
-    constexpr int TS = kTermLds / kLsBatch;
+    const int TS = (kTermLds / K) & ~1;
     const double* XH = w.xhat;
     const int nj = sc.j1 - sc.j0;
     const int nel = max(nj, sc.nc);
@@ -671,7 +671,8 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
     }
     if (threadIdx.x < 5 * K) red[threadIdx.x] = acc;
     __syncthreads();
-    for (int k = 0; k < K; ++k) {
+    if (threadIdx.x < K) {  // lane k: trial k's values (vals, cvals in LDS)
+        const int k = threadIdx.x;
         const double vin = red[5 * k], vg = red[5 * k + 1], vj = red[5 * k + 2], vb = red[5 * k + 3],
                      vf = red[5 * k + 4];
         double val = vin;
@@ -1503,6 +1504,11 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     // the accepted trial instead of evaluating them again.
     double fx_next = 0.0, fc_next = 0.0;
     bool have_f = false;
+    __shared__ double lsv[2 * kLsBatch];  // trial values, then contact values
+    int kprev = kLsBatchLarge;            // trials the previous line search used
+    const int Kb = 3 * (sc.n + (sc.n & 1) + max(sc.j1 - sc.j0, sc.nc) + 1) <= ((kTermLds / kLsBatch) & ~1)
+                       ? kLsBatch
+                       : kLsBatchLarge;
     for (; it < maxit; ++it) {
         eval_elements(d, w, sc, Xg, b2arr);
         const double ginf = objective_grad(d, w, sc, Xg, G, CG, red);
@@ -1535,33 +1541,39 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         double descent, cdesc;
         serial_dot2(G, CG, DX, sc.nd, red, &descent, &cdesc, nl.terms);
         if (descent > 0) break;
-        // The trials alpha = 1, 1/2, 1/4, ... (at most 20) are evaluated kLsBatch at a
-        // time; the first that passes is taken, and only the trials up to it count.
+        // The trials alpha = 1, 1/2, 1/4, ... (at most 20) are evaluated in batches: the
+        // first as many as the previous line search needed, the next ones all that
+        // remain, each at most Kb (20 where their terms fit one LDS round, 8
+        // otherwise). The first that passes is taken, and only the trials up to it count.
         double alpha = 1.0;
         bool accepted = false;
+        int ls_used = 0;
         for (int ls = 0; ls < 20 && !accepted;) {
-            const int K = min(kLsBatch, 20 - ls);
-            double fn[kLsBatch], fcn[kLsBatch];
-            objective_value_batch(d, w, sc, K, w.x, w.dx, alpha, red, fn, fcn, nl.terms);
+            const int K = min(Kb, ls == 0 ? kprev : 20 - ls);
+            objective_value_batch(d, w, sc, K, w.x, w.dx, alpha, red, lsv, lsv + kLsBatch, nl.terms);
             int k = 0;
             for (; k < K; ++k) {
-                const bool cok = (cdesc <= 0.0) ? (fcn[k] <= fc + 1e-4 * alpha * cdesc) : (fcn[k] <= fc);
-                if (fn[k] <= fx + 1e-4 * alpha * descent && cok) {
+                const double fnk = lsv[k], fcnk = lsv[kLsBatch + k];
+                const bool cok = (cdesc <= 0.0) ? (fcnk <= fc + 1e-4 * alpha * cdesc) : (fcnk <= fc);
+                if (fnk <= fx + 1e-4 * alpha * descent && cok) {
                     accepted = true;
-                    fx_next = fn[k];
-                    fc_next = fcn[k];
+                    fx_next = fnk;
+                    fc_next = fcnk;
                     have_f = true;
                     break;
                 }
                 alpha *= 0.5;
             }
+            __syncthreads();  // lsv is rewritten by the next batch
             const int used = accepted ? k + 1 : K;
             if (threadIdx.x == 0) {
                 sp.ls_trials += used;
                 sp.trials_total += double(used);
             }
             ls += used;
+            ls_used = ls;
         }
+        kprev = ls_used;
         if (!accepted) break;
         for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) X[i] = X[i] + alpha * DX[i];
         __syncthreads();
