@@ -1257,6 +1257,94 @@ __device__ bool small_component_solve(const Dev& d, const Work& w, const ScnCtx&
     return true;
 }
 
+// Right-looking LDL^T of the m x m matrix M (row-major, lower triangle): at step k,
+// D_k = A_kk, L_ik = A_ik / D_k, then the trailing update A_ij -= (L_ik D_k) L_jk
+// (A_jj -= (L_jk L_jk) D_k). Every entry receives the same subtractions in the same
+// (k ascending) order as in the row-oriented recurrences of the oracle's skyline
+// factorization, so the factor is bitwise theirs. L_ik is mirrored to the upper
+// triangle (never read by the factorization) so that the forward substitution reads
+// rows. Blocked in panels of P columns: within a panel each column is finished in turn
+// and only the panel's later columns are updated at once; the trailing columns then
+// receive the panel's P updates, k ascending, on 2 x 2 register tiles, so the matrix
+// in HBM is read and written once per panel instead of once per column. The panel's
+// L_ik and L_ik D_k (the same rounded product the update uses) are staged in LDS. Not
+// inlined: inside the Newton kernel it would raise register pressure into spills.
+// Returns false on a zero or non-finite pivot.
+template <int P>
+__device__ __noinline__ bool dense_ldlt_panels(double* M, int m) {
+    __shared__ double pl[kDenseRows * P], pld[kDenseRows * P], pd[P];  // the panel
+    for (int k0 = 0; k0 < m; k0 += P) {
+        const int k1 = min(m, k0 + P), np = k1 - k0;
+        for (int k = k0; k < k1; ++k) {
+            const int kk = k - k0;
+            const double Dk = M[k * m + k];
+            if (Dk == 0.0 || !isfinite(Dk)) return false;
+            for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) {
+                const double l = M[i * m + k] / Dk;
+                M[i * m + k] = l;
+                M[k * m + i] = l;
+                pl[i * P + kk] = l;
+                pld[i * P + kk] = l * Dk;
+            }
+            if (threadIdx.x == 0) pd[kk] = Dk;
+            __syncthreads();
+            const int nc = k1 - k - 1;  // the panel's later columns j, rows i >= j
+            if (nc > 0) {
+                for (int t = threadIdx.x; t < (m - k - 1) * nc; t += blockDim.x) {
+                    const int i = k + 1 + t / nc, j = k + 1 + t % nc;
+                    if (j > i) continue;
+                    const double lik = pl[i * P + kk];
+                    if (i == j) M[i * m + i] -= (lik * lik) * Dk;
+                    else M[i * m + j] -= pld[i * P + kk] * pl[j * P + kk];
+                }
+                __syncthreads();
+            }
+        }
+        const int r = m - k1;  // trailing rows and columns k1 .. m-1
+        if (r <= 0) break;
+        const int nt = (r + 1) >> 1;
+        const int ntri = nt * (nt + 1) / 2;
+        for (int t = threadIdx.x; t < ntri; t += blockDim.x) {
+            int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+            while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+            while (ti * (ti + 1) / 2 > t) --ti;
+            const int tj = t - ti * (ti + 1) / 2;
+            const int i0 = k1 + 2 * ti, j0 = k1 + 2 * tj;
+            const bool r1 = i0 + 1 < m, c1 = j0 + 1 < m;
+            const bool diag = ti == tj;
+            // a00 (i0, j0), a10 (i0+1, j0), a01 (i0, j0+1), a11 (i0+1, j0+1); upper entries of
+            // a diagonal tile are not part of the lower triangle
+            const bool h01 = c1 && !diag, h10 = r1, h11 = r1 && c1;
+            double a00 = M[i0 * m + j0];
+            double a10 = h10 ? M[(i0 + 1) * m + j0] : 0.0;
+            double a01 = h01 ? M[i0 * m + j0 + 1] : 0.0;
+            double a11 = h11 ? M[(i0 + 1) * m + j0 + 1] : 0.0;
+            for (int q = 0; q < np; ++q) {
+                const double d = pd[q];
+                const double ld0 = pld[i0 * P + q], ld1 = r1 ? pld[(i0 + 1) * P + q] : 0.0;
+                const double lj0 = pl[j0 * P + q], lj1 = c1 ? pl[(j0 + 1) * P + q] : 0.0;
+                if (diag) {
+                    a00 -= (lj0 * lj0) * d;
+                    a10 -= ld1 * lj0;
+                    a11 -= (lj1 * lj1) * d;
+                } else {
+                    a00 -= ld0 * lj0;
+                    a10 -= ld1 * lj0;
+                    a01 -= ld0 * lj1;
+                    a11 -= ld1 * lj1;
+                }
+            }
+            M[i0 * m + j0] = a00;
+            if (h10) M[(i0 + 1) * m + j0] = a10;
+            if (h01) M[i0 * m + j0 + 1] = a01;
+            if (h11) M[(i0 + 1) * m + j0 + 1] = a11;
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+template <int P>
 __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* rhs,
                                double* xo, CompLds& L, int* ok_sh) {
     build_components(d, w, sc, L);
@@ -1374,31 +1462,7 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
             }
             __syncthreads();
         }
-        // right-looking LDL^T: at step k, D_k = A_kk, L_ik = A_ik / D_k, then the trailing
-        // update A_ij -= (L_ik D_k) L_jk (A_jj -= (L_jk L_jk) D_k) of the lower triangle.
-        // Every entry receives the same subtractions in the same (k ascending) order as
-        // in the row-oriented recurrences above, so the factor is bitwise theirs. L_ik is
-        // mirrored to the upper triangle (never read by the factorization) so that the
-        // forward substitution reads rows.
-        for (int k = 0; k < m; ++k) {
-            const double Dk = M[k * m + k];
-            if (Dk == 0.0 || !isfinite(Dk)) return 1;
-            for (int i = k + 1 + threadIdx.x; i < m; i += blockDim.x) {
-                const double l = M[i * m + k] / Dk;
-                M[i * m + k] = l;
-                M[k * m + i] = l;
-            }
-            __syncthreads();
-            const int r = m - k - 1;
-            for (int t = threadIdx.x; t < r * r; t += blockDim.x) {
-                const int i = k + 1 + t / r, j = k + 1 + t % r;
-                if (j > i) continue;
-                const double lik = M[i * m + k], ljk = M[j * m + k];
-                if (i == j) M[i * m + i] -= (lik * lik) * Dk;
-                else M[i * m + j] -= (lik * Dk) * ljk;
-            }
-            __syncthreads();
-        }
+        if (!dense_ldlt_panels<P>(M, m)) return 1;
         // substitutions by the first wavefront, rows lane + 64 q in registers: forward
         // x_i -= L_ik x_k (k ascending), x_i /= D_i, backward x_k -= L_ik x_i (i
         // descending), the per-row operation order of the column-oriented substitutions
@@ -1517,7 +1581,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         assemble_diag(d, w, sc, b2arr);
         for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) NG[i] = -G[i];
         __syncthreads();
-        const int cs = component_solve(d, w, sc, b2arr, NG, DX, comp, &ok_sh);
+        const int cs = component_solve<8>(d, w, sc, b2arr, NG, DX, comp, &ok_sh);
         int pit = 0;
         const bool ok = cs == 0 ? true : cs == 2 ? pcg_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh, &pit) : false;
         if (threadIdx.x == 0) {
@@ -2316,7 +2380,7 @@ __device__ bool adj_solve_gmres(const Dev& d, const Work& w, const Adj& a, const
     if (!(bn > 0.0)) return true;
     const double tol = 1e-13 * bn;
     auto precond = [&](const double* in, double* out) -> bool {
-        const int c = component_solve(d, w, sc, b2arr, in, out, nl.comp, ok_sh);
+        const int c = component_solve<4>(d, w, sc, b2arr, in, out, nl.comp, ok_sh);
         const bool ok = c == 0 ? true : c == 2 ? pcg_solve(d, w, sc, b2arr, in, out, red, ok_sh) : false;
         __syncthreads();
         return ok;
