@@ -1406,62 +1406,52 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
                 for (int q = 0; q < 6; ++q) M[(6 * t + r) * m + 6 * t + q] = D[r * 6 + q];
         }
         __syncthreads();
-        // lower-triangle off-diagonal blocks: hinges, then barrier, then friction contacts,
-        // each in element order. The component's elements are compacted in that order by
-        // work-group scans; then one element at a time adds its block entries in parallel.
-        int* el = w.sort_idx + (long long)sc.s * kSceneScratch;
-        int ne = 0;
-        for (int kind = 0; kind < 3; ++kind) {
-            const int cnt = kind == 0 ? sc.j1 - sc.j0 : sc.nc;
-            for (int base = 0; base < cnt; base += blockDim.x) {
-                const int i = base + threadIdx.x;
-                int f = 0;
-                if (i < cnt) {
-                    if (kind == 0) {
-                        const int j = sc.j0 + i;
-                        f = d.jtype[j] == 1 && L.lab[d.ja[j] - sc.b0] == c;
-                    } else {
-                        const int act = w.cactive[(long long)sc.s * kMaxContacts + i];
-                        f = (act & (kind == 1 ? 1 : 2)) && L.lab[w.cint[5 * (sc.cb + i)] - sc.b0] == c;
+        // Lower-triangle off-diagonal blocks: every entry receives the hinges, then the
+        // barrier contacts, then the friction contacts that couple its two bodies, each in
+        // element order (the order of a scan over the scene's elements). Restricted to the
+        // block of members (hi, lo), hi > lo, that order is the incidence lists of member
+        // hi: its joints (ascending), then its contacts (ascending) with barrier terms,
+        // then the same contacts with friction terms. Seven groups of 36 lanes take every
+        // seventh block row; in a group lane (r, q) owns entry (r, q) of the block an
+        // element adds to, so no two lanes of the work-group touch the same entry and the
+        // rows need no barriers.
+        {
+            const int* coff = w.csr_off + sc.b0 + sc.s;
+            const int* cent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+            const int grp = threadIdx.x / 36, t = threadIdx.x % 36, r = t / 6, q = t % 6;
+            for (int hi = grp; grp < 7 && hi < mb; hi += 7) {
+                const int lb = L.list[c0 + hi], b = sc.b0 + lb;
+                double* Mrow = M + (6 * hi + r) * m + q;
+                for (int e = d.bj_off[b]; e < d.bj_off[b + 1]; ++e) {
+                    const int j = d.bj_ent[e] >> 1, side = d.bj_ent[e] & 1;
+                    if (d.jtype[j] != 1) continue;
+                    const int oth = side ? d.ja[j] : d.jb[j];
+                    if (oth < 0) continue;
+                    const int lo = lpos(oth - sc.b0);
+                    if (lo >= hi) continue;
+                    const double* jb = w.jblk + 108 * j + 72;  // rows a, cols b
+                    Mrow[6 * lo] += side == 0 ? jb[r * 6 + q] : jb[q * 6 + r];
+                }
+                for (int kind = 1; kind <= 2; ++kind) {
+                    for (int e = coff[lb]; e < coff[lb + 1]; ++e) {
+                        const int i = cent[e] >> 1, side = cent[e] & 1;
+                        const long long cc = sc.cb + i, ce = (long long)sc.s * kMaxContacts + i;
+                        if (!(w.cactive[ce] & kind)) continue;
+                        const int lo = lpos(w.cint[5 * cc + (side ? 0 : 1)] - sc.b0);
+                        if (lo >= hi) continue;
+                        if (kind == 1) {
+                            if (r < 3 && q < 3) {
+                                const double* cd = w.cdbl + 7 * cc;
+                                Mrow[6 * lo] += (-(b2arr[i] * (cd[3 + r] * cd[3 + q]))) * sc.h2;
+                            }
+                        } else {
+                            Mrow[6 * lo] += w.cblk[144 * ce + (side == 0 ? 72 : 108) + r * 6 + q];
+                        }
                     }
                 }
-                int ex;
-                const int tot = wg_exclusive_scan(f, L.scan, &ex);
-                if (f) el[ne + ex] = (kind << 24) | i;
-                ne += tot;
             }
         }
         __syncthreads();
-        for (int e = 0; e < ne; ++e) {
-            const int code = el[e], kind = code >> 24, i = code & 0xffffff;
-            const int t = threadIdx.x;
-            if (kind == 0) {
-                const int j = sc.j0 + i;
-                const int pa = lpos(d.ja[j] - sc.b0), pb = lpos(d.jb[j] - sc.b0);
-                const double* jb = w.jblk + 108 * j + 72;  // rows a, cols b
-                if (t < 36) {
-                    const int r = t / 6, q = t % 6;
-                    if (pa > pb) M[(6 * pa + r) * m + 6 * pb + q] += jb[r * 6 + q];
-                    else M[(6 * pb + r) * m + 6 * pa + q] += jb[q * 6 + r];
-                }
-            } else {
-                const long long cc = sc.cb + i, ce = (long long)sc.s * kMaxContacts + i;
-                const int pa = lpos(w.cint[5 * cc] - sc.b0), pb = lpos(w.cint[5 * cc + 1] - sc.b0);
-                const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
-                if (kind == 1) {
-                    if (t < 9) {
-                        const int r = t / 3, q = t % 3;
-                        const double* cd = w.cdbl + 7 * cc;
-                        M[(6 * hi + r) * m + 6 * lo + q] += (-(b2arr[i] * (cd[3 + r] * cd[3 + q]))) * sc.h2;
-                    }
-                } else if (t < 36) {
-                    const int r = t / 6, q = t % 6;
-                    const double* blk = w.cblk + 144 * ce + (pa > pb ? 72 : 108);
-                    M[(6 * hi + r) * m + 6 * lo + q] += blk[r * 6 + q];
-                }
-            }
-            __syncthreads();
-        }
         if (!dense_ldlt_panels<P>(M, m)) return 1;
         // substitutions by the first wavefront, rows lane + 64 q in registers: forward
         // x_i -= L_ik x_k (k ascending), x_i /= D_i, backward x_k -= L_ik x_i (i
