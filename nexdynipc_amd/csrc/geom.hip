@@ -257,7 +257,7 @@ __device__ void lbvh_query(const Dev& d, const Work& w, int s, int i, double r, 
 // ascending, then j descending (BroadPhase.cpp, BVH.cpp query/build). The contact list,
 // and with it every per-body sum of the Newton objective, inherits this order.
 __device__ __forceinline__ void bp_sort_insert(int* outp, int o, int k, int i, int j) {
-    int pos = k;
+    int pos = k;  // entries [0, k) hold j descending; j goes to its place
     while (pos > 0 && outp[2 * (o + pos - 1) + 1] < j) {
         outp[2 * (o + pos)] = outp[2 * (o + pos - 1)];
         outp[2 * (o + pos) + 1] = outp[2 * (o + pos - 1) + 1];
@@ -298,9 +298,12 @@ __device__ __forceinline__ Box3 box_at(const double* b, int i) {
 // full and after the traversal, so the lanes of a wavefront process their hits in the
 // same few iterations instead of one divergent call per hit. Callers must not depend
 // on the order of the calls (pass 0 counts and takes minima, pass 1 inserts sorted).
+// With `sorted`, each batch of buffered leaves is handed over by body index descending
+// (an insertion sort of the lane's buffer), the order of the emitted pair lists.
 constexpr int kBvhStack = 72;
 template <typename F>
-__device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, double r, F&& on_leaf) {
+__device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, double r, bool sorted,
+                               F&& on_leaf) {
     const Box3 qa = box_inflated(bi, r);
     if (m == 1) return;
     unsigned short* st = T.stk + threadIdx.x;
@@ -308,6 +311,22 @@ __device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, do
     int nh = 0;
     int sp = 1;
     st[0] = 0;
+    auto flush = [&]() {
+        if (sorted) {
+            for (int h = 1; h < nh; ++h) {
+                const unsigned short v = hb[kBlock * h];
+                const int key = T.lbody[v];
+                int p = h;
+                while (p > 0 && T.lbody[hb[kBlock * (p - 1)]] < key) {
+                    hb[kBlock * p] = hb[kBlock * (p - 1)];
+                    --p;
+                }
+                hb[kBlock * p] = v;
+            }
+        }
+        for (int h = 0; h < nh; ++h) on_leaf(hb[kBlock * h]);
+        nh = 0;
+    };
     while (sp > 0) {
         const int node = st[kBlock * --sp];
         const int cs[2] = {T.child[2 * node], T.child[2 * node + 1]};
@@ -316,10 +335,7 @@ __device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, do
             if (c < 0) {
                 if (T.lbody[~c] > i && box_overlap(qa, box_inflated(box_at(T.lbox, ~c), r))) {
                     hb[kBlock * nh++] = (unsigned short)(~c);
-                    if (nh == kBvhHits) {
-                        for (int h = 0; h < nh; ++h) on_leaf(hb[kBlock * h]);
-                        nh = 0;
-                    }
+                    if (nh == kBvhHits) flush();
                 }
             } else {
                 if (box_overlap(qa, box_inflated(box_at(T.nbox, c), r)) && sp < kBvhStack)
@@ -327,7 +343,7 @@ __device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, do
             }
         }
     }
-    for (int h = 0; h < nh; ++h) on_leaf(hb[kBlock * h]);
+    flush();
 }
 
 // Grid (scene, chunk of kBlock bodies). With `lds` the work-group first copies its
@@ -382,12 +398,22 @@ __global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass) {
         const int oc = pass ? w.boff[i] : 0, op = pass ? w.boff[d.N + i] : 0;
         const double margin = sp.prune_margin;
         const int idi = d.body_id[i];
+        // pass 1: hits arriving in descending j (sorted batches) are appended; a hit
+        // above the last one written goes to its place by insertion
+        int lastc = 0x7fffffff, lastp = 0x7fffffff;
         auto hit = [&](int j, int idj, const Box3& bj) {
             const double gap = box_distance(bi, bj);
             if (box_overlap(q0, box_inflated(bj, r0)) && gap <= r0) {
                 if (pass) {
-                    bp_sort_insert(w.pairs, w.pair_off[s] + oc, nc, i, j);
-                    w.pair_scene[w.pair_off[s] + oc + nc] = s;
+                    const int o = w.pair_off[s] + oc;
+                    if (j < lastc) {
+                        w.pairs[2 * (o + nc)] = i;
+                        w.pairs[2 * (o + nc) + 1] = j;
+                        lastc = j;
+                    } else {
+                        bp_sort_insert(w.pairs, o, nc, i, j);
+                    }
+                    w.pair_scene[o + nc] = s;
                 }
                 ++nc;
             }
@@ -395,8 +421,15 @@ __global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass) {
                 gmin = dmin(gmin, gap);
                 if (gap <= margin) {
                     if (pass) {
-                        bp_sort_insert(w.prox_pairs, w.prox_off[s] + op, np, i, j);
-                        w.prox_scene[w.prox_off[s] + op + np] = s;
+                        const int o = w.prox_off[s] + op;
+                        if (j < lastp) {
+                            w.prox_pairs[2 * (o + np)] = i;
+                            w.prox_pairs[2 * (o + np) + 1] = j;
+                            lastp = j;
+                        } else {
+                            bp_sort_insert(w.prox_pairs, o, np, i, j);
+                        }
+                        w.prox_scene[o + np] = s;
                     }
                     ++np;
                 }
@@ -404,7 +437,8 @@ __global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass) {
             if (do2 && !any && box_overlap(q2, box_inflated(bj, r2)) && pair_allowed(d, s, idi, idj)) any = 1;
         };
         if constexpr (lds)
-            lbvh_query_lds(T, m, i, bi, rmax, [&](int leaf) { hit(T.lbody[leaf], T.lbid[leaf], box_at(T.lbox, leaf)); });
+            lbvh_query_lds(T, m, i, bi, rmax, pass != 0,
+                           [&](int leaf) { hit(T.lbody[leaf], T.lbid[leaf], box_at(T.lbox, leaf)); });
         else
             lbvh_query(d, w, s, i, rmax, [&](int j) { hit(j, d.body_id[j], ldbox(w.wbox, j)); });
         if (!pass) {
