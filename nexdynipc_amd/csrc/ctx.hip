@@ -223,6 +223,7 @@ int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= device) return -2;
     if (hipSetDevice(device) != hipSuccess) return -3;
     nd::geom_configure();
+    if (nd::geom_configure_smf() != 0) return -5;
     auto* c = new ndipc_ctx();
     c->device = device;
     if (hipStreamCreate(&c->stream) != hipSuccess) {
@@ -376,6 +377,11 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     d.N = N;
     d.max_nb = 0;
     for (int s = 0; s < S; ++s) d.max_nb = std::max(d.max_nb, ctx->body_off[s + 1] - ctx->body_off[s]);
+    d.max_nv = d.max_nf = 1;
+    for (int b = 0; b < N; ++b) {
+        d.max_nv = std::max(d.max_nv, ctx->nv[b]);
+        d.max_nf = std::max(d.max_nf, ctx->nf[b]);
+    }
     d.body_off = dupload(ctx, ctx->body_off.data(), S + 1);
     d.body_scene = dupload(ctx, body_scene.data(), N);
     d.body_id = dupload(ctx, ctx->body_id.data(), N);

@@ -1380,12 +1380,25 @@ void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_a
 // face, in chunks of kSmfChunk faces, and every (vertex, face) candidate of a pair is one
 // work item: the same expressions as the per-vertex loop, evaluated once.
 constexpr int kSmfChunk = 256;
+// LDS of k_scene_min_flag, sized at launch by the largest mesh (NV vertices, NF faces):
+// J's world vertices and I's candidate vertices (3 NV doubles each), the face chunk
+// (v0, n, inverse: 15 doubles per face), its validity flags and J's active faces.
+__host__ __device__ inline size_t smf_lds_bytes(int NV, int NF) {
+    return sizeof(double) * (6 * size_t(NV) + 15 * kSmfChunk) + sizeof(int) * (kSmfChunk + size_t(NF));
+}
+
 __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int ungated) {
-    __shared__ double wvx[kMaxPrims], wvy[kMaxPrims], wvz[kMaxPrims];  // J's vertices, world
-    __shared__ double ivx[kMaxPrims], ivy[kMaxPrims], ivz[kMaxPrims];  // I's candidate vertices
-    __shared__ double fd[kSmfChunk * 15];                              // v0(3) n(3) inverse(9)
-    __shared__ int fok[kSmfChunk];
-    __shared__ int act[kMaxPrims];
+    extern __shared__ __align__(16) unsigned char smf_lds[];
+    const int NV = d.max_nv;
+    double* wvx = reinterpret_cast<double*>(smf_lds);
+    double* wvy = wvx + NV;
+    double* wvz = wvy + NV;
+    double* ivx = wvz + NV;
+    double* ivy = ivx + NV;
+    double* ivz = ivy + NV;
+    double* fd = ivz + NV;
+    int* fok = reinterpret_cast<int*>(fd + 15 * kSmfChunk);
+    int* act = fok + kSmfChunk;
     __shared__ int nact, nvi;
     const int total = *w.n_prox_total;
     for (int p = blockIdx.x; p < total; p += gridDim.x) {
@@ -1452,24 +1465,30 @@ __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int un
                 for (int k = 0; k < 9; ++k) o[6 + k] = Mi.m[k];
             }
             __syncthreads();
+            // a wavefront takes a face (its data loaded once, a broadcast read) and its
+            // lanes sweep the candidate vertices
             bool hit = false;
-            for (int t = threadIdx.x; t < nv * nc && !hit; t += blockDim.x) {
-                const int vi = t / nc, fi = t % nc;
+            const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            for (int fi = wv; fi < nc; fi += kBlock / 64) {
                 if (!fok[fi]) continue;
                 const double* o = fd + 15 * fi;
-                const V3 r = v3(ivx[vi], ivy[vi], ivz[vi]) - v3(o[0], o[1], o[2]);
-                const double dist = fabs(dot(r, v3(o[3], o[4], o[5])));
-                if (!(dist < 1e-9)) continue;
+                const V3 v0 = v3(o[0], o[1], o[2]), n = v3(o[3], o[4], o[5]);
                 M3 Mi;
                 for (int k = 0; k < 9; ++k) Mi.m[k] = o[6 + k];
-                const V3 co = m3v(Mi, r);
-                const double b0 = 1.0 - co.x - co.y, b1 = co.x, b2 = co.y;
-                double bmn = b0, bmx = b0;
-                if (b1 < bmn) bmn = b1;
-                if (b2 < bmn) bmn = b2;
-                if (bmx < b1) bmx = b1;
-                if (bmx < b2) bmx = b2;
-                if (bmn >= -1e-6 && bmx <= 1.0 + 1e-6) hit = true;
+                for (int vi = lane; vi < nv; vi += 64) {
+                    const V3 r = v3(ivx[vi], ivy[vi], ivz[vi]) - v0;
+                    const double dist = fabs(dot(r, n));
+                    if (!(dist < 1e-9)) continue;
+                    const V3 co = m3v(Mi, r);
+                    const double b0 = 1.0 - co.x - co.y, b1 = co.x, b2 = co.y;
+                    double bmn = b0, bmx = b0;
+                    if (b1 < bmn) bmn = b1;
+                    if (b2 < bmn) bmn = b2;
+                    if (bmx < b1) bmx = b1;
+                    if (bmx < b2) bmx = b2;
+                    if (bmn >= -1e-6 && bmx <= 1.0 + 1e-6) hit = true;
+                }
+                if (__any(hit)) break;
             }
             if (__syncthreads_or(hit)) {
                 if (threadIdx.x == 0) atomicOr(&w.flag[s], 1);
@@ -1484,7 +1503,17 @@ void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_
     const int cap = w.grid_smf > 0 ? w.grid_smf : 2048;
     int grid = max_prox < cap ? max_prox : cap;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_scene_min_flag, dim3(grid), dim3(kBlock), 0, st, d, w, ungated);
+    hipLaunchKernelGGL(k_scene_min_flag, dim3(grid), dim3(kBlock), smf_lds_bytes(d.max_nv, d.max_nf), st, d, w,
+                       ungated);
+}
+
+int geom_configure_smf() {
+    // k_scene_min_flag takes up to smf_lds_bytes(kMaxPrims, kMaxPrims) (~138 KiB)
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_scene_min_flag),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(smf_lds_bytes(kMaxPrims, kMaxPrims))) == hipSuccess
+               ? 0
+               : -1;
 }
 
 // ----------------------------------------------------------------------------- CCD

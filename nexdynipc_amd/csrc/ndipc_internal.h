@@ -56,6 +56,7 @@ struct SceneParams {
 struct Dev {
     int S, N;
     int max_nb;            // largest scene body count (host-side launch shapes)
+    int max_nv, max_nf;    // largest mesh vertex / face counts (host-side LDS sizes)
     // scene structure
     const int* body_off;   // [S+1]
     const int* body_scene; // [N]
@@ -163,6 +164,7 @@ void launch_broad_phase(const Dev& d, const Work& w, int mode, double r, bool us
                         hipStream_t st);
 void launch_broad_phase_step(const Dev& d, const Work& w, hipStream_t st);
 int geom_configure();  // kernel attributes, once per context creation (no stream capture)
+int geom_configure_smf();  // the scene-min kernel's LDS limit; nonzero on failure
 void launch_topk(const Dev& d, const Work& w, int max_pairs, hipStream_t st);
 void launch_pose_from_state(const Dev& d, const Work& w, const double* x, hipStream_t st);
 void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st);
