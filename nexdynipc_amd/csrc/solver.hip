@@ -102,21 +102,36 @@ __device__ void fric_kin(const Dev& d, const Work& w, long long c, const double*
             k.JB[i][3 + j] = -PB.m[3 * i + j];
         }
 }
-// fric_kin's tangential displacement u alone (the friction value needs no Jacobian),
-// from the Newton variables xa, xb of the two bodies: the same expressions.
-__device__ V3 fric_u_x(const Dev& d, const Work& w, long long c, const double* xa, const double* xb, double dt) {
+// The parts of the friction displacement that do not depend on the Newton variables, computed once per
+// contact: the step-start poses, the contact point in both body frames and the unit
+// normal (the same expressions as in fric_kin).
+struct FricPre {
+    V3 pA0, pB0, lA, lB, nn;
+    Q qA0, qB0;
+};
+__device__ __forceinline__ FricPre fric_pre(const Dev& d, const Work& w, long long c) {
     const int A = w.cint[5 * c], B = w.cint[5 * c + 1];
     const double* cd = w.cdbl + 7 * c;
     const V3 pt = v3(cd[0], cd[1], cd[2]);
     const V3 nr = v3(cd[3], cd[4], cd[5]);
-    const V3 pA0 = ld3(d.p, A), pB0 = ld3(d.p, B);
-    const Q qA0 = ld4(d.q, A), qB0 = ld4(d.q, B);
+    FricPre f;
+    f.pA0 = ld3(d.p, A);
+    f.pB0 = ld3(d.p, B);
+    f.qA0 = ld4(d.q, A);
+    f.qB0 = ld4(d.q, B);
+    f.lA = rotate(qconj(f.qA0), pt - f.pA0);
+    f.lB = rotate(qconj(f.qB0), pt - f.pB0);
+    f.nn = (sqnorm(nr) > 1e-16) ? normalized(nr) : v3(0.0, 1.0, 0.0);
+    return f;
+}
+// fric_kin's tangential displacement u alone (the friction value needs no Jacobian),
+// from the Newton variables xa, xb of the two bodies: the same expressions.
+__device__ __forceinline__ V3 fric_u_pre(const FricPre& f, const double* xa, const double* xb, double dt) {
     const V3 pA = v3(xa[0], xa[1], xa[2]), pB = v3(xb[0], xb[1], xb[2]);
     const V3 thA = v3(xa[3], xa[4], xa[5]), thB = v3(xb[3], xb[4], xb[5]);
-    const Q qA = apply_theta(qA0, thA), qB = apply_theta(qB0, thB);
-    const V3 dA = pA - pA0, dB = pB - pB0;
-    const V3 lA = rotate(qconj(qA0), pt - pA0), lB = rotate(qconj(qB0), pt - pB0);
-    const V3 rA = rotate(qA, lA), rB = rotate(qB, lB);
+    const Q qA = apply_theta(f.qA0, thA), qB = apply_theta(f.qB0, thB);
+    const V3 dA = pA - f.pA0, dB = pB - f.pB0;
+    const V3 rA = rotate(qA, f.lA), rB = rotate(qB, f.lB);
     V3 wA = v3(0, 0, 0), wB = v3(0, 0, 0), vA = v3(0, 0, 0), vB = v3(0, 0, 0);
     if (dt > 1e-12) {
         wA = thA / dt;
@@ -127,8 +142,7 @@ __device__ V3 fric_u_x(const Dev& d, const Work& w, long long c, const double* x
     vA = vA + cross(wA, rA);
     vB = vB + cross(wB, rB);
     const V3 rel = vB - vA;
-    const V3 nn = (sqnorm(nr) > 1e-16) ? normalized(nr) : v3(0.0, 1.0, 0.0);
-    const V3 tv = rel - dot(rel, nn) * nn;
+    const V3 tv = rel - dot(rel, f.nn) * f.nn;
     return tv * dt;
 }
 // FrictionPotential (Friction.cpp:5-62)
@@ -641,6 +655,9 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
                 }
                 const bool fr = sc.has_friction && li < sc.nfc;
                 const double N = fr ? w.nf[(long long)sc.s * kMaxContacts + li] : 0.0;
+                const FricPre fp = fric_pre(d, w, c);  // the contact's data, loaded once
+                const double* cd = w.cdbl + 7 * c;
+                const V3 cn = v3(cd[3], cd[4], cd[5]);
                 for (int k = g; k < K; k += G) {
                     double ak = alpha;
                     for (int q = 0; q < k; ++q) ak *= 0.5;
@@ -650,9 +667,14 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
                         xb[q] = x0b[q] + ak * dxb[q];
                     }
                     double* Tk = T + k * TS;
-                    Tk[OB + tt] = sc.has_barrier ? bar_v(contact_dist_x(w, c, xa, xb), sc.dhat, sc.kappa) : 0.0;
+                    if (sc.has_barrier) {  // contact_dist_x with the normal in registers
+                        const double pr = dot(v3(xb[0], xb[1], xb[2]) - v3(xa[0], xa[1], xa[2]), cn);
+                        Tk[OB + tt] = bar_v(pr > 1e-9 ? pr : 1e-9, sc.dhat, sc.kappa);
+                    } else {
+                        Tk[OB + tt] = 0.0;
+                    }
                     double fv = 0.0;
-                    if (fr && N > 0.0) fv = fric_v(fric_u_x(d, w, c, xa, xb, sc.fdt), N, sc.fmu, sc.feps);
+                    if (fr && N > 0.0) fv = fric_v(fric_u_pre(fp, xa, xb, sc.fdt), N, sc.fmu, sc.feps);
                     Tk[OF + tt] = fv;
                 }
             }
