@@ -8,6 +8,8 @@
 #include "ndipc_internal.h"
 #include "topk_select.h"
 
+#include <algorithm>
+
 namespace nd {
 
 constexpr int kBlock = 256;
@@ -277,12 +279,11 @@ struct BvhLds {
     int* lbid;      // [m] scene-local id of the leaf's body
     double* lbox;   // [6 m]
     unsigned short* stk;  // traversal stacks, kBvhStackBytes
-    unsigned short* hit;  // per-lane hit buffers, kBvhHitBytes
+    unsigned short* hit;  // per-lane hit buffers (nhits entries each)
 };
 constexpr int kBvhLdsPerLeaf = 2 * 4 + 6 * 8 + 4 + 4 + 6 * 8;
 constexpr int kBvhStackBytes = 72 * kBlock * 2;  // 72 entries of 2 bytes per lane
-constexpr int kBvhHits = 32;                     // buffered hits per lane
-constexpr int kBvhHitBytes = kBvhHits * kBlock * 2;
+constexpr int kBvhMinHits = 32;                  // buffered hits per lane, at least
 
 __device__ __forceinline__ Box3 box_at(const double* b, int i) {
     Box3 r;
@@ -302,7 +303,7 @@ __device__ __forceinline__ Box3 box_at(const double* b, int i) {
 // (an insertion sort of the lane's buffer), the order of the emitted pair lists.
 constexpr int kBvhStack = 72;
 template <typename F>
-__device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, double r, bool sorted,
+__device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, double r, bool sorted, int nhits,
                                F&& on_leaf) {
     const Box3 qa = box_inflated(bi, r);
     if (m == 1) return;
@@ -335,7 +336,7 @@ __device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, do
             if (c < 0) {
                 if (T.lbody[~c] > i && box_overlap(qa, box_inflated(box_at(T.lbox, ~c), r))) {
                     hb[kBlock * nh++] = (unsigned short)(~c);
-                    if (nh == kBvhHits) flush();
+                    if (nh == nhits) flush();
                 }
             } else {
                 if (box_overlap(qa, box_inflated(box_at(T.nbox, c), r)) && sp < kBvhStack)
@@ -348,9 +349,10 @@ __device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, do
 
 // Grid (scene, chunk of kBlock bodies). With `lds` the work-group first copies its
 // scene's tree into dynamic LDS (kBvhLdsPerLeaf bytes per leaf, sized for the largest
-// scene at launch); otherwise the traversal reads the tree in HBM.
+// scene at launch), followed by the lanes' stacks and hit buffers of nhits entries;
+// otherwise the traversal reads the tree in HBM.
 template <bool lds>
-__global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass) {
+__global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass, int nhits) {
     extern __shared__ __align__(16) unsigned char bp_lds[];
     const int s = blockIdx.x;
     const int b0 = d.body_off[s], nb = d.body_off[s + 1] - b0;
@@ -437,7 +439,7 @@ __global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass) {
             if (do2 && !any && box_overlap(q2, box_inflated(bj, r2)) && pair_allowed(d, s, idi, idj)) any = 1;
         };
         if constexpr (lds)
-            lbvh_query_lds(T, m, i, bi, rmax, pass != 0,
+            lbvh_query_lds(T, m, i, bi, rmax, pass != 0, nhits,
                            [&](int leaf) { hit(T.lbody[leaf], T.lbid[leaf], box_at(T.lbox, leaf)); });
         else
             lbvh_query(d, w, s, i, rmax, [&](int j) { hit(j, d.body_id[j], ldbox(w.wbox, j)); });
@@ -502,19 +504,24 @@ void launch_broad_phase_step(const Dev& d, const Work& w, hipStream_t st) {
     if (d.N == 0) return;
     const dim3 grid(d.S, (d.max_nb + kBlock - 1) / kBlock);
     // the tree of the largest scene in LDS, with the lanes' traversal stacks and hit
-    // buffers, when all fit (694 bodies with 128 KiB); otherwise the tree in HBM
-    const size_t bytes = size_t(d.max_nb) * kBvhLdsPerLeaf + kBvhStackBytes + kBvhHitBytes;
-    const int lds = bytes <= g_bp_lds_cap ? 1 : 0;
+    // buffers (the rest of the LDS, at least kBvhMinHits entries a lane: 548 bodies with
+    // 128 KiB), otherwise the tree in HBM. A lane whose hits overflow its buffer emits
+    // them batch by batch (later batches by insertion).
+    const size_t fixed = size_t(d.max_nb) * kBvhLdsPerLeaf + kBvhStackBytes;
+    int nhits = 0;
+    if (fixed < g_bp_lds_cap) nhits = int(std::min<size_t>((g_bp_lds_cap - fixed) / (2 * kBlock), 1024));
+    const int lds = nhits >= kBvhMinHits ? 1 : 0;
+    const size_t bytes = fixed + size_t(nhits) * 2 * kBlock;
     timing_begin("bp_step", st);
-    if (lds) hipLaunchKernelGGL(k_bp_step<true>, grid, dim3(kBlock), bytes, st, d, w, 0);
-    else hipLaunchKernelGGL(k_bp_step<false>, grid, dim3(kBlock), 0, st, d, w, 0);
+    if (lds) hipLaunchKernelGGL(k_bp_step<true>, grid, dim3(kBlock), bytes, st, d, w, 0, nhits);
+    else hipLaunchKernelGGL(k_bp_step<false>, grid, dim3(kBlock), 0, st, d, w, 0, 0);
     timing_end("bp_step", st);
     hipLaunchKernelGGL(k_bp_scan, dim3(d.S, 2), dim3(kBlock), 0, st, d, w);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, st, w.pair_cnt, w.pair_off, w.n_pairs_total, d.S);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, st, w.prox_cnt, w.prox_off, w.n_prox_total, d.S);
     timing_begin("bp_step", st);
-    if (lds) hipLaunchKernelGGL(k_bp_step<true>, grid, dim3(kBlock), bytes, st, d, w, 1);
-    else hipLaunchKernelGGL(k_bp_step<false>, grid, dim3(kBlock), 0, st, d, w, 1);
+    if (lds) hipLaunchKernelGGL(k_bp_step<true>, grid, dim3(kBlock), bytes, st, d, w, 1, nhits);
+    else hipLaunchKernelGGL(k_bp_step<false>, grid, dim3(kBlock), 0, st, d, w, 1, 0);
     timing_end("bp_step", st);
 }
 

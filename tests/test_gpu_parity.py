@@ -337,7 +337,7 @@ def test_step_graph_replays_match_eager_steps(engine_factory, name):
 @pytest.mark.parametrize("name", ["tie", "c3x8"])
 def test_broad_phase_tree_in_hbm_matches_tree_in_lds(engine_factory, name):
     """The step's broad phase traverses a copy of each scene's tree in LDS when the
-    largest scene fits (694 bodies), else the tree in HBM. With NDIPC_BP_NO_LDS=1 at
+    largest scene fits (548 bodies), else the tree in HBM. With NDIPC_BP_NO_LDS=1 at
     context creation the HBM path runs on these small scenes: same steps, same bits."""
     sc = perturbed_c2_scene() if name == "tie" else scenes.pack(scenes.config3(n_scenes=8))
     a = engine_factory(sc)
