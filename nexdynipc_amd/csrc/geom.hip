@@ -1704,31 +1704,34 @@ void launch_ccd_batch(int kind, int n, const double* pa, const double* pb, const
 // CCDSystem::computeEarliestTOI (CCD.h:22-82) over the FV candidates of every close
 // pair: vertex body moves p -> p + v dt with frozen rotation, face body likewise.
 // Each candidate starts from earliest = 1; the scene result is the minimum.
+// Work units of one wavefront: (close pair, chunk of 64 of its at most 2048 face-vertex
+// candidates), persistent over the units. A unit loads the pair's two step poses once
+// and its lanes take the chunk's candidates (faces of A x vertices of B, then faces of
+// B x vertices of A, from the top-K lists); units beyond a pair's candidate count end
+// after one check.
+constexpr int kCcdChunks = 2 * kTopK * kTopK / 64;
 __global__ void k_ccd_fv(Dev d, Work w, int gate_any) {
     const int total = *w.n_pairs_total;
-    const long long ncand = (long long)total * 2048;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncand;
-         c += (long long)gridDim.x * blockDim.x) {
-        const int p = int(c / 2048), local = int(c % 2048);
+    const int lane = threadIdx.x & 63;
+    const long long wid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long u = wid; u < (long long)total * kCcdChunks; u += nw) {
+        const int p = int(u / kCcdChunks), chunk = int(u % kCcdChunks);
+        const int* k = w.lcnt + p * 6;
+        const int k0 = k[0], k1 = k[1], n1 = k[4] * k1, n2 = k[5] * k0;
+        if (chunk * 64 >= n1 + n2) continue;
         const int s = w.pair_scene[p];
         SceneParams& sp = d.sp[s];
         if (gate_any && !(sp.enable_ccd && sp.ccd_any)) continue;
+        const int local = chunk * 64 + lane;
+        if (local >= n1 + n2) continue;
         const int A = w.pairs[2 * p], B = w.pairs[2 * p + 1];
-        const int* k = w.lcnt + p * 6;
-        int fb, vb, fl, vl;
-        if (local < 1024) {
-            if (k[1] == 0 || local >= k[4] * k[1]) continue;
-            fb = A; vb = B;
-            fl = w.lists[(p * 6 + 4) * kTopK + local / k[1]];
-            vl = w.lists[(p * 6 + 1) * kTopK + local % k[1]];
-        } else {
-            const int cc = local - 1024;
-            if (k[0] == 0 || cc >= k[5] * k[0]) continue;
-            fb = B; vb = A;
-            fl = w.lists[(p * 6 + 5) * kTopK + cc / k[0]];
-            vl = w.lists[(p * 6 + 0) * kTopK + cc % k[0]];
-        }
         const double dt = sp.dt;
+        const bool first = local < n1;
+        const int cc = first ? local : local - n1;
+        const int fb = first ? A : B, vb = first ? B : A;
+        const int fl = first ? w.lists[(p * 6 + 4) * kTopK + cc / k1] : w.lists[(p * 6 + 5) * kTopK + cc / k0];
+        const int vl = first ? w.lists[(p * 6 + 1) * kTopK + cc % k1] : w.lists[(p * 6 + 0) * kTopK + cc % k0];
         CcdQuery q;
         q.kind = 2;
         q.A.p0 = ld3(d.p, vb); q.A.q0 = ld4(d.q, vb); q.A.p1 = q.A.p0 + ld3(d.v, vb) * dt; q.A.q1 = q.A.q0;
@@ -1743,8 +1746,7 @@ __global__ void k_ccd_fv(Dev d, Work w, int gate_any) {
 }
 
 void launch_ccd_earliest(const Dev& d, const Work& w, int max_pairs, hipStream_t st) {
-    long long n = (long long)max_pairs * 2048;
-    long long g = (n + 255) / 256;
+    long long g = ((long long)max_pairs * kCcdChunks * 64 + 255) / 256;  // a wavefront per unit
     const long long cap = w.grid_ccd > 0 ? w.grid_ccd : 4096;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
