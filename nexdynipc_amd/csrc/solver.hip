@@ -416,8 +416,9 @@ __device__ double objective_grad(const Dev& d, const Work& w, const ScnCtx& sc, 
 
 // Objective value (IPCSolver.cpp:154-165) and contact value (IPCSolver.cpp:223-235).
 // ------------------------------------------------------------------ ordered sums
-// The reference accumulates the objective and the line-search slopes in plain loops.
-// A parallel tree sum rounds differently, and the Armijo test and the ALM stopping test
+// The reference accumulates the objective in plain loops over bodies, joints and
+// contacts (its line-search slopes are Eigen dot products: serial_dot2 below). A
+// parallel tree sum rounds differently, and the Armijo test and the ALM stopping test
 // compare these values in the last bits, so they are summed left to right here: all
 // lanes compute a window of terms into LDS, then one lane per accumulator adds them in
 // order (LDS reads pipelined ahead of the dependent adds); independent accumulators sit
@@ -709,7 +710,6 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
     __syncthreads();
 }
 
-// (sum_i a0[i] b[i], sum_i a1[i] b[i]) left to right; T: kTermLds doubles of LDS.
 // (a0.dot(b), a1.dot(b)) for dynamic-size vectors, in Eigen's order (vecsum.h): the
 // four packet-lane chains of each product sum run on lanes 0..7 of the first wavefront
 // (products stored per chain, contiguous, so each lane streams its own run), then the
