@@ -75,6 +75,10 @@ typedef struct ndipc_params {
 } ndipc_params;
 
 /* ---- context and data ---------------------------------------------------------- */
+/* A context on HIP device `device` with its own stream. Returns 0, or -1 (out is NULL),
+ * -2 (no such device), -3 (device not selectable), -4 (stream creation failed), -5 (the
+ * kernels' LDS limits could not be raised: the device lacks the 160 KiB of LDS per
+ * work-group of gfx950). */
 int32_t ndipc_create(int32_t device, ndipc_ctx** out);
 int32_t ndipc_destroy(ndipc_ctx* ctx);
 const char* ndipc_last_error(const ndipc_ctx* ctx);
