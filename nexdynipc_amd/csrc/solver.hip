@@ -366,48 +366,59 @@ __device__ void eval_elements(const Dev& d, const Work& w, const ScnCtx& sc, con
 
 // Gradient of the Newton objective (IPCSolver.cpp:167-190) per body, gathered in
 // contact order through the body CSR. Returns max |g|.
+// Gradient of body li (6 entries) and its contact part without inertia, gravity and
+// joints (cgv): inertia, h^2 gravity, joints in joint order, then h^2 barrier and h^2
+// friction over the body's contacts in contact order.
+__device__ __forceinline__ void body_grad(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, int li,
+                                          const int* off, const int* ent, double g[6], double cgv[6]) {
+    const double* XH = w.xhat;
+    const int b = sc.b0 + li;
+    const double m = d.mass[b];
+    const double* I = d.inertia + 9 * b;
+    const V3 dp = xpos(X, b) - xpos(XH, b);
+    const V3 gp = m * dp;
+    const V3 dt = xrot(X, b) - xrot(XH, b);
+    g[0] = 0.0 + gp.x; g[1] = 0.0 + gp.y; g[2] = 0.0 + gp.z;
+    g[3] = 0.0 + (I[0] * dt.x + I[1] * dt.y + I[2] * dt.z);
+    g[4] = 0.0 + (I[3] * dt.x + I[4] * dt.y + I[5] * dt.z);
+    g[5] = 0.0 + (I[6] * dt.x + I[7] * dt.y + I[8] * dt.z);
+    const double fy = (-m) * kG;
+    g[0] = g[0] + sc.h2 * ((-m) * 0.0);
+    g[1] = g[1] + sc.h2 * fy;
+    g[2] = g[2] + sc.h2 * ((-m) * 0.0);
+    g[3] = g[3] + sc.h2 * 0.0; g[4] = g[4] + sc.h2 * 0.0; g[5] = g[5] + sc.h2 * 0.0;
+    // joints (ConstraintForm accumulates each joint's J^T f)
+    double gj[6] = {0, 0, 0, 0, 0, 0};
+    for (int e = d.bj_off[b]; e < d.bj_off[b + 1]; ++e) {  // the body's joints, j ascending
+        const double* jg = w.jgrad + 12 * (d.bj_ent[e] >> 1) + 6 * (d.bj_ent[e] & 1);
+        for (int k = 0; k < 6; ++k) gj[k] += jg[k];
+    }
+    for (int k = 0; k < 6; ++k) g[k] = g[k] + gj[k];
+    double gb[6] = {0, 0, 0, 0, 0, 0}, gf[6] = {0, 0, 0, 0, 0, 0};
+    for (int e = off[li]; e < off[li + 1]; ++e) {
+        const int c = ent[e] >> 1, side = ent[e] & 1;
+        const double* cg = w.cgrad + 18 * ((long long)sc.s * kMaxContacts + c);
+        for (int k = 0; k < 3; ++k) gb[k] += cg[side * 3 + k];
+        for (int k = 0; k < 6; ++k) gf[k] += cg[6 + side * 6 + k];
+    }
+    for (int k = 0; k < 6; ++k) {
+        g[k] = g[k] + sc.h2 * gb[k];
+        g[k] = g[k] + sc.h2 * gf[k];
+        cgv[k] = (0.0 + sc.h2 * gb[k]) + sc.h2 * gf[k];
+    }
+}
+
 __device__ double objective_grad(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* G,
                                  double* cgout, double* red) {
     double gmax = 0.0;
-    const double* XH = w.xhat;
     const int* off = w.csr_off + sc.b0 + sc.s;
     const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
     for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
-        const int b = sc.b0 + li;
-        const double m = d.mass[b];
-        const double* I = d.inertia + 9 * b;
-        double g[6];
-        const V3 dp = xpos(X, b) - xpos(XH, b);
-        const V3 gp = m * dp;
-        const V3 dt = xrot(X, b) - xrot(XH, b);
-        g[0] = 0.0 + gp.x; g[1] = 0.0 + gp.y; g[2] = 0.0 + gp.z;
-        g[3] = 0.0 + (I[0] * dt.x + I[1] * dt.y + I[2] * dt.z);
-        g[4] = 0.0 + (I[3] * dt.x + I[4] * dt.y + I[5] * dt.z);
-        g[5] = 0.0 + (I[6] * dt.x + I[7] * dt.y + I[8] * dt.z);
-        const double fy = (-m) * kG;
-        g[0] = g[0] + sc.h2 * ((-m) * 0.0);
-        g[1] = g[1] + sc.h2 * fy;
-        g[2] = g[2] + sc.h2 * ((-m) * 0.0);
-        g[3] = g[3] + sc.h2 * 0.0; g[4] = g[4] + sc.h2 * 0.0; g[5] = g[5] + sc.h2 * 0.0;
-        // joints (ConstraintForm accumulates each joint's J^T f)
-        double gj[6] = {0, 0, 0, 0, 0, 0};
-        for (int e = d.bj_off[b]; e < d.bj_off[b + 1]; ++e) {  // the body's joints, j ascending
-            const double* jg = w.jgrad + 12 * (d.bj_ent[e] >> 1) + 6 * (d.bj_ent[e] & 1);
-            for (int k = 0; k < 6; ++k) gj[k] += jg[k];
-        }
-        for (int k = 0; k < 6; ++k) g[k] = g[k] + gj[k];
-        double gb[6] = {0, 0, 0, 0, 0, 0}, gf[6] = {0, 0, 0, 0, 0, 0};
-        for (int e = off[li]; e < off[li + 1]; ++e) {
-            const int c = ent[e] >> 1, side = ent[e] & 1;
-            const double* cg = w.cgrad + 18 * ((long long)sc.s * kMaxContacts + c);
-            for (int k = 0; k < 3; ++k) gb[k] += cg[side * 3 + k];
-            for (int k = 0; k < 6; ++k) gf[k] += cg[6 + side * 6 + k];
-        }
+        double g[6], cgv[6];
+        body_grad(d, w, sc, X, li, off, ent, g, cgv);
         for (int k = 0; k < 6; ++k) {
-            g[k] = g[k] + sc.h2 * gb[k];
-            g[k] = g[k] + sc.h2 * gf[k];
             G[6 * li + k] = g[k];
-            if (cgout) cgout[6 * li + k] = (0.0 + sc.h2 * gb[k]) + sc.h2 * gf[k];
+            if (cgout) cgout[6 * li + k] = cgv[k];
             gmax = dmax(gmax, fabs(g[k]));
         }
     }
@@ -751,50 +762,55 @@ __device__ void serial_dot2(const double* a0, const double* a1, const double* b,
     __syncthreads();
 }
 
+// Diagonal 6x6 block of body li, accumulated in registers (constant indices) in the
+// reference's triplet order: inertia, joints, barrier contacts, friction contacts.
+__device__ __forceinline__ void body_diag(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, int li,
+                                          const int* off, const int* ent, double D[36]) {
+    const int b = sc.b0 + li;
+#pragma unroll
+    for (int k = 0; k < 36; ++k) D[k] = 0.0;
+    const double m = d.mass[b];
+    D[0] = m; D[7] = m; D[14] = m;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) D[(3 + r) * 6 + 3 + c] = d.inertia[9 * b + 3 * r + c];
+    for (int e = d.bj_off[b]; e < d.bj_off[b + 1]; ++e) {  // the body's joints, j ascending
+        const double* jb = w.jblk + 108 * (d.bj_ent[e] >> 1) + 36 * (d.bj_ent[e] & 1);
+#pragma unroll
+        for (int k = 0; k < 36; ++k) D[k] = D[k] + jb[k];
+    }
+    for (int e = off[li]; e < off[li + 1]; ++e) {
+        const int c = ent[e] >> 1;
+        const long long cc = sc.cb + c;
+        const long long ce = (long long)sc.s * kMaxContacts + c;
+        if (!(w.cactive[ce] & 1)) continue;
+        const double* cd = w.cdbl + 7 * cc;
+        const double n[3] = {cd[3], cd[4], cd[5]};
+        const double b2 = b2arr[c];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) D[r * 6 + q] = D[r * 6 + q] + (b2 * (n[r] * n[q])) * sc.h2;
+    }
+    for (int e = off[li]; e < off[li + 1]; ++e) {
+        const int c = ent[e] >> 1, side = ent[e] & 1;
+        const long long ce = (long long)sc.s * kMaxContacts + c;
+        if (!(w.cactive[ce] & 2)) continue;
+        const double* blk = w.cblk + 144 * ce + side * 36;
+#pragma unroll
+        for (int k = 0; k < 36; ++k) D[k] = D[k] + blk[k];
+    }
+}
+
 // Diagonal 6x6 blocks: inertia + joints + h2 (barrier, then friction) contributions.
 __device__ void assemble_diag(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr) {
     const int* off = w.csr_off + sc.b0 + sc.s;
     const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
     for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
-        const int b = sc.b0 + li;
-        // accumulated in registers (constant indices), in the reference's triplet order:
-        // inertia, joints, barrier contacts, friction contacts
         double D[36];
-#pragma unroll
-        for (int k = 0; k < 36; ++k) D[k] = 0.0;
-        const double m = d.mass[b];
-        D[0] = m; D[7] = m; D[14] = m;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) D[(3 + r) * 6 + 3 + c] = d.inertia[9 * b + 3 * r + c];
-        for (int e = d.bj_off[b]; e < d.bj_off[b + 1]; ++e) {  // the body's joints, j ascending
-            const double* jb = w.jblk + 108 * (d.bj_ent[e] >> 1) + 36 * (d.bj_ent[e] & 1);
-#pragma unroll
-            for (int k = 0; k < 36; ++k) D[k] = D[k] + jb[k];
-        }
-        for (int e = off[li]; e < off[li + 1]; ++e) {
-            const int c = ent[e] >> 1;
-            const long long cc = sc.cb + c;
-            const long long ce = (long long)sc.s * kMaxContacts + c;
-            if (!(w.cactive[ce] & 1)) continue;
-            const double* cd = w.cdbl + 7 * cc;
-            const double n[3] = {cd[3], cd[4], cd[5]};
-            const double b2 = b2arr[c];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int q = 0; q < 3; ++q) D[r * 6 + q] = D[r * 6 + q] + (b2 * (n[r] * n[q])) * sc.h2;
-        }
-        for (int e = off[li]; e < off[li + 1]; ++e) {
-            const int c = ent[e] >> 1, side = ent[e] & 1;
-            const long long ce = (long long)sc.s * kMaxContacts + c;
-            if (!(w.cactive[ce] & 2)) continue;
-            const double* blk = w.cblk + 144 * ce + side * 36;
-#pragma unroll
-            for (int k = 0; k < 36; ++k) D[k] = D[k] + blk[k];
-        }
-        double* Dout = w.D + 36 * b;
+        body_diag(d, w, sc, b2arr, li, off, ent, D);
+        double* Dout = w.D + 36 * (sc.b0 + li);
 #pragma unroll
         for (int k = 0; k < 36; ++k) Dout[k] = D[k];
     }
@@ -1366,17 +1382,55 @@ __device__ __noinline__ bool dense_ldlt_panels(double* M, int m) {
     return true;
 }
 
+// objective_grad, assemble_diag and the singleton solves of component_solve in one pass
+// over the bodies, with the gradient and the 6x6 block in registers: the same values
+// as the three passes (a singleton's right-hand side is -g). The components of this
+// iteration must be built (build_components); *ok_sh is cleared if a singleton's block
+// has a zero or non-finite pivot.
+__device__ double grad_diag_singletons(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* G,
+                                       double* cgout, const double* b2arr, const CompLds& L, double* xo, double* red,
+                                       int* ok_sh) {
+    double gmax = 0.0;
+    const int* off = w.csr_off + sc.b0 + sc.s;
+    const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        double g[6], cgv[6], D[36];
+        body_grad(d, w, sc, X, li, off, ent, g, cgv);
+        for (int k = 0; k < 6; ++k) {
+            G[6 * li + k] = g[k];
+            cgout[6 * li + k] = cgv[k];
+            gmax = dmax(gmax, fabs(g[k]));
+        }
+        body_diag(d, w, sc, b2arr, li, off, ent, D);
+        double* Dout = w.D + 36 * (sc.b0 + li);
+#pragma unroll
+        for (int k = 0; k < 36; ++k) Dout[k] = D[k];
+        if (L.cnt[L.lab[li]] <= 1) {
+            double nb[6];
+            for (int k = 0; k < 6; ++k) nb[k] = -g[k];
+            if (!ldlt6_solve(D, nb, xo + 6 * li)) atomicAnd(ok_sh, 0);
+        }
+    }
+    return wg_reduce(gmax, red, OpMax());
+}
+
+// singles_done: the caller has built the components and solved the singletons
+// (grad_diag_singletons), *ok_sh holding their outcome.
 template <int P>
 __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, const double* b2arr, const double* rhs,
-                               double* xo, CompLds& L, int* ok_sh) {
-    build_components(d, w, sc, L);
-    if (L.maxsize > kDenseBodies) return 2;
-    if (threadIdx.x == 0) *ok_sh = 1;
-    __syncthreads();
-    // singletons (a body without coupling elements has cnt 0)
-    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
-        if (L.cnt[L.lab[li]] > 1) continue;
-        if (!ldlt6_solve(w.D + 36 * (sc.b0 + li), rhs + 6 * li, xo + 6 * li)) atomicAnd(ok_sh, 0);
+                               double* xo, CompLds& L, int* ok_sh, bool singles_done = false) {
+    if (!singles_done) {
+        build_components(d, w, sc, L);
+        if (L.maxsize > kDenseBodies) return 2;
+        if (threadIdx.x == 0) *ok_sh = 1;
+        __syncthreads();
+        // singletons (a body without coupling elements has cnt 0)
+        for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+            if (L.cnt[L.lab[li]] > 1) continue;
+            if (!ldlt6_solve(w.D + 36 * (sc.b0 + li), rhs + 6 * li, xo + 6 * li)) atomicAnd(ok_sh, 0);
+        }
+    } else if (L.maxsize > kDenseBodies) {
+        return 2;
     }
     // components of up to 4 bodies: wavefront v takes the small components numbered
     // v, v + 4, ... in list order
@@ -1587,13 +1641,19 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
                        : kLsBatchLarge;
     for (; it < maxit; ++it) {
         eval_elements(d, w, sc, Xg, b2arr);
-        const double ginf = objective_grad(d, w, sc, Xg, G, CG, red);
+        // coupled components (active contacts and hinges of this iterate), then the
+        // gradient, the diagonal blocks and the singleton solves in one pass
+        build_components(d, w, sc, comp);
+        if (threadIdx.x == 0) ok_sh = 1;
+        __syncthreads();
+        const double ginf = grad_diag_singletons(d, w, sc, Xg, G, CG, b2arr, comp, DX, red, &ok_sh);
         __syncthreads();
         if (ginf < sp.newton_tol) { conv = 1; break; }
-        assemble_diag(d, w, sc, b2arr);
         for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) NG[i] = -G[i];
         __syncthreads();
-        const int cs = component_solve<8>(d, w, sc, b2arr, NG, DX, comp, &ok_sh);
+        const int cs = comp.maxsize > kDenseBodies ? 2
+                       : !ok_sh                   ? 1
+                                                  : component_solve<8>(d, w, sc, b2arr, NG, DX, comp, &ok_sh, true);
         int pit = 0;
         const bool ok = cs == 0 ? true : cs == 2 ? pcg_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh, &pit) : false;
         if (threadIdx.x == 0) {
