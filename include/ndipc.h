@@ -80,6 +80,9 @@ typedef struct ndipc_params {
  * kernels' LDS limits could not be raised: the device lacks the 160 KiB of LDS per
  * work-group of gfx950). */
 int32_t ndipc_create(int32_t device, ndipc_ctx** out);
+/* Frees the context and its device memory (a null context is a no-op). Returns 0, or
+ * -1 when the device or the stream reported an error during teardown (the context is
+ * freed either way). */
 int32_t ndipc_destroy(ndipc_ctx* ctx);
 const char* ndipc_last_error(const ndipc_ctx* ctx);
 /* Stream the context launches on (hipStream_t). */

@@ -74,6 +74,25 @@ int fail(ndipc_ctx* c, const std::string& m) {
         if (e_ != hipSuccess) return fail(ctx, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device scratch of one call: freed when the call returns, on every path.
+struct ScratchBufs {
+    std::vector<void*> p;
+    ScratchBufs() = default;
+    ScratchBufs(const ScratchBufs&) = delete;
+    ScratchBufs& operator=(const ScratchBufs&) = delete;
+    ~ScratchBufs() {
+        for (void* q : p) (void)hipFree(q);
+    }
+    template <typename T>
+    hipError_t alloc(T** out, size_t bytes) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, bytes);
+        if (e == hipSuccess) p.push_back(q);
+        *out = static_cast<T*>(q);
+        return e;
+    }
+};
+
 template <typename T>
 T* dalloc(ndipc_ctx* ctx, size_t n) {
     if (n == 0) n = 1;
@@ -86,12 +105,12 @@ T* dalloc(ndipc_ctx* ctx, size_t n) {
 template <typename T>
 T* dupload(ndipc_ctx* ctx, const T* h, size_t n) {
     T* p = dalloc<T>(ctx, n);
-    if (p && h && n) hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice);
+    if (p && h && n && hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     return p;
 }
 
 void free_adjoint(ndipc_ctx* ctx) {
-    for (void* p : ctx->adj_allocs) hipFree(p);
+    for (void* p : ctx->adj_allocs) (void)hipFree(p);
     ctx->adj_allocs.clear();
     ctx->adj = nd::Adj{};
     ctx->adj_every = ctx->adj_steps = ctx->adj_nck = 0;
@@ -108,7 +127,7 @@ void drop_graph(ndipc_ctx* ctx) {
 void free_all(ndipc_ctx* ctx) {
     drop_graph(ctx);
     free_adjoint(ctx);
-    for (void* p : ctx->allocs) hipFree(p);
+    for (void* p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     ctx->uploaded = false;
 }
@@ -236,13 +255,15 @@ int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
 
 int32_t ndipc_destroy(ndipc_ctx* ctx) {
     if (!ctx) return 0;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    // tear down in any case; -1 reports that the device or the stream had failed
+    int rc = 0;
+    if (hipSetDevice(ctx->device) != hipSuccess) rc = -1;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = -1;
     if (g_timing_ctx == ctx) g_timing_ctx = nullptr;
     free_all(ctx);
-    hipStreamDestroy(ctx->stream);
+    if (hipStreamDestroy(ctx->stream) != hipSuccess) rc = -1;
     delete ctx;
-    return 0;
+    return rc;
 }
 
 const char* ndipc_last_error(const ndipc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -876,16 +897,15 @@ int32_t ndipc_select_top_k(ndipc_ctx* ctx, const double* scores, int32_t n, int3
     if (n < 0 || n > nd::kMaxPrims) return fail(ctx, "n must be in [0, 2048]");
     double* ds = nullptr;
     int* di = nullptr;
-    HIPCHK(hipMalloc(&ds, sizeof(double) * std::max(n, 1)));
-    HIPCHK(hipMalloc(&di, sizeof(int) * (std::max(n, 1) + 1)));
+    ScratchBufs sb;  // freed on every return
+    HIPCHK(sb.alloc(&ds, sizeof(double) * std::max(n, 1)));
+    HIPCHK(sb.alloc(&di, sizeof(int) * (std::max(n, 1) + 1)));
     HIPCHK(hipMemcpy(ds, scores, sizeof(double) * n, hipMemcpyHostToDevice));
     nd::launch_select_topk_scores(ds, n, k, di + 1, di, ctx->stream);
     int m = 0;
     HIPCHK(hipMemcpyAsync(&m, di, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (m > 0) HIPCHK(hipMemcpy(out, di + 1, 4 * size_t(m), hipMemcpyDeviceToHost));
-    hipFree(ds);
-    hipFree(di);
     return m;
 }
 
@@ -978,7 +998,8 @@ static int forms_eval(ndipc_ctx* ctx, int scene, int which, int nc, const int32_
     const size_t nd6 = 6 * size_t(nb);
     const size_t need = 8 * (nd6 + nd6 + 1 + (hess ? nd6 * nd6 : 0) + nd::kMaxContacts);
     double* buf = nullptr;
-    HIPCHK(hipMalloc(&buf, need));
+    ScratchBufs sb;  // freed on every return
+    HIPCHK(sb.alloc(&buf, need));
     double* dx = buf;
     double* dg = buf + nd6;
     double* dv = buf + 2 * nd6;
@@ -992,7 +1013,6 @@ static int forms_eval(ndipc_ctx* ctx, int scene, int which, int nc, const int32_
     if (grad) HIPCHK(hipMemcpy(grad, dg, 8 * nd6, hipMemcpyDeviceToHost));
     if (hess) HIPCHK(hipMemcpy(hess, dh, 8 * nd6 * nd6, hipMemcpyDeviceToHost));
     if (nfo && nc) HIPCHK(hipMemcpy(nfo, dnf, 8 * size_t(nc), hipMemcpyDeviceToHost));
-    hipFree(buf);
     return 0;
 }
 
@@ -1020,8 +1040,9 @@ int32_t ndipc_ccd_toi_batch(ndipc_ctx* ctx, int32_t kind, int32_t n, const doubl
     const size_t dbls = size_t(n) * (14 + 14 + 6 + 9 + 1 + 1);
     double* buf = nullptr;
     int* ib = nullptr;
-    HIPCHK(hipMalloc(&buf, 8 * dbls));
-    HIPCHK(hipMalloc(&ib, 8 * size_t(n)));
+    ScratchBufs sb;  // freed on every return
+    HIPCHK(sb.alloc(&buf, 8 * dbls));
+    HIPCHK(sb.alloc(&ib, 8 * size_t(n)));
     double* dpa = buf;
     double* dpb = dpa + 14 * size_t(n);
     double* da = dpb + 14 * size_t(n);
@@ -1038,8 +1059,6 @@ int32_t ndipc_ccd_toi_batch(ndipc_ctx* ctx, int32_t kind, int32_t n, const doubl
     HIPCHK(hipMemcpy(hit, ib, 4 * size_t(n), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(toi, dt, 8 * size_t(n), hipMemcpyDeviceToHost));
     if (pops) HIPCHK(hipMemcpy(pops, ib + n, 4 * size_t(n), hipMemcpyDeviceToHost));
-    hipFree(buf);
-    hipFree(ib);
     return 0;
 }
 
@@ -1072,8 +1091,9 @@ int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t nb, const double* diag, int32_t 
     const size_t dbls = 36 * size_t(nb) + 36 * size_t(std::max(noff, 1)) + nd6 + nd6 + 4 * nd6 + 36 * size_t(nb) + 2;
     double* buf = nullptr;
     int* ib = nullptr;
-    HIPCHK(hipMalloc(&buf, 8 * dbls));
-    HIPCHK(hipMalloc(&ib, 4 * (2 * size_t(std::max(noff, 1)) + 1)));
+    ScratchBufs sb;  // freed on every return
+    HIPCHK(sb.alloc(&buf, 8 * dbls));
+    HIPCHK(sb.alloc(&ib, 4 * (2 * size_t(std::max(noff, 1)) + 1)));
     double* dD = buf;
     double* dO = dD + 36 * size_t(nb);
     double* db = dO + 36 * size_t(std::max(noff, 1));
@@ -1091,8 +1111,6 @@ int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t nb, const double* diag, int32_t 
     HIPCHK(hipMemcpy(x, dx, 8 * nd6, hipMemcpyDeviceToHost));
     if (iters) HIPCHK(hipMemcpy(iters, ib, 4, hipMemcpyDeviceToHost));
     if (rel) HIPCHK(hipMemcpy(rel, drel, 8, hipMemcpyDeviceToHost));
-    hipFree(buf);
-    hipFree(ib);
     return 0;
 }
 
@@ -1202,7 +1220,7 @@ int32_t ndipc_timing_get(ndipc_ctx* ctx, const char* kernel, double* avg_ms, int
     double tot = 0.0;
     for (auto& e : it->second.ev) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e.first, e.second);
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
         tot += ms;
     }
     *count = int32_t(it->second.ev.size());
@@ -1215,13 +1233,13 @@ int32_t ndipc_round_ops(ndipc_ctx* ctx, const double* a, const double* b, int32_
     HIPCHK(hipSetDevice(ctx->device));
     if (n <= 0) return 0;
     double* buf = nullptr;
-    HIPCHK(hipMalloc(&buf, 8 * size_t(n) * 8));
+    ScratchBufs sb;  // freed on every return
+    HIPCHK(sb.alloc(&buf, 8 * size_t(n) * 8));
     HIPCHK(hipMemcpy(buf, a, 8 * size_t(n), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(buf + n, b, 8 * size_t(n), hipMemcpyHostToDevice));
     nd::launch_round_ops(buf, buf + n, n, buf + 2 * size_t(n), ctx->stream);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out6, buf + 2 * size_t(n), 8 * 6 * size_t(n), hipMemcpyDeviceToHost));
-    hipFree(buf);
     return 0;
 }
 
@@ -1231,14 +1249,14 @@ int32_t ndipc_elemfn(ndipc_ctx* ctx, int32_t fn, const double* x, const double* 
     HIPCHK(hipSetDevice(ctx->device));
     if (n <= 0) return 0;
     double* buf = nullptr;
-    HIPCHK(hipMalloc(&buf, 3 * 8 * size_t(n)));
+    ScratchBufs sb;  // freed on every return
+    HIPCHK(sb.alloc(&buf, 3 * 8 * size_t(n)));
     HIPCHK(hipMemcpy(buf, x, 8 * size_t(n), hipMemcpyHostToDevice));
     if (y) HIPCHK(hipMemcpy(buf + n, y, 8 * size_t(n), hipMemcpyHostToDevice));
     else HIPCHK(hipMemset(buf + n, 0, 8 * size_t(n)));
     nd::launch_elemfn(fn, buf, buf + n, n, buf + 2 * size_t(n), ctx->stream);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, buf + 2 * size_t(n), 8 * size_t(n), hipMemcpyDeviceToHost));
-    hipFree(buf);
     return 0;
 }
 
