@@ -364,13 +364,21 @@ __device__ void eval_elements(const Dev& d, const Work& w, const ScnCtx& sc, con
     __syncthreads();
 }
 
-// Gradient of the Newton objective (IPCSolver.cpp:167-190) per body, gathered in
-// contact order through the body CSR. Returns max |g|.
-// Gradient of body li (6 entries) and its contact part without inertia, gravity and
-// joints (cgv): inertia, h^2 gravity, joints in joint order, then h^2 barrier and h^2
-// friction over the body's contacts in contact order.
-__device__ __forceinline__ void body_grad(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, int li,
-                                          const int* off, const int* ent, double g[6], double cgv[6]) {
+// Gradient of the Newton objective (IPCSolver.cpp:167-190) and the diagonal 6x6 block
+// of the Hessian of body li, gathered through the body CSR.
+//   g: inertia, h^2 gravity, joints in joint order, then h^2 barrier and h^2 friction
+//      summed over the body's contact entries in contact order; cgv: the contact part
+//      alone.
+//   D: as body_diag (inertia, joints, barrier contacts, friction contacts).
+// A contact entry's data (its gradient rows, active flags, normal and b'') are loaded
+// together, with the next entry's index fetched beside them, so each entry waits out
+// one memory latency rather than a chain of dependent loads; the friction blocks
+// follow in a second pass over the entries found friction-active (a mask for the
+// first 64 entries, a plain scan beyond). Every accumulator adds its terms in the same
+// order as body_diag and the gradient sums.
+__device__ __forceinline__ void body_grad_diag(const Dev& d, const Work& w, const ScnCtx& sc, const double* X,
+                                               const double* b2arr, int li, const int* off, const int* ent,
+                                               double g[6], double cgv[6], double D[36]) {
     const double* XH = w.xhat;
     const int b = sc.b0 + li;
     const double m = d.mass[b];
@@ -387,42 +395,70 @@ __device__ __forceinline__ void body_grad(const Dev& d, const Work& w, const Scn
     g[1] = g[1] + sc.h2 * fy;
     g[2] = g[2] + sc.h2 * ((-m) * 0.0);
     g[3] = g[3] + sc.h2 * 0.0; g[4] = g[4] + sc.h2 * 0.0; g[5] = g[5] + sc.h2 * 0.0;
-    // joints (ConstraintForm accumulates each joint's J^T f)
+#pragma unroll
+    for (int k = 0; k < 36; ++k) D[k] = 0.0;
+    D[0] = m; D[7] = m; D[14] = m;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) D[(3 + r) * 6 + 3 + c] = I[3 * r + c];
+    // joints (ConstraintForm accumulates each joint's J^T f and its blocks)
     double gj[6] = {0, 0, 0, 0, 0, 0};
     for (int e = d.bj_off[b]; e < d.bj_off[b + 1]; ++e) {  // the body's joints, j ascending
-        const double* jg = w.jgrad + 12 * (d.bj_ent[e] >> 1) + 6 * (d.bj_ent[e] & 1);
+        const int je = d.bj_ent[e];
+        const double* jg = w.jgrad + 12 * (je >> 1) + 6 * (je & 1);
         for (int k = 0; k < 6; ++k) gj[k] += jg[k];
+        const double* jb = w.jblk + 108 * (je >> 1) + 36 * (je & 1);
+#pragma unroll
+        for (int k = 0; k < 36; ++k) D[k] = D[k] + jb[k];
     }
     for (int k = 0; k < 6; ++k) g[k] = g[k] + gj[k];
+    // contacts: gradient rows of every entry, barrier blocks of the barrier-active ones
+    const long long sbase = (long long)sc.s * kMaxContacts;
+    const int e0 = off[li], e1 = off[li + 1];
     double gb[6] = {0, 0, 0, 0, 0, 0}, gf[6] = {0, 0, 0, 0, 0, 0};
-    for (int e = off[li]; e < off[li + 1]; ++e) {
-        const int c = ent[e] >> 1, side = ent[e] & 1;
-        const double* cg = w.cgrad + 18 * ((long long)sc.s * kMaxContacts + c);
+    unsigned long long fmask = 0;  // entry e0 + i (i < 64) has friction active
+    int en = e0 < e1 ? ent[e0] : 0;
+    for (int e = e0; e < e1; ++e) {
+        const int c = en >> 1, side = en & 1;
+        en = e + 1 < e1 ? ent[e + 1] : 0;
+        const double* cg = w.cgrad + 18 * (sbase + c);
+        const int act = w.cactive[sbase + c];
+        const double* cd = w.cdbl + 7 * (sc.cb + c);
+        const double n[3] = {cd[3], cd[4], cd[5]};
+        const double b2 = b2arr[c];
         for (int k = 0; k < 3; ++k) gb[k] += cg[side * 3 + k];
         for (int k = 0; k < 6; ++k) gf[k] += cg[6 + side * 6 + k];
+        if (act & 1) {
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) D[r * 6 + q] = D[r * 6 + q] + (b2 * (n[r] * n[q])) * sc.h2;
+        }
+        if ((act & 2) && e - e0 < 64) fmask |= 1ull << (e - e0);
     }
     for (int k = 0; k < 6; ++k) {
         g[k] = g[k] + sc.h2 * gb[k];
         g[k] = g[k] + sc.h2 * gf[k];
         cgv[k] = (0.0 + sc.h2 * gb[k]) + sc.h2 * gf[k];
     }
-}
-
-__device__ double objective_grad(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* G,
-                                 double* cgout, double* red) {
-    double gmax = 0.0;
-    const int* off = w.csr_off + sc.b0 + sc.s;
-    const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
-    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
-        double g[6], cgv[6];
-        body_grad(d, w, sc, X, li, off, ent, g, cgv);
-        for (int k = 0; k < 6; ++k) {
-            G[6 * li + k] = g[k];
-            if (cgout) cgout[6 * li + k] = cgv[k];
-            gmax = dmax(gmax, fabs(g[k]));
-        }
+    // friction blocks, entries in order
+    int ef = fmask ? ent[e0 + __ffsll(fmask) - 1] : 0;
+    while (fmask) {
+        fmask &= fmask - 1;
+        const int c = ef >> 1, side = ef & 1;
+        ef = fmask ? ent[e0 + __ffsll(fmask) - 1] : 0;
+        const double* blk = w.cblk + 144 * (sbase + c) + side * 36;
+#pragma unroll
+        for (int k = 0; k < 36; ++k) D[k] = D[k] + blk[k];
     }
-    return wg_reduce(gmax, red, OpMax());
+    for (int e = e0 + 64; e < e1; ++e) {
+        const int c = ent[e] >> 1, side = ent[e] & 1;
+        if (!(w.cactive[sbase + c] & 2)) continue;
+        const double* blk = w.cblk + 144 * (sbase + c) + side * 36;
+#pragma unroll
+        for (int k = 0; k < 36; ++k) D[k] = D[k] + blk[k];
+    }
 }
 
 // Objective value (IPCSolver.cpp:154-165) and contact value (IPCSolver.cpp:223-235).
@@ -1382,9 +1418,9 @@ __device__ __noinline__ bool dense_ldlt_panels(double* M, int m) {
     return true;
 }
 
-// objective_grad, assemble_diag and the singleton solves of component_solve in one pass
-// over the bodies, with the gradient and the 6x6 block in registers: the same values
-// as the three passes (a singleton's right-hand side is -g). The components of this
+// The gradient, the diagonal blocks (assemble_diag) and the singleton solves of
+// component_solve in one pass over the bodies, with the gradient and the 6x6 block in
+// registers: the same values as separate passes (a singleton's right-hand side is -g). The components of this
 // iteration must be built (build_components); *ok_sh is cleared if a singleton's block
 // has a zero or non-finite pivot.
 __device__ double grad_diag_singletons(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* G,
@@ -1395,13 +1431,12 @@ __device__ double grad_diag_singletons(const Dev& d, const Work& w, const ScnCtx
     const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
     for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
         double g[6], cgv[6], D[36];
-        body_grad(d, w, sc, X, li, off, ent, g, cgv);
+        body_grad_diag(d, w, sc, X, b2arr, li, off, ent, g, cgv, D);
         for (int k = 0; k < 6; ++k) {
             G[6 * li + k] = g[k];
             cgout[6 * li + k] = cgv[k];
             gmax = dmax(gmax, fabs(g[k]));
         }
-        body_diag(d, w, sc, b2arr, li, off, ent, D);
         double* Dout = w.D + 36 * (sc.b0 + li);
 #pragma unroll
         for (int k = 0; k < 36; ++k) Dout[k] = D[k];
