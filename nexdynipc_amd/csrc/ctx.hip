@@ -31,6 +31,10 @@ struct TimingRec {
 struct ndipc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream for the Newton line-search helpers, forked from and joined to
+    // `stream` around each Newton launch
+    hipStream_t hstream = nullptr;
+    hipEvent_t hfork = nullptr, hjoin = nullptr;
     std::string err;
     bool uploaded = false;
     Dev d{};
@@ -41,6 +45,7 @@ struct ndipc_ctx {
     long long pair_cap = 0;
     std::vector<SceneParams> sp;
     std::vector<void*> allocs;
+    bool alloc_failed = false;  // a dalloc / dupload of the current upload failed
     double* d_dt = nullptr;
     double* d_scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -97,15 +102,24 @@ template <typename T>
 T* dalloc(ndipc_ctx* ctx, size_t n) {
     if (n == 0) n = 1;
     void* p = nullptr;
-    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
-    if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) {
+        ctx->alloc_failed = true;
+        return nullptr;
+    }
     ctx->allocs.push_back(p);
+    if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) {
+        ctx->alloc_failed = true;
+        return nullptr;
+    }
     return static_cast<T*>(p);
 }
 template <typename T>
 T* dupload(ndipc_ctx* ctx, const T* h, size_t n) {
     T* p = dalloc<T>(ctx, n);
-    if (p && h && n && hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    if (p && h && n && hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+        ctx->alloc_failed = true;
+        return nullptr;
+    }
     return p;
 }
 
@@ -129,6 +143,7 @@ void free_all(ndipc_ctx* ctx) {
     free_adjoint(ctx);
     for (void* p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
+    ctx->alloc_failed = false;
     ctx->uploaded = false;
 }
 
@@ -249,6 +264,15 @@ int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
         delete c;
         return -4;
     }
+    if (hipStreamCreate(&c->hstream) != hipSuccess ||
+        hipEventCreateWithFlags(&c->hfork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->hjoin, hipEventDisableTiming) != hipSuccess) {
+        if (c->hfork) (void)hipEventDestroy(c->hfork);
+        if (c->hstream) (void)hipStreamDestroy(c->hstream);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return -4;
+    }
     *out = c;
     return 0;
 }
@@ -261,6 +285,10 @@ int32_t ndipc_destroy(ndipc_ctx* ctx) {
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = -1;
     if (g_timing_ctx == ctx) g_timing_ctx = nullptr;
     free_all(ctx);
+    if (hipStreamSynchronize(ctx->hstream) != hipSuccess) rc = -1;
+    if (hipEventDestroy(ctx->hfork) != hipSuccess) rc = -1;
+    if (hipEventDestroy(ctx->hjoin) != hipSuccess) rc = -1;
+    if (hipStreamDestroy(ctx->hstream) != hipSuccess) rc = -1;
     if (hipStreamDestroy(ctx->stream) != hipSuccess) rc = -1;
     delete ctx;
     return rc;
@@ -482,6 +510,15 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     w.node_box = dalloc<double>(ctx, 6 * size_t(N));
     w.nleaves = dalloc<int>(ctx, S);
     w.bvh_age = dalloc<int>(ctx, S);  // zero: the first call builds
+    {
+        // line-search helper work-groups for small batches (NDIPC_NEWTON_HELPERS=0: off)
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 0;
+        const char* env = std::getenv("NDIPC_NEWTON_HELPERS");
+        const bool off = env && env[0] == '0';
+        w.newton_helpers = (!off && (long long)S * (1 + nd::kLsBatch) <= ncu) ? 1 : 0;
+        w.hbox = dalloc<nd::HelperBox>(ctx, S);
+    }
     w.pair_cnt = dalloc<int>(ctx, S);
     w.pair_off = dalloc<int>(ctx, S + 1);
     w.pairs = dalloc<int>(ctx, 2 * PC);
@@ -535,8 +572,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     ctx->d_dt = dalloc<double>(ctx, S);
     ctx->scratch_bytes = 0;
     ctx->d_scratch = nullptr;
-    for (void* p : ctx->allocs)
-        if (!p) return fail(ctx, "device allocation failed");
+    if (ctx->alloc_failed) return fail(ctx, "device allocation or upload failed");
     ctx->uploaded = true;
     return sync_sp_to_dev(ctx);
 }
@@ -683,6 +719,12 @@ static int enqueue_step(ndipc_ctx* ctx, const nd::Adj* rec, int k, int alm_max) 
     nd::launch_pose_from_state(d, w, nullptr, st);
     nd::launch_narrow_phase(d, w, mp, false, st);
     nd::launch_contacts_setup(d, w, st);
+    if (w.newton_helpers && alm_max > 0) {  // line-search helpers for the step's Newton launches
+        HIPCHK(hipEventRecord(ctx->hfork, st));
+        HIPCHK(hipStreamWaitEvent(ctx->hstream, ctx->hfork, 0));
+        nd::launch_newton_helpers(d, w, alm_max, ctx->hstream);
+        HIPCHK(hipEventRecord(ctx->hjoin, ctx->hstream));
+    }
     for (int a = 0; a < alm_max; ++a) {
         if (rec) nd::launch_adj_snapshot(d, w, *rec, k * rec->A + a, st);
         nd::launch_newton(d, w, st);
@@ -691,6 +733,7 @@ static int enqueue_step(ndipc_ctx* ctx, const nd::Adj* rec, int k, int alm_max) 
         nd::launch_narrow_phase(d, w, mp, true, st);
         nd::launch_alm_post(d, w, st);
     }
+    if (w.newton_helpers && alm_max > 0) HIPCHK(hipStreamWaitEvent(st, ctx->hjoin, 0));
     nd::launch_finalize(d, w, st);
     return 0;
 }
@@ -1268,9 +1311,16 @@ template <typename T>
 T* aalloc(ndipc_ctx* ctx, size_t n) {
     if (n == 0) n = 1;
     void* p = nullptr;
-    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
-    if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return nullptr;
+    // a failure leaves a null entry, which the caller's check of adj_allocs finds
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) {
+        ctx->adj_allocs.push_back(nullptr);
+        return nullptr;
+    }
     ctx->adj_allocs.push_back(p);
+    if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) {
+        ctx->adj_allocs.push_back(nullptr);
+        return nullptr;
+    }
     return static_cast<T*>(p);
 }
 

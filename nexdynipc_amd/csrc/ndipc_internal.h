@@ -29,6 +29,20 @@ constexpr int kDenseRows = 6 * kDenseBodies;
 constexpr int kLsBatch = 20;          // line-search trials evaluated together, at most
 constexpr int kLsBatchLarge = 8;      // ... for scenes whose 20 trials need several LDS rounds
 
+// Mailbox between a scene's Newton work-group and its kLsBatch line-search helper
+// work-groups (small batches only; solver.hip, k_newton_helper). Zero-initialised.
+struct HelperBox {
+    int epoch;           // Newton launches opened so far
+    int bseq;            // batches posted in the current launch (0: none)
+    int K;               // trials in the posted batch
+    int quit;            // 1: the current launch's helpers leave
+    int dead;            // 1: a helper left on its time bound
+    int hep[kLsBatch];   // per helper: the epoch it last served
+    int ack[kLsBatch];   // per helper: the batch it last evaluated
+    double alpha;        // step length of trial 0 of the posted batch
+    double val[kLsBatch], cval[kLsBatch];
+};
+
 struct SceneParams {
     // settings
     double dhat, kappa, mu, ccd_safety;
@@ -109,6 +123,10 @@ struct Work {
     double* node_box;        // [N*6]
     int* nleaves;            // [S]
     int* bvh_age;            // [S] LBVH builds and refits since the last full build
+    // Newton line-search helpers: 1 when every scene gets kLsBatch helper work-groups
+    // beside its Newton work-group (S * (1 + kLsBatch) fit the CUs at one per CU)
+    int newton_helpers;
+    HelperBox* hbox;         // [S]
     // body pairs
     long long* pair_cap_off; // [S+1] capacity offsets (n(n-1)/2 per scene)
     int* pair_cnt;           // [S]
@@ -229,6 +247,9 @@ void launch_ccd_dt(const Dev& d, const Work& w, hipStream_t st);
 void launch_pack(const Dev& d, const Work& w, hipStream_t st);
 void launch_contacts_setup(const Dev& d, const Work& w, hipStream_t st);
 void launch_newton(const Dev& d, const Work& w, hipStream_t st);
+// k_newton's line-search helpers (w.newton_helpers) for the n_launch Newton launches of
+// one step, to run beside the step's kernels
+void launch_newton_helpers(const Dev& d, const Work& w, int n_launch, hipStream_t st);
 void launch_alm_post(const Dev& d, const Work& w, hipStream_t st);
 void launch_finalize(const Dev& d, const Work& w, hipStream_t st);
 void launch_barrier_init_ccd(const Dev& d, const Work& w, hipStream_t st);

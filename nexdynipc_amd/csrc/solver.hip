@@ -1642,15 +1642,179 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
     return 0;
 }
 
-// ------------------------------------------------------------------ Newton kernel
+// ------------------------------------------------------------------ line-search helpers
+// In a small batch (S (1 + kLsBatch) work-groups fit the CUs at one per CU) each scene
+// lends its line search to kLsBatch helper work-groups running beside it: the scene's
+// work-group posts a batch of K trials (K, the first step length) in the scene's
+// HelperBox and waits, and helper h evaluates trial h with objective_value_batch at
+// K = 1, whose sums are those of the scene's own batch evaluation, so the values are
+// the same bits whichever work-group computes them. Every wait is bounded in time: a
+// helper that hears nothing for kHelperIdleTicks marks the box dead and leaves, and a
+// scene work-group that waits longer than kHelperWaitTicks for an acknowledgement
+// evaluates the batch itself and sends its helpers away; results never depend on the
+// helpers being there. Ticks: wall_clock64, 100 MHz on gfx950.
+constexpr unsigned long long kHelperIdleTicks = 20000000;  // 200 ms
+constexpr unsigned long long kHelperWaitTicks = 2000000;  // 20 ms
+
+__device__ __forceinline__ void st_rel(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_rlx(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_rlx(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rlx(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rlx(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Scene work-group, lane 0: opens this launch for the helpers (before any batch).
+__device__ void helpers_open(HelperBox* hb) {
+    st_rlx(&hb->bseq, 0);
+    st_rlx(&hb->quit, 0);
+    for (int h = 0; h < kLsBatch; ++h) st_rlx(&hb->ack[h], 0);
+    __hip_atomic_fetch_add(&hb->epoch, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Scene work-group, all lanes: trials alpha, alpha / 2, ... (K of them) by the helpers,
+// values to vals / cvals (LDS); bseq: this batch's number (1, 2, ... per launch).
+// Returns false, with the helpers sent away, when they did not answer in time.
+__device__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, double* vals, double* cvals,
+                              int* flag) {
+    __threadfence();  // the work-group's stores of x and dx, visible to the helpers
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st_rlx(&hb->K, K);
+        st_rlx(&hb->alpha, alpha);
+        st_rel(&hb->bseq, bseq);
+        const unsigned long long t0 = wall_clock64();
+        int ok = 1;
+        for (int h = 0; h < K && ok; ++h) {
+            while (ld_rlx(&hb->ack[h]) != bseq) {
+                if (ld_rlx(&hb->dead) || wall_clock64() - t0 > kHelperWaitTicks) {
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        // one acquire for all the acknowledgements (an acquire invalidates caches, so
+        // the polls above are relaxed)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (ok) {
+            for (int k = 0; k < K; ++k) {
+                vals[k] = ld_rlx(&hb->val[k]);
+                cvals[k] = ld_rlx(&hb->cval[k]);
+            }
+        } else {
+            st_rel(&hb->quit, 1);
+        }
+        *flag = ok;
+    }
+    __syncthreads();
+    return *flag != 0;
+}
+
+// Helper work-group blockIdx.x = s kLsBatch + h serves trial h of each batch of scene s
+// through the n_launch Newton launches of a step (each opens an epoch and ends with
+// quit), then leaves; it also leaves, marking the box dead for the rest of the step, when
+// it hears nothing for kHelperIdleTicks. It runs beside the step's kernels, on a second
+// stream forked from and joined to theirs.
+__global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_launch) {
+    __shared__ double red[kBlk];
+    __shared__ alignas(16) double T[kTermLds];
+    __shared__ double lv[2];
+    __shared__ int cmd;
+    __shared__ int K_sh;
+    __shared__ double alpha_sh;
+    const int s = blockIdx.x / kLsBatch, h = blockIdx.x % kLsBatch;
+    HelperBox* hb = w.hbox + s;
+    for (int launch = 0; launch < n_launch; ++launch) {
+        if (threadIdx.x == 0) {  // wait for the next Newton launch to open
+            const int last = ld_rlx(&hb->hep[h]);
+            const unsigned long long t0 = wall_clock64();
+            int ep = last;
+            while ((ep = ld_rlx(&hb->epoch)) == last && wall_clock64() - t0 <= kHelperIdleTicks)
+                __builtin_amdgcn_s_sleep(8);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (ep == last) {
+                st_rel(&hb->dead, 1);
+                cmd = -1;
+            } else {
+                st_rlx(&hb->hep[h], ep);
+                cmd = 0;
+            }
+        }
+        __syncthreads();
+        if (cmd < 0) return;
+        // the scene's parameters of this launch (contact count, barrier stiffness, ...),
+        // set by the kernels before it
+        ScnCtx sc;
+        load_scn(d, s, sc);
+        int served = 0;
+        for (;;) {
+            if (threadIdx.x == 0) {
+                const unsigned long long t0 = wall_clock64();
+                int c = -1;
+                for (;;) {  // relaxed polls (an acquire invalidates caches), then one acquire
+                    if (ld_rlx(&hb->quit)) break;
+                    const int b = ld_rlx(&hb->bseq);
+                    if (b != served) {
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                        c = b;
+                        K_sh = ld_rlx(&hb->K);
+                        alpha_sh = ld_rlx(&hb->alpha);
+                        break;
+                    }
+                    if (wall_clock64() - t0 > kHelperIdleTicks) {
+                        st_rel(&hb->dead, 1);
+                        c = -2;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(4);
+                }
+                cmd = c;
+            }
+            __syncthreads();
+            const int c = cmd;
+            if (c == -2) return;
+            if (c < 0) break;  // this launch is closed
+            served = c;        // lane 0's acquire, then the barrier: x, dx of the batch are visible
+            if (h < K_sh) {
+                double a = alpha_sh;  // alpha / 2^h, halved as the scene's batch halves it
+                for (int q = 0; q < h; ++q) a *= 0.5;
+                objective_value_batch(d, w, sc, 1, w.x, w.dx, a, red, lv, lv + 1, T);
+                if (threadIdx.x == 0) {
+                    st_rlx(&hb->val[h], lv[0]);
+                    st_rlx(&hb->cval[h], lv[1]);
+                    st_rel(&hb->ack[h], c);
+                }
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+}
+
+// One work-group per scene; with newton_helpers, k_newton_helper runs beside it.
 __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     __shared__ double red[kBlk];
     __shared__ int ok_sh;
     __shared__ NewtonLds nl;
+    __shared__ double lsv[2 * kLsBatch];  // trial values, then contact values
     CompLds& comp = nl.comp;
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
-    if (sp.alm_done) return;
+    HelperBox* hb = w.newton_helpers ? w.hbox + s : nullptr;
+    if (hb && threadIdx.x == 0) helpers_open(hb);
+    if (sp.alm_done) {
+        if (hb && threadIdx.x == 0) st_rel(&hb->quit, 1);
+        return;
+    }
     ScnCtx sc;
     load_scn(d, s, sc);
     double* X = w.x + 6 * sc.b0;
@@ -1669,8 +1833,9 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     // the accepted trial instead of evaluating them again.
     double fx_next = 0.0, fc_next = 0.0;
     bool have_f = false;
-    __shared__ double lsv[2 * kLsBatch];  // trial values, then contact values
     int kprev = kLsBatchLarge;            // trials the previous line search used
+    bool helped = hb != nullptr;          // the helpers answer (until they fail to)
+    int bseq = 0;                         // batches posted to the helpers
     const int Kb = 3 * (sc.n + (sc.n & 1) + max(sc.j1 - sc.j0, sc.nc) + 1) <= ((kTermLds / kLsBatch) & ~1)
                        ? kLsBatch
                        : kLsBatchLarge;
@@ -1715,13 +1880,15 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         // The trials alpha = 1, 1/2, 1/4, ... (at most 20) are evaluated in batches: the
         // first as many as the previous line search needed, the next ones all that
         // remain, each at most Kb (20 where their terms fit one LDS round, 8
-        // otherwise). The first that passes is taken, and only the trials up to it count.
+        // otherwise), or with helpers all remaining trials at once, one per helper.
+        // The first that passes is taken, and only the trials up to it count.
         double alpha = 1.0;
         bool accepted = false;
         int ls_used = 0;
         for (int ls = 0; ls < 20 && !accepted;) {
-            const int K = min(Kb, ls == 0 ? kprev : 20 - ls);
-            objective_value_batch(d, w, sc, K, w.x, w.dx, alpha, red, lsv, lsv + kLsBatch, nl.terms);
+            const int K = helped ? 20 - ls : min(Kb, ls == 0 ? kprev : 20 - ls);
+            if (helped) helped = helpers_batch(hb, ++bseq, K, alpha, lsv, lsv + kLsBatch, &ok_sh);
+            if (!helped) objective_value_batch(d, w, sc, K, w.x, w.dx, alpha, red, lsv, lsv + kLsBatch, nl.terms);
             int k = 0;
             for (; k < K; ++k) {
                 const double fnk = lsv[k], fcnk = lsv[kLsBatch + k];
@@ -1749,6 +1916,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) X[i] = X[i] + alpha * DX[i];
         __syncthreads();
     }
+    if (hb && threadIdx.x == 0) st_rel(&hb->quit, 1);
     __syncthreads();
     // ALM bookkeeping (IPCSolver.cpp:1084-1110): violation, dual residual, hardening,
     // lambda += mu C with the hardened mu.
@@ -1781,6 +1949,10 @@ void launch_newton(const Dev& d, const Work& w, hipStream_t st) {
     timing_begin("newton", st);
     hipLaunchKernelGGL(k_newton, dim3(d.S), dim3(kBlk), 0, st, d, w);
     timing_end("newton", st);
+}
+
+void launch_newton_helpers(const Dev& d, const Work& w, int n_launch, hipStream_t st) {
+    hipLaunchKernelGGL(k_newton_helper, dim3(d.S * kLsBatch), dim3(kBlk), 0, st, d, w, n_launch);
 }
 
 // ------------------------------------------------------------------ contacts setup
@@ -1957,6 +2129,7 @@ __global__ void k_step_begin(Dev d, Work w, const double* dt) {
     sp.pcg_iters = 0;
     sp.comp_max = 0;
     w.flag[s] = 0;
+    if (w.newton_helpers) w.hbox[s].dead = 0;  // this step's helpers are alive until they time out
 }
 
 void launch_step_begin(const Dev& d, const Work& w, const double* dt, hipStream_t st) {

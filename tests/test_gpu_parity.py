@@ -354,6 +354,28 @@ def test_broad_phase_tree_in_hbm_matches_tree_in_lds(engine_factory, name):
     assert np.array_equal(ref[1], b.stats())
 
 
+@pytest.mark.parametrize("name", ["tie", "c3x8"])
+def test_line_search_helpers_match_in_group_trials(engine_factory, name):
+    """In small batches (1 and 8 scenes here) each scene's line-search trials are
+    evaluated by 20 helper work-groups beside the Newton kernel; with
+    NDIPC_NEWTON_HELPERS=0 at upload the scene's own work-group evaluates them. The
+    trials' sums are the same either way: same steps, same bits, same counters."""
+    sc = perturbed_c2_scene() if name == "tie" else scenes.pack(scenes.config3(n_scenes=8))
+    a = engine_factory(sc)
+    a.step(6)
+    ref = a.get_state(), a.stats(), a.solver_counters()
+    os.environ["NDIPC_NEWTON_HELPERS"] = "0"
+    try:
+        b = engine_factory(sc)
+    finally:
+        del os.environ["NDIPC_NEWTON_HELPERS"]
+    b.step(6)
+    for x, y in zip(ref[0], b.get_state()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(ref[1], b.stats())
+    assert np.array_equal(ref[2], b.solver_counters())
+
+
 def test_autotune_restores_state_and_keeps_results(engine_factory):
     """ndipc_autotune times launch shapes from the current state and restores it; the
     chosen shapes change speed only: the next steps give the untuned engine's bits."""
