@@ -516,7 +516,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) ncu = 0;
         const char* env = std::getenv("NDIPC_NEWTON_HELPERS");
         const bool off = env && env[0] == '0';
-        w.newton_helpers = (!off && (long long)S * (1 + nd::kLsBatch) <= ncu) ? 1 : 0;
+        w.newton_helpers = (!off && (long long)S * (1 + nd::kHelpers) <= ncu) ? 1 : 0;
         w.hbox = dalloc<nd::HelperBox>(ctx, S);
     }
     w.pair_cnt = dalloc<int>(ctx, S);

@@ -29,18 +29,23 @@ constexpr int kDenseRows = 6 * kDenseBodies;
 constexpr int kLsBatch = 20;          // line-search trials evaluated together, at most
 constexpr int kLsBatchLarge = 8;      // ... for scenes whose 20 trials need several LDS rounds
 
-// Mailbox between a scene's Newton work-group and its kLsBatch line-search helper
-// work-groups (small batches only; solver.hip, k_newton_helper). Zero-initialised.
+// Line-search helper work-groups per scene (small batches only; solver.hip,
+// k_newton_helper): one per trial of a batch, and one for the line search's slopes.
+constexpr int kHelpers = kLsBatch + 1;
+
+// Mailbox between a scene's Newton work-group and its helpers. Zero-initialised.
 struct HelperBox {
     int epoch;           // Newton launches opened so far
     int bseq;            // batches posted in the current launch (0: none)
     int K;               // trials in the posted batch
+    int want_slopes;     // 1: the posted batch also asks for the slopes
     int quit;            // 1: the current launch's helpers leave
     int dead;            // 1: a helper left on its time bound
-    int hep[kLsBatch];   // per helper: the epoch it last served
-    int ack[kLsBatch];   // per helper: the batch it last evaluated
+    int hep[kHelpers];   // per helper: the epoch it last served
+    int ack[kHelpers];   // per helper: the batch it last evaluated
     double alpha;        // step length of trial 0 of the posted batch
     double val[kLsBatch], cval[kLsBatch];
+    double slope[2];     // g.dx and its contact part cg.dx
 };
 
 struct SceneParams {
@@ -123,8 +128,8 @@ struct Work {
     double* node_box;        // [N*6]
     int* nleaves;            // [S]
     int* bvh_age;            // [S] LBVH builds and refits since the last full build
-    // Newton line-search helpers: 1 when every scene gets kLsBatch helper work-groups
-    // beside its Newton work-group (S * (1 + kLsBatch) fit the CUs at one per CU)
+    // Newton line-search helpers: 1 when every scene gets kHelpers helper work-groups
+    // beside its Newton work-group (S * (1 + kHelpers) fit the CUs at one per CU)
     int newton_helpers;
     HelperBox* hbox;         // [S]
     // body pairs

@@ -1643,11 +1643,12 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
 }
 
 // ------------------------------------------------------------------ line-search helpers
-// In a small batch (S (1 + kLsBatch) work-groups fit the CUs at one per CU) each scene
-// lends its line search to kLsBatch helper work-groups running beside it: the scene's
+// In a small batch (S (1 + kHelpers) work-groups fit the CUs at one per CU) each scene
+// lends its line search to kHelpers helper work-groups running beside it: the scene's
 // work-group posts a batch of K trials (K, the first step length) in the scene's
-// HelperBox and waits, and helper h evaluates trial h with objective_value_batch at
-// K = 1, whose sums are those of the scene's own batch evaluation, so the values are
+// HelperBox and waits; helper h < kLsBatch evaluates trial h with objective_value_batch
+// at K = 1, whose sums are those of the scene's own batch evaluation, and helper
+// kLsBatch, when asked, sums the slopes g.dx, cg.dx with serial_dot2, so the values are
 // the same bits whichever work-group computes them. Every wait is bounded in time: a
 // helper that hears nothing for kHelperIdleTicks marks the box dead and leaves, and a
 // scene work-group that waits longer than kHelperWaitTicks for an acknowledgement
@@ -1676,24 +1677,27 @@ __device__ __forceinline__ void st_rlx(double* p, double v) {
 __device__ void helpers_open(HelperBox* hb) {
     st_rlx(&hb->bseq, 0);
     st_rlx(&hb->quit, 0);
-    for (int h = 0; h < kLsBatch; ++h) st_rlx(&hb->ack[h], 0);
+    for (int h = 0; h < kHelpers; ++h) st_rlx(&hb->ack[h], 0);
     __hip_atomic_fetch_add(&hb->epoch, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Scene work-group, all lanes: trials alpha, alpha / 2, ... (K of them) by the helpers,
-// values to vals / cvals (LDS); bseq: this batch's number (1, 2, ... per launch).
-// Returns false, with the helpers sent away, when they did not answer in time.
+// values to vals / cvals (LDS), and with slopes (LDS, 2) also g.dx and cg.dx; bseq:
+// this batch's number (1, 2, ... per launch). Returns false, with the helpers sent
+// away, when they did not answer in time.
 __device__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, double* vals, double* cvals,
-                              int* flag) {
-    __threadfence();  // the work-group's stores of x and dx, visible to the helpers
+                              double* slopes, int* flag) {
+    __threadfence();  // the work-group's stores of x, g, cg and dx, visible to the helpers
     __syncthreads();
     if (threadIdx.x == 0) {
         st_rlx(&hb->K, K);
         st_rlx(&hb->alpha, alpha);
+        st_rlx(&hb->want_slopes, slopes ? 1 : 0);
         st_rel(&hb->bseq, bseq);
         const unsigned long long t0 = wall_clock64();
         int ok = 1;
-        for (int h = 0; h < K && ok; ++h) {
+        for (int h = 0; h < kHelpers && ok; ++h) {
+            if (h >= K && (h < kLsBatch || !slopes)) continue;
             while (ld_rlx(&hb->ack[h]) != bseq) {
                 if (ld_rlx(&hb->dead) || wall_clock64() - t0 > kHelperWaitTicks) {
                     ok = 0;
@@ -1710,6 +1714,10 @@ __device__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, doub
                 vals[k] = ld_rlx(&hb->val[k]);
                 cvals[k] = ld_rlx(&hb->cval[k]);
             }
+            if (slopes) {
+                slopes[0] = ld_rlx(&hb->slope[0]);
+                slopes[1] = ld_rlx(&hb->slope[1]);
+            }
         } else {
             st_rel(&hb->quit, 1);
         }
@@ -1719,7 +1727,8 @@ __device__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, doub
     return *flag != 0;
 }
 
-// Helper work-group blockIdx.x = s kLsBatch + h serves trial h of each batch of scene s
+// Helper work-group blockIdx.x = s kHelpers + h serves trial h (h < kLsBatch) or the
+// slopes (h = kLsBatch) of each batch of scene s
 // through the n_launch Newton launches of a step (each opens an epoch and ends with
 // quit), then leaves; it also leaves, marking the box dead for the rest of the step, when
 // it hears nothing for kHelperIdleTicks. It runs beside the step's kernels, on a second
@@ -1729,9 +1738,9 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
     __shared__ alignas(16) double T[kTermLds];
     __shared__ double lv[2];
     __shared__ int cmd;
-    __shared__ int K_sh;
+    __shared__ int K_sh, slopes_sh;
     __shared__ double alpha_sh;
-    const int s = blockIdx.x / kLsBatch, h = blockIdx.x % kLsBatch;
+    const int s = blockIdx.x / kHelpers, h = blockIdx.x % kHelpers;
     HelperBox* hb = w.hbox + s;
     for (int launch = 0; launch < n_launch; ++launch) {
         if (threadIdx.x == 0) {  // wait for the next Newton launch to open
@@ -1767,6 +1776,7 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                         c = b;
                         K_sh = ld_rlx(&hb->K);
+                        slopes_sh = ld_rlx(&hb->want_slopes);
                         alpha_sh = ld_rlx(&hb->alpha);
                         break;
                     }
@@ -1784,13 +1794,21 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
             if (c == -2) return;
             if (c < 0) break;  // this launch is closed
             served = c;        // lane 0's acquire, then the barrier: x, dx of the batch are visible
-            if (h < K_sh) {
+            if (h < kLsBatch && h < K_sh) {
                 double a = alpha_sh;  // alpha / 2^h, halved as the scene's batch halves it
                 for (int q = 0; q < h; ++q) a *= 0.5;
                 objective_value_batch(d, w, sc, 1, w.x, w.dx, a, red, lv, lv + 1, T);
                 if (threadIdx.x == 0) {
                     st_rlx(&hb->val[h], lv[0]);
                     st_rlx(&hb->cval[h], lv[1]);
+                    st_rel(&hb->ack[h], c);
+                }
+            } else if (h == kLsBatch && slopes_sh) {
+                double dsc, cdsc;
+                serial_dot2(w.g + 6 * sc.b0, w.cg + 6 * sc.b0, w.dx + 6 * sc.b0, sc.nd, red, &dsc, &cdsc, T);
+                if (threadIdx.x == 0) {
+                    st_rlx(&hb->slope[0], dsc);
+                    st_rlx(&hb->slope[1], cdsc);
                     st_rel(&hb->ack[h], c);
                 }
             }
@@ -1806,6 +1824,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     __shared__ int ok_sh;
     __shared__ NewtonLds nl;
     __shared__ double lsv[2 * kLsBatch];  // trial values, then contact values
+    __shared__ double slp[2];             // slopes from a helper
     CompLds& comp = nl.comp;
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
@@ -1874,9 +1893,10 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         // line search (LineSearch.cpp:6-58), alpha0 = min(1, maxStep) = 1
         double fx = fx_next, fc = fc_next;
         if (!have_f) objective_value(d, w, sc, Xg, red, &fx, &fc, nl.terms);
-        double descent, cdesc;
-        serial_dot2(G, CG, DX, sc.nd, red, &descent, &cdesc, nl.terms);
-        if (descent > 0) break;
+        // slopes g.dx and cg.dx: by a helper beside the first batch, or here before it;
+        // an uphill direction (g.dx > 0) ends the Newton loop before any trial counts
+        double descent = 0.0, cdesc = 0.0;
+        bool have_desc = false;
         // The trials alpha = 1, 1/2, 1/4, ... (at most 20) are evaluated in batches: the
         // first as many as the previous line search needed, the next ones all that
         // remain, each at most Kb (20 where their terms fit one LDS round, 8
@@ -1887,7 +1907,19 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         int ls_used = 0;
         for (int ls = 0; ls < 20 && !accepted;) {
             const int K = helped ? 20 - ls : min(Kb, ls == 0 ? kprev : 20 - ls);
-            if (helped) helped = helpers_batch(hb, ++bseq, K, alpha, lsv, lsv + kLsBatch, &ok_sh);
+            if (helped) {
+                helped = helpers_batch(hb, ++bseq, K, alpha, lsv, lsv + kLsBatch, have_desc ? nullptr : slp, &ok_sh);
+                if (helped && !have_desc) {
+                    descent = slp[0];
+                    cdesc = slp[1];
+                    have_desc = true;
+                }
+            }
+            if (!have_desc) {
+                serial_dot2(G, CG, DX, sc.nd, red, &descent, &cdesc, nl.terms);
+                have_desc = true;
+            }
+            if (descent > 0) break;  // uphill: not accepted, the Newton loop ends
             if (!helped) objective_value_batch(d, w, sc, K, w.x, w.dx, alpha, red, lsv, lsv + kLsBatch, nl.terms);
             int k = 0;
             for (; k < K; ++k) {
@@ -1952,7 +1984,7 @@ void launch_newton(const Dev& d, const Work& w, hipStream_t st) {
 }
 
 void launch_newton_helpers(const Dev& d, const Work& w, int n_launch, hipStream_t st) {
-    hipLaunchKernelGGL(k_newton_helper, dim3(d.S * kLsBatch), dim3(kBlk), 0, st, d, w, n_launch);
+    hipLaunchKernelGGL(k_newton_helper, dim3(d.S * kHelpers), dim3(kBlk), 0, st, d, w, n_launch);
 }
 
 // ------------------------------------------------------------------ contacts setup
