@@ -1687,6 +1687,7 @@ __device__ void helpers_open(HelperBox* hb) {
 // away, when they did not answer in time.
 __device__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, double* vals, double* cvals,
                               double* slopes, int* flag) {
+    static_assert(kHelpers <= 64, "one lane of the first wavefront per helper");
     __threadfence();  // the work-group's stores of x, g, cg and dx, visible to the helpers
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1694,34 +1695,35 @@ __device__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, doub
         st_rlx(&hb->alpha, alpha);
         st_rlx(&hb->want_slopes, slopes ? 1 : 0);
         st_rel(&hb->bseq, bseq);
+    }
+    if (threadIdx.x < 64) {  // lane h waits for helper h's acknowledgement, if one is due
+        const int h = threadIdx.x;
+        const bool due = h < K || (h == kLsBatch && slopes);
         const unsigned long long t0 = wall_clock64();
+        bool mine = !due || ld_rlx(&hb->ack[h]) == bseq;
         int ok = 1;
-        for (int h = 0; h < kHelpers && ok; ++h) {
-            if (h >= K && (h < kLsBatch || !slopes)) continue;
-            while (ld_rlx(&hb->ack[h]) != bseq) {
-                if (ld_rlx(&hb->dead) || wall_clock64() - t0 > kHelperWaitTicks) {
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
+        while (!__all(mine)) {
+            if (ld_rlx(&hb->dead) || wall_clock64() - t0 > kHelperWaitTicks) {
+                ok = 0;
+                break;
             }
+            __builtin_amdgcn_s_sleep(1);
+            if (!mine) mine = ld_rlx(&hb->ack[h]) == bseq;
         }
+        ok = __all(ok);
         // one acquire for all the acknowledgements (an acquire invalidates caches, so
         // the polls above are relaxed)
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (ok) {
-            for (int k = 0; k < K; ++k) {
-                vals[k] = ld_rlx(&hb->val[k]);
-                cvals[k] = ld_rlx(&hb->cval[k]);
+            if (h < K) {
+                vals[h] = ld_rlx(&hb->val[h]);
+                cvals[h] = ld_rlx(&hb->cval[h]);
             }
-            if (slopes) {
-                slopes[0] = ld_rlx(&hb->slope[0]);
-                slopes[1] = ld_rlx(&hb->slope[1]);
-            }
-        } else {
+            if (slopes && h < 2) slopes[h] = ld_rlx(&hb->slope[h]);
+        } else if (h == 0) {
             st_rel(&hb->quit, 1);
         }
-        *flag = ok;
+        if (h == 0) *flag = ok;
     }
     __syncthreads();
     return *flag != 0;
