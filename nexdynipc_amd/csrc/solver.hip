@@ -604,8 +604,8 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
     // Each trial owns TS doubles of LDS per round: 3 WBb body terms (two inertia terms
     // and one gravity term per body) and 3 WBe element terms (joint, barrier, friction),
     // the two widths chosen per scene so that the fewest rounds cover the bodies and
-    // the elements (widths even: every window starts 16-byte aligned; at most one
-    // element window per work-group of lanes).
+    // the elements (widths even: every window starts 16-byte aligned; with K > 1 at
+    // most one element per lane, a single trial takes its window in slots of 256).
     const int TS = (kTermLds / K) & ~1;
     const double* XH = w.xhat;
     const int nj = sc.j1 - sc.j0;
@@ -616,7 +616,8 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
         WBe = (nel + rounds - 1) / rounds;
         WBb += WBb & 1;
         WBe += WBe & 1;
-        if (3 * (WBb + WBe) <= TS && WBe <= kBlk) break;  // an element per lane at most
+        // an element per lane at most when trials share an element's lanes (K > 1)
+        if (3 * (WBb + WBe) <= TS && (WBe <= kBlk || K == 1)) break;
     }
     const int OI = 0, OG = 2 * WBb, OJ = 3 * WBb, OB = 3 * WBb + WBe, OF = 3 * WBb + 2 * WBe;
     double acc = 0.0;  // lane L < 5K: accumulator L % 5 of trial L / 5
@@ -663,9 +664,9 @@ __device__ void objective_value_batch(const Dev& d, const Work& w, const ScnCtx&
         // trial points of its bodies formed in registers (alpha_k by k halvings, as the
         // line search forms it)
         const int ne = min(WBe, max(0, nel - e_lo));
-        if (ne > 0) {
-            const int G = min(K, max(1, (int)blockDim.x / ne));
-            const int tt = threadIdx.x % ne, g = threadIdx.x / ne;
+        const int G = ne > 0 ? min(K, max(1, (int)blockDim.x / ne)) : 0;
+        for (int u = threadIdx.x; u < ne * G; u += blockDim.x) {  // slot u: element u % ne, lane g
+            const int tt = u % ne, g = u / ne;
             const int li = e_lo + tt;
             if (g < G && li < nj) {
                 const int j = sc.j0 + li;
@@ -1418,14 +1419,15 @@ __device__ __noinline__ bool dense_ldlt_panels(double* M, int m) {
     return true;
 }
 
-// The gradient, the diagonal blocks (assemble_diag) and the singleton solves of
-// component_solve in one pass over the bodies, with the gradient and the 6x6 block in
-// registers: the same values as separate passes (a singleton's right-hand side is -g). The components of this
+// The gradient (G, and NG = -G), the diagonal blocks (assemble_diag) and the singleton
+// solves of component_solve in one pass over the bodies, with the gradient and the 6x6
+// block in registers: the same values as separate passes (a singleton's right-hand side
+// is -g). The components of this
 // iteration must be built (build_components); *ok_sh is cleared if a singleton's block
 // has a zero or non-finite pivot.
 __device__ double grad_diag_singletons(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* G,
-                                       double* cgout, const double* b2arr, const CompLds& L, double* xo, double* red,
-                                       int* ok_sh) {
+                                       double* NG, double* cgout, const double* b2arr, const CompLds& L, double* xo,
+                                       double* red, int* ok_sh) {
     double gmax = 0.0;
     const int* off = w.csr_off + sc.b0 + sc.s;
     const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
@@ -1434,6 +1436,7 @@ __device__ double grad_diag_singletons(const Dev& d, const Work& w, const ScnCtx
         body_grad_diag(d, w, sc, X, b2arr, li, off, ent, g, cgv, D);
         for (int k = 0; k < 6; ++k) {
             G[6 * li + k] = g[k];
+            NG[6 * li + k] = -g[k];
             cgout[6 * li + k] = cgv[k];
             gmax = dmax(gmax, fabs(g[k]));
         }
@@ -1867,11 +1870,9 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         build_components(d, w, sc, comp);
         if (threadIdx.x == 0) ok_sh = 1;
         __syncthreads();
-        const double ginf = grad_diag_singletons(d, w, sc, Xg, G, CG, b2arr, comp, DX, red, &ok_sh);
+        const double ginf = grad_diag_singletons(d, w, sc, Xg, G, NG, CG, b2arr, comp, DX, red, &ok_sh);
         __syncthreads();
         if (ginf < sp.newton_tol) { conv = 1; break; }
-        for (int i = threadIdx.x; i < sc.nd; i += blockDim.x) NG[i] = -G[i];
-        __syncthreads();
         const int cs = comp.maxsize > kDenseBodies ? 2
                        : !ok_sh                   ? 1
                                                   : component_solve<8>(d, w, sc, b2arr, NG, DX, comp, &ok_sh, true);
