@@ -518,6 +518,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
         const bool off = env && env[0] == '0';
         w.newton_helpers = (!off && (long long)S * (1 + nd::kHelpers) <= ncu) ? 1 : 0;
         w.hbox = dalloc<nd::HelperBox>(ctx, S);
+        w.hsingle = dalloc<int>(ctx, N);
     }
     w.pair_cnt = dalloc<int>(ctx, S);
     w.pair_off = dalloc<int>(ctx, S + 1);

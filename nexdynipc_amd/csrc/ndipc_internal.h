@@ -38,6 +38,7 @@ struct HelperBox {
     int epoch;           // Newton launches opened so far
     int bseq;            // batches posted in the current launch (0: none)
     int K;               // trials in the posted batch
+    int job;             // 0: line-search trials (and slopes); 1: gradient and diagonal blocks
     int want_slopes;     // 1: the posted batch also asks for the slopes
     int quit;            // 1: the current launch's helpers leave
     int dead;            // 1: a helper left on its time bound
@@ -46,6 +47,8 @@ struct HelperBox {
     double alpha;        // step length of trial 0 of the posted batch
     double val[kLsBatch], cval[kLsBatch];
     double slope[2];     // g.dx and its contact part cg.dx
+    double gm[kHelpers]; // job 1, per helper: max |g| over its bodies
+    int okf[kHelpers];   // job 1, per helper: 0 if a singleton's 6x6 factor failed
 };
 
 struct SceneParams {
@@ -132,6 +135,7 @@ struct Work {
     // beside its Newton work-group (S * (1 + kHelpers) fit the CUs at one per CU)
     int newton_helpers;
     HelperBox* hbox;         // [S]
+    int* hsingle;            // [N] with helpers: 1 if the body is alone in its component
     // body pairs
     long long* pair_cap_off; // [S+1] capacity offsets (n(n-1)/2 per scene)
     int* pair_cnt;           // [S]

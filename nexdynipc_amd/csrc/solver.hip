@@ -1425,30 +1425,39 @@ __device__ __noinline__ bool dense_ldlt_panels(double* M, int m) {
 // is -g). The components of this
 // iteration must be built (build_components); *ok_sh is cleared if a singleton's block
 // has a zero or non-finite pivot.
+// Body li of grad_diag_singletons: G, NG, cg and the diagonal block, and with single
+// the singleton solve into xo. Returns false if that 6x6 factor fails; gmax takes the
+// body's max |g|.
+__device__ __forceinline__ bool grad_diag_body(const Dev& d, const Work& w, const ScnCtx& sc, const double* X,
+                                               double* G, double* NG, double* cgout, const double* b2arr, int li,
+                                               const int* off, const int* ent, bool single, double* xo,
+                                               double& gmax) {
+    double g[6], cgv[6], D[36];
+    body_grad_diag(d, w, sc, X, b2arr, li, off, ent, g, cgv, D);
+    for (int k = 0; k < 6; ++k) {
+        G[6 * li + k] = g[k];
+        NG[6 * li + k] = -g[k];
+        cgout[6 * li + k] = cgv[k];
+        gmax = dmax(gmax, fabs(g[k]));
+    }
+    double* Dout = w.D + 36 * (sc.b0 + li);
+#pragma unroll
+    for (int k = 0; k < 36; ++k) Dout[k] = D[k];
+    if (!single) return true;
+    double nb[6];
+    for (int k = 0; k < 6; ++k) nb[k] = -g[k];
+    return ldlt6_solve(D, nb, xo + 6 * li);
+}
+
 __device__ double grad_diag_singletons(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* G,
                                        double* NG, double* cgout, const double* b2arr, const CompLds& L, double* xo,
                                        double* red, int* ok_sh) {
     double gmax = 0.0;
     const int* off = w.csr_off + sc.b0 + sc.s;
     const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
-    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
-        double g[6], cgv[6], D[36];
-        body_grad_diag(d, w, sc, X, b2arr, li, off, ent, g, cgv, D);
-        for (int k = 0; k < 6; ++k) {
-            G[6 * li + k] = g[k];
-            NG[6 * li + k] = -g[k];
-            cgout[6 * li + k] = cgv[k];
-            gmax = dmax(gmax, fabs(g[k]));
-        }
-        double* Dout = w.D + 36 * (sc.b0 + li);
-#pragma unroll
-        for (int k = 0; k < 36; ++k) Dout[k] = D[k];
-        if (L.cnt[L.lab[li]] <= 1) {
-            double nb[6];
-            for (int k = 0; k < 6; ++k) nb[k] = -g[k];
-            if (!ldlt6_solve(D, nb, xo + 6 * li)) atomicAnd(ok_sh, 0);
-        }
-    }
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x)
+        if (!grad_diag_body(d, w, sc, X, G, NG, cgout, b2arr, li, off, ent, L.cnt[L.lab[li]] <= 1, xo, gmax))
+            atomicAnd(ok_sh, 0);
     return wg_reduce(gmax, red, OpMax());
 }
 
@@ -1684,16 +1693,76 @@ __device__ void helpers_open(HelperBox* hb) {
     __hip_atomic_fetch_add(&hb->epoch, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Scene work-group, first wavefront: waits until every helper h with due(h) on its lane
+// has acknowledged batch bseq, or the helpers died, or kHelperWaitTicks passed; then one
+// acquire. Returns 1 (all answered) or 0, the same on every lane.
+__device__ __forceinline__ int helpers_wait(HelperBox* hb, int bseq, bool due) {
+    const int h = threadIdx.x;
+    const unsigned long long t0 = wall_clock64();
+    bool mine = !due || ld_rlx(&hb->ack[h]) == bseq;
+    int ok = 1;
+    while (!__all(mine)) {
+        if (ld_rlx(&hb->dead) || wall_clock64() - t0 > kHelperWaitTicks) {
+            ok = 0;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (!mine) mine = ld_rlx(&hb->ack[h]) == bseq;
+    }
+    ok = __all(ok);
+    // one acquire for all the acknowledgements (an acquire invalidates caches, so the
+    // polls above are relaxed)
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return ok;
+}
+
+// Scene work-group, all lanes: the gradient, NG = -G, the contact part of the gradient,
+// the diagonal blocks and the singleton solves of every body by the helpers (body li on
+// helper li % kHelpers), w.hsingle holding which bodies are singletons. *ginf: max |g|;
+// *ok_sh cleared if a singleton's factor failed. Returns false, with the helpers sent
+// away, when they did not answer in time.
+__device__ __noinline__ bool helpers_grad(HelperBox* hb, int bseq, double* ginf, int* ok_sh, int* flag) {
+    __threadfence();  // the work-group's element data and singleton flags, visible to the helpers
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st_rlx(&hb->job, 1);
+        st_rel(&hb->bseq, bseq);
+    }
+    if (threadIdx.x < 64) {
+        const int h = threadIdx.x;
+        const bool due = h < kHelpers;
+        const int ok = helpers_wait(hb, bseq, due);
+        if (ok) {
+            double gm = due ? ld_rlx(&hb->gm[h]) : 0.0;
+            int okh = due ? ld_rlx(&hb->okf[h]) : 1;
+            for (int o = 32; o > 0; o >>= 1) {
+                gm = dmax(gm, __shfl_xor(gm, o));
+                okh &= __shfl_xor(okh, o);
+            }
+            if (h == 0) {
+                *ginf = gm;
+                if (!okh) *ok_sh = 0;
+            }
+        } else if (h == 0) {
+            st_rel(&hb->quit, 1);
+        }
+        if (h == 0) *flag = ok;
+    }
+    __syncthreads();
+    return *flag != 0;
+}
+
 // Scene work-group, all lanes: trials alpha, alpha / 2, ... (K of them) by the helpers,
 // values to vals / cvals (LDS), and with slopes (LDS, 2) also g.dx and cg.dx; bseq:
 // this batch's number (1, 2, ... per launch). Returns false, with the helpers sent
 // away, when they did not answer in time.
-__device__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, double* vals, double* cvals,
+__device__ __noinline__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, double* vals, double* cvals,
                               double* slopes, int* flag) {
     static_assert(kHelpers <= 64, "one lane of the first wavefront per helper");
     __threadfence();  // the work-group's stores of x, g, cg and dx, visible to the helpers
     __syncthreads();
     if (threadIdx.x == 0) {
+        st_rlx(&hb->job, 0);
         st_rlx(&hb->K, K);
         st_rlx(&hb->alpha, alpha);
         st_rlx(&hb->want_slopes, slopes ? 1 : 0);
@@ -1701,22 +1770,7 @@ __device__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, doub
     }
     if (threadIdx.x < 64) {  // lane h waits for helper h's acknowledgement, if one is due
         const int h = threadIdx.x;
-        const bool due = h < K || (h == kLsBatch && slopes);
-        const unsigned long long t0 = wall_clock64();
-        bool mine = !due || ld_rlx(&hb->ack[h]) == bseq;
-        int ok = 1;
-        while (!__all(mine)) {
-            if (ld_rlx(&hb->dead) || wall_clock64() - t0 > kHelperWaitTicks) {
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            if (!mine) mine = ld_rlx(&hb->ack[h]) == bseq;
-        }
-        ok = __all(ok);
-        // one acquire for all the acknowledgements (an acquire invalidates caches, so
-        // the polls above are relaxed)
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int ok = helpers_wait(hb, bseq, h < K || (h == kLsBatch && slopes));
         if (ok) {
             if (h < K) {
                 vals[h] = ld_rlx(&hb->val[h]);
@@ -1743,7 +1797,7 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
     __shared__ alignas(16) double T[kTermLds];
     __shared__ double lv[2];
     __shared__ int cmd;
-    __shared__ int K_sh, slopes_sh;
+    __shared__ int K_sh, slopes_sh, job_sh, okh_sh;
     __shared__ double alpha_sh;
     const int s = blockIdx.x / kHelpers, h = blockIdx.x % kHelpers;
     HelperBox* hb = w.hbox + s;
@@ -1780,6 +1834,7 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
                     if (b != served) {
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                         c = b;
+                        job_sh = ld_rlx(&hb->job);
                         K_sh = ld_rlx(&hb->K);
                         slopes_sh = ld_rlx(&hb->want_slopes);
                         alpha_sh = ld_rlx(&hb->alpha);
@@ -1799,7 +1854,24 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
             if (c == -2) return;
             if (c < 0) break;  // this launch is closed
             served = c;        // lane 0's acquire, then the barrier: x, dx of the batch are visible
-            if (h < kLsBatch && h < K_sh) {
+            if (job_sh == 1) {  // gradient, diagonal blocks, singleton solves: bodies h, h + kHelpers, ...
+                const int* off = w.csr_off + sc.b0 + sc.s;
+                const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+                const double* b2arr = w.sort_key + (long long)s * kSceneScratch;
+                if (threadIdx.x == 0) okh_sh = 1;
+                __syncthreads();
+                double gmax = 0.0;
+                for (int li = h + kHelpers * (int)threadIdx.x; li < sc.n; li += kHelpers * (int)blockDim.x)
+                    if (!grad_diag_body(d, w, sc, w.x, w.g + 6 * sc.b0, w.tmp + 6 * sc.b0, w.cg + 6 * sc.b0, b2arr, li,
+                                        off, ent, w.hsingle[sc.b0 + li] != 0, w.dx + 6 * sc.b0, gmax))
+                        atomicAnd(&okh_sh, 0);
+                gmax = wg_reduce(gmax, red, OpMax());
+                if (threadIdx.x == 0) {
+                    st_rlx(&hb->gm[h], gmax);
+                    st_rlx(&hb->okf[h], okh_sh);
+                    st_rel(&hb->ack[h], c);
+                }
+            } else if (h < kLsBatch && h < K_sh) {
                 double a = alpha_sh;  // alpha / 2^h, halved as the scene's batch halves it
                 for (int q = 0; q < h; ++q) a *= 0.5;
                 objective_value_batch(d, w, sc, 1, w.x, w.dx, a, red, lv, lv + 1, T);
@@ -1830,6 +1902,8 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     __shared__ NewtonLds nl;
     __shared__ double lsv[2 * kLsBatch];  // trial values, then contact values
     __shared__ double slp[2];             // slopes from a helper
+    __shared__ double gsh;                // max |g| from the helpers
+    __shared__ int hflag;                 // the helpers answered
     CompLds& comp = nl.comp;
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
@@ -1870,7 +1944,13 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         build_components(d, w, sc, comp);
         if (threadIdx.x == 0) ok_sh = 1;
         __syncthreads();
-        const double ginf = grad_diag_singletons(d, w, sc, Xg, G, NG, CG, b2arr, comp, DX, red, &ok_sh);
+        if (helped) {  // by the helpers, one body per lane
+            for (int li = threadIdx.x; li < sc.n; li += blockDim.x)
+                w.hsingle[sc.b0 + li] = comp.cnt[comp.lab[li]] <= 1;
+            helped = helpers_grad(hb, ++bseq, &gsh, &ok_sh, &hflag);
+        }
+        const double ginf =
+            helped ? gsh : grad_diag_singletons(d, w, sc, Xg, G, NG, CG, b2arr, comp, DX, red, &ok_sh);
         __syncthreads();
         if (ginf < sp.newton_tol) { conv = 1; break; }
         const int cs = comp.maxsize > kDenseBodies ? 2
@@ -1911,7 +1991,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         for (int ls = 0; ls < 20 && !accepted;) {
             const int K = helped ? 20 - ls : min(Kb, ls == 0 ? kprev : 20 - ls);
             if (helped) {
-                helped = helpers_batch(hb, ++bseq, K, alpha, lsv, lsv + kLsBatch, have_desc ? nullptr : slp, &ok_sh);
+                helped = helpers_batch(hb, ++bseq, K, alpha, lsv, lsv + kLsBatch, have_desc ? nullptr : slp, &hflag);
                 if (helped && !have_desc) {
                     descent = slp[0];
                     cdesc = slp[1];
