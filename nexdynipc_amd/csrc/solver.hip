@@ -1662,12 +1662,19 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
 // at K = 1, whose sums are those of the scene's own batch evaluation, and helper
 // kLsBatch, when asked, sums the slopes g.dx, cg.dx with serial_dot2, so the values are
 // the same bits whichever work-group computes them. Every wait is bounded in time: a
-// helper that hears nothing for kHelperIdleTicks marks the box dead and leaves, and a
-// scene work-group that waits longer than kHelperWaitTicks for an acknowledgement
-// evaluates the batch itself and sends its helpers away; results never depend on the
-// helpers being there. Ticks: wall_clock64, 100 MHz on gfx950.
+// helper that hears nothing in time marks the box dead and leaves, and a scene
+// work-group that waits too long for an acknowledgement does the work itself and sends
+// its helpers away; results never depend on the helpers being there. Ticks:
+// wall_clock64, 100 MHz on gfx950.
+// Within a Newton launch a helper waits up to kHelperIdleTicks for the next batch (the
+// scene's work-group computes in between); for a launch to open, kHelperEpochTicks,
+// since the helpers and the Newton launches run side by side. A scene work-group waits
+// kHelperWaitTicks for an answer. When dispatches are serialised (a profiler collecting
+// counters, say) the helpers never run beside the Newton kernel: whichever side times
+// out first marks the box dead, and the step's remaining launches run in-group at once.
 constexpr unsigned long long kHelperIdleTicks = 20000000;  // 200 ms
-constexpr unsigned long long kHelperWaitTicks = 2000000;  // 20 ms
+constexpr unsigned long long kHelperEpochTicks = 500000;   // 5 ms
+constexpr unsigned long long kHelperWaitTicks = 500000;    // 5 ms
 
 __device__ __forceinline__ void st_rel(int* p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -1743,7 +1750,8 @@ __device__ __noinline__ bool helpers_grad(HelperBox* hb, int bseq, double* ginf,
                 *ginf = gm;
                 if (!okh) *ok_sh = 0;
             }
-        } else if (h == 0) {
+        } else if (h == 0) {  // no answer: send this launch's helpers away, none for the step
+            st_rlx(&hb->dead, 1);
             st_rel(&hb->quit, 1);
         }
         if (h == 0) *flag = ok;
@@ -1777,7 +1785,8 @@ __device__ __noinline__ bool helpers_batch(HelperBox* hb, int bseq, int K, doubl
                 cvals[h] = ld_rlx(&hb->cval[h]);
             }
             if (slopes && h < 2) slopes[h] = ld_rlx(&hb->slope[h]);
-        } else if (h == 0) {
+        } else if (h == 0) {  // no answer: send this launch's helpers away, none for the step
+            st_rlx(&hb->dead, 1);
             st_rel(&hb->quit, 1);
         }
         if (h == 0) *flag = ok;
@@ -1806,7 +1815,7 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
             const int last = ld_rlx(&hb->hep[h]);
             const unsigned long long t0 = wall_clock64();
             int ep = last;
-            while ((ep = ld_rlx(&hb->epoch)) == last && wall_clock64() - t0 <= kHelperIdleTicks)
+            while ((ep = ld_rlx(&hb->epoch)) == last && wall_clock64() - t0 <= kHelperEpochTicks)
                 __builtin_amdgcn_s_sleep(8);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             if (ep == last) {
