@@ -288,7 +288,9 @@ __device__ __forceinline__ void load_scn(const Dev& d, int s, ScnCtx& c) {
 // Per-contact element data at X: barrier b'' (unscaled), gradients, friction blocks.
 // cgrad[c][18] = gbA(3) gbB(3) gfA(6) gfB(6);  cblk[c][4*36] = h2 * friction AA BB AB BA;
 // cactive[c] bit0 barrier active, bit1 friction active; the unscaled barrier b'' of
-// contact c is written to b2arr[c].
+// contact c is written to b2arr[c]. Without kBlocks the friction blocks are left
+// unwritten (a caller that needs only the gradients).
+template <bool kBlocks = true>
 __device__ void eval_elements(const Dev& d, const Work& w, const ScnCtx& sc, const double* X, double* b2arr) {
     for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
         const long long c = sc.cb + i;
@@ -318,10 +320,10 @@ __device__ void eval_elements(const Dev& d, const Work& w, const ScnCtx& sc, con
                 g[12 + j] = k.JB[0][j] * fg.x + k.JB[1][j] * fg.y + k.JB[2][j] * fg.z;
             }
             double H[9];
-            fric_h(k.u, N, sc.fmu, sc.feps, H);
+            if (kBlocks) fric_h(k.u, N, sc.fmu, sc.feps, H);
             double* blk = w.cblk + 144 * ((long long)sc.s * kMaxContacts + i);
             // (X^T H) Y for (A,A), (B,B), (A,B), (B,A)   (FrictionForm.cpp:137-140)
-            for (int which = 0; which < 4; ++which) {
+            for (int which = 0; which < (kBlocks ? 4 : 0); ++which) {
                 double (*Xm)[6] = (which == 1 || which == 3) ? k.JB : k.JA;
                 double (*Ym)[6] = (which == 0 || which == 3) ? k.JA : k.JB;
                 double T[6][3];
@@ -2198,14 +2200,17 @@ __global__ void __launch_bounds__(kBlk) k_alm_post(Dev d, Work w) {
         ScnCtx sb = sc;
         sb.j1 = sb.j0;  // no joints here
         sb.has_barrier = 1;  // barrier form exists whenever contact forms are needed
-        eval_elements(d, w, sb, X, b2arr);
+        eval_elements<false>(d, w, sb, X, b2arr);  // gradients only (the blocks are not read)
         const int* off = w.csr_off + sc.b0 + sc.s;
         const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
         double cm = 0.0;
         for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
             double gb[6] = {0, 0, 0, 0, 0, 0}, gf[6] = {0, 0, 0, 0, 0, 0};
-            for (int e = off[li]; e < off[li + 1]; ++e) {
-                const int c = ent[e] >> 1, side = ent[e] & 1;
+            const int e0 = off[li], e1 = off[li + 1];
+            int en = e0 < e1 ? ent[e0] : 0;  // the next entry's index, fetched a step ahead
+            for (int e = e0; e < e1; ++e) {
+                const int c = en >> 1, side = en & 1;
+                en = e + 1 < e1 ? ent[e + 1] : 0;
                 const double* cg = w.cgrad + 18 * ((long long)sc.s * kMaxContacts + c);
                 for (int k = 0; k < 3; ++k) gb[k] += cg[side * 3 + k];
                 for (int k = 0; k < 6; ++k) gf[k] += cg[6 + side * 6 + k];
@@ -2316,16 +2321,35 @@ void launch_kappa(const Dev& d, const Work& w, hipStream_t st) {
 }
 
 // IPCSolver::computeMaxStep result (IPCSolver.cpp:377-391).
-__global__ void k_ccd_dt(Dev d, Work w) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= d.S) return;
+// One work-group per scene. The candidate counts are sums of integer products, exact in
+// any order, so the work-group sums them in 64-bit integers (the reference's double
+// sums of the same integers are exact below 2^53).
+__global__ void __launch_bounds__(kBlk) k_ccd_dt(Dev d, Work w) {
+    const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
-    double nfv = 0.0, nee = 0.0;
-    for (int p = w.pair_off[s]; p < w.pair_off[s + 1]; ++p) {
+    long long fv = 0, ee = 0;
+    for (int p = w.pair_off[s] + threadIdx.x; p < w.pair_off[s + 1]; p += blockDim.x) {
         const int* k = w.lcnt + 6 * p;
-        nfv += double(k[4] * k[1] + k[5] * k[0]);
-        nee += double(k[2] * k[3]);
+        fv += (long long)k[4] * k[1] + (long long)k[5] * k[0];
+        ee += (long long)k[2] * k[3];
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        fv += __shfl_xor(fv, o);
+        ee += __shfl_xor(ee, o);
+    }
+    __shared__ long long part[2][kBlk / 64];
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = fv;
+        part[1][threadIdx.x >> 6] = ee;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    fv = ee = 0;
+    for (int q = 0; q < kBlk / 64; ++q) {
+        fv += part[0][q];
+        ee += part[1][q];
+    }
+    const double nfv = double(fv), nee = double(ee);
     sp.n_fv = nfv;
     sp.n_ee = nee;
     double adt = sp.dt;
@@ -2335,7 +2359,7 @@ __global__ void k_ccd_dt(Dev d, Work w) {
 }
 
 void launch_ccd_dt(const Dev& d, const Work& w, hipStream_t st) {
-    hipLaunchKernelGGL(k_ccd_dt, dim3((d.S + 255) / 256), dim3(256), 0, st, d, w);
+    hipLaunchKernelGGL(k_ccd_dt, dim3(d.S), dim3(kBlk), 0, st, d, w);
 }
 
 // Pack x, v, a and the implicit-Euler prediction x_hat = x + h v
