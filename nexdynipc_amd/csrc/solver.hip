@@ -2083,12 +2083,84 @@ void launch_newton_helpers(const Dev& d, const Work& w, int n_launch, hipStream_
 
 // ------------------------------------------------------------------ contacts setup
 // Body -> contact CSR in contact order (each contact once under A, once under B).
+// Scenes of up to 2 kMaxContacts bodies: per-body counts by LDS atomics, a scan, then
+// one wavefront places 64 contacts per round in contact order: an entry's place in its
+// body's list is the body's running count plus the entries of the same body in lower
+// lanes (A before B within a contact), found with equality masks built from one ballot
+// per key bit. Larger scenes: every body's lane scans the contacts (LDS broadcasts).
 __device__ void build_csr(const Work& w, const ScnCtx& sc) {
     int* off = w.csr_off + sc.b0 + sc.s;
     int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
+    __shared__ int2 cab[kMaxContacts];
+    __shared__ int csr_sh[kBlk / 64];
+    if (sc.n <= 2 * kMaxContacts) {
+        static_assert(2 * kMaxContacts <= (1 << 12), "body keys of 12 bits");
+        int* cnt = reinterpret_cast<int*>(cab);  // per-body counts, then running places
+        for (int li = threadIdx.x; li < sc.n; li += blockDim.x) cnt[li] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+            const int* ci = w.cint + 5 * (sc.cb + i);
+            const int a = ci[0] - sc.b0, b = ci[1] - sc.b0;
+            if (a >= 0 && a < sc.n) atomicAdd(&cnt[a], 1);
+            if (b >= 0 && b < sc.n) atomicAdd(&cnt[b], 1);
+        }
+        __syncthreads();
+        int run = 0;
+        for (int base = 0; base < sc.n; base += blockDim.x) {  // counts -> offsets
+            const int li = base + threadIdx.x;
+            const int v = li < sc.n ? cnt[li] : 0;
+            int ex;
+            const int tot = wg_scan_shfl(v, csr_sh, &ex);
+            if (li < sc.n) {
+                off[li + 1] = run + ex + v;
+                cnt[li] = run + ex;
+            }
+            run += tot;
+        }
+        if (threadIdx.x == 0) off[0] = 0;
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            const unsigned long long below = (1ull << lane) - 1;
+            for (int base = 0; base < sc.nc; base += 64) {
+                const int i = base + lane;
+                int a = -1, b = -1;
+                if (i < sc.nc) {
+                    const int* ci = w.cint + 5 * (sc.cb + i);
+                    a = ci[0] - sc.b0;
+                    b = ci[1] - sc.b0;
+                }
+                const bool va = a >= 0 && a < sc.n, vb = b >= 0 && b < sc.n;
+                // eqXY: lanes whose X-side body equals this lane's Y-side body
+                unsigned long long eqAA = __ballot(va), eqBA = __ballot(vb), eqAB = eqAA, eqBB = eqBA;
+                for (int k = 0; k < 12; ++k) {
+                    const unsigned long long mA = __ballot(va && ((a >> k) & 1));
+                    const unsigned long long mB = __ballot(vb && ((b >> k) & 1));
+                    const bool ak = (a >> k) & 1, bk = (b >> k) & 1;
+                    eqAA &= ak ? mA : ~mA;
+                    eqBA &= ak ? mB : ~mB;
+                    eqAB &= bk ? mA : ~mA;
+                    eqBB &= bk ? mB : ~mB;
+                }
+                int pa = 0, pb = 0;
+                if (va) pa = cnt[a] + __popcll(eqAA & below) + __popcll(eqBA & below);
+                if (vb) pb = cnt[b] + __popcll(eqAB & below) + __popcll(eqBB & below) + (va && a == b);
+                if (va) ent[pa] = (i << 1) | 0;
+                if (vb) ent[pb] = (i << 1) | 1;
+                __builtin_amdgcn_wave_barrier();  // every lane has read its places
+                // the first entry of a body in this round advances the body's running place
+                if (va && !((eqAA | eqBA) & below))
+                    cnt[a] += __popcll(eqAA) + __popcll(eqBA);
+                if (vb && !((eqAB | eqBB) & below) && !(va && a == b))
+                    cnt[b] += __popcll(eqAB) + __popcll(eqBB);
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __syncthreads();
+        return;
+    }
     // the contacts' body pairs in LDS: every lane then scans them in the same order, so
     // each read is a broadcast
-    __shared__ int2 cab[kMaxContacts];
     for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
         const int* ci = w.cint + 5 * (sc.cb + i);
         cab[i] = make_int2(ci[0], ci[1]);
@@ -2104,7 +2176,6 @@ __device__ void build_csr(const Work& w, const ScnCtx& sc) {
         off[li + 1] = c;
     }
     __syncthreads();
-    __shared__ int csr_sh[kBlk / 64];
     int run = 0;
     for (int base = 0; base < sc.n; base += blockDim.x) {  // counts -> inclusive prefix sums
         const int li = base + threadIdx.x;
