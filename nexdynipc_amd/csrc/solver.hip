@@ -1107,21 +1107,29 @@ __device__ void build_components(const Dev& d, const Work& w, const ScnCtx& sc, 
     // offsets of the multi-body components (by root, ascending) and the coupled bodies in
     // ascending order; then one lane places each coupled body after the earlier members
     // of its component (the off[] of a root serves as its cursor meanwhile)
-    int run = 0, runc = 0;
-    for (int base = 0; base < n; base += blockDim.x) {
-        const int i = base + threadIdx.x;
-        const bool in = i < n;
-        const int root = in ? L.lab[i] : 0;
-        const int v = (in && root == i && L.cnt[i] > 1) ? L.cnt[i] : 0;
-        int ex;
-        const int t = wg_exclusive_scan(v, L.scan, &ex);
-        if (v) L.off[i] = run + ex;
-        run += t;
-        const int c = (in && L.cnt[root] > 1) ? 1 : 0;
-        const int tc = wg_exclusive_scan(c, L.scan, &ex);
-        if (c) L.tmp[runc + ex] = i;
-        runc += tc;
+    // (each lane a contiguous run of bodies; one scan of both counts, packed: each sum is
+    // at most n <= 4096)
+    static_assert(kMaxSceneBodies < (1 << 15), "packed scan of two body counts");
+    const int T = (n + blockDim.x - 1) / blockDim.x;
+    const int i0 = min(n, (int)threadIdx.x * T), i1 = min(n, i0 + T);
+    int sv = 0, ncp = 0;
+    for (int i = i0; i < i1; ++i) {
+        const int root = L.lab[i];
+        if (root == i && L.cnt[i] > 1) sv += L.cnt[i];
+        if (L.cnt[root] > 1) ++ncp;
     }
+    int ex;
+    const int tot = wg_exclusive_scan(sv | (ncp << 16), L.scan, &ex);
+    int ov = ex & 0xFFFF, oc = ex >> 16;
+    for (int i = i0; i < i1; ++i) {
+        const int root = L.lab[i];
+        if (root == i && L.cnt[i] > 1) {
+            L.off[i] = ov;
+            ov += L.cnt[i];
+        }
+        if (L.cnt[root] > 1) L.tmp[oc++] = i;
+    }
+    const int runc = tot >> 16;
     __syncthreads();
     if (threadIdx.x == 0) {
         L.ncoupled = runc;
