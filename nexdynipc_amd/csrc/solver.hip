@@ -1951,6 +1951,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     double fx_next = 0.0, fc_next = 0.0;
     bool have_f = false;
     int kprev = kLsBatchLarge;            // trials the previous line search used
+    int n_newton = 0, n_pcg = 0, n_pcg_iters = 0, n_direct = 0, comp_max = 0, n_trials = 0;  // launch counts
     bool helped = hb != nullptr;          // the helpers answer (until they fail to)
     int bseq = 0;                         // batches posted to the helpers
     const int Kb = 3 * (sc.n + (sc.n & 1) + max(sc.j1 - sc.j0, sc.nc) + 1) <= ((kTermLds / kLsBatch) & ~1)
@@ -1977,17 +1978,15 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
                                                   : component_solve<8>(d, w, sc, b2arr, NG, DX, comp, &ok_sh, true);
         int pit = 0;
         const bool ok = cs == 0 ? true : cs == 2 ? pcg_solve(d, w, sc, b2arr, NG, DX, red, &ok_sh, &pit) : false;
-        if (threadIdx.x == 0) {
-            sp.newton_total += 1.0;
-            if (cs == 2) {
-                sp.solves_pcg += 1;
-                sp.pcg_iters += pit;
-            } else {
-                sp.solves_direct += 1;
-            }
-            sp.comp_max = max(sp.comp_max, comp.maxsize);
+        ++n_newton;
+        if (cs == 2) {
+            ++n_pcg;
+            n_pcg_iters += pit;
+        } else {
+            ++n_direct;
         }
-        __syncthreads();
+        comp_max = max(comp_max, comp.maxsize);
+        __syncthreads();  // comp shares LDS with the term windows of the line search
         if (!ok) {
             if (threadIdx.x == 0) atomicOr(&sp.solve_fail, 1);
             break;
@@ -2038,10 +2037,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
             }
             __syncthreads();  // lsv is rewritten by the next batch
             const int used = accepted ? k + 1 : K;
-            if (threadIdx.x == 0) {
-                sp.ls_trials += used;
-                sp.trials_total += double(used);
-            }
+            n_trials += used;
             ls += used;
             ls_used = ls;
         }
@@ -2072,6 +2068,14 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     viol = wg_reduce(viol, red, OpMax());
     dual = wg_reduce(dual, red, OpMax());
     if (threadIdx.x == 0) {
+        // this launch's counts (integers: the double totals are exact sums either way)
+        sp.newton_total += double(n_newton);
+        sp.solves_pcg += n_pcg;
+        sp.pcg_iters += n_pcg_iters;
+        sp.solves_direct += n_direct;
+        sp.comp_max = max(sp.comp_max, comp_max);
+        sp.ls_trials += n_trials;
+        sp.trials_total += double(n_trials);
         sp.converged = conv;
         sp.newton_iters += (it < maxit ? it : maxit - 1) + 1;  // NewtonSolver reports the last index
         sp.viol = viol;
