@@ -30,8 +30,9 @@ constexpr int kLsBatch = 20;          // line-search trials evaluated together, 
 constexpr int kLsBatchLarge = 8;      // ... for scenes whose 20 trials need several LDS rounds
 
 // Line-search helper work-groups per scene (small batches only; solver.hip,
-// k_newton_helper): one per trial of a batch, and one for the line search's slopes.
-constexpr int kHelpers = kLsBatch + 1;
+// k_newton_helper): one per trial of a batch, one for the line search's slopes and one
+// for the objective at the current iterate.
+constexpr int kHelpers = kLsBatch + 2;
 
 // Mailbox between a scene's Newton work-group and its helpers. Zero-initialised.
 struct HelperBox {
@@ -40,6 +41,7 @@ struct HelperBox {
     int K;               // trials in the posted batch
     int job;             // 0: line-search trials (and slopes); 1: gradient and diagonal blocks
     int want_slopes;     // 1: the posted batch also asks for the slopes
+    int want_f;          // 1: ... and for the objective and contact values at x
     int quit;            // 1: the current launch's helpers leave
     int dead;            // 1: a helper left on its time bound
     int hep[kHelpers];   // per helper: the epoch it last served
@@ -47,6 +49,7 @@ struct HelperBox {
     double alpha;        // step length of trial 0 of the posted batch
     double val[kLsBatch], cval[kLsBatch];
     double slope[2];     // g.dx and its contact part cg.dx
+    double fx[2];        // objective and contact value at x
     double gm[kHelpers]; // job 1, per helper: max |g| over its bodies
     int okf[kHelpers];   // job 1, per helper: 0 if a singleton's 6x6 factor failed
 };

@@ -1669,9 +1669,10 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
 // lends its line search to kHelpers helper work-groups running beside it: the scene's
 // work-group posts a batch of K trials (K, the first step length) in the scene's
 // HelperBox and waits; helper h < kLsBatch evaluates trial h with objective_value_batch
-// at K = 1, whose sums are those of the scene's own batch evaluation, and helper
-// kLsBatch, when asked, sums the slopes g.dx, cg.dx with serial_dot2, so the values are
-// the same bits whichever work-group computes them. Every wait is bounded in time: a
+// at K = 1, whose sums are those of the scene's own batch evaluation, helper kLsBatch,
+// when asked, sums the slopes g.dx, cg.dx with serial_dot2, and helper kLsBatch + 1 the
+// objective and contact values at x with objective_value, so the values are the same
+// bits whichever work-group computes them. Every wait is bounded in time: a
 // helper that hears nothing in time marks the box dead and leaves, and a scene
 // work-group that waits too long for an acknowledgement does the work itself and sends
 // its helpers away; results never depend on the helpers being there. Ticks:
@@ -1771,11 +1772,12 @@ __device__ __noinline__ bool helpers_grad(HelperBox* hb, int bseq, double* ginf,
 }
 
 // Scene work-group, all lanes: trials alpha, alpha / 2, ... (K of them) by the helpers,
-// values to vals / cvals (LDS), and with slopes (LDS, 2) also g.dx and cg.dx; bseq:
-// this batch's number (1, 2, ... per launch). Returns false, with the helpers sent
-// away, when they did not answer in time.
+// values to vals / cvals (LDS), with slopes (LDS, 2) also g.dx and cg.dx, with fxc
+// (LDS, 2) also the objective and contact values at x; bseq: this batch's number
+// (1, 2, ... per launch). Returns false, with the helpers sent away, when they did not
+// answer in time.
 __device__ __noinline__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, double* vals, double* cvals,
-                              double* slopes, int* flag) {
+                              double* slopes, double* fxc, int* flag) {
     static_assert(kHelpers <= 64, "one lane of the first wavefront per helper");
     __threadfence();  // the work-group's stores of x, g, cg and dx, visible to the helpers
     __syncthreads();
@@ -1784,17 +1786,19 @@ __device__ __noinline__ bool helpers_batch(HelperBox* hb, int bseq, int K, doubl
         st_rlx(&hb->K, K);
         st_rlx(&hb->alpha, alpha);
         st_rlx(&hb->want_slopes, slopes ? 1 : 0);
+        st_rlx(&hb->want_f, fxc ? 1 : 0);
         st_rel(&hb->bseq, bseq);
     }
     if (threadIdx.x < 64) {  // lane h waits for helper h's acknowledgement, if one is due
         const int h = threadIdx.x;
-        const int ok = helpers_wait(hb, bseq, h < K || (h == kLsBatch && slopes));
+        const int ok = helpers_wait(hb, bseq, h < K || (h == kLsBatch && slopes) || (h == kLsBatch + 1 && fxc));
         if (ok) {
             if (h < K) {
                 vals[h] = ld_rlx(&hb->val[h]);
                 cvals[h] = ld_rlx(&hb->cval[h]);
             }
             if (slopes && h < 2) slopes[h] = ld_rlx(&hb->slope[h]);
+            if (fxc && h < 2) fxc[h] = ld_rlx(&hb->fx[h]);
         } else if (h == 0) {  // no answer: send this launch's helpers away, none for the step
             st_rlx(&hb->dead, 1);
             st_rel(&hb->quit, 1);
@@ -1816,7 +1820,7 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
     __shared__ alignas(16) double T[kTermLds];
     __shared__ double lv[2];
     __shared__ int cmd;
-    __shared__ int K_sh, slopes_sh, job_sh, okh_sh;
+    __shared__ int K_sh, slopes_sh, f_sh, job_sh, okh_sh;
     __shared__ double alpha_sh;
     const int s = blockIdx.x / kHelpers, h = blockIdx.x % kHelpers;
     HelperBox* hb = w.hbox + s;
@@ -1856,6 +1860,7 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
                         job_sh = ld_rlx(&hb->job);
                         K_sh = ld_rlx(&hb->K);
                         slopes_sh = ld_rlx(&hb->want_slopes);
+                        f_sh = ld_rlx(&hb->want_f);
                         alpha_sh = ld_rlx(&hb->alpha);
                         break;
                     }
@@ -1907,6 +1912,14 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
                     st_rlx(&hb->slope[1], cdsc);
                     st_rel(&hb->ack[h], c);
                 }
+            } else if (h == kLsBatch + 1 && f_sh) {
+                double fv, fcv;
+                objective_value(d, w, sc, w.x, red, &fv, &fcv, T);
+                if (threadIdx.x == 0) {
+                    st_rlx(&hb->fx[0], fv);
+                    st_rlx(&hb->fx[1], fcv);
+                    st_rel(&hb->ack[h], c);
+                }
             }
             __syncthreads();
         }
@@ -1921,6 +1934,7 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     __shared__ NewtonLds nl;
     __shared__ double lsv[2 * kLsBatch];  // trial values, then contact values
     __shared__ double slp[2];             // slopes from a helper
+    __shared__ double fxs[2];             // f(x) and the contact value from a helper
     __shared__ double gsh;                // max |g| from the helpers
     __shared__ int hflag;                 // the helpers answered
     CompLds& comp = nl.comp;
@@ -1992,8 +2006,9 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
             break;
         }
         // line search (LineSearch.cpp:6-58), alpha0 = min(1, maxStep) = 1
+        // f(x): carried over from the accepted trial, else by a helper beside the first
+        // batch or here before it
         double fx = fx_next, fc = fc_next;
-        if (!have_f) objective_value(d, w, sc, Xg, red, &fx, &fc, nl.terms);
         // slopes g.dx and cg.dx: by a helper beside the first batch, or here before it;
         // an uphill direction (g.dx > 0) ends the Newton loop before any trial counts
         double descent = 0.0, cdesc = 0.0;
@@ -2009,16 +2024,26 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         for (int ls = 0; ls < 20 && !accepted;) {
             const int K = helped ? 20 - ls : min(Kb, ls == 0 ? kprev : 20 - ls);
             if (helped) {
-                helped = helpers_batch(hb, ++bseq, K, alpha, lsv, lsv + kLsBatch, have_desc ? nullptr : slp, &hflag);
+                helped = helpers_batch(hb, ++bseq, K, alpha, lsv, lsv + kLsBatch, have_desc ? nullptr : slp,
+                                       have_f ? nullptr : fxs, &hflag);
                 if (helped && !have_desc) {
                     descent = slp[0];
                     cdesc = slp[1];
                     have_desc = true;
                 }
+                if (helped && !have_f) {
+                    fx = fxs[0];
+                    fc = fxs[1];
+                    have_f = true;
+                }
             }
             if (!have_desc) {
                 serial_dot2(G, CG, DX, sc.nd, red, &descent, &cdesc, nl.terms);
                 have_desc = true;
+            }
+            if (!have_f) {
+                objective_value(d, w, sc, Xg, red, &fx, &fc, nl.terms);
+                have_f = true;
             }
             if (descent > 0) break;  // uphill: not accepted, the Newton loop ends
             if (!helped) objective_value_batch(d, w, sc, K, w.x, w.dx, alpha, red, lsv, lsv + kLsBatch, nl.terms);
