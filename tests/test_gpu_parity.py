@@ -356,10 +356,11 @@ def test_broad_phase_tree_in_hbm_matches_tree_in_lds(engine_factory, name):
 
 @pytest.mark.parametrize("name", ["tie", "c3x8"])
 def test_line_search_helpers_match_in_group_trials(engine_factory, name):
-    """In small batches (1 and 8 scenes here) each scene's line-search trials are
-    evaluated by 20 helper work-groups beside the Newton kernel; with
-    NDIPC_NEWTON_HELPERS=0 at upload the scene's own work-group evaluates them. The
-    trials' sums are the same either way: same steps, same bits, same counters."""
+    """In small batches (1 and 8 scenes here) each scene's line-search trials, slopes,
+    f(x) and gradient pass are computed by 22 helper work-groups beside the Newton
+    kernel; with NDIPC_NEWTON_HELPERS=0 at upload the scene's own work-group computes
+    them. Every value comes from the same code in the same order either way: same
+    steps, same bits, same counters."""
     sc = perturbed_c2_scene() if name == "tie" else scenes.pack(scenes.config3(n_scenes=8))
     a = engine_factory(sc)
     a.step(6)
