@@ -207,6 +207,18 @@ int32_t ndipc_adjoint_backward(ndipc_ctx* ctx, const double* dl_dp, const double
  * component in bodies; since upload Newton iterations and line-search trials.
  * No reference counterpart (NewtonSolver.cpp:10-95 only logs). */
 int32_t ndipc_solver_counters(ndipc_ctx* ctx, double* out8);
+/* Line-search helper work-groups (single scenes and small batches), 4 per scene:
+ * whether the scenes have helpers, and since upload the batches they answered, the
+ * waits a scene gave up (it then did the work itself) and the give-ups that found a
+ * helper still busy after the straggle bound. No reference counterpart. */
+int32_t ndipc_helper_counters(ndipc_ctx* ctx, int32_t* out4);
+/* Wait bounds of the helper protocol in device clock ticks (100 MHz): a scene's wait
+ * for an answer, a helper's wait for the next Newton launch and for the next batch, and
+ * a scene's wait for a helper already busy with a batch it gave up; a negative value
+ * restores the default (5 ms, 5 ms, 200 ms, 100 ms). Taken at the next step's begin.
+ * Results never depend on them. No reference counterpart. */
+int32_t ndipc_helper_config(ndipc_ctx* ctx, int64_t wait_ticks, int64_t epoch_ticks, int64_t idle_ticks,
+                            int64_t straggle_ticks);
 /* Average device duration (ms) and count of launches of the named kernel since the
  * last reset, measured with HIP events on the context stream while enabled (steps then
  * enqueue their kernels one by one instead of replaying the step graph). */

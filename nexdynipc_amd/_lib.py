@@ -50,7 +50,7 @@ EXPORTS = [
     "ndipc_scene_min_distance_flag", "ndipc_barrier_eval", "ndipc_friction_eval", "ndipc_ccd_toi_batch",
     "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters", "ndipc_set_joint_state",
     "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
-    "ndipc_autotune",
+    "ndipc_autotune", "ndipc_helper_counters", "ndipc_helper_config",
 ]
 
 
@@ -111,6 +111,8 @@ def library():
         "ndipc_round_ops": ([vp, f64p, f64p, C.c_int32, f64p], C.c_int32),
         "ndipc_elemfn": ([vp, C.c_int32, f64p, f64p, C.c_int32, f64p], C.c_int32),
         "ndipc_solver_counters": ([vp, f64p], C.c_int32),
+        "ndipc_helper_counters": ([vp, i32p], C.c_int32),
+        "ndipc_helper_config": ([vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64], C.c_int32),
         "ndipc_set_joint_state": ([vp, f64p, f64p], C.c_int32),
         "ndipc_set_mass": ([vp, f64p], C.c_int32),
         "ndipc_adjoint_forward": ([vp, f64p, C.c_int32, C.c_int32], C.c_int32),
@@ -284,6 +286,18 @@ class Engine:
         o = np.zeros((self.sc.n_scenes, 8))
         self._ck(self.L.ndipc_solver_counters(self.ctx, ptr(o, f64p)), "solver_counters")
         return o
+
+    def helper_counters(self):
+        """Per scene: helpers on (0/1), and since upload the batches the helpers answered,
+        the waits the scene gave up, and give-ups that outlasted the straggle bound."""
+        o = np.zeros((self.sc.n_scenes, 4), dtype=np.int32)
+        self._ck(self.L.ndipc_helper_counters(self.ctx, ptr(o, i32p)), "helper_counters")
+        return o
+
+    def helper_config(self, wait_ticks=-1, epoch_ticks=-1, idle_ticks=-1, straggle_ticks=-1):
+        """Wait bounds of the helper protocol in 100 MHz ticks (-1: default)."""
+        self._ck(self.L.ndipc_helper_config(self.ctx, int(wait_ticks), int(epoch_ticks), int(idle_ticks),
+                                            int(straggle_ticks)), "helper_config")
 
     def eval_counter(self):
         o = np.zeros(self.sc.n_scenes)

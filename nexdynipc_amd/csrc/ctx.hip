@@ -519,6 +519,9 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
         w.newton_helpers = (!off && (long long)S * (1 + nd::kHelpers) <= ncu) ? 1 : 0;
         w.hbox = dalloc<nd::HelperBox>(ctx, S);
         w.hsingle = dalloc<int>(ctx, N);
+        const nd::HelperCfg cfg{nd::kHelperWaitTicks, nd::kHelperEpochTicks, nd::kHelperIdleTicks,
+                                nd::kHelperStraggleTicks};
+        w.hcfg = dupload(ctx, &cfg, 1);
     }
     w.pair_cnt = dalloc<int>(ctx, S);
     w.pair_off = dalloc<int>(ctx, S + 1);
@@ -723,7 +726,7 @@ static int enqueue_step(ndipc_ctx* ctx, const nd::Adj* rec, int k, int alm_max) 
     if (w.newton_helpers && alm_max > 0) {  // line-search helpers for the step's Newton launches
         HIPCHK(hipEventRecord(ctx->hfork, st));
         HIPCHK(hipStreamWaitEvent(ctx->hstream, ctx->hfork, 0));
-        nd::launch_newton_helpers(d, w, alm_max, ctx->hstream);
+        nd::launch_newton_helpers(d, w, ctx->hstream);
         HIPCHK(hipEventRecord(ctx->hjoin, ctx->hstream));
     }
     for (int a = 0; a < alm_max; ++a) {
@@ -734,7 +737,10 @@ static int enqueue_step(ndipc_ctx* ctx, const nd::Adj* rec, int k, int alm_max) 
         nd::launch_narrow_phase(d, w, mp, true, st);
         nd::launch_alm_post(d, w, st);
     }
-    if (w.newton_helpers && alm_max > 0) HIPCHK(hipStreamWaitEvent(st, ctx->hjoin, 0));
+    if (w.newton_helpers && alm_max > 0) {
+        nd::launch_helpers_close(d, w, st);
+        HIPCHK(hipStreamWaitEvent(st, ctx->hjoin, 0));
+    }
     nd::launch_finalize(d, w, st);
     return 0;
 }
@@ -846,6 +852,35 @@ int32_t ndipc_solver_counters(ndipc_ctx* ctx, double* out8) {
         o[3] = q.solves_pcg;
         o[4] = q.pcg_iters;
         o[5] = q.comp_max;
+    }
+    return 0;
+}
+
+int32_t ndipc_helper_config(ndipc_ctx* ctx, int64_t wait_ticks, int64_t epoch_ticks, int64_t idle_ticks,
+                            int64_t straggle_ticks) {
+    if (check_ctx(ctx)) return -1;
+    if (!ctx->w.hcfg) return fail(ctx, "helper_config: no scenes uploaded");
+    const nd::HelperCfg cfg{
+        wait_ticks >= 0 ? (unsigned long long)wait_ticks : nd::kHelperWaitTicks,
+        epoch_ticks >= 0 ? (unsigned long long)epoch_ticks : nd::kHelperEpochTicks,
+        idle_ticks >= 0 ? (unsigned long long)idle_ticks : nd::kHelperIdleTicks,
+        straggle_ticks >= 0 ? (unsigned long long)straggle_ticks : nd::kHelperStraggleTicks};
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(ctx->w.hcfg, &cfg, sizeof(cfg), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int32_t ndipc_helper_counters(ndipc_ctx* ctx, int32_t* out4) {
+    if (check_ctx(ctx)) return -1;
+    if (!ctx->w.hbox) return fail(ctx, "helper_counters: no scenes uploaded");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<nd::HelperBox> hb(ctx->S);
+    HIPCHK(hipMemcpy(hb.data(), ctx->w.hbox, sizeof(nd::HelperBox) * ctx->S, hipMemcpyDeviceToHost));
+    for (int s = 0; s < ctx->S; ++s) {
+        out4[4 * s + 0] = ctx->w.newton_helpers;
+        out4[4 * s + 1] = hb[s].n_helped;
+        out4[4 * s + 2] = hb[s].n_giveup;
+        out4[4 * s + 3] = hb[s].n_straggle;
     }
     return 0;
 }

@@ -34,24 +34,46 @@ constexpr int kLsBatchLarge = 8;      // ... for scenes whose 20 trials need sev
 // for the objective at the current iterate.
 constexpr int kHelpers = kLsBatch + 2;
 
-// Mailbox between a scene's Newton work-group and its helpers. Zero-initialised.
+// Mailbox between a scene's Newton work-group and its helpers. Zero-initialised; the
+// per-step fields are cleared by k_step_begin before the helpers of a step start.
 struct HelperBox {
-    int epoch;           // Newton launches opened so far
-    int bseq;            // batches posted in the current launch (0: none)
+    int epoch;           // Newton launches opened in this step
+    int bseq;            // batches posted in this step (numbers never repeat within a step)
     int K;               // trials in the posted batch
     int job;             // 0: line-search trials (and slopes); 1: gradient and diagonal blocks
     int want_slopes;     // 1: the posted batch also asks for the slopes
     int want_f;          // 1: ... and for the objective and contact values at x
     int quit;            // 1: the current launch's helpers leave
-    int dead;            // 1: a helper left on its time bound
+    int dead;            // 1: this step's helpers are gone (a wait ran out on either side)
+    int closed;          // 1: the step's Newton launches are over; the helpers leave
+    int bopen;           // bseq when the current launch opened (later batches are its own)
     int hep[kHelpers];   // per helper: the epoch it last served
     int ack[kHelpers];   // per helper: the batch it last evaluated
+    int busy[kHelpers];  // per helper: the batch it took up (set before it re-checks quit)
+    // bounds of every wait (wall_clock64 ticks), copied from HelperCfg at step begin
+    unsigned long long wait_ticks, epoch_ticks, idle_ticks, straggle_ticks;
     double alpha;        // step length of trial 0 of the posted batch
     double val[kLsBatch], cval[kLsBatch];
     double slope[2];     // g.dx and its contact part cg.dx
     double fx[2];        // objective and contact value at x
     double gm[kHelpers]; // job 1, per helper: max |g| over its bodies
     int okf[kHelpers];   // job 1, per helper: 0 if a singleton's 6x6 factor failed
+    // since upload (never reset): batches the helpers answered, waits the scene gave up,
+    // and give-ups that found a helper still busy after straggle_ticks
+    int n_helped, n_giveup, n_straggle, pad;
+};
+
+// Wait bounds of the helper protocol (host-settable, read at every step's begin), in
+// wall_clock64 ticks (100 MHz on gfx950), and their defaults.
+constexpr unsigned long long kHelperWaitTicks = 500000;        // 5 ms
+constexpr unsigned long long kHelperEpochTicks = 500000;       // 5 ms
+constexpr unsigned long long kHelperIdleTicks = 20000000;      // 200 ms
+constexpr unsigned long long kHelperStraggleTicks = 10000000;  // 100 ms
+struct HelperCfg {
+    unsigned long long wait_ticks;      // scene: for an answer
+    unsigned long long epoch_ticks;     // helper: for the next Newton launch to open
+    unsigned long long idle_ticks;      // helper: for the next batch inside a launch
+    unsigned long long straggle_ticks;  // scene, after giving up: for a helper mid-job
 };
 
 struct SceneParams {
@@ -138,6 +160,7 @@ struct Work {
     // beside its Newton work-group (S * (1 + kHelpers) fit the CUs at one per CU)
     int newton_helpers;
     HelperBox* hbox;         // [S]
+    HelperCfg* hcfg;         // [1] wait bounds of the helper protocol
     int* hsingle;            // [N] with helpers: 1 if the body is alone in its component
     // body pairs
     long long* pair_cap_off; // [S+1] capacity offsets (n(n-1)/2 per scene)
@@ -261,7 +284,8 @@ void launch_contacts_setup(const Dev& d, const Work& w, hipStream_t st);
 void launch_newton(const Dev& d, const Work& w, hipStream_t st);
 // k_newton's line-search helpers (w.newton_helpers) for the n_launch Newton launches of
 // one step, to run beside the step's kernels
-void launch_newton_helpers(const Dev& d, const Work& w, int n_launch, hipStream_t st);
+void launch_newton_helpers(const Dev& d, const Work& w, hipStream_t st);
+void launch_helpers_close(const Dev& d, const Work& w, hipStream_t st);
 void launch_alm_post(const Dev& d, const Work& w, hipStream_t st);
 void launch_finalize(const Dev& d, const Work& w, hipStream_t st);
 void launch_barrier_init_ccd(const Dev& d, const Work& w, hipStream_t st);

@@ -1674,23 +1674,26 @@ __device__ int component_solve(const Dev& d, const Work& w, const ScnCtx& sc, co
 // objective and contact values at x with objective_value, so the values are the same
 // bits whichever work-group computes them. Every wait is bounded in time: a
 // helper that hears nothing in time marks the box dead and leaves, and a scene
-// work-group that waits too long for an acknowledgement does the work itself and sends
-// its helpers away; results never depend on the helpers being there. Ticks:
-// wall_clock64, 100 MHz on gfx950.
-// Within a Newton launch a helper waits up to kHelperIdleTicks for the next batch (the
-// scene's work-group computes in between); for a launch to open, kHelperEpochTicks,
-// since the helpers and the Newton launches run side by side. A scene work-group waits
-// kHelperWaitTicks for an answer. When dispatches are serialised (a profiler collecting
-// counters, say) the helpers never run beside the Newton kernel: whichever side times
-// out first marks the box dead, and the step's remaining launches run in-group at once.
-constexpr unsigned long long kHelperIdleTicks = 20000000;  // 200 ms
-constexpr unsigned long long kHelperEpochTicks = 500000;   // 5 ms
-constexpr unsigned long long kHelperWaitTicks = 500000;    // 5 ms
+// work-group that waits too long for an acknowledgement marks it dead, waits out any
+// helper already working on the batch, and does the work itself; results never depend
+// on the helpers being there. Ticks: wall_clock64, 100 MHz on gfx950.
+// Within a Newton launch a helper waits up to idle_ticks for the next batch (the
+// scene's work-group computes in between); for a launch to open, epoch_ticks, since the
+// helpers and the Newton launches run side by side. A scene work-group waits wait_ticks
+// for an answer. When dispatches are serialised (a profiler collecting counters, say)
+// the helpers never run beside the Newton kernel: whichever side times out first marks
+// the box dead, and the step's remaining launches run in-group at once. Batch numbers
+// increase through a step and every per-step field is cleared at the step's begin, so
+// nothing a helper saw in one launch or step is mistaken for a later one.
+// The default bounds are kHelper*Ticks (ndipc_internal.h); ndipc_helper_config sets them.
 
 __device__ __forceinline__ void st_rel(int* p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ int ld_rlx(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double ld_rlx(const double* p) {
@@ -1702,25 +1705,41 @@ __device__ __forceinline__ void st_rlx(int* p, int v) {
 __device__ __forceinline__ void st_rlx(double* p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// sequentially consistent: the give-up handshake between a scene and its helpers
+__device__ __forceinline__ void st_sc(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_sc(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_SEQ_CST, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Scene work-group, lane 0: opens this launch for the helpers (before any batch).
 __device__ void helpers_open(HelperBox* hb) {
-    st_rlx(&hb->bseq, 0);
     st_rlx(&hb->quit, 0);
-    for (int h = 0; h < kHelpers; ++h) st_rlx(&hb->ack[h], 0);
+    st_rlx(&hb->bopen, ld_rlx(&hb->bseq));
     __hip_atomic_fetch_add(&hb->epoch, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Scene work-group, first wavefront: waits until every helper h with due(h) on its lane
-// has acknowledged batch bseq, or the helpers died, or kHelperWaitTicks passed; then one
-// acquire. Returns 1 (all answered) or 0, the same on every lane.
+// Scene work-group, first wavefront (lane h: helper h; due: an answer is expected from
+// it): waits until every due helper has acknowledged batch bseq, or the box is dead, or
+// wait_ticks passed. All answered: one acquire makes the answers visible. Otherwise the
+// scene gives up: it marks the box dead and quit, then waits for every due helper that
+// already took the batch up (busy == bseq). A helper stores busy before it re-checks
+// quit and dead, and the scene stores those before it reads busy, all sequentially
+// consistent, so either the helper sees the give-up and does nothing or the scene sees
+// busy and waits for its acknowledgement: no helper store of the batch lands after the
+// scene has moved on (unless a helper stays busy past straggle_ticks, which is
+// counted). A helper that sees the give-up after taking the batch up withdraws (busy
+// back to 0) instead of answering. Returns 1 (all answered) or 0, the same on every
+// lane.
 __device__ __forceinline__ int helpers_wait(HelperBox* hb, int bseq, bool due) {
     const int h = threadIdx.x;
-    const unsigned long long t0 = wall_clock64();
+    const unsigned long long bound = ld_rlx(&hb->wait_ticks);
+    unsigned long long t0 = wall_clock64();
     bool mine = !due || ld_rlx(&hb->ack[h]) == bseq;
     int ok = 1;
     while (!__all(mine)) {
-        if (ld_rlx(&hb->dead) || wall_clock64() - t0 > kHelperWaitTicks) {
+        if (ld_rlx(&hb->dead) || wall_clock64() - t0 > bound) {
             ok = 0;
             break;
         }
@@ -1728,24 +1747,58 @@ __device__ __forceinline__ int helpers_wait(HelperBox* hb, int bseq, bool due) {
         if (!mine) mine = ld_rlx(&hb->ack[h]) == bseq;
     }
     ok = __all(ok);
+    if (ok) {
+        if (h == 0) atomicAdd(&hb->n_helped, 1);
+    } else {
+        st_sc(&hb->dead, 1);
+        st_sc(&hb->quit, 1);
+        int straggle = 0;
+        if (due && ld_sc(&hb->busy[h]) == bseq) {
+            const unsigned long long sb = ld_rlx(&hb->straggle_ticks);
+            t0 = wall_clock64();
+            // until it answers, or withdraws (it saw the give-up after taking the batch)
+            while (ld_rlx(&hb->ack[h]) != bseq && ld_rlx(&hb->busy[h]) == bseq) {
+                if (wall_clock64() - t0 > sb) {
+                    straggle = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        straggle = __any(straggle);
+        if (h == 0) {
+            atomicAdd(&hb->n_giveup, 1);
+            if (straggle) atomicAdd(&hb->n_straggle, 1);
+        }
+    }
     // one acquire for all the acknowledgements (an acquire invalidates caches, so the
     // polls above are relaxed)
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return ok;
 }
 
+// Scene work-group, lane 0: posts a batch unless the box is dead; returns 1 if posted.
+__device__ __forceinline__ int helpers_post(HelperBox* hb, int bseq) {
+    if (ld_rlx(&hb->dead)) return 0;
+    st_rel(&hb->bseq, bseq);
+    return 1;
+}
+
 // Scene work-group, all lanes: the gradient, NG = -G, the contact part of the gradient,
 // the diagonal blocks and the singleton solves of every body by the helpers (body li on
 // helper li % kHelpers), w.hsingle holding which bodies are singletons. *ginf: max |g|;
 // *ok_sh cleared if a singleton's factor failed. Returns false, with the helpers sent
-// away, when they did not answer in time.
+// away, when they did not answer in time (or were gone already).
 __device__ __noinline__ bool helpers_grad(HelperBox* hb, int bseq, double* ginf, int* ok_sh, int* flag) {
     __threadfence();  // the work-group's element data and singleton flags, visible to the helpers
     __syncthreads();
     if (threadIdx.x == 0) {
         st_rlx(&hb->job, 1);
-        st_rel(&hb->bseq, bseq);
+        *flag = helpers_post(hb, bseq);
     }
+    __syncthreads();
+    if (!*flag) return false;
+    __syncthreads();  // every lane has read *flag
     if (threadIdx.x < 64) {
         const int h = threadIdx.x;
         const bool due = h < kHelpers;
@@ -1761,9 +1814,6 @@ __device__ __noinline__ bool helpers_grad(HelperBox* hb, int bseq, double* ginf,
                 *ginf = gm;
                 if (!okh) *ok_sh = 0;
             }
-        } else if (h == 0) {  // no answer: send this launch's helpers away, none for the step
-            st_rlx(&hb->dead, 1);
-            st_rel(&hb->quit, 1);
         }
         if (h == 0) *flag = ok;
     }
@@ -1773,9 +1823,9 @@ __device__ __noinline__ bool helpers_grad(HelperBox* hb, int bseq, double* ginf,
 
 // Scene work-group, all lanes: trials alpha, alpha / 2, ... (K of them) by the helpers,
 // values to vals / cvals (LDS), with slopes (LDS, 2) also g.dx and cg.dx, with fxc
-// (LDS, 2) also the objective and contact values at x; bseq: this batch's number
-// (1, 2, ... per launch). Returns false, with the helpers sent away, when they did not
-// answer in time.
+// (LDS, 2) also the objective and contact values at x; bseq: this batch's number.
+// Returns false, with the helpers sent away, when they did not answer in time (or were
+// gone already).
 __device__ __noinline__ bool helpers_batch(HelperBox* hb, int bseq, int K, double alpha, double* vals, double* cvals,
                               double* slopes, double* fxc, int* flag) {
     static_assert(kHelpers <= 64, "one lane of the first wavefront per helper");
@@ -1787,8 +1837,11 @@ __device__ __noinline__ bool helpers_batch(HelperBox* hb, int bseq, int K, doubl
         st_rlx(&hb->alpha, alpha);
         st_rlx(&hb->want_slopes, slopes ? 1 : 0);
         st_rlx(&hb->want_f, fxc ? 1 : 0);
-        st_rel(&hb->bseq, bseq);
+        *flag = helpers_post(hb, bseq);
     }
+    __syncthreads();
+    if (!*flag) return false;
+    __syncthreads();  // every lane has read *flag
     if (threadIdx.x < 64) {  // lane h waits for helper h's acknowledgement, if one is due
         const int h = threadIdx.x;
         const int ok = helpers_wait(hb, bseq, h < K || (h == kLsBatch && slopes) || (h == kLsBatch + 1 && fxc));
@@ -1799,9 +1852,6 @@ __device__ __noinline__ bool helpers_batch(HelperBox* hb, int bseq, int K, doubl
             }
             if (slopes && h < 2) slopes[h] = ld_rlx(&hb->slope[h]);
             if (fxc && h < 2) fxc[h] = ld_rlx(&hb->fx[h]);
-        } else if (h == 0) {  // no answer: send this launch's helpers away, none for the step
-            st_rlx(&hb->dead, 1);
-            st_rel(&hb->quit, 1);
         }
         if (h == 0) *flag = ok;
     }
@@ -1809,13 +1859,13 @@ __device__ __noinline__ bool helpers_batch(HelperBox* hb, int bseq, int K, doubl
     return *flag != 0;
 }
 
-// Helper work-group blockIdx.x = s kHelpers + h serves trial h (h < kLsBatch) or the
-// slopes (h = kLsBatch) of each batch of scene s
-// through the n_launch Newton launches of a step (each opens an epoch and ends with
-// quit), then leaves; it also leaves, marking the box dead for the rest of the step, when
-// it hears nothing for kHelperIdleTicks. It runs beside the step's kernels, on a second
-// stream forked from and joined to theirs.
-__global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_launch) {
+// Helper work-group blockIdx.x = s kHelpers + h serves trial h (h < kLsBatch), the
+// slopes (h = kLsBatch) or f(x) (h = kLsBatch + 1) of each batch of scene s, and a share
+// of the bodies in gradient jobs, through the Newton launches of a step (each opens an
+// epoch and ends with quit) until the step closes; it also leaves, marking the box dead
+// for the rest of the step, when it hears nothing in time. It runs beside the step's
+// kernels, on a second stream forked from and joined to theirs.
+__global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w) {
     __shared__ double red[kBlk];
     __shared__ alignas(16) double T[kTermLds];
     __shared__ double lv[2];
@@ -1824,20 +1874,23 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
     __shared__ double alpha_sh;
     const int s = blockIdx.x / kHelpers, h = blockIdx.x % kHelpers;
     HelperBox* hb = w.hbox + s;
-    for (int launch = 0; launch < n_launch; ++launch) {
-        if (threadIdx.x == 0) {  // wait for the next Newton launch to open
+    int served = 0;  // lane 0: the last batch taken up
+    for (;;) {
+        if (threadIdx.x == 0) {  // wait for the next Newton launch to open, or the step to close
             const int last = ld_rlx(&hb->hep[h]);
+            const unsigned long long bound = ld_rlx(&hb->epoch_ticks);
             const unsigned long long t0 = wall_clock64();
-            int ep = last;
-            while ((ep = ld_rlx(&hb->epoch)) == last && wall_clock64() - t0 <= kHelperEpochTicks)
+            int ep = last, cl = 0;
+            while ((ep = ld_rlx(&hb->epoch)) == last && !(cl = ld_rlx(&hb->closed)) && wall_clock64() - t0 <= bound)
                 __builtin_amdgcn_s_sleep(8);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            if (ep == last) {
-                st_rel(&hb->dead, 1);
-                cmd = -1;
-            } else {
+            if (ep != last) {
                 st_rlx(&hb->hep[h], ep);
+                served = ld_rlx(&hb->bopen);  // batches up to here belong to earlier launches
                 cmd = 0;
+            } else {
+                if (!cl) st_sc(&hb->dead, 1);  // timed out
+                cmd = -1;
             }
         }
         __syncthreads();
@@ -1846,15 +1899,24 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
         // set by the kernels before it
         ScnCtx sc;
         load_scn(d, s, sc);
-        int served = 0;
         for (;;) {
             if (threadIdx.x == 0) {
+                const unsigned long long bound = ld_rlx(&hb->idle_ticks);
                 const unsigned long long t0 = wall_clock64();
                 int c = -1;
                 for (;;) {  // relaxed polls (an acquire invalidates caches), then one acquire
                     if (ld_rlx(&hb->quit)) break;
                     const int b = ld_rlx(&hb->bseq);
                     if (b != served) {
+                        served = b;
+                        // take the batch up, then re-check whether the scene gave up on it
+                        st_sc(&hb->busy[h], b);
+                        const int gone = ld_sc(&hb->dead), q = ld_sc(&hb->quit);
+                        if (gone || q) {  // given up: withdraw, so the scene need not wait
+                            st_sc(&hb->busy[h], 0);
+                            if (gone) c = -2;
+                            break;
+                        }
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                         c = b;
                         job_sh = ld_rlx(&hb->job);
@@ -1864,8 +1926,8 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
                         alpha_sh = ld_rlx(&hb->alpha);
                         break;
                     }
-                    if (wall_clock64() - t0 > kHelperIdleTicks) {
-                        st_rel(&hb->dead, 1);
+                    if (wall_clock64() - t0 > bound) {
+                        st_sc(&hb->dead, 1);
                         c = -2;
                         break;
                     }
@@ -1877,7 +1939,7 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w, int n_lau
             const int c = cmd;
             if (c == -2) return;
             if (c < 0) break;  // this launch is closed
-            served = c;        // lane 0's acquire, then the barrier: x, dx of the batch are visible
+            // lane 0's acquire, then the barrier: x, dx of the batch are visible
             if (job_sh == 1) {  // gradient, diagonal blocks, singleton solves: bodies h, h + kHelpers, ...
                 const int* off = w.csr_off + sc.b0 + sc.s;
                 const int* ent = w.csr_ent + (long long)sc.s * 2 * kMaxContacts;
@@ -1937,15 +1999,21 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     __shared__ double fxs[2];             // f(x) and the contact value from a helper
     __shared__ double gsh;                // max |g| from the helpers
     __shared__ int hflag;                 // the helpers answered
+    __shared__ int hopen[2];              // batch number at this launch's open, helpers alive
     CompLds& comp = nl.comp;
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
     HelperBox* hb = w.newton_helpers ? w.hbox + s : nullptr;
-    if (hb && threadIdx.x == 0) helpers_open(hb);
+    if (hb && threadIdx.x == 0) {
+        helpers_open(hb);
+        hopen[0] = ld_rlx(&hb->bseq);
+        hopen[1] = !ld_rlx(&hb->dead);
+    }
     if (sp.alm_done) {
         if (hb && threadIdx.x == 0) st_rel(&hb->quit, 1);
         return;
     }
+    __syncthreads();
     ScnCtx sc;
     load_scn(d, s, sc);
     double* X = w.x + 6 * sc.b0;
@@ -1966,8 +2034,8 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     bool have_f = false;
     int kprev = kLsBatchLarge;            // trials the previous line search used
     int n_newton = 0, n_pcg = 0, n_pcg_iters = 0, n_direct = 0, comp_max = 0, n_trials = 0;  // launch counts
-    bool helped = hb != nullptr;          // the helpers answer (until they fail to)
-    int bseq = 0;                         // batches posted to the helpers
+    bool helped = hb && hopen[1];         // the helpers answer (until they fail to)
+    int bseq = hb ? hopen[0] : 0;         // batches posted to the helpers in this step
     const int Kb = 3 * (sc.n + (sc.n & 1) + max(sc.j1 - sc.j0, sc.nc) + 1) <= ((kTermLds / kLsBatch) & ~1)
                        ? kLsBatch
                        : kLsBatchLarge;
@@ -2114,8 +2182,19 @@ void launch_newton(const Dev& d, const Work& w, hipStream_t st) {
     timing_end("newton", st);
 }
 
-void launch_newton_helpers(const Dev& d, const Work& w, int n_launch, hipStream_t st) {
-    hipLaunchKernelGGL(k_newton_helper, dim3(d.S * kHelpers), dim3(kBlk), 0, st, d, w, n_launch);
+void launch_newton_helpers(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_newton_helper, dim3(d.S * kHelpers), dim3(kBlk), 0, st, d, w);
+}
+
+// Ends the step's helper launch: once the step's last Newton launch has run, every
+// scene's helpers leave (they wait for the next launch or for this flag).
+__global__ void k_helpers_close(Dev d, Work w) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < d.S) st_rel(&w.hbox[s].closed, 1);
+}
+
+void launch_helpers_close(const Dev& d, const Work& w, hipStream_t st) {
+    hipLaunchKernelGGL(k_helpers_close, dim3((d.S + 255) / 256), dim3(256), 0, st, d, w);
 }
 
 // ------------------------------------------------------------------ contacts setup
@@ -2366,7 +2445,24 @@ __global__ void k_step_begin(Dev d, Work w, const double* dt) {
     sp.pcg_iters = 0;
     sp.comp_max = 0;
     w.flag[s] = 0;
-    if (w.newton_helpers) w.hbox[s].dead = 0;  // this step's helpers are alive until they time out
+    if (w.newton_helpers) {  // the step's helper protocol starts afresh (no helper runs yet)
+        HelperBox& hb = w.hbox[s];
+        hb.epoch = 0;
+        hb.bseq = 0;
+        hb.bopen = 0;
+        hb.quit = 0;
+        hb.dead = 0;
+        hb.closed = 0;
+        for (int h = 0; h < kHelpers; ++h) {
+            hb.hep[h] = 0;
+            hb.ack[h] = 0;
+            hb.busy[h] = 0;
+        }
+        hb.wait_ticks = w.hcfg->wait_ticks;
+        hb.epoch_ticks = w.hcfg->epoch_ticks;
+        hb.idle_ticks = w.hcfg->idle_ticks;
+        hb.straggle_ticks = w.hcfg->straggle_ticks;
+    }
 }
 
 void launch_step_begin(const Dev& d, const Work& w, const double* dt, hipStream_t st) {

@@ -375,6 +375,51 @@ def test_line_search_helpers_match_in_group_trials(engine_factory, name):
         assert np.array_equal(x, y)
     assert np.array_equal(ref[1], b.stats())
     assert np.array_equal(ref[2], b.solver_counters())
+    # the helper path really ran: every scene's batches were answered, none given up
+    hc = a.helper_counters()
+    assert np.all(hc[:, 0] == 1) and np.all(hc[:, 1] > 0), hc
+    assert np.all(hc[:, 2] == 0) and np.all(hc[:, 3] == 0), hc
+    assert np.all(b.helper_counters()[:, 0] == 0)
+
+
+@pytest.mark.parametrize("bound", ["wait", "epoch"])
+def test_helpers_give_up_and_recover(engine_factory, bound):
+    """Forces the helper protocol's time-out from the host: for two steps the scene's
+    wait for an answer (or the helpers' wait for a Newton launch to open) is 0 ticks, so
+    the waits run out at once and the scene does the work itself, after waiting out any
+    helper already busy with the batch. Then the default bounds return and the next
+    steps are helped again. Every step gives the in-group engine's bits."""
+    sc = perturbed_c2_scene()
+    a = engine_factory(sc)
+    os.environ["NDIPC_NEWTON_HELPERS"] = "0"
+    try:
+        b = engine_factory(sc)
+    finally:
+        del os.environ["NDIPC_NEWTON_HELPERS"]
+    a.step(2)
+    b.step(2)
+    h0 = a.helper_counters()
+    assert h0[0, 0] == 1 and h0[0, 1] > 0 and h0[0, 2] == 0
+    if bound == "wait":
+        a.helper_config(wait_ticks=0)
+    else:
+        a.helper_config(epoch_ticks=0)
+    a.step(2)
+    b.step(2)
+    h1 = a.helper_counters()
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert np.array_equal(x, y)
+    if bound == "wait":
+        assert h1[0, 2] > h0[0, 2], (h0, h1)  # the scene gave up
+    a.helper_config()
+    a.step(2)
+    b.step(2)
+    h2 = a.helper_counters()
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.stats(), b.stats())
+    assert h2[0, 1] > h1[0, 1], (h1, h2)  # helped again
+    assert h2[0, 2] == h1[0, 2] and h2[0, 3] == 0, (h1, h2)
 
 
 def test_autotune_restores_state_and_keeps_results(engine_factory):

@@ -155,15 +155,19 @@ def test_body_pairs_equal_brute_force(seed):
     boxes = _world_aabbs(sc)
     for r in (0.0, 0.05, 0.5):
         got = {tuple(p) for p in o.body_pairs(r).tolist()}
-        ref = set()
+        ref, near = set(), set()
         for i in range(len(boxes)):
             for j in range(i + 1, len(boxes)):
                 a0, a1 = boxes[i][0] - r, boxes[i][1] + r
                 b0, b1 = boxes[j][0] - r, boxes[j][1] + r
                 if np.all(a0 <= b1) and np.all(a1 >= b0):
                     ref.add((i, j))
-        # world boxes here are recomputed in numpy: compare only pairs not on a boundary
-        assert ref <= got | ref and len(got ^ ref) <= 1
+                # world boxes here are recomputed in numpy: a pair whose boxes touch
+                # within 1e-9 on some axis may fall either way
+                if np.min(np.abs(np.concatenate([a0 - b1, a1 - b0]))) < 1e-9:
+                    near.add((i, j))
+        assert len(ref) > 0
+        assert got - near == ref - near
 
 
 def test_touching_boxes_overlap_and_cube_ground_candidates():
