@@ -223,10 +223,19 @@ int32_t ndipc_helper_config(ndipc_ctx* ctx, int64_t wait_ticks, int64_t epoch_ti
  * last reset, measured with HIP events on the context stream while enabled (steps then
  * enqueue their kernels one by one instead of replaying the step graph). */
 int32_t ndipc_timing_enable(ndipc_ctx* ctx, int32_t on);
-/* ndipc_step replays one whole step captured as a HIP graph (default, on = 1) or
- * enqueues the step's kernels one by one (on = 0); both run the same kernels with the
- * same arguments. No reference counterpart (host launch strategy). */
+/* ndipc_step replays captured HIP graphs (default, on = 1: the step's prefix and one ALM
+ * round in adaptive mode, the whole step in fixed-round mode) or enqueues the step's
+ * kernels one by one (on = 0); both run the same kernels with the same arguments. No
+ * reference counterpart (host launch strategy). */
 int32_t ndipc_use_graph(ndipc_ctx* ctx, int32_t on);
+/* Adaptive ALM rounds (adaptive = 1, default): each step issues ALM rounds only until
+ * every scene's ALM loop has ended (a round reports to pinned host memory), at most
+ * `lookahead` rounds ahead of the last one reported; adaptive = 0 enqueues all
+ * alm_max_iters rounds every step (scenes that finished skip them on the device). Same
+ * results either way. The adjoint's recorded forward always runs the fixed rounds.
+ * Environment at create: NDIPC_FIXED_ROUNDS=1, NDIPC_ROUND_LOOKAHEAD=n. No reference
+ * counterpart (IPCSolver.cpp:965-1195 loops on the host). */
+int32_t ndipc_set_step_mode(ndipc_ctx* ctx, int32_t adaptive, int32_t lookahead);
 /* Autotuner of the launch shapes (work-group counts of the persistent top-K, narrow-phase,
  * scene-min-distance and CCD kernels) for the uploaded batch: from the current state,
  * each candidate runs trial_steps steps of dt (timed with HIP events), one knob at a

@@ -51,6 +51,7 @@ EXPORTS = [
     "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters", "ndipc_set_joint_state",
     "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
     "ndipc_autotune", "ndipc_helper_counters", "ndipc_helper_config",
+    "ndipc_set_step_mode",
 ]
 
 
@@ -106,6 +107,7 @@ def library():
                              f64p], C.c_int32),
         "ndipc_timing_enable": ([vp, C.c_int32], C.c_int32),
         "ndipc_use_graph": ([vp, C.c_int32], C.c_int32),
+        "ndipc_set_step_mode": ([vp, C.c_int32, C.c_int32], C.c_int32),
         "ndipc_autotune": ([vp, f64p, C.c_int32, i32p, f64p], C.c_int32),
         "ndipc_timing_get": ([vp, C.c_char_p, f64p, i32p], C.c_int32),
         "ndipc_round_ops": ([vp, f64p, f64p, C.c_int32, f64p], C.c_int32),
@@ -286,6 +288,10 @@ class Engine:
         o = np.zeros((self.sc.n_scenes, 8))
         self._ck(self.L.ndipc_solver_counters(self.ctx, ptr(o, f64p)), "solver_counters")
         return o
+
+    def set_step_mode(self, adaptive=True, lookahead=1):
+        """Adaptive ALM rounds (issue only the rounds the scenes need) or fixed rounds."""
+        self._ck(self.L.ndipc_set_step_mode(self.ctx, 1 if adaptive else 0, int(lookahead)), "set_step_mode")
 
     def helper_counters(self):
         """Per scene: helpers on (0/1), and since upload the batches the helpers answered,

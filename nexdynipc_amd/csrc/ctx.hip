@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <unordered_set>
@@ -64,6 +65,13 @@ struct ndipc_ctx {
     bool use_graph = true;
     hipGraphExec_t step_exec = nullptr;
     int graph_alm = -1;  // ALM iterations the captured step enqueues
+    // adaptive ALM rounds (default): the step's prefix and one ALM round captured as
+    // graphs; the host issues rounds until every scene's ALM loop has ended
+    bool adaptive = true;
+    int round_lookahead = 1;          // rounds issued beyond the last one reported
+    hipGraphExec_t pre_exec = nullptr, round_exec = nullptr;
+    unsigned int* hflag = nullptr;    // pinned host memory: [round number, all scenes done]
+    unsigned int rounds_issued = 0;   // rounds with a flag issued since upload
 };
 
 namespace {
@@ -134,7 +142,9 @@ void free_adjoint(ndipc_ctx* ctx) {
 
 void drop_graph(ndipc_ctx* ctx) {
     if (ctx->step_exec) (void)hipGraphExecDestroy(ctx->step_exec);
-    ctx->step_exec = nullptr;
+    if (ctx->pre_exec) (void)hipGraphExecDestroy(ctx->pre_exec);
+    if (ctx->round_exec) (void)hipGraphExecDestroy(ctx->round_exec);
+    ctx->step_exec = ctx->pre_exec = ctx->round_exec = nullptr;
     ctx->graph_alm = -1;
 }
 
@@ -273,6 +283,18 @@ int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
         delete c;
         return -4;
     }
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->hflag), 2 * sizeof(unsigned int), hipHostMallocCoherent) !=
+        hipSuccess) {
+        (void)hipEventDestroy(c->hjoin);
+        (void)hipEventDestroy(c->hfork);
+        (void)hipStreamDestroy(c->hstream);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return -4;
+    }
+    c->hflag[0] = c->hflag[1] = 0;
+    if (const char* env = std::getenv("NDIPC_FIXED_ROUNDS")) c->adaptive = env[0] != '1';
+    if (const char* env = std::getenv("NDIPC_ROUND_LOOKAHEAD")) c->round_lookahead = std::max(0, std::atoi(env));
     *out = c;
     return 0;
 }
@@ -290,6 +312,7 @@ int32_t ndipc_destroy(ndipc_ctx* ctx) {
     if (hipEventDestroy(ctx->hjoin) != hipSuccess) rc = -1;
     if (hipStreamDestroy(ctx->hstream) != hipSuccess) rc = -1;
     if (hipStreamDestroy(ctx->stream) != hipSuccess) rc = -1;
+    if (hipHostFree(ctx->hflag) != hipSuccess) rc = -1;
     delete ctx;
     return rc;
 }
@@ -303,6 +326,8 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     free_all(ctx);
+    ctx->hflag[0] = ctx->hflag[1] = 0;  // the device's round counter restarts with the upload
+    ctx->rounds_issued = 0;
     const int S = dsc->n_scenes, N = dsc->n_bodies;
     if (S <= 0 || N < 0) return fail(ctx, "empty scene batch");
     if (dsc->scene_body_off[0] != 0 || dsc->scene_body_off[S] != N) return fail(ctx, "scene_body_off inconsistent");
@@ -522,6 +547,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
         const nd::HelperCfg cfg{nd::kHelperWaitTicks, nd::kHelperEpochTicks, nd::kHelperIdleTicks,
                                 nd::kHelperStraggleTicks};
         w.hcfg = dupload(ctx, &cfg, 1);
+        w.round_seq = dalloc<unsigned int>(ctx, 1);
     }
     w.pair_cnt = dalloc<int>(ctx, S);
     w.pair_off = dalloc<int>(ctx, S + 1);
@@ -688,11 +714,15 @@ int32_t ndipc_get_joint_state(ndipc_ctx* ctx, double* lambda6, double* mu) {
     return 0;
 }
 
-// The kernel sequence of one IPCSolver::step for every scene. With `rec`, the data
-// the adjoint needs is recorded into slot k of the segment: the step-start state, the
-// objective of every Newton solve, the converged variable and the solve's iteration
-// count. Kernels only (no copies, fills or host work), except the recording copies.
-static int enqueue_step(ndipc_ctx* ctx, const nd::Adj* rec, int k, int alm_max) {
+// The kernel sequence of one IPCSolver::step for every scene, in three parts: the
+// prefix (candidates, stiffness control, CCD gate, first narrow phase, contact setup),
+// one ALM round (Newton, poses, narrow phase at the new iterate, ALM post-pass; scenes
+// whose ALM loop has ended skip through device flags) and the suffix (write-back).
+// With `rec`, the data the adjoint needs is recorded into slot k of the segment: the
+// step-start state, the objective of every Newton solve, the converged variable and the
+// solve's iteration count. Kernels only (no copies, fills or host work), except the
+// recording copies.
+static int enqueue_prefix(ndipc_ctx* ctx, const nd::Adj* rec, int k) {
     const Dev& d = ctx->d;
     const Work& w = ctx->w;
     hipStream_t st = ctx->stream;
@@ -723,39 +753,61 @@ static int enqueue_step(ndipc_ctx* ctx, const nd::Adj* rec, int k, int alm_max) 
     nd::launch_pose_from_state(d, w, nullptr, st);
     nd::launch_narrow_phase(d, w, mp, false, st);
     nd::launch_contacts_setup(d, w, st);
-    if (w.newton_helpers && alm_max > 0) {  // line-search helpers for the step's Newton launches
-        HIPCHK(hipEventRecord(ctx->hfork, st));
-        HIPCHK(hipStreamWaitEvent(ctx->hstream, ctx->hfork, 0));
-        nd::launch_newton_helpers(d, w, ctx->hstream);
-        HIPCHK(hipEventRecord(ctx->hjoin, ctx->hstream));
-    }
-    for (int a = 0; a < alm_max; ++a) {
-        if (rec) nd::launch_adj_snapshot(d, w, *rec, k * rec->A + a, st);
-        nd::launch_newton(d, w, st);
-        if (rec) nd::launch_adj_snapshot_post(d, w, *rec, k * rec->A + a, st);
-        nd::launch_pose_from_state(d, w, w.x, st);
-        nd::launch_narrow_phase(d, w, mp, true, st);
-        nd::launch_alm_post(d, w, st);
-    }
-    if (w.newton_helpers && alm_max > 0) {
-        nd::launch_helpers_close(d, w, st);
-        HIPCHK(hipStreamWaitEvent(st, ctx->hjoin, 0));
-    }
-    nd::launch_finalize(d, w, st);
     return 0;
 }
 
-// Captures one step (no recording, no timing) into a graph. The step's kernels take only
-// device pointers and sizes fixed at upload, and read dt and every per-step value from
-// device memory, so one capture serves every later step until the next upload or a
-// change of the ALM iteration count.
-static int capture_step(ndipc_ctx* ctx, int alm_max) {
-    drop_graph(ctx);
+// One ALM round; with `flag`, k_round_flag reports to the host whether every scene's
+// ALM loop has ended.
+static int enqueue_round(ndipc_ctx* ctx, const nd::Adj* rec, int k, int a, bool flag) {
+    const Dev& d = ctx->d;
+    const Work& w = ctx->w;
+    hipStream_t st = ctx->stream;
+    if (rec) nd::launch_adj_snapshot(d, w, *rec, k * rec->A + a, st);
+    nd::launch_newton(d, w, st);
+    if (rec) nd::launch_adj_snapshot_post(d, w, *rec, k * rec->A + a, st);
+    nd::launch_pose_from_state(d, w, w.x, st);
+    nd::launch_narrow_phase(d, w, max_grid_pairs(ctx), true, st);
+    nd::launch_alm_post(d, w, st);
+    if (flag) nd::launch_round_flag(d, w, ctx->hflag, st);
+    return 0;
+}
+
+// The line-search helpers of the step's Newton launches, on the second stream.
+static int fork_helpers(ndipc_ctx* ctx) {
+    HIPCHK(hipEventRecord(ctx->hfork, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->hstream, ctx->hfork, 0));
+    nd::launch_newton_helpers(ctx->d, ctx->w, ctx->hstream);
+    HIPCHK(hipEventRecord(ctx->hjoin, ctx->hstream));
+    return 0;
+}
+
+static int join_helpers(ndipc_ctx* ctx) {
+    nd::launch_helpers_close(ctx->d, ctx->w, ctx->stream);
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->hjoin, 0));
+    return 0;
+}
+
+// The whole step with a fixed number of ALM rounds (every round enqueued; scenes whose
+// loop ended skip through device flags): the adjoint's recorded path and the
+// fixed-round mode.
+static int enqueue_step(ndipc_ctx* ctx, const nd::Adj* rec, int k, int alm_max) {
+    const bool helpers = ctx->w.newton_helpers && alm_max > 0;
+    if (enqueue_prefix(ctx, rec, k)) return -1;
+    if (helpers && fork_helpers(ctx)) return -1;
+    for (int a = 0; a < alm_max; ++a)
+        if (enqueue_round(ctx, rec, k, a, false)) return -1;
+    if (helpers && join_helpers(ctx)) return -1;
+    nd::launch_finalize(ctx->d, ctx->w, ctx->stream);
+    return 0;
+}
+
+// Captures `fn` (kernels on the context stream only) into an executable graph.
+static int capture(ndipc_ctx* ctx, hipGraphExec_t* out, const std::function<int()>& fn) {
     hipStream_t st = ctx->stream;
     ndipc_ctx* saved = g_timing_ctx;
     g_timing_ctx = nullptr;  // no event records inside the graph
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_step(ctx, nullptr, 0, alm_max);
+    const int rc = fn();
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(st, &g);
     g_timing_ctx = saved;
@@ -764,29 +816,105 @@ static int capture_step(ndipc_ctx* ctx, int alm_max) {
         return rc;
     }
     if (ec != hipSuccess) return fail(ctx, std::string("step capture: ") + hipGetErrorString(ec));
-    const hipError_t ei = hipGraphInstantiate(&ctx->step_exec, g, nullptr, nullptr, 0);
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ei != hipSuccess) {
-        ctx->step_exec = nullptr;
+        *out = nullptr;
         return fail(ctx, std::string("step graph: ") + hipGetErrorString(ei));
+    }
+    return 0;
+}
+
+// Captures the graphs of the step. The step's kernels take only device pointers and
+// sizes fixed at upload, and read dt and every per-step value from device memory, so
+// one capture serves every later step until the next upload (or, for the fixed-round
+// whole-step graph, a change of the ALM iteration count).
+static int capture_step(ndipc_ctx* ctx, int alm_max) {
+    drop_graph(ctx);
+    if (ctx->adaptive) {
+        if (capture(ctx, &ctx->pre_exec, [&] { return enqueue_prefix(ctx, nullptr, 0); })) return -1;
+        if (capture(ctx, &ctx->round_exec, [&] { return enqueue_round(ctx, nullptr, 0, 0, true); })) return -1;
+    } else {
+        // the helpers' fork and join are part of the whole-step graph
+        if (capture(ctx, &ctx->step_exec, [&] { return enqueue_step(ctx, nullptr, 0, alm_max); })) return -1;
     }
     ctx->graph_alm = alm_max;
     return 0;
 }
 
-// The step sequence of IPCSolver::step for every scene, n_steps times: the captured
-// graph replayed per step, or the kernels enqueued one by one when recording for the
-// adjoint, when kernel timing is on, or when graphs are switched off.
+// Waits (host) until round `seq` has reported; *all_done: every scene's ALM loop ended.
+static int wait_round(ndipc_ctx* ctx, unsigned int seq, int* all_done) {
+    volatile unsigned int* f = ctx->hflag;
+    for (long long spin = 0;; ++spin) {
+        const unsigned int v = __atomic_load_n(&f[0], __ATOMIC_ACQUIRE);
+        if (int(v - seq) >= 0) {
+            *all_done = v == seq ? int(__atomic_load_n(&f[1], __ATOMIC_RELAXED)) : 0;
+            return 0;
+        }
+        if ((spin & 0xffff) == 0xffff) {  // a stream that has ended or failed will not report
+            const hipError_t q = hipStreamQuery(ctx->stream);
+            if (q == hipSuccess && int(__atomic_load_n(&f[0], __ATOMIC_ACQUIRE) - seq) < 0)
+                return fail(ctx, "ALM round flag: the stream ended without reporting the round");
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail(ctx, std::string("ALM round flag: ") + hipGetErrorString(q));
+        }
+        __builtin_ia32_pause();
+    }
+}
+
+// One step with as many ALM rounds as the slowest scene needs (at most alm_max): the
+// host issues rounds ahead of the device by `round_lookahead` and stops once a round
+// reports that every scene's ALM loop has ended; the rounds a scene runs and their
+// results are those of the fixed-round step (a round after the end is a no-op).
+static int run_step_adaptive(ndipc_ctx* ctx, int alm_max, bool graph) {
+    const bool helpers = ctx->w.newton_helpers && alm_max > 0;
+    hipStream_t st = ctx->stream;
+    if (graph) HIPCHK(hipGraphLaunch(ctx->pre_exec, st));
+    else if (enqueue_prefix(ctx, nullptr, 0)) return -1;
+    if (helpers && fork_helpers(ctx)) return -1;
+    const unsigned int base = ctx->rounds_issued;
+    int issued = 0;
+    auto issue = [&]() -> int {
+        if (graph) HIPCHK(hipGraphLaunch(ctx->round_exec, st));
+        else if (enqueue_round(ctx, nullptr, 0, 0, true)) return -1;
+        ++issued;
+        ++ctx->rounds_issued;
+        return 0;
+    };
+    while (issued < std::min(alm_max, 1 + ctx->round_lookahead))
+        if (issue()) return -1;
+    for (int r = 0; r < alm_max; ++r) {
+        int done = 0;
+        if (wait_round(ctx, base + unsigned(r) + 1u, &done)) return -1;
+        if (done) break;
+        if (issued < alm_max && issue()) return -1;
+    }
+    if (helpers && join_helpers(ctx)) return -1;
+    nd::launch_finalize(ctx->d, ctx->w, st);
+    return 0;
+}
+
+// The step sequence of IPCSolver::step for every scene, n_steps times. When recording
+// for the adjoint, or in fixed-round mode, every step enqueues all alm_max rounds (the
+// whole step replayed from one graph, or kernel by kernel with timing or graphs off);
+// otherwise each step issues only the rounds the scenes need (adaptive mode), from the
+// prefix and round graphs or kernel by kernel.
 static int run_steps(ndipc_ctx* ctx, int n_steps, const nd::Adj* rec) {
     int alm_max = 1;
     for (auto& q : ctx->sp) alm_max = std::max(alm_max, q.alm_max_iters);
     const bool graph = !rec && ctx->use_graph && !ctx->timing;
-    if (graph && (!ctx->step_exec || ctx->graph_alm != alm_max))
+    const bool have = ctx->adaptive ? ctx->pre_exec && ctx->round_exec : ctx->step_exec && ctx->graph_alm == alm_max;
+    if (graph && !have)
         if (capture_step(ctx, alm_max)) return -1;
     g_timing_ctx = ctx;
     for (int k = 0; k < n_steps; ++k) {
-        if (graph) HIPCHK(hipGraphLaunch(ctx->step_exec, ctx->stream));
-        else if (enqueue_step(ctx, rec, k, alm_max)) return -1;
+        if (!rec && ctx->adaptive) {
+            if (run_step_adaptive(ctx, alm_max, graph)) return -1;
+        } else if (graph) {
+            HIPCHK(hipGraphLaunch(ctx->step_exec, ctx->stream));
+        } else if (enqueue_step(ctx, rec, k, alm_max)) {
+            return -1;
+        }
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -1274,6 +1402,16 @@ int32_t ndipc_autotune(ndipc_ctx* ctx, const double* dt, int32_t trial_steps, in
 int32_t ndipc_use_graph(ndipc_ctx* ctx, int32_t on) {
     if (!ctx) return -1;
     ctx->use_graph = on != 0;
+    return 0;
+}
+
+int32_t ndipc_set_step_mode(ndipc_ctx* ctx, int32_t adaptive, int32_t lookahead) {
+    if (check_ctx(ctx)) return -1;
+    if (lookahead < 0) return fail(ctx, "set_step_mode: lookahead must be >= 0");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if ((adaptive != 0) != ctx->adaptive) drop_graph(ctx);
+    ctx->adaptive = adaptive != 0;
+    ctx->round_lookahead = lookahead;
     return 0;
 }
 

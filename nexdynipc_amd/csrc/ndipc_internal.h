@@ -202,6 +202,7 @@ struct Work {
     double* jgrad;           // [J*12]
     double* dense;           // [S][kDenseRows^2] component LDL^T factors
     int* flag;               // [S] scene-min flag scratch
+    unsigned int* round_seq; // [1] ALM rounds ended by k_round_flag since upload
     int* bcnt;               // [2N] per-body pair counts (candidates, prox)
     int* boff;               // [2N] per-body offsets within the scene
 };
@@ -288,6 +289,7 @@ void launch_newton_helpers(const Dev& d, const Work& w, hipStream_t st);
 void launch_helpers_close(const Dev& d, const Work& w, hipStream_t st);
 void launch_alm_post(const Dev& d, const Work& w, hipStream_t st);
 void launch_finalize(const Dev& d, const Work& w, hipStream_t st);
+void launch_round_flag(const Dev& d, const Work& w, unsigned int* hflag, hipStream_t st);
 void launch_barrier_init_ccd(const Dev& d, const Work& w, hipStream_t st);
 // which 0: DistanceBarrierForm, 1: FrictionForm, on the contacts stored for scene s
 void launch_forms_eval(const Dev& d, const Work& w, int s, int which, double dhat, double kappa, double mu,

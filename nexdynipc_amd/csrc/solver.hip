@@ -2424,6 +2424,30 @@ void launch_alm_post(const Dev& d, const Work& w, hipStream_t st) {
     timing_end("alm_post", st);
 }
 
+// End of one ALM round of a host-driven step: whether every scene's ALM loop has ended
+// (IPCSolver.cpp:1160-1165 break, or the iteration cap), written to host-visible memory
+// as (round number since upload, all ended), the number stored last with release, so
+// the host can stop issuing rounds once every scene is done. One work-group.
+__global__ void __launch_bounds__(256) k_round_flag(Dev d, Work w, unsigned int* hflag) {
+    __shared__ int all;
+    if (threadIdx.x == 0) all = 1;
+    __syncthreads();
+    int mine = 1;
+    for (int s = threadIdx.x; s < d.S; s += blockDim.x) mine &= d.sp[s].alm_done;
+    if (!mine) atomicAnd(&all, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int seq = w.round_seq[0] + 1u;
+        w.round_seq[0] = seq;
+        __hip_atomic_store(hflag + 1, (unsigned int)all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(hflag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+void launch_round_flag(const Dev& d, const Work& w, unsigned int* hflag, hipStream_t st) {
+    hipLaunchKernelGGL(k_round_flag, dim3(1), dim3(256), 0, st, d, w, hflag);
+}
+
 // ------------------------------------------------------------------ step prologue
 __global__ void k_step_begin(Dev d, Work w, const double* dt) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;

@@ -334,6 +334,38 @@ def test_step_graph_replays_match_eager_steps(engine_factory, name):
         ee.set_state(*se)
 
 
+@pytest.mark.parametrize("name", ["c1", "micro", "c3x8", "c4"])
+def test_adaptive_rounds_match_fixed_rounds(engine_factory, name):
+    """Adaptive mode issues ALM rounds only until every scene's ALM loop has ended (the
+    rounds report to pinned host memory); fixed mode enqueues all alm_max rounds, the
+    finished scenes skipping them on the device. Same bits and counters, with graphs
+    and kernel by kernel, one round or none issued ahead. c4's hinges run all 10 rounds."""
+    if name in ("c1", "c4"):
+        sc = scenes.make_config(name)
+    elif name == "micro":
+        sc = scenes.pack([scenes.micro_contact_scene(seed=3, n_cubes=4)])
+    else:
+        sc = scenes.pack(scenes.config3(n_scenes=8))
+    fixed = engine_factory(sc)
+    fixed.set_step_mode(adaptive=False)
+    runs = []
+    for graph, look in ((True, 1), (True, 0), (False, 1)):
+        e = engine_factory(sc)
+        e.set_step_mode(adaptive=True, lookahead=look)
+        e.use_graph(graph)
+        runs.append(e)
+    for chunk in (1, 4, 2):
+        fixed.step(chunk)
+        ref = fixed.get_state()
+        for e in runs:
+            e.step(chunk)
+            for a, b in zip(e.get_state(), ref):
+                assert np.array_equal(a, b)
+            assert np.array_equal(e.stats(), fixed.stats())
+            assert np.array_equal(e.solver_counters(), fixed.solver_counters())
+            assert np.array_equal(e.joint_state()[0], fixed.joint_state()[0])
+
+
 @pytest.mark.parametrize("name", ["tie", "c3x8"])
 def test_broad_phase_tree_in_hbm_matches_tree_in_lds(engine_factory, name):
     """The step's broad phase traverses a copy of each scene's tree in LDS when the
