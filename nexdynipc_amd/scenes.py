@@ -202,6 +202,9 @@ class Scenes:
     def n_joints(self):
         return int(self.scene_joint_off[-1])
 
+    def n_joints_in(self, s):
+        return int(self.scene_joint_off[s + 1] - self.scene_joint_off[s])
+
     def scene_slice(self, s):
         return slice(int(self.scene_body_off[s]), int(self.scene_body_off[s + 1]))
 

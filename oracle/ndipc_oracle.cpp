@@ -392,6 +392,25 @@ static M3 friction_hess(const V3& u, double N, double mu, double eps) {
     return r;
 }
 
+// AdaptiveBarrier::initialize / updateStiffness (AdaptiveBarrier.h:27-37) with the
+// default AdaptiveBarrierParams (initial dhat 1e-3, dhat_epsilon 1e-9, min stiffness
+// scale 1e11, growth 2, max stiffness 1e20 unless given).
+struct AdaptiveBarrierO {
+    double kappa = 0.0, dhat = 1e-3, kappa_min = 0.0, max_stiffness = 1e20;
+    void initialize(double average_mass, double bbox_diagonal) {
+        kappa_min = 1e11 * average_mass / (bbox_diagonal * bbox_diagonal);
+        kappa = kappa_min;
+        dhat = 1e-3;
+    }
+    bool update(double min_distance) {
+        if (min_distance < 1e-9 && kappa < max_stiffness) {
+            kappa = std::min(kappa * 2.0, max_stiffness);
+            return true;
+        }
+        return false;
+    }
+};
+
 // Simple triplet accumulator (row, col) -> value, summed in insertion order.
 struct SymAccum {
     int n = 0;
@@ -507,7 +526,20 @@ struct Solver {
     // Recording for the CPU adjoint (adjoint_sweep): every Newton solve of every step
     // with the data its objective reads, the converged x, and the joint stiffness after
     // the hardening that follows it.
+    // Optional record of a Newton solve's iterations (tests pin the Newton loop, the
+    // linear solve and the line search against it): per iteration the iterate, the
+    // solved direction, f(x), the slopes, the contact value and every trial.
+    struct LsTrial {
+        double alpha, fn, fcn;
+    };
+    struct NewtonRec {
+        vector<double> x, dx;
+        double fx = 0, descent = 0, fc = 0, cdesc = 0;
+        vector<LsTrial> trials;
+    };
+    mutable vector<NewtonRec>* newton_rec = nullptr;
     struct AdjSolve {
+        vector<NewtonRec> iters;  // filled when newton_rec recording is on
         vector<double> x_star;
         vector<Joint> joints;  // multipliers before the solve, stiffness of the solve
         vector<double> mu_post;
@@ -525,10 +557,13 @@ struct Solver {
         vector<AdjSolve> solves;
     };
     vector<AdjStep>* adj_rec = nullptr;
+    bool record_newton = false;  // with adj_rec: also record the Newton iterations
 
     // CPU-baseline sampling of computeSceneMinDistance: only every k-th vertex of the
     // outer loop is evaluated and the measured time is scaled by k (bench only).
     int sample_stride = 1;
+    bool prune_scene_min = false;  // exact for the 1e-9 comparison (scene_min_distance)
+    bool narrow_raw = false;       // test hook: narrow_phase skips the 2000-contact cap
     double t_scene_min = 0, t_step = 0;
 
     // statistics of the last step
@@ -733,7 +768,7 @@ struct Solver {
                 append(c[0], c[1], 0, c[2], c[3], 0.5 * (a + b), d, dist);
             }
         }
-        if (contacts.size() > kMax) {
+        if (contacts.size() > kMax && !narrow_raw) {
             std::sort(contacts.begin(), contacts.end(), [](const Contact& a, const Contact& b) { return a.dist < b.dist; });
             contacts.resize(kMax);
         }
@@ -1094,12 +1129,15 @@ struct Solver {
         const double descent = nd::eigen_vec_sum(int(g.size()), [&](int i) { return g[i] * dir[i]; });
         const double fc = contact_value(x);
         const double cdesc = nd::eigen_vec_sum(int(cg.size()), [&](int i) { return cg[i] * dir[i]; });
+        NewtonRec* nr = newton_rec && !newton_rec->empty() ? &newton_rec->back() : nullptr;
+        if (nr) { nr->fx = fx; nr->descent = descent; nr->fc = fc; nr->cdesc = cdesc; }
         if (descent > 0) return 0.0;
         vector<double> xn(x.size());
         for (int it = 0; it < 20; ++it) {
             for (size_t i = 0; i < x.size(); ++i) xn[i] = x[i] + alpha * dir[i];
             const double fn = obj_value(xn);
             const double fcn = contact_value(xn);
+            if (nr) nr->trials.push_back({alpha, fn, fcn});
             const bool cok = (cdesc <= 0.0) ? (fcn <= fc + 1e-4 * alpha * cdesc) : (fcn <= fc);
             if (fn <= fx + 1e-4 * alpha * descent && cok) return alpha;
             alpha *= 0.5;
@@ -1124,6 +1162,12 @@ struct Solver {
             for (size_t i = 0; i < g.size(); ++i) ng[i] = -g[i];
             ldlt.solve(ng, dx);
             contact_grad(x, cg);
+            if (newton_rec) {
+                NewtonRec nr;
+                nr.x = x;
+                nr.dx = dx;
+                newton_rec->push_back(std::move(nr));
+            }
             const double alpha = line_search(x, dx, g, cg, std::min(1.0, 1.0));
             if (alpha == 0.0) return false;
             for (size_t i = 0; i < x.size(); ++i) x[i] = x[i] + alpha * dx[i];
@@ -1141,15 +1185,37 @@ struct Solver {
         const double avg = tm / double(bodies.size());
         double diag = norm(mx - mn);
         if (diag < 1e-6) diag = 1.0;
-        kappa_min = 1e11 * avg / (diag * diag);
-        kappa = kappa_min;
-        dhat = 1e-3;
+        AdaptiveBarrierO ab;
+        ab.initialize(avg, diag);
+        kappa_min = ab.kappa_min;
+        kappa = ab.kappa;
+        dhat = ab.dhat;
     }
 
-    // computeMinDistance / computeSceneMinDistance (AdaptiveBarrier.h:89-162)
+    // computeMinDistance / computeSceneMinDistance (AdaptiveBarrier.h:89-162). With
+    // prune_scene_min the body pairs whose world boxes are farther apart than
+    // 1e-8 + 1e-5 * (longest edge of either mesh) are skipped: a vertex-face distance
+    // below 1e-9 with barycentric coordinates inside [-1e-6, 1 + 1e-6] puts the vertex
+    // within 1e-9 + 2e-6 * (longest edge) of the face, so the result's comparison with
+    // 1e-9 (IPCSolver.cpp:824-831) is unchanged; only the value of larger minima may
+    // differ (+inf when every pair is skipped).
     double scene_min_distance() const {
         double md = std::numeric_limits<double>::infinity();
         long long vcount = 0;
+        vector<AABB> wb;
+        vector<double> emax;
+        if (prune_scene_min) {
+            for (int i = 0; i < n(); ++i) {
+                const Body& b = bodies[i];
+                if (b.mesh < 0) { wb.emplace_back(); emax.push_back(0.0); continue; }
+                const Mesh& M = meshes[b.mesh];
+                wb.push_back(transform_local_aabb(M.lmin, M.lmax, b.p, b.q));
+                double e = 0.0;
+                for (auto& f : M.F)
+                    for (int k = 0; k < 3; ++k) e = std::max(e, norm(M.V[f[(k + 1) % 3]] - M.V[f[k]]));
+                emax.push_back(e);
+            }
+        }
         for (int i = 0; i < n(); ++i) {
             if (bodies[i].mesh < 0) continue;
             const Mesh& MA = meshes[bodies[i].mesh];
@@ -1158,6 +1224,8 @@ struct Solver {
                 const V3 p = bodies[i].p + rotate(bodies[i].q, MA.V[vi]);
                 for (int j = i + 1; j < n(); ++j) {
                     if (bodies[j].mesh < 0) continue;
+                    if (prune_scene_min && aabb_distance(wb[i], wb[j]) > 1e-8 + 1e-5 * std::max(emax[i], emax[j]))
+                        continue;
                     const Mesh& MB = meshes[bodies[j].mesh];
                     vector<V3> VB(MB.V.size());
                     for (size_t k = 0; k < VB.size(); ++k) VB[k] = bodies[j].p + rotate(bodies[j].q, MB.V[k]);
@@ -1225,7 +1293,8 @@ struct Solver {
                 t_scene_min += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() *
                                double(sample_stride);
                 if (md < std::numeric_limits<double>::infinity()) {
-                    if (md < 1e-9 && kappa < 1e20) { kappa = std::min(kappa * 2.0, 1e20); st_kappa = 1; }
+                    AdaptiveBarrierO ab{kappa, dhat, kappa_min, 1e20};
+                    if (ab.update(md)) { kappa = ab.kappa; st_kappa = 1; }
                 }
             }
         }
@@ -1308,7 +1377,9 @@ struct Solver {
                 rec_solve.has_barrier = has_barrier; rec_solve.has_friction = has_friction;
             }
             int nit = 0;
+            if (adj_rec && record_newton) newton_rec = &rec_solve.iters;
             const bool conv = newton(x_new, nit);
+            newton_rec = nullptr;
             st_newton += nit + 1;
             double viol = 0.0, dual = 0.0;
             for (auto& j : joints) {  // IPCSolver.cpp:1084-1108
@@ -1796,6 +1867,8 @@ using namespace ndo;
 namespace {
 struct Handle {
     Solver s;
+    std::vector<Solver::AdjStep> rec;  // ndo_record_steps
+    Solver probe;                      // a recorded solve's objective (ndo_record_objective)
 };
 Solver& S(void* h) { return static_cast<Handle*>(h)->s; }
 
@@ -2190,6 +2263,188 @@ int32_t ndo_elemfn(int32_t fn, const double* x, const double* y, int32_t n, doub
             default: return -1;
         }
     }
+    return 0;
+}
+
+// ---- function-level pins of the narrow phase's primitives (IPCSolver.cpp:32-134)
+int32_t ndo_closest_points_segments(const double* s12, double* out6) {
+    V3 c1, c2;
+    closest_points_on_segments(V3(s12[0], s12[1], s12[2]), V3(s12[3], s12[4], s12[5]), V3(s12[6], s12[7], s12[8]),
+                               V3(s12[9], s12[10], s12[11]), c1, c2);
+    out6[0] = c1.x; out6[1] = c1.y; out6[2] = c1.z;
+    out6[3] = c2.x; out6[4] = c2.y; out6[5] = c2.z;
+    return 0;
+}
+
+int32_t ndo_point_in_triangle(const double* p, const double* tri9) {
+    return point_in_triangle(V3(p[0], p[1], p[2]), V3(tri9[0], tri9[1], tri9[2]), V3(tri9[3], tri9[4], tri9[5]),
+                             V3(tri9[6], tri9[7], tri9[8])) ? 1 : 0;
+}
+
+int32_t ndo_safe_normalized(const double* v, const double* fb, double* out3) {
+    const V3 r = safe_normalized(V3(v[0], v[1], v[2]), V3(fb[0], fb[1], fb[2]));
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+    return 0;
+}
+
+// ---- BVH over given boxes (BVH.cpp), broad phase with inflation (BroadPhase.cpp)
+static vector<AABB> boxes_in(const double* b6, int32_t n) {
+    vector<AABB> v;
+    for (int i = 0; i < n; ++i)
+        v.emplace_back(V3(b6[6 * i], b6[6 * i + 1], b6[6 * i + 2]), V3(b6[6 * i + 3], b6[6 * i + 4], b6[6 * i + 5]));
+    return v;
+}
+
+int32_t ndo_bvh_query(const double* boxes6, int32_t n, const double* q6, int32_t* out, int32_t cap) {
+    BVH bvh;
+    bvh.init(boxes_in(boxes6, n));
+    vector<unsigned> hits;
+    bvh.query(AABB(V3(q6[0], q6[1], q6[2]), V3(q6[3], q6[4], q6[5])), hits);
+    for (int i = 0; i < int(hits.size()) && i < cap; ++i) out[i] = int32_t(hits[i]);
+    return int32_t(hits.size());
+}
+
+int32_t ndo_broad_phase_boxes(const double* boxes6, int32_t n, double inflation, int32_t* out_pairs, int32_t cap) {
+    auto b = boxes_in(boxes6, n);
+    for (auto& x : b) x = x.inflated(inflation);
+    const auto pr = broad_phase_detect(b);
+    for (int i = 0; i < int(pr.size()) && i < cap; ++i) {
+        out_pairs[2 * i] = pr[i].first;
+        out_pairs[2 * i + 1] = pr[i].second;
+    }
+    return int32_t(pr.size());
+}
+
+// ---- interval operations of the CCD (Interval.h): op 0 a+b, 1 a-b, 2 a*b, 3 width(a),
+// 4 median(a), 5 zero_in(a)
+int32_t ndo_interval_op(int32_t op, const double* a2, const double* b2, double* out2) {
+    const ndo::I a(a2[0], a2[1]), b(b2[0], b2[1]);
+    ndo::I r;
+    switch (op) {
+        case 0: r = a + b; break;
+        case 1: r = a - b; break;
+        case 2: r = a * b; break;
+        case 3: out2[0] = out2[1] = ndo::width(a); return 0;
+        case 4: out2[0] = out2[1] = ndo::median(a); return 0;
+        case 5: out2[0] = out2[1] = ndo::zero_in(a) ? 1.0 : 0.0; return 0;
+        default: return -1;
+    }
+    out2[0] = r.lo;
+    out2[1] = r.hi;
+    return 0;
+}
+
+// ---- AdaptiveBarrier: initialize(avg_mass, diag), then updateStiffness(min_dist[k]);
+// kappas[0] after initialize, kappas[k + 1] after update k.
+int32_t ndo_adaptive_barrier(double avg_mass, double diag, double max_stiffness, int32_t n, const double* min_dist,
+                             double* kappas, double* kappa_min) {
+    AdaptiveBarrierO ab;
+    if (max_stiffness > 0) ab.max_stiffness = max_stiffness;
+    ab.initialize(avg_mass, diag);
+    kappas[0] = ab.kappa;
+    for (int k = 0; k < n; ++k) {
+        ab.update(min_dist[k]);
+        kappas[k + 1] = ab.kappa;
+    }
+    *kappa_min = ab.kappa_min;
+    return 0;
+}
+
+int32_t ndo_set_narrow_raw(void* h, int32_t on) {
+    S(h).narrow_raw = on != 0;
+    return 0;
+}
+
+int32_t ndo_set_scene_min_pruned(void* h, int32_t on) {
+    S(h).prune_scene_min = on != 0;
+    return 0;
+}
+
+// ---- recorded steps: every Newton solve (and its iterations) of n_steps steps
+int32_t ndo_record_steps(void* h, double dt, int32_t n_steps) {
+    Handle& H = *static_cast<Handle*>(h);
+    H.rec.clear();
+    H.s.adj_rec = &H.rec;
+    H.s.record_newton = true;
+    for (int k = 0; k < n_steps; ++k) H.s.step(dt);
+    H.s.adj_rec = nullptr;
+    H.s.record_newton = false;
+    return 0;
+}
+
+// counts: [steps]; with step >= 0: [solves of the step]; with solve >= 0 also
+// [converged, newton iterations reported, recorded iterations, trials of iteration it]
+int32_t ndo_record_info(void* h, int32_t step, int32_t solve, int32_t it, int32_t* out4) {
+    Handle& H = *static_cast<Handle*>(h);
+    out4[0] = int32_t(H.rec.size());
+    if (step < 0) return 0;
+    if (step >= int(H.rec.size())) return -1;
+    const auto& st = H.rec[step];
+    out4[0] = int32_t(st.solves.size());
+    if (solve < 0) return 0;
+    if (solve >= int(st.solves.size())) return -1;
+    const auto& so = st.solves[solve];
+    out4[0] = so.converged ? 1 : 0;
+    out4[1] = so.newton_iters;
+    out4[2] = int32_t(so.iters.size());
+    out4[3] = (it >= 0 && it < int(so.iters.size())) ? int32_t(so.iters[it].trials.size()) : 0;
+    return 0;
+}
+
+// x_hat of the step and x* of the solve (6n each); per iteration it (>= 0): x and dx (6n
+// each, into x_hat / x_star), scal[4] = f(x), g.dx, contact value, cg.dx and trials[3k]
+// = alpha, f, contact value of every trial. Joint state before the solve: lambda6 (6J)
+// and mu (J); after: mu_post (J).
+int32_t ndo_record_solve(void* h, int32_t step, int32_t solve, int32_t it, double* x_hat, double* x_star,
+                         double* scal, double* trials, double* lambda6, double* mu, double* mu_post) {
+    Handle& H = *static_cast<Handle*>(h);
+    if (step < 0 || step >= int(H.rec.size())) return -1;
+    const auto& st = H.rec[step];
+    if (solve < 0 || solve >= int(st.solves.size())) return -1;
+    const auto& so = st.solves[solve];
+    if (it < 0) {
+        if (x_hat) std::copy(st.x_hat.begin(), st.x_hat.end(), x_hat);
+        if (x_star) std::copy(so.x_star.begin(), so.x_star.end(), x_star);
+    } else {
+        if (it >= int(so.iters.size())) return -1;
+        const auto& nr = so.iters[it];
+        if (x_hat) std::copy(nr.x.begin(), nr.x.end(), x_hat);
+        if (x_star) std::copy(nr.dx.begin(), nr.dx.end(), x_star);
+        if (scal) { scal[0] = nr.fx; scal[1] = nr.descent; scal[2] = nr.fc; scal[3] = nr.cdesc; }
+        if (trials)
+            for (size_t k = 0; k < nr.trials.size(); ++k) {
+                trials[3 * k] = nr.trials[k].alpha;
+                trials[3 * k + 1] = nr.trials[k].fn;
+                trials[3 * k + 2] = nr.trials[k].fcn;
+            }
+    }
+    for (size_t j = 0; j < so.joints.size(); ++j) {
+        if (lambda6) for (int k = 0; k < 6; ++k) lambda6[6 * j + k] = so.joints[j].lambda[k];
+        if (mu) mu[j] = so.joints[j].mu;
+        if (mu_post) mu_post[j] = so.mu_post[j];
+    }
+    return 0;
+}
+
+// The objective of a recorded solve at x (value, gradient, dense Hessian, contact value
+// and contact gradient; any may be NULL) and the joints' constraints C (6J).
+int32_t ndo_record_objective(void* h, int32_t step, int32_t solve, const double* x, double* value, double* grad,
+                             double* hess, double* cvalue, double* cgrad, double* jointC) {
+    Handle& H = *static_cast<Handle*>(h);
+    if (step < 0 || step >= int(H.rec.size())) return -1;
+    if (solve < 0 || solve >= int(H.rec[step].solves.size())) return -1;
+    H.probe = H.s;
+    H.probe.adj_rec = nullptr;
+    H.probe.adj_load(H.rec[step], H.rec[step].solves[solve]);
+    const Solver& P = H.probe;
+    std::vector<double> xv(x, x + size_t(P.n()) * 6);
+    if (value) *value = P.obj_value(xv);
+    if (grad) { std::vector<double> g; P.obj_grad(xv, g); std::copy(g.begin(), g.end(), grad); }
+    if (hess) { SymAccum A; P.obj_hess(xv, A); dense_out(A, hess); }
+    if (cvalue) *cvalue = P.contact_value(xv);
+    if (cgrad) { std::vector<double> g; P.contact_grad(xv, g); std::copy(g.begin(), g.end(), cgrad); }
+    if (jointC)
+        for (size_t j = 0; j < P.joints.size(); ++j) P.joint_C(P.joints[j], xv, jointC + 6 * j);
     return 0;
 }
 

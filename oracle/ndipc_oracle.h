@@ -128,6 +128,35 @@ int32_t ndo_adjoint(void* h, double dt, int32_t n_steps, const double* dl_dp, co
                     const double* dl_dv, const double* dl_dw, double* g_p0, double* g_q0, double* g_v0,
                     double* g_w0, double* g_mass, double* g_mu);
 
+/* Primitives of the narrow phase (IPCSolver.cpp:32-134): closest points of segments
+ * (p1 q1 p2 q2 -> c1 c2), barycentric point-in-triangle (1/0), safeNormalized. */
+int32_t ndo_closest_points_segments(const double* s12, double* out6);
+int32_t ndo_point_in_triangle(const double* p, const double* tri9);
+int32_t ndo_safe_normalized(const double* v, const double* fb, double* out3);
+/* BVH::intersectBox over given boxes (min, max per box; returns the hit count) and
+ * BVHBroadPhase::detectWithInflation (returns the pair count). */
+int32_t ndo_bvh_query(const double* boxes6, int32_t n, const double* q6, int32_t* out, int32_t cap);
+int32_t ndo_broad_phase_boxes(const double* boxes6, int32_t n, double inflation, int32_t* out_pairs, int32_t cap);
+/* Interval operations of the CCD: op 0 a+b, 1 a-b, 2 a*b, 3 width(a), 4 median(a),
+ * 5 zero_in(a); intervals as (lo, hi). */
+int32_t ndo_interval_op(int32_t op, const double* a2, const double* b2, double* out2);
+/* AdaptiveBarrier::initialize(avg_mass, diag) then updateStiffness(min_dist[k]);
+ * kappas[n+1]; max_stiffness <= 0: the default 1e20. */
+int32_t ndo_adaptive_barrier(double avg_mass, double diag, double max_stiffness, int32_t n, const double* min_dist,
+                             double* kappas, double* kappa_min);
+/* Test hook: the narrow phase returns its raw contacts (up to 40000), without the
+ * 2000-contact cap (on = 1). */
+int32_t ndo_set_narrow_raw(void* h, int32_t on);
+/* computeSceneMinDistance with exact box pruning for its 1e-9 comparison (on = 1). */
+int32_t ndo_set_scene_min_pruned(void* h, int32_t on);
+/* n_steps steps of dt with every Newton solve and iteration recorded. */
+int32_t ndo_record_steps(void* h, double dt, int32_t n_steps);
+int32_t ndo_record_info(void* h, int32_t step, int32_t solve, int32_t it, int32_t* out4);
+int32_t ndo_record_solve(void* h, int32_t step, int32_t solve, int32_t it, double* x_hat, double* x_star,
+                         double* scal, double* trials, double* lambda6, double* mu, double* mu_post);
+int32_t ndo_record_objective(void* h, int32_t step, int32_t solve, const double* x, double* value, double* grad,
+                             double* hess, double* cvalue, double* cgrad, double* jointC);
+
 /* Newton system at x (SimulationProblem with the current contact forms): */
 int32_t ndo_objective(void* h, const double* x, double* value, double* grad, double* hess);
 

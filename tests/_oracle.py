@@ -107,6 +107,22 @@ def lib():
         L.ndo_set_newton.argtypes = [C.c_void_p, C.c_double, C.c_int32]
         L.ndo_set_mass.restype = C.c_int32
         L.ndo_set_mass.argtypes = [C.c_void_p, f64p]
+        for name, args in (
+                ("ndo_closest_points_segments", [f64p, f64p]),
+                ("ndo_point_in_triangle", [f64p, f64p]),
+                ("ndo_safe_normalized", [f64p, f64p, f64p]),
+                ("ndo_bvh_query", [f64p, C.c_int32, f64p, i32p, C.c_int32]),
+                ("ndo_broad_phase_boxes", [f64p, C.c_int32, C.c_double, i32p, C.c_int32]),
+                ("ndo_interval_op", [C.c_int32, f64p, f64p, f64p]),
+                ("ndo_adaptive_barrier", [C.c_double, C.c_double, C.c_double, C.c_int32, f64p, f64p, f64p]),
+                ("ndo_set_scene_min_pruned", [C.c_void_p, C.c_int32]),
+                ("ndo_set_narrow_raw", [C.c_void_p, C.c_int32]),
+                ("ndo_record_steps", [C.c_void_p, C.c_double, C.c_int32]),
+                ("ndo_record_info", [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p]),
+                ("ndo_record_solve", [C.c_void_p, C.c_int32, C.c_int32, C.c_int32] + [f64p] * 7),
+                ("ndo_record_objective", [C.c_void_p, C.c_int32, C.c_int32] + [f64p] * 7)):
+            getattr(L, name).restype = C.c_int32
+            getattr(L, name).argtypes = args
         L.ndo_adjoint.restype = C.c_int32
         L.ndo_adjoint.argtypes = [C.c_void_p, C.c_double, C.c_int32] + [f64p] * 10
         _lib = L
@@ -309,6 +325,51 @@ class OracleScene:
                                  mu, eps, dt, C.byref(val), ptr(g, f64p), ptr(H, f64p))
         return val.value, g, H
 
+    def set_narrow_raw(self, on=True):
+        self.L.ndo_set_narrow_raw(self.h, 1 if on else 0)
+
+    def set_scene_min_pruned(self, on=True):
+        self.L.ndo_set_scene_min_pruned(self.h, 1 if on else 0)
+
+    # ---- recorded Newton solves (ndo_record_steps)
+    def record_steps(self, dt, n_steps):
+        self.L.ndo_record_steps(self.h, float(dt), int(n_steps))
+
+    def record_info(self, step=-1, solve=-1, it=-1):
+        o = np.zeros(4, dtype=np.int32)
+        assert self.L.ndo_record_info(self.h, step, solve, it, ptr(o, i32p)) == 0
+        return o
+
+    def record_solve(self, step, solve):
+        nd, J = 6 * self.n, max(self.sc.n_joints_in(self.scene), 1)
+        xh, xs = np.zeros(nd), np.zeros(nd)
+        lam, mu, mup = np.zeros(6 * J), np.zeros(J), np.zeros(J)
+        assert self.L.ndo_record_solve(self.h, step, solve, -1, ptr(xh, f64p), ptr(xs, f64p), None, None,
+                                       ptr(lam, f64p), ptr(mu, f64p), ptr(mup, f64p)) == 0
+        info = self.record_info(step, solve)
+        return dict(x_hat=xh, x_star=xs, converged=bool(info[0]), newton_iters=int(info[1]), n_iters=int(info[2]),
+                    lam=lam.reshape(-1, 6), mu=mu, mu_post=mup)
+
+    def record_iteration(self, step, solve, it):
+        nd = 6 * self.n
+        x, dx, scal = np.zeros(nd), np.zeros(nd), np.zeros(4)
+        nt = int(self.record_info(step, solve, it)[3])
+        tr = np.zeros(3 * max(nt, 1))
+        assert self.L.ndo_record_solve(self.h, step, solve, it, ptr(x, f64p), ptr(dx, f64p), ptr(scal, f64p),
+                                       ptr(tr, f64p), None, None, None) == 0
+        return dict(x=x, dx=dx, fx=scal[0], descent=scal[1], fc=scal[2], cdesc=scal[3],
+                    trials=tr[: 3 * nt].reshape(-1, 3))
+
+    def record_objective(self, step, solve, x, hess=False):
+        nd, J = 6 * self.n, max(self.sc.n_joints_in(self.scene), 1)
+        v, cv = C.c_double(), C.c_double()
+        g, cg, jc = np.zeros(nd), np.zeros(nd), np.zeros(6 * J)
+        H = np.zeros((nd, nd)) if hess else None
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert self.L.ndo_record_objective(self.h, step, solve, ptr(x, f64p), C.byref(v), ptr(g, f64p),
+                                           ptr(H, f64p), C.byref(cv), ptr(cg, f64p), ptr(jc, f64p)) == 0
+        return dict(f=v.value, g=g, H=H, fc=cv.value, cg=cg, C=jc.reshape(-1, 6))
+
     def ccd_earliest_toi(self, dt):
         o = C.c_double()
         self.L.ndo_ccd_earliest_toi(self.h, dt, C.byref(o))
@@ -380,3 +441,66 @@ def elemfn(fn, x, y=None):
     out = np.zeros(len(x))
     assert L.ndo_elemfn(ELEMFN.index(fn), ptr(x, f64p), ptr(yy, f64p), len(x), ptr(out, f64p)) == 0
     return out
+
+
+def closest_points_segments(p1, q1, p2, q2):
+    L = lib()
+    a = np.ascontiguousarray(np.concatenate([p1, q1, p2, q2]), dtype=np.float64)
+    o = np.zeros(6)
+    L.ndo_closest_points_segments(ptr(a, f64p), ptr(o, f64p))
+    return o[:3], o[3:]
+
+
+def point_in_triangle(p, a, b, c):
+    L = lib()
+    pp = np.ascontiguousarray(p, dtype=np.float64)
+    t = np.ascontiguousarray(np.concatenate([a, b, c]), dtype=np.float64)
+    return bool(L.ndo_point_in_triangle(ptr(pp, f64p), ptr(t, f64p)))
+
+
+def safe_normalized(v, fb):
+    L = lib()
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    b = np.ascontiguousarray(fb, dtype=np.float64)
+    o = np.zeros(3)
+    L.ndo_safe_normalized(ptr(a, f64p), ptr(b, f64p), ptr(o, f64p))
+    return o
+
+
+def bvh_query(boxes, qmin, qmax):
+    L = lib()
+    b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1)
+    q = np.ascontiguousarray(np.concatenate([qmin, qmax]), dtype=np.float64)
+    n = len(b) // 6
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    m = L.ndo_bvh_query(ptr(b, f64p), n, ptr(q, f64p), ptr(out, i32p), n)
+    return out[:m].tolist()
+
+
+def broad_phase_boxes(boxes, inflation=0.0):
+    L = lib()
+    b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1)
+    n = len(b) // 6
+    cap = max(n * n, 1)
+    out = np.zeros(2 * cap, dtype=np.int32)
+    m = L.ndo_broad_phase_boxes(ptr(b, f64p), n, float(inflation), ptr(out, i32p), cap)
+    return [tuple(p) for p in out[: 2 * m].reshape(-1, 2).tolist()]
+
+
+def interval_op(op, a, b=(0.0, 0.0)):
+    L = lib()
+    x = np.ascontiguousarray(a, dtype=np.float64)
+    y = np.ascontiguousarray(b, dtype=np.float64)
+    o = np.zeros(2)
+    assert L.ndo_interval_op(int(op), ptr(x, f64p), ptr(y, f64p), ptr(o, f64p)) == 0
+    return o
+
+
+def adaptive_barrier(avg_mass, diag, min_dists, max_stiffness=0.0):
+    L = lib()
+    m = np.ascontiguousarray(min_dists, dtype=np.float64)
+    k = np.zeros(len(m) + 1)
+    kmin = C.c_double()
+    L.ndo_adaptive_barrier(float(avg_mass), float(diag), float(max_stiffness), len(m), ptr(m, f64p), ptr(k, f64p),
+                           C.byref(kmin))
+    return k, kmin.value
