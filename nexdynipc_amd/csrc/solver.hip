@@ -995,7 +995,10 @@ __device__ bool pcg_solve(const Dev& d, const Work& w, const ScnCtx& sc, const d
     for (int i = threadIdx.x; i < nd; i += blockDim.x) p[i] = z[i];
     __syncthreads();
     double rz = wg_dot(r, z, nd, red);
-    for (int it = 0; it < 400; ++it) {
+    // at least twice the rows (exact arithmetic ends within nd iterations), so stiff
+    // chains reach the 1e-14 relative residual instead of the 1e-10 fallback
+    const int max_it = max(400, 2 * nd);
+    for (int it = 0; it < max_it; ++it) {
         if (iters) *iters = it + 1;
         spmv(d, w, sc, b2arr, p, Ap);
         __syncthreads();

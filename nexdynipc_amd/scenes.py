@@ -468,6 +468,25 @@ def micro_contact_scene(seed=0, n_cubes=4, size=6e-4, gap=2e-4, dhat=1e-3, mu=0.
     return SceneSpec(bodies, dt=1e-3, dhat=dhat, mu=mu, name=f"micro{seed}")
 
 
+def friction_slide_scene(n_cubes=2, size=6e-4, gap=2e-4, mass=1e-3, vx=0.05, mu=0.4) -> Scenes:
+    """Millimetre cubes sliding along +x (and a little +z) just above a static plate,
+    centre separations below dhat, with masses raised to `mass` (1 mg by default, not
+    the density's 0.2 mg) so that their weight keeps the contact residual above the ALM
+    break tolerance (IPCSolver.cpp:1132-1165): the ALM loop then runs several iterations
+    and the lagged friction, zero in the first, acts in the later ones. This makes the
+    friction coefficient's gradient non-zero (a differentiation test scene)."""
+    bodies = [_box_body("mplate", 0.01, size, 0.01, [0.0, -0.5 * size, 0.0], static=True)]
+    for k in range(n_cubes):
+        bodies.append(_box_body(f"mcube{size}", size, size, size,
+                                [(k - 0.5 * (n_cubes - 1)) * (size + 4e-4), 0.5 * size + gap, 0.0],
+                                v=np.array([vx * (1 + k), 0.0, 0.3 * vx])))
+    sc = pack([SceneSpec(bodies, dt=1e-3, dhat=1e-3, mu=mu, name="slide")])
+    m0 = float(sc.mass[1])
+    sc.mass[1:] = mass
+    sc.inertia.reshape(-1, 3, 3)[1:] *= mass / m0
+    return sc
+
+
 def hinge_chain_scene(n_links=3, joint_mu=1e4, seed=5) -> Scenes:
     """A free chain of 0.4 x 0.1 x 0.1 m links joined by hinges about z, with seeded
     initial velocities N(0, 0.3) and angular velocities N(0, 1), no static body. The

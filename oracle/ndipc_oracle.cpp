@@ -579,6 +579,7 @@ struct Solver {
     // ContactAssembler::build (ContactAssembler.cpp:124-154)
     void assembler_build() {
         bcm.clear();
+        id_bcm_n = size_t(-1);
         for (int i = 0; i < n(); ++i) {
             const Body& b = bodies[i];
             if (b.mesh < 0) continue;
@@ -675,13 +676,27 @@ struct Solver {
         }
     }
 
+    // id -> world index / contact mesh (the first body with the id; ids are unique in
+    // every scene here), hashed like the reference's maps (IPCSolver.cpp:401-410)
+    mutable std::unordered_map<int, int> id_index, id_bcm;
+    mutable size_t id_index_n = size_t(-1), id_bcm_n = size_t(-1);
     int world_index_of_id(int id) const {
-        for (int i = 0; i < n(); ++i) if (bodies[i].id == id) return i;
-        return -1;
+        if (id_index_n != bodies.size()) {
+            id_index.clear();
+            for (int i = 0; i < n(); ++i) id_index.emplace(bodies[i].id, i);
+            id_index_n = bodies.size();
+        }
+        const auto it = id_index.find(id);
+        return it == id_index.end() ? -1 : it->second;
     }
     const BodyContactMesh* mesh_by_id(int id) const {
-        for (auto& m : bcm) if (m.body_id == id) return &m;
-        return nullptr;
+        if (id_bcm_n != bcm.size()) {
+            id_bcm.clear();
+            for (size_t k = 0; k < bcm.size(); ++k) id_bcm.emplace(bcm[k].body_id, int(k));
+            id_bcm_n = bcm.size();
+        }
+        const auto it = id_bcm.find(id);
+        return it == id_bcm.end() ? nullptr : &bcm[it->second];
     }
 
     // Body pose used by detectContactsAtState / FrictionForm::getBodyState.
@@ -698,15 +713,27 @@ struct Solver {
         if (bodies.empty()) return contacts;
         const double dh = dhat;
         const size_t kMaxRaw = 40000, kMax = 2000;
+        // world vertices, each computed once per call (the same expression the reference
+        // evaluates per fetch, so the same values)
+        vector<vector<V3>> wv(bodies.size());
+        vector<vector<char>> wv_ok(bodies.size());
         auto world_vertex = [&](int body_id, int lv, V3& out) -> bool {
             const BodyContactMesh* m = mesh_by_id(body_id);
             const int wi = world_index_of_id(body_id);
             if (!m || wi < 0) return false;
             if (lv < 0 || lv >= int(m->mesh->V.size())) return false;
             if (x && (wi * 6 + 5 >= int(x->size()))) return false;
-            V3 p; Q q;
-            pose_at(x, wi, p, q);
-            out = p + rotate(q, m->mesh->V[lv]);
+            if (wv[wi].empty()) {
+                wv[wi].resize(m->mesh->V.size());
+                wv_ok[wi].assign(m->mesh->V.size(), 0);
+            }
+            if (!wv_ok[wi][lv]) {
+                V3 p; Q q;
+                pose_at(x, wi, p, q);
+                wv[wi][lv] = p + rotate(q, m->mesh->V[lv]);
+                wv_ok[wi][lv] = 1;
+            }
+            out = wv[wi][lv];
             return true;
         };
         auto append = [&](int ida, int idb, int type, int pa, int pb, const V3& pt, const V3& nrm, double d) {
@@ -1200,22 +1227,9 @@ struct Solver {
     // 1e-9 (IPCSolver.cpp:824-831) is unchanged; only the value of larger minima may
     // differ (+inf when every pair is skipped).
     double scene_min_distance() const {
+        if (prune_scene_min) return scene_min_distance_pruned();
         double md = std::numeric_limits<double>::infinity();
         long long vcount = 0;
-        vector<AABB> wb;
-        vector<double> emax;
-        if (prune_scene_min) {
-            for (int i = 0; i < n(); ++i) {
-                const Body& b = bodies[i];
-                if (b.mesh < 0) { wb.emplace_back(); emax.push_back(0.0); continue; }
-                const Mesh& M = meshes[b.mesh];
-                wb.push_back(transform_local_aabb(M.lmin, M.lmax, b.p, b.q));
-                double e = 0.0;
-                for (auto& f : M.F)
-                    for (int k = 0; k < 3; ++k) e = std::max(e, norm(M.V[f[(k + 1) % 3]] - M.V[f[k]]));
-                emax.push_back(e);
-            }
-        }
         for (int i = 0; i < n(); ++i) {
             if (bodies[i].mesh < 0) continue;
             const Mesh& MA = meshes[bodies[i].mesh];
@@ -1224,8 +1238,6 @@ struct Solver {
                 const V3 p = bodies[i].p + rotate(bodies[i].q, MA.V[vi]);
                 for (int j = i + 1; j < n(); ++j) {
                     if (bodies[j].mesh < 0) continue;
-                    if (prune_scene_min && aabb_distance(wb[i], wb[j]) > 1e-8 + 1e-5 * std::max(emax[i], emax[j]))
-                        continue;
                     const Mesh& MB = meshes[bodies[j].mesh];
                     vector<V3> VB(MB.V.size());
                     for (size_t k = 0; k < VB.size(); ++k) VB[k] = bodies[j].p + rotate(bodies[j].q, MB.V[k]);
@@ -1248,6 +1260,75 @@ struct Solver {
                         if (bmin >= -1e-6 && bmax <= 1.0 + 1e-6) d = std::min(d, dist);
                     }
                     md = std::min(md, d);
+                }
+            }
+        }
+        return md;
+    }
+
+    // The same test with exact pruning for the result's comparison with 1e-9
+    // (IPCSolver.cpp:824-831): body pairs whose world boxes are farther apart than
+    // m = 1e-8 + 1e-5 * (longest edge of either mesh), and within a pair the faces whose
+    // world box is farther than m from body i's box and the vertices farther than m from
+    // body j's box, are skipped. A vertex-face distance below 1e-9 with barycentric
+    // coordinates inside [-1e-6, 1 + 1e-6] puts the vertex within 1e-9 + 2e-6 * (longest
+    // edge) of the face, so every such pair is kept; each kept test computes the same
+    // values as the loop above (per-face data computed once instead of per vertex), so
+    // the minimum is the same whenever it is below m. Larger minima may differ (+inf when
+    // everything is skipped).
+    double scene_min_distance_pruned() const {
+        double md = std::numeric_limits<double>::infinity();
+        const int N = n();
+        vector<AABB> wb(N);
+        vector<double> emax(N, 0.0);
+        for (int i = 0; i < N; ++i) {
+            const Body& b = bodies[i];
+            if (b.mesh < 0) continue;
+            const Mesh& M = meshes[b.mesh];
+            wb[i] = transform_local_aabb(M.lmin, M.lmax, b.p, b.q);
+            for (auto& f : M.F)
+                for (int k = 0; k < 3; ++k) emax[i] = std::max(emax[i], norm(M.V[f[(k + 1) % 3]] - M.V[f[k]]));
+        }
+        struct FaceData { V3 v0, nn; M3 inv; AABB box; };
+        for (int i = 0; i < N; ++i) {
+            if (bodies[i].mesh < 0) continue;
+            const Mesh& MA = meshes[bodies[i].mesh];
+            for (int j = i + 1; j < N; ++j) {
+                if (bodies[j].mesh < 0) continue;
+                const double m = 1e-8 + 1e-5 * std::max(emax[i], emax[j]);
+                if (aabb_distance(wb[i], wb[j]) > m) continue;
+                const Mesh& MB = meshes[bodies[j].mesh];
+                vector<V3> VB(MB.V.size());
+                for (size_t k = 0; k < VB.size(); ++k) VB[k] = bodies[j].p + rotate(bodies[j].q, MB.V[k]);
+                vector<FaceData> fd;
+                for (auto& f : MB.F) {
+                    const V3 v0 = VB[f[0]], v1 = VB[f[1]], v2 = VB[f[2]];
+                    AABB fb;
+                    fb.expand(v0); fb.expand(v1); fb.expand(v2);
+                    if (aabb_distance(fb, wb[i]) > m) continue;
+                    const V3 e1 = v1 - v0, e2 = v2 - v0;
+                    V3 nn = cross(e1, e2);
+                    const double area = norm(nn);
+                    if (area < 1e-12) continue;
+                    nn = normalized(nn);
+                    M3 M;
+                    M(0, 0) = e1.x; M(1, 0) = e1.y; M(2, 0) = e1.z;
+                    M(0, 1) = e2.x; M(1, 1) = e2.y; M(2, 1) = e2.z;
+                    M(0, 2) = nn.x; M(1, 2) = nn.y; M(2, 2) = nn.z;
+                    fd.push_back({v0, nn, inverse(M), fb});
+                }
+                if (fd.empty()) continue;
+                for (size_t vi = 0; vi < MA.V.size(); ++vi) {
+                    const V3 p = bodies[i].p + rotate(bodies[i].q, MA.V[vi]);
+                    if (point_aabb_distance(p, wb[j]) > m) continue;
+                    for (const FaceData& F : fd) {
+                        if (point_aabb_distance(p, F.box) > m) continue;
+                        const double dist = std::abs(dot(p - F.v0, F.nn));
+                        const V3 co = mul(F.inv, p - F.v0);
+                        const double b0 = 1.0 - co.x - co.y, b1 = co.x, b2 = co.y;
+                        const double bmin = std::min({b0, b1, b2}), bmax = std::max({b0, b1, b2});
+                        if (bmin >= -1e-6 && bmax <= 1.0 + 1e-6) md = std::min(md, dist);
+                    }
                 }
             }
         }

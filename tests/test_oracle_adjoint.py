@@ -75,3 +75,4 @@ def test_cpu_adjoint_c4_free_cubes():
     cq = rng.normal(0.0, 1e-2, size=(sc.n_bodies, 4))
     comps = [("v0", (b, c)) for b in (20, 41) for c in range(3)] + [("w0", (33, c)) for c in range(3)]
     _check(sc, 2, np.asarray(targets), cq, comps, 1e-5)
+
