@@ -179,6 +179,17 @@ int32_t ndipc_ccd_earliest_toi(ndipc_ctx* ctx, int32_t scene, double dt, double*
 int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t n_blocks, const double* diag, int32_t n_off, const int32_t* off_ij,
                         const double* off_blocks, const double* b, double tol, int32_t max_it, double* x,
                         int32_t* iters, double* rel_res);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops), the same block-Jacobi PCG as ndipc_pcg_solve
+ * with every array in device memory (d_off_ij: 2 per off-diagonal block; d_iters: one
+ * int32, d_rel: one double) and the solver's work vectors in a fixed array of the
+ * library, one per GPU, sized for NDIPC_PCG_NOALLOC_MAX_BLOCKS blocks (1.9 MB). Larger
+ * systems are rejected. Enqueued on the context stream (ndipc_stream); returns without
+ * synchronising. Calls that overlap on one GPU share that array and race. */
+#define NDIPC_PCG_NOALLOC_MAX_BLOCKS 4096
+int32_t ndipc_pcg_solve_noalloc(ndipc_ctx* ctx, int32_t n_blocks, const double* d_diag, int32_t n_off,
+                                const int32_t* d_off_ij, const double* d_offb, const double* d_b, double tol,
+                                int32_t max_it, double* d_x, int32_t* d_iters, double* d_rel);
 
 /* ---- adjoint (no reference counterpart: the reference has no differentiation) ------ */
 /* n_steps of ndipc_step, saving the full state (poses, velocities, accelerations, joint

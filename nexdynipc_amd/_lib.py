@@ -51,7 +51,7 @@ EXPORTS = [
     "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters", "ndipc_set_joint_state",
     "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
     "ndipc_autotune", "ndipc_helper_counters", "ndipc_helper_config",
-    "ndipc_set_step_mode",
+    "ndipc_set_step_mode", "ndipc_pcg_solve_noalloc",
 ]
 
 
@@ -105,6 +105,8 @@ def library():
         "ndipc_ccd_earliest_toi": ([vp, C.c_int32, C.c_double, f64p], C.c_int32),
         "ndipc_pcg_solve": ([vp, C.c_int32, f64p, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_int32, f64p, i32p,
                              f64p], C.c_int32),
+        "ndipc_pcg_solve_noalloc": ([vp, C.c_int32, vp, C.c_int32, vp, vp, vp, C.c_double, C.c_int32, vp, vp, vp],
+                                    C.c_int32),
         "ndipc_timing_enable": ([vp, C.c_int32], C.c_int32),
         "ndipc_use_graph": ([vp, C.c_int32], C.c_int32),
         "ndipc_set_step_mode": ([vp, C.c_int32, C.c_int32], C.c_int32),
@@ -424,6 +426,12 @@ class Engine:
         self._ck(self.L.ndipc_pcg_solve(self.ctx, nb, ptr(d, f64p), len(oij), ptr(oij, i32p), ptr(ob, f64p),
                                         ptr(bb, f64p), tol, max_it, ptr(x, f64p), C.byref(it), C.byref(rel)), "pcg")
         return x, it.value, rel.value
+
+    def pcg_solve_noalloc(self, nb, d_diag, noff, d_offij, d_offb, d_b, tol, max_it, d_x, d_iters, d_rel):
+        """ndipc_pcg_solve_noalloc: every argument a device address (int); enqueued on the
+        context stream, not synchronised."""
+        self._ck(self.L.ndipc_pcg_solve_noalloc(self.ctx, int(nb), d_diag, int(noff), d_offij, d_offb, d_b,
+                                                float(tol), int(max_it), d_x, d_iters, d_rel), "pcg_solve_noalloc")
 
     def autotune(self, trial_steps=3, dt=None):
         """Chooses the persistent kernels' work-group counts for this batch (state is

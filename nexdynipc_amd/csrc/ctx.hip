@@ -9,6 +9,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_set>
 #include <vector>
@@ -259,6 +260,29 @@ void timing_end(const char* name, hipStream_t st) {
 }
 }  // namespace nd
 
+// Work vectors of ndipc_pcg_solve_noalloc: one fixed array per GPU, held until the
+// process ends (per block r, z, p, A p: 24 doubles; the inverted diagonal block: 36).
+constexpr int kNoAllocBlockDoubles = 60;
+static __device__ __align__(256) double scratch[size_t(NDIPC_PCG_NOALLOC_MAX_BLOCKS) * kNoAllocBlockDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each GPU
+
+// Once per device, with the device current: the device address of `scratch`. Host code
+// uses only this address (the array's name is a host-side stand-in).
+int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -1;
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+    return 0;
+}
+}  // namespace
+
 extern "C" {
 
 int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
@@ -268,6 +292,7 @@ int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return -3;
     nd::geom_configure();
     if (nd::geom_configure_smf() != 0) return -5;
+    if (scratch_setup(device) != 0) return -6;
     auto* c = new ndipc_ctx();
     c->device = device;
     if (hipStreamCreate(&c->stream) != hipSuccess) {
@@ -1318,6 +1343,26 @@ int32_t ndipc_pcg_solve(ndipc_ctx* ctx, int32_t nb, const double* diag, int32_t 
     HIPCHK(hipMemcpy(x, dx, 8 * nd6, hipMemcpyDeviceToHost));
     if (iters) HIPCHK(hipMemcpy(iters, ib, 4, hipMemcpyDeviceToHost));
     if (rel) HIPCHK(hipMemcpy(rel, drel, 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int32_t ndipc_pcg_solve_noalloc(ndipc_ctx* ctx, int32_t nb, const double* d_diag, int32_t noff,
+                                const int32_t* d_offij, const double* d_offb, const double* d_b, double tol,
+                                int32_t max_it, double* d_x, int32_t* d_iters, double* d_rel) {
+    if (!ctx) return -1;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (nb <= 0) return fail(ctx, "pcg_solve_noalloc: no blocks");
+    if (nb > NDIPC_PCG_NOALLOC_MAX_BLOCKS)
+        return fail(ctx, "pcg_solve_noalloc: " + std::to_string(nb) + " blocks exceed the maximum of " +
+                             std::to_string(NDIPC_PCG_NOALLOC_MAX_BLOCKS));
+    if (noff < 0 || (noff > 0 && (!d_offij || !d_offb)))
+        return fail(ctx, "pcg_solve_noalloc: bad off-diagonal blocks");
+    if (!d_diag || !d_b || !d_x || !d_iters || !d_rel) return fail(ctx, "pcg_solve_noalloc: null pointer");
+    double* scr = ctx->device >= 0 && ctx->device < kMaxDevices ? g_scratch_ptr[ctx->device] : nullptr;
+    if (!scr) return fail(ctx, "pcg_solve_noalloc: scratch not set up on this device");
+    // the kernel writes every work vector before reading it: nothing to clear
+    nd::launch_pcg(nb, d_diag, noff, d_offij, d_offb, d_b, tol, max_it, d_x, d_iters, d_rel, scr, ctx->stream);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
