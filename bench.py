@@ -37,12 +37,20 @@ TIMED = ("newton", "scene_min_flag", "topk", "np_count", "ccd_fv", "bp_step", "l
 # test is 34 flops (the barycentric test, 53 more, only runs after it passes); the EE
 # segment-segment distance is 67 flops.
 FLOPS_FV, FLOPS_EE = 34.0, 67.0
-# Newton kernel, compulsory HBM bytes of one Newton iteration / one line-search trial
-# (IPCSolver.cpp:137-262 forms at x): per body x, x_hat (6 doubles each), mass and
-# body-frame inertia (10 doubles), the step dx (6 doubles) = 224 B; per contact its
-# record (5 ints + 7 doubles) and lagged normal force = 84 B; per joint its data,
-# multipliers and stiffness (16 + 6 + 1 doubles) = 184 B.
-NEWTON_B_BODY, NEWTON_B_CONTACT, NEWTON_B_JOINT = 224.0, 84.0, 184.0
+# Newton kernel, compulsory unique HBM bytes of one launch (one Newton solve of every
+# scene, IPCSolver.cpp:137-262 forms; perfect on-chip reuse across its iterations and
+# line-search trials): per body x, x_hat (6 doubles each), mass and body-frame inertia
+# (10 doubles), the step-start pose (7 doubles) read, x written (6 doubles) = 280 B; per
+# contact its record (5 ints + 7 doubles) and lagged normal force = 84 B; per joint its
+# data, multipliers and stiffness (16 + 6 + 1 doubles) read, multipliers written (6) =
+# 232 B.
+NEWTON_B_BODY, NEWTON_B_CONTACT, NEWTON_B_JOINT = 280.0, 84.0, 232.0
+# Latency floor of the Newton kernel: the objective value and the line-search slopes are
+# ordered left-to-right FP64 sums (the reference's loop order, kept for bit parity), so
+# every Newton iteration holds at least two chains of (bodies + contacts + joints)
+# dependent adds; 4.5 ns per dependent FP64 add on this part (measured with a chained-add
+# probe kernel).
+FP64_ADD_LATENCY_S = 4.5e-9
 
 
 def cpu_baseline(sc, stride, steps=1):
@@ -56,6 +64,21 @@ def cpu_baseline(sc, stride, steps=1):
     t_step, t_min_scaled = o.timing()
     est = t_step - t_min_scaled / stride + t_min_scaled
     return steps / est, t_step
+
+
+def cpu_baseline_c1(steps=200):
+    """The oracle's full C1 run (8 cubes + ground, 200 steps) on one host core."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    from nexdynipc_amd import scenes
+
+    sc = scenes.make_config("c1")
+    o = O.OracleScene(sc, 0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(float(sc.dt[0]))
+    T = time.perf_counter() - t0
+    return steps / T, T
 
 
 def load_traffic(kernel):
@@ -170,18 +193,26 @@ def roofline_for(name, r, sc):
         f_fv = n_fv / max(n_fv + n_ee, 1.0)
         flops = r["evals"] * (FLOPS_FV * f_fv + FLOPS_EE * (1.0 - f_fv))
         achieved = flops / t / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        roof = {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS}
     elif name == "newton":
         contacts = float(np.sum(stats[:, 2]))
-        per = NEWTON_B_BODY * sc.n_bodies + NEWTON_B_CONTACT * contacts + NEWTON_B_JOINT * sc.n_joints
-        bytes_ = (r["newton_iters"] + r["trials"]) * per
-        achieved = bytes_ / t / 1e9
+        per_launch = NEWTON_B_BODY * sc.n_bodies + NEWTON_B_CONTACT * contacts + NEWTON_B_JOINT * sc.n_joints
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS}
+                "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": per_launch}
+        # the kernel is latency-bound: its ordered FP64 sums against the dependent-add floor
+        chain = sc.n_bodies + contacts + sc.n_joints
+        floor_s = r["newton_iters"] * 2.0 * chain * FP64_ADD_LATENCY_S
+        roof["latency_floor"] = {"dependent_fp64_adds_per_launch": 2.0 * chain * r["newton_iters"] / cnt,
+                                 "add_latency_ns": FP64_ADD_LATENCY_S * 1e9,
+                                 "floor_ms_per_launch": 1e3 * floor_s / cnt,
+                                 "achieved_over_floor": t / floor_s if floor_s > 0 else None}
     else:
         return None
     roof["traffic"] = load_traffic(name)
+    if roof["traffic"] is not None and name == "newton":
+        roof["traffic_over_algorithmic"] = roof["traffic"] / roof["algorithmic_bytes_per_launch"]
     roof["kernel"] = "k_" + name
     roof["avg_launch_ms"] = avg_ms
     roof["launches"] = cnt
@@ -306,12 +337,22 @@ def main():
     if roof is not None:
         roof["share_of_timed_kernels"] = totals[dominant] / max(sum(totals.values()), 1e-30)
     cpu = None
+    c1 = None
     if not args.no_cpu_baseline:
         v, wall = cpu_baseline(sc, args.cpu_stride)
         cpu = {"value": v, "unit": "steps/s", "cores": 1, "kind": "port",
                "sample": f"1 step of c2 by the C++ oracle on 1 host core; computeSceneMinDistance's vertex loop "
                          f"sampled 1/{args.cpu_stride} and its time scaled x{args.cpu_stride} "
                          f"(measured wall {wall:.1f} s)"}
+        # C1 (the reference's own CPU-runnable case) fully measured on both sides
+        sc1 = scenes.make_config("c1")
+        r1 = run_plain(Engine, torch, sc1, 200, 0)
+        v1, wall1 = cpu_baseline_c1(200)
+        c1 = {"workload": "c1: 8 cubes + ground box, 200 steps, dt 1e-2", "gpu_steps_per_s": 200 / r1["T"],
+              "cpu_steps_per_s": v1, "cpu_cores": 1, "cpu_kind": "port",
+              "cpu_sample": f"all 200 steps of c1 by the C++ oracle on 1 host core (wall {wall1:.2f} s)",
+              "gpu_note": "the same 200 steps from the start state (graph capture inside the timing), "
+                          "single-scene path with step graphs"}
     out = {
         "metric": METRIC,
         "value": args.steps / T,
@@ -341,6 +382,7 @@ def main():
         "batch": batch,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
+        "c1": c1,
     }
     print(json.dumps(out))
 

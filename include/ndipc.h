@@ -120,6 +120,11 @@ int32_t ndipc_step(ndipc_ctx* ctx, const double* dt, int32_t n_steps);
  * iterations, contacts after the last narrow phase, FV candidates, EE candidates,
  * narrow-phase candidate evaluations, kappa doubled (0/1), earliest TOI. */
 int32_t ndipc_last_stats(ndipc_ctx* ctx, double* out8);
+/* Per-scene residuals of the last step's last ALM iteration, 3 doubles: the largest
+ * joint constraint violation, the dual residual and the contact residual
+ * (IPCSolver::lastMaxConstraintViolation / lastDualResidual / lastContactResidual,
+ * IPCSolver.h:74-79, set at IPCSolver.cpp:1156-1158). */
+int32_t ndipc_last_residuals(ndipc_ctx* ctx, double* out3);
 /* Narrow-phase candidate evaluations (FV + EE, VV when the fallback runs) per scene
  * since upload: the "contact pairs" of the throughput metric. */
 int32_t ndipc_eval_counter(ndipc_ctx* ctx, double* out);

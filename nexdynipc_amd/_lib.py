@@ -51,7 +51,7 @@ EXPORTS = [
     "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters", "ndipc_set_joint_state",
     "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
     "ndipc_autotune", "ndipc_helper_counters", "ndipc_helper_config",
-    "ndipc_set_step_mode", "ndipc_pcg_solve_noalloc",
+    "ndipc_set_step_mode", "ndipc_pcg_solve_noalloc", "ndipc_last_residuals",
 ]
 
 
@@ -88,6 +88,7 @@ def library():
         "ndipc_get_joint_state": ([vp, f64p, f64p], C.c_int32),
         "ndipc_step": ([vp, f64p, C.c_int32], C.c_int32),
         "ndipc_last_stats": ([vp, f64p], C.c_int32),
+        "ndipc_last_residuals": ([vp, f64p], C.c_int32),
         "ndipc_eval_counter": ([vp, f64p], C.c_int32),
         "ndipc_pair_count": ([vp, i32p], C.c_int32),
         "ndipc_detect_body_pairs": ([vp, C.c_int32, C.c_double, i32p, C.c_int32, i32p], C.c_int32),
@@ -214,6 +215,10 @@ class Engine:
         n = sc.n_bodies
         self.set_state(sc.p, sc.q, sc.v, sc.w, np.zeros((n, 3)), np.zeros((n, 3)))
 
+    def configure(self, params):
+        """ndipc_configure: the same IPCSolver settings for every scene."""
+        self._ck(self.L.ndipc_configure(self.ctx, C.byref(params)), "configure")
+
     def set_state(self, p=None, q=None, v=None, w=None, al=None, aa=None):
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (p, q, v, w, al, aa)]
         self._ck(self.L.ndipc_set_state(self.ctx, *[ptr(a, f64p) for a in arrs]), "set_state")
@@ -281,6 +286,13 @@ class Engine:
     def stats(self):
         o = np.zeros((self.sc.n_scenes, 8))
         self._ck(self.L.ndipc_last_stats(self.ctx, ptr(o, f64p)), "last_stats")
+        return o
+
+    def last_residuals(self):
+        """Per scene: joint violation, dual residual, contact residual of the last step's
+        last ALM iteration."""
+        o = np.zeros((self.sc.n_scenes, 3))
+        self._ck(self.L.ndipc_last_residuals(self.ctx, ptr(o, f64p)), "last_residuals")
         return o
 
     def solver_counters(self):

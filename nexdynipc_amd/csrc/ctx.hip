@@ -991,6 +991,18 @@ int32_t ndipc_last_stats(ndipc_ctx* ctx, double* out8) {
     return 0;
 }
 
+int32_t ndipc_last_residuals(ndipc_ctx* ctx, double* out3) {
+    if (check_ctx(ctx)) return -1;
+    if (sync_sp_from_dev(ctx)) return -1;
+    for (int s = 0; s < ctx->S; ++s) {
+        const SceneParams& q = ctx->sp[s];
+        out3[3 * s] = q.viol;
+        out3[3 * s + 1] = q.dual;
+        out3[3 * s + 2] = q.cres;
+    }
+    return 0;
+}
+
 int32_t ndipc_solver_counters(ndipc_ctx* ctx, double* out8) {
     if (check_ctx(ctx)) return -1;
     if (sync_sp_from_dev(ctx)) return -1;
