@@ -77,6 +77,28 @@ ND_HD Q from_angle_axis(double angle, V3 axis) {
     ef::sincos(ha, &s, &c);
     return quat(c, s * axis.x, s * axis.y, s * axis.z);
 }
+// BarrierPotential (Barrier.cpp:11-46): b = -k (d - dh)^2 ln(d / dh) on (0, dh), 0 at
+// and above dh, +inf at and below 0; its first and second derivatives in d.
+ND_HD double bar_v(double d, double dh, double k) {
+    if (d >= dh) return 0.0;
+    if (d <= 0.0) return 1.0 / 0.0;
+    const double r = d / dh;
+    return -k * (d - dh) * (d - dh) * ef::log(r);
+}
+ND_HD double bar_g(double d, double dh, double k) {
+    if (d >= dh) return 0.0;
+    if (d <= 0.0) return -1.0 / 0.0;
+    const double diff = d - dh;
+    const double lr = ef::log(d / dh);
+    return -k * (2 * diff * lr + diff * diff / d);
+}
+ND_HD double bar_h(double d, double dh, double k) {
+    if (d >= dh) return 0.0;
+    if (d <= 0.0) return 1.0 / 0.0;
+    const double diff = d - dh;
+    const double lr = ef::log(d / dh);
+    return -k * (2 * lr + 4 * diff / d - (diff * diff) / (d * d));
+}
 // Pose of a body at Newton variable theta: normalize(AngleAxis(|t|, t/|t|) * q) for
 // |t| > 1e-10 (IPCSolver.cpp:616-626, FrictionForm.cpp:206-223).
 ND_HD Q apply_theta(Q q, V3 th) {

@@ -15,27 +15,7 @@ constexpr int kTermLds = 8192;  // doubles of LDS for the ordered sums of the Ne
 constexpr double kG = -9.81;  // gravity (0, -9.81, 0), IPCSolver.cpp:142
 
 // ------------------------------------------------------------------ potentials
-// BarrierPotential (Barrier.cpp:11-46)
-__device__ __forceinline__ double bar_v(double d, double dh, double k) {
-    if (d >= dh) return 0.0;
-    if (d <= 0.0) return 1.0 / 0.0;
-    const double r = d / dh;
-    return -k * (d - dh) * (d - dh) * ef::log(r);
-}
-__device__ __forceinline__ double bar_g(double d, double dh, double k) {
-    if (d >= dh) return 0.0;
-    if (d <= 0.0) return -1.0 / 0.0;
-    const double diff = d - dh;
-    const double lr = ef::log(d / dh);
-    return -k * (2 * diff * lr + diff * diff / d);
-}
-__device__ __forceinline__ double bar_h(double d, double dh, double k) {
-    if (d >= dh) return 0.0;
-    if (d <= 0.0) return 1.0 / 0.0;
-    const double diff = d - dh;
-    const double lr = ef::log(d / dh);
-    return -k * (2 * lr + 4 * diff / d - (diff * diff) / (d * d));
-}
+// BarrierPotential (Barrier.cpp:11-46): bar_v / bar_g / bar_h in dmath.h
 
 struct ScnCtx {
     int s, b0, n, nd, j0, j1, nc;

@@ -19,6 +19,7 @@
 #include "ndipc_oracle.h"
 #include "ndo_interval.h"
 #include "ndo_math.h"
+#include "ipc_contact.h"
 
 #include <array>
 #include <chrono>
@@ -1226,6 +1227,73 @@ struct Solver {
     // within 1e-9 + 2e-6 * (longest edge) of the face, so the result's comparison with
     // 1e-9 (IPCSolver.cpp:824-831) is unchanged; only the value of larger minima may
     // differ (+inf when every pair is skipped).
+    // ---- ipc mode, function level: the primitive-distance barrier of rigid IPC over a
+    // list of pairs (kind 0 PT: body of the vertex, body of the face, vertex, face; kind 1
+    // EE: bodies and mesh edge indices of edge a and edge b) at Newton variable x. Pairs
+    // whose bodies form the same unordered pair and follow each other form a segment;
+    // per segment the slot-0 body is the lower index. Sums in pair order.
+    struct IpcSeg {
+        int lo, hi;
+        double g[12], H[144];
+    };
+    bool ipc_pair_vertices(const int32_t* pr, const vector<double>& x, double xw[12], int owner[4],
+                           double Jth[4][9], int& lo, int& hi) const {
+        const int kind = pr[0], b0 = pr[1], b1 = pr[2];
+        if (b0 < 0 || b1 < 0 || b0 >= n() || b1 >= n() || b0 == b1) return false;
+        if (bodies[b0].mesh < 0 || bodies[b1].mesh < 0) return false;
+        const Mesh& M0 = meshes[bodies[b0].mesh];
+        const Mesh& M1 = meshes[bodies[b1].mesh];
+        int vb[4];
+        V3 r[4];
+        if (kind == 0) {
+            if (pr[3] < 0 || pr[3] >= int(M0.V.size()) || pr[4] < 0 || pr[4] >= int(M1.F.size())) return false;
+            vb[0] = b0; r[0] = M0.V[pr[3]];
+            for (int k = 0; k < 3; ++k) { vb[1 + k] = b1; r[1 + k] = M1.V[M1.F[pr[4]][k]]; }
+        } else {
+            if (pr[3] < 0 || pr[3] >= int(M0.E.size()) || pr[4] < 0 || pr[4] >= int(M1.E.size())) return false;
+            for (int k = 0; k < 2; ++k) {
+                vb[k] = b0; r[k] = M0.V[M0.E[pr[3]][k]];
+                vb[2 + k] = b1; r[2 + k] = M1.V[M1.E[pr[4]][k]];
+            }
+        }
+        lo = std::min(b0, b1);
+        hi = std::max(b0, b1);
+        for (int v = 0; v < 4; ++v) {
+            const Body& b = bodies[vb[v]];
+            const double q[4] = {b.q.w, b.q.x, b.q.y, b.q.z};
+            const double rr[3] = {r[v].x, r[v].y, r[v].z};
+            nd::ipc::vertex_and_jacobian(&x[6 * size_t(vb[v])], q, rr, xw + 3 * v, Jth[v]);
+            owner[v] = vb[v] == lo ? 0 : 1;
+        }
+        return true;
+    }
+    // total energy and the per-segment gradient / Hessian blocks; false on a bad pair
+    bool ipc_barrier_eval(const int32_t* pairs, int np, const vector<double>& x, double dhat, double kappa, bool psd,
+                          double& value, vector<IpcSeg>& segs) const {
+        value = 0.0;
+        segs.clear();
+        for (int i = 0; i < np; ++i) {
+            const int32_t* pr = pairs + 5 * i;
+            double xw[12], Jth[4][9], g[12], H[144], gb[12], Hb[144], E = 0.0;
+            int owner[4], lo, hi;
+            if (!ipc_pair_vertices(pr, x, xw, owner, Jth, lo, hi)) return false;
+            if (segs.empty() || segs.back().lo != lo || segs.back().hi != hi) {
+                IpcSeg sg{};
+                sg.lo = lo;
+                sg.hi = hi;
+                segs.push_back(sg);
+            }
+            if (!nd::ipc::pair_energy(pr[0], xw, dhat, kappa, &E, g, H, nullptr, nullptr)) continue;
+            value += E;
+            if (psd) nd::ipc::psd_project<12>(H);
+            nd::ipc::fold_to_bodies(g, H, owner, Jth, gb, Hb);
+            IpcSeg& sg = segs.back();
+            for (int k = 0; k < 12; ++k) sg.g[k] += gb[k];
+            for (int k = 0; k < 144; ++k) sg.H[k] += Hb[k];
+        }
+        return true;
+    }
+
     double scene_min_distance() const {
         if (prune_scene_min) return scene_min_distance_pruned();
         double md = std::numeric_limits<double>::infinity();
@@ -2526,6 +2594,49 @@ int32_t ndo_record_objective(void* h, int32_t step, int32_t solve, const double*
     if (cgrad) { std::vector<double> g; P.contact_grad(xv, g); std::copy(g.begin(), g.end(), cgrad); }
     if (jointC)
         for (size_t j = 0; j < P.joints.size(); ++j) P.joint_C(P.joints[j], xv, jointC + 6 * j);
+    return 0;
+}
+
+// ---- ipc mode (function level)
+int32_t ndo_ipc_pair(int32_t kind, const double* x12, double dhat, double kappa, int32_t psd, double* E, double* g12,
+                     double* H144, int32_t* type, double* d) {
+    int t = 0;
+    double dd = 0.0;
+    nd::ipc::pair_energy(kind, x12, dhat, kappa, E, g12, H144, &t, &dd);
+    if (psd) nd::ipc::psd_project<12>(H144);
+    if (type) *type = t;
+    if (d) *d = dd;
+    return 0;
+}
+
+int32_t ndo_ipc_distance(int32_t kind, const double* x12, int32_t* type, double* d) {
+    int t = 0;
+    *d = nd::ipc::pair_distance(kind, x12, &t);
+    if (type) *type = t;
+    return 0;
+}
+
+double ndo_psd_project12(double* H144) { return nd::ipc::psd_project<12>(H144); }
+
+int32_t ndo_vertex_jacobian(const double* xb6, const double* q4, const double* r3, double* xw3, double* J9) {
+    nd::ipc::vertex_and_jacobian(xb6, q4, r3, xw3, J9);
+    return 0;
+}
+
+int32_t ndo_ipc_barrier_eval(void* h, int32_t np, const int32_t* pairs5, const double* x, double dhat, double kappa,
+                             int32_t psd, double* value, int32_t cap, int32_t* n_seg, int32_t* seg_bodies,
+                             double* seg_g, double* seg_H) {
+    Solver& s = S(h);
+    std::vector<double> xv(x, x + size_t(s.n()) * 6);
+    std::vector<Solver::IpcSeg> segs;
+    if (!s.ipc_barrier_eval(pairs5, np, xv, dhat, kappa, psd != 0, *value, segs)) return -1;
+    *n_seg = int32_t(segs.size());
+    for (size_t k = 0; k < segs.size() && int32_t(k) < cap; ++k) {
+        seg_bodies[2 * k] = segs[k].lo;
+        seg_bodies[2 * k + 1] = segs[k].hi;
+        std::copy(segs[k].g, segs[k].g + 12, seg_g + 12 * k);
+        std::copy(segs[k].H, segs[k].H + 144, seg_H + 144 * k);
+    }
     return 0;
 }
 

@@ -157,6 +157,24 @@ int32_t ndo_record_solve(void* h, int32_t step, int32_t solve, int32_t it, doubl
 int32_t ndo_record_objective(void* h, int32_t step, int32_t solve, const double* x, double* value, double* grad,
                              double* hess, double* cvalue, double* cgrad, double* jointC);
 
+/* ipc mode (function level; primitive-distance barrier of rigid IPC, see
+ * nexdynipc_amd/csrc/ipc_contact.h): one pair of world points x12 (PT: p t0 t1 t2; EE:
+ * a0 a1 b0 b1): E, gradient, Hessian (optionally projected to PSD), distance type, d. */
+int32_t ndo_ipc_pair(int32_t kind, const double* x12, double dhat, double kappa, int32_t psd, double* E, double* g12,
+                     double* H144, int32_t* type, double* d);
+int32_t ndo_ipc_distance(int32_t kind, const double* x12, int32_t* type, double* d);
+/* Projects a symmetric 12x12 (row-major, in place) to PSD; returns the smallest eigenvalue before. */
+double ndo_psd_project12(double* H144);
+/* world position of body-frame point r at Newton variable xb (p, theta) with orientation q, and d/dtheta */
+int32_t ndo_vertex_jacobian(const double* xb6, const double* q4, const double* r3, double* xw3, double* J9);
+/* pairs5[5 i] = kind, body0, body1, prim0, prim1 (PT: vertex body, face body, vertex, face;
+ * EE: edge bodies and mesh edge indices); per segment of consecutive pairs of the same
+ * unordered body pair: the bodies (lower first), the gradient (12) and Hessian (144) in
+ * the two bodies' DOFs. */
+int32_t ndo_ipc_barrier_eval(void* h, int32_t np, const int32_t* pairs5, const double* x, double dhat, double kappa,
+                             int32_t psd, double* value, int32_t cap, int32_t* n_seg, int32_t* seg_bodies,
+                             double* seg_g, double* seg_H);
+
 /* Newton system at x (SimulationProblem with the current contact forms): */
 int32_t ndo_objective(void* h, const double* x, double* value, double* grad, double* hess);
 

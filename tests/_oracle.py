@@ -120,9 +120,16 @@ def lib():
                 ("ndo_record_steps", [C.c_void_p, C.c_double, C.c_int32]),
                 ("ndo_record_info", [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p]),
                 ("ndo_record_solve", [C.c_void_p, C.c_int32, C.c_int32, C.c_int32] + [f64p] * 7),
-                ("ndo_record_objective", [C.c_void_p, C.c_int32, C.c_int32] + [f64p] * 7)):
+                ("ndo_record_objective", [C.c_void_p, C.c_int32, C.c_int32] + [f64p] * 7),
+                ("ndo_ipc_pair", [C.c_int32, f64p, C.c_double, C.c_double, C.c_int32, f64p, f64p, f64p, i32p, f64p]),
+                ("ndo_ipc_distance", [C.c_int32, f64p, i32p, f64p]),
+                ("ndo_vertex_jacobian", [f64p, f64p, f64p, f64p, f64p]),
+                ("ndo_ipc_barrier_eval", [C.c_void_p, C.c_int32, i32p, f64p, C.c_double, C.c_double, C.c_int32,
+                                          f64p, C.c_int32, i32p, i32p, f64p, f64p])):
             getattr(L, name).restype = C.c_int32
             getattr(L, name).argtypes = args
+        L.ndo_psd_project12.restype = C.c_double
+        L.ndo_psd_project12.argtypes = [f64p]
         L.ndo_adjoint.restype = C.c_int32
         L.ndo_adjoint.argtypes = [C.c_void_p, C.c_double, C.c_int32] + [f64p] * 10
         _lib = L
@@ -370,6 +377,23 @@ class OracleScene:
                                            ptr(H, f64p), C.byref(cv), ptr(cg, f64p), ptr(jc, f64p)) == 0
         return dict(f=v.value, g=g, H=H, fc=cv.value, cg=cg, C=jc.reshape(-1, 6))
 
+    def ipc_barrier_eval(self, pairs, x, dhat, kappa, psd=True):
+        """ipc mode: total energy and per-segment (lo, hi, g[12], H[12, 12])."""
+        pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 5)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        cap = max(len(pr), 1)
+        val = C.c_double()
+        ns = C.c_int32()
+        sb = np.zeros(2 * cap, dtype=np.int32)
+        sg = np.zeros(12 * cap)
+        sH = np.zeros(144 * cap)
+        rc = self.L.ndo_ipc_barrier_eval(self.h, len(pr), ptr(pr, i32p), ptr(x, f64p), dhat, kappa, 1 if psd else 0,
+                                         C.byref(val), cap, C.byref(ns), ptr(sb, i32p), ptr(sg, f64p), ptr(sH, f64p))
+        if rc != 0:
+            raise ValueError("ndo_ipc_barrier_eval: bad pair")
+        k = ns.value
+        return val.value, sb[: 2 * k].reshape(-1, 2), sg[: 12 * k].reshape(-1, 12), sH[: 144 * k].reshape(-1, 12, 12)
+
     def ccd_earliest_toi(self, dt):
         o = C.c_double()
         self.L.ndo_ccd_earliest_toi(self.h, dt, C.byref(o))
@@ -504,3 +528,41 @@ def adaptive_barrier(avg_mass, diag, min_dists, max_stiffness=0.0):
     L.ndo_adaptive_barrier(float(avg_mass), float(diag), float(max_stiffness), len(m), ptr(m, f64p), ptr(k, f64p),
                            C.byref(kmin))
     return k, kmin.value
+
+
+def ipc_pair(kind, x12, dhat, kappa, psd=False):
+    L = lib()
+    x = np.ascontiguousarray(x12, dtype=np.float64)
+    E = C.c_double()
+    g = np.zeros(12)
+    H = np.zeros(144)
+    t = C.c_int32()
+    d = C.c_double()
+    L.ndo_ipc_pair(int(kind), ptr(x, f64p), dhat, kappa, 1 if psd else 0, C.byref(E), ptr(g, f64p), ptr(H, f64p),
+                   C.byref(t), C.byref(d))
+    return E.value, g, H.reshape(12, 12), t.value, d.value
+
+
+def ipc_distance(kind, x12):
+    L = lib()
+    x = np.ascontiguousarray(x12, dtype=np.float64)
+    t = C.c_int32()
+    d = C.c_double()
+    L.ndo_ipc_distance(int(kind), ptr(x, f64p), C.byref(t), C.byref(d))
+    return d.value, t.value
+
+
+def psd_project12(H):
+    L = lib()
+    A = np.ascontiguousarray(H, dtype=np.float64).reshape(144).copy()
+    lmin = L.ndo_psd_project12(ptr(A, f64p))
+    return A.reshape(12, 12), lmin
+
+
+def vertex_jacobian(xb6, q4, r3):
+    L = lib()
+    a, b, c = (np.ascontiguousarray(v, dtype=np.float64) for v in (xb6, q4, r3))
+    xw = np.zeros(3)
+    J = np.zeros(9)
+    L.ndo_vertex_jacobian(ptr(a, f64p), ptr(b, f64p), ptr(c, f64p), ptr(xw, f64p), ptr(J, f64p))
+    return xw, J.reshape(3, 3)
