@@ -168,6 +168,23 @@ int32_t ndipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int3
 int32_t ndipc_friction_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int32_t* ints, const double* dbl,
                             const double* normal_forces, const double* x, double mu, double eps, double dt,
                             double* value, double* grad, double* hess);
+/* ipc mode: the primitive-distance barrier of rigid IPC that replaces the centre-
+ * distance barrier in the reference's plan (documents/2026.02.28/
+ * NexDynIPC_接触链路rigid-ipc完整替换实施规划.md:37-81; the reference has no code for it
+ * yet). np pairs of scene-local bodies, 5 int32 each: kind 0 point-triangle (body of the
+ * vertex, body of the face, vertex, face) or 1 edge-edge (body and unique-edge index of
+ * edge a, then of edge b); x is the scene's Newton variable (6 per body: p, theta about
+ * the stored orientation). Per pair E = kappa b(d) (b: Barrier.cpp:11-46) of the
+ * primitive distance d, with its gradient and Hessian (projected to PSD when psd != 0)
+ * folded into the two bodies' DOFs (Gauss-Newton in the rigid map). Consecutive pairs of
+ * one unordered body pair form a segment: per segment seg_bodies[2] (lower body first),
+ * seg_g[12] and seg_H[144] (row-major, lower body's 6 DOFs first); value = total energy.
+ * n_seg receives the segment count; at most cap segments are written. Fails (naming
+ * the first bad pair) on a bad kind, body or primitive index. np <= NDIPC_IPC_MAX_PAIRS. */
+#define NDIPC_IPC_MAX_PAIRS (1 << 22)
+int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const int32_t* pairs5, const double* x,
+                               double dhat, double kappa, int32_t psd, double* value, int32_t cap, int32_t* n_seg,
+                               int32_t* seg_bodies, double* seg_g, double* seg_H);
 /* Interval TOI batch (TimeOfImpact.cpp:8-150): kind 0 edge-vertex, 1 edge-edge,
  * 2 face-vertex. Per query: poseA[14], poseB[14] = p0(3) q0(4) p1(3) q1(4);
  * a_pts[6] (vertex or edge), b_pts[9] (edge or face). out: hit (1/0, -1 on NaN),

@@ -51,7 +51,7 @@ EXPORTS = [
     "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters", "ndipc_set_joint_state",
     "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
     "ndipc_autotune", "ndipc_helper_counters", "ndipc_helper_config",
-    "ndipc_set_step_mode", "ndipc_pcg_solve_noalloc", "ndipc_last_residuals",
+    "ndipc_set_step_mode", "ndipc_pcg_solve_noalloc", "ndipc_last_residuals", "ndipc_ipc_barrier_eval",
 ]
 
 
@@ -99,6 +99,8 @@ def library():
         "ndipc_scene_min_distance_flag": ([vp, C.c_int32, i32p], C.c_int32),
         "ndipc_barrier_eval": ([vp, C.c_int32, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_double, f64p, f64p,
                                 f64p, f64p], C.c_int32),
+        "ndipc_ipc_barrier_eval": ([vp, C.c_int32, C.c_int32, i32p, f64p, C.c_double, C.c_double, C.c_int32, f64p,
+                                    C.c_int32, i32p, i32p, f64p, f64p], C.c_int32),
         "ndipc_friction_eval": ([vp, C.c_int32, C.c_int32, i32p, f64p, f64p, f64p, C.c_double, C.c_double,
                                  C.c_double, f64p, f64p, f64p], C.c_int32),
         "ndipc_ccd_toi_batch": ([vp, C.c_int32, C.c_int32, f64p, f64p, f64p, f64p, f64p, C.c_double, i32p, f64p,
@@ -389,6 +391,23 @@ class Engine:
                                            dhat, kappa, C.byref(val), ptr(g, f64p), ptr(H, f64p), ptr(nf, f64p)),
                  "barrier_eval")
         return val.value, g, H, nf[: len(ints)]
+
+    def ipc_barrier_eval(self, scene, pairs, x, dhat, kappa, psd=True):
+        """ipc mode over pairs (n x 5 int32) of scene at Newton variable x: total energy and
+        per segment (lower body, higher body), gradient [12] and Hessian [12, 12]."""
+        pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 5)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        cap = max(len(pr), 1)
+        val = C.c_double()
+        ns = C.c_int32()
+        sb = np.zeros(2 * cap, dtype=np.int32)
+        sg = np.zeros(12 * cap)
+        sH = np.zeros(144 * cap)
+        self._ck(self.L.ndipc_ipc_barrier_eval(self.ctx, scene, len(pr), ptr(pr, i32p), ptr(x, f64p), dhat, kappa,
+                                               1 if psd else 0, C.byref(val), cap, C.byref(ns), ptr(sb, i32p),
+                                               ptr(sg, f64p), ptr(sH, f64p)), "ipc_barrier_eval")
+        k = ns.value
+        return val.value, sb[: 2 * k].reshape(-1, 2), sg[: 12 * k].reshape(-1, 12), sH[: 144 * k].reshape(-1, 12, 12)
 
     def friction_eval(self, scene, ints, dbl, nf, x, mu, eps, dt, hess=True):
         nd = len(x)

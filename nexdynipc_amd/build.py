@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libndipc.so")
-SOURCES = ["ctx.hip", "geom.hip", "solver.hip"]
+SOURCES = ["ctx.hip", "geom.hip", "solver.hip", "ipc.hip"]
 HOST_SOURCES = ["scene_io.cpp"]  # host-only C++ (scene and state I/O)
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",

@@ -1274,6 +1274,84 @@ int32_t ndipc_friction_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int
     return forms_eval(ctx, scene, 1, nc, ints, dbl, nf, x, 1e-3, 0.0, mu, eps, dt, value, grad, hess, nullptr);
 }
 
+int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const int32_t* pairs5, const double* x,
+                               double dhat, double kappa, int32_t psd, double* value, int32_t cap, int32_t* n_seg,
+                               int32_t* seg_bodies, double* seg_g, double* seg_H) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    if (np < 0 || np > NDIPC_IPC_MAX_PAIRS) return fail(ctx, "pair count out of range");
+    if (!(dhat > 0.0)) return fail(ctx, "dhat must be positive");
+    if (!value || !n_seg || (cap > 0 && (!seg_bodies || !seg_g || !seg_H))) return fail(ctx, "null output");
+    if (np == 0) {
+        *value = 0.0;
+        *n_seg = 0;
+        return 0;
+    }
+    if (!pairs5 || !x) return fail(ctx, "null input");
+    const int b0 = ctx->body_off[scene], nb = ctx->body_off[scene + 1] - b0;
+    const size_t n = size_t(np);
+    ScratchBufs sb;  // freed on every return
+    nd::IpcEvalBufs B;
+    int *ip = nullptr, *iw = nullptr;
+    double *dp = nullptr, *dx = nullptr;
+    unsigned long long* key = nullptr;
+    HIPCHK(sb.alloc(&ip, 4 * (5 * n + n + 1 + n + 1 + 1 + 2 * n)));
+    HIPCHK(sb.alloc(&key, 8 * n));
+    HIPCHK(sb.alloc(&dp, 8 * (n + 12 * n + 144 * n + n + 12 * n + 144 * n + 1)));
+    HIPCHK(sb.alloc(&dx, 8 * 6 * size_t(nb)));
+    iw = ip;
+    B.pairs = iw;
+    iw += 5 * n;
+    B.act = iw;
+    iw += n;
+    B.bad = iw;
+    iw += 1;
+    B.seg_start = iw;
+    iw += n + 1;
+    B.n_seg = iw;
+    iw += 1;
+    B.seg_bodies = iw;
+    B.key = key;
+    B.x = dx;
+    double* w = dp;
+    B.pe = w;
+    w += n;
+    B.pg = w;
+    w += 12 * n;
+    B.pH = w;
+    w += 144 * n;
+    B.seg_E = w;
+    w += n;
+    B.seg_g = w;
+    w += 12 * n;
+    B.seg_H = w;
+    w += 144 * n;
+    B.value = w;
+    HIPCHK(hipMemcpy(ip, pairs5, 4 * 5 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B.bad, &np, 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dx, x, 8 * 6 * size_t(nb), hipMemcpyHostToDevice));
+    nd::launch_ipc_eval(ctx->d, B, b0, nb, np, dhat, kappa, psd, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int bad = np, ns = 0;
+    HIPCHK(hipMemcpy(&bad, B.bad, 4, hipMemcpyDeviceToHost));
+    if (bad < np) {
+        char msg[96];
+        std::snprintf(msg, sizeof msg, "ipc pair %d: bad kind, body or primitive index", bad);
+        return fail(ctx, msg);
+    }
+    HIPCHK(hipMemcpy(&ns, B.n_seg, 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(value, B.value, 8, hipMemcpyDeviceToHost));
+    *n_seg = ns;
+    const size_t k = size_t(std::min(ns, std::max(cap, 0)));
+    if (k) {
+        HIPCHK(hipMemcpy(seg_bodies, B.seg_bodies, 4 * 2 * k, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(seg_g, B.seg_g, 8 * 12 * k, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(seg_H, B.seg_H, 8 * 144 * k, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
 int32_t ndipc_ccd_toi_batch(ndipc_ctx* ctx, int32_t kind, int32_t n, const double* pa, const double* pb,
                             const double* a, const double* b, const double* earliest, double tol, int32_t* hit,
                             double* toi, int32_t* pops) {

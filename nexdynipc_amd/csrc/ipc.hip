@@ -1,0 +1,346 @@
+// ipc contact mode on gfx950: the primitive-distance barrier of rigid IPC over a list of
+// point-triangle / edge-edge pairs of one scene (nexdynipc_amd/csrc/ipc_contact.h holds
+// the per-pair mathematics, shared with the host).
+//
+//   k_ipc_pairs     sixteen lanes per pair: world points and rotation Jacobians of the
+//                   four vertices, distance type, E = kappa b(d) with its vertex
+//                   gradient and Hessian (hyper-dual, split over lanes by derivative
+//                   slot), the PSD projection (Jacobi rotations over lanes, in LDS), the
+//                   fold into the two bodies' 12 DOFs; results pair-major in HBM;
+//   k_ipc_segments  one work-group: consecutive pairs of one unordered body pair form a
+//                   segment; chunked count + scan gives every segment's first pair;
+//   k_ipc_reduce    one work-group per segment (persistent over segments), one lane per
+//                   output element (E, 12 gradient, 144 Hessian entries), each summing
+//                   its element over the segment's pairs in pair order;
+//   k_ipc_total     the scene energy, the segments' energies summed in segment order.
+//
+// Every sum runs in a fixed order, so the results are bitwise those of the host
+// evaluation of the same pairs. The per-pair work is FP64 VALU and LDS (most of it the
+// 12 x 12 Jacobi eigen-decomposition of the projection: up to 66 dependent rotations
+// per sweep) with a few hundred bytes of gathered input; the reduction streams 157
+// doubles per pair.
+#include "ipc_contact.h"
+#include "ndipc_internal.h"
+
+namespace nd {
+
+constexpr int kIpcScanThreads = 1024;
+constexpr int kIpcReduceThreads = 192;  // 157 outputs per segment, three wavefronts
+constexpr int kIpcLanes = 16;           // lanes per pair
+constexpr int kIpcPairsPerWg = 4;       // pairs per work-group (one wavefront)
+constexpr int kIpcSlots = 5;            // second-derivative slots per lane: 16 x 5 >= 78
+
+// One slot of an ipc::H12 hyper-dual: the value, the first derivatives d[i], d[j] and the
+// second derivative h[(i, j)] of the slot's index pair (i <= j). Every operation of H12
+// computes d[k] from d[k] alone and h[(i, j)] from h[(i, j)], d[i], d[j], so a lane that
+// carries slot (i, j) through the same expression gets the same bits as the full H12.
+struct HL {
+    double v, di, dj, h;
+};
+__device__ __forceinline__ HL operator+(const HL& a, const HL& b) {
+    return {a.v + b.v, a.di + b.di, a.dj + b.dj, a.h + b.h};
+}
+__device__ __forceinline__ HL operator-(const HL& a, const HL& b) {
+    return {a.v - b.v, a.di - b.di, a.dj - b.dj, a.h - b.h};
+}
+__device__ __forceinline__ HL operator*(const HL& a, const HL& b) {
+    return {a.v * b.v, a.di * b.v + a.v * b.di, a.dj * b.v + a.v * b.dj,
+            (a.h * b.v + a.v * b.h) + (a.di * b.dj + a.dj * b.di)};
+}
+__device__ __forceinline__ HL operator/(const HL& a, const HL& b) {
+    const double f = 1.0 / b.v, f1 = -f * f, f2 = -2.0 * f1 * f;
+    const HL ib = {f, f1 * b.di, f1 * b.dj, f1 * b.h + f2 * b.di * b.dj};
+    return a * ib;
+}
+__device__ __forceinline__ HL hl_var(double v, int var, int si, int sj) {
+    return {v, si == var ? 1.0 : 0.0, sj == var ? 1.0 : 0.0, 0.0};
+}
+// (i, j) of packed upper-triangle index t (ipc::hix order)
+__device__ __forceinline__ void slot_ij(int t, int& i, int& j) {
+    i = 0;
+    while (t >= ipc::kN - i) {
+        t -= ipc::kN - i;
+        ++i;
+    }
+    j = i + t;
+}
+// entry (r, c) of the 12 x 12 map from the two bodies' DOFs to the four vertices
+// (ipc::fold_to_bodies' J)
+__device__ __forceinline__ double ipc_jel(int r, int c, const int* own, const double* jth) {
+    const int v = r / 3, a = r - 3 * v, o = 6 * own[v];
+    if (c == o + a) return 1.0;
+    if (c >= o + 3 && c < o + 6) return jth[9 * v + 3 * a + (c - o - 3)];
+    return 0.0;
+}
+
+// Sixteen lanes per pair, four pairs per work-group of one wavefront. Lanes 0-3 map the
+// four vertices; every lane carries up to five second-derivative slots of the squared
+// distance; the PSD projection is ipc::psd_project with each rotation's column, row and
+// eigenvector updates spread over lanes 0-11 (the rotation angle, the sweep test and
+// the break are computed as there, in the same order); the fold takes one output entry
+// per lane at a time. Control flow is uniform over the wavefront: a pair that is done
+// (or out of dhat) masks its lanes' work.
+__global__ __launch_bounds__(64) void k_ipc_pairs(Dev d, int b0, int nb, int np, const int* pairs, const double* x,
+                                                  double dhat, double kappa, int psd, unsigned long long* key,
+                                                  int* act, int* bad, double* pe, double* pg, double* pH) {
+    constexpr int N = ipc::kN;
+    __shared__ double sA[kIpcPairsPerWg][N * N], sV[kIpcPairsPerWg][N * N], sP[kIpcPairsPerWg][N * N];
+    __shared__ double sG[kIpcPairsPerWg][N], sJ[kIpcPairsPerWg][36], sX[kIpcPairsPerWg][N];
+    __shared__ int sOwn[kIpcPairsPerWg][4], sFlag[kIpcPairsPerWg];
+    const int grp = threadIdx.x / kIpcLanes, l = threadIdx.x % kIpcLanes;
+    const int i = blockIdx.x * kIpcPairsPerWg + grp;
+    int kind = -1, a = -1, b = -1, ia = -1, ib = -1;
+    if (i < np) {
+        const int* pr = pairs + 5 * i;
+        kind = pr[0];
+        a = pr[1];
+        b = pr[2];
+        ia = pr[3];
+        ib = pr[4];
+    }
+    bool ok = i < np && (kind == 0 || kind == 1) && a >= 0 && b >= 0 && a < nb && b < nb && a != b;
+    const int ga = ok ? b0 + a : 0, gb = ok ? b0 + b : 0;
+    ok = ok && d.nv[ga] > 0 && d.nv[gb] > 0 && ia >= 0 && ib >= 0;
+    ok = ok && (kind == 0 ? (ia < d.nv[ga] && ib < d.nf[gb]) : (ia < d.ne[ga] && ib < d.ne[gb]));
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    if (i < np && l == 0) {
+        if (!ok) {
+            key[i] = ~0ull;
+            act[i] = 0;
+            atomicMin(bad, i);
+        } else {
+            key[i] = (static_cast<unsigned long long>(lo) << 32) | static_cast<unsigned long long>(hi);
+        }
+    }
+    if (ok && l < 4) {
+        int g, vl;
+        if (kind == 0) {
+            g = l == 0 ? ga : gb;
+            vl = l == 0 ? ia : d.faces[3 * (d.f_off[gb] + ib) + (l - 1)];
+        } else {
+            g = l < 2 ? ga : gb;
+            vl = l < 2 ? d.edges[2 * (d.e_off[ga] + ia) + l] : d.edges[2 * (d.e_off[gb] + ib) + (l - 2)];
+        }
+        const double q[4] = {d.q[4 * g], d.q[4 * g + 1], d.q[4 * g + 2], d.q[4 * g + 3]};
+        const double* r = d.verts + 3 * (d.v_off[g] + vl);
+        const double rr[3] = {r[0], r[1], r[2]};
+        ipc::vertex_and_jacobian(x + 6 * (g - b0), q, rr, &sX[grp][3 * l], &sJ[grp][9 * l]);
+        sOwn[grp][l] = (g - b0) == lo ? 0 : 1;
+    }
+    __syncthreads();
+    double xw[N];
+    for (int k = 0; k < N; ++k) xw[k] = ok ? sX[grp][k] : 0.0;
+    const int type = ok ? ipc::pair_type(kind, xw) : 0;
+    bool active = false;
+    double dist = 0.0, E = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma unroll 1
+    for (int s = 0; s < kIpcSlots; ++s) {
+        const int t = l + kIpcLanes * s;
+        int si, sj;
+        slot_ij(t < ipc::kH ? t : 0, si, sj);
+        ipc::P3<HL> v[4];
+        for (int k = 0; k < 4; ++k)
+            v[k] = {hl_var(xw[3 * k], 3 * k, si, sj), hl_var(xw[3 * k + 1], 3 * k + 1, si, sj),
+                    hl_var(xw[3 * k + 2], 3 * k + 2, si, sj)};
+        const HL d2 = ipc::pair_d2<HL>(kind == 1 ? 1 : 0, type, v);
+        if (s == 0) {
+            dist = sqrt(d2.v > 0.0 ? d2.v : 0.0);
+            active = ok && dist < dhat;
+            if (active) {
+                E = bar_v(dist, dhat, kappa);
+                if (dist > 0.0) {
+                    const double b1 = bar_g(dist, dhat, kappa), b2 = bar_h(dist, dhat, kappa);
+                    e1 = b1 / (2.0 * dist);
+                    e2 = (b2 - b1 / dist) / (4.0 * dist * dist);
+                }
+            }
+        }
+        if (active && t < ipc::kH) {
+            double hv = 0.0, gv = 0.0;
+            if (dist > 0.0) {
+                hv = e2 * d2.di * d2.dj + e1 * d2.h;
+                gv = e1 * d2.di;
+            }
+            sA[grp][si * N + sj] = hv;
+            sA[grp][sj * N + si] = hv;
+            if (si == sj) sG[grp][si] = gv;
+        }
+    }
+    if (ok && l == 0) {
+        act[i] = active ? 1 : 0;
+        if (active) pe[i] = E;
+    }
+    __syncthreads();
+    const double* Hs = sA[grp];
+    if (psd) {
+        double* A = sA[grp];
+        double* V = sV[grp];
+        for (int idx = l; idx < N * N; idx += kIpcLanes) V[idx] = (idx / N == idx % N) ? 1.0 : 0.0;
+        double tot = 0.0;
+        if (active && l == 0)
+            for (int k = 0; k < N * N; ++k) tot += A[k] * A[k];
+        for (int sweep = 0; sweep < 50; ++sweep) {
+            if (l == 0) {
+                int done = 1;
+                if (active) {
+                    double off = 0.0;
+                    for (int r = 0; r < N; ++r)
+                        for (int c = r + 1; c < N; ++c) off += A[r * N + c] * A[r * N + c];
+                    done = (off > 1e-30 * tot) ? 0 : 1;
+                }
+                sFlag[grp] = done;
+            }
+            __syncthreads();
+            const bool gdone = sFlag[grp] != 0;
+            const bool all = sFlag[0] && sFlag[1] && sFlag[2] && sFlag[3];
+            __syncthreads();
+            if (all) break;
+            for (int p = 0; p < N; ++p)
+                for (int q = p + 1; q < N; ++q) {
+                    const double apq = A[p * N + q];
+                    const bool rot = !gdone && apq != 0.0;
+                    double c = 0.0, sn = 0.0;
+                    if (rot) {
+                        const double app = A[p * N + p], aqq = A[q * N + q];
+                        const double theta = (aqq - app) / (2.0 * apq);
+                        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        sn = t * c;
+                    }
+                    __syncthreads();
+                    if (rot && l < N) {
+                        const int k = l;
+                        const double akp = A[k * N + p], akq = A[k * N + q];
+                        A[k * N + p] = c * akp - sn * akq;
+                        A[k * N + q] = sn * akp + c * akq;
+                        const double vkp = V[k * N + p], vkq = V[k * N + q];
+                        V[k * N + p] = c * vkp - sn * vkq;
+                        V[k * N + q] = sn * vkp + c * vkq;
+                    }
+                    __syncthreads();
+                    if (rot && l < N) {
+                        const int k = l;
+                        const double apk = A[p * N + k], aqk = A[q * N + k];
+                        A[p * N + k] = c * apk - sn * aqk;
+                        A[q * N + k] = sn * apk + c * aqk;
+                    }
+                    __syncthreads();
+                }
+        }
+        if (active)
+            for (int idx = l; idx < N * N; idx += kIpcLanes) {
+                const int r = idx / N, c = idx % N;
+                double s = 0.0;
+                for (int k = 0; k < N; ++k) {
+                    const double lk = A[k * N + k];
+                    if (lk > 0.0) s += V[r * N + k] * lk * V[c * N + k];
+                }
+                sP[grp][idx] = s;
+            }
+        __syncthreads();
+        Hs = sP[grp];
+    }
+    const int* own = sOwn[grp];
+    const double* jth = sJ[grp];
+    if (active) {
+        if (l < N) {
+            double s = 0.0;
+            for (int r = 0; r < N; ++r) s += ipc_jel(r, l, own, jth) * sG[grp][r];
+            pg[12 * size_t(i) + l] = s;
+        }
+        for (int idx = l; idx < N * N; idx += kIpcLanes) {
+            const int r = idx / N, c = idx % N;
+            double s = 0.0;
+            for (int k = 0; k < N; ++k) s += Hs[r * N + k] * ipc_jel(k, c, own, jth);
+            sV[grp][idx] = s;
+        }
+    }
+    __syncthreads();
+    if (active)
+        for (int idx = l; idx < N * N; idx += kIpcLanes) {
+            const int r = idx / N, c = idx % N;
+            double s = 0.0;
+            for (int k = 0; k < N; ++k) s += ipc_jel(k, r, own, jth) * sV[grp][k * N + c];
+            pH[144 * size_t(i) + idx] = s;
+        }
+}
+
+// seg_start[s] = first pair of segment s (s < n_seg), seg_start[n_seg] = np
+__global__ __launch_bounds__(kIpcScanThreads) void k_ipc_segments(int np, const unsigned long long* key,
+                                                                   int* seg_start, int* n_seg) {
+    __shared__ int cnt[kIpcScanThreads];
+    const int t = threadIdx.x;
+    const int chunk = (np + kIpcScanThreads - 1) / kIpcScanThreads;
+    const int i0 = t * chunk < np ? t * chunk : np;
+    const int i1 = i0 + chunk < np ? i0 + chunk : np;
+    int c = 0;
+    for (int i = i0; i < i1; ++i) c += (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+    cnt[t] = c;
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int k = 0; k < kIpcScanThreads; ++k) {
+            const int ck = cnt[k];
+            cnt[k] = run;
+            run += ck;
+        }
+        *n_seg = run;
+        seg_start[run] = np;
+    }
+    __syncthreads();
+    int s = cnt[t];
+    for (int i = i0; i < i1; ++i)
+        if (i == 0 || key[i] != key[i - 1]) seg_start[s++] = i;
+}
+
+__global__ __launch_bounds__(kIpcReduceThreads) void k_ipc_reduce(const int* seg_start, const int* n_seg,
+                                                                  const unsigned long long* key, const int* act,
+                                                                  const double* pe, const double* pg,
+                                                                  const double* pH, int* seg_bodies, double* seg_E,
+                                                                  double* seg_g, double* seg_H) {
+    const int ns = *n_seg;
+    const int t = threadIdx.x;
+    for (int s = blockIdx.x; s < ns; s += gridDim.x) {
+        const int i0 = seg_start[s], i1 = seg_start[s + 1];
+        double acc = 0.0;
+        if (t == 0) {
+            for (int i = i0; i < i1; ++i)
+                if (act[i]) acc += pe[i];
+            seg_E[s] = acc;
+            const unsigned long long k = key[i0];
+            seg_bodies[2 * s] = static_cast<int>(k >> 32);
+            seg_bodies[2 * s + 1] = static_cast<int>(k & 0xffffffffull);
+        } else if (t <= 12) {
+            for (int i = i0; i < i1; ++i)
+                if (act[i]) acc += pg[12 * size_t(i) + (t - 1)];
+            seg_g[12 * size_t(s) + (t - 1)] = acc;
+        } else if (t <= 156) {
+            for (int i = i0; i < i1; ++i)
+                if (act[i]) acc += pH[144 * size_t(i) + (t - 13)];
+            seg_H[144 * size_t(s) + (t - 13)] = acc;
+        }
+    }
+}
+
+__global__ void k_ipc_total(const int* n_seg, const double* seg_E, double* value) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double v = 0.0;
+    const int ns = *n_seg;
+    for (int s = 0; s < ns; ++s) v += seg_E[s];
+    *value = v;
+}
+
+void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np, double dhat, double kappa, int psd,
+                     hipStream_t st) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_ipc_pairs, dim3((np + kIpcPairsPerWg - 1) / kIpcPairsPerWg), dim3(64), 0, st, d, b0, nb, np,
+                       B.pairs, B.x, dhat, kappa,
+                       psd, B.key, B.act, B.bad, B.pe, B.pg, B.pH);
+    hipLaunchKernelGGL(k_ipc_segments, dim3(1), dim3(kIpcScanThreads), 0, st, np, B.key, B.seg_start, B.n_seg);
+    const int grid = np < 4096 ? np : 4096;
+    hipLaunchKernelGGL(k_ipc_reduce, dim3(grid), dim3(kIpcReduceThreads), 0, st, B.seg_start, B.n_seg, B.key, B.act,
+                       B.pe, B.pg, B.pH, B.seg_bodies, B.seg_E, B.seg_g, B.seg_H);
+    hipLaunchKernelGGL(k_ipc_total, dim3(1), dim3(64), 0, st, B.n_seg, B.seg_E, B.value);
+}
+
+}  // namespace nd

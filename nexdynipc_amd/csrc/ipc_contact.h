@@ -1,6 +1,7 @@
 // Contact energy of the `ipc` mode: the primitive-distance barrier of rigid IPC that the
-// reference's authors plan to move to (documents/2026.02.28/..._完整替换实施规划.md:37-81)
-// in place of the centre-distance barrier of DistanceBarrierForm (ref mode). Per
+// reference's authors plan to move to (documents/2026.02.28/
+// NexDynIPC_接触链路rigid-ipc完整替换实施规划.md:37-81) in place of the centre-distance
+// barrier of DistanceBarrierForm (ref mode). Per
 // point-triangle (PT) or edge-edge (EE) pair of world points x[12] (PT: p, t0, t1, t2;
 // EE: a0, a1, b0, b1):
 //   - the distance type: which Voronoi region of the triangle / which clamped closest-

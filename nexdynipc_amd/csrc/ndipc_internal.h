@@ -295,6 +295,23 @@ void launch_barrier_init_ccd(const Dev& d, const Work& w, hipStream_t st);
 void launch_forms_eval(const Dev& d, const Work& w, int s, int which, double dhat, double kappa, double mu,
                        double eps, double dt, const double* x, double* value, double* grad, double* hess, double* nf,
                        hipStream_t st);
+// ipc mode over np pairs of one scene (bodies b0 .. b0 + nb - 1): per-pair terms,
+// segments of consecutive pairs of one body pair, per-segment sums, total energy
+struct IpcEvalBufs {
+    const int* pairs;              // [5 np] kind, body, body, primitive, primitive
+    const double* x;               // [6 nb] Newton variable of the scene
+    unsigned long long* key;       // [np]
+    int* act;                      // [np]
+    int* bad;                      // [1] first invalid pair, np if none (set before launch)
+    double *pe, *pg, *pH;          // [np], [12 np], [144 np]
+    int* seg_start;                // [np + 1]
+    int* n_seg;                    // [1]
+    int* seg_bodies;               // [2 np]
+    double *seg_E, *seg_g, *seg_H; // [np], [12 np], [144 np]
+    double* value;                 // [1]
+};
+void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np, double dhat, double kappa, int psd,
+                     hipStream_t st);
 void launch_pcg(int nb, const double* diag, int noff, const int* offij, const double* offb, const double* b,
                 double tol, int max_it, double* x, int* iters, double* rel, double* scratch, hipStream_t st);
 

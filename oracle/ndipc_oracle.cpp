@@ -1231,14 +1231,17 @@ struct Solver {
     // list of pairs (kind 0 PT: body of the vertex, body of the face, vertex, face; kind 1
     // EE: bodies and mesh edge indices of edge a and edge b) at Newton variable x. Pairs
     // whose bodies form the same unordered pair and follow each other form a segment;
-    // per segment the slot-0 body is the lower index. Sums in pair order.
+    // per segment the slot-0 body is the lower index. Per segment the energy, gradient
+    // and Hessian are summed in pair order over the pairs within dhat; the total energy
+    // is the sum of the segments' energies in segment order.
     struct IpcSeg {
         int lo, hi;
-        double g[12], H[144];
+        double E, g[12], H[144];
     };
     bool ipc_pair_vertices(const int32_t* pr, const vector<double>& x, double xw[12], int owner[4],
                            double Jth[4][9], int& lo, int& hi) const {
         const int kind = pr[0], b0 = pr[1], b1 = pr[2];
+        if (kind != 0 && kind != 1) return false;
         if (b0 < 0 || b1 < 0 || b0 >= n() || b1 >= n() || b0 == b1) return false;
         if (bodies[b0].mesh < 0 || bodies[b1].mesh < 0) return false;
         const Mesh& M0 = meshes[bodies[b0].mesh];
@@ -1284,13 +1287,14 @@ struct Solver {
                 segs.push_back(sg);
             }
             if (!nd::ipc::pair_energy(pr[0], xw, dhat, kappa, &E, g, H, nullptr, nullptr)) continue;
-            value += E;
             if (psd) nd::ipc::psd_project<12>(H);
             nd::ipc::fold_to_bodies(g, H, owner, Jth, gb, Hb);
             IpcSeg& sg = segs.back();
+            sg.E += E;
             for (int k = 0; k < 12; ++k) sg.g[k] += gb[k];
             for (int k = 0; k < 144; ++k) sg.H[k] += Hb[k];
         }
+        for (const IpcSeg& sg : segs) value += sg.E;
         return true;
     }
 
