@@ -185,10 +185,27 @@ int32_t ndipc_friction_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int
 int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const int32_t* pairs5, const double* x,
                                double dhat, double kappa, int32_t psd, double* value, int32_t cap, int32_t* n_seg,
                                int32_t* seg_bodies, double* seg_g, double* seg_H);
+/* ipc mode, OptimizationProblem::maxStep (NewtonSolver.h:42, called at
+ * NewtonSolver.cpp:53-54) over the pairs of ndipc_ipc_barrier_eval's format: per pair,
+ * conservative advancement along the line-search path x + t dx (positions p + t dp,
+ * orientations apply_theta(q, theta + t dtheta)), t in [0, 1], keeping sep (in [0, 1))
+ * of the pair's current distance: the pair's distance provably stays above sep * d(0)
+ * up to its advance (at most 1000 steps per pair; a pair that would need more reports
+ * where it stopped, which is still safe). toi = the least advance (+inf if every pair
+ * clears the whole step); alpha = toi * safety when toi < 1 (the form of
+ * IPCSolver::computeMaxStep, IPCSolver.cpp:385-391), else 1. Fails naming the first
+ * bad pair. */
+int32_t ndipc_ipc_max_step(ndipc_ctx* ctx, int32_t scene, int32_t np, const int32_t* pairs5, const double* x,
+                           const double* dx, double sep, double safety, double* alpha, double* toi);
 /* Interval TOI batch (TimeOfImpact.cpp:8-150): kind 0 edge-vertex, 1 edge-edge,
- * 2 face-vertex. Per query: poseA[14], poseB[14] = p0(3) q0(4) p1(3) q1(4);
- * a_pts[6] (vertex or edge), b_pts[9] (edge or face). out: hit (1/0, -1 on NaN),
- * toi, pops. */
+ * 2 face-vertex as the reference computes them; the ipc mode's 3 face-vertex (boxes that
+ * meet the triangle u, v >= 0, u + v <= 1 are kept, where the reference's predicate
+ * drops the root box) and 4 edge-edge, both in body-relative form (relative translation
+ * linear in t over each box, rotated vectors as slerp-endpoint hulls widened by the
+ * arc's distance from its chord), searched earliest first; when the 1000-pop cap stops
+ * them they report an impact at the least t.lo still unexplored (a lower bound). Per query:
+ * poseA[14], poseB[14] = p0(3) q0(4) p1(3) q1(4); a_pts[6] (vertex or edge), b_pts[9]
+ * (edge or face). out: hit (1/0, -1 on NaN, -2 on stack overflow), toi, pops. */
 int32_t ndipc_ccd_toi_batch(ndipc_ctx* ctx, int32_t kind, int32_t n, const double* poseA, const double* poseB,
                             const double* a_pts, const double* b_pts, const double* earliest, double tol,
                             int32_t* hit, double* toi, int32_t* pops);

@@ -52,6 +52,7 @@ EXPORTS = [
     "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
     "ndipc_autotune", "ndipc_helper_counters", "ndipc_helper_config",
     "ndipc_set_step_mode", "ndipc_pcg_solve_noalloc", "ndipc_last_residuals", "ndipc_ipc_barrier_eval",
+    "ndipc_ipc_max_step",
 ]
 
 
@@ -101,6 +102,8 @@ def library():
                                 f64p, f64p], C.c_int32),
         "ndipc_ipc_barrier_eval": ([vp, C.c_int32, C.c_int32, i32p, f64p, C.c_double, C.c_double, C.c_int32, f64p,
                                     C.c_int32, i32p, i32p, f64p, f64p], C.c_int32),
+        "ndipc_ipc_max_step": ([vp, C.c_int32, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_double, f64p, f64p],
+                               C.c_int32),
         "ndipc_friction_eval": ([vp, C.c_int32, C.c_int32, i32p, f64p, f64p, f64p, C.c_double, C.c_double,
                                  C.c_double, f64p, f64p, f64p], C.c_int32),
         "ndipc_ccd_toi_batch": ([vp, C.c_int32, C.c_int32, f64p, f64p, f64p, f64p, f64p, C.c_double, i32p, f64p,
@@ -408,6 +411,17 @@ class Engine:
                                                ptr(sg, f64p), ptr(sH, f64p)), "ipc_barrier_eval")
         k = ns.value
         return val.value, sb[: 2 * k].reshape(-1, 2), sg[: 12 * k].reshape(-1, 12), sH[: 144 * k].reshape(-1, 12, 12)
+
+    def ipc_max_step(self, scene, pairs, x, dx, sep=0.1, safety=0.8):
+        """ipc mode maxStep over pairs (n x 5 int32) of scene along dx from x, keeping sep of
+        each pair's distance: (alpha, toi)."""
+        pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 5)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        dx = np.ascontiguousarray(dx, dtype=np.float64)
+        a, t = C.c_double(), C.c_double()
+        self._ck(self.L.ndipc_ipc_max_step(self.ctx, scene, len(pr), ptr(pr, i32p), ptr(x, f64p), ptr(dx, f64p), sep,
+                                           safety, C.byref(a), C.byref(t)), "ipc_max_step")
+        return a.value, t.value
 
     def friction_eval(self, scene, ints, dbl, nf, x, mu, eps, dt, hess=True):
         nd = len(x)

@@ -1352,12 +1352,59 @@ int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const 
     return 0;
 }
 
+int32_t ndipc_ipc_max_step(ndipc_ctx* ctx, int32_t scene, int32_t np, const int32_t* pairs5, const double* x,
+                           const double* dx, double sep, double safety, double* alpha, double* toi) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    if (np < 0 || np > NDIPC_IPC_MAX_PAIRS) return fail(ctx, "pair count out of range");
+    if (!(sep >= 0.0 && sep < 1.0)) return fail(ctx, "sep must be in [0, 1)");
+    if (!alpha || !toi) return fail(ctx, "null output");
+    if (np == 0) {
+        *toi = 1.0 / 0.0;
+        *alpha = 1.0;
+        return 0;
+    }
+    if (!pairs5 || !x || !dx) return fail(ctx, "null input");
+    const int b0 = ctx->body_off[scene], nb = ctx->body_off[scene + 1] - b0;
+    const size_t n = size_t(np), nd6 = 6 * size_t(nb);
+    ScratchBufs sb;  // freed on every return
+    int* ib = nullptr;
+    double* db = nullptr;
+    HIPCHK(sb.alloc(&ib, 4 * (5 * n + n + 1)));
+    HIPCHK(sb.alloc(&db, 8 * (2 * nd6 + n + 1)));
+    int* dpairs = ib;
+    int* dst = ib + 5 * n;
+    int* dbad = dst + n;
+    double* dx0 = db;
+    double* ddx = db + nd6;
+    double* dtoi = ddx + nd6;
+    double* dout_t = dtoi + n;
+    HIPCHK(hipMemcpy(dpairs, pairs5, 4 * 5 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dx0, x, 8 * nd6, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ddx, dx, 8 * nd6, hipMemcpyHostToDevice));
+    nd::launch_ipc_advance(ctx->d, b0, nb, np, dpairs, dx0, ddx, sep, dtoi, dst, dout_t, dbad, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int bad = np;
+    double t = 0.0;
+    HIPCHK(hipMemcpy(&bad, dbad, 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&t, dout_t, 8, hipMemcpyDeviceToHost));
+    if (bad < np) {
+        char msg[96];
+        std::snprintf(msg, sizeof msg, "ipc pair %d: bad kind, body or primitive index", bad);
+        return fail(ctx, msg);
+    }
+    *toi = t;
+    *alpha = t < 1.0 ? t * safety : 1.0;
+    return 0;
+}
+
 int32_t ndipc_ccd_toi_batch(ndipc_ctx* ctx, int32_t kind, int32_t n, const double* pa, const double* pb,
                             const double* a, const double* b, const double* earliest, double tol, int32_t* hit,
                             double* toi, int32_t* pops) {
     if (!ctx) return -1;
     HIPCHK(hipSetDevice(ctx->device));
-    if (kind < 0 || kind > 2) return fail(ctx, "kind must be 0, 1 or 2");
+    if (kind < 0 || kind > 4) return fail(ctx, "kind must be 0 to 4");
     if (n <= 0) return 0;
     const size_t dbls = size_t(n) * (14 + 14 + 6 + 9 + 1 + 1);
     double* buf = nullptr;

@@ -1565,6 +1565,36 @@ __device__ __forceinline__ IV3 face_traj(const CPose& ps, V3 a, V3 b, V3 c, Iv t
     return r;
 }
 
+// Query kinds 0-2 (edge-vertex, edge-edge, face-vertex) follow the reference. Kinds 3
+// (face-vertex) and 4 (edge-edge) are the ipc mode's, written in body-relative form:
+// the relative translation p_A(t) - p_B(t), linear in t over the box, plus each rotated
+// body-frame point or edge vector as rot_hull, the slerp-endpoint hull widened by the
+// arc's largest distance from its chord, |r| (1 - cos(phi / 2)) with cos(phi / 2) =
+// |<q(t.lo), q(t.hi)>|, plus 8e-15 |r| for the rounding of the endpoint rotations.
+__device__ __forceinline__ IV3 rot_hull(const CPose& ps, V3 r, Iv t) {
+    const Q qlo = slerp(ps.q0, t.lo, ps.q1), qhi = slerp(ps.q0, t.hi, ps.q1);
+    const V3 a = rotate(qlo, r), b = rotate(qhi, r);
+    const double cd = dmin(1.0, fabs(((qlo.w * qhi.w + qlo.x * qhi.x) + qlo.y * qhi.y) + qlo.z * qhi.z));
+    const double rl = sqrt((r.x * r.x + r.y * r.y) + r.z * r.z);
+    const double e = rl * (1.0 - cd) + 8e-15 * rl;
+    IV3 o;
+    o.x = iv(sub_dn(dmin(a.x, b.x), e), add_up(dmax(a.x, b.x), e));
+    o.y = iv(sub_dn(dmin(a.y, b.y), e), add_up(dmax(a.y, b.y), e));
+    o.z = iv(sub_dn(dmin(a.z, b.z), e), add_up(dmax(a.z, b.z), e));
+    return o;
+}
+__device__ __forceinline__ Iv rel_trans1(double a0, double a1, double b0, double b1, Iv t) {
+    const Iv pa0 = iv(a0, a0), pb0 = iv(b0, b0);
+    return (pa0 - pb0) + t * ((iv(a1, a1) - pa0) - (iv(b1, b1) - pb0));
+}
+__device__ __forceinline__ IV3 rel_trans(const CPose& A, const CPose& B, Iv t) {
+    IV3 o;
+    o.x = rel_trans1(A.p0.x, A.p1.x, B.p0.x, B.p1.x, t);
+    o.y = rel_trans1(A.p0.y, A.p1.y, B.p0.y, B.p1.y, t);
+    o.z = rel_trans1(A.p0.z, A.p1.z, B.p0.z, B.p1.z, t);
+    return o;
+}
+
 struct CcdQuery {
     int kind;
     CPose A, B;
@@ -1581,27 +1611,52 @@ __device__ __forceinline__ IV3 ccd_f(const CcdQuery& q, const Iv* x, bool& nan) 
         const IV3 ea = edge_traj(q.A, q.a[0], q.a[1], x[0], x[1]);
         const IV3 eb = edge_traj(q.B, q.b[0], q.b[1], x[0], x[2]);
         r.x = ea.x - eb.x; r.y = ea.y - eb.y; r.z = ea.z - eb.z;
-    } else {
+    } else if (q.kind == 2) {
         const IV3 v = vertex_traj(q.A, q.a[0], x[0]);
         const IV3 f = face_traj(q.B, q.b[0], q.b[1], q.b[2], x[0], x[1], x[2]);
         r.x = v.x - f.x; r.y = v.y - f.y; r.z = v.z - f.z;
+    } else if (q.kind == 3) {
+        // (p_A - p_B) + R_A a - R_B b0 - (u R_B (b1 - b0) + v R_B (b2 - b0))
+        const Iv t = x[0], u = x[1], v = x[2];
+        const IV3 tr = rel_trans(q.A, q.B, t), ra = rot_hull(q.A, q.a[0], t), rb = rot_hull(q.B, q.b[0], t);
+        const IV3 r1 = rot_hull(q.B, q.b[1] - q.b[0], t), r2 = rot_hull(q.B, q.b[2] - q.b[0], t);
+        r.x = ((tr.x + ra.x) - rb.x) - (u * r1.x + v * r2.x);
+        r.y = ((tr.y + ra.y) - rb.y) - (u * r1.y + v * r2.y);
+        r.z = ((tr.z + ra.z) - rb.z) - (u * r1.z + v * r2.z);
+    } else {
+        // (p_A - p_B) + R_A a0 + alpha R_A (a1 - a0) - (R_B b0 + beta R_B (b1 - b0))
+        const Iv t = x[0], al = x[1], be = x[2];
+        const IV3 tr = rel_trans(q.A, q.B, t), ra = rot_hull(q.A, q.a[0], t), rea = rot_hull(q.A, q.a[1] - q.a[0], t);
+        const IV3 rb = rot_hull(q.B, q.b[0], t), reb = rot_hull(q.B, q.b[1] - q.b[0], t);
+        r.x = ((tr.x + ra.x) + al * rea.x) - (rb.x + be * reb.x);
+        r.y = ((tr.y + ra.y) + al * rea.y) - (rb.y + be * reb.y);
+        r.z = ((tr.z + ra.z) + al * rea.z) - (rb.z + be * reb.z);
     }
     if (iv_nan(r.x) || iv_nan(r.y) || iv_nan(r.z)) nan = true;
     return r;
 }
 
+// Kind 2 keeps the reference's predicate (every barycentric weight certainly in [0, 1],
+// which the root box u = v = [0, 1] fails). Kind 3 keeps a box that meets the triangle
+// u, v >= 0, u + v <= 1: with u and v inside [0, 1] by construction, the box meets it
+// when its lowest corner does, upper(1 - u.lo - v.lo) >= 0.
 __device__ __forceinline__ bool ccd_valid(const CcdQuery& q, const Iv* x) {
-    if (q.kind != 2) return true;
+    if (q.kind != 2 && q.kind != 3) return true;
     const Iv u = x[1], v = x[2];
     const Iv w = (iv(1.0, 1.0) - u) - v;
+    if (q.kind == 3) return u.hi >= 0 && v.hi >= 0 && w.hi >= 0;
     return u.lo >= 0 && u.hi <= 1 && v.lo >= 0 && v.hi <= 1 && w.lo >= 0 && w.hi <= 1;
 }
 
 constexpr int kCcdStack = 100;
 
-// Returns 1 on impact, 0 otherwise, -1 on NaN, -2 on stack overflow.
+// Returns 1 on impact, 0 otherwise, -1 on NaN, -2 on stack overflow. The ipc mode's
+// kinds 3 and 4 search earliest first (low half popped first), and a search stopped by
+// the pop cap reports an impact at the least t.lo of its best leaf and of every box
+// still on the stack (a lower bound of any impact it did not rule out).
 __device__ int ccd_toi(const CcdQuery& q, double earliest, double tol, double* toi, int* pops) {
     const int D = q.kind == 0 ? 2 : 3;
+    const bool early = q.kind >= 3;
     const double tl[3] = {tol, 1.0, 1.0};
     Iv stk[kCcdStack][3];
     int sp = 0;
@@ -1644,12 +1699,20 @@ __device__ int ccd_toi(const CcdQuery& q, double earliest, double tol, double* t
         if (sp + 2 > kCcdStack) { *pops = it; return -2; }
         const double mid = iv_median(x[sd]);
         for (int k = 0; k < 3; ++k) { stk[sp][k] = x[k]; stk[sp + 1][k] = x[k]; }
-        stk[sp][sd] = iv(x[sd].lo, mid);
-        stk[sp + 1][sd] = iv(mid, x[sd].hi);
+        // the half on top (sp + 1) is popped next: the high half, or for the ipc
+        // mode's kinds the low half
+        stk[sp][sd] = early ? iv(mid, x[sd].hi) : iv(x[sd].lo, mid);
+        stk[sp + 1][sd] = early ? iv(x[sd].lo, mid) : iv(mid, x[sd].hi);
         sp += 2;
     }
     *pops = it;
     if (nan) return -1;
+    if (early && it >= 1000 && sp > 0) {
+        double m = best[0].lo;
+        for (int k = 0; k < sp; ++k) m = dmin(m, stk[k][0].lo);
+        best[0] = iv(m, m);
+        found = true;
+    }
     *toi = found ? best[0].lo : 1.0 / 0.0;
     return found ? 1 : 0;
 }
@@ -1684,7 +1747,7 @@ __global__ void k_ccd_batch(int kind, int n, const double* pa, const double* pb,
     q.kind = kind;
     q.A = ld_cpose(pa + 14 * i);
     q.B = ld_cpose(pb + 14 * i);
-    const int na = kind == 1 ? 2 : 1, nb = kind == 2 ? 3 : 2;
+    const int na = (kind == 1 || kind == 4) ? 2 : 1, nb = (kind == 2 || kind == 3) ? 3 : 2;
     for (int k = 0; k < na; ++k) q.a[k] = ld3(a + 6 * i, k);
     for (int k = 0; k < nb; ++k) q.b[k] = ld3(b + 9 * i, k);
     double t = 1.0 / 0.0;

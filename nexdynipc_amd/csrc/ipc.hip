@@ -12,7 +12,9 @@
 //   k_ipc_reduce    one work-group per segment (persistent over segments), one lane per
 //                   output element (E, 12 gradient, 144 Hessian entries), each summing
 //                   its element over the segment's pairs in pair order;
-//   k_ipc_total     the scene energy, the segments' energies summed in segment order.
+//   k_ipc_total     the scene energy, the segments' energies summed in segment order;
+//   k_ipc_advance   maxStep: one lane per pair, conservative advancement along the
+//                   line-search path, then the least over the pairs.
 //
 // Every sum runs in a fixed order, so the results are bitwise those of the host
 // evaluation of the same pairs. The per-pair work is FP64 VALU and LDS (most of it the
@@ -328,6 +330,117 @@ __global__ void k_ipc_total(const int* n_seg, const double* seg_E, double* value
     const int ns = *n_seg;
     for (int s = 0; s < ns; ++s) v += seg_E[s];
     *value = v;
+}
+
+// ipc mode, OptimizationProblem::maxStep (NewtonSolver.h:42, NewtonSolver.cpp:53-54),
+// one lane per pair: conservative advancement (ipc::pair_advance) along the line-search
+// path x + t dx (positions p + t dp, orientations apply_theta(q, theta + t dtheta)),
+// keeping sep of the pair's current distance. mu = sum over the two bodies of |dp| +
+// |dtheta| r_max (r_max: the largest body-frame radius of the pair's vertices on that
+// body), margin = 2e-10 (r_max A + r_max B) for the rotation skipped below |theta| =
+// 1e-10. toi[i] = the pair's advance (+inf: no impact in the step); st[i] = -3 for a
+// bad pair.
+constexpr int kIpcAdvanceIters = 1000;
+
+__global__ void k_ipc_advance(Dev d, int b0, int nb, int np, const int* pairs, const double* x, const double* dx,
+                              double sep, double* toi, int* st) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int* pr = pairs + 5 * i;
+    const int kind = pr[0], a = pr[1], b = pr[2], ia = pr[3], ib = pr[4];
+    bool ok = (kind == 0 || kind == 1) && a >= 0 && b >= 0 && a < nb && b < nb && a != b;
+    const int ga = ok ? b0 + a : 0, gb = ok ? b0 + b : 0;
+    ok = ok && d.nv[ga] > 0 && d.nv[gb] > 0 && ia >= 0 && ib >= 0;
+    ok = ok && (kind == 0 ? (ia < d.nv[ga] && ib < d.nf[gb]) : (ia < d.ne[ga] && ib < d.ne[gb]));
+    if (!ok) {
+        toi[i] = 1.0 / 0.0;
+        st[i] = -3;
+        return;
+    }
+    int vb[4];
+    V3 r[4];
+    if (kind == 0) {
+        vb[0] = a;
+        r[0] = ld3(d.verts, d.v_off[ga] + ia);
+        const int* f = d.faces + 3 * (d.f_off[gb] + ib);
+        for (int k = 0; k < 3; ++k) {
+            vb[1 + k] = b;
+            r[1 + k] = ld3(d.verts, d.v_off[gb] + f[k]);
+        }
+    } else {
+        const int* ea = d.edges + 2 * (d.e_off[ga] + ia);
+        const int* eb = d.edges + 2 * (d.e_off[gb] + ib);
+        for (int k = 0; k < 2; ++k) {
+            vb[k] = a;
+            r[k] = ld3(d.verts, d.v_off[ga] + ea[k]);
+            vb[2 + k] = b;
+            r[2 + k] = ld3(d.verts, d.v_off[gb] + eb[k]);
+        }
+    }
+    double rmax[2] = {0.0, 0.0};
+    for (int v = 0; v < 4; ++v) {
+        const int side = vb[v] == a ? 0 : 1;
+        rmax[side] = dmax(rmax[side], sqrt((r[v].x * r[v].x + r[v].y * r[v].y) + r[v].z * r[v].z));
+    }
+    double mu = 0.0;
+    for (int side = 0; side < 2; ++side) {
+        const int o = 6 * (side == 0 ? a : b);
+        const double lp = sqrt((dx[o] * dx[o] + dx[o + 1] * dx[o + 1]) + dx[o + 2] * dx[o + 2]);
+        const double lt = sqrt((dx[o + 3] * dx[o + 3] + dx[o + 4] * dx[o + 4]) + dx[o + 5] * dx[o + 5]);
+        mu = mu + (lp + lt * rmax[side]);
+    }
+    const double margin = 2e-10 * (rmax[0] + rmax[1]);
+    Q q0[2];
+    q0[0] = ld4(d.q, ga);
+    q0[1] = ld4(d.q, gb);
+    auto world = [&](double t, double* xw) {
+        for (int v = 0; v < 4; ++v) {
+            const int o = 6 * vb[v];
+            const V3 p = v3(x[o] + t * dx[o], x[o + 1] + t * dx[o + 1], x[o + 2] + t * dx[o + 2]);
+            const V3 th = v3(x[o + 3] + t * dx[o + 3], x[o + 4] + t * dx[o + 4], x[o + 5] + t * dx[o + 5]);
+            const V3 w = p + rotate(apply_theta(q0[vb[v] == a ? 0 : 1], th), r[v]);
+            xw[3 * v] = w.x;
+            xw[3 * v + 1] = w.y;
+            xw[3 * v + 2] = w.z;
+        }
+    };
+    toi[i] = ipc::pair_advance(kind, world, mu, sep, margin, kIpcAdvanceIters);
+    st[i] = 0;
+}
+
+// least advance (order-free, so exact) and the first bad pair (np: none)
+__global__ __launch_bounds__(1024) void k_ipc_advance_min(int np, const double* toi, const int* st, double* out_t,
+                                                          int* out_bad) {
+    __shared__ double sm[1024];
+    __shared__ int sb[1024];
+    double m = 1.0 / 0.0;
+    int fb = np;
+    for (int i = threadIdx.x; i < np; i += 1024) {
+        m = dmin(m, toi[i]);
+        if (st[i] == -3 && i < fb) fb = i;
+    }
+    sm[threadIdx.x] = m;
+    sb[threadIdx.x] = fb;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            sm[threadIdx.x] = dmin(sm[threadIdx.x], sm[threadIdx.x + h]);
+            sb[threadIdx.x] = min(sb[threadIdx.x], sb[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out_t = sm[0];
+        *out_bad = sb[0];
+    }
+}
+
+void launch_ipc_advance(const Dev& d, int b0, int nb, int np, const int* pairs, const double* x, const double* dx,
+                        double sep, double* toi, int* st, double* out_t, int* out_bad, hipStream_t stream) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_ipc_advance, dim3((np + 63) / 64), dim3(64), 0, stream, d, b0, nb, np, pairs, x, dx, sep, toi,
+                       st);
+    hipLaunchKernelGGL(k_ipc_advance_min, dim3(1), dim3(1024), 0, stream, np, toi, st, out_t, out_bad);
 }
 
 void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np, double dhat, double kappa, int psd,

@@ -243,6 +243,33 @@ ND_DU double pair_distance(int kind, const double* x, int* type_out) {
     return sqrt(d2 > 0.0 ? d2 : 0.0);
 }
 
+// Conservative advancement of one pair (Mirtich 1996; the additive form of Li, Kaufman
+// and Jiang 2021) along t in [0, 1]: world(t, xw) writes the pair's four world points at
+// t; mu bounds how fast the distance can shrink (|d'(t)| <= mu); the pair keeps
+// sep * d(0) + margin. From d = d(0), t advances by 0.9 (d - keep) / mu, so d stays
+// above keep up to the returned t. Returns that t once the gap d - keep is down to 1e-6
+// of its start (or after max_iter advances), +inf when t passes 1 first or nothing
+// moves.
+template <class F>
+ND_DU double pair_advance(int kind, F&& world, double mu, double sep, double margin, int max_iter) {
+    double xw[kN];
+    world(0.0, xw);
+    const double d0 = pair_distance(kind, xw, nullptr);
+    const double keep = sep * d0 + margin;
+    if (!(mu > 0.0)) return __builtin_inf();
+    const double stop = 1e-6 * (d0 - keep);
+    double t = 0.0, d = d0;
+    for (int k = 0; k < max_iter; ++k) {
+        const double gap = d - keep;
+        if (!(gap > stop) || !(gap > 0.0)) return t;
+        t = t + 0.9 * gap / mu;
+        if (!(t < 1.0)) return __builtin_inf();
+        world(t, xw);
+        d = pair_distance(kind, xw, nullptr);
+    }
+    return t;
+}
+
 // E = kappa b(d) of the pair with gradient g[12] and Hessian H[144] (row-major) in x.
 // Returns false (E = 0, g = H = 0) when d >= dhat; E = +inf when d <= 0.
 ND_DU bool pair_energy(int kind, const double* x, double dhat, double kappa, double* E, double* g, double* H,

@@ -312,6 +312,10 @@ struct IpcEvalBufs {
 };
 void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np, double dhat, double kappa, int psd,
                      hipStream_t st);
+// ipc mode maxStep: conservative advancement of every pair from x along dx; out_t the
+// least advance (+inf: none), out_bad the first bad pair (np: none)
+void launch_ipc_advance(const Dev& d, int b0, int nb, int np, const int* pairs, const double* x, const double* dx,
+                        double sep, double* toi, int* st, double* out_t, int* out_bad, hipStream_t stream);
 void launch_pcg(int nb, const double* diag, int noff, const int* offij, const double* offb, const double* b,
                 double tol, int max_it, double* x, int* iters, double* rel, double* scratch, hipStream_t st);
 

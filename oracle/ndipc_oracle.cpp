@@ -1234,26 +1234,34 @@ struct Solver {
     // per segment the slot-0 body is the lower index. Per segment the energy, gradient
     // and Hessian are summed in pair order over the pairs within dhat; the total energy
     // is the sum of the segments' energies in segment order.
+    static constexpr int kIpcAdvanceIters = 1000;  // conservative-advancement steps per pair
     struct IpcSeg {
         int lo, hi;
         double E, g[12], H[144];
     };
-    bool ipc_pair_vertices(const int32_t* pr, const vector<double>& x, double xw[12], int owner[4],
-                           double Jth[4][9], int& lo, int& hi) const {
+    bool ipc_pair_ok(const int32_t* pr) const {
         const int kind = pr[0], b0 = pr[1], b1 = pr[2];
         if (kind != 0 && kind != 1) return false;
         if (b0 < 0 || b1 < 0 || b0 >= n() || b1 >= n() || b0 == b1) return false;
         if (bodies[b0].mesh < 0 || bodies[b1].mesh < 0) return false;
         const Mesh& M0 = meshes[bodies[b0].mesh];
         const Mesh& M1 = meshes[bodies[b1].mesh];
+        if (pr[3] < 0 || pr[4] < 0) return false;
+        if (kind == 0) return pr[3] < int(M0.V.size()) && pr[4] < int(M1.F.size());
+        return pr[3] < int(M0.E.size()) && pr[4] < int(M1.E.size());
+    }
+    bool ipc_pair_vertices(const int32_t* pr, const vector<double>& x, double xw[12], int owner[4],
+                           double Jth[4][9], int& lo, int& hi) const {
+        if (!ipc_pair_ok(pr)) return false;
+        const int kind = pr[0], b0 = pr[1], b1 = pr[2];
+        const Mesh& M0 = meshes[bodies[b0].mesh];
+        const Mesh& M1 = meshes[bodies[b1].mesh];
         int vb[4];
         V3 r[4];
         if (kind == 0) {
-            if (pr[3] < 0 || pr[3] >= int(M0.V.size()) || pr[4] < 0 || pr[4] >= int(M1.F.size())) return false;
             vb[0] = b0; r[0] = M0.V[pr[3]];
             for (int k = 0; k < 3; ++k) { vb[1 + k] = b1; r[1 + k] = M1.V[M1.F[pr[4]][k]]; }
         } else {
-            if (pr[3] < 0 || pr[3] >= int(M0.E.size()) || pr[4] < 0 || pr[4] >= int(M1.E.size())) return false;
             for (int k = 0; k < 2; ++k) {
                 vb[k] = b0; r[k] = M0.V[M0.E[pr[3]][k]];
                 vb[2 + k] = b1; r[2 + k] = M1.V[M1.E[pr[4]][k]];
@@ -1297,6 +1305,9 @@ struct Solver {
         for (const IpcSeg& sg : segs) value += sg.E;
         return true;
     }
+    // ipc mode maxStep over a pair list; 0, or 1 with fail_index = the first bad pair
+    int ipc_max_step(const int32_t* pairs, int np, const vector<double>& x, const vector<double>& dx, double sep,
+                     double safety, double& toi, double& alpha, int& fail_index) const;
 
     double scene_min_distance() const {
         if (prune_scene_min) return scene_min_distance_pruned();
@@ -1878,13 +1889,41 @@ static I3 face_traj(const Pose& ps, const V3& a, const V3& b, const V3& c, const
     return {wi * v0.x + ui * v1.x + vi * v2.x, wi * v0.y + ui * v1.y + vi * v2.y, wi * v0.z + ui * v1.z + vi * v2.z};
 }
 
+// ipc mode (query kinds 3 and 4, no reference counterpart). R(t) r over the box: the
+// slerp-endpoint hull widened by the arc's largest distance from its chord,
+// |r| (1 - cos(phi / 2)) with cos(phi / 2) = |<q(t.lo), q(t.hi)>|, plus 8e-15 |r| for
+// the rounding of the endpoint rotations.
+static I3 rot_hull(const Pose& ps, const V3& r, const I& t) {
+    const Q qlo = slerp(ps.q0, t.lo, ps.q1), qhi = slerp(ps.q0, t.hi, ps.q1);
+    const V3 a = rotate(qlo, r), b = rotate(qhi, r);
+    const double cd = std::min(1.0, std::abs(((qlo.w * qhi.w + qlo.x * qhi.x) + qlo.y * qhi.y) + qlo.z * qhi.z));
+    const double rl = std::sqrt((r.x * r.x + r.y * r.y) + r.z * r.z);
+    const double e = rl * (1.0 - cd) + 8e-15 * rl;
+    return {I(sub_dn(std::min(a.x, b.x), e), add_up(std::max(a.x, b.x), e)),
+            I(sub_dn(std::min(a.y, b.y), e), add_up(std::max(a.y, b.y), e)),
+            I(sub_dn(std::min(a.z, b.z), e), add_up(std::max(a.z, b.z), e))};
+}
+// p_A(t) - p_B(t) over the box, both translations linear in t
+static I rel_trans1(double a0, double a1, double b0, double b1, const I& t) {
+    const I pa0(a0), pb0(b0);
+    return (pa0 - pb0) + t * ((I(a1) - pa0) - (I(b1) - pb0));
+}
+static I3 rel_trans(const Pose& A, const Pose& B, const I& t) {
+    return {rel_trans1(A.p0.x, A.p1.x, B.p0.x, B.p1.x, t), rel_trans1(A.p0.y, A.p1.y, B.p0.y, B.p1.y, t),
+            rel_trans1(A.p0.z, A.p1.z, B.p0.z, B.p1.z, t)};
+}
+
 struct Box {
     I d[3];
 };
 
-// IntervalRootFinder.h:20-114 (capped DFS, max 1000 pops)
+// IntervalRootFinder.h:20-114 (capped DFS, max 1000 pops). earliest_first (the ipc
+// mode's kinds 3 and 4, no reference counterpart): the low half is popped first, and a
+// search stopped by the cap reports an impact at the least t.lo of the best leaf and of
+// every box still on the stack (a lower bound of any impact it did not rule out).
 static bool root_finder(int D, const std::function<I3(const Box&)>& f, const std::function<bool(const Box&)>& valid,
-                        const Box& x0, const double* tol, Box& out, int* pops, int max_it = 1000) {
+                        const Box& x0, const double* tol, Box& out, int* pops, int max_it = 1000,
+                        bool earliest_first = false) {
     Box earliest;
     for (int i = 0; i < D; ++i) earliest.d[i] = I(std::numeric_limits<double>::infinity());
     bool found = false;
@@ -1918,10 +1957,21 @@ static bool root_finder(int D, const std::function<I3(const Box&)>& f, const std
         Box lo = x, hi = x;
         lo.d[sd] = I(x.d[sd].lo, mid);
         hi.d[sd] = I(mid, x.d[sd].hi);
-        st.push(lo);
-        st.push(hi);
+        if (earliest_first) {
+            st.push(hi);
+            st.push(lo);
+        } else {
+            st.push(lo);
+            st.push(hi);
+        }
     }
     if (pops) *pops = it;
+    if (earliest_first && it >= max_it && !st.empty()) {
+        double m = earliest.d[0].lo;
+        for (; !st.empty(); st.pop()) m = std::min(m, st.top().d[0].lo);
+        earliest.d[0] = I(m, m);
+        found = true;
+    }
     if (found) out = earliest;
     return found;
 }
@@ -1936,7 +1986,13 @@ static Pose make_pose(const double* ps) {
 }
 
 // computeEdgeVertexTOIImpl / computeEdgeEdgeTOIImpl / computeFaceVertexTOIImpl
-// (TimeOfImpact.cpp:8-150). Body A carries the vertex (or edge A), body B the edge/face.
+// (TimeOfImpact.cpp:8-150), kinds 0-2. Body A carries the vertex (or edge A), body B the
+// edge/face. Kinds 3 (face-vertex) and 4 (edge-edge) are the ipc mode's: the
+// separation written in body-relative form (relative translation linear in t, each
+// rotated body-frame vector and edge vector as its rot_hull), the earliest-first
+// search, and for 3 a box is kept when it meets the triangle u, v >= 0, u + v <= 1 (u
+// and v stay inside [0, 1], so: upper(1 - u.lo - v.lo) >= 0) where the reference keeps
+// only boxes whose weights are all certainly inside [0, 1].
 static int ccd_toi(int kind, const Pose& A, const Pose& B, const V3* a, const V3* b, double earliest, double tol,
                    double* toi, int* pops) {
     g_interval_nan = false;
@@ -1963,6 +2019,46 @@ static int ccd_toi(int kind, const Pose& A, const Pose& B, const V3* a, const V3
             return I3{ea.x - eb.x, ea.y - eb.y, ea.z - eb.z};
         };
         hit = root_finder(3, f, [](const Box&) { return true; }, x0, tl, out, pops);
+    } else if (kind == 4) {
+        // (p_A - p_B) + R_A a0 + alpha R_A (a1 - a0) - (R_B b0 + beta R_B (b1 - b0))
+        const double tl[3] = {tol, 1.0, 1.0};
+        x0.d[0] = I(0, earliest);
+        x0.d[1] = I(0, 1);
+        x0.d[2] = I(0, 1);
+        const V3 ea = a[1] - a[0], eb = b[1] - b[0];
+        auto f = [&](const Box& x) {
+            const I& t = x.d[0];
+            const I3 tr = rel_trans(A, B, t), ra = rot_hull(A, a[0], t), rea = rot_hull(A, ea, t);
+            const I3 rb = rot_hull(B, b[0], t), reb = rot_hull(B, eb, t);
+            const I& al = x.d[1];
+            const I& be = x.d[2];
+            return I3{((tr.x + ra.x) + al * rea.x) - (rb.x + be * reb.x),
+                      ((tr.y + ra.y) + al * rea.y) - (rb.y + be * reb.y),
+                      ((tr.z + ra.z) + al * rea.z) - (rb.z + be * reb.z)};
+        };
+        hit = root_finder(3, f, [](const Box&) { return true; }, x0, tl, out, pops, 1000, true);
+    } else if (kind == 3) {
+        // (p_A - p_B) + R_A a - R_B b0 - (u R_B (b1 - b0) + v R_B (b2 - b0))
+        const double tl[3] = {tol, 1.0, 1.0};
+        x0.d[0] = I(0, earliest);
+        x0.d[1] = I(0, 1);
+        x0.d[2] = I(0, 1);
+        const V3 e1 = b[1] - b[0], e2 = b[2] - b[0];
+        auto f = [&](const Box& x) {
+            const I& t = x.d[0];
+            const I3 tr = rel_trans(A, B, t), ra = rot_hull(A, a[0], t), rb = rot_hull(B, b[0], t);
+            const I3 r1 = rot_hull(B, e1, t), r2 = rot_hull(B, e2, t);
+            const I& u = x.d[1];
+            const I& v = x.d[2];
+            return I3{((tr.x + ra.x) - rb.x) - (u * r1.x + v * r2.x), ((tr.y + ra.y) - rb.y) - (u * r1.y + v * r2.y),
+                      ((tr.z + ra.z) - rb.z) - (u * r1.z + v * r2.z)};
+        };
+        auto valid = [](const Box& x) {
+            const I u = x.d[1], v = x.d[2];
+            const I w = I(1.0) - u - v;
+            return u.hi >= 0 && v.hi >= 0 && w.hi >= 0;
+        };
+        hit = root_finder(3, f, valid, x0, tl, out, pops, 1000, true);
     } else {
         const double tl[3] = {tol, 1.0, 1.0};
         x0.d[0] = I(0, earliest);
@@ -1983,6 +2079,73 @@ static int ccd_toi(int kind, const Pose& A, const Pose& B, const V3* a, const V3
     if (g_interval_nan) return -1;
     *toi = hit ? out.d[0].lo : std::numeric_limits<double>::infinity();
     return hit ? 1 : 0;
+}
+
+// ipc mode, OptimizationProblem::maxStep (NewtonSolver.h:42, NewtonSolver.cpp:53-54):
+// per pair, conservative advancement (nd::ipc::pair_advance) along the line-search path
+// x + t dx (positions p + t dp, orientations apply_theta(q, theta + t dtheta)), keeping
+// sep of the pair's current distance; mu = sum over the two bodies of |dp| + |dtheta|
+// r_max (r_max: the largest body-frame radius of the pair's vertices on that body, which
+// bounds their speed along the path), margin = 2e-10 (r_max A + r_max B) for the
+// rotation skipped below |theta| = 1e-10. toi = the least over the pairs; alpha = toi *
+// safety when toi < 1 (the form of IPCSolver::computeMaxStep, IPCSolver.cpp:385-391),
+// else 1.
+int Solver::ipc_max_step(const int32_t* pairs, int np, const vector<double>& x, const vector<double>& dx, double sep,
+                         double safety, double& toi, double& alpha, int& fail_index) const {
+    for (int i = 0; i < np; ++i)
+        if (!ipc_pair_ok(pairs + 5 * i)) {
+            fail_index = i;
+            return 1;
+        }
+    toi = std::numeric_limits<double>::infinity();
+    for (int i = 0; i < np; ++i) {
+        const int32_t* pr = pairs + 5 * i;
+        const int kind = pr[0];
+        int vb[4];
+        V3 r[4];
+        const Mesh& MA = meshes[bodies[pr[1]].mesh];
+        const Mesh& MB = meshes[bodies[pr[2]].mesh];
+        if (kind == 0) {
+            vb[0] = pr[1];
+            r[0] = MA.V[pr[3]];
+            for (int k = 0; k < 3; ++k) { vb[1 + k] = pr[2]; r[1 + k] = MB.V[MB.F[pr[4]][k]]; }
+        } else {
+            for (int k = 0; k < 2; ++k) {
+                vb[k] = pr[1];
+                r[k] = MA.V[MA.E[pr[3]][k]];
+                vb[2 + k] = pr[2];
+                r[2 + k] = MB.V[MB.E[pr[4]][k]];
+            }
+        }
+        double rmax[2] = {0.0, 0.0};
+        for (int v = 0; v < 4; ++v) {
+            const int side = vb[v] == pr[1] ? 0 : 1;
+            rmax[side] = std::max(rmax[side], std::sqrt((r[v].x * r[v].x + r[v].y * r[v].y) + r[v].z * r[v].z));
+        }
+        double mu = 0.0;
+        for (int side = 0; side < 2; ++side) {
+            const size_t o = 6 * size_t(pr[1 + side]);
+            const double lp = std::sqrt((dx[o] * dx[o] + dx[o + 1] * dx[o + 1]) + dx[o + 2] * dx[o + 2]);
+            const double lt = std::sqrt((dx[o + 3] * dx[o + 3] + dx[o + 4] * dx[o + 4]) + dx[o + 5] * dx[o + 5]);
+            mu = mu + (lp + lt * rmax[side]);
+        }
+        const double margin = 2e-10 * (rmax[0] + rmax[1]);
+        auto world = [&](double t, double* xw) {
+            for (int v = 0; v < 4; ++v) {
+                const size_t o = 6 * size_t(vb[v]);
+                const V3 p(x[o] + t * dx[o], x[o + 1] + t * dx[o + 1], x[o + 2] + t * dx[o + 2]);
+                const V3 th(x[o + 3] + t * dx[o + 3], x[o + 4] + t * dx[o + 4], x[o + 5] + t * dx[o + 5]);
+                const V3 w = p + rotate(apply_theta(bodies[vb[v]].q, th), r[v]);
+                xw[3 * v] = w.x;
+                xw[3 * v + 1] = w.y;
+                xw[3 * v + 2] = w.z;
+            }
+        };
+        const double t = nd::ipc::pair_advance(kind, world, mu, sep, margin, kIpcAdvanceIters);
+        toi = std::min(toi, t);
+    }
+    alpha = toi < 1.0 ? toi * safety : 1.0;
+    return 0;
 }
 
 // CCDSystem::computeEarliestTOI (CCD.h:22-82)
@@ -2359,7 +2522,8 @@ int32_t ndo_ccd_toi(int32_t kind, const double* pa, const double* pb, const doub
                     double earliest, double tol, double* toi, int32_t* pops) {
     const Pose A = make_pose(pa), B = make_pose(pb);
     V3 a[2], b[3];
-    const int na = (kind == 1) ? 2 : 1, nb = (kind == 0) ? 2 : (kind == 1 ? 2 : 3);
+    if (kind < 0 || kind > 4) return -3;
+    const int na = (kind == 1 || kind == 4) ? 2 : 1, nb = (kind == 2 || kind == 3) ? 3 : 2;
     for (int i = 0; i < na; ++i) a[i] = V3(a_pts[3 * i], a_pts[3 * i + 1], a_pts[3 * i + 2]);
     for (int i = 0; i < nb; ++i) b[i] = V3(b_pts[3 * i], b_pts[3 * i + 1], b_pts[3 * i + 2]);
     int p = 0;
@@ -2625,6 +2789,17 @@ double ndo_psd_project12(double* H144) { return nd::ipc::psd_project<12>(H144); 
 int32_t ndo_vertex_jacobian(const double* xb6, const double* q4, const double* r3, double* xw3, double* J9) {
     nd::ipc::vertex_and_jacobian(xb6, q4, r3, xw3, J9);
     return 0;
+}
+
+int32_t ndo_ipc_max_step(void* h, int32_t np, const int32_t* pairs5, const double* x, const double* dx, double sep,
+                         double safety, double* alpha, double* toi, int32_t* fail_index) {
+    Solver& s = S(h);
+    const size_t nd = size_t(s.n()) * 6;
+    std::vector<double> xv(x, x + nd), dv(dx, dx + nd);
+    int fi = -1;
+    const int r = s.ipc_max_step(pairs5, np, xv, dv, sep, safety, *toi, *alpha, fi);
+    if (fail_index) *fail_index = fi;
+    return -r;
 }
 
 int32_t ndo_ipc_barrier_eval(void* h, int32_t np, const int32_t* pairs5, const double* x, double dhat, double kappa,

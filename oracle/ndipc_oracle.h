@@ -171,6 +171,10 @@ int32_t ndo_vertex_jacobian(const double* xb6, const double* q4, const double* r
  * EE: edge bodies and mesh edge indices); per segment of consecutive pairs of the same
  * unordered body pair: the bodies (lower first), the gradient (12) and Hessian (144) in
  * the two bodies' DOFs. */
+/* ipc mode maxStep over pairs5 (conservative advancement from x along dx, keeping sep of
+ * each pair's distance): 0, or -1 with fail_index = the first bad pair */
+int32_t ndo_ipc_max_step(void* h, int32_t np, const int32_t* pairs5, const double* x, const double* dx, double sep,
+                         double safety, double* alpha, double* toi, int32_t* fail_index);
 int32_t ndo_ipc_barrier_eval(void* h, int32_t np, const int32_t* pairs5, const double* x, double dhat, double kappa,
                              int32_t psd, double* value, int32_t cap, int32_t* n_seg, int32_t* seg_bodies,
                              double* seg_g, double* seg_H);

@@ -125,7 +125,9 @@ def lib():
                 ("ndo_ipc_distance", [C.c_int32, f64p, i32p, f64p]),
                 ("ndo_vertex_jacobian", [f64p, f64p, f64p, f64p, f64p]),
                 ("ndo_ipc_barrier_eval", [C.c_void_p, C.c_int32, i32p, f64p, C.c_double, C.c_double, C.c_int32,
-                                          f64p, C.c_int32, i32p, i32p, f64p, f64p])):
+                                          f64p, C.c_int32, i32p, i32p, f64p, f64p]),
+                ("ndo_ipc_max_step", [C.c_void_p, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_double, f64p, f64p,
+                                      i32p])):
             getattr(L, name).restype = C.c_int32
             getattr(L, name).argtypes = args
         L.ndo_psd_project12.restype = C.c_double
@@ -393,6 +395,18 @@ class OracleScene:
             raise ValueError("ndo_ipc_barrier_eval: bad pair")
         k = ns.value
         return val.value, sb[: 2 * k].reshape(-1, 2), sg[: 12 * k].reshape(-1, 12), sH[: 144 * k].reshape(-1, 12, 12)
+
+    def ipc_max_step(self, pairs, x, dx, sep=0.1, safety=0.8):
+        """ipc mode maxStep: (alpha, toi); ValueError naming the first bad pair."""
+        pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 5)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        dx = np.ascontiguousarray(dx, dtype=np.float64)
+        a, t, fi = C.c_double(), C.c_double(), C.c_int32()
+        rc = self.L.ndo_ipc_max_step(self.h, len(pr), ptr(pr, i32p), ptr(x, f64p), ptr(dx, f64p), sep, safety,
+                                     C.byref(a), C.byref(t), C.byref(fi))
+        if rc != 0:
+            raise ValueError(f"ndo_ipc_max_step: bad pair {fi.value}")
+        return a.value, t.value
 
     def ccd_earliest_toi(self, dt):
         o = C.c_double()

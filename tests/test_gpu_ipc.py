@@ -1,8 +1,9 @@
-"""ipc mode on the GPU (ndipc_ipc_barrier_eval) against the oracle's evaluation of the
-same pairs: total energy, segments, per-segment body gradient and Hessian, bitwise,
-with and without the PSD projection; the segment structure of interleaved pair lists;
-the error for a bad pair; and a size-independent check at 100k pairs (the segment sums
-against the per-pair evaluations summed on the host)."""
+"""ipc mode on the GPU against the oracle's evaluation of the same inputs:
+ndipc_ipc_barrier_eval (total energy, segments, per-segment body gradient and Hessian,
+bitwise, with and without the PSD projection; the segment structure of interleaved
+pair lists; the error for a bad pair; a size-independent check at 100k pairs, the
+segment sums against the per-pair evaluations summed on the host), the CCD query kinds
+3 and 4 (bitwise on random moving primitives) and ndipc_ipc_max_step (bitwise)."""
 import numpy as np
 import pytest
 
@@ -107,3 +108,76 @@ def test_ipc_eval_large_list(engine_factory):
         g_ref[tuple(sb[0])] += c * sg[0]
     for s, k in enumerate(((0, 1), (1, 2))):
         assert np.allclose(dev[2][s], g_ref[k], rtol=1e-9, atol=1e-12 * np.abs(g_ref[k]).max())
+
+
+def _qrot(q, v):
+    w, x, y, z = q
+    u = np.array([x, y, z])
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def _ccd_queries(kind, n, rng):
+    """Random queries of kind 3 or 4, most of whose motions pass near a point of B's
+    primitive (a random convex combination of its corners at t = 0), with small
+    rotations of both bodies: hits and misses both."""
+    pa = np.zeros((n, 14))
+    pb = np.zeros((n, 14))
+    for P in (pa, pb):
+        q = rng.normal(size=(n, 4))
+        P[:, 3:7] = q / np.linalg.norm(q, axis=1, keepdims=True)
+        dq = P[:, 3:7] + 0.05 * rng.normal(size=(n, 4))
+        P[:, 10:14] = dq / np.linalg.norm(dq, axis=1, keepdims=True)
+    pb[:, 0:3] = rng.normal(size=(n, 3)) * 0.1
+    pb[:, 7:10] = pb[:, 0:3] + rng.normal(size=(n, 3)) * 0.02
+    a = rng.normal(size=(n, 6)) * 0.05
+    b = rng.normal(size=(n, 9)) * 0.5
+    nbp = 3 if kind == 3 else 2
+    for i in range(n):
+        w = rng.dirichlet(np.ones(nbp))
+        c = pb[i, 0:3] + sum(w[k] * _qrot(pb[i, 3:7], b[i, 3 * k:3 * k + 3]) for k in range(nbp))
+        d = rng.normal(size=3) * 0.3
+        pa[i, 0:3] = c + d
+        pa[i, 7:10] = c - d + rng.normal(size=3) * 0.05
+    return pa, pb, a, b
+
+
+def test_ipc_ccd_kinds_match_oracle(engine_factory):
+    """Query kinds 3 and 4 on the device against the oracle, bitwise, on random moving
+    and spinning primitives (hits and misses)."""
+    sc, x, V, F = _three_cubes()
+    e = engine_factory(sc)
+    rng = np.random.default_rng(21)
+    n = 300
+    for kind in (3, 4):
+        pa, pb, a, b = _ccd_queries(kind, n, rng)
+        na, nb = (3, 9) if kind == 3 else (6, 6)
+        hit, toi, pops = e.ccd_toi_batch(kind, pa, pb, a[:, :na], b[:, :nb], 1.0)
+        assert n // 10 < (hit == 1).sum() < n - n // 10
+        for i in range(n):
+            r, t, p = O.ccd_toi(kind, pa[i], pb[i], a[i, :na], b[i, :nb])
+            assert (hit[i], pops[i]) == (r, p) and (t == toi[i] or (r == 0 and toi[i] == np.inf)), (kind, i)
+
+
+def test_ipc_max_step_matches_oracle(engine_factory):
+    sc, x, V, F = _three_cubes()
+    e = engine_factory(sc)
+    o = O.OracleScene(sc)
+    pairs, _, _ = _pairs_near(sc, x, V, F, 1e-3)
+    rng = np.random.default_rng(5)
+    tois = []
+    for k in range(6):
+        dx = rng.normal(size=len(x)) * 3e-4
+        for c in (3, 4, 5):
+            dx[c::6] *= 0.05  # small rotations
+        dx[:6] = 0.0  # the ground stays
+        dev = e.ipc_max_step(0, pairs, x, dx)
+        ora = o.ipc_max_step(pairs, x, dx)
+        assert dev == ora, (k, dev, ora)
+        tois.append(dev[1])
+    assert min(tois) < 1.0
+    bad = pairs[:5].copy()
+    bad[3, 0] = 7
+    with pytest.raises(RuntimeError, match="ipc pair 3"):
+        e.ipc_max_step(0, bad, x, dx)
+    assert e.ipc_max_step(0, pairs[:0], x, dx) == (1.0, np.inf)

@@ -493,3 +493,122 @@ def test_segments_and_bad_pairs(three_cubes):
     for bad in ([0, 1, 1, 0, 0], [0, 0, 5, 0, 0], [0, 0, 1, 99, 0], [1, 0, 1, 0, 99], [0, -1, 1, 0, 0]):
         with pytest.raises(ValueError):
             o.ipc_barrier_eval(np.array([bad], dtype=np.int32), x, dhat, kappa)
+
+
+# ------------------------------------------------------------------ ipc-mode CCD and maxStep
+_ID = [1.0, 0.0, 0.0, 0.0]
+_TRI = [0, 0, 0, 1, 0, 0, 0, 1, 0]
+
+
+def test_face_vertex_ccd_triangle_domain():
+    """A vertex falling through the triangle's interior meets it at t = 0.5: kind 3
+    reports it, conservatively (toi <= 0.5, within the time tolerance); the same fall
+    in the corner u + v > 1 of the parameter box misses; the reference's kind 2 never
+    reports (its predicate drops the root box)."""
+    tol = 1e-8
+    hit, toi, pops = O.ccd_toi(3, [0.25, 0.25, 1.0] + _ID + [0.25, 0.25, -1.0] + _ID, [0] * 3 + _ID + [0] * 3 + _ID,
+                               [0, 0, 0], _TRI, tol=tol)
+    assert hit == 1 and 0.5 - 2 * tol <= toi <= 0.5 and pops < 1000
+    for k in (2, 3):
+        hit, toi, _ = O.ccd_toi(k, [0.8, 0.8, 1.0] + _ID + [0.8, 0.8, -1.0] + _ID, [0] * 3 + _ID + [0] * 3 + _ID,
+                                [0, 0, 0], _TRI, tol=tol)
+        assert hit == 0 and toi == math.inf
+    hit, _, pops = O.ccd_toi(2, [0.25, 0.25, 1.0] + _ID + [0.25, 0.25, -1.0] + _ID, [0] * 3 + _ID + [0] * 3 + _ID,
+                             [0, 0, 0], _TRI, tol=tol)
+    assert hit == 0 and pops == 1
+    # the face moves instead (its body translates up through the still vertex)
+    hit, toi, _ = O.ccd_toi(3, [0.2, 0.3, 0.5] + _ID + [0.2, 0.3, 0.5] + _ID, [0] * 3 + _ID + [0, 0, 2.0] + _ID,
+                            [0, 0, 0], _TRI, tol=tol)
+    assert hit == 1 and 0.25 - 2 * tol <= toi <= 0.25
+
+
+def test_edge_edge_ccd_linear_translation_and_rotation():
+    """kind 4: (a) an edge translating onto a crossing edge meets it at the fraction of
+    the step that closes the gap; the reference's kind 1 (hull over the whole step)
+    reports a time near the end of the step instead."""
+    tol = 1e-8
+    a = [-1, 0, 0, 1, 0, 0]
+    b = [0, -1, 0, 0, 1, 0]
+    pa = [0, 0, 1.0] + _ID + [0, 0, -3.0] + _ID  # the crossing is reached at t = 1/4
+    pb = [0, 0, 0.0] + _ID + [0, 0, 0.0] + _ID
+    hit, toi, _ = O.ccd_toi(4, pa, pb, a, b, tol=tol)
+    assert hit == 1 and 0.25 - 2 * tol <= toi <= 0.25
+    hit1, toi1, pops1 = O.ccd_toi(1, pa, pb, a, b, tol=tol)
+    assert hit1 == 1 and toi1 > 0.99 and pops1 == 1000
+    # (b) an edge spinning 0.5 rad about a point on it sweeps a vertical edge at radius
+    # 0.37 and angle 0.5 / 3: t = 1/3 (radius and heights off the dyadic split points)
+    ar = [-1.1, 0, 0, 0.9, 0, 0]
+    qz = scenes.quat_from_axis_angle([0, 0, 1], 0.5).tolist()
+    c, s = math.cos(0.5 / 3), math.sin(0.5 / 3)
+    hit, toi, pops = O.ccd_toi(4, [0, 0, 0] + _ID + [0, 0, 0] + qz, [0] * 3 + _ID + [0] * 3 + _ID, ar,
+                               [0.37 * c, 0.37 * s, -0.7, 0.37 * c, 0.37 * s, 1.1], tol=tol)
+    assert hit == 1 and 1 / 3 - 1e-7 <= toi <= 1 / 3 and pops < 1000
+    # the same edge at radius 1.5 is never reached
+    hit, _, _ = O.ccd_toi(4, [0, 0, 0] + _ID + [0, 0, 0] + qz, [0] * 3 + _ID + [0] * 3 + _ID, ar,
+                          [1.5 * c, 1.5 * s, -0.7, 1.5 * c, 1.5 * s, 1.1], tol=tol)
+    assert hit == 0
+    # a search stopped by the pop cap still reports a lower bound: the quarter turn
+    qq = scenes.quat_from_axis_angle([0, 0, 1], math.pi / 2).tolist()
+    c, s = math.cos(math.pi / 6), math.sin(math.pi / 6)
+    hit, toi, pops = O.ccd_toi(4, [0, 0, 0] + _ID + [0, 0, 0] + qq, [0] * 3 + _ID + [0] * 3 + _ID, ar,
+                               [0.37 * c, 0.37 * s, -0.7, 0.37 * c, 0.37 * s, 1.1], tol=tol)
+    assert hit == 1 and toi <= 1 / 3
+
+
+def test_ipc_max_step(three_cubes):
+    """maxStep by conservative advancement, keeping sep = 0.1 of each pair's distance:
+    (a) the upper cube, level over the lower one at gap g, moved straight down by 3 g:
+    every close pair reaches 0.1 g at t = 0.3, and the bound converges to it from below;
+    (b) moved sideways by half the gap, nothing comes closer: alpha = 1; (c) a tumbling
+    step: sampled densely along the path (independent map, textbook distances), every
+    pair stays above 0.1 of its starting distance up to toi, and some pair is within 1%
+    of that at toi; (d) the bound over a list is the least of its pairs' bounds; bad
+    pairs are named."""
+    o, sc, x, V, F = three_cubes
+    size = 6e-4
+    lvl = O.OracleScene(_level(sc))
+    xs = x.copy()
+    xs[6 + 3:6 + 6] = 0.0
+    xs[12 + 3:12 + 6] = 0.0
+    gap = 1e-4
+    xs[12:15] = [xs[6], xs[7] + size + gap, xs[8]]
+    pairs, _, _ = _pairs_near(lvl.sc, xs, V, F, 2 * gap)
+    pairs = pairs[(pairs[:, 1] + pairs[:, 2]) == 3]  # bodies 1 and 2
+    assert len(pairs) > 8
+    dx = np.zeros_like(xs)
+    dx[13] = -3 * gap
+    alpha, toi = lvl.ipc_max_step(pairs, xs, dx)
+    assert 0.3 * (1 - 1e-5) <= toi <= 0.3 and alpha == toi * 0.8
+    dx2 = np.zeros_like(xs)
+    dx2[12] = 0.5 * gap
+    assert lvl.ipc_max_step(pairs, xs, dx2) == (1.0, math.inf)
+    # (c) tumbling: translation and rotation of the upper cube
+    dx3 = np.zeros_like(xs)
+    dx3[12:18] = [0.3 * gap, -2.5 * gap, 0.2 * gap, 0.4, -0.3, 0.5]
+    alpha, toi = lvl.ipc_max_step(pairs, xs, dx3)
+    assert 0.0 < toi < 1.0
+    q = lvl.sc.q
+    E = [_unique_edges(F[b]) for b in range(3)]
+
+    def dists(t):
+        xt = xs + t * dx3
+        W = [np.array([_xw_ref(xt[6 * b:6 * b + 6], q[b], r) for r in V[b]]) for b in range(3)]
+        return np.array([(_pt_ref if p[0] == 0 else _ee_ref)(_pair_world(p, W, F, E)) for p in pairs])
+
+    d0 = dists(0.0)
+    for t in np.linspace(0.0, toi, 41):
+        assert (dists(t) >= 0.1 * d0 * (1 - 1e-9)).all(), t
+    assert ((dists(toi) - 0.1 * d0) / d0).min() < 0.01
+    per = [lvl.ipc_max_step(p[None, :], xs, dx3)[1] for p in pairs]
+    assert toi == min(per)
+    bad = pairs[:4].copy()
+    bad[2, 3] = 999
+    with pytest.raises(ValueError, match="bad pair 2"):
+        lvl.ipc_max_step(bad, xs, dx)
+
+
+def _level(sc):
+    import copy
+    s2 = copy.deepcopy(sc)
+    s2.q[1:] = [1.0, 0.0, 0.0, 0.0]
+    return s2
