@@ -13,7 +13,8 @@ the plain single-scene path (a bounded sample of scenes, steps/s summed over sce
 and the adjoint: gradients/s of the differentiation workload (scenes.config4, 100 steps,
 checkpoint every 10) and of the 128-scene batch (scenes.config5, 50 steps), the
 backward's cost relative to a plain forward of the same rollout, and a central-difference
-check (h = 1e-6) of a few components.
+check (h = 1e-6) of a few components, and the ipc mode (the primitive-distance barrier
+and its maxStep) on c2's own contact candidates: pairs/s and k_ipc_pairs' roofline.
 
 Prints one JSON line. --gpus above 1 is not supported: this build runs on one GPU.
 """
@@ -181,6 +182,101 @@ def adjoint_fd_check(Engine, sc, targets, n_steps, comps, h=1e-6):
     return worst
 
 
+# ipc mode (k_ipc_pairs), compulsory HBM bytes per pair: the pair (5 int32), the two
+# bodies' Newton variables and step-start orientations (2 x (6 + 4) doubles), the four
+# body-frame vertices (12 doubles) and the face's or edges' vertex indices (<= 4 int32)
+# read, its segment key and activity flag written = 300 B; per pair within dhat its
+# energy, body gradient (12) and Hessian (144 doubles) written = 1256 B more.
+IPC_B_PAIR, IPC_B_ACTIVE = 300.0, 1256.0
+
+
+def ipc_pairs_from_candidates(sc, ee, fv):
+    """The ipc pair list (5 int32 each) of one scene's candidate tuples: each FV candidate
+    (face body id, vertex body id, face, vertex) a point-triangle pair (vertex body, face
+    body, vertex, face); each EE candidate (body ids, the two edges' vertex ids) an
+    edge-edge pair with the edges' unique-edge indices (face order, (min, max))."""
+    idx = {int(b): i for i, b in enumerate(sc.body_id)}
+    F = sc.faces.reshape(-1, 3)
+    cache = {}
+
+    def edge_index(b):
+        key = (int(sc.face_off[b]), int(sc.n_faces[b]))
+        if key not in cache:
+            d = {}
+            for f in F[key[0]:key[0] + key[1]]:
+                for a, c in ((f[0], f[1]), (f[1], f[2]), (f[2], f[0])):
+                    k = (min(a, c), max(a, c))
+                    if k not in d:
+                        d[k] = len(d)
+            cache[key] = d
+        return cache[key]
+
+    pt = np.stack([np.zeros(len(fv), dtype=np.int64), [idx[int(v)] for v in fv[:, 1]],
+                   [idx[int(v)] for v in fv[:, 0]], fv[:, 3], fv[:, 2]], axis=1) if len(fv) else np.zeros((0, 5))
+    rows = []
+    for b0, b1, a0, a1, c0, c1 in ee:
+        i0, i1 = idx[int(b0)], idx[int(b1)]
+        rows.append((1, i0, i1, edge_index(i0)[(min(a0, a1), max(a0, a1))], edge_index(i1)[(min(c0, c1), max(c0, c1))]))
+    pe = np.array(rows, dtype=np.int64).reshape(-1, 5)
+    return np.ascontiguousarray(np.concatenate([pt, pe]).astype(np.int32))
+
+
+def run_ipc(Engine, torch, steps=200, dhat=1e-2, kappa=1e5, max_pairs=1 << 20, reps=5):
+    """ipc mode on c2's own contact candidates: the scene after `steps` device steps, every
+    FV and EE candidate of its close body pairs (top-K lists) as an ipc pair (at most
+    max_pairs), evaluated at the current poses (theta = 0) with the PSD projection; then
+    maxStep along the next step's free-flight displacement v dt. Kernel times from HIP
+    events on the context stream, averaged over `reps` calls."""
+    from nexdynipc_amd import scenes
+
+    sc = scenes.make_config("c2")
+    e = Engine(0)
+    e.upload(sc)
+    e.step(steps)
+    p, q, v, w = e.get_state()[:4]
+    vv, ev, ee, fv = e.candidates(0, float(sc.dhat[0]))
+    pairs = ipc_pairs_from_candidates(sc, ee, fv)[:max_pairs]
+    x = np.concatenate([p, np.zeros_like(p)], axis=1).reshape(-1)
+    dt = float(sc.dt[0])
+    dx = np.concatenate([v * dt, w * dt], axis=1).reshape(-1)
+    n_close = int(e.pair_count())
+    e.ipc_barrier_eval(0, pairs, x, dhat, kappa, True, cap=4 * n_close + 16)  # warm-up
+    e.ipc_max_step(0, pairs, x, dx)
+    torch.cuda.synchronize()
+    e.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        val, segb, _, _, active = e.ipc_barrier_eval(0, pairs, x, dhat, kappa, True, cap=4 * n_close + 16,
+                                                     with_active=True)
+    t_call = (time.perf_counter() - t0) / reps
+    for _ in range(reps):
+        alpha, toi = e.ipc_max_step(0, pairs, x, dx)
+    kf, nk = e.timing_get("ipc_filter")
+    kp, _ = e.timing_get("ipc_pairs")
+    kr, _ = e.timing_get("ipc_reduce")
+    ka, _ = e.timing_get("ipc_advance")
+    e.timing_enable(False)
+    e.close()
+    n = len(pairs)
+    # the filter pass (k_ipc_filter + segment scan) streams every pair; the pair pass
+    # (k_ipc_pairs) writes the terms of the pairs within dhat
+    per_launch = IPC_B_PAIR * n + IPC_B_ACTIVE * active
+    achieved = per_launch / ((kf + kp) * 1e-3) / 1e9
+    return {"workload": f"c2 after {steps} steps: its FV and EE candidates (top-K lists of every close body "
+                        f"pair, {n_close} pairs) as {n} ipc pairs ({int((pairs[:, 0] == 0).sum())} PT, "
+                        f"{int((pairs[:, 0] == 1).sum())} EE), dhat {dhat} m, kappa {kappa:g}, PSD projection",
+            "pairs": n, "active_pairs": active, "segments": int(len(segb)),
+            "pairs_per_s": n / ((kf + kp + kr) * 1e-3),
+            "kernel_ms": {"ipc_filter": kf, "ipc_pairs": kp, "ipc_reduce": kr},
+            "active_pairs_per_s": active / (kp * 1e-3) if kp > 0 else None,
+            "host_call_ms": 1e3 * t_call,
+            "host_call_note": "ndipc_ipc_barrier_eval from host buffers: allocation, PCIe copies, launches",
+            "max_step": {"alpha": alpha, "toi": toi, "kernel_ms": ka, "pairs_per_s": n / (ka * 1e-3)},
+            "roofline": {"kernel": "k_ipc_filter + k_ipc_seg_* + k_ipc_pairs", "bound": "hbm", "achieved": achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": kf + kp, "launches": nk}}
+
+
 def roofline_for(name, r, sc):
     """Roofline of kernel `name` over the timed region: algorithmic work / total time."""
     avg_ms, cnt = r["kern"][name]
@@ -232,6 +328,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs (c4, c5)")
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--cpu-stride", type=int, default=2000)
+    ap.add_argument("--ipc-only", action="store_true", help="only the ipc-mode leg (for profiling)")
+    ap.add_argument("--no-ipc", action="store_true", help="skip the ipc-mode leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 20 device copies of 256 MiB (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -253,6 +351,10 @@ def main():
 
     from nexdynipc_amd import scenes
     from nexdynipc_amd._lib import Engine
+
+    if args.ipc_only:
+        print(json.dumps({"ipc": run_ipc(Engine, torch)}))
+        return
 
     batch = None
     if not args.no_batch:
@@ -383,6 +485,7 @@ def main():
         "adjoint": adjoint,
         "cpu_baseline": cpu,
         "c1": c1,
+        "ipc": None if args.no_ipc else run_ipc(Engine, torch),
     }
     print(json.dumps(out))
 

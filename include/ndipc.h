@@ -179,12 +179,13 @@ int32_t ndipc_friction_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int
  * folded into the two bodies' DOFs (Gauss-Newton in the rigid map). Consecutive pairs of
  * one unordered body pair form a segment: per segment seg_bodies[2] (lower body first),
  * seg_g[12] and seg_H[144] (row-major, lower body's 6 DOFs first); value = total energy.
- * n_seg receives the segment count; at most cap segments are written. Fails (naming
- * the first bad pair) on a bad kind, body or primitive index. np <= NDIPC_IPC_MAX_PAIRS. */
+ * n_seg receives the segment count; at most cap segments are written; n_active (may be
+ * NULL) the number of pairs within dhat. Fails (naming the first bad pair) on a bad
+ * kind, body or primitive index. np <= NDIPC_IPC_MAX_PAIRS. */
 #define NDIPC_IPC_MAX_PAIRS (1 << 22)
 int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const int32_t* pairs5, const double* x,
                                double dhat, double kappa, int32_t psd, double* value, int32_t cap, int32_t* n_seg,
-                               int32_t* seg_bodies, double* seg_g, double* seg_H);
+                               int32_t* seg_bodies, double* seg_g, double* seg_H, int32_t* n_active);
 /* ipc mode, OptimizationProblem::maxStep (NewtonSolver.h:42, called at
  * NewtonSolver.cpp:53-54) over the pairs of ndipc_ipc_barrier_eval's format: per pair,
  * conservative advancement along the line-search path x + t dx (positions p + t dp,

@@ -101,7 +101,7 @@ def library():
         "ndipc_barrier_eval": ([vp, C.c_int32, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_double, f64p, f64p,
                                 f64p, f64p], C.c_int32),
         "ndipc_ipc_barrier_eval": ([vp, C.c_int32, C.c_int32, i32p, f64p, C.c_double, C.c_double, C.c_int32, f64p,
-                                    C.c_int32, i32p, i32p, f64p, f64p], C.c_int32),
+                                    C.c_int32, i32p, i32p, f64p, f64p, i32p], C.c_int32),
         "ndipc_ipc_max_step": ([vp, C.c_int32, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_double, f64p, f64p],
                                C.c_int32),
         "ndipc_friction_eval": ([vp, C.c_int32, C.c_int32, i32p, f64p, f64p, f64p, C.c_double, C.c_double,
@@ -395,22 +395,26 @@ class Engine:
                  "barrier_eval")
         return val.value, g, H, nf[: len(ints)]
 
-    def ipc_barrier_eval(self, scene, pairs, x, dhat, kappa, psd=True):
+    def ipc_barrier_eval(self, scene, pairs, x, dhat, kappa, psd=True, cap=None, with_active=False):
         """ipc mode over pairs (n x 5 int32) of scene at Newton variable x: total energy and
-        per segment (lower body, higher body), gradient [12] and Hessian [12, 12]."""
+        per segment (lower body, higher body), gradient [12] and Hessian [12, 12] (at most
+        cap segments returned; default: one per pair); with_active: also the number of
+        pairs within dhat."""
         pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 5)
         x = np.ascontiguousarray(x, dtype=np.float64)
-        cap = max(len(pr), 1)
+        cap = max(len(pr) if cap is None else int(cap), 1)
         val = C.c_double()
         ns = C.c_int32()
         sb = np.zeros(2 * cap, dtype=np.int32)
         sg = np.zeros(12 * cap)
         sH = np.zeros(144 * cap)
+        na = C.c_int32()
         self._ck(self.L.ndipc_ipc_barrier_eval(self.ctx, scene, len(pr), ptr(pr, i32p), ptr(x, f64p), dhat, kappa,
                                                1 if psd else 0, C.byref(val), cap, C.byref(ns), ptr(sb, i32p),
-                                               ptr(sg, f64p), ptr(sH, f64p)), "ipc_barrier_eval")
-        k = ns.value
-        return val.value, sb[: 2 * k].reshape(-1, 2), sg[: 12 * k].reshape(-1, 12), sH[: 144 * k].reshape(-1, 12, 12)
+                                               ptr(sg, f64p), ptr(sH, f64p), C.byref(na)), "ipc_barrier_eval")
+        k = min(ns.value, cap)
+        out = (val.value, sb[: 2 * k].reshape(-1, 2), sg[: 12 * k].reshape(-1, 12), sH[: 144 * k].reshape(-1, 12, 12))
+        return out + (na.value,) if with_active else out
 
     def ipc_max_step(self, scene, pairs, x, dx, sep=0.1, safety=0.8):
         """ipc mode maxStep over pairs (n x 5 int32) of scene along dx from x, keeping sep of

@@ -1276,7 +1276,7 @@ int32_t ndipc_friction_eval(ndipc_ctx* ctx, int32_t scene, int32_t nc, const int
 
 int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const int32_t* pairs5, const double* x,
                                double dhat, double kappa, int32_t psd, double* value, int32_t cap, int32_t* n_seg,
-                               int32_t* seg_bodies, double* seg_g, double* seg_H) {
+                               int32_t* seg_bodies, double* seg_g, double* seg_H, int32_t* n_active) {
     if (check_ctx(ctx)) return -1;
     if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
     if (np < 0 || np > NDIPC_IPC_MAX_PAIRS) return fail(ctx, "pair count out of range");
@@ -1285,6 +1285,7 @@ int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const 
     if (np == 0) {
         *value = 0.0;
         *n_seg = 0;
+        if (n_active) *n_active = 0;
         return 0;
     }
     if (!pairs5 || !x) return fail(ctx, "null input");
@@ -1295,7 +1296,8 @@ int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const 
     int *ip = nullptr, *iw = nullptr;
     double *dp = nullptr, *dx = nullptr;
     unsigned long long* key = nullptr;
-    HIPCHK(sb.alloc(&ip, 4 * (5 * n + n + 1 + n + 1 + 1 + 2 * n)));
+    const size_t nchunk = (n + 1023) / 1024;
+    HIPCHK(sb.alloc(&ip, 4 * (5 * n + n + 1 + n + 1 + 1 + 1 + 2 * n + 2 * nchunk + n)));
     HIPCHK(sb.alloc(&key, 8 * n));
     HIPCHK(sb.alloc(&dp, 8 * (n + 12 * n + 144 * n + n + 12 * n + 144 * n + 1)));
     HIPCHK(sb.alloc(&dx, 8 * 6 * size_t(nb)));
@@ -1310,7 +1312,15 @@ int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const 
     iw += n + 1;
     B.n_seg = iw;
     iw += 1;
+    B.n_act = iw;
+    iw += 1;
     B.seg_bodies = iw;
+    iw += 2 * n;
+    B.chunk_cnt = iw;
+    iw += nchunk;
+    B.chunk_act = iw;
+    iw += nchunk;
+    B.alist = iw;
     B.key = key;
     B.x = dx;
     double* w = dp;
@@ -1342,6 +1352,7 @@ int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const 
     }
     HIPCHK(hipMemcpy(&ns, B.n_seg, 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(value, B.value, 8, hipMemcpyDeviceToHost));
+    if (n_active) HIPCHK(hipMemcpy(n_active, B.n_act, 4, hipMemcpyDeviceToHost));
     *n_seg = ns;
     const size_t k = size_t(std::min(ns, std::max(cap, 0)));
     if (k) {
@@ -1382,6 +1393,9 @@ int32_t ndipc_ipc_max_step(ndipc_ctx* ctx, int32_t scene, int32_t np, const int3
     HIPCHK(hipMemcpy(dpairs, pairs5, 4 * 5 * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dx0, x, 8 * nd6, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ddx, dx, 8 * nd6, hipMemcpyHostToDevice));
+    const double inf = 1.0 / 0.0;
+    HIPCHK(hipMemcpy(dout_t, &inf, 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dbad, &np, 4, hipMemcpyHostToDevice));
     nd::launch_ipc_advance(ctx->d, b0, nb, np, dpairs, dx0, ddx, sep, dtoi, dst, dout_t, dbad, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -2,13 +2,16 @@
 // point-triangle / edge-edge pairs of one scene (nexdynipc_amd/csrc/ipc_contact.h holds
 // the per-pair mathematics, shared with the host).
 //
-//   k_ipc_pairs     sixteen lanes per pair: world points and rotation Jacobians of the
-//                   four vertices, distance type, E = kappa b(d) with its vertex
+//   k_ipc_filter    one lane per pair: the pair's check, segment key and whether it is
+//                   within dhat;
+//   k_ipc_seg_*     consecutive pairs of one unordered body pair form a segment: per
+//                   chunk counts, one scan, per chunk writes give every segment's first
+//                   pair and the in-order list of the pairs within dhat;
+//   k_ipc_pairs     sixteen lanes per listed pair: world points and rotation Jacobians
+//                   of the four vertices, distance type, E = kappa b(d) with its vertex
 //                   gradient and Hessian (hyper-dual, split over lanes by derivative
 //                   slot), the PSD projection (Jacobi rotations over lanes, in LDS), the
 //                   fold into the two bodies' 12 DOFs; results pair-major in HBM;
-//   k_ipc_segments  one work-group: consecutive pairs of one unordered body pair form a
-//                   segment; chunked count + scan gives every segment's first pair;
 //   k_ipc_reduce    one work-group per segment (persistent over segments), one lane per
 //                   output element (E, 12 gradient, 144 Hessian entries), each summing
 //                   its element over the segment's pairs in pair order;
@@ -75,24 +78,76 @@ __device__ __forceinline__ double ipc_jel(int r, int c, const int* own, const do
     return 0.0;
 }
 
-// Sixteen lanes per pair, four pairs per work-group of one wavefront. Lanes 0-3 map the
-// four vertices; every lane carries up to five second-derivative slots of the squared
-// distance; the PSD projection is ipc::psd_project with each rotation's column, row and
-// eigenvector updates spread over lanes 0-11 (the rotation angle, the sweep test and
-// the break are computed as there, in the same order); the fold takes one output entry
-// per lane at a time. Control flow is uniform over the wavefront: a pair that is done
-// (or out of dhat) masks its lanes' work.
-__global__ __launch_bounds__(64) void k_ipc_pairs(Dev d, int b0, int nb, int np, const int* pairs, const double* x,
-                                                  double dhat, double kappa, int psd, unsigned long long* key,
-                                                  int* act, int* bad, double* pe, double* pg, double* pH) {
+// Pass 1, one lane per pair: the pair's check, its segment key, and whether it is within
+// dhat, from the same world points (ipc::vertex_and_jacobian) and the same squared
+// distance value (the value part of the hyper-dual, whose division is a * (1 / b)) as
+// pass 2, so both passes agree on every pair.
+__global__ __launch_bounds__(256) void k_ipc_filter(Dev d, int b0, int nb, int np, const int* pairs, const double* x,
+                                                    double dhat, unsigned long long* key, int* act, int* bad) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= np) return;
+    const int* pr = pairs + 5 * i;
+    const int kind = pr[0], a = pr[1], b = pr[2], ia = pr[3], ib = pr[4];
+    bool ok = (kind == 0 || kind == 1) && a >= 0 && b >= 0 && a < nb && b < nb && a != b;
+    const int ga = ok ? b0 + a : 0, gb = ok ? b0 + b : 0;
+    ok = ok && d.nv[ga] > 0 && d.nv[gb] > 0 && ia >= 0 && ib >= 0;
+    ok = ok && (kind == 0 ? (ia < d.nv[ga] && ib < d.nf[gb]) : (ia < d.ne[ga] && ib < d.ne[gb]));
+    if (!ok) {
+        key[i] = ~0ull;
+        act[i] = 0;
+        atomicMin(bad, i);
+        return;
+    }
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    key[i] = (static_cast<unsigned long long>(lo) << 32) | static_cast<unsigned long long>(hi);
+    double xw[ipc::kN], jth[9];
+    for (int l = 0; l < 4; ++l) {
+        int g, vl;
+        if (kind == 0) {
+            g = l == 0 ? ga : gb;
+            vl = l == 0 ? ia : d.faces[3 * (d.f_off[gb] + ib) + (l - 1)];
+        } else {
+            g = l < 2 ? ga : gb;
+            vl = l < 2 ? d.edges[2 * (d.e_off[ga] + ia) + l] : d.edges[2 * (d.e_off[gb] + ib) + (l - 2)];
+        }
+        const double q[4] = {d.q[4 * g], d.q[4 * g + 1], d.q[4 * g + 2], d.q[4 * g + 3]};
+        const double* r = d.verts + 3 * (d.v_off[g] + vl);
+        const double rr[3] = {r[0], r[1], r[2]};
+        ipc::vertex_and_jacobian(x + 6 * (g - b0), q, rr, xw + 3 * l, jth);
+    }
+    const int type = ipc::pair_type(kind, xw);
+    ipc::P3<HL> v[4];
+    for (int k = 0; k < 4; ++k)
+        v[k] = {hl_var(xw[3 * k], 3 * k, 0, 0), hl_var(xw[3 * k + 1], 3 * k + 1, 0, 0),
+                hl_var(xw[3 * k + 2], 3 * k + 2, 0, 0)};
+    const HL d2 = ipc::pair_d2<HL>(kind, type, v);
+    const double dist = sqrt(d2.v > 0.0 ? d2.v : 0.0);
+    act[i] = dist < dhat ? 1 : 0;
+}
+
+// Pass 2 over the pairs within dhat (alist[0 .. *n_act), pair order), sixteen lanes per
+// pair, four pairs per work-group of one wavefront, persistent over the list. Lanes 0-3
+// map the four vertices; every lane carries up to five second-derivative slots of the
+// squared distance; the PSD projection is ipc::psd_project with each rotation's column,
+// row and eigenvector updates spread over lanes 0-11 (the rotation angle, the sweep test
+// and the break are computed as there, in the same order); the fold takes one output
+// entry per lane at a time. Control flow is uniform over the wavefront: a pair that is
+// done (or a slot past the list's end) masks its lanes' work.
+__global__ __launch_bounds__(64) void k_ipc_pairs(Dev d, int b0, const int* pairs, const double* x, double dhat,
+                                                  double kappa, int psd, const int* alist, const int* n_act,
+                                                  double* pe, double* pg, double* pH) {
     constexpr int N = ipc::kN;
     __shared__ double sA[kIpcPairsPerWg][N * N], sV[kIpcPairsPerWg][N * N], sP[kIpcPairsPerWg][N * N];
     __shared__ double sG[kIpcPairsPerWg][N], sJ[kIpcPairsPerWg][36], sX[kIpcPairsPerWg][N];
     __shared__ int sOwn[kIpcPairsPerWg][4], sFlag[kIpcPairsPerWg];
     const int grp = threadIdx.x / kIpcLanes, l = threadIdx.x % kIpcLanes;
-    const int i = blockIdx.x * kIpcPairsPerWg + grp;
-    int kind = -1, a = -1, b = -1, ia = -1, ib = -1;
-    if (i < np) {
+    const int na = *n_act;
+    for (int base = blockIdx.x * kIpcPairsPerWg; base < na; base += gridDim.x * kIpcPairsPerWg) {
+    const int j = base + grp;
+    const bool ok = j < na;
+    const int i = ok ? alist[j] : 0;
+    int kind = 0, a = 0, b = 0, ia = 0, ib = 0;
+    if (ok) {
         const int* pr = pairs + 5 * i;
         kind = pr[0];
         a = pr[1];
@@ -100,20 +155,8 @@ __global__ __launch_bounds__(64) void k_ipc_pairs(Dev d, int b0, int nb, int np,
         ia = pr[3];
         ib = pr[4];
     }
-    bool ok = i < np && (kind == 0 || kind == 1) && a >= 0 && b >= 0 && a < nb && b < nb && a != b;
-    const int ga = ok ? b0 + a : 0, gb = ok ? b0 + b : 0;
-    ok = ok && d.nv[ga] > 0 && d.nv[gb] > 0 && ia >= 0 && ib >= 0;
-    ok = ok && (kind == 0 ? (ia < d.nv[ga] && ib < d.nf[gb]) : (ia < d.ne[ga] && ib < d.ne[gb]));
-    const int lo = a < b ? a : b, hi = a < b ? b : a;
-    if (i < np && l == 0) {
-        if (!ok) {
-            key[i] = ~0ull;
-            act[i] = 0;
-            atomicMin(bad, i);
-        } else {
-            key[i] = (static_cast<unsigned long long>(lo) << 32) | static_cast<unsigned long long>(hi);
-        }
-    }
+    const int ga = b0 + a, gb = b0 + b;
+    const int lo = a < b ? a : b;
     if (ok && l < 4) {
         int g, vl;
         if (kind == 0) {
@@ -148,13 +191,12 @@ __global__ __launch_bounds__(64) void k_ipc_pairs(Dev d, int b0, int nb, int np,
         if (s == 0) {
             dist = sqrt(d2.v > 0.0 ? d2.v : 0.0);
             active = ok && dist < dhat;
-            if (active) {
-                E = bar_v(dist, dhat, kappa);
-                if (dist > 0.0) {
-                    const double b1 = bar_g(dist, dhat, kappa), b2 = bar_h(dist, dhat, kappa);
-                    e1 = b1 / (2.0 * dist);
-                    e2 = (b2 - b1 / dist) / (4.0 * dist * dist);
-                }
+            if (!active) break;  // no barrier term: the remaining slots are not needed
+            E = bar_v(dist, dhat, kappa);
+            if (dist > 0.0) {
+                const double b1 = bar_g(dist, dhat, kappa), b2 = bar_h(dist, dhat, kappa);
+                e1 = b1 / (2.0 * dist);
+                e2 = (b2 - b1 / dist) / (4.0 * dist * dist);
             }
         }
         if (active && t < ipc::kH) {
@@ -168,10 +210,7 @@ __global__ __launch_bounds__(64) void k_ipc_pairs(Dev d, int b0, int nb, int np,
             if (si == sj) sG[grp][si] = gv;
         }
     }
-    if (ok && l == 0) {
-        act[i] = active ? 1 : 0;
-        if (active) pe[i] = E;
-    }
+    if (active && l == 0) pe[i] = E;
     __syncthreads();
     const double* Hs = sA[grp];
     if (psd) {
@@ -197,7 +236,9 @@ __global__ __launch_bounds__(64) void k_ipc_pairs(Dev d, int b0, int nb, int np,
             const bool all = sFlag[0] && sFlag[1] && sFlag[2] && sFlag[3];
             __syncthreads();
             if (all) break;
+#pragma unroll 1
             for (int p = 0; p < N; ++p)
+#pragma unroll 1
                 for (int q = p + 1; q < N; ++q) {
                     const double apq = A[p * N + q];
                     const bool rot = !gdone && apq != 0.0;
@@ -265,60 +306,176 @@ __global__ __launch_bounds__(64) void k_ipc_pairs(Dev d, int b0, int nb, int np,
             for (int k = 0; k < N; ++k) s += ipc_jel(k, r, own, jth) * sV[grp][k * N + c];
             pH[144 * size_t(i) + idx] = s;
         }
+    __syncthreads();  // the next list entries reuse the LDS
+    }
 }
 
-// seg_start[s] = first pair of segment s (s < n_seg), seg_start[n_seg] = np
-__global__ __launch_bounds__(kIpcScanThreads) void k_ipc_segments(int np, const unsigned long long* key,
-                                                                   int* seg_start, int* n_seg) {
-    __shared__ int cnt[kIpcScanThreads];
+// Segments in three passes over chunks of kIpcChunk pairs: per chunk the number of
+// segment starts (pair 0, or a key unlike the previous pair's) and of pairs within dhat;
+// one work-group's exclusive scan of the chunk counts; per chunk the starts written in
+// order. seg_start[s] = first pair of segment s (s < n_seg), seg_start[n_seg] = np;
+// n_act = pairs within dhat.
+constexpr int kIpcChunk = 1024;
+constexpr int kIpcChunkThreads = 256;
+constexpr int kIpcPerThread = kIpcChunk / kIpcChunkThreads;
+
+__device__ __forceinline__ bool seg_first(const unsigned long long* key, int i) {
+    return i == 0 || key[i] != key[i - 1];
+}
+
+__global__ __launch_bounds__(kIpcChunkThreads) void k_ipc_seg_count(int np, const unsigned long long* key,
+                                                                    const int* act, int* chunk_cnt,
+                                                                    int* chunk_act) {
+    __shared__ int sc[kIpcChunkThreads], sa[kIpcChunkThreads];
+    const int t = threadIdx.x, c0 = blockIdx.x * kIpcChunk;
+    int c = 0, ca = 0;
+    for (int k = 0; k < kIpcPerThread; ++k) {
+        const int i = c0 + k * kIpcChunkThreads + t;
+        if (i < np) {
+            c += seg_first(key, i) ? 1 : 0;
+            ca += act[i];
+        }
+    }
+    sc[t] = c;
+    sa[t] = ca;
+    __syncthreads();
+    for (int h = kIpcChunkThreads / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            sc[t] += sc[t + h];
+            sa[t] += sa[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        chunk_cnt[blockIdx.x] = sc[0];
+        chunk_act[blockIdx.x] = sa[0];
+    }
+}
+
+// exclusive scan of the chunk counts in place (nchunk <= kIpcScanThreads * 4)
+__global__ __launch_bounds__(kIpcScanThreads) void k_ipc_seg_scan(int np, int nchunk, int* chunk_cnt,
+                                                                  int* chunk_act, int* seg_start, int* n_seg,
+                                                                  int* n_act) {
+    __shared__ int part[kIpcScanThreads], apart[kIpcScanThreads];
     const int t = threadIdx.x;
-    const int chunk = (np + kIpcScanThreads - 1) / kIpcScanThreads;
-    const int i0 = t * chunk < np ? t * chunk : np;
-    const int i1 = i0 + chunk < np ? i0 + chunk : np;
-    int c = 0;
-    for (int i = i0; i < i1; ++i) c += (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
-    cnt[t] = c;
+    constexpr int kPer = 4;
+    // chunk_act is scanned as well: it becomes each chunk's offset in the active list
+    int c = 0, ca = 0;
+    for (int k = 0; k < kPer; ++k) {
+        const int j = t * kPer + k;
+        if (j < nchunk) {
+            c += chunk_cnt[j];
+            ca += chunk_act[j];
+        }
+    }
+    part[t] = c;
+    apart[t] = ca;
     __syncthreads();
     if (t == 0) {
-        int run = 0;
+        int run = 0, arun = 0;
         for (int k = 0; k < kIpcScanThreads; ++k) {
-            const int ck = cnt[k];
-            cnt[k] = run;
-            run += ck;
+            const int v = part[k], va = apart[k];
+            part[k] = run;
+            apart[k] = arun;
+            run += v;
+            arun += va;
         }
         *n_seg = run;
+        *n_act = arun;
         seg_start[run] = np;
     }
     __syncthreads();
-    int s = cnt[t];
-    for (int i = i0; i < i1; ++i)
-        if (i == 0 || key[i] != key[i - 1]) seg_start[s++] = i;
+    int run = part[t], arun = apart[t];
+    for (int k = 0; k < kPer; ++k) {
+        const int j = t * kPer + k;
+        if (j < nchunk) {
+            const int v = chunk_cnt[j], va = chunk_act[j];
+            chunk_cnt[j] = run;
+            chunk_act[j] = arun;
+            run += v;
+            arun += va;
+        }
+    }
 }
 
+__global__ __launch_bounds__(kIpcChunkThreads) void k_ipc_seg_write(int np, const unsigned long long* key,
+                                                                    const int* act, const int* chunk_off,
+                                                                    const int* chunk_aoff, int* seg_start,
+                                                                    int* alist) {
+    __shared__ int sc[kIpcChunkThreads], sa[kIpcChunkThreads];
+    const int t = threadIdx.x, c0 = blockIdx.x * kIpcChunk + t * kIpcPerThread;
+    int f[kIpcPerThread], fa[kIpcPerThread], c = 0, ca = 0;
+    for (int k = 0; k < kIpcPerThread; ++k) {
+        const int i = c0 + k;
+        f[k] = (i < np && seg_first(key, i)) ? 1 : 0;
+        fa[k] = (i < np && act[i] != 0) ? 1 : 0;
+        c += f[k];
+        ca += fa[k];
+    }
+    sc[t] = c;
+    sa[t] = ca;
+    __syncthreads();
+    for (int h = 1; h < kIpcChunkThreads; h <<= 1) {  // inclusive scans (Hillis-Steele)
+        const int v = t >= h ? sc[t - h] : 0, va = t >= h ? sa[t - h] : 0;
+        __syncthreads();
+        sc[t] += v;
+        sa[t] += va;
+        __syncthreads();
+    }
+    int pos = chunk_off[blockIdx.x] + sc[t] - c, apos = chunk_aoff[blockIdx.x] + sa[t] - ca;
+    for (int k = 0; k < kIpcPerThread; ++k) {
+        if (f[k]) seg_start[pos++] = c0 + k;
+        if (fa[k]) alist[apos++] = c0 + k;
+    }
+}
+
+// Per segment, its pairs are taken in chunks of kIpcReduceThreads: the chunk's pairs
+// within dhat are compacted in order into LDS (ballot + popcount), then every output lane
+// adds its element over them in order. Pairs out of dhat contribute nothing, so the sums
+// are the same as over the whole segment in pair order.
 __global__ __launch_bounds__(kIpcReduceThreads) void k_ipc_reduce(const int* seg_start, const int* n_seg,
                                                                   const unsigned long long* key, const int* act,
                                                                   const double* pe, const double* pg,
                                                                   const double* pH, int* seg_bodies, double* seg_E,
                                                                   double* seg_g, double* seg_H) {
+    constexpr int kW = kIpcReduceThreads / 64;
+    __shared__ int list[kIpcReduceThreads];
+    __shared__ int wcnt[kW];
     const int ns = *n_seg;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, wv = t / 64, ln = t % 64;
     for (int s = blockIdx.x; s < ns; s += gridDim.x) {
         const int i0 = seg_start[s], i1 = seg_start[s + 1];
         double acc = 0.0;
+        for (int c0 = i0; c0 < i1; c0 += kIpcReduceThreads) {
+            const int i = c0 + t;
+            const bool a = i < i1 && act[i] != 0;
+            const unsigned long long m = __ballot(a);
+            if (ln == 0) wcnt[wv] = __popcll(m);
+            __syncthreads();
+            int base = 0, total = 0;
+            for (int k = 0; k < kW; ++k) {
+                if (k < wv) base += wcnt[k];
+                total += wcnt[k];
+            }
+            if (a) list[base + __popcll(m & ((1ull << ln) - 1ull))] = i;
+            __syncthreads();
+            if (t == 0) {
+                for (int k = 0; k < total; ++k) acc += pe[list[k]];
+            } else if (t <= 12) {
+                for (int k = 0; k < total; ++k) acc += pg[12 * size_t(list[k]) + (t - 1)];
+            } else if (t <= 156) {
+                for (int k = 0; k < total; ++k) acc += pH[144 * size_t(list[k]) + (t - 13)];
+            }
+            __syncthreads();
+        }
         if (t == 0) {
-            for (int i = i0; i < i1; ++i)
-                if (act[i]) acc += pe[i];
             seg_E[s] = acc;
             const unsigned long long k = key[i0];
             seg_bodies[2 * s] = static_cast<int>(k >> 32);
             seg_bodies[2 * s + 1] = static_cast<int>(k & 0xffffffffull);
         } else if (t <= 12) {
-            for (int i = i0; i < i1; ++i)
-                if (act[i]) acc += pg[12 * size_t(i) + (t - 1)];
             seg_g[12 * size_t(s) + (t - 1)] = acc;
         } else if (t <= 156) {
-            for (int i = i0; i < i1; ++i)
-                if (act[i]) acc += pH[144 * size_t(i) + (t - 13)];
             seg_H[144 * size_t(s) + (t - 13)] = acc;
         }
     }
@@ -338,12 +495,12 @@ __global__ void k_ipc_total(const int* n_seg, const double* seg_E, double* value
 // keeping sep of the pair's current distance. mu = sum over the two bodies of |dp| +
 // |dtheta| r_max (r_max: the largest body-frame radius of the pair's vertices on that
 // body), margin = 2e-10 (r_max A + r_max B) for the rotation skipped below |theta| =
-// 1e-10. toi[i] = the pair's advance (+inf: no impact in the step); st[i] = -3 for a
-// bad pair.
+// 1e-10. toi[i] = the pair's advance (+inf: no impact in the step), or where it stopped
+// once it could no longer be the least; st[i] = -3 for a bad pair.
 constexpr int kIpcAdvanceIters = 1000;
 
 __global__ void k_ipc_advance(Dev d, int b0, int nb, int np, const int* pairs, const double* x, const double* dx,
-                              double sep, double* toi, int* st) {
+                              double sep, double* toi, int* st, unsigned long long* best_bits) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const int* pr = pairs + 5 * i;
@@ -404,56 +561,84 @@ __global__ void k_ipc_advance(Dev d, int b0, int nb, int np, const int* pairs, c
             xw[3 * v + 2] = w.z;
         }
     };
-    toi[i] = ipc::pair_advance(kind, world, mu, sep, margin, kIpcAdvanceIters);
+    // a pair whose advance has reached the least finished advance so far cannot be the
+    // least: it stops there (its true advance is no smaller), which leaves the least
+    // over all pairs unchanged
+    const volatile unsigned long long* bb = best_bits;
+    auto done = [&](double t) { return !(t < __longlong_as_double(static_cast<long long>(*bb))); };
+    const double t = ipc::pair_advance(kind, world, mu, sep, margin, kIpcAdvanceIters, done);
+    toi[i] = t;
     st[i] = 0;
+    if (t < 1.0) atomicMin(best_bits, static_cast<unsigned long long>(__double_as_longlong(t)));
 }
 
-// least advance (order-free, so exact) and the first bad pair (np: none)
-__global__ __launch_bounds__(1024) void k_ipc_advance_min(int np, const double* toi, const int* st, double* out_t,
-                                                          int* out_bad) {
-    __shared__ double sm[1024];
-    __shared__ int sb[1024];
-    double m = 1.0 / 0.0;
+// least advance and first bad pair over every pair: a grid-wide min through 64-bit
+// atomics on the advances' bit patterns (non-negative doubles, +inf included, order as
+// their bits do), exact in any order; *out_bits starts at the bits of +inf, *out_bad at np
+__global__ __launch_bounds__(256) void k_ipc_advance_min(int np, const double* toi, const int* st,
+                                                         unsigned long long* out_bits, int* out_bad) {
+    __shared__ unsigned long long sm[256];
+    __shared__ int sb[256];
+    unsigned long long m = 0x7ff0000000000000ull;
     int fb = np;
-    for (int i = threadIdx.x; i < np; i += 1024) {
-        m = dmin(m, toi[i]);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < np; i += gridDim.x * 256) {
+        const unsigned long long b = __double_as_longlong(toi[i]);
+        m = b < m ? b : m;
         if (st[i] == -3 && i < fb) fb = i;
     }
     sm[threadIdx.x] = m;
     sb[threadIdx.x] = fb;
     __syncthreads();
-    for (int h = 512; h > 0; h >>= 1) {
+    for (int h = 128; h > 0; h >>= 1) {
         if (threadIdx.x < h) {
-            sm[threadIdx.x] = dmin(sm[threadIdx.x], sm[threadIdx.x + h]);
+            sm[threadIdx.x] = sm[threadIdx.x + h] < sm[threadIdx.x] ? sm[threadIdx.x + h] : sm[threadIdx.x];
             sb[threadIdx.x] = min(sb[threadIdx.x], sb[threadIdx.x + h]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        *out_t = sm[0];
-        *out_bad = sb[0];
+        atomicMin(out_bits, sm[0]);
+        atomicMin(out_bad, sb[0]);
     }
 }
 
 void launch_ipc_advance(const Dev& d, int b0, int nb, int np, const int* pairs, const double* x, const double* dx,
                         double sep, double* toi, int* st, double* out_t, int* out_bad, hipStream_t stream) {
     if (np <= 0) return;
+    timing_begin("ipc_advance", stream);
     hipLaunchKernelGGL(k_ipc_advance, dim3((np + 63) / 64), dim3(64), 0, stream, d, b0, nb, np, pairs, x, dx, sep, toi,
-                       st);
-    hipLaunchKernelGGL(k_ipc_advance_min, dim3(1), dim3(1024), 0, stream, np, toi, st, out_t, out_bad);
+                       st, reinterpret_cast<unsigned long long*>(out_t));
+    const int grid = (np + 255) / 256 < 1024 ? (np + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_ipc_advance_min, dim3(grid), dim3(256), 0, stream, np, toi, st,
+                       reinterpret_cast<unsigned long long*>(out_t), out_bad);
+    timing_end("ipc_advance", stream);
 }
 
 void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np, double dhat, double kappa, int psd,
                      hipStream_t st) {
     if (np <= 0) return;
-    hipLaunchKernelGGL(k_ipc_pairs, dim3((np + kIpcPairsPerWg - 1) / kIpcPairsPerWg), dim3(64), 0, st, d, b0, nb, np,
-                       B.pairs, B.x, dhat, kappa,
-                       psd, B.key, B.act, B.bad, B.pe, B.pg, B.pH);
-    hipLaunchKernelGGL(k_ipc_segments, dim3(1), dim3(kIpcScanThreads), 0, st, np, B.key, B.seg_start, B.n_seg);
+    const int nchunk = (np + kIpcChunk - 1) / kIpcChunk;
+    timing_begin("ipc_filter", st);
+    hipLaunchKernelGGL(k_ipc_filter, dim3((np + 255) / 256), dim3(256), 0, st, d, b0, nb, np, B.pairs, B.x, dhat,
+                       B.key, B.act, B.bad);
+    hipLaunchKernelGGL(k_ipc_seg_count, dim3(nchunk), dim3(kIpcChunkThreads), 0, st, np, B.key, B.act, B.chunk_cnt,
+                       B.chunk_act);
+    hipLaunchKernelGGL(k_ipc_seg_scan, dim3(1), dim3(kIpcScanThreads), 0, st, np, nchunk, B.chunk_cnt, B.chunk_act,
+                       B.seg_start, B.n_seg, B.n_act);
+    hipLaunchKernelGGL(k_ipc_seg_write, dim3(nchunk), dim3(kIpcChunkThreads), 0, st, np, B.key, B.act, B.chunk_cnt,
+                       B.chunk_act, B.seg_start, B.alist);
+    timing_end("ipc_filter", st);
+    timing_begin("ipc_pairs", st);
+    const int gp = (np + kIpcPairsPerWg - 1) / kIpcPairsPerWg;
+    hipLaunchKernelGGL(k_ipc_pairs, dim3(gp < 2048 ? gp : 2048), dim3(64), 0, st, d, b0, B.pairs, B.x, dhat, kappa, psd,
+                       B.alist, B.n_act, B.pe, B.pg, B.pH);
+    timing_end("ipc_pairs", st);
+    timing_begin("ipc_reduce", st);
     const int grid = np < 4096 ? np : 4096;
     hipLaunchKernelGGL(k_ipc_reduce, dim3(grid), dim3(kIpcReduceThreads), 0, st, B.seg_start, B.n_seg, B.key, B.act,
                        B.pe, B.pg, B.pH, B.seg_bodies, B.seg_E, B.seg_g, B.seg_H);
     hipLaunchKernelGGL(k_ipc_total, dim3(1), dim3(64), 0, st, B.n_seg, B.seg_E, B.value);
+    timing_end("ipc_reduce", st);
 }
 
 }  // namespace nd

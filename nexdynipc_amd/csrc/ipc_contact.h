@@ -249,9 +249,10 @@ ND_DU double pair_distance(int kind, const double* x, int* type_out) {
 // sep * d(0) + margin. From d = d(0), t advances by 0.9 (d - keep) / mu, so d stays
 // above keep up to the returned t. Returns that t once the gap d - keep is down to 1e-6
 // of its start (or after max_iter advances), +inf when t passes 1 first or nothing
-// moves.
-template <class F>
-ND_DU double pair_advance(int kind, F&& world, double mu, double sep, double margin, int max_iter) {
+// moves. done(t) may end the advance early (returning t, a lower bound of the full
+// result) when the caller knows the pair cannot give the least result any more.
+template <class F, class G>
+ND_DU double pair_advance(int kind, F&& world, double mu, double sep, double margin, int max_iter, G&& done) {
     double xw[kN];
     world(0.0, xw);
     const double d0 = pair_distance(kind, xw, nullptr);
@@ -261,7 +262,7 @@ ND_DU double pair_advance(int kind, F&& world, double mu, double sep, double mar
     double t = 0.0, d = d0;
     for (int k = 0; k < max_iter; ++k) {
         const double gap = d - keep;
-        if (!(gap > stop) || !(gap > 0.0)) return t;
+        if (!(gap > stop) || !(gap > 0.0) || done(t)) return t;
         t = t + 0.9 * gap / mu;
         if (!(t < 1.0)) return __builtin_inf();
         world(t, xw);

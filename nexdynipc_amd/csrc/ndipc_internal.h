@@ -306,6 +306,9 @@ struct IpcEvalBufs {
     double *pe, *pg, *pH;          // [np], [12 np], [144 np]
     int* seg_start;                // [np + 1]
     int* n_seg;                    // [1]
+    int* n_act;                    // [1] pairs within dhat
+    int *chunk_cnt, *chunk_act;    // [ceil(np / 1024)] per-chunk segment starts / active pairs
+    int* alist;                    // [np] the pairs within dhat, in pair order
     int* seg_bodies;               // [2 np]
     double *seg_E, *seg_g, *seg_H; // [np], [12 np], [144 np]
     double* value;                 // [1]
@@ -313,7 +316,8 @@ struct IpcEvalBufs {
 void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np, double dhat, double kappa, int psd,
                      hipStream_t st);
 // ipc mode maxStep: conservative advancement of every pair from x along dx; out_t the
-// least advance (+inf: none), out_bad the first bad pair (np: none)
+// least advance (+inf: none; must hold +inf before the launch), out_bad the first bad
+// pair (np: none; must hold np before the launch)
 void launch_ipc_advance(const Dev& d, int b0, int nb, int np, const int* pairs, const double* x, const double* dx,
                         double sep, double* toi, int* st, double* out_t, int* out_bad, hipStream_t stream);
 void launch_pcg(int nb, const double* diag, int noff, const int* offij, const double* offb, const double* b,

@@ -2141,7 +2141,8 @@ int Solver::ipc_max_step(const int32_t* pairs, int np, const vector<double>& x, 
                 xw[3 * v + 2] = w.z;
             }
         };
-        const double t = nd::ipc::pair_advance(kind, world, mu, sep, margin, kIpcAdvanceIters);
+        const double t = nd::ipc::pair_advance(kind, world, mu, sep, margin, kIpcAdvanceIters,
+                                               [](double) { return false; });
         toi = std::min(toi, t);
     }
     alpha = toi < 1.0 ? toi * safety : 1.0;

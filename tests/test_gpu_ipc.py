@@ -56,6 +56,8 @@ def test_ipc_eval_matches_oracle(engine_factory, psd):
         far, _, _ = _pairs_near(sc, x, V, F, 2.5 * dhat)
         for pl in (pairs, far, far[::-1]):
             _same(e.ipc_barrier_eval(0, pl, x, dhat, kappa, psd), o.ipc_barrier_eval(pl, x, dhat, kappa, psd))
+        # the pairs within dhat are the close list (textbook distances)
+        assert e.ipc_barrier_eval(0, far, x, dhat, kappa, psd, with_active=True)[4] == len(pairs)
 
 
 def test_ipc_eval_segments_and_errors(engine_factory):
