@@ -14,7 +14,7 @@
 //                   fold into the two bodies' 12 DOFs; results pair-major in HBM;
 //   k_ipc_reduce    one work-group per segment (persistent over segments), one lane per
 //                   output element (E, 12 gradient, 144 Hessian entries), each summing
-//                   its element over the segment's pairs in pair order;
+//                   its element over the segment's pairs within dhat in pair order;
 //   k_ipc_total     the scene energy, the segments' energies summed in segment order;
 //   k_ipc_advance   maxStep: one lane per pair, conservative advancement along the
 //                   line-search path, then the least over the pairs.
@@ -429,64 +429,63 @@ __global__ __launch_bounds__(kIpcChunkThreads) void k_ipc_seg_write(int np, cons
     }
 }
 
-// Per segment, its pairs are taken in chunks of kIpcReduceThreads: the chunk's pairs
-// within dhat are compacted in order into LDS (ballot + popcount), then every output lane
-// adds its element over them in order. Pairs out of dhat contribute nothing, so the sums
-// are the same as over the whole segment in pair order.
+// The pairs within dhat of segment s are a contiguous run of the in-order list alist:
+// [first entry >= seg_start[s], first entry >= seg_start[s + 1]), found by binary search.
+// Every output lane adds its element over that run in order; pairs out of dhat
+// contribute nothing, so the sums are the same as over the whole segment in pair order.
+__device__ __forceinline__ int lower_entry(const int* a, int n, int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(kIpcReduceThreads) void k_ipc_reduce(const int* seg_start, const int* n_seg,
-                                                                  const unsigned long long* key, const int* act,
-                                                                  const double* pe, const double* pg,
-                                                                  const double* pH, int* seg_bodies, double* seg_E,
-                                                                  double* seg_g, double* seg_H) {
-    constexpr int kW = kIpcReduceThreads / 64;
-    __shared__ int list[kIpcReduceThreads];
-    __shared__ int wcnt[kW];
-    const int ns = *n_seg;
-    const int t = threadIdx.x, wv = t / 64, ln = t % 64;
+                                                                  const unsigned long long* key, const int* alist,
+                                                                  const int* n_act, const double* pe,
+                                                                  const double* pg, const double* pH,
+                                                                  int* seg_bodies, double* seg_E, double* seg_g,
+                                                                  double* seg_H) {
+    const int ns = *n_seg, na = *n_act;
+    const int t = threadIdx.x;
     for (int s = blockIdx.x; s < ns; s += gridDim.x) {
         const int i0 = seg_start[s], i1 = seg_start[s + 1];
+        const int k0 = lower_entry(alist, na, i0), k1 = lower_entry(alist, na, i1);
         double acc = 0.0;
-        for (int c0 = i0; c0 < i1; c0 += kIpcReduceThreads) {
-            const int i = c0 + t;
-            const bool a = i < i1 && act[i] != 0;
-            const unsigned long long m = __ballot(a);
-            if (ln == 0) wcnt[wv] = __popcll(m);
-            __syncthreads();
-            int base = 0, total = 0;
-            for (int k = 0; k < kW; ++k) {
-                if (k < wv) base += wcnt[k];
-                total += wcnt[k];
-            }
-            if (a) list[base + __popcll(m & ((1ull << ln) - 1ull))] = i;
-            __syncthreads();
-            if (t == 0) {
-                for (int k = 0; k < total; ++k) acc += pe[list[k]];
-            } else if (t <= 12) {
-                for (int k = 0; k < total; ++k) acc += pg[12 * size_t(list[k]) + (t - 1)];
-            } else if (t <= 156) {
-                for (int k = 0; k < total; ++k) acc += pH[144 * size_t(list[k]) + (t - 13)];
-            }
-            __syncthreads();
-        }
         if (t == 0) {
+            for (int k = k0; k < k1; ++k) acc += pe[alist[k]];
             seg_E[s] = acc;
-            const unsigned long long k = key[i0];
-            seg_bodies[2 * s] = static_cast<int>(k >> 32);
-            seg_bodies[2 * s + 1] = static_cast<int>(k & 0xffffffffull);
+            const unsigned long long kk = key[i0];
+            seg_bodies[2 * s] = static_cast<int>(kk >> 32);
+            seg_bodies[2 * s + 1] = static_cast<int>(kk & 0xffffffffull);
         } else if (t <= 12) {
+            for (int k = k0; k < k1; ++k) acc += pg[12 * size_t(alist[k]) + (t - 1)];
             seg_g[12 * size_t(s) + (t - 1)] = acc;
         } else if (t <= 156) {
+            for (int k = k0; k < k1; ++k) acc += pH[144 * size_t(alist[k]) + (t - 13)];
             seg_H[144 * size_t(s) + (t - 13)] = acc;
         }
     }
 }
 
-__global__ void k_ipc_total(const int* n_seg, const double* seg_E, double* value) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double v = 0.0;
+// the segments' energies summed in segment order: staged through LDS in blocks (all
+// lanes load), each block summed in order by one lane
+__global__ __launch_bounds__(1024) void k_ipc_total(const int* n_seg, const double* seg_E, double* value) {
+    __shared__ double se[1024];
     const int ns = *n_seg;
-    for (int s = 0; s < ns; ++s) v += seg_E[s];
-    *value = v;
+    double v = 0.0;
+    for (int s0 = 0; s0 < ns; s0 += 1024) {
+        const int m = ns - s0 < 1024 ? ns - s0 : 1024;
+        if (static_cast<int>(threadIdx.x) < m) se[threadIdx.x] = seg_E[s0 + threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < m; ++k) v += se[k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *value = v;
 }
 
 // ipc mode, OptimizationProblem::maxStep (NewtonSolver.h:42, NewtonSolver.cpp:53-54),
@@ -499,8 +498,9 @@ __global__ void k_ipc_total(const int* n_seg, const double* seg_E, double* value
 // once it could no longer be the least; st[i] = -3 for a bad pair.
 constexpr int kIpcAdvanceIters = 1000;
 
-__global__ void k_ipc_advance(Dev d, int b0, int nb, int np, const int* pairs, const double* x, const double* dx,
-                              double sep, double* toi, int* st, unsigned long long* best_bits) {
+__global__ __launch_bounds__(64) void k_ipc_advance(Dev d, int b0, int nb, int np, const int* pairs, const double* x,
+                                                    const double* dx, double sep, double* toi, int* st,
+                                                    unsigned long long* best_bits) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const int* pr = pairs + 5 * i;
@@ -514,32 +514,34 @@ __global__ void k_ipc_advance(Dev d, int b0, int nb, int np, const int* pairs, c
         st[i] = -3;
         return;
     }
-    int vb[4];
     V3 r[4];
     if (kind == 0) {
-        vb[0] = a;
         r[0] = ld3(d.verts, d.v_off[ga] + ia);
         const int* f = d.faces + 3 * (d.f_off[gb] + ib);
-        for (int k = 0; k < 3; ++k) {
-            vb[1 + k] = b;
-            r[1 + k] = ld3(d.verts, d.v_off[gb] + f[k]);
-        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[1 + k] = ld3(d.verts, d.v_off[gb] + f[k]);
     } else {
         const int* ea = d.edges + 2 * (d.e_off[ga] + ia);
         const int* eb = d.edges + 2 * (d.e_off[gb] + ib);
+#pragma unroll
         for (int k = 0; k < 2; ++k) {
-            vb[k] = a;
             r[k] = ld3(d.verts, d.v_off[ga] + ea[k]);
-            vb[2 + k] = b;
             r[2 + k] = ld3(d.verts, d.v_off[gb] + eb[k]);
         }
     }
-    double rmax[2] = {0.0, 0.0};
+    // every per-vertex quantity in registers (unrolled, fixed indices): vertex v is on
+    // body a for v < na_ (one vertex of a point-triangle pair, two of an edge-edge pair)
+    const int na_ = kind == 0 ? 1 : 2;
+    double rmax_a = 0.0, rmax_b = 0.0;
+#pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int side = vb[v] == a ? 0 : 1;
-        rmax[side] = dmax(rmax[side], sqrt((r[v].x * r[v].x + r[v].y * r[v].y) + r[v].z * r[v].z));
+        const double rl = sqrt((r[v].x * r[v].x + r[v].y * r[v].y) + r[v].z * r[v].z);
+        if (v < na_) rmax_a = dmax(rmax_a, rl);
+        else rmax_b = dmax(rmax_b, rl);
     }
+    const double rmax[2] = {rmax_a, rmax_b};
     double mu = 0.0;
+#pragma unroll
     for (int side = 0; side < 2; ++side) {
         const int o = 6 * (side == 0 ? a : b);
         const double lp = sqrt((dx[o] * dx[o] + dx[o + 1] * dx[o + 1]) + dx[o + 2] * dx[o + 2]);
@@ -547,15 +549,16 @@ __global__ void k_ipc_advance(Dev d, int b0, int nb, int np, const int* pairs, c
         mu = mu + (lp + lt * rmax[side]);
     }
     const double margin = 2e-10 * (rmax[0] + rmax[1]);
-    Q q0[2];
-    q0[0] = ld4(d.q, ga);
-    q0[1] = ld4(d.q, gb);
+    const Q qa = ld4(d.q, ga), qb = ld4(d.q, gb);
+    const int oa = 6 * a, ob = 6 * b;
     auto world = [&](double t, double* xw) {
+#pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int o = 6 * vb[v];
+            const bool on_a = v < na_;
+            const int o = on_a ? oa : ob;
             const V3 p = v3(x[o] + t * dx[o], x[o + 1] + t * dx[o + 1], x[o + 2] + t * dx[o + 2]);
             const V3 th = v3(x[o + 3] + t * dx[o + 3], x[o + 4] + t * dx[o + 4], x[o + 5] + t * dx[o + 5]);
-            const V3 w = p + rotate(apply_theta(q0[vb[v] == a ? 0 : 1], th), r[v]);
+            const V3 w = p + rotate(apply_theta(on_a ? qa : qb, th), r[v]);
             xw[3 * v] = w.x;
             xw[3 * v + 1] = w.y;
             xw[3 * v + 2] = w.z;
@@ -635,9 +638,9 @@ void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np,
     timing_end("ipc_pairs", st);
     timing_begin("ipc_reduce", st);
     const int grid = np < 4096 ? np : 4096;
-    hipLaunchKernelGGL(k_ipc_reduce, dim3(grid), dim3(kIpcReduceThreads), 0, st, B.seg_start, B.n_seg, B.key, B.act,
-                       B.pe, B.pg, B.pH, B.seg_bodies, B.seg_E, B.seg_g, B.seg_H);
-    hipLaunchKernelGGL(k_ipc_total, dim3(1), dim3(64), 0, st, B.n_seg, B.seg_E, B.value);
+    hipLaunchKernelGGL(k_ipc_reduce, dim3(grid), dim3(kIpcReduceThreads), 0, st, B.seg_start, B.n_seg, B.key, B.alist,
+                       B.n_act, B.pe, B.pg, B.pH, B.seg_bodies, B.seg_E, B.seg_g, B.seg_H);
+    hipLaunchKernelGGL(k_ipc_total, dim3(1), dim3(1024), 0, st, B.n_seg, B.seg_E, B.value);
     timing_end("ipc_reduce", st);
 }
 
