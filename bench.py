@@ -82,17 +82,21 @@ def cpu_baseline_c1(steps=200):
     return steps / T, T
 
 
-def load_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_c2.json")
+def load_traffic(kernel, workload="c2"):
+    """HBM bytes per launch of `kernel` (a name or a list of names, summed) from the
+    committed counter summary of `workload`, or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    k = t.get("kernels", {}).get("k_" + kernel)
-    if not k:
-        return None
-    return 1024.0 * (k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"])
+    total = 0.0
+    for name in [kernel] if isinstance(kernel, str) else kernel:
+        k = t.get("kernels", {}).get("k_" + name)
+        if not k:
+            return None
+        total += 1024.0 * (k["fetch_kib"] * t["fetch_factor"] + k["write_kib"] * t["write_factor"])
+    return total
 
 
 def run_plain(Engine, torch, sc, steps, warmup, timing=False, graph=True, autotune=False):
@@ -278,11 +282,13 @@ def run_ipc(Engine, torch, steps=200, dhat=1e-2, kappa=1e5, max_pairs=1 << 20, r
             "roofline": {"kernel": "k_ipc_pairs", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": b_pairs,
                          "avg_launch_ms": kp, "launches": nk,
+                         "traffic": load_traffic("ipc_pairs", "ipc"),
                          "note": "latency-bound: per pair a 12 x 12 Jacobi eigen-decomposition whose rotations "
                                  "are dependent LDS steps, one wavefront per SIMD"},
             "roofline_filter": {"kernel": "k_ipc_filter + k_ipc_seg_*", "bound": "hbm", "achieved": achieved_f,
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_f / HBM_PEAK_GBS,
-                                "algorithmic_bytes_per_launch": b_filter, "avg_launch_ms": kf, "launches": nk}}
+                                "algorithmic_bytes_per_launch": b_filter, "avg_launch_ms": kf, "launches": nk,
+                                "traffic": load_traffic(["ipc_filter", "ipc_seg_count", "ipc_seg_write"], "ipc")}}
 
 
 def roofline_for(name, r, sc):
