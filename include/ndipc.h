@@ -198,6 +198,23 @@ int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const 
  * bad pair. */
 int32_t ndipc_ipc_max_step(ndipc_ctx* ctx, int32_t scene, int32_t np, const int32_t* pairs5, const double* x,
                            const double* dx, double sep, double safety, double* alpha, double* toi);
+/* ipc mode time step of one scene: IPCSolver::step's implicit-Euler Newton solve
+ * (IPCSolver.cpp:137-262, NewtonSolver.cpp:10-95) with the ipc barrier in place of the
+ * centre-distance barrier and maxStep wired into the line search, for scenes whose only
+ * joints pin single bodies (those bodies are static). Objective as the reference's:
+ * sum over moving bodies of 1/2 m |x_p - x^_p|^2 + 1/2 dtheta^T I dtheta, + h^2 (gravity
+ * + E_ipc), x^ = (p + h v, h w). Pairs: the top-K lists of the close pairs at `inflation`
+ * (faces x vertices both ways, edges x edges), fixed for the step. Each Newton iteration
+ * solves H dx = -g by block-Jacobi PCG (pcg_tol, pcg_max_it), stops when |dx|_inf <=
+ * newton_tol or dx does not descend, starts the line search at min(1, maxStep) (sep,
+ * safety; ndipc_ipc_max_step) and halves until Armijo (1e-4) holds (max_ls trials).
+ * Writes p, q, v, w of the moving bodies. stats4 (may be NULL): Newton iterations,
+ * line-search trials, pair records, PCG iterations. */
+typedef struct ndipc_ipc_params {
+    double dhat, kappa, sep, safety, newton_tol, pcg_tol, inflation;
+    int32_t max_newton, max_ls, pcg_max_it;
+} ndipc_ipc_params;
+int32_t ndipc_ipc_step(ndipc_ctx* ctx, int32_t scene, double dt, const ndipc_ipc_params* params, int32_t* stats4);
 /* Interval TOI batch (TimeOfImpact.cpp:8-150): kind 0 edge-vertex, 1 edge-edge,
  * 2 face-vertex as the reference computes them; the ipc mode's 3 face-vertex (boxes that
  * meet the triangle u, v >= 0, u + v <= 1 are kept, where the reference's predicate

@@ -42,6 +42,12 @@ class Params(C.Structure):
     ]
 
 
+class IpcParams(C.Structure):
+    _fields_ = [("dhat", C.c_double), ("kappa", C.c_double), ("sep", C.c_double), ("safety", C.c_double),
+                ("newton_tol", C.c_double), ("pcg_tol", C.c_double), ("inflation", C.c_double),
+                ("max_newton", C.c_int32), ("max_ls", C.c_int32), ("pcg_max_it", C.c_int32)]
+
+
 EXPORTS = [
     "ndipc_create", "ndipc_destroy", "ndipc_last_error", "ndipc_stream", "ndipc_upload_scenes", "ndipc_configure",
     "ndipc_set_scene_mu", "ndipc_set_state", "ndipc_get_state", "ndipc_get_barrier", "ndipc_set_barrier",
@@ -52,7 +58,7 @@ EXPORTS = [
     "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
     "ndipc_autotune", "ndipc_helper_counters", "ndipc_helper_config",
     "ndipc_set_step_mode", "ndipc_pcg_solve_noalloc", "ndipc_last_residuals", "ndipc_ipc_barrier_eval",
-    "ndipc_ipc_max_step",
+    "ndipc_ipc_max_step", "ndipc_ipc_step",
 ]
 
 
@@ -104,6 +110,7 @@ def library():
                                     C.c_int32, i32p, i32p, f64p, f64p, i32p], C.c_int32),
         "ndipc_ipc_max_step": ([vp, C.c_int32, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_double, f64p, f64p],
                                C.c_int32),
+        "ndipc_ipc_step": ([vp, C.c_int32, C.c_double, C.POINTER(IpcParams), i32p], C.c_int32),
         "ndipc_friction_eval": ([vp, C.c_int32, C.c_int32, i32p, f64p, f64p, f64p, C.c_double, C.c_double,
                                  C.c_double, f64p, f64p, f64p], C.c_int32),
         "ndipc_ccd_toi_batch": ([vp, C.c_int32, C.c_int32, f64p, f64p, f64p, f64p, f64p, C.c_double, i32p, f64p,
@@ -426,6 +433,16 @@ class Engine:
         self._ck(self.L.ndipc_ipc_max_step(self.ctx, scene, len(pr), ptr(pr, i32p), ptr(x, f64p), ptr(dx, f64p), sep,
                                            safety, C.byref(a), C.byref(t)), "ipc_max_step")
         return a.value, t.value
+
+    def ipc_step(self, scene, dt, dhat=1e-3, kappa=1e4, sep=0.1, safety=0.8, newton_tol=1e-9, pcg_tol=1e-12,
+                 inflation=None, max_newton=50, max_ls=30, pcg_max_it=2000):
+        """ipc mode time step of one scene; (Newton iterations, line-search trials, pairs,
+        PCG iterations)."""
+        prm = IpcParams(dhat, kappa, sep, safety, newton_tol, pcg_tol, dhat if inflation is None else inflation,
+                        max_newton, max_ls, pcg_max_it)
+        st = np.zeros(4, dtype=np.int32)
+        self._ck(self.L.ndipc_ipc_step(self.ctx, scene, dt, C.byref(prm), ptr(st, i32p)), "ipc_step")
+        return tuple(int(v) for v in st)
 
     def friction_eval(self, scene, ints, dbl, nf, x, mu, eps, dt, hess=True):
         nd = len(x)

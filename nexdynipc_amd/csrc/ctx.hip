@@ -1363,6 +1363,189 @@ int32_t ndipc_ipc_barrier_eval(ndipc_ctx* ctx, int32_t scene, int32_t np, const 
     return 0;
 }
 
+int32_t ndipc_ipc_step(ndipc_ctx* ctx, int32_t scene, double dt, const ndipc_ipc_params* prm, int32_t* stats4) {
+    if (check_ctx(ctx)) return -1;
+    if (scene < 0 || scene >= ctx->S) return fail(ctx, "scene out of range");
+    if (!prm) return fail(ctx, "null params");
+    const ndipc_ipc_params P = *prm;
+    if (!(dt > 0.0) || !(P.dhat > 0.0) || !(P.sep >= 0.0 && P.sep < 1.0) || P.max_newton < 0 || P.max_ls < 1)
+        return fail(ctx, "bad ipc step parameters");
+    const int b0 = ctx->body_off[scene], nb = ctx->body_off[scene + 1] - b0, nd = 6 * nb;
+    if (nb == 0) return 0;
+    // static bodies: those pinned by a single-body fixed joint; other joints are not run here
+    const int j0 = ctx->joint_off[scene], j1 = ctx->joint_off[scene + 1];
+    std::vector<int> stat(nb, 0);
+    if (j1 > j0) {
+        std::vector<int> jt(j1 - j0), ja(j1 - j0), jb(j1 - j0);
+        HIPCHK(hipMemcpy(jt.data(), ctx->d.jtype + j0, 4 * size_t(j1 - j0), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ja.data(), ctx->d.ja + j0, 4 * size_t(j1 - j0), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(jb.data(), ctx->d.jb + j0, 4 * size_t(j1 - j0), hipMemcpyDeviceToHost));
+        for (int j = 0; j < j1 - j0; ++j) {
+            if (jt[j] != 0 || jb[j] >= 0) return fail(ctx, "ipc step: only single-body fixed joints are supported");
+            stat[ja[j] - b0] = 1;
+        }
+    }
+    ScratchBufs sb;  // freed on every return
+    int* dstat = nullptr;
+    HIPCHK(sb.alloc(&dstat, 4 * size_t(nb)));
+    HIPCHK(hipMemcpy(dstat, stat.data(), 4 * size_t(nb), hipMemcpyHostToDevice));
+    // pair records of the close pairs' top-K lists
+    prep_geometry(ctx);
+    nd::launch_broad_phase(ctx->d, ctx->w, 0, std::max(0.0, P.inflation), false, &scene, ctx->stream);
+    nd::launch_topk(ctx->d, ctx->w, max_grid_pairs(ctx), ctx->stream);
+    int ncl = 0;
+    HIPCHK(hipMemcpyAsync(&ncl, ctx->w.n_pairs_total, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int *dcnt = nullptr, *doff = nullptr;
+    HIPCHK(sb.alloc(&dcnt, 4 * size_t(std::max(ncl, 1))));
+    HIPCHK(sb.alloc(&doff, 4 * size_t(std::max(ncl, 1))));
+    nd::launch_ipc_emit(ctx->w, ncl, dstat, b0, dcnt, nullptr, nullptr, 0, ctx->stream);
+    std::vector<int> cnt(std::max(ncl, 1), 0), off(std::max(ncl, 1), 0);
+    if (ncl) HIPCHK(hipMemcpy(cnt.data(), dcnt, 4 * size_t(ncl), hipMemcpyDeviceToHost));
+    long long np_ll = 0;
+    for (int p = 0; p < ncl; ++p) {
+        off[p] = int(np_ll);
+        np_ll += cnt[p];
+    }
+    if (np_ll > NDIPC_IPC_MAX_PAIRS) return fail(ctx, "ipc step: more pair records than NDIPC_IPC_MAX_PAIRS");
+    const int np = int(np_ll);
+    const size_t n = size_t(std::max(np, 1)), nchunk = (n + 1023) / 1024, nsg = size_t(std::max(ncl, 1));
+    nd::IpcEvalBufs B;
+    int *ip = nullptr, *offij = nullptr;
+    unsigned long long* key = nullptr;
+    double *dp = nullptr, *v = nullptr;
+    HIPCHK(sb.alloc(&ip, 4 * (5 * n + n + 1 + n + 1 + 1 + 1 + 2 * n + 2 * nchunk + n + n)));
+    HIPCHK(sb.alloc(&key, 8 * n));
+    HIPCHK(sb.alloc(&dp, 8 * (n + 12 * n + 144 * n + n + 12 * n + 144 * n + 1 + n)));
+    HIPCHK(sb.alloc(&offij, 4 * 2 * nsg));
+    // x, x^, xt, dx, g, rhs (nd each), diag (36 nb), offb (36 per segment), ebody, gterm
+    // (nb each), sums (4), PCG work (10 nd), the advance's out (1)
+    HIPCHK(sb.alloc(&v, 8 * (6 * size_t(nd) + 36 * size_t(nb) + 36 * nsg + 2 * size_t(nb) + 4 + 10 * size_t(nd) + 1)));
+    int* iw = ip;
+    int* dpairs = iw;
+    iw += 5 * n;
+    B.pairs = dpairs;
+    B.act = iw; iw += n;
+    B.bad = iw; iw += 1;
+    B.seg_start = iw; iw += n + 1;
+    B.n_seg = iw; iw += 1;
+    B.n_act = iw; iw += 1;
+    B.seg_bodies = iw; iw += 2 * n;
+    B.chunk_cnt = iw; iw += nchunk;
+    B.chunk_act = iw; iw += nchunk;
+    B.alist = iw; iw += n;
+    int* ast = iw;  // the advance's per-pair status
+    B.key = key;
+    double* w = dp;
+    B.pe = w; w += n;
+    B.pg = w; w += 12 * n;
+    B.pH = w; w += 144 * n;
+    B.seg_E = w; w += n;
+    B.seg_g = w; w += 12 * n;
+    B.seg_H = w; w += 144 * n;
+    B.value = w; w += 1;
+    double* atoi = w;
+    double* x = v;
+    double* xh = x + nd;
+    double* xt = xh + nd;
+    double* dx = xt + nd;
+    double* g = dx + nd;
+    double* rhs = g + nd;
+    double* diag = rhs + nd;
+    double* offb = diag + 36 * size_t(nb);
+    double* ebody = offb + 36 * nsg;
+    double* gterm = ebody + nb;
+    double* sums = gterm + nb;
+    double* pcgw = sums + 4;
+    double* aout = pcgw + 10 * size_t(nd);
+    int* piters = B.bad;  // reused: the PCG's iteration count (B.bad is set before every evaluation)
+    double prel = 0.0;
+    (void)prel;
+    if (np) {
+        HIPCHK(hipMemcpy(doff, off.data(), 4 * size_t(ncl), hipMemcpyHostToDevice));
+        nd::launch_ipc_emit(ctx->w, ncl, dstat, b0, dcnt, doff, dpairs, 1, ctx->stream);
+    }
+    const double h = dt, h2 = dt * dt;
+    nd::launch_ipc_start(ctx->d, b0, nb, h, x, xh, ctx->stream);
+    // energy of the moving bodies + h^2 E_ipc at xx (full: also gradient terms and blocks)
+    auto evaluate = [&](const double* xx, bool full, double& E) -> int {
+        B.x = xx;
+        double ev = 0.0;
+        if (np) {
+            HIPCHK(hipMemcpy(B.bad, &np, 4, hipMemcpyHostToDevice));
+            nd::launch_ipc_eval(ctx->d, B, b0, nb, np, P.dhat, P.kappa, 1, ctx->stream, full ? 0 : 1);
+        } else {
+            const int zero = 0;
+            HIPCHK(hipMemcpy(B.n_seg, &zero, 4, hipMemcpyHostToDevice));
+        }
+        nd::launch_ipc_body(ctx->d, b0, nb, dstat, xx, xh, h2, full ? 0 : 1, B, g, diag, ebody, gterm, ctx->stream);
+        nd::launch_ipc_sums(nb, dstat, ebody, gterm, nullptr, nullptr, sums, ctx->stream);
+        double sm[2];
+        HIPCHK(hipMemcpyAsync(sm, sums, 16, hipMemcpyDeviceToHost, ctx->stream));
+        if (np) HIPCHK(hipMemcpyAsync(&ev, B.value, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        E = (sm[0] + h2 * sm[1]) + h2 * ev;
+        return 0;
+    };
+    int newton = 0, trials = 0, pcg_total = 0;
+    for (int it = 0; it < P.max_newton; ++it) {
+        double E0 = 0.0;
+        if (evaluate(x, true, E0)) return -1;
+        int nseg = 0;
+        HIPCHK(hipMemcpy(&nseg, B.n_seg, 4, hipMemcpyDeviceToHost));
+        nd::launch_ipc_offblocks(nd, nseg, dstat, h2, B, g, offij, offb, rhs, ctx->stream);
+        nd::launch_pcg(nb, diag, nseg, offij, offb, rhs, P.pcg_tol, P.pcg_max_it, dx, piters, sums + 3, pcgw,
+                       ctx->stream);
+        int pit = 0;
+        HIPCHK(hipMemcpyAsync(&pit, piters, 4, hipMemcpyDeviceToHost, ctx->stream));
+        nd::launch_ipc_sums(nb, dstat, ebody, gterm, g, dx, sums, ctx->stream);
+        double sm[4];
+        HIPCHK(hipMemcpyAsync(sm, sums, 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        pcg_total += pit;
+        const double slope = sm[2], dmax = sm[3];
+        if (!(dmax > P.newton_tol)) break;
+        if (!(slope < 0.0)) break;
+        double alpha = 1.0;
+        if (np) {
+            const double inf = 1.0 / 0.0;
+            HIPCHK(hipMemcpy(aout, &inf, 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(B.bad, &np, 4, hipMemcpyHostToDevice));
+            nd::launch_ipc_advance(ctx->d, b0, nb, np, dpairs, x, dx, P.sep, atoi, ast, aout, B.bad, ctx->stream);
+            double toi = 0.0;
+            HIPCHK(hipMemcpyAsync(&toi, aout, 8, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            alpha = toi < 1.0 ? toi * P.safety : 1.0;
+            alpha = std::min(1.0, alpha);
+        }
+        bool ok = false;
+        for (int k = 0; k < P.max_ls; ++k) {
+            ++trials;
+            nd::launch_ipc_axpy(nd, x, alpha, dx, xt, ctx->stream);
+            double Et = 0.0;
+            if (evaluate(xt, false, Et)) return -1;
+            if (Et <= E0 + 1e-4 * alpha * slope) {
+                ok = true;
+                break;
+            }
+            alpha *= 0.5;
+        }
+        if (!ok) break;
+        std::swap(x, xt);
+        ++newton;
+    }
+    nd::launch_ipc_writeback(ctx->d, b0, nb, dstat, x, h, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (stats4) {
+        stats4[0] = newton;
+        stats4[1] = trials;
+        stats4[2] = np;
+        stats4[3] = pcg_total;
+    }
+    return 0;
+}
+
 int32_t ndipc_ipc_max_step(ndipc_ctx* ctx, int32_t scene, int32_t np, const int32_t* pairs5, const double* x,
                            const double* dx, double sep, double safety, double* alpha, double* toi) {
     if (check_ctx(ctx)) return -1;

@@ -83,7 +83,8 @@ __device__ __forceinline__ double ipc_jel(int r, int c, const int* own, const do
 // distance value (the value part of the hyper-dual, whose division is a * (1 / b)) as
 // pass 2, so both passes agree on every pair.
 __global__ __launch_bounds__(256) void k_ipc_filter(Dev d, int b0, int nb, int np, const int* pairs, const double* x,
-                                                    double dhat, unsigned long long* key, int* act, int* bad) {
+                                                    double dhat, double kappa, int value_only, unsigned long long* key,
+                                                    int* act, int* bad, double* pe) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= np) return;
     const int* pr = pairs + 5 * i;
@@ -123,6 +124,8 @@ __global__ __launch_bounds__(256) void k_ipc_filter(Dev d, int b0, int nb, int n
     const HL d2 = ipc::pair_d2<HL>(kind, type, v);
     const double dist = sqrt(d2.v > 0.0 ? d2.v : 0.0);
     act[i] = dist < dhat ? 1 : 0;
+    // value only (line searches): the pair's energy here, as pass 2 computes it
+    if (value_only && dist < dhat) pe[i] = bar_v(dist, dhat, kappa);
 }
 
 // Pass 2 over the pairs within dhat (alist[0 .. *n_act), pair order), sixteen lanes per
@@ -446,7 +449,7 @@ __device__ __forceinline__ int lower_entry(const int* a, int n, int v) {
 __global__ __launch_bounds__(kIpcReduceThreads) void k_ipc_reduce(const int* seg_start, const int* n_seg,
                                                                   const unsigned long long* key, const int* alist,
                                                                   const int* n_act, const double* pe,
-                                                                  const double* pg, const double* pH,
+                                                                  const double* pg, const double* pH, int value_only,
                                                                   int* seg_bodies, double* seg_E, double* seg_g,
                                                                   double* seg_H) {
     const int ns = *n_seg, na = *n_act;
@@ -461,6 +464,7 @@ __global__ __launch_bounds__(kIpcReduceThreads) void k_ipc_reduce(const int* seg
             const unsigned long long kk = key[i0];
             seg_bodies[2 * s] = static_cast<int>(kk >> 32);
             seg_bodies[2 * s + 1] = static_cast<int>(kk & 0xffffffffull);
+        } else if (value_only) {
         } else if (t <= 12) {
             for (int k = k0; k < k1; ++k) acc += pg[12 * size_t(alist[k]) + (t - 1)];
             seg_g[12 * size_t(s) + (t - 1)] = acc;
@@ -618,12 +622,12 @@ void launch_ipc_advance(const Dev& d, int b0, int nb, int np, const int* pairs, 
 }
 
 void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np, double dhat, double kappa, int psd,
-                     hipStream_t st) {
+                     hipStream_t st, int value_only) {
     if (np <= 0) return;
     const int nchunk = (np + kIpcChunk - 1) / kIpcChunk;
     timing_begin("ipc_filter", st);
     hipLaunchKernelGGL(k_ipc_filter, dim3((np + 255) / 256), dim3(256), 0, st, d, b0, nb, np, B.pairs, B.x, dhat,
-                       B.key, B.act, B.bad);
+                       kappa, value_only, B.key, B.act, B.bad, B.pe);
     hipLaunchKernelGGL(k_ipc_seg_count, dim3(nchunk), dim3(kIpcChunkThreads), 0, st, np, B.key, B.act, B.chunk_cnt,
                        B.chunk_act);
     hipLaunchKernelGGL(k_ipc_seg_scan, dim3(1), dim3(kIpcScanThreads), 0, st, np, nchunk, B.chunk_cnt, B.chunk_act,
@@ -631,17 +635,206 @@ void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np,
     hipLaunchKernelGGL(k_ipc_seg_write, dim3(nchunk), dim3(kIpcChunkThreads), 0, st, np, B.key, B.act, B.chunk_cnt,
                        B.chunk_act, B.seg_start, B.alist);
     timing_end("ipc_filter", st);
-    timing_begin("ipc_pairs", st);
-    const int gp = (np + kIpcPairsPerWg - 1) / kIpcPairsPerWg;
-    hipLaunchKernelGGL(k_ipc_pairs, dim3(gp < 2048 ? gp : 2048), dim3(64), 0, st, d, b0, B.pairs, B.x, dhat, kappa, psd,
-                       B.alist, B.n_act, B.pe, B.pg, B.pH);
-    timing_end("ipc_pairs", st);
+    if (!value_only) {
+        timing_begin("ipc_pairs", st);
+        const int gp = (np + kIpcPairsPerWg - 1) / kIpcPairsPerWg;
+        hipLaunchKernelGGL(k_ipc_pairs, dim3(gp < 2048 ? gp : 2048), dim3(64), 0, st, d, b0, B.pairs, B.x, dhat, kappa,
+                           psd, B.alist, B.n_act, B.pe, B.pg, B.pH);
+        timing_end("ipc_pairs", st);
+    }
     timing_begin("ipc_reduce", st);
     const int grid = np < 4096 ? np : 4096;
     hipLaunchKernelGGL(k_ipc_reduce, dim3(grid), dim3(kIpcReduceThreads), 0, st, B.seg_start, B.n_seg, B.key, B.alist,
-                       B.n_act, B.pe, B.pg, B.pH, B.seg_bodies, B.seg_E, B.seg_g, B.seg_H);
+                       B.n_act, B.pe, B.pg, B.pH, value_only, B.seg_bodies, B.seg_E, B.seg_g, B.seg_H);
     hipLaunchKernelGGL(k_ipc_total, dim3(1), dim3(1024), 0, st, B.n_seg, B.seg_E, B.value);
     timing_end("ipc_reduce", st);
+}
+
+// ------------------------------------------------------------------ ipc-mode step
+// Per close pair of the scene: the number of pair records of its top-K lists (faces of A
+// x vertices of B, faces of B x vertices of A, edges of A x edges of B), 0 when both
+// bodies are static.
+__global__ void k_ipc_emit_count(int ncl, const int* cpairs, const int* lcnt, const int* stat, int b0, int* cnt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ncl) return;
+    const int A = cpairs[2 * p] - b0, B = cpairs[2 * p + 1] - b0;
+    const int* k = lcnt + 6 * p;
+    cnt[p] = (stat[A] && stat[B]) ? 0 : k[4] * k[1] + k[5] * k[0] + k[2] * k[3];
+}
+
+// One work-group per close pair writes its records from off[p], in the order above.
+__global__ void k_ipc_emit_write(int ncl, const int* cpairs, const int* lists, const int* lcnt, const int* off,
+                                 const int* cnt, int b0, int* out) {
+    const int p = blockIdx.x;
+    if (p >= ncl) return;
+    const int A = cpairs[2 * p] - b0, B = cpairs[2 * p + 1] - b0;
+    const int* l = lists + size_t(p) * 6 * kTopK;
+    const int* k = lcnt + 6 * p;
+    const int n1 = k[4] * k[1], n2 = k[5] * k[0];
+    for (int t = threadIdx.x; t < cnt[p]; t += blockDim.x) {
+        int* o = out + 5 * size_t(off[p] + t);
+        if (t < n1) {
+            const int i = t / k[1], j = t % k[1];
+            o[0] = 0; o[1] = B; o[2] = A; o[3] = l[kTopK + j]; o[4] = l[4 * kTopK + i];
+        } else if (t < n1 + n2) {
+            const int u = t - n1, i = u / k[0], j = u % k[0];
+            o[0] = 0; o[1] = A; o[2] = B; o[3] = l[j]; o[4] = l[5 * kTopK + i];
+        } else {
+            const int u = t - n1 - n2, i = u / k[3], j = u % k[3];
+            o[0] = 1; o[1] = A; o[2] = B; o[3] = l[2 * kTopK + i]; o[4] = l[3 * kTopK + j];
+        }
+    }
+}
+
+// x = (p, 0), x^ = (p + h v, h w) per body of the scene
+__global__ void k_ipc_start(Dev d, int b0, int nb, double h, double* x, double* xh) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int g = b0 + b;
+    for (int k = 0; k < 3; ++k) {
+        x[6 * b + k] = d.p[3 * g + k];
+        x[6 * b + 3 + k] = 0.0;
+        xh[6 * b + k] = d.p[3 * g + k] + h * d.v[3 * g + k];
+        xh[6 * b + 3 + k] = h * d.w[3 * g + k];
+    }
+}
+
+// Per body of the scene: its inertia energy and gravity term (ipc::body_terms); unless
+// energy_only, its gradient and diagonal block: inertia, h^2 x the gravity force, then
+// h^2 x its parts of the barrier segments in segment order. Static bodies: identity
+// block, zero gradient, no energy.
+__global__ void k_ipc_body(Dev d, int b0, int nb, const int* stat, const double* x, const double* xh, double h2,
+                           int energy_only, const int* n_seg, const int* seg_bodies, const double* seg_g,
+                           const double* seg_H, double* g, double* diag, double* ebody, double* gterm) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int gb = b0 + b;
+    double gl[6], D[36];
+    if (stat[b]) {
+        ebody[b] = 0.0;
+        gterm[b] = 0.0;
+        if (energy_only) return;
+        for (int k = 0; k < 36; ++k) diag[36 * b + k] = (k % 7 == 0) ? 1.0 : 0.0;
+        for (int k = 0; k < 6; ++k) g[6 * b + k] = 0.0;
+        return;
+    }
+    double e, sgd = 0.0;
+    const double m = d.mass[gb];
+    ipc::body_terms(m, d.inertia + 9 * gb, x + 6 * b, xh + 6 * b, e, gl, D, &sgd);
+    ebody[b] = e;
+    gterm[b] = sgd;
+    if (energy_only) return;
+    for (int k = 0; k < 3; ++k) gl[k] += h2 * (-m * ipc::gravity(k));
+    const int ns = *n_seg;
+    for (int sg = 0; sg < ns; ++sg) {
+        const int lo = seg_bodies[2 * sg], hi = seg_bodies[2 * sg + 1];
+        const int side = lo == b ? 0 : (hi == b ? 1 : -1);
+        if (side < 0) continue;
+        for (int r = 0; r < 6; ++r) {
+            gl[r] += h2 * seg_g[12 * size_t(sg) + 6 * side + r];
+            for (int c = 0; c < 6; ++c)
+                D[6 * r + c] += h2 * seg_H[144 * size_t(sg) + 12 * (6 * side + r) + 6 * side + c];
+        }
+    }
+    for (int k = 0; k < 6; ++k) g[6 * b + k] = gl[k];
+    for (int k = 0; k < 36; ++k) diag[36 * b + k] = D[k];
+}
+
+// one off-diagonal block per segment: h^2 x its (lower, higher) block, zero when either
+// body is static; and rhs = -g
+__global__ void k_ipc_offblocks(int nd, const int* stat, double h2, const int* n_seg, const int* seg_bodies,
+                                const double* seg_H, const double* g, int* offij, double* offb, double* rhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nd) rhs[i] = -g[i];
+    const int ns = *n_seg;
+    if (i >= ns) return;
+    const int lo = seg_bodies[2 * i], hi = seg_bodies[2 * i + 1];
+    offij[2 * i] = lo;
+    offij[2 * i + 1] = hi;
+    const bool both = !stat[lo] && !stat[hi];
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c)
+            offb[36 * size_t(i) + 6 * r + c] = both ? h2 * seg_H[144 * size_t(i) + 12 * r + 6 + c] : 0.0;
+}
+
+// out[0] = sum of the moving bodies' inertia energies, out[1] = of their gravity terms
+// (body order); with dx: out[2] = g . dx (in order), out[3] = |dx|_inf
+__global__ void k_ipc_sums(int nb, const int* stat, const double* ebody, const double* gterm, const double* g,
+                           const double* dx, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double si = 0.0, sg = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        if (stat[b]) continue;
+        si += ebody[b];
+        sg += gterm[b];
+    }
+    out[0] = si;
+    out[1] = sg;
+    if (!dx) return;
+    double slope = 0.0, dmax = 0.0;
+    for (int i = 0; i < 6 * nb; ++i) {
+        slope += g[i] * dx[i];
+        dmax = dmax < fabs(dx[i]) ? fabs(dx[i]) : dmax;
+    }
+    out[2] = slope;
+    out[3] = dmax;
+}
+
+__global__ void k_ipc_axpy(int nd, const double* x, double alpha, const double* dx, double* xt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nd) xt[i] = x[i] + alpha * dx[i];
+}
+
+// write-back of the moving bodies (IPCSolver::step): p = x_p, v = (x_p - p) / h,
+// w = x_theta / h, q = apply_theta(q, x_theta)
+__global__ void k_ipc_writeback(Dev d, int b0, int nb, const int* stat, const double* x, double h) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb || stat[b]) return;
+    const int g = b0 + b;
+    const V3 pn = v3(x[6 * b], x[6 * b + 1], x[6 * b + 2]), th = v3(x[6 * b + 3], x[6 * b + 4], x[6 * b + 5]);
+    const V3 p0 = ld3(d.p, g);
+    const V3 v = (pn - p0) / h, w = th / h;
+    d.v[3 * g] = v.x; d.v[3 * g + 1] = v.y; d.v[3 * g + 2] = v.z;
+    d.w[3 * g] = w.x; d.w[3 * g + 1] = w.y; d.w[3 * g + 2] = w.z;
+    d.p[3 * g] = pn.x; d.p[3 * g + 1] = pn.y; d.p[3 * g + 2] = pn.z;
+    const Q q = apply_theta(ld4(d.q, g), th);
+    d.q[4 * g] = q.w; d.q[4 * g + 1] = q.x; d.q[4 * g + 2] = q.y; d.q[4 * g + 3] = q.z;
+}
+
+void launch_ipc_emit(const Work& w, int ncl, const int* stat, int b0, int* cnt, const int* off, int* out, int phase,
+                     hipStream_t st) {
+    if (ncl <= 0) return;
+    if (phase == 0)
+        hipLaunchKernelGGL(k_ipc_emit_count, dim3((ncl + 255) / 256), dim3(256), 0, st, ncl, w.pairs, w.lcnt, stat, b0,
+                           cnt);
+    else
+        hipLaunchKernelGGL(k_ipc_emit_write, dim3(ncl), dim3(256), 0, st, ncl, w.pairs, w.lists, w.lcnt, off, cnt, b0,
+                           out);
+}
+void launch_ipc_start(const Dev& d, int b0, int nb, double h, double* x, double* xh, hipStream_t st) {
+    hipLaunchKernelGGL(k_ipc_start, dim3((nb + 255) / 256), dim3(256), 0, st, d, b0, nb, h, x, xh);
+}
+void launch_ipc_body(const Dev& d, int b0, int nb, const int* stat, const double* x, const double* xh, double h2,
+                     int energy_only, const IpcEvalBufs& B, double* g, double* diag, double* ebody, double* gterm,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_ipc_body, dim3((nb + 63) / 64), dim3(64), 0, st, d, b0, nb, stat, x, xh, h2, energy_only,
+                       B.n_seg, B.seg_bodies, B.seg_g, B.seg_H, g, diag, ebody, gterm);
+}
+void launch_ipc_offblocks(int nd, int nseg_max, const int* stat, double h2, const IpcEvalBufs& B, const double* g,
+                          int* offij, double* offb, double* rhs, hipStream_t st) {
+    const int n = nd > nseg_max ? nd : nseg_max;
+    hipLaunchKernelGGL(k_ipc_offblocks, dim3((n + 255) / 256), dim3(256), 0, st, nd, stat, h2, B.n_seg, B.seg_bodies,
+                       B.seg_H, g, offij, offb, rhs);
+}
+void launch_ipc_sums(int nb, const int* stat, const double* ebody, const double* gterm, const double* g,
+                     const double* dx, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_ipc_sums, dim3(1), dim3(64), 0, st, nb, stat, ebody, gterm, g, dx, out);
+}
+void launch_ipc_axpy(int nd, const double* x, double alpha, const double* dx, double* xt, hipStream_t st) {
+    hipLaunchKernelGGL(k_ipc_axpy, dim3((nd + 255) / 256), dim3(256), 0, st, nd, x, alpha, dx, xt);
+}
+void launch_ipc_writeback(const Dev& d, int b0, int nb, const int* stat, const double* x, double h, hipStream_t st) {
+    hipLaunchKernelGGL(k_ipc_writeback, dim3((nb + 255) / 256), dim3(256), 0, st, d, b0, nb, stat, x, h);
 }
 
 }  // namespace nd

@@ -360,6 +360,38 @@ ND_DU double psd_project(double* H) {
     return lmin;
 }
 
+// gravity (0, -9.81, 0) (IPCSolver.cpp:142)
+ND_DU double gravity(int k) { return k == 1 ? -9.81 : 0.0; }
+
+// A moving body's inertia terms of the ipc-mode step at its Newton variable xb (p,
+// theta), x^ = xh (InertiaForm.cpp:14-53, body-frame inertia I, row-major): energy
+// e = 1/2 m |dp|^2 + (I^T dtheta / 2) . dtheta, gradient g = (m dp, I dtheta), Hessian
+// block H = diag(m, m, m, I); and its gravity energy -m g.x_p added to *grav.
+ND_DU void body_terms(double m, const double* I, const double* xb, const double* xh, double& e, double* g,
+                      double* H, double* grav) {
+    const double dp0 = xb[0] - xh[0], dp1 = xb[1] - xh[1], dp2 = xb[2] - xh[2];
+    const double dt0 = xb[3] - xh[3], dt1 = xb[4] - xh[4], dt2 = xb[5] - xh[5];
+    e = 0.5 * m * ((dp0 * dp0 + dp1 * dp1) + dp2 * dp2);
+    const double r0 = 0.5 * dt0, r1 = 0.5 * dt1, r2 = 0.5 * dt2;
+    const double s0 = r0 * I[0] + r1 * I[3] + r2 * I[6];
+    const double s1 = r0 * I[1] + r1 * I[4] + r2 * I[7];
+    const double s2 = r0 * I[2] + r1 * I[5] + r2 * I[8];
+    e = e + (s0 * dt0 + s1 * dt1 + s2 * dt2);
+    g[0] = m * dp0;
+    g[1] = m * dp1;
+    g[2] = m * dp2;
+    g[3] = (I[0] * dt0 + I[1] * dt1) + I[2] * dt2;
+    g[4] = (I[3] * dt0 + I[4] * dt1) + I[5] * dt2;
+    g[5] = (I[6] * dt0 + I[7] * dt1) + I[8] * dt2;
+    for (int k = 0; k < 36; ++k) H[k] = 0.0;
+    H[0] = m;
+    H[7] = m;
+    H[14] = m;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) H[6 * (3 + r) + 3 + c] = I[3 * r + c];
+    *grav = *grav - m * ((gravity(0) * xb[0] + gravity(1) * xb[1]) + gravity(2) * xb[2]);
+}
+
 // World position of body-frame point r of a body at Newton variable (p, theta) with
 // step-start orientation q (the narrow phase's pose, IPCSolver.cpp:616-626), and its
 // derivative in theta (3 x 3, column k = d x / d theta_k).

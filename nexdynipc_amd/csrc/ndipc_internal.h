@@ -313,8 +313,25 @@ struct IpcEvalBufs {
     double *seg_E, *seg_g, *seg_H; // [np], [12 np], [144 np]
     double* value;                 // [1]
 };
+// value_only: the energies alone (line searches): no gradient or Hessian terms
 void launch_ipc_eval(const Dev& d, const IpcEvalBufs& B, int b0, int nb, int np, double dhat, double kappa, int psd,
+                     hipStream_t st, int value_only = 0);
+// ipc-mode step pieces (ipc.hip): pair records from the top-K lists of the close pairs
+// (phase 0: counts per close pair into cnt; phase 1: records from the offsets off), the
+// start and inertia point, per-body terms and blocks, off-diagonal blocks and rhs, the
+// ordered sums, x + alpha dx, the write-back
+void launch_ipc_emit(const Work& w, int ncl, const int* stat, int b0, int* cnt, const int* off, int* out, int phase,
                      hipStream_t st);
+void launch_ipc_start(const Dev& d, int b0, int nb, double h, double* x, double* xh, hipStream_t st);
+void launch_ipc_body(const Dev& d, int b0, int nb, const int* stat, const double* x, const double* xh, double h2,
+                     int energy_only, const IpcEvalBufs& B, double* g, double* diag, double* ebody, double* gterm,
+                     hipStream_t st);
+void launch_ipc_offblocks(int nd, int nseg_max, const int* stat, double h2, const IpcEvalBufs& B, const double* g,
+                          int* offij, double* offb, double* rhs, hipStream_t st);
+void launch_ipc_sums(int nb, const int* stat, const double* ebody, const double* gterm, const double* g,
+                     const double* dx, double* out, hipStream_t st);
+void launch_ipc_axpy(int nd, const double* x, double alpha, const double* dx, double* xt, hipStream_t st);
+void launch_ipc_writeback(const Dev& d, int b0, int nb, const int* stat, const double* x, double h, hipStream_t st);
 // ipc mode maxStep: conservative advancement of every pair from x along dx; out_t the
 // least advance (+inf: none; must hold +inf before the launch), out_bad the first bad
 // pair (np: none; must hold np before the launch)

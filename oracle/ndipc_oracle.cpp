@@ -627,46 +627,54 @@ struct Solver {
 
     static V3 wv(const BodyContactMesh* m, int vid) { return m->pos + rotate(m->ori, m->mesh->V[vid]); }
 
+    // The six top-K lists of a close pair (ContactAssembler.cpp:249-302): vertices of A
+    // and of B, edges of A and of B, faces of A and of B nearest the other body's box.
+    void pair_lists(const BodyContactMesh* A, const BodyContactMesh* B, vector<int> (&L)[6]) const {
+        const int K = 32;
+        auto vscores = [&](const BodyContactMesh* X, const BodyContactMesh* Y) {
+            vector<double> s(X->mesh->V.size());
+            for (size_t i = 0; i < s.size(); ++i) s[i] = point_aabb_distance(wv(X, int(i)), Y->world_aabb);
+            return s;
+        };
+        auto escores = [&](const BodyContactMesh* X, const BodyContactMesh* Y) {
+            vector<double> s(X->mesh->E.size());
+            for (size_t i = 0; i < s.size(); ++i) {
+                const V3 a = wv(X, X->mesh->E[i][0]), b = wv(X, X->mesh->E[i][1]);
+                s[i] = std::min(point_aabb_distance(a, Y->world_aabb), point_aabb_distance(b, Y->world_aabb));
+            }
+            return s;
+        };
+        auto fscores = [&](const BodyContactMesh* X, const BodyContactMesh* Y) {
+            vector<double> s(X->mesh->F.size());
+            for (size_t i = 0; i < s.size(); ++i) {
+                const auto& f = X->mesh->F[i];
+                const V3 a = wv(X, f[0]), b = wv(X, f[1]), c = wv(X, f[2]);
+                s[i] = std::min({point_aabb_distance(a, Y->world_aabb), point_aabb_distance(b, Y->world_aabb),
+                                 point_aabb_distance(c, Y->world_aabb)});
+            }
+            return s;
+        };
+        L[0] = select_top_k(vscores(A, B), std::min<int>(K, int(A->mesh->V.size())));
+        L[1] = select_top_k(vscores(B, A), std::min<int>(K, int(B->mesh->V.size())));
+        L[2] = select_top_k(escores(A, B), std::min<int>(K, int(A->mesh->E.size())));
+        L[3] = select_top_k(escores(B, A), std::min<int>(K, int(B->mesh->E.size())));
+        L[4] = select_top_k(fscores(A, B), std::min<int>(K, int(A->mesh->F.size())));
+        L[5] = select_top_k(fscores(B, A), std::min<int>(K, int(B->mesh->F.size())));
+    }
+
     // ContactAssembler::buildCandidateSet (ContactAssembler.cpp:214-343)
     void build_candidates(double act, CandSet& out) const {
         out.clear();
         if (bcm.size() < 2) return;
         auto pairs = detect_body_pairs(act);
-        const int K = 32;
         for (auto& p : pairs) {
             const BodyContactMesh* A = mesh_by_world(p.first);
             const BodyContactMesh* B = mesh_by_world(p.second);
             if (!A || !B) continue;
             if (aabb_distance(A->world_aabb, B->world_aabb) > act) continue;
-            auto vscores = [&](const BodyContactMesh* X, const BodyContactMesh* Y) {
-                vector<double> s(X->mesh->V.size());
-                for (size_t i = 0; i < s.size(); ++i) s[i] = point_aabb_distance(wv(X, int(i)), Y->world_aabb);
-                return s;
-            };
-            auto escores = [&](const BodyContactMesh* X, const BodyContactMesh* Y) {
-                vector<double> s(X->mesh->E.size());
-                for (size_t i = 0; i < s.size(); ++i) {
-                    const V3 a = wv(X, X->mesh->E[i][0]), b = wv(X, X->mesh->E[i][1]);
-                    s[i] = std::min(point_aabb_distance(a, Y->world_aabb), point_aabb_distance(b, Y->world_aabb));
-                }
-                return s;
-            };
-            auto fscores = [&](const BodyContactMesh* X, const BodyContactMesh* Y) {
-                vector<double> s(X->mesh->F.size());
-                for (size_t i = 0; i < s.size(); ++i) {
-                    const auto& f = X->mesh->F[i];
-                    const V3 a = wv(X, f[0]), b = wv(X, f[1]), c = wv(X, f[2]);
-                    s[i] = std::min({point_aabb_distance(a, Y->world_aabb), point_aabb_distance(b, Y->world_aabb),
-                                     point_aabb_distance(c, Y->world_aabb)});
-                }
-                return s;
-            };
-            const auto vA = select_top_k(vscores(A, B), std::min<int>(K, int(A->mesh->V.size())));
-            const auto vB = select_top_k(vscores(B, A), std::min<int>(K, int(B->mesh->V.size())));
-            const auto eA = select_top_k(escores(A, B), std::min<int>(K, int(A->mesh->E.size())));
-            const auto eB = select_top_k(escores(B, A), std::min<int>(K, int(B->mesh->E.size())));
-            const auto fA = select_top_k(fscores(A, B), std::min<int>(K, int(A->mesh->F.size())));
-            const auto fB = select_top_k(fscores(B, A), std::min<int>(K, int(B->mesh->F.size())));
+            vector<int> L[6];
+            pair_lists(A, B, L);
+            const auto &vA = L[0], &vB = L[1], &eA = L[2], &eB = L[3], &fA = L[4], &fB = L[5];
             for (int i : vA) for (int j : vB) out.vv.push_back({A->body_id, B->body_id, i, j});
             for (int i : eA) for (int j : vB) out.ev.push_back({A->body_id, B->body_id, A->mesh->E[i][0], A->mesh->E[i][1], j});
             for (int i : eB) for (int j : vA) out.ev.push_back({B->body_id, A->body_id, B->mesh->E[i][0], B->mesh->E[i][1], j});
@@ -674,6 +682,27 @@ struct Solver {
                 out.ee.push_back({A->body_id, B->body_id, A->mesh->E[i][0], A->mesh->E[i][1], B->mesh->E[j][0], B->mesh->E[j][1]});
             for (int i : fA) for (int j : vB) out.fv.push_back({A->body_id, B->body_id, i, j});
             for (int i : fB) for (int j : vA) out.fv.push_back({B->body_id, A->body_id, i, j});
+        }
+    }
+
+    // ipc mode: the point-triangle and edge-edge pair records of every close pair's top-K
+    // lists (the candidates above): faces of A x vertices of B, faces of B x vertices of
+    // A, edges of A x edges of B, in that order; pairs of two static bodies skipped.
+    void ipc_pairs_from_lists(double act, const vector<char>& stat, vector<std::array<int32_t, 5>>& out) const {
+        out.clear();
+        if (bcm.size() < 2) return;
+        for (auto& p : detect_body_pairs(act)) {
+            const BodyContactMesh* A = mesh_by_world(p.first);
+            const BodyContactMesh* B = mesh_by_world(p.second);
+            if (!A || !B) continue;
+            if (aabb_distance(A->world_aabb, B->world_aabb) > act) continue;
+            if (stat[p.first] && stat[p.second]) continue;
+            vector<int> L[6];
+            pair_lists(A, B, L);
+            const int a = p.first, b = p.second;
+            for (int f : L[4]) for (int v : L[1]) out.push_back({0, b, a, v, f});
+            for (int f : L[5]) for (int v : L[0]) out.push_back({0, a, b, v, f});
+            for (int i : L[2]) for (int j : L[3]) out.push_back({1, a, b, i, j});
         }
     }
 
@@ -1308,6 +1337,13 @@ struct Solver {
     // ipc mode maxStep over a pair list; 0, or 1 with fail_index = the first bad pair
     int ipc_max_step(const int32_t* pairs, int np, const vector<double>& x, const vector<double>& dx, double sep,
                      double safety, double& toi, double& alpha, int& fail_index) const;
+    // ipc mode time step (see ndo_ipc_step); 0, or -1 for a scene with joints other than
+    // single-body fixed joints
+    struct IpcStepParams {
+        double dhat, kappa, sep, safety, newton_tol, pcg_tol, inflation;
+        int max_newton, max_ls, pcg_max_it;
+    };
+    int ipc_step(double h, const IpcStepParams& P, int32_t* stats);
 
     double scene_min_distance() const {
         if (prune_scene_min) return scene_min_distance_pruned();
@@ -2149,6 +2185,236 @@ int Solver::ipc_max_step(const int32_t* pairs, int np, const vector<double>& x, 
     return 0;
 }
 
+// Block-Jacobi PCG over 6x6 blocks in the device solver's order of operations: every dot
+// product as 256 strided partial sums (lane t takes i = t, t + 256, ...) combined by a
+// halving tree; each block row's product sums its diagonal block, then the off-diagonal
+// blocks in list order (a block (i, j) also serves row j transposed); block-Jacobi
+// preconditioner by the 6x6 Cholesky inverse.
+static bool inv6_o(const double* A, double* out) {
+    double L[36];
+    for (int k = 0; k < 36; ++k) L[k] = 0.0;
+    for (int j = 0; j < 6; ++j) {
+        double s = A[j * 6 + j];
+        for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
+        if (!(s > 0.0)) return false;
+        const double dj = std::sqrt(s);
+        L[j * 6 + j] = dj;
+        for (int i = j + 1; i < 6; ++i) {
+            double t = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) t -= L[i * 6 + k] * L[j * 6 + k];
+            L[i * 6 + j] = t / dj;
+        }
+    }
+    for (int c = 0; c < 6; ++c) {
+        double y[6];
+        for (int i = 0; i < 6; ++i) {
+            double t = (i == c) ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) t -= L[i * 6 + k] * y[k];
+            y[i] = t / L[i * 6 + i];
+        }
+        for (int i = 5; i >= 0; --i) {
+            double t = y[i];
+            for (int k = i + 1; k < 6; ++k) t -= L[k * 6 + i] * out[k * 6 + c];
+            out[i * 6 + c] = t / L[i * 6 + i];
+        }
+    }
+    return true;
+}
+
+static int pcg_ordered(int nb, const vector<double>& diag, const vector<int>& offij, const vector<double>& offb,
+                       const vector<double>& b, double tol, int max_it, vector<double>& x) {
+    const int nd = 6 * nb, noff = int(offij.size() / 2), T = 256;
+    auto dot = [&](const vector<double>& a, const vector<double>& c) {
+        double part[256];
+        for (int t = 0; t < T; ++t) {
+            double s = 0.0;
+            for (int i = t; i < nd; i += T) s += a[i] * c[i];
+            part[t] = s;
+        }
+        for (int h = T / 2; h > 0; h >>= 1)
+            for (int t = 0; t < h; ++t) part[t] = part[t] + part[t + h];
+        return part[0];
+    };
+    vector<double> Dinv(36 * size_t(nb), 0.0), r(nd), z(nd), p(nd), Ap(nd);
+    for (int i = 0; i < nb; ++i) inv6_o(&diag[36 * size_t(i)], &Dinv[36 * size_t(i)]);
+    auto mv = [&](const vector<double>& v, vector<double>& y) {
+        for (int i = 0; i < nb; ++i)
+            for (int a = 0; a < 6; ++a) {
+                double s = 0.0;
+                for (int c = 0; c < 6; ++c) s += diag[36 * size_t(i) + a * 6 + c] * v[6 * i + c];
+                for (int o = 0; o < noff; ++o) {
+                    const int bi = offij[2 * o], bj = offij[2 * o + 1];
+                    const double* B = &offb[36 * size_t(o)];
+                    if (bi == i) for (int c = 0; c < 6; ++c) s += B[a * 6 + c] * v[6 * bj + c];
+                    if (bj == i) for (int c = 0; c < 6; ++c) s += B[c * 6 + a] * v[6 * bi + c];
+                }
+                y[6 * i + a] = s;
+            }
+    };
+    auto prec = [&](const vector<double>& v, vector<double>& y) {
+        for (int i = 0; i < nb; ++i)
+            for (int a = 0; a < 6; ++a) {
+                double s = 0.0;
+                for (int c = 0; c < 6; ++c) s += Dinv[36 * size_t(i) + a * 6 + c] * v[6 * i + c];
+                y[6 * i + a] = s;
+            }
+    };
+    x.assign(nd, 0.0);
+    r = b;
+    prec(r, z);
+    p = z;
+    double rz = dot(r, z);
+    const double bn = std::sqrt(dot(b, b));
+    double rn = bn;
+    int it = 0;
+    for (; it < max_it && rn > tol * bn; ++it) {
+        mv(p, Ap);
+        const double pAp = dot(p, Ap);
+        const double alpha = rz / pAp;
+        for (int i = 0; i < nd; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * Ap[i];
+        }
+        rn = std::sqrt(dot(r, r));
+        prec(r, z);
+        const double rz2 = dot(r, z);
+        const double beta = rz2 / rz;
+        rz = rz2;
+        for (int i = 0; i < nd; ++i) p[i] = z[i] + beta * p[i];
+    }
+    return it;
+}
+
+// ipc mode time step: IPCSolver::step's implicit-Euler Newton solve (IPCSolver.cpp:137-262,
+// NewtonSolver.cpp:10-95) with the ipc barrier in place of the centre-distance barrier and
+// maxStep wired into the line search, for scenes whose only joints pin single bodies
+// (those bodies are static: their DOFs are held). Objective, as the reference's:
+// sum over moving bodies of 1/2 m |x_p - x^_p|^2 + 1/2 dtheta^T I dtheta (InertiaForm,
+// body-frame I), + h^2 (-sum m g.x_p (GravityForm) + E_ipc(x)), x^ = [p + h v, h w].
+// Pairs: the top-K lists of the close pairs at inflation P.inflation (ipc_pairs_from_lists),
+// fixed for the step. Newton: H dx = -g by block-Jacobi PCG (pcg_ordered; diagonal blocks:
+// inertia then h^2 x each segment's block in segment order; one off-diagonal block per
+// segment, zero when a body is static), stop when |dx|_inf <= newton_tol or the direction
+// does not descend; alpha starts at min(1, maxStep) (ipc_max_step, sep, safety) and halves
+// until Armijo (c = 1e-4) holds. Sums run in body, segment and pair order. Write-back as
+// IPCSolver::step's: p = x_p, v = (x_p - p) / h, w = x_theta / h, q = apply_theta(q, x_theta).
+int Solver::ipc_step(double h, const IpcStepParams& P, int32_t* stats) {
+    const int N = n(), nd = 6 * N;
+    vector<char> stat(N, 0);
+    for (auto& j : joints) {
+        if (j.type != 0 || j.b >= 0 || j.a < 0) return -1;
+        stat[j.a] = 1;
+    }
+    assembler_build();
+    vector<std::array<int32_t, 5>> pr;
+    ipc_pairs_from_lists(P.inflation, stat, pr);
+    const int np = int(pr.size());
+    vector<int32_t> flat(5 * size_t(np));
+    for (int i = 0; i < np; ++i)
+        for (int k = 0; k < 5; ++k) flat[5 * size_t(i) + k] = pr[i][k];
+    const double h2 = h * h;
+    vector<double> x(nd, 0.0), xh(nd, 0.0);
+    for (int b = 0; b < N; ++b) {
+        const Body& B = bodies[b];
+        x[6 * b] = B.p.x; x[6 * b + 1] = B.p.y; x[6 * b + 2] = B.p.z;
+        xh[6 * b] = B.p.x + h * B.v.x; xh[6 * b + 1] = B.p.y + h * B.v.y; xh[6 * b + 2] = B.p.z + h * B.v.z;
+        xh[6 * b + 3] = h * B.w.x; xh[6 * b + 4] = h * B.w.y; xh[6 * b + 5] = h * B.w.z;
+    }
+    auto energy = [&](const vector<double>& xx, double* e_ipc_out, vector<IpcSeg>* segs_out) {
+        double si = 0.0, sg = 0.0;
+        for (int b = 0; b < N; ++b) {
+            if (stat[b]) continue;
+            double e, g6[6], H36[36];
+            nd::ipc::body_terms(bodies[b].mass, &bodies[b].I.m[0][0], &xx[6 * b], &xh[6 * b], e, g6, H36, &sg);
+            si += e;
+        }
+        double ev = 0.0;
+        vector<IpcSeg> segs;
+        ipc_barrier_eval(flat.data(), np, xx, P.dhat, P.kappa, true, ev, segs);
+        if (e_ipc_out) *e_ipc_out = ev;
+        if (segs_out) *segs_out = std::move(segs);
+        return (si + h2 * sg) + h2 * ev;
+    };
+    int newton = 0, trials = 0, pcg_total = 0;
+    for (int it = 0; it < P.max_newton; ++it) {
+        vector<IpcSeg> segs;
+        const double E0 = energy(x, nullptr, &segs);
+        // gradient and blocks
+        vector<double> g(nd, 0.0), diag(36 * size_t(N), 0.0);
+        for (int b = 0; b < N; ++b) {
+            double* D = &diag[36 * size_t(b)];
+            if (stat[b]) {
+                for (int k = 0; k < 6; ++k) D[7 * k] = 1.0;
+                continue;
+            }
+            double e, sgd = 0.0;
+            nd::ipc::body_terms(bodies[b].mass, &bodies[b].I.m[0][0], &x[6 * b], &xh[6 * b], e, &g[6 * b], D, &sgd);
+            for (int k = 0; k < 3; ++k) g[6 * b + k] += h2 * (-bodies[b].mass * nd::ipc::gravity(k));
+            for (const IpcSeg& sgm : segs) {
+                const int side = sgm.lo == b ? 0 : (sgm.hi == b ? 1 : -1);
+                if (side < 0) continue;
+                for (int r = 0; r < 6; ++r) {
+                    g[6 * b + r] += h2 * sgm.g[6 * side + r];
+                    for (int c = 0; c < 6; ++c) D[6 * r + c] += h2 * sgm.H[12 * (6 * side + r) + 6 * side + c];
+                }
+            }
+        }
+        vector<int> offij;
+        vector<double> offb;
+        for (const IpcSeg& sgm : segs) {
+            offij.push_back(sgm.lo);
+            offij.push_back(sgm.hi);
+            const bool both = !stat[sgm.lo] && !stat[sgm.hi];
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) offb.push_back(both ? h2 * sgm.H[12 * r + 6 + c] : 0.0);
+        }
+        vector<double> rhs(nd), dx;
+        for (int i = 0; i < nd; ++i) rhs[i] = -g[i];
+        pcg_total += pcg_ordered(N, diag, offij, offb, rhs, P.pcg_tol, P.pcg_max_it, dx);
+        double dmax = 0.0;
+        for (int i = 0; i < nd; ++i) dmax = std::max(dmax, std::abs(dx[i]));
+        if (!(dmax > P.newton_tol)) break;
+        double slope = 0.0;
+        for (int i = 0; i < nd; ++i) slope += g[i] * dx[i];
+        if (!(slope < 0.0)) break;
+        double alpha = 1.0, toi = 0.0;
+        int fi = -1;
+        ipc_max_step(flat.data(), np, x, dx, P.sep, P.safety, toi, alpha, fi);
+        alpha = std::min(1.0, alpha);
+        bool ok = false;
+        vector<double> xt(nd);
+        for (int k = 0; k < P.max_ls; ++k) {
+            ++trials;
+            for (int i = 0; i < nd; ++i) xt[i] = x[i] + alpha * dx[i];
+            const double Et = energy(xt, nullptr, nullptr);
+            if (Et <= E0 + 1e-4 * alpha * slope) {
+                ok = true;
+                break;
+            }
+            alpha *= 0.5;
+        }
+        if (!ok) break;
+        x = xt;
+        ++newton;
+    }
+    for (int b = 0; b < N; ++b) {
+        if (stat[b]) continue;
+        Body& B = bodies[b];
+        const V3 pn = seg3(x, 6 * b), th = seg3(x, 6 * b + 3);
+        B.v = (pn - B.p) / h;
+        B.w = th / h;
+        B.p = pn;
+        B.q = apply_theta(B.q, th);
+    }
+    if (stats) {
+        stats[0] = newton;
+        stats[1] = trials;
+        stats[2] = np;
+        stats[3] = pcg_total;
+    }
+    return 0;
+}
+
 // CCDSystem::computeEarliestTOI (CCD.h:22-82)
 double Solver::ccd_earliest_toi(double dt) const {
     if (bodies.empty() || cands.fv.empty()) return 1.0;
@@ -2801,6 +3067,11 @@ int32_t ndo_ipc_max_step(void* h, int32_t np, const int32_t* pairs5, const doubl
     const int r = s.ipc_max_step(pairs5, np, xv, dv, sep, safety, *toi, *alpha, fi);
     if (fail_index) *fail_index = fi;
     return -r;
+}
+
+int32_t ndo_ipc_step(void* h, double dt, const double* prm7, const int32_t* prm3, int32_t* stats4) {
+    Solver::IpcStepParams P{prm7[0], prm7[1], prm7[2], prm7[3], prm7[4], prm7[5], prm7[6], prm3[0], prm3[1], prm3[2]};
+    return S(h).ipc_step(dt, P, stats4);
 }
 
 int32_t ndo_ipc_barrier_eval(void* h, int32_t np, const int32_t* pairs5, const double* x, double dhat, double kappa,

@@ -175,6 +175,11 @@ int32_t ndo_vertex_jacobian(const double* xb6, const double* q4, const double* r
  * each pair's distance): 0, or -1 with fail_index = the first bad pair */
 int32_t ndo_ipc_max_step(void* h, int32_t np, const int32_t* pairs5, const double* x, const double* dx, double sep,
                          double safety, double* alpha, double* toi, int32_t* fail_index);
+/* ipc mode time step of the scene (single-body fixed joints only): prm7 = dhat, kappa,
+ * sep, safety, newton_tol, pcg_tol, inflation; prm3 = max_newton, max_ls, pcg_max_it;
+ * stats4 = Newton iterations, line-search trials, pairs, PCG iterations. -1: joints other
+ * than single-body fixed joints. */
+int32_t ndo_ipc_step(void* h, double dt, const double* prm7, const int32_t* prm3, int32_t* stats4);
 int32_t ndo_ipc_barrier_eval(void* h, int32_t np, const int32_t* pairs5, const double* x, double dhat, double kappa,
                              int32_t psd, double* value, int32_t cap, int32_t* n_seg, int32_t* seg_bodies,
                              double* seg_g, double* seg_H);

@@ -127,7 +127,8 @@ def lib():
                 ("ndo_ipc_barrier_eval", [C.c_void_p, C.c_int32, i32p, f64p, C.c_double, C.c_double, C.c_int32,
                                           f64p, C.c_int32, i32p, i32p, f64p, f64p]),
                 ("ndo_ipc_max_step", [C.c_void_p, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_double, f64p, f64p,
-                                      i32p])):
+                                      i32p]),
+                ("ndo_ipc_step", [C.c_void_p, C.c_double, f64p, i32p, i32p])):
             getattr(L, name).restype = C.c_int32
             getattr(L, name).argtypes = args
         L.ndo_psd_project12.restype = C.c_double
@@ -407,6 +408,17 @@ class OracleScene:
         if rc != 0:
             raise ValueError(f"ndo_ipc_max_step: bad pair {fi.value}")
         return a.value, t.value
+
+    def ipc_step(self, dt, dhat=1e-3, kappa=1e4, sep=0.1, safety=0.8, newton_tol=1e-9, pcg_tol=1e-12,
+                 inflation=None, max_newton=50, max_ls=30, pcg_max_it=2000):
+        """ipc mode time step; returns (Newton iterations, line-search trials, pairs, PCG iterations)."""
+        f = np.array([dhat, kappa, sep, safety, newton_tol, pcg_tol, dhat if inflation is None else inflation])
+        i = np.array([max_newton, max_ls, pcg_max_it], dtype=np.int32)
+        st = np.zeros(4, dtype=np.int32)
+        rc = self.L.ndo_ipc_step(self.h, dt, ptr(f, f64p), ptr(i, i32p), ptr(st, i32p))
+        if rc != 0:
+            raise ValueError("ndo_ipc_step: the scene has joints other than single-body fixed joints")
+        return tuple(int(v) for v in st)
 
     def ccd_earliest_toi(self, dt):
         o = C.c_double()

@@ -183,3 +183,24 @@ def test_ipc_max_step_matches_oracle(engine_factory):
     with pytest.raises(RuntimeError, match="ipc pair 3"):
         e.ipc_max_step(0, bad, x, dx)
     assert e.ipc_max_step(0, pairs[:0], x, dx) == (1.0, np.inf)
+
+
+def test_ipc_step_matches_oracle(engine_factory):
+    """The ipc-mode time step on the device against the oracle's: three tilted cubes
+    dropped in a stack onto a static box, 40 steps, every state bitwise, the same Newton
+    iterations, line-search trials, pair records and PCG iterations."""
+    from test_oracle_ipc import _drop_scene
+
+    sc = _drop_scene(n_cubes=3, seed=4)
+    e = engine_factory(sc)
+    o = O.OracleScene(sc)
+    kw = dict(dhat=1e-2, kappa=1e5, inflation=0.05)
+    for k in range(40):
+        so = o.ipc_step(1e-2, **kw)
+        sg = e.ipc_step(0, 1e-2, **kw)
+        assert so == sg, (k, so, sg)
+        for a, b in zip(o.get_state()[:4], e.get_state()[:4]):
+            assert np.array_equal(a, b), k
+    with pytest.raises(RuntimeError, match="single-body fixed"):
+        e2 = engine_factory(scenes.hinge_chain_scene(n_links=3))
+        e2.ipc_step(0, 1e-2)

@@ -612,3 +612,50 @@ def _level(sc):
     s2 = copy.deepcopy(sc)
     s2.q[1:] = [1.0, 0.0, 0.0, 0.0]
     return s2
+
+
+# ------------------------------------------------------------------ ipc-mode time step
+def _drop_scene(n_cubes=1, seed=0):
+    rng = np.random.default_rng(seed)
+    body = scenes._box_body
+    bodies = [body("g", 4, 1, 4, [0, -0.5, 0], static=True)]
+    for k in range(n_cubes):
+        bodies.append(body("c", 1, 1, 1, [0.1 + 0.05 * k, 0.6 + 1.05 * k, -0.05],
+                           q=scenes.quat_from_axis_angle(rng.normal(size=3), 0.1 + 0.05 * k)))
+    return scenes.pack([scenes.SceneSpec(bodies, dt=1e-2, dhat=1e-3, mu=0.0)])
+
+
+def _lowest_vertex(sc, p, q, b):
+    V = sc.verts.reshape(-1, 3)[sc.vert_off[b]:sc.vert_off[b] + sc.n_verts[b]]
+    return min((p[b] + _qmat(q[b]) @ v)[1] for v in V)
+
+
+def test_ipc_step_cube_lands_and_rests_without_penetration():
+    """A tilted unit cube dropped 0.1 m onto a static box (its top face at y = 0): with the
+    ipc mode's primitive-distance barrier (dhat 1 cm) it lands, never crosses the face,
+    and comes to rest within dhat of it (there is no friction, so it may still slide and
+    spin about the vertical);
+    the reference's centre-distance barrier (ref mode, the same scene) lets it sink into
+    the box."""
+    sc = _drop_scene()
+    o = O.OracleScene(sc)
+    lows = []
+    for k in range(80):
+        st = o.ipc_step(1e-2, dhat=1e-2, kappa=1e5, inflation=0.05)
+        assert st[2] > 0 and st[0] <= 50
+        p, q, v, w = o.get_state()[:4]
+        lows.append(_lowest_vertex(sc, p, q, 1))
+    assert min(lows) > 0.0
+    # at rest vertically and not tipping (no friction: it may still slide and spin about y)
+    assert 0.0 < lows[-1] < 1e-2 and abs(v[1][1]) < 1e-4 and max(abs(w[1][0]), abs(w[1][2])) < 1e-3
+    ref = O.OracleScene(sc)
+    for k in range(80):
+        ref.step(1e-2)
+    p, q = ref.get_state()[:2]
+    assert _lowest_vertex(sc, p, q, 1) < -0.1
+
+
+def test_ipc_step_rejects_other_joints():
+    sc = scenes.hinge_chain_scene(n_links=3)
+    with pytest.raises(ValueError, match="single-body fixed"):
+        O.OracleScene(sc).ipc_step(1e-2)
