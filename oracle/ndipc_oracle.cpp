@@ -3069,6 +3069,15 @@ int32_t ndo_ipc_max_step(void* h, int32_t np, const int32_t* pairs5, const doubl
     return -r;
 }
 
+int32_t ndo_pcg_ordered(int32_t nb, const double* diag, int32_t noff, const int32_t* offij, const double* offb,
+                        const double* b, double tol, int32_t max_it, double* x) {
+    std::vector<double> dg(diag, diag + 36 * size_t(nb)), ob(offb, offb + 36 * size_t(noff)), bb(b, b + 6 * size_t(nb)), xx;
+    std::vector<int> oi(offij, offij + 2 * size_t(noff));
+    const int it = pcg_ordered(nb, dg, oi, ob, bb, tol, max_it, xx);
+    std::copy(xx.begin(), xx.end(), x);
+    return it;
+}
+
 int32_t ndo_ipc_step(void* h, double dt, const double* prm7, const int32_t* prm3, int32_t* stats4) {
     Solver::IpcStepParams P{prm7[0], prm7[1], prm7[2], prm7[3], prm7[4], prm7[5], prm7[6], prm3[0], prm3[1], prm3[2]};
     return S(h).ipc_step(dt, P, stats4);

@@ -180,6 +180,9 @@ int32_t ndo_ipc_max_step(void* h, int32_t np, const int32_t* pairs5, const doubl
  * stats4 = Newton iterations, line-search trials, pairs, PCG iterations. -1: joints other
  * than single-body fixed joints. */
 int32_t ndo_ipc_step(void* h, double dt, const double* prm7, const int32_t* prm3, int32_t* stats4);
+/* the ipc step's block-Jacobi PCG (the device solver's order of operations): iterations */
+int32_t ndo_pcg_ordered(int32_t nb, const double* diag, int32_t noff, const int32_t* offij, const double* offb,
+                        const double* b, double tol, int32_t max_it, double* x);
 int32_t ndo_ipc_barrier_eval(void* h, int32_t np, const int32_t* pairs5, const double* x, double dhat, double kappa,
                              int32_t psd, double* value, int32_t cap, int32_t* n_seg, int32_t* seg_bodies,
                              double* seg_g, double* seg_H);

@@ -128,7 +128,8 @@ def lib():
                                           f64p, C.c_int32, i32p, i32p, f64p, f64p]),
                 ("ndo_ipc_max_step", [C.c_void_p, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_double, f64p, f64p,
                                       i32p]),
-                ("ndo_ipc_step", [C.c_void_p, C.c_double, f64p, i32p, i32p])):
+                ("ndo_ipc_step", [C.c_void_p, C.c_double, f64p, i32p, i32p]),
+                ("ndo_pcg_ordered", [C.c_int32, f64p, C.c_int32, i32p, f64p, f64p, C.c_double, C.c_int32, f64p])):
             getattr(L, name).restype = C.c_int32
             getattr(L, name).argtypes = args
         L.ndo_psd_project12.restype = C.c_double
@@ -592,3 +593,16 @@ def vertex_jacobian(xb6, q4, r3):
     J = np.zeros(9)
     L.ndo_vertex_jacobian(ptr(a, f64p), ptr(b, f64p), ptr(c, f64p), ptr(xw, f64p), ptr(J, f64p))
     return xw, J.reshape(3, 3)
+
+
+def pcg_ordered(diag, offij, offb, b, tol=1e-14, max_it=1000):
+    L = lib()
+    nb = len(diag) // 36
+    d = np.ascontiguousarray(diag, dtype=np.float64)
+    oi = np.ascontiguousarray(offij, dtype=np.int32).reshape(-1)
+    ob = np.ascontiguousarray(offb, dtype=np.float64).reshape(-1)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(6 * nb)
+    it = L.ndo_pcg_ordered(nb, ptr(d, f64p), len(oi) // 2, ptr(oi, i32p), ptr(ob, f64p), ptr(bb, f64p), tol, max_it,
+                           ptr(x, f64p))
+    return x, it

@@ -659,3 +659,29 @@ def test_ipc_step_rejects_other_joints():
     sc = scenes.hinge_chain_scene(n_links=3)
     with pytest.raises(ValueError, match="single-body fixed"):
         O.OracleScene(sc).ipc_step(1e-2)
+
+
+def test_step_pcg_solves_block_systems():
+    """The ipc step's PCG (the device solver's order of operations) against numpy's
+    dense solve on random SPD block systems, off-diagonal blocks listed once (some pairs
+    twice, as repeated body-pair segments are), with relative tolerance 1e-14."""
+    rng = np.random.default_rng(31)
+    for nb in (1, 3, 9, 40):
+        pairs = [(i, j) for i in range(nb) for j in range(i + 1, nb) if rng.random() < 0.3]
+        pairs += pairs[:2]
+        offij = np.array(pairs, dtype=np.int32).reshape(-1, 2)
+        offb = rng.normal(size=(len(pairs), 6, 6)) * 0.3
+        A = np.zeros((6 * nb, 6 * nb))
+        for (i, j), B in zip(pairs, offb):
+            A[6 * i:6 * i + 6, 6 * j:6 * j + 6] += B
+            A[6 * j:6 * j + 6, 6 * i:6 * i + 6] += B.T
+        diag = np.zeros((nb, 6, 6))
+        for i in range(nb):
+            M = rng.normal(size=(6, 6))
+            diag[i] = M @ M.T + 6.0 * np.eye(6) + np.abs(A[6 * i:6 * i + 6]).sum() * np.eye(6)
+            A[6 * i:6 * i + 6, 6 * i:6 * i + 6] += diag[i]
+        b = rng.normal(size=6 * nb)
+        x, it = O.pcg_ordered(diag.reshape(-1), offij, offb, b, tol=1e-14, max_it=2000)
+        ref = np.linalg.solve(A, b)
+        assert np.abs(x - ref).max() <= 1e-11 * np.abs(ref).max(), nb
+        assert 0 < it < 2000
