@@ -291,6 +291,49 @@ def run_ipc(Engine, torch, steps=200, dhat=1e-2, kappa=1e5, max_pairs=1 << 20, r
                                 "traffic": load_traffic(["ipc_filter", "ipc_seg_count", "ipc_seg_write"], "ipc")}}
 
 
+def run_ipc_step(Engine, torch, steps=100, warmup=5, cpu_steps=2):
+    """The ipc mode's time step on a grid of 32 cubes dropped onto a ground box
+    (scenes.ipc_drop_scene: 4 x 4 x 2 cubes of 0.2 m), dhat 1 cm: steps/s on the device
+    (inputs resident; each step's Newton loop driven from the host), and the oracle's
+    same steps on one host core."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from nexdynipc_amd import scenes
+
+    sc = scenes.ipc_drop_scene()
+    kw = dict(dhat=1e-2, kappa=1e5, inflation=0.05)
+    e = Engine(0)
+    e.upload(sc)
+    for _ in range(warmup):
+        e.ipc_step(0, 1e-2, **kw)
+    torch.cuda.synchronize()
+    newton = trials = pcg = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = e.ipc_step(0, 1e-2, **kw)
+        newton += st[0]
+        trials += st[1]
+        pcg += st[3]
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    pairs = st[2]
+    e.close()
+    import _oracle as O
+
+    o = O.OracleScene(sc)
+    t0 = time.perf_counter()
+    for _ in range(cpu_steps):
+        o.ipc_step(1e-2, **kw)
+    Tc = time.perf_counter() - t0
+    return {"workload": f"ipc step: {sc.n_bodies - 1} cubes (4 x 4 x 2, 0.2 m) dropped onto a ground box, dt 1e-2, "
+                        "dhat 1 cm, kappa 1e5, pairs from the top-K lists at inflation 5 cm",
+            "steps": steps, "steps_per_s": steps / T, "ms_per_step": 1e3 * T / steps,
+            "newton_iters_per_step": newton / steps, "line_search_trials_per_step": trials / steps,
+            "pcg_iters_per_step": pcg / steps, "pair_records": pairs,
+            "cpu_steps_per_s": cpu_steps / Tc, "cpu_cores": 1, "cpu_kind": "port",
+            "cpu_sample": f"the first {cpu_steps} steps of the same scene by the C++ oracle on 1 core (the device "
+                          f"figure is steps {warmup + 1}-{warmup + steps})"}
+
+
 def roofline_for(name, r, sc):
     """Roofline of kernel `name` over the timed region: algorithmic work / total time."""
     avg_ms, cnt = r["kern"][name]
@@ -367,7 +410,7 @@ def main():
     from nexdynipc_amd._lib import Engine
 
     if args.ipc_only:
-        print(json.dumps({"ipc": run_ipc(Engine, torch)}))
+        print(json.dumps({"ipc": run_ipc(Engine, torch), "ipc_step": run_ipc_step(Engine, torch)}))
         return
 
     batch = None
@@ -500,6 +543,7 @@ def main():
         "cpu_baseline": cpu,
         "c1": c1,
         "ipc": None if args.no_ipc else run_ipc(Engine, torch),
+        "ipc_step": None if args.no_ipc else run_ipc_step(Engine, torch),
     }
     print(json.dumps(out))
 

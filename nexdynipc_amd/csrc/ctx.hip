@@ -30,6 +30,12 @@ struct TimingRec {
     hipEvent_t open = nullptr;
 };
 
+// device buffer kept across calls, grown on demand (the ipc-mode step's work space)
+struct GrowBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
 struct ndipc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -73,7 +79,24 @@ struct ndipc_ctx {
     hipGraphExec_t pre_exec = nullptr, round_exec = nullptr;
     unsigned int* hflag = nullptr;    // pinned host memory: [round number, all scenes done]
     unsigned int rounds_issued = 0;   // rounds with a flag issued since upload
+    GrowBuf ipc_buf[8];               // ndipc_ipc_step's work space
 };
+
+namespace {
+template <typename T>
+hipError_t grow(GrowBuf& b, size_t bytes, T** out) {
+    if (bytes > b.bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        const hipError_t e = hipMalloc(&b.p, bytes);
+        if (e != hipSuccess) return e;
+        b.bytes = bytes;
+    }
+    *out = static_cast<T*>(b.p);
+    return hipSuccess;
+}
+}  // namespace
 
 namespace {
 ndipc_ctx* g_timing_ctx = nullptr;  // context whose stream is being timed (one at a time)
@@ -332,6 +355,8 @@ int32_t ndipc_destroy(ndipc_ctx* ctx) {
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = -1;
     if (g_timing_ctx == ctx) g_timing_ctx = nullptr;
     free_all(ctx);
+    for (GrowBuf& b : ctx->ipc_buf)
+        if (b.p && hipFree(b.p) != hipSuccess) rc = -1;
     if (hipStreamSynchronize(ctx->hstream) != hipSuccess) rc = -1;
     if (hipEventDestroy(ctx->hfork) != hipSuccess) rc = -1;
     if (hipEventDestroy(ctx->hjoin) != hipSuccess) rc = -1;
@@ -1385,9 +1410,9 @@ int32_t ndipc_ipc_step(ndipc_ctx* ctx, int32_t scene, double dt, const ndipc_ipc
             stat[ja[j] - b0] = 1;
         }
     }
-    ScratchBufs sb;  // freed on every return
+    // work space kept in the context across steps (grown on demand)
     int* dstat = nullptr;
-    HIPCHK(sb.alloc(&dstat, 4 * size_t(nb)));
+    HIPCHK(grow(ctx->ipc_buf[0], 4 * size_t(nb), &dstat));
     HIPCHK(hipMemcpy(dstat, stat.data(), 4 * size_t(nb), hipMemcpyHostToDevice));
     // pair records of the close pairs' top-K lists
     prep_geometry(ctx);
@@ -1397,8 +1422,8 @@ int32_t ndipc_ipc_step(ndipc_ctx* ctx, int32_t scene, double dt, const ndipc_ipc
     HIPCHK(hipMemcpyAsync(&ncl, ctx->w.n_pairs_total, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     int *dcnt = nullptr, *doff = nullptr;
-    HIPCHK(sb.alloc(&dcnt, 4 * size_t(std::max(ncl, 1))));
-    HIPCHK(sb.alloc(&doff, 4 * size_t(std::max(ncl, 1))));
+    HIPCHK(grow(ctx->ipc_buf[1], 8 * size_t(std::max(ncl, 1)), &dcnt));
+    doff = dcnt + std::max(ncl, 1);
     nd::launch_ipc_emit(ctx->w, ncl, dstat, b0, dcnt, nullptr, nullptr, 0, ctx->stream);
     std::vector<int> cnt(std::max(ncl, 1), 0), off(std::max(ncl, 1), 0);
     if (ncl) HIPCHK(hipMemcpy(cnt.data(), dcnt, 4 * size_t(ncl), hipMemcpyDeviceToHost));
@@ -1414,13 +1439,16 @@ int32_t ndipc_ipc_step(ndipc_ctx* ctx, int32_t scene, double dt, const ndipc_ipc
     int *ip = nullptr, *offij = nullptr;
     unsigned long long* key = nullptr;
     double *dp = nullptr, *v = nullptr;
-    HIPCHK(sb.alloc(&ip, 4 * (5 * n + n + 1 + n + 1 + 1 + 1 + 2 * n + 2 * nchunk + n + n)));
-    HIPCHK(sb.alloc(&key, 8 * n));
-    HIPCHK(sb.alloc(&dp, 8 * (n + 12 * n + 144 * n + n + 12 * n + 144 * n + 1 + n)));
-    HIPCHK(sb.alloc(&offij, 4 * 2 * nsg));
+    HIPCHK(grow(ctx->ipc_buf[2], 4 * (5 * n + n + 1 + n + 1 + 1 + 1 + 2 * n + 2 * nchunk + n + n), &ip));
+    HIPCHK(grow(ctx->ipc_buf[3], 8 * n, &key));
+    HIPCHK(grow(ctx->ipc_buf[4], 8 * (n + 12 * n + 144 * n + n + 12 * n + 144 * n + 1 + n), &dp));
+    // off-diagonal block pairs (2 per segment), then the PCG's row lists (nb + 1 + 2 per segment)
+    HIPCHK(grow(ctx->ipc_buf[5], 4 * (2 * nsg + size_t(nb) + 1 + 2 * nsg), &offij));
+    int* pcgi = offij + 2 * nsg;
     // x, x^, xt, dx, g, rhs (nd each), diag (36 nb), offb (36 per segment), ebody, gterm
     // (nb each), sums (4), PCG work (10 nd), the advance's out (1)
-    HIPCHK(sb.alloc(&v, 8 * (6 * size_t(nd) + 36 * size_t(nb) + 36 * nsg + 2 * size_t(nb) + 4 + 10 * size_t(nd) + 1)));
+    HIPCHK(grow(ctx->ipc_buf[6], 8 * (6 * size_t(nd) + 36 * size_t(nb) + 36 * nsg + 2 * size_t(nb) + 4 + 10 * size_t(nd) + 1),
+                &v));
     int* iw = ip;
     int* dpairs = iw;
     iw += 5 * n;
@@ -1494,8 +1522,8 @@ int32_t ndipc_ipc_step(ndipc_ctx* ctx, int32_t scene, double dt, const ndipc_ipc
         int nseg = 0;
         HIPCHK(hipMemcpy(&nseg, B.n_seg, 4, hipMemcpyDeviceToHost));
         nd::launch_ipc_offblocks(nd, nseg, dstat, h2, B, g, offij, offb, rhs, ctx->stream);
-        nd::launch_pcg(nb, diag, nseg, offij, offb, rhs, P.pcg_tol, P.pcg_max_it, dx, piters, sums + 3, pcgw,
-                       ctx->stream);
+        nd::launch_pcg_rows(nb, diag, nseg, offij, offb, rhs, P.pcg_tol, P.pcg_max_it, dx, piters, sums + 3, pcgw,
+                            pcgi, ctx->stream);
         int pit = 0;
         HIPCHK(hipMemcpyAsync(&pit, piters, 4, hipMemcpyDeviceToHost, ctx->stream));
         nd::launch_ipc_sums(nb, dstat, ebody, gterm, g, dx, sums, ctx->stream);

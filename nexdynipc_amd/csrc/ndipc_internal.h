@@ -337,6 +337,10 @@ void launch_ipc_writeback(const Dev& d, int b0, int nb, const int* stat, const d
 // pair (np: none; must hold np before the launch)
 void launch_ipc_advance(const Dev& d, int b0, int nb, int np, const int* pairs, const double* x, const double* dx,
                         double sep, double* toi, int* st, double* out_t, int* out_bad, hipStream_t stream);
+// launch_pcg's solve with per-row block lists (iscratch: nb + 1 + 2 noff ints)
+void launch_pcg_rows(int nb, const double* diag, int noff, const int* offij, const double* offb, const double* b,
+                     double tol, int max_it, double* x, int* iters, double* rel, double* scratch, int* iscratch,
+                     hipStream_t st);
 void launch_pcg(int nb, const double* diag, int noff, const int* offij, const double* offb, const double* b,
                 double tol, int max_it, double* x, int* iters, double* rel, double* scratch, hipStream_t st);
 

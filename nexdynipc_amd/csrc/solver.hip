@@ -2787,6 +2787,107 @@ __global__ void __launch_bounds__(kBlk) k_pcg(int nb, const double* diag, int no
     }
 }
 
+// The same solve (the same values in the same order of operations: each block row's
+// product sums its diagonal block, then its off-diagonal blocks in list order) with each
+// row's blocks indexed once per solve (row_off, ent: (block << 1) | transposed) and a
+// lane per row entry of the products, for systems with many off-diagonal blocks.
+__global__ void __launch_bounds__(kBlk) k_pcg_rows(int nb, const double* diag, int noff, const int* offij,
+                                                   const double* offb, const double* b, double tol, int max_it,
+                                                   double* x, int* iters, double* rel, double* scr, int* iscr) {
+    __shared__ double red[kBlk];
+    const int nd = 6 * nb;
+    double* r = scr;
+    double* z = scr + nd;
+    double* p = scr + 2 * nd;
+    double* Ap = scr + 3 * nd;
+    double* Dinv = scr + 4 * nd;
+    int* row_off = iscr;
+    int* ent = iscr + nb + 1;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        int c = 0;
+        for (int o = 0; o < noff; ++o) c += (offij[2 * o] == i ? 1 : 0) + (offij[2 * o + 1] == i ? 1 : 0);
+        row_off[i + 1] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        row_off[0] = 0;
+        for (int i = 0; i < nb; ++i) row_off[i + 1] += row_off[i];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        int e = row_off[i];
+        for (int o = 0; o < noff; ++o) {
+            if (offij[2 * o] == i) ent[e++] = o << 1;
+            if (offij[2 * o + 1] == i) ent[e++] = (o << 1) | 1;
+        }
+    }
+    auto mv = [&](const double* v, double* y) {
+        for (int idx = threadIdx.x; idx < nd; idx += blockDim.x) {
+            const int i = idx / 6, a = idx - 6 * i;
+            double s = 0.0;
+            for (int c = 0; c < 6; ++c) s += diag[36 * i + a * 6 + c] * v[6 * i + c];
+            for (int e = row_off[i]; e < row_off[i + 1]; ++e) {
+                const int o = ent[e] >> 1;
+                const double* B = offb + 36 * o;
+                if ((ent[e] & 1) == 0) {
+                    const int bj = offij[2 * o + 1];
+                    for (int c = 0; c < 6; ++c) s += B[a * 6 + c] * v[6 * bj + c];
+                } else {
+                    const int bi = offij[2 * o];
+                    for (int c = 0; c < 6; ++c) s += B[c * 6 + a] * v[6 * bi + c];
+                }
+            }
+            y[idx] = s;
+        }
+        __syncthreads();
+    };
+    auto prec = [&](const double* v, double* y) {
+        for (int idx = threadIdx.x; idx < nd; idx += blockDim.x) {
+            const int i = idx / 6, a = idx - 6 * i;
+            double s = 0.0;
+            for (int c = 0; c < 6; ++c) s += Dinv[36 * i + a * 6 + c] * v[6 * i + c];
+            y[idx] = s;
+        }
+        __syncthreads();
+    };
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) inv6(diag + 36 * i, Dinv + 36 * i);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) { x[i] = 0.0; r[i] = b[i]; }
+    __syncthreads();
+    prec(r, z);
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) p[i] = z[i];
+    __syncthreads();
+    double rz = wg_dot(r, z, nd, red);
+    const double bn = sqrt(wg_dot(b, b, nd, red));
+    double rn = bn;
+    int it = 0;
+    for (; it < max_it && rn > tol * bn; ++it) {
+        mv(p, Ap);
+        const double pAp = wg_dot(p, Ap, nd, red);
+        const double alpha = rz / pAp;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; }
+        __syncthreads();
+        rn = sqrt(wg_dot(r, r, nd, red));
+        prec(r, z);
+        const double rz2 = wg_dot(r, z, nd, red);
+        const double beta = rz2 / rz;
+        rz = rz2;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) p[i] = z[i] + beta * p[i];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *iters = it;
+        *rel = bn > 0.0 ? rn / bn : 0.0;
+    }
+}
+
+void launch_pcg_rows(int nb, const double* diag, int noff, const int* offij, const double* offb, const double* b,
+                     double tol, int max_it, double* x, int* iters, double* rel, double* scratch, int* iscratch,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_pcg_rows, dim3(1), dim3(kBlk), 0, st, nb, diag, noff, offij, offb, b, tol, max_it, x, iters,
+                       rel, scratch, iscratch);
+}
+
 void launch_pcg(int nb, const double* diag, int noff, const int* offij, const double* offb, const double* b,
                 double tol, int max_it, double* x, int* iters, double* rel, double* scratch, hipStream_t st) {
     hipLaunchKernelGGL(k_pcg, dim3(1), dim3(kBlk), 0, st, nb, diag, noff, offij, offb, b, tol, max_it, x, iters, rel,

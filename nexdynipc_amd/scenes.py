@@ -448,6 +448,22 @@ def config5(n_scenes=128):
     return out
 
 
+def ipc_drop_scene(nx=4, nz=4, layers=2, size=0.2, gap=0.05, tilt_deg=6.0, seed=0) -> Scenes:
+    """A grid of nx x nz x layers cubes (edge `size`, `gap` apart, small random tilts) held
+    just above a static ground box: the ipc mode's step workload (the cubes fall onto the
+    ground and onto each other)."""
+    rng = np.random.default_rng(SEED_BASE + 7000 + seed)
+    half = 0.5 * (max(nx, nz) * (size + gap) + 1.0)
+    bodies = [_box_body("ipc_ground", 2 * half, 0.5, 2 * half, [0.0, -0.25, 0.0], static=True)]
+    for k in range(layers):
+        for i in range(nx):
+            for j in range(nz):
+                p = [(i - 0.5 * (nx - 1)) * (size + gap), 0.5 * size + 0.02 + k * (size + gap),
+                     (j - 0.5 * (nz - 1)) * (size + gap)]
+                bodies.append(_box_body(f"ipc_cube{size}", size, size, size, p, random_tilt(rng, tilt_deg)))
+    return pack([SceneSpec(bodies, dt=1e-2, dhat=1e-3, mu=0.0, name="ipc_drop")])
+
+
 def micro_contact_scene(seed=0, n_cubes=4, size=6e-4, gap=2e-4, dhat=1e-3, mu=0.5, tilt_deg=0.0,
                         v_scale=0.01):
     """Millimetre cubes stacked so that centre separations are below dhat.

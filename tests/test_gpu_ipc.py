@@ -204,3 +204,16 @@ def test_ipc_step_matches_oracle(engine_factory):
     with pytest.raises(RuntimeError, match="single-body fixed"):
         e2 = engine_factory(scenes.hinge_chain_scene(n_links=3))
         e2.ipc_step(0, 1e-2)
+
+
+def test_ipc_step_grid_of_cubes_matches_oracle(engine_factory):
+    """The bench's ipc-step workload (32 cubes over a ground box, ~5e4 pair records): the
+    first two steps bitwise against the oracle."""
+    sc = scenes.ipc_drop_scene()
+    e = engine_factory(sc)
+    o = O.OracleScene(sc)
+    kw = dict(dhat=1e-2, kappa=1e5, inflation=0.05)
+    for k in range(2):
+        assert o.ipc_step(1e-2, **kw) == e.ipc_step(0, 1e-2, **kw), k
+        for a, b in zip(o.get_state()[:4], e.get_state()[:4]):
+            assert np.array_equal(a, b), k
