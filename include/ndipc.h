@@ -90,7 +90,8 @@ void* ndipc_stream(ndipc_ctx* ctx);
 
 /* Replaces World::addBody/addJoint + ContactAssembler::build's per-mesh copies
  * (ContactAssembler.cpp:124-154): meshes, unique edges and local AABBs are uploaded
- * once and stay resident. */
+ * once and stay resident. A mesh may have up to 65535 vertices, unique edges and faces
+ * and a scene up to 4096 bodies; larger inputs are rejected with an error. */
 int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* desc);
 /* Same settings for every scene (IPCSolver setters). */
 int32_t ndipc_configure(ndipc_ctx* ctx, const ndipc_params* params);
@@ -147,7 +148,7 @@ int32_t ndipc_build_candidates(ndipc_ctx* ctx, int32_t scene, double dhat, int32
                                int32_t* ev, int32_t* ee, int32_t* fv, int32_t cap);
 /* select_top_k_indices (ContactAssembler.cpp:67-88) on given scores: the indices kept
  * by std::nth_element, in the order std::sort leaves them (libstdc++'s introselect and
- * introsort, ties included). n <= 2048. */
+ * introsort, ties included). n <= 65535 and min(n, k) <= 2048. */
 int32_t ndipc_select_top_k(ndipc_ctx* ctx, const double* scores, int32_t n, int32_t k, int32_t* out);
 /* IPCSolver::detectContacts (x == NULL, IPCSolver.cpp:394-580) and
  * detectContactsAtState (x = Newton variable of scene, IPCSolver.cpp:582-806) on the

@@ -21,10 +21,6 @@ using nd::Dev;
 using nd::SceneParams;
 using nd::Work;
 
-namespace nd {
-void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, hipStream_t st);
-}
-
 struct TimingRec {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     hipEvent_t open = nullptr;
@@ -110,6 +106,11 @@ int fail(ndipc_ctx* c, const std::string& m) {
         hipError_t e_ = (x);                                                                   \
         if (e_ != hipSuccess) return fail(ctx, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// Work-groups of the top-K and scene-min kernels when a mesh exceeds the LDS-resident
+// size (one HBM slab each; 1024 single-wave top-K groups, 256 scene-min groups of 256).
+constexpr int kTopkSlabBlocks = 1024;
+constexpr int kSmfSlabBlocks = 256;
 
 // Device scratch of one call: freed when the call returns, on every path.
 struct ScratchBufs {
@@ -415,7 +416,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     for (int b = 0; b < N; ++b) {
         const int nvb = ctx->nv[b], nfb = ctx->nf[b];
         if (nvb == 0) continue;
-        if (nvb > nd::kMaxPrims || nfb > nd::kMaxPrims) return fail(ctx, "mesh exceeds 2048 vertices or faces");
+        if (nvb > nd::kMaxPrims || nfb > nd::kMaxPrims) return fail(ctx, "mesh exceeds 65535 vertices or faces");
         const double* V = dsc->verts + 3 * size_t(ctx->v_off[b]);
         double mn[3] = {V[0], V[1], V[2]}, mx[3] = {V[0], V[1], V[2]};
         for (int v = 0; v < nvb; ++v)
@@ -438,7 +439,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
         auto it = edge_cache.find(key);
         if (it == edge_cache.end()) {
             const std::vector<int> E = unique_edges(F, nfb);
-            if (int(E.size() / 2) > nd::kMaxPrims) return fail(ctx, "mesh exceeds 2048 edges");
+            if (int(E.size() / 2) > nd::kMaxPrims) return fail(ctx, "mesh exceeds 65535 edges");
             const int off = int(ctx->edges_host.size() / 2);
             ctx->edges_host.insert(ctx->edges_host.end(), E.begin(), E.end());
             it = edge_cache.emplace(key, std::make_pair(off, int(E.size() / 2))).first;
@@ -649,6 +650,24 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     w.flag = dalloc<int>(ctx, S);
     w.bcnt = dalloc<int>(ctx, 2 * size_t(N));
     w.boff = dalloc<int>(ctx, 2 * size_t(N));
+    // meshes above the LDS-resident size: per work-group HBM slabs of the top-K and
+    // scene-min kernels (their grids are kept within the slab counts at launch)
+    {
+        int max_ne = 1;
+        for (int b = 0; b < N; ++b) max_ne = std::max(max_ne, ctx->ne[b]);
+        const int M = std::max(max_ne, std::max(d.max_nv, d.max_nf));
+        if (M > nd::kTkLdsPrims) {
+            w.tk_slab_m = M;
+            w.tk_slab_blocks = kTopkSlabBlocks;
+            w.tk_slab_stride = nd::tk_slab_bytes(M);
+            w.tk_slab = dalloc<unsigned char>(ctx, w.tk_slab_stride * kTopkSlabBlocks);
+        }
+        if (d.max_nv > nd::kTkLdsPrims || d.max_nf > nd::kTkLdsPrims) {
+            w.smf_slab_blocks = kSmfSlabBlocks;
+            w.smf_slab_stride = nd::smf_slab_bytes(d.max_nv, d.max_nf);
+            w.smf_slab = dalloc<unsigned char>(ctx, w.smf_slab_stride * kSmfSlabBlocks);
+        }
+    }
     ctx->d_dt = dalloc<double>(ctx, S);
     ctx->scratch_bytes = 0;
     ctx->d_scratch = nullptr;
@@ -1163,14 +1182,17 @@ int32_t ndipc_build_candidates(ndipc_ctx* ctx, int32_t scene, double dhat, int32
 int32_t ndipc_select_top_k(ndipc_ctx* ctx, const double* scores, int32_t n, int32_t k, int32_t* out) {
     if (!ctx) return -1;
     HIPCHK(hipSetDevice(ctx->device));
-    if (n < 0 || n > nd::kMaxPrims) return fail(ctx, "n must be in [0, 2048]");
+    if (n < 0 || n > nd::kMaxPrims) return fail(ctx, "n must be in [0, 65535]");
+    if (std::min(n, k) > nd::kTkLdsPrims) return fail(ctx, "min(n, k) must be at most 2048");
     double* ds = nullptr;
     int* di = nullptr;
+    unsigned char* slab = nullptr;
     ScratchBufs sb;  // freed on every return
     HIPCHK(sb.alloc(&ds, sizeof(double) * std::max(n, 1)));
     HIPCHK(sb.alloc(&di, sizeof(int) * (std::max(n, 1) + 1)));
+    if (n > nd::kTkLdsPrims) HIPCHK(sb.alloc(&slab, nd::tk_slab_bytes(n)));
     HIPCHK(hipMemcpy(ds, scores, sizeof(double) * n, hipMemcpyHostToDevice));
-    nd::launch_select_topk_scores(ds, n, k, di + 1, di, ctx->stream);
+    nd::launch_select_topk_scores(ds, n, k, di + 1, di, slab, ctx->stream);
     int m = 0;
     HIPCHK(hipMemcpyAsync(&m, di, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -654,27 +654,59 @@ void launch_broad_phase(const Dev& d, const Work& w, int mode, double r, bool, c
 // the host): each Hoare partition is computed from the positions it swaps with a wave
 // scan, median-of-three and the heap fallbacks run on lane 0, and the final insertion
 // sorts are per-block rank sorts with one lane per element.
+// Lists of up to kTkLdsPrims entries live in the wave's LDS; longer ones (meshes of up to
+// kMaxPrims primitives) run the same steps on the same arrays in a per-work-group slab
+// of HBM (TkHbm), with only the range stack in LDS.
 __device__ __forceinline__ bool key_less(double sa, int ia, double sb, int ib) {
     return sa < sb || (sa == sb && ia < ib);
 }
 
 constexpr int kTkLanes = 64;
-constexpr int kTkPer = kMaxPrims / kTkLanes;  // list entries per lane at most
+constexpr int kTkPer = kTkLdsPrims / kTkLanes;  // kept entries per lane at most (k <= kTkLdsPrims)
 
 struct TkLds {
-    double key[kMaxPrims];
-    unsigned short idx[kMaxPrims];
-    unsigned short lpos[kMaxPrims], rpos[kMaxPrims];
-    unsigned char start[kMaxPrims];
+    double key[kTkLdsPrims];
+    unsigned short idx[kTkLdsPrims];
+    unsigned short lpos[kTkLdsPrims], rpos[kTkLdsPrims];
+    unsigned char start[kTkLdsPrims];
     int stk[3 * 64];
 };
 
-// Exclusive prefix sum over the wave's 64 lanes, and the total.
-__device__ __forceinline__ int tk_wave_excl_sum(int v, int* total) {
-    int x = v;
+// The arrays of TkLds for a list of up to M <= kMaxPrims entries in a slab of HBM
+// (tk_slab_bytes(M) bytes, 256-byte aligned parts); the range stack stays in LDS.
+struct TkHbm {
+    double* key;
+    unsigned short *idx, *lpos, *rpos;
+    unsigned char* start;
+    int* stk;
+};
+
+__host__ __device__ inline size_t tk_part(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+size_t tk_slab_bytes(int M) {
+    return tk_part(8 * size_t(M)) + 3 * tk_part(2 * size_t(M)) + tk_part(size_t(M));
+}
+__device__ inline TkHbm tk_hbm(unsigned char* base, int M, int* stk) {
+    TkHbm h;
+    h.key = reinterpret_cast<double*>(base);
+    base += tk_part(8 * size_t(M));
+    h.idx = reinterpret_cast<unsigned short*>(base);
+    base += tk_part(2 * size_t(M));
+    h.lpos = reinterpret_cast<unsigned short*>(base);
+    base += tk_part(2 * size_t(M));
+    h.rpos = reinterpret_cast<unsigned short*>(base);
+    base += tk_part(2 * size_t(M));
+    h.start = base;
+    h.stk = stk;
+    return h;
+}
+
+// Exclusive prefix sum over the wave's 64 lanes, and the total (unsigned: two 16-bit
+// counts of lists up to 65535 entries are packed in one word).
+__device__ __forceinline__ unsigned tk_wave_excl_sum(unsigned v, unsigned* total) {
+    unsigned x = v;
 #pragma unroll
     for (int o = 1; o < kTkLanes; o <<= 1) {
-        const int y = __shfl_up(x, o, kTkLanes);
+        const unsigned y = __shfl_up(x, o, kTkLanes);
         if ((int)threadIdx.x >= o) x += y;
     }
     *total = __shfl(x, kTkLanes - 1, kTkLanes);
@@ -693,25 +725,26 @@ __device__ int tk_wave_partition(const KV16Store& a, int lo, int hi, int pivot, 
     const int len = hi - lo;
     const int per = (len + kTkLanes - 1) / kTkLanes;
     const int i0 = lo + lane * per, i1 = min(hi, i0 + per);
-    int cl = 0, cr = 0;
+    unsigned cl = 0, cr = 0;
     for (int i = i0; i < i1; ++i) {
         const double v = a.k[i];
         cl += !(v < pv);
         cr += !(pv < v);
     }
-    int tot;
-    const int ex = tk_wave_excl_sum(cl | (cr << 16), &tot);
-    const int nL = tot & 0xffff, nR = tot >> 16;
-    int kl = ex & 0xffff, kr = ex >> 16;
+    unsigned tot;
+    const unsigned ex = tk_wave_excl_sum(cl | (cr << 16), &tot);
+    const int nL = int(tot & 0xffffu), nR = int(tot >> 16);
+    int kl = int(ex & 0xffffu), kr = int(ex >> 16);
     for (int i = i0; i < i1; ++i) {
         const double v = a.k[i];
         if (!(v < pv)) lpos[kl++] = (unsigned short)i;
         if (!(pv < v)) rpos[nR - 1 - kr++] = (unsigned short)i;
     }
     __syncthreads();
-    // m by tk_first_crossing's two rounds of 64 probes
+    // m by tk_first_crossing: 64 probes a step apart, then the probes of the one step
+    // that holds the crossing, 64 at a time (step > 64 for lists above 4096 entries)
     const int mm = min(nL, nR);
-    const int step = (mm + kTkLanes - 1) / kTkLanes;  // at most 32: mm < kMaxPrims
+    const int step = (mm + kTkLanes - 1) / kTkLanes;
     const int nq = step ? (mm + step - 1) / step : 0;  // probes below mm
     int m;
     {
@@ -722,9 +755,15 @@ __device__ int tk_wave_partition(const KV16Store& a, int lo, int hi, int pivot, 
             m = 0;
         } else {  // m in ((f - 1) * step, min(f * step, mm)]
             const int qa = (f - 1) * step + 1, qb = min(f * step, mm);
-            const int q2 = qa + lane;
-            const unsigned long long b2 = __ballot(q2 < qb && lpos[q2] >= rpos[q2]);
-            m = b2 ? qa + __ffsll((unsigned long long)b2) - 1 : qb;
+            m = qb;
+            for (int base = qa; base < qb; base += kTkLanes) {
+                const int q2 = base + lane;
+                const unsigned long long b2 = __ballot(q2 < qb && lpos[q2] >= rpos[q2]);
+                if (b2) {
+                    m = base + __ffsll((unsigned long long)b2) - 1;
+                    break;
+                }
+            }
         }
     }
     int cut = m < nL ? (int)lpos[m] : hi;
@@ -781,7 +820,8 @@ __device__ void tk_wave_block_rank_sort(const KV16Store& a, int n, const unsigne
 }
 
 // tk_nth_element_lr(a, nth, n) by the wave.
-__device__ void tk_wave_nth_element(const KV16Store& a, int nth, int n, TkLds& L) {
+template <class TL>
+__device__ void tk_wave_nth_element(const KV16Store& a, int nth, int n, TL& L) {
     if (n == 0 || nth == n) return;
     const int lane = threadIdx.x;
     int first = 0, last = n;
@@ -823,10 +863,11 @@ __device__ void tk_wave_nth_element(const KV16Store& a, int nth, int n, TkLds& L
     __syncthreads();
 }
 
-// tk_sort_blocks(a, n) by the wave, n <= 64 * PER; lane 0 keeps the range stack in LDS.
-template <int PER>
-__device__ void tk_wave_sort(const KV16Store& a, int n, TkLds& L) {
-    if (n < 2) return;
+// The introsort loop of tk_sort_blocks(a, n) by the wave: leaves [0, n) cut into blocks
+// (L.start[i] = 1 where one begins) that its final insertion sort orders one by one.
+// Lane 0 keeps the range stack in LDS.
+template <class TL>
+__device__ void tk_wave_sort_loop(const KV16Store& a, int n, TL& L) {
     const int lane = threadIdx.x;
     int* sf = L.stk;
     int* sl = L.stk + 64;
@@ -869,92 +910,128 @@ __device__ void tk_wave_sort(const KV16Store& a, int n, TkLds& L) {
         if (!heap && lane == 0 && first < n) L.start[first] = 1;
     }
     __syncthreads();
+}
+
+// tk_sort_blocks(a, n) by the wave, n <= 64 * PER.
+template <int PER, class TL>
+__device__ void tk_wave_sort(const KV16Store& a, int n, TL& L) {
+    if (n < 2) return;
+    tk_wave_sort_loop(a, n, L);
     tk_wave_block_rank_sort<PER>(a, n, L.start);
 }
 
 // The list's scores are in L.key[0..n) with L.idx[i] = i; leaves the kept k indices
 // (k <= 64 * PER), in the reference's order, in L.idx[0..k).
-template <int PER>
-__device__ void tk_wave_select(TkLds& L, int n, int k) {
+template <int PER, class TL>
+__device__ void tk_wave_select(TL& L, int n, int k) {
     const KV16Store a{L.key, L.idx};
     if (k < n) tk_wave_nth_element(a, k, n, L);
     tk_wave_sort<PER>(a, k, L);
 }
 
+// Scores of list `list` of close pair p into L.key[0..n) (L.idx[i] = i), the selection,
+// and the kept indices out to w.lists.
+template <class TL>
+__device__ void tk_list(const Dev& d, const Work& w, TL& L, int p, int list, int n, int X, int Y) {
+    const int lane = threadIdx.x;
+    const Box3 by = ldbox(w.wbox, Y);
+    const V3 px = ld3(d.p, X);
+    const Q qx = ld4(d.q, X);
+    const int kind = list >> 1;  // 0 vertices, 1 edges, 2 faces
+    for (int i = lane; i < n; i += kTkLanes) {
+        double sc;
+        if (kind == 0) {
+            sc = point_box_distance(world_vertex(d, X, i, px, qx), by);
+        } else if (kind == 1) {
+            const int* e = d.edges + 2 * (d.e_off[X] + i);
+            const double a = point_box_distance(world_vertex(d, X, e[0], px, qx), by);
+            const double b = point_box_distance(world_vertex(d, X, e[1], px, qx), by);
+            sc = (b < a) ? b : a;
+        } else {
+            const int* f = d.faces + 3 * (d.f_off[X] + i);
+            const double a = point_box_distance(world_vertex(d, X, f[0], px, qx), by);
+            const double b = point_box_distance(world_vertex(d, X, f[1], px, qx), by);
+            const double c = point_box_distance(world_vertex(d, X, f[2], px, qx), by);
+            sc = a;
+            if (b < sc) sc = b;
+            if (c < sc) sc = c;
+        }
+        L.key[i] = sc;
+        L.idx[i] = (unsigned short)i;
+    }
+    __syncthreads();
+    const int k = n < kTopK ? n : kTopK;
+    tk_wave_select<(kTopK + kTkLanes - 1) / kTkLanes>(L, n, k);
+    for (int i = lane; i < k; i += kTkLanes) w.lists[(p * 6 + list) * kTopK + i] = L.idx[i];
+    if (lane == 0) w.lcnt[p * 6 + list] = k;
+    __syncthreads();
+}
+
+// One wave per list. A list longer than kTkLdsPrims uses work-group blockIdx.x's slab of
+// w.tk_slab (allocated at upload for such meshes; launch_topk keeps the grid within
+// w.tk_slab_blocks then).
 __global__ void __launch_bounds__(kTkLanes) k_topk(Dev d, Work w) {
     __shared__ TkLds L;
-    const int lane = threadIdx.x;
     const int total = *w.n_pairs_total;
     for (int task = blockIdx.x; task < total * 6; task += gridDim.x) {
         const int p = task / 6, list = task % 6;
         const int A = w.pairs[2 * p], B = w.pairs[2 * p + 1];
         const int X = (list & 1) ? B : A;
         const int Y = (list & 1) ? A : B;
-        const Box3 by = ldbox(w.wbox, Y);
-        const V3 px = ld3(d.p, X);
-        const Q qx = ld4(d.q, X);
-        const int kind = list >> 1;  // 0 vertices, 1 edges, 2 faces
+        const int kind = list >> 1;
         const int n = kind == 0 ? d.nv[X] : (kind == 1 ? d.ne[X] : d.nf[X]);
-        for (int i = lane; i < n; i += kTkLanes) {
-            double sc;
-            if (kind == 0) {
-                sc = point_box_distance(world_vertex(d, X, i, px, qx), by);
-            } else if (kind == 1) {
-                const int* e = d.edges + 2 * (d.e_off[X] + i);
-                const double a = point_box_distance(world_vertex(d, X, e[0], px, qx), by);
-                const double b = point_box_distance(world_vertex(d, X, e[1], px, qx), by);
-                sc = (b < a) ? b : a;
-            } else {
-                const int* f = d.faces + 3 * (d.f_off[X] + i);
-                const double a = point_box_distance(world_vertex(d, X, f[0], px, qx), by);
-                const double b = point_box_distance(world_vertex(d, X, f[1], px, qx), by);
-                const double c = point_box_distance(world_vertex(d, X, f[2], px, qx), by);
-                sc = a;
-                if (b < sc) sc = b;
-                if (c < sc) sc = c;
-            }
-            L.key[i] = sc;
-            L.idx[i] = (unsigned short)i;
+        if (n <= kTkLdsPrims) {
+            tk_list(d, w, L, p, list, n, X, Y);
+        } else {
+            TkHbm H = tk_hbm(w.tk_slab + size_t(blockIdx.x) * w.tk_slab_stride, w.tk_slab_m, L.stk);
+            tk_list(d, w, H, p, list, n, X, Y);
         }
-        __syncthreads();
-        const int k = n < kTopK ? n : kTopK;
-        tk_wave_select<(kTopK + kTkLanes - 1) / kTkLanes>(L, n, k);
-        for (int i = lane; i < k; i += kTkLanes) w.lists[(p * 6 + list) * kTopK + i] = L.idx[i];
-        if (lane == 0) w.lcnt[p * 6 + list] = k;
-        __syncthreads();
     }
 }
 
 void launch_topk(const Dev& d, const Work& w, int max_pairs, hipStream_t st) {
-    const int cap = w.grid_topk > 0 ? w.grid_topk : 4096;
+    int cap = w.grid_topk > 0 ? w.grid_topk : 4096;
+    if (w.tk_slab && cap > w.tk_slab_blocks) cap = w.tk_slab_blocks;
     int grid = max_pairs * 6;
     if (grid > cap) grid = cap;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(k_topk, dim3(grid), dim3(kTkLanes), 0, st, d, w);
 }
 
-// Standalone selection on given scores (one wave).
-__global__ void __launch_bounds__(kTkLanes) k_select_scores(const double* scores, int n, int k, int* out,
-                                                           int* out_n) {
-    __shared__ TkLds L;
+template <class TL>
+__device__ void tk_select_given(TL& L, const double* scores, int n, int kk, int* out, int* out_n) {
     const int lane = threadIdx.x;
     for (int i = lane; i < n; i += kTkLanes) {
         L.key[i] = scores[i];
         L.idx[i] = (unsigned short)i;
     }
     __syncthreads();
-    const int kk = k < n ? k : n;
-    if (kk <= 0) {
-        if (lane == 0) *out_n = 0;
-        return;
-    }
     tk_wave_select<kTkPer>(L, n, kk);
     for (int i = lane; i < kk; i += kTkLanes) out[i] = L.idx[i];
     if (lane == 0) *out_n = kk;
 }
 
-void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, hipStream_t st) {
-    hipLaunchKernelGGL(k_select_scores, dim3(1), dim3(kTkLanes), 0, st, scores, n, k, out, out_n);
+// Standalone selection on given scores (one wave): n <= kTkLdsPrims in LDS, else in
+// `slab` (tk_slab_bytes(n) bytes); k <= kTkLdsPrims.
+__global__ void __launch_bounds__(kTkLanes) k_select_scores(const double* scores, int n, int k, int* out,
+                                                           int* out_n, unsigned char* slab) {
+    __shared__ TkLds L;
+    const int kk = k < n ? k : n;
+    if (kk <= 0) {
+        if (threadIdx.x == 0) *out_n = 0;
+        return;
+    }
+    if (n <= kTkLdsPrims) {
+        tk_select_given(L, scores, n, kk, out, out_n);
+    } else {
+        TkHbm H = tk_hbm(slab, n, L.stk);
+        tk_select_given(H, scores, n, kk, out, out_n);
+    }
+}
+
+void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, unsigned char* slab,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_select_scores, dim3(1), dim3(kTkLanes), 0, st, scores, n, k, out, out_n, slab);
 }
 
 // ------------------------------------------------------------------------- poses
@@ -1299,8 +1376,65 @@ __global__ void __launch_bounds__(kBlock) k_np_write(Dev d, Work w, int vv, int 
     }
 }
 
-// Truncation to 2000 contacts by distance (IPCSolver.cpp:571-577): bitonic sort of
-// (distance, position) in global memory per scene, then the first 2000 are kept.
+// Truncation to 2000 contacts by distance (IPCSolver.cpp:571-577): std::sort of the raw
+// list by distance, then the first 2000 are kept. With tied distances the kept set and
+// its order are those of libstdc++'s introsort, so k_np_cap_order replays it on
+// (distance, position) pairs by the wave, as the top-K lists do (tk_sort_blocks): the
+// introsort loop's partitions, then its final insertion sort as per-block rank sorts,
+// whose destinations below 2000 give kept[dst] = raw position. k_np_cap gathers them.
+// Per-scene scratch (at most kMaxRawContacts = 40000 entries): in sort_key the
+// distances (320,000 B) and the partition positions lpos, rpos (80,000 B each); in
+// sort_idx the 16-bit positions (80,000 B), the block starts (40,000 B) and kept.
+constexpr size_t kCapLposB = 8 * size_t(kMaxRawContacts);
+constexpr size_t kCapRposB = kCapLposB + 2 * size_t(kMaxRawContacts);
+constexpr size_t kCapStartB = 2 * size_t(kMaxRawContacts);
+constexpr size_t kCapKeptB = (kCapStartB + size_t(kMaxRawContacts) + 255) & ~size_t(255);
+static_assert(kCapRposB + 2 * size_t(kMaxRawContacts) <= sizeof(double) * size_t(kSceneScratch), "cap scratch");
+static_assert(kCapKeptB + sizeof(int) * kMaxContacts <= sizeof(int) * size_t(kSceneScratch), "cap scratch");
+static_assert(kMaxRawContacts <= 65535, "16-bit positions");
+
+__global__ void __launch_bounds__(kTkLanes) k_np_cap_order(Dev d, Work w, int gate) {
+    __shared__ int stk[3 * 64];
+    const int s = blockIdx.x;
+    const SceneParams& sp = d.sp[s];
+    if (gate && sp.alm_done) return;
+    const int n = w.raw_cnt[s];
+    if (n <= kMaxContacts) return;
+    const int lane = threadIdx.x;
+    unsigned char* kb = reinterpret_cast<unsigned char*>(w.sort_key + (long long)s * kSceneScratch);
+    unsigned char* ib = reinterpret_cast<unsigned char*>(w.sort_idx + (long long)s * kSceneScratch);
+    TkHbm L;
+    L.key = reinterpret_cast<double*>(kb);
+    L.lpos = reinterpret_cast<unsigned short*>(kb + kCapLposB);
+    L.rpos = reinterpret_cast<unsigned short*>(kb + kCapRposB);
+    L.idx = reinterpret_cast<unsigned short*>(ib);
+    L.start = ib + kCapStartB;
+    L.stk = stk;
+    int* kept = reinterpret_cast<int*>(ib + kCapKeptB);
+    const long long cb = (long long)s * kMaxRawContacts;
+    for (int i = lane; i < n; i += kTkLanes) {
+        L.key[i] = w.cdbl[7 * (cb + i) + 6];
+        L.idx[i] = (unsigned short)i;
+    }
+    __syncthreads();
+    const KV16Store a{L.key, L.idx};
+    tk_wave_sort_loop(a, n, L);
+    // the final insertion sort: each element's rank within its block; the keys do not
+    // move, so the destinations are computed in place
+    for (int i = lane; i < n; i += kTkLanes) {
+        int b0 = i, e = i + 1;
+        while (!L.start[b0]) --b0;
+        while (e < n && !L.start[e]) ++e;
+        const double v = L.key[i];
+        int r = b0;
+        for (int t = b0; t < e; ++t) {
+            const double u = L.key[t];
+            r += (u < v) || (t < i && !(v < u));
+        }
+        if (r < kMaxContacts) kept[r] = L.idx[i];
+    }
+}
+
 __global__ void __launch_bounds__(1024) k_np_cap(Dev d, Work w, int gate) {
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
@@ -1310,42 +1444,15 @@ __global__ void __launch_bounds__(1024) k_np_cap(Dev d, Work w, int gate) {
         if (threadIdx.x == 0) sp.n_contacts = n;
         return;
     }
-    int P = 1;
-    while (P < n) P <<= 1;
-    double* sk = w.sort_key + (long long)s * kSceneScratch;
-    int* si = w.sort_idx + (long long)s * kSceneScratch;
+    const int* kept = reinterpret_cast<const int*>(reinterpret_cast<const unsigned char*>(
+                                                       w.sort_idx + (long long)s * kSceneScratch) + kCapKeptB);
     const long long cb = (long long)s * kMaxRawContacts;
-    for (int i = threadIdx.x; i < P; i += blockDim.x) {
-        sk[i] = i < n ? w.cdbl[7 * (cb + i) + 6] : 1.0 / 0.0;
-        si[i] = i < n ? i : 0x7fffffff;
-    }
-    __syncthreads();
-    for (int size = 2; size <= P; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int k = threadIdx.x; k < P / 2; k += blockDim.x) {
-                const int i = 2 * k - (k & (stride - 1));
-                const int j = i + stride;
-                const bool up = ((i & size) == 0);
-                const bool gt = key_less(sk[j], si[j], sk[i], si[i]);
-                if (gt == up) {
-                    const double ts = sk[i]; sk[i] = sk[j]; sk[j] = ts;
-                    const int ti = si[i]; si[i] = si[j]; si[j] = ti;
-                }
-            }
-            __threadfence_block();
-            __syncthreads();
-        }
-    }
-    // gather the kept contacts into the sort scratch, then copy back
-    int* ti = reinterpret_cast<int*>(sk);  // reuse: 2000 * 5 ints fit in the key scratch
-    double* td = w.sort_key + (long long)s * kSceneScratch + 8192;
+    // gather the kept contacts into sort_key (free once the order is known), then copy back
+    int* ti = reinterpret_cast<int*>(w.sort_key + (long long)s * kSceneScratch);  // 2000 * 5 ints
+    double* td = w.sort_key + (long long)s * kSceneScratch + 8192;                   // 2000 * 7 doubles
     for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
-        const int src = si[i];
+        const int src = kept[i];
         for (int k = 0; k < 7; ++k) td[7 * i + k] = w.cdbl[7 * (cb + src) + k];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
-        const int src = si[i];
         for (int k = 0; k < 5; ++k) ti[5 * i + k] = w.cint[5 * (cb + src) + k];
     }
     __syncthreads();
@@ -1370,6 +1477,7 @@ void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_a
     hipLaunchKernelGGL(k_np_count, dim3(grid), dim3(kBlock), 0, st, d, w, 1, gate);
     hipLaunchKernelGGL(k_np_scan, dim3(d.S), dim3(kBlock), 0, st, d, w, 1, gate);
     hipLaunchKernelGGL(k_np_write, dim3(grid), dim3(kBlock), 0, st, d, w, 1, gate);
+    hipLaunchKernelGGL(k_np_cap_order, dim3(d.S), dim3(kTkLanes), 0, st, d, w, gate);
     hipLaunchKernelGGL(k_np_cap, dim3(d.S), dim3(1024), 0, st, d, w, gate);
 }
 
@@ -1389,23 +1497,30 @@ void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_a
 constexpr int kSmfChunk = 256;
 // LDS of k_scene_min_flag, sized at launch by the largest mesh (NV vertices, NF faces):
 // J's world vertices and I's candidate vertices (3 NV doubles each), the face chunk
-// (v0, n, inverse: 15 doubles per face), its validity flags and J's active faces.
+// (v0, n, inverse: 15 doubles per face), its validity flags and J's active faces. For
+// meshes above kTkLdsPrims (HBM = true) the vertex arrays and the active faces are in a
+// per-work-group slab of HBM (smf_slab_bytes) and only the face chunk is in LDS.
 __host__ __device__ inline size_t smf_lds_bytes(int NV, int NF) {
     return sizeof(double) * (6 * size_t(NV) + 15 * kSmfChunk) + sizeof(int) * (kSmfChunk + size_t(NF));
 }
+constexpr size_t kSmfChunkLds = sizeof(double) * 15 * kSmfChunk + sizeof(int) * kSmfChunk;
+size_t smf_slab_bytes(int NV, int NF) {
+    return (sizeof(double) * 6 * size_t(NV) + sizeof(int) * size_t(NF) + 255) & ~size_t(255);
+}
 
+template <bool HBM>
 __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int ungated) {
     extern __shared__ __align__(16) unsigned char smf_lds[];
     const int NV = d.max_nv;
-    double* wvx = reinterpret_cast<double*>(smf_lds);
+    double* wvx = reinterpret_cast<double*>(HBM ? w.smf_slab + size_t(blockIdx.x) * w.smf_slab_stride : smf_lds);
     double* wvy = wvx + NV;
     double* wvz = wvy + NV;
     double* ivx = wvz + NV;
     double* ivy = ivx + NV;
     double* ivz = ivy + NV;
-    double* fd = ivz + NV;
+    double* fd = HBM ? reinterpret_cast<double*>(smf_lds) : ivz + NV;
     int* fok = reinterpret_cast<int*>(fd + 15 * kSmfChunk);
-    int* act = fok + kSmfChunk;
+    int* act = HBM ? reinterpret_cast<int*>(ivz + NV) : fok + kSmfChunk;
     __shared__ int nact, nvi;
     const int total = *w.n_prox_total;
     for (int p = blockIdx.x; p < total; p += gridDim.x) {
@@ -1507,18 +1622,22 @@ __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int un
 }
 
 void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_t st, int ungated) {
-    const int cap = w.grid_smf > 0 ? w.grid_smf : 2048;
+    int cap = w.grid_smf > 0 ? w.grid_smf : 2048;
+    if (w.smf_slab && cap > w.smf_slab_blocks) cap = w.smf_slab_blocks;
     int grid = max_prox < cap ? max_prox : cap;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_scene_min_flag, dim3(grid), dim3(kBlock), smf_lds_bytes(d.max_nv, d.max_nf), st, d, w,
-                       ungated);
+    if (w.smf_slab)
+        hipLaunchKernelGGL(k_scene_min_flag<true>, dim3(grid), dim3(kBlock), kSmfChunkLds, st, d, w, ungated);
+    else
+        hipLaunchKernelGGL(k_scene_min_flag<false>, dim3(grid), dim3(kBlock), smf_lds_bytes(d.max_nv, d.max_nf), st,
+                           d, w, ungated);
 }
 
 int geom_configure_smf() {
-    // k_scene_min_flag takes up to smf_lds_bytes(kMaxPrims, kMaxPrims) (~138 KiB)
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_scene_min_flag),
+    // k_scene_min_flag<false> takes up to smf_lds_bytes(kTkLdsPrims, kTkLdsPrims) (~138 KiB)
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(k_scene_min_flag<false>),
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(smf_lds_bytes(kMaxPrims, kMaxPrims))) == hipSuccess
+                               int(smf_lds_bytes(kTkLdsPrims, kTkLdsPrims))) == hipSuccess
                ? 0
                : -1;
 }

@@ -20,7 +20,8 @@ namespace nd {
 constexpr int kTopK = 32;             // ContactAssembler.cpp:221-223
 constexpr int kMaxRawContacts = 40000;  // IPCSolver.cpp:413
 constexpr int kMaxContacts = 2000;    // IPCSolver.cpp:414
-constexpr int kMaxPrims = 2048;       // per-mesh vertex/edge/face limit of the LDS top-K
+constexpr int kMaxPrims = 65535;      // per-mesh vertex/edge/face limit (16-bit list indices)
+constexpr int kTkLdsPrims = 2048;     // longest top-K list / mesh of the LDS-resident paths
 constexpr int kMaxSceneBodies = 4096; // per-scene LBVH limit (LDS bitonic sort)
 constexpr int kHitWords = 48;         // 3072 candidate bits per pair (2x1024 FV + 1024 EE)
 constexpr int kSceneScratch = 65536;  // per-scene scratch doubles (sort keys, barrier b'')
@@ -202,6 +203,15 @@ struct Work {
     double* jgrad;           // [J*12]
     double* dense;           // [S][kDenseRows^2] component LDL^T factors
     int* flag;               // [S] scene-min flag scratch
+    // meshes above kTkLdsPrims vertices, edges or faces (null otherwise): per work-group
+    // slabs in HBM for the top-K lists (tk_slab_bytes(tk_slab_m) each) and for the
+    // scene-min flag's world and candidate vertices and active faces
+    unsigned char* tk_slab;
+    size_t tk_slab_stride;
+    int tk_slab_m, tk_slab_blocks;
+    unsigned char* smf_slab;
+    size_t smf_slab_stride;
+    int smf_slab_blocks;
     unsigned int* round_seq; // [1] ALM rounds ended by k_round_flag since upload
     int* bcnt;               // [2N] per-body pair counts (candidates, prox)
     int* boff;               // [2N] per-body offsets within the scene
@@ -228,7 +238,11 @@ void launch_ccd_batch(int kind, int n, const double* pa, const double* pb, const
                       const double* earliest, double tol, int* hit, double* toi, int* pops, hipStream_t st);
 void launch_round_ops(const double* a, const double* b, int n, double* out, hipStream_t st);
 void launch_elemfn(int fn, const double* x, const double* y, int n, double* out, hipStream_t st);
-void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, hipStream_t st);
+// n <= kTkLdsPrims: `slab` unused; else tk_slab_bytes(n) bytes of device memory
+void launch_select_topk_scores(const double* scores, int n, int k, int* out, int* out_n, unsigned char* slab,
+                               hipStream_t st);
+size_t tk_slab_bytes(int M);                          // top-K HBM slab for lists of up to M entries
+size_t smf_slab_bytes(int NV, int NF);                // scene-min HBM slab (meshes above kTkLdsPrims)
 
 // Adjoint of a step (implicit-function theorem at every converged Newton point of the
 // step's ALM loop). Slot k*A + a holds ALM iteration a of step k of a checkpoint

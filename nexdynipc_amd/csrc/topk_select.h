@@ -288,8 +288,8 @@ ND_HD void tk_block_rank_sort(const St& a, int n, const unsigned char* start, ty
 // ascending lpos and a descending rpos (the test is monotone in q), found as the device
 // finds it with W = 64 lanes: probes at q = j * step (j < nq, step = ceil(mm / W),
 // nq = ceil(mm / step)) bracket it to ((f - 1) * step, min(f * step, mm)], f the first
-// probe that holds (nq if none), and a second round of probes inside the bracket
-// (fewer than step <= W positions) finds it.
+// probe that holds (nq if none), and rounds of W probes through the bracket (fewer than
+// step positions; one round up to mm = 4096) find it.
 template <typename T>
 ND_HD int tk_first_crossing(const T* lpos, const T* rpos, int mm) {
     const int W = 64;
@@ -303,8 +303,9 @@ ND_HD int tk_first_crossing(const T* lpos, const T* rpos, int mm) {
         }
     if (f == 0) return 0;
     const int qa = (f - 1) * step + 1, qb = f * step < mm ? f * step : mm;
-    for (int j = 0; j < W; ++j)
-        if (qa + j < qb && lpos[qa + j] >= rpos[qa + j]) return qa + j;
+    for (int base = qa; base < qb; base += W)
+        for (int j = 0; j < W; ++j)
+            if (base + j < qb && lpos[base + j] >= rpos[base + j]) return base + j;
     return qb;
 }
 

@@ -90,6 +90,41 @@ def icosphere_mesh(subdiv: int, radius: float):
     return V, F
 
 
+def grid_box_mesh(sx: float, sy: float, sz: float, m: int):
+    """Axis-aligned box centred at the origin whose faces are m x m grids of cells, two
+    outward triangles each: 6 (m-1)^2 + 12 (m-1) + 8 vertices, 12 m^2 faces. Vertices
+    sit at (L / m - 1/2) * size on the integer lattice L in [0, m]^3, so the six face
+    planes are exactly +-size / 2."""
+    size = np.array([sx, sy, sz], dtype=np.float64)
+    index = {}
+    verts = []
+
+    def vid(L):
+        key = tuple(L)
+        if key not in index:
+            index[key] = len(verts)
+            verts.append((np.asarray(L, dtype=np.float64) / m - 0.5) * size)
+        return index[key]
+
+    faces = []
+    for a in range(3):
+        u, v = (a + 1) % 3, (a + 2) % 3
+        for side in (0, m):
+            for i in range(m):
+                for j in range(m):
+                    c = []
+                    for di, dj in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        L = [0, 0, 0]
+                        L[a], L[u], L[v] = side, i + di, j + dj
+                        c.append(vid(L))
+                    # e_u x e_v = e_a: counter-clockwise seen from +a
+                    if side == m:
+                        faces += [[c[0], c[1], c[2]], [c[0], c[2], c[3]]]
+                    else:
+                        faces += [[c[0], c[2], c[1]], [c[0], c[3], c[2]]]
+    return np.asarray(verts, dtype=np.float64), np.asarray(faces, dtype=np.int32)
+
+
 def box_mass_properties(sx, sy, sz, density=DENSITY):
     m = density * sx * sy * sz
     I = np.diag([m * (sy * sy + sz * sz) / 12.0, m * (sx * sx + sz * sz) / 12.0, m * (sx * sx + sy * sy) / 12.0])
@@ -462,6 +497,33 @@ def ipc_drop_scene(nx=4, nz=4, layers=2, size=0.2, gap=0.05, tilt_deg=6.0, seed=
                      (j - 0.5 * (nz - 1)) * (size + gap)]
                 bodies.append(_box_body(f"ipc_cube{size}", size, size, size, p, random_tilt(rng, tilt_deg)))
     return pack([SceneSpec(bodies, dt=1e-2, dhat=1e-3, mu=0.0, name="ipc_drop")])
+
+
+def large_mesh_scene(seed=0, subdiv=4, m=24, n_spheres=2, scale=1e-3, gap=0.67, dt=1e-3, dhat=1e-3,
+                     mu=0.3) -> Scenes:
+    """Meshes above 2048 primitives: a plate meshed as an m x m grid per face (m = 24:
+    3,458 vertices, 10,368 edges, 6,912 faces) above a static ground box, with icospheres
+    of subdivision `subdiv` (4: 2,562 vertices, 7,680 edges, 5,120 faces) and a small
+    box above the plate, each `gap` above the body below it. Lengths are in units of
+    `scale`. At 1e-3 with the default gap the plate's centre and the spheres' are
+    0.97 dhat from the centre below them, where the barrier on the centres' offset along
+    the contact normal acts (as in micro_contact_scene); at scale 1 with a small gap the
+    meshes start within dhat of each other but far beyond it centre to centre."""
+    rng = np.random.default_rng(SEED_BASE + 8000 + seed)
+    L = float(scale)
+    Vp, Fp = grid_box_mesh(1.5 * L, 0.1 * L, 1.5 * L, m)
+    mp, Ip = polyhedron_mass_properties(Vp, Fp)
+    yp = 0.05 + gap
+    bodies = [_box_body(f"lm_ground{L!r}", 4.0 * L, 0.5 * L, 4.0 * L, [0.0, -0.25 * L, 0.0], static=True),
+              BodySpec(f"grid_box{m}_{L!r}", Vp, Fp, mp, Ip, np.array([0.0, yp * L, 0.0]), _IDENT.copy(),
+                       np.zeros(3), np.zeros(3))]
+    for k in range(n_spheres):
+        c = [(k - 0.5 * (n_spheres - 1) + rng.uniform(-0.02, 0.02)) * L, (yp + 0.3 + gap) * L,
+             rng.uniform(-0.1, 0.1) * L]
+        bodies.append(_ico_body(subdiv, 0.25 * L, c, q=random_tilt(rng, 20.0), v=rng.normal(0.0, 0.01, size=3)))
+    bodies.append(_box_body(f"lm_box{L!r}", 0.1 * L, 0.1 * L, 0.1 * L, [0.6 * L, (yp + 0.1 + gap) * L, 0.6 * L],
+                            random_tilt(rng, 3.0), v=rng.normal(0.0, 0.01, size=3)))
+    return pack([SceneSpec(bodies, dt=dt, dhat=dhat, mu=mu, name="large_mesh")])
 
 
 def micro_contact_scene(seed=0, n_cubes=4, size=6e-4, gap=2e-4, dhat=1e-3, mu=0.5, tilt_deg=0.0,

@@ -60,9 +60,11 @@ int main(int argc, char** argv) {
         // the adversarial inputs as text: "n k" then n values, one case per line
         FILE* f = std::fopen(argv[2], "w");
         if (!f) return 2;
-        for (int n : {17, 33, 64, 100, 257, 1000, 2048}) {
+        for (int n : {17, 33, 64, 100, 257, 1000, 2048, 4096, 16384}) {
             for (int k : {n, 1, 5, 32, n / 2}) {
                 if (k < 1 || k > n) continue;
+                // beyond 2048 entries: the device path's kept counts (k <= 2048), few cases
+                if (n > 2048 && !(k == 32 || (n == 4096 && (k == 1 || k == n / 2)))) continue;
                 for (int target : {k, n}) {
                     const std::vector<double> s = killer_input(n, target);
                     std::fprintf(f, "%d %d", n, k);
@@ -76,10 +78,15 @@ int main(int argc, char** argv) {
     }
     std::mt19937_64 rng(12345);
     long cases = 0;
-    // the device's two-round search for the first crossing against a linear scan, on
-    // every crossing position of every length up to the list limit
-    for (int mm = 0; mm <= 2048; ++mm) {
-        for (int m = 0; m <= mm; m += (mm > 300 ? 7 : 1)) {
+    // the device's search for the first crossing against a linear scan, on every
+    // crossing position of every length up to the LDS list limit, then on sampled
+    // lengths and positions up to the 65535-entry limit (several second-round passes)
+    std::vector<int> mms;
+    for (int mm = 0; mm <= 2048; ++mm) mms.push_back(mm);
+    for (int mm = 2049; mm <= 65535; mm += 997) mms.push_back(mm);
+    for (int mm : {4095, 4096, 4097, 8191, 8192, 65534, 65535}) mms.push_back(mm);
+    for (int mm : mms) {
+        for (int m = 0; m <= mm; m += (mm > 2048 ? 1 + mm / 400 : (mm > 300 ? 7 : 1))) {
             std::vector<int> lp(mm), rp(mm);
             for (int q = 0; q < mm; ++q) {
                 lp[q] = 2 * q;
@@ -94,7 +101,7 @@ int main(int argc, char** argv) {
             ++cases;
         }
     }
-    for (int n : {17, 33, 64, 100, 257, 1000, 2048}) {
+    for (int n : {17, 33, 64, 100, 257, 1000, 2048, 4096, 16384}) {
         for (int k : {n, 1, 5, 32, n / 2}) {
             if (k < 1 || k > n) continue;
             if (!check_case(killer_input(n, k), k) || !check_case(killer_input(n, n), k)) {
@@ -105,7 +112,8 @@ int main(int argc, char** argv) {
         }
     }
     for (int trial = 0; trial < 40000; ++trial) {
-        const int n = 1 + int(rng() % 2100);
+        // one trial in 100 at a list length of the HBM path (2049 to 65535 entries)
+        const int n = trial % 100 == 99 ? 2049 + int(rng() % 63487) : 1 + int(rng() % 2100);
         const int k = 1 + int(rng() % std::min(n, 40));
         const int levels = 1 + int(rng() % 6);
         std::vector<double> s(n);
@@ -152,9 +160,18 @@ int main(int argc, char** argv) {
         }
         ++cases;
     }
-    // whole-array sorts: random, presorted, reversed, constant, few levels
+    // whole-array sorts: random, presorted, reversed, constant, few levels; one trial in
+    // 50 at a length of the contact cap's sort (2001 to 40000 raw contacts), and the
+    // adversary's input for std::sort at 40000
+    for (int k : {40000, 2000}) {
+        if (!check_case(killer_input(40000, 40000), k)) {
+            std::printf("MISMATCH adversarial sort n 40000 k %d\n", k);
+            return 1;
+        }
+        ++cases;
+    }
     for (int trial = 0; trial < 4000; ++trial) {
-        const int n = 1 + int(rng() % 3000);
+        const int n = trial % 50 == 49 ? 2001 + int(rng() % 38000) : 1 + int(rng() % 3000);
         const int levels = 1 + int(rng() % 8);
         std::vector<double> s(n);
         for (int i = 0; i < n; ++i) s[i] = double(rng() % levels);

@@ -199,7 +199,7 @@ def test_select_top_k_adversarial_inputs_keep_the_smallest():
     fallbacks taken) the kept scores are still the k smallest, in ascending order, with
     distinct indices."""
     adv = np.load(os.path.join(os.path.dirname(__file__), "golden", "topk_adversarial.npz"))
-    assert len(adv.files) == 68
+    assert len(adv.files) == 76
     for name in adv.files:
         s = adv[name].astype(np.float64)
         k = int(name.split("_k")[1])

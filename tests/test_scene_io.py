@@ -186,3 +186,28 @@ def test_exporter_round_trip(tmp_path, basic_contact):
         krows = list(csv.reader(f))
     assert krows[0] == ["frame_idx", "time", "V_w", "T_sat", "R_dual", "max_constraint_violation"]
     assert len(krows) == 4
+
+
+@pytest.mark.parametrize("m", [1, 2, 5, 24])
+def test_grid_box_mesh_is_a_closed_outward_box(m):
+    """The gridded box: vertex and face counts, Euler characteristic 2, every edge shared
+    by two faces that traverse it in opposite directions, the face planes exactly at
+    +-size/2 and the enclosed volume and inertia those of the box."""
+    sx, sy, sz = 1.5, 0.1, 0.75
+    V, F = scenes.grid_box_mesh(sx, sy, sz, m)
+    assert len(V) == 6 * (m - 1) ** 2 + 12 * (m - 1) + 8 and len(F) == 12 * m * m
+    directed = {}
+    for a, b, c in F:
+        for u, v in ((a, b), (b, c), (c, a)):
+            assert (u, v) not in directed
+            directed[(u, v)] = 1
+    assert all((v, u) in directed for (u, v) in directed)
+    n_edges = len(directed) // 2
+    assert len(V) - n_edges + len(F) == 2
+    for k, h in enumerate((sx, sy, sz)):
+        assert set(np.unique(np.abs(V[:, k])).tolist()) >= {0.5 * h}
+        assert np.abs(V[:, k]).max() == 0.5 * h
+    mass, inertia = scenes.polyhedron_mass_properties(V, F)
+    mb, Ib = scenes.box_mass_properties(sx, sy, sz)
+    assert abs(mass - mb) <= 1e-12 * mb
+    assert np.allclose(inertia, Ib, rtol=1e-10, atol=1e-12 * Ib.max())
