@@ -39,6 +39,10 @@ struct ndipc_ctx {
     // `stream` around each Newton launch
     hipStream_t hstream = nullptr;
     hipEvent_t hfork = nullptr, hjoin = nullptr;
+    // third stream for the step prefix's stiffness control (scene-min flag, kappa update),
+    // forked after the broad phase and joined before the contact setup
+    hipStream_t pstream = nullptr;
+    hipEvent_t pfork = nullptr, pjoin = nullptr;
     std::string err;
     bool uploaded = false;
     Dev d{};
@@ -265,7 +269,7 @@ hipEvent_t pooled_event() {
 namespace nd {
 void timing_begin(const char* name, hipStream_t st) {
     ndipc_ctx* c = g_timing_ctx;
-    if (!c || !c->timing || st != c->stream) return;
+    if (!c || !c->timing || (st != c->stream && st != c->pstream)) return;
     hipEvent_t e = pooled_event();
     if (!e) return;
     (void)hipEventRecord(e, st);
@@ -273,7 +277,7 @@ void timing_begin(const char* name, hipStream_t st) {
 }
 void timing_end(const char* name, hipStream_t st) {
     ndipc_ctx* c = g_timing_ctx;
-    if (!c || !c->timing || st != c->stream) return;
+    if (!c || !c->timing || (st != c->stream && st != c->pstream)) return;
     TimingRec& r = c->tim[name];
     if (!r.open) return;
     hipEvent_t e = pooled_event();
@@ -323,22 +327,28 @@ int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
         delete c;
         return -4;
     }
-    if (hipStreamCreate(&c->hstream) != hipSuccess ||
-        hipEventCreateWithFlags(&c->hfork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->hjoin, hipEventDisableTiming) != hipSuccess) {
+    auto release = [&] {
+        if (c->pjoin) (void)hipEventDestroy(c->pjoin);
+        if (c->pfork) (void)hipEventDestroy(c->pfork);
+        if (c->pstream) (void)hipStreamDestroy(c->pstream);
+        if (c->hjoin) (void)hipEventDestroy(c->hjoin);
         if (c->hfork) (void)hipEventDestroy(c->hfork);
         if (c->hstream) (void)hipStreamDestroy(c->hstream);
         (void)hipStreamDestroy(c->stream);
         delete c;
+    };
+    if (hipStreamCreate(&c->hstream) != hipSuccess ||
+        hipEventCreateWithFlags(&c->hfork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->hjoin, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreate(&c->pstream) != hipSuccess ||
+        hipEventCreateWithFlags(&c->pfork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->pjoin, hipEventDisableTiming) != hipSuccess) {
+        release();
         return -4;
     }
     if (hipHostMalloc(reinterpret_cast<void**>(&c->hflag), 2 * sizeof(unsigned int), hipHostMallocCoherent) !=
         hipSuccess) {
-        (void)hipEventDestroy(c->hjoin);
-        (void)hipEventDestroy(c->hfork);
-        (void)hipStreamDestroy(c->hstream);
-        (void)hipStreamDestroy(c->stream);
-        delete c;
+        release();
         return -4;
     }
     c->hflag[0] = c->hflag[1] = 0;
@@ -359,9 +369,13 @@ int32_t ndipc_destroy(ndipc_ctx* ctx) {
     for (GrowBuf& b : ctx->ipc_buf)
         if (b.p && hipFree(b.p) != hipSuccess) rc = -1;
     if (hipStreamSynchronize(ctx->hstream) != hipSuccess) rc = -1;
+    if (hipStreamSynchronize(ctx->pstream) != hipSuccess) rc = -1;
     if (hipEventDestroy(ctx->hfork) != hipSuccess) rc = -1;
     if (hipEventDestroy(ctx->hjoin) != hipSuccess) rc = -1;
     if (hipStreamDestroy(ctx->hstream) != hipSuccess) rc = -1;
+    if (hipEventDestroy(ctx->pfork) != hipSuccess) rc = -1;
+    if (hipEventDestroy(ctx->pjoin) != hipSuccess) rc = -1;
+    if (hipStreamDestroy(ctx->pstream) != hipSuccess) rc = -1;
     if (hipStreamDestroy(ctx->stream) != hipSuccess) rc = -1;
     if (hipHostFree(ctx->hflag) != hipSuccess) rc = -1;
     delete ctx;
@@ -620,6 +634,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     w.raw_cnt = dalloc<int>(ctx, S);
     w.cint = dalloc<int>(ctx, size_t(S) * nd::kMaxRawContacts * 5);
     w.cdbl = dalloc<double>(ctx, size_t(S) * nd::kMaxRawContacts * 7);
+    w.cext = dalloc<int>(ctx, size_t(S) * nd::kMaxRawContacts * 2);
     w.sort_idx = dalloc<int>(ctx, size_t(S) * nd::kSceneScratch);
     w.sort_key = dalloc<double>(ctx, size_t(S) * nd::kSceneScratch);
     const size_t V6 = 6 * size_t(N);
@@ -809,18 +824,26 @@ static int enqueue_prefix(ndipc_ctx* ctx, const nd::Adj* rec, int k) {
     nd::launch_lbvh_build(d, w, st);
     nd::launch_barrier_init_ccd(d, w, st);
     nd::launch_broad_phase_step(d, w, st);
+    // The stiffness control reads only the step-start poses and the broad phase's pairs,
+    // and writes only the scene flag and kappa, which nothing before the contact setup
+    // reads: it runs on the third stream beside the candidate build, the CCD gate and the
+    // first narrow phase (latency-bound kernels that leave most CUs idle).
+    HIPCHK(hipEventRecord(ctx->pfork, st));
+    HIPCHK(hipStreamWaitEvent(ctx->pstream, ctx->pfork, 0));
+    nd::timing_begin("scene_min_flag", ctx->pstream);
+    nd::launch_scene_min_flag(d, w, mp, ctx->pstream);
+    nd::timing_end("scene_min_flag", ctx->pstream);
+    nd::launch_kappa(d, w, ctx->pstream);
+    HIPCHK(hipEventRecord(ctx->pjoin, ctx->pstream));
     nd::timing_begin("topk", st);
     nd::launch_topk(d, w, mp, st);
     nd::timing_end("topk", st);
-    nd::timing_begin("scene_min_flag", st);
-    nd::launch_scene_min_flag(d, w, mp, st);
-    nd::timing_end("scene_min_flag", st);
-    nd::launch_kappa(d, w, st);
     nd::launch_ccd_earliest(d, w, mp, st);
     nd::launch_ccd_dt(d, w, st);
     nd::launch_pack(d, w, st);
     nd::launch_pose_from_state(d, w, nullptr, st);
     nd::launch_narrow_phase(d, w, mp, false, st);
+    HIPCHK(hipStreamWaitEvent(st, ctx->pjoin, 0));
     nd::launch_contacts_setup(d, w, st);
     return 0;
 }
@@ -2035,6 +2058,7 @@ int32_t ndipc_adjoint_forward(ndipc_ctx* ctx, const double* dt, int32_t n_steps,
     a.s_sp = aalloc<SceneParams>(ctx, LA * S);
     a.s_cint = aalloc<int>(ctx, LA * S * nd::kMaxContacts * 5);
     a.s_cdbl = aalloc<double>(ctx, LA * S * nd::kMaxContacts * 7);
+    a.s_cext = aalloc<int>(ctx, LA * S * nd::kMaxContacts * 2);
     a.s_nf = aalloc<double>(ctx, LA * S * nd::kMaxContacts);
     a.s_jl = aalloc<double>(ctx, LA * 6 * J);
     a.s_jm = aalloc<double>(ctx, LA * J);
@@ -2054,6 +2078,7 @@ int32_t ndipc_adjoint_forward(ndipc_ctx* ctx, const double* dt, int32_t n_steps,
     a.tmp = aalloc<double>(ctx, 6 * N);
     a.kry = aalloc<double>(ctx, 25 * 6 * N);
     a.cvjp = aalloc<double>(ctx, S * nd::kMaxContacts * 15);
+    a.gvjp = aalloc<double>(ctx, S * nd::kMaxContacts * 14);
     a.ejac_c = aalloc<double>(ctx, S * nd::kMaxContacts * 144);
     a.ejac_j = aalloc<double>(ctx, 144 * J);
     a.ye_c = aalloc<double>(ctx, S * nd::kMaxContacts * 12);

@@ -282,15 +282,15 @@ ND_DU M3T<T> m3mul(const M3T<T>& a, const M3T<T>& b) {
 // Friction gradient of one contact (FrictionForm::gradient, FrictionForm.cpp:44-90 with
 // FrictionPotential::gradient, Friction.cpp:20-31): g[0..5] on body A, g[6..11] on B,
 // as a function of the Newton variables of A (xa[6]) and B (xb[6]), of the step-start
-// poses (pA0, qA0, pB0, qB0) and of mu; pt / nr the contact point and normal.
+// poses (pA0, qA0, pB0, qB0), of mu, and of the contact point P, normal nr and lagged
+// normal force N (which follow the state the narrow phase ran at).
 template <typename T>
-ND_DU void friction_grad_t(const T* xa, const T* xb, V3T<T> pA0, QT<T> qA0, V3T<T> pB0, QT<T> qB0,
-                           const double* pt, const double* nr, double N, T mu, double eps, double dt, T* g) {
+ND_DU void friction_grad_g(const T* xa, const T* xb, V3T<T> pA0, QT<T> qA0, V3T<T> pB0, QT<T> qB0, V3T<T> P,
+                           V3T<T> nr, T N, T mu, double eps, double dt, T* g) {
     const V3T<T> pA = mk3<T>(xa[0], xa[1], xa[2]), pB = mk3<T>(xb[0], xb[1], xb[2]);
     const V3T<T> thA = mk3<T>(xa[3], xa[4], xa[5]), thB = mk3<T>(xb[3], xb[4], xb[5]);
     const QT<T> qA = rot_update(qA0, thA, true), qB = rot_update(qB0, thB, true);
     const V3T<T> dA = sub(pA, pA0), dB = sub(pB, pB0);
-    const V3T<T> P = lift3<T>(pt[0], pt[1], pt[2]);
     const V3T<T> lA = rotate(qconj(qA0), sub(P, pA0)), lB = rotate(qconj(qB0), sub(P, pB0));
     const V3T<T> rA = rotate(qA, lA), rB = rotate(qB, lB);
     V3T<T> wA = lift3<T>(0.0, 0.0, 0.0), wB = wA, vA = wA, vB = wA;
@@ -303,21 +303,21 @@ ND_DU void friction_grad_t(const T* xa, const T* xb, V3T<T> pA0, QT<T> qA0, V3T<
     vA = add(vA, cross(wA, rA));
     vB = add(vB, cross(wB, rB));
     const V3T<T> rel = sub(vB, vA);
-    double n0 = nr[0], n1 = nr[1], n2 = nr[2];
-    const double nn2 = n0 * n0 + n1 * n1 + n2 * n2;
-    if (nn2 > 1e-16) {
-        const double l = sqrt(nn2);
+    T n0 = nr.x, n1 = nr.y, n2 = nr.z;
+    const T nn2 = n0 * n0 + n1 * n1 + n2 * n2;
+    if (val(nn2) > 1e-16) {
+        const T l = dsqrt(nn2);
         n0 = n0 / l; n1 = n1 / l; n2 = n2 / l;
     } else {
-        n0 = 0.0; n1 = 1.0; n2 = 0.0;
+        n0 = lift<T>(0.0); n1 = lift<T>(1.0); n2 = lift<T>(0.0);
     }
-    const double nv[3] = {n0, n1, n2};
+    const T nv[3] = {n0, n1, n2};
     const T rn = rel.x * n0 + rel.y * n1 + rel.z * n2;
     const V3T<T> tv = mk3<T>(rel.x - rn * n0, rel.y - rn * n1, rel.z - rn * n2);
     const V3T<T> u = scl(dt, tv);
     // FrictionPotential::gradient
     V3T<T> fg = lift3<T>(0.0, 0.0, 0.0);
-    if (N > 0) {
+    if (val(N) > 0) {
         const T un = dsqrt(dot(u, u));
         if (!(val(un) < 1e-12)) {
             const T sn = dsqrt(un * un + eps * eps);
@@ -328,7 +328,7 @@ ND_DU void friction_grad_t(const T* xa, const T* xb, V3T<T> pA0, QT<T> qA0, V3T<
     // J_A = [-P, P [rA]x], J_B = [P, -P [rB]x], g = J^T fg
     M3T<T> Pm;
     for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) Pm.m[3 * i + j] = lift<T>((i == j ? 1.0 : 0.0) - nv[i] * nv[j]);
+        for (int j = 0; j < 3; ++j) Pm.m[3 * i + j] = (i == j ? 1.0 : 0.0) - nv[i] * nv[j];
     const M3T<T> PA = m3mul(Pm, skew(rA)), PB = m3mul(Pm, skew(rB));
     const T f[3] = {fg.x, fg.y, fg.z};
     for (int j = 0; j < 3; ++j) {
@@ -337,6 +337,93 @@ ND_DU void friction_grad_t(const T* xa, const T* xb, V3T<T> pA0, QT<T> qA0, V3T<
         g[6 + j] = Pm.m[j] * f[0] + Pm.m[3 + j] * f[1] + Pm.m[6 + j] * f[2];
         g[9 + j] = (-PB.m[j]) * f[0] + (-PB.m[3 + j]) * f[1] + (-PB.m[6 + j]) * f[2];
     }
+}
+// The same with a fixed contact point, normal and normal force.
+template <typename T>
+ND_DU void friction_grad_t(const T* xa, const T* xb, V3T<T> pA0, QT<T> qA0, V3T<T> pB0, QT<T> qB0,
+                           const double* pt, const double* nr, double N, T mu, double eps, double dt, T* g) {
+    friction_grad_g<T>(xa, xb, pA0, qA0, pB0, qB0, lift3<T>(pt[0], pt[1], pt[2]), lift3<T>(nr[0], nr[1], nr[2]),
+                       lift<T>(N), mu, eps, dt, g);
+}
+
+// ------------------------------------------------------------------ contact geometry
+// The narrow phase's point and normal of one contact (IPCSolver.cpp:32-134, 394-806) as
+// functions of the world vertices of its primitives and the two body positions; the
+// branch decisions follow the values, so these are the derivatives of the formulas on
+// the branch the narrow phase took.
+// safeNormalized (IPCSolver.cpp:92-105)
+template <typename T>
+ND_DU V3T<T> safe_normalized_t(V3T<T> v, V3T<T> fb) {
+    const T n = dsqrt(dot(v, v));
+    if (val(n) > 1e-12) return dvs(v, n);
+    const T fn = dsqrt(dot(fb, fb));
+    if (val(fn) > 1e-12) return dvs(fb, fn);
+    return lift3<T>(0.0, 1.0, 0.0);
+}
+template <typename T>
+ND_DU T clamp01_t(T x) { return val(x) < 0.0 ? lift<T>(0.0) : (1.0 < val(x) ? lift<T>(1.0) : x); }
+// closestPointsOnSegments (IPCSolver.cpp:32-90)
+template <typename T>
+ND_DU void closest_points_t(V3T<T> p1, V3T<T> q1, V3T<T> p2, V3T<T> q2, V3T<T>& c1, V3T<T>& c2) {
+    const double eps = 1e-12;
+    const V3T<T> d1 = sub(q1, p1), d2 = sub(q2, p2), r = sub(p1, p2);
+    const T a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    T s, t;
+    if (val(a) <= eps && val(e) <= eps) {
+        c1 = p1;
+        c2 = p2;
+        return;
+    }
+    if (val(a) <= eps) {
+        s = lift<T>(0.0);
+        t = clamp01_t(f / e);
+    } else {
+        const T c = dot(d1, r);
+        if (val(e) <= eps) {
+            t = lift<T>(0.0);
+            s = clamp01_t(-c / a);
+        } else {
+            const T b = dot(d1, d2);
+            const T den = a * e - b * b;
+            s = (fabs(val(den)) > eps) ? clamp01_t((b * f - c * e) / den) : lift<T>(0.0);
+            t = (b * s + f) / e;
+            if (val(t) < 0.0) {
+                t = lift<T>(0.0);
+                s = clamp01_t(-c / a);
+            } else if (val(t) > 1.0) {
+                t = lift<T>(1.0);
+                s = clamp01_t((b - c) / a);
+            }
+        }
+    }
+    c1 = add(p1, scl(s, d1));
+    c2 = add(p2, scl(t, d2));
+}
+// kind 0 face-vertex: v[0..2] the face of body A, v[3] the vertex of body B;
+// kind 1 edge-edge: v[0], v[1] the edge of A, v[2], v[3] the edge of B;
+// kind 2 vertex-vertex: v[0] of A, v[1] of B. pA, pB: the body positions; the normal
+// is oriented towards pB - pA (append_contact).
+template <typename T>
+ND_DU void contact_geom_t(int kind, const V3T<T>* v, V3T<T> pA, V3T<T> pB, V3T<T>& point, V3T<T>& normal) {
+    V3T<T> nrm;
+    if (kind == 0) {
+        const V3T<T> nn = safe_normalized_t(cross(sub(v[1], v[0]), sub(v[2], v[0])), lift3<T>(0.0, 1.0, 0.0));
+        const T sd = dot(sub(v[3], v[0]), nn);
+        const V3T<T> proj = sub(v[3], scl(sd, nn));
+        point = scl(0.5, add(proj, v[3]));
+        nrm = val(sd) >= 0.0 ? nn : scl(-1.0, nn);
+    } else if (kind == 1) {
+        V3T<T> c0, c1;
+        closest_points_t(v[0], v[1], v[2], v[3], c0, c1);
+        point = scl(0.5, add(c0, c1));
+        nrm = sub(c1, c0);
+    } else {
+        point = scl(0.5, add(v[0], v[1]));
+        nrm = sub(v[1], v[0]);
+    }
+    const V3T<T> ab = sub(pB, pA);
+    normal = safe_normalized_t(nrm, ab);
+    if (val(dot(normal, ab)) < 0.0) normal = scl(-1.0, normal);
 }
 
 // Joint gradient J^T (mu C + lambda) (Joint.cpp:21-45) and the constraint C, as

@@ -1296,13 +1296,15 @@ __global__ void __launch_bounds__(kBlock) k_np_scan(Dev d, Work w, int vv, int g
 // append_contact (IPCSolver.cpp:431-469 / 651-695): orient n toward pB - pA (with
 // safeNormalized fallback), distance = max(0, d).
 __device__ void write_contact(const Dev& d, const Work& w, int s, int pos, int A, int B, int type, int pa, int pb,
-                              V3 pt, V3 nrm, double dist) {
+                              V3 pt, V3 nrm, double dist, int pa2, int pb2) {
     const V3 pAp = ld3(w.pose_p, A), pBp = ld3(w.pose_p, B);
     V3 n = safe_normalized(nrm, pBp - pAp);
     if (dot(n, pBp - pAp) < 0.0) n = -n;
     const long long c = (long long)s * kMaxRawContacts + pos;
     int* ci = w.cint + 5 * c;
     ci[0] = A; ci[1] = B; ci[2] = type; ci[3] = pa; ci[4] = pb;
+    w.cext[2 * c] = pa2;
+    w.cext[2 * c + 1] = pb2;
     double* cd = w.cdbl + 7 * c;
     cd[0] = pt.x; cd[1] = pt.y; cd[2] = pt.z;
     cd[3] = n.x; cd[4] = n.y; cd[5] = n.z;
@@ -1343,7 +1345,7 @@ __global__ void __launch_bounds__(kBlock) k_np_write(Dev d, Work w, int vv, int 
                 const int i = c / kb, j = c % kb;
                 const V3 a = g.vA[i], b = g.vB[j];
                 const V3 dd = b - a;
-                write_contact(d, w, s, pos, g.A, g.B, 0, g.lvA[i], g.lvB[j], 0.5 * (a + b), dd, norm(dd));
+                write_contact(d, w, s, pos, g.A, g.B, 0, g.lvA[i], g.lvB[j], 0.5 * (a + b), dd, norm(dd), -2, -2);
             } else if (c < 1024) {
                 const int kb = g.k[1];
                 const int i = c / kb, j = c % kb;
@@ -1351,7 +1353,7 @@ __global__ void __launch_bounds__(kBlock) k_np_write(Dev d, Work w, int vv, int 
                 V3 proj;
                 fv_test(g.fA[i], g.nA[i], g.vB[j], g.dhat, sd, proj);
                 write_contact(d, w, s, pos, g.A, g.B, 0, g.lfA[i], g.lvB[j], 0.5 * (proj + g.vB[j]),
-                              (sd >= 0.0) ? g.nA[i] : -g.nA[i], fabs(sd));
+                              (sd >= 0.0) ? g.nA[i] : -g.nA[i], fabs(sd), -1, -1);
             } else if (c < 2048) {
                 const int cc = c - 1024;
                 const int ka = g.k[0];
@@ -1360,7 +1362,7 @@ __global__ void __launch_bounds__(kBlock) k_np_write(Dev d, Work w, int vv, int 
                 V3 proj;
                 fv_test(g.fB[i], g.nB[i], g.vA[j], g.dhat, sd, proj);
                 write_contact(d, w, s, pos, g.B, g.A, 0, g.lfB[i], g.lvA[j], 0.5 * (proj + g.vA[j]),
-                              (sd >= 0.0) ? g.nB[i] : -g.nB[i], fabs(sd));
+                              (sd >= 0.0) ? g.nB[i] : -g.nB[i], fabs(sd), -1, -1);
             } else {
                 const int cc = c - 2048;
                 const int kb = g.k[3];
@@ -1369,7 +1371,7 @@ __global__ void __launch_bounds__(kBlock) k_np_write(Dev d, Work w, int vv, int 
                 closest_points(g.eA[i][0], g.eA[i][1], g.eB[j][0], g.eB[j][1], c0, c1);
                 const V3 delta = c1 - c0;
                 write_contact(d, w, s, pos, g.A, g.B, 1, g.leA[i][0], g.leB[j][0], 0.5 * (c0 + c1), delta,
-                              norm(delta));
+                              norm(delta), g.leA[i][1], g.leB[j][1]);
             }
         }
         __syncthreads();
@@ -1449,16 +1451,19 @@ __global__ void __launch_bounds__(1024) k_np_cap(Dev d, Work w, int gate) {
     const long long cb = (long long)s * kMaxRawContacts;
     // gather the kept contacts into sort_key (free once the order is known), then copy back
     int* ti = reinterpret_cast<int*>(w.sort_key + (long long)s * kSceneScratch);  // 2000 * 5 ints
+    int* te = ti + 5 * kMaxContacts;                                                  // 2000 * 2 ints
     double* td = w.sort_key + (long long)s * kSceneScratch + 8192;                   // 2000 * 7 doubles
     for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
         const int src = kept[i];
         for (int k = 0; k < 7; ++k) td[7 * i + k] = w.cdbl[7 * (cb + src) + k];
         for (int k = 0; k < 5; ++k) ti[5 * i + k] = w.cint[5 * (cb + src) + k];
+        for (int k = 0; k < 2; ++k) te[2 * i + k] = w.cext[2 * (cb + src) + k];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
         for (int k = 0; k < 7; ++k) w.cdbl[7 * (cb + i) + k] = td[7 * i + k];
         for (int k = 0; k < 5; ++k) w.cint[5 * (cb + i) + k] = ti[5 * i + k];
+        for (int k = 0; k < 2; ++k) w.cext[2 * (cb + i) + k] = te[2 * i + k];
     }
     if (threadIdx.x == 0) sp.n_contacts = kMaxContacts;
 }

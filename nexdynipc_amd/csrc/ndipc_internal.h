@@ -187,6 +187,7 @@ struct Work {
     int* raw_cnt;            // [S]
     int* cint;               // [S * 40000 * 5]
     double* cdbl;            // [S * 40000 * 7]
+    int* cext;               // [S * 40000 * 2] second edge vertices (EE); -1 -1 FV, -2 -2 VV
     int* sort_idx;           // [S * kSceneScratch] scratch for the 2000-cap sort
     double* sort_key;        // [S * kSceneScratch] sort keys / Newton scratch
     // solver vectors, 6 per body
@@ -256,6 +257,7 @@ struct Adj {
     SceneParams* s_sp;      // [L*A][S] (alm_done = 1: the scene did not solve in that slot)
     int* s_cint;            // [L*A][S*kMaxContacts*5]
     double* s_cdbl;         // [L*A][S*kMaxContacts*7]
+    int* s_cext;            // [L*A][S*kMaxContacts*2]
     double* s_nf;           // [L*A][S*kMaxContacts]
     double* s_jl;           // [L*A][6J] multipliers before the solve
     double* s_jm;           // [L*A][J] stiffness of the solve
@@ -273,6 +275,7 @@ struct Adj {
     double* tmp;            // [6N]
     double* kry;            // [25][6N] Krylov basis of the transposed solve
     double* cvjp;           // [S*kMaxContacts][15]
+    double* gvjp;           // [S*kMaxContacts][14] cotangent through the contact geometry
     double* ejac_c;         // [S*kMaxContacts][144] exact friction Jacobian (h^2), transposed
     double* ejac_j;         // [J][144] exact hinge Jacobian, transposed
     double* ye_c;           // [S*kMaxContacts][12] element products of a matrix-vector product

@@ -2914,6 +2914,7 @@ __global__ void __launch_bounds__(kBlk) k_adj_snapshot(Dev d, Work w, Adj a, int
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
         for (int k = 0; k < 5; ++k) a.s_cint[5 * (so + i) + k] = w.cint[5 * (cb + i) + k];
         for (int k = 0; k < 7; ++k) a.s_cdbl[7 * (so + i) + k] = w.cdbl[7 * (cb + i) + k];
+        for (int k = 0; k < 2; ++k) a.s_cext[2 * (so + i) + k] = w.cext[2 * (cb + i) + k];
         a.s_nf[so + i] = w.nf[(long long)s * kMaxContacts + i];
     }
     for (int j = d.joint_off[s] + threadIdx.x; j < d.joint_off[s + 1]; j += blockDim.x) {
@@ -3358,6 +3359,112 @@ __device__ int adj_solve_direct(const Dev& d, const Work& w, const Adj& a, const
 // update's mu_a (dC/dx)^T lbar; eta = A_a^-T xbar_a; z and lambda_{a-1} receive -eta^T
 // dg_a/dz and -eta^T dg_a/dlambda. The contact set, lagged normal forces, barrier
 // stiffness and the hardening decisions are the solves' data and are held fixed.
+// b'(d) over dual numbers: its derivative is b''(d) d'.
+template <int NV>
+__device__ du::Dn<NV> bar_g_dn(const du::Dn<NV>& dd, double dh, double k) {
+    du::Dn<NV> r;
+    r.v = bar_g(dd.v, dh, k);
+    const double h = bar_h(dd.v, dh, k);
+    for (int i = 0; i < NV; ++i) r.d[i] = h * dd.d[i];
+    return r;
+}
+
+// Pose of body b as dual numbers seeded at slots O.. : at state xp (x_p at O..O+2, theta at
+// O+3..O+5; q = q_step_start updated by theta) or, with xp == nullptr, the step-start pose
+// (p at O..O+2, q at O+3..O+6). The slots are compile-time constants.
+template <int O>
+__device__ __forceinline__ void adj_geom_seed(const Dev& d, const double* xp, int b, du::V3T<du::Dn<14>>& P,
+                                              du::QT<du::Dn<14>>& Qr) {
+    if (xp) {
+        P = du::mk3(du::var<14>(xp[6 * b], O), du::var<14>(xp[6 * b + 1], O + 1), du::var<14>(xp[6 * b + 2], O + 2));
+        const du::V3T<du::Dn<14>> th = du::mk3(du::var<14>(xp[6 * b + 3], O + 3), du::var<14>(xp[6 * b + 4], O + 4),
+                                               du::var<14>(xp[6 * b + 5], O + 5));
+        Qr = du::rot_update(du::liftq<du::Dn<14>>(d.q + 4 * b), th, true);
+    } else {
+        P = du::mk3(du::var<14>(d.p[3 * b], O), du::var<14>(d.p[3 * b + 1], O + 1), du::var<14>(d.p[3 * b + 2], O + 2));
+        Qr = du::mkq(du::var<14>(d.q[4 * b], O + 3), du::var<14>(d.q[4 * b + 1], O + 4),
+                     du::var<14>(d.q[4 * b + 2], O + 5), du::var<14>(d.q[4 * b + 3], O + 6));
+    }
+}
+
+// Cotangent of a solve's objective through the geometry of its contact i. The contact's
+// point and normal, and in a solve after the first its lagged normal force, are functions
+// of the state the narrow phase ran at (the contact set itself is discrete and held
+// fixed): for a solve after the first the previous solve's x* (xp; 12 variables, the
+// Newton variables of A then B: out[7 side + k], k < 6), for the first the step-start
+// poses (xp == nullptr; 14 variables, p and q of A then B: out[7 side + k]). out = -h^2
+// d(eta_A . g_A + eta_B . g_B)/d(variables), g the contact's barrier and friction gradient
+// at the solve's x* (w.x).
+__device__ void adj_contact_geom(const Dev& d, const Work& w, const ScnCtx& sc, int i, const double* xp,
+                                 const double* ETA, int b0, double* out) {
+    typedef du::Dn<14> D;
+    const long long cc = sc.cb + i;
+    const long long ce = (long long)sc.s * kMaxContacts + i;
+    const int* ci = w.cint + 5 * cc;
+    const int A = ci[0], B = ci[1], type = ci[2], pa = ci[3], pb = ci[4];
+    const int pa2 = w.cext[2 * cc], pb2 = w.cext[2 * cc + 1];
+    const int bod[2] = {A, B};
+    du::V3T<D> P[2];
+    du::QT<D> Qr[2];
+    adj_geom_seed<0>(d, xp, A, P[0], Qr[0]);
+    adj_geom_seed<7>(d, xp, B, P[1], Qr[1]);
+    auto wv = [&](int side, int lv) {
+        const double* vl = d.verts + 3 * (d.v_off[bod[side]] + lv);
+        return du::add(P[side], du::rotate(Qr[side], du::lift3<D>(vl[0], vl[1], vl[2])));
+    };
+    du::V3T<D> v[4];
+    int kind = 0;
+    if (type == 1) {
+        kind = 1;
+        v[0] = wv(0, pa); v[1] = wv(0, pa2); v[2] = wv(1, pb); v[3] = wv(1, pb2);
+    } else if (pa2 == -2) {
+        kind = 2;
+        v[0] = wv(0, pa); v[1] = wv(1, pb);
+    } else {
+        const int* f = d.faces + 3 * (d.f_off[A] + pa);
+        v[0] = wv(0, f[0]); v[1] = wv(0, f[1]); v[2] = wv(0, f[2]); v[3] = wv(1, pb);
+    }
+    du::V3T<D> pt, nr;
+    du::contact_geom_t<D>(kind, v, P[0], P[1], pt, nr);
+    const double* x = w.x;
+    const int act = w.cactive[ce];
+    D g[12];
+    for (int k = 0; k < 12; ++k) g[k] = du::cst<14>(0.0);
+    if (act & 1) {  // barrier: -b'(d) n on A, b'(d) n on B, d = max(1e-9, (pB - pA).n) at x*
+        const D pr = (x[6 * B] - x[6 * A]) * nr.x + (x[6 * B + 1] - x[6 * A + 1]) * nr.y +
+                     (x[6 * B + 2] - x[6 * A + 2]) * nr.z;
+        const D dd = pr.v > 1e-9 ? pr : du::cst<14>(1e-9);
+        const D b1 = bar_g_dn(dd, sc.dhat, sc.kappa);
+        g[0] = -(b1 * nr.x); g[1] = -(b1 * nr.y); g[2] = -(b1 * nr.z);
+        g[6] = b1 * nr.x; g[7] = b1 * nr.y; g[8] = b1 * nr.z;
+    }
+    if (act & 2) {  // friction, N = |b'(d)| at the narrow phase's state (computeNormalForces)
+        D N = du::cst<14>(w.nf[ce]);
+        if (xp) {
+            const du::V3T<D> ab = du::sub(P[1], P[0]);
+            const D pr = du::dot(ab, nr);
+            const D dd = pr.v > 1e-9 ? pr : du::cst<14>(1e-9);
+            N = (dd.v < sc.dhat && dd.v > 0.0) ? du::dfabs(bar_g_dn(dd, sc.dhat, sc.kappa)) : du::cst<14>(0.0);
+        }
+        D xa[6], xb[6], gf[12];
+        for (int k = 0; k < 6; ++k) {
+            xa[k] = du::cst<14>(x[6 * A + k]);
+            xb[k] = du::cst<14>(x[6 * B + k]);
+        }
+        du::friction_grad_g<D>(xa, xb, du::lift3<D>(d.p[3 * A], d.p[3 * A + 1], d.p[3 * A + 2]),
+                               du::liftq<D>(d.q + 4 * A), du::lift3<D>(d.p[3 * B], d.p[3 * B + 1], d.p[3 * B + 2]),
+                               du::liftq<D>(d.q + 4 * B), pt, nr, N, du::cst<14>(sc.fmu), sc.feps, sc.fdt, gf);
+        for (int k = 0; k < 12; ++k) g[k] = g[k] + gf[k];
+    }
+    const double* lA = ETA + 6 * (A - b0);
+    const double* lB = ETA + 6 * (B - b0);
+    for (int k = 0; k < 14; ++k) {
+        double acc = 0.0;
+        for (int j = 0; j < 6; ++j) acc += lA[j] * g[j].d[k] + lB[j] * g[6 + j].d[k];
+        out[k] = -(acc * sc.h2);
+    }
+}
+
 __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int step, int gstep) {
     __shared__ double red[kBlk];
     __shared__ int ok_sh;
@@ -3429,6 +3536,7 @@ __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int ste
             for (int i = threadIdx.x; i < nc; i += blockDim.x) {
                 for (int k = 0; k < 5; ++k) w.cint[5 * (cb + i) + k] = a.s_cint[5 * (so + i) + k];
                 for (int k = 0; k < 7; ++k) w.cdbl[7 * (cb + i) + k] = a.s_cdbl[7 * (so + i) + k];
+                for (int k = 0; k < 2; ++k) w.cext[2 * (cb + i) + k] = a.s_cext[2 * (so + i) + k];
                 w.nf[(long long)s * kMaxContacts + i] = a.s_nf[so + i];
             }
             for (int j = d.joint_off[s] + threadIdx.x; j < d.joint_off[s + 1]; j += blockDim.x) {
@@ -3578,6 +3686,19 @@ __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int ste
             fi[4] = rgm;
         }
         const double* ETA = a.eta + 6 * (long long)b0;
+        // contact geometry and lagged normal forces: to x*_{a-1} or the step-start poses
+        {
+            const double* xp = it > 0 ? a.s_x + (slot - 1) * 6 * N : nullptr;
+            for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
+                const long long ce = (long long)s * kMaxContacts + i;
+                double* out = a.gvjp + 14 * ce;
+                if (!(w.cactive[ce] & 3)) {
+                    for (int k = 0; k < 14; ++k) out[k] = 0.0;
+                    continue;
+                }
+                adj_contact_geom(d, w, sc, i, xp, ETA, b0, out);
+            }
+        }
         // friction: lagged poses of both bodies and mu
         for (int i = threadIdx.x; i < sc.nc; i += blockDim.x) {
             const long long ce = (long long)s * kMaxContacts + i;
@@ -3670,6 +3791,13 @@ __global__ void __launch_bounds__(kBlk) k_adj_step(Dev d, Work w, Adj a, int ste
                 const double* cv = a.cvjp + 15 * ((long long)s * kMaxContacts + c) + 7 * side;
                 for (int k = 0; k < 3; ++k) a.bp[3 * b + k] += cv[k];
                 for (int k = 0; k < 4; ++k) a.bq[4 * b + k] += cv[3 + k];
+                const double* gv = a.gvjp + 14 * ((long long)s * kMaxContacts + c) + 7 * side;
+                if (it > 0) {
+                    for (int k = 0; k < 6; ++k) a.carry[6 * b + k] += gv[k];
+                } else {
+                    for (int k = 0; k < 3; ++k) a.bp[3 * b + k] += gv[k];
+                    for (int k = 0; k < 4; ++k) a.bq[4 * b + k] += gv[3 + k];
+                }
             }
             for (int j = sc.j0; j < sc.j1; ++j) {
                 if (d.ja[j] == b) for (int k = 0; k < 4; ++k) a.bq[4 * b + k] += a.jvjp[8 * (long long)j + k];

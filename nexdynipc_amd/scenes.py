@@ -546,22 +546,37 @@ def micro_contact_scene(seed=0, n_cubes=4, size=6e-4, gap=2e-4, dhat=1e-3, mu=0.
     return SceneSpec(bodies, dt=1e-3, dhat=dhat, mu=mu, name=f"micro{seed}")
 
 
-def friction_slide_scene(n_cubes=2, size=6e-4, gap=2e-4, mass=1e-3, vx=0.05, mu=0.4) -> Scenes:
-    """Millimetre cubes sliding along +x (and a little +z) just above a static plate,
-    centre separations below dhat, with masses raised to `mass` (1 mg by default, not
-    the density's 0.2 mg) so that their weight keeps the contact residual above the ALM
-    break tolerance (IPCSolver.cpp:1132-1165): the ALM loop then runs several iterations
-    and the lagged friction, zero in the first, acts in the later ones. This makes the
-    friction coefficient's gradient non-zero (a differentiation test scene)."""
+# Barrier stiffness of friction_slide_scene (set as initialised barrier state): with it the
+# cubes' weight is carried at a centre distance inside dhat, so they slide in contact.
+FRICTION_SLIDE_KAPPA = 20.0
+
+
+def friction_slide_scene(n_cubes=2, size=6e-4, gap=5e-5, mass=1e-3, vx=0.1, mu=0.4, spacing=3e-3,
+                         inertia_scale=1e6) -> Scenes:
+    """Millimetre cubes sliding along +x (and a little +z) on a static plate, their centres
+    closer than dhat to the plate's so that the reference's centre-distance barrier carries
+    them (the 50 um gap is about where it balances their weight), 3 mm apart (no cube-cube
+    contact) and away from the plate's face diagonals (no edge-edge contact whose closest
+    points sit on a crossing, where the Newton solve's last rounding error has a large
+    derivative). They slide at 0.1 and 0.2 m/s, fast enough that the friction stays far
+    from its smoothed stick region over the test's 10 steps. Masses are raised to `mass` (1 mg) so that the
+    contact residual stays above the ALM break tolerance (IPCSolver.cpp:1132-1165): the
+    ALM loop runs its iterations and the lagged friction, zero in the first, acts in the
+    later ones, with normal forces and contact points taken at the previous iterate. The
+    cubes' rotational inertia is scaled by `inertia_scale` so that they slide without
+    tipping. With the barrier stiffness FRICTION_SLIDE_KAPPA and a tight Newton tolerance
+    every solve converges or accepts no step, which makes the run the solution map that
+    the adjoint differentiates (a differentiation test scene for the friction
+    coefficient)."""
     bodies = [_box_body("mplate", 0.01, size, 0.01, [0.0, -0.5 * size, 0.0], static=True)]
     for k in range(n_cubes):
         bodies.append(_box_body(f"mcube{size}", size, size, size,
-                                [(k - 0.5 * (n_cubes - 1)) * (size + 4e-4), 0.5 * size + gap, 0.0],
+                                [(k - 0.5 * (n_cubes - 1)) * spacing, 0.5 * size + gap, 0.0],
                                 v=np.array([vx * (1 + k), 0.0, 0.3 * vx])))
     sc = pack([SceneSpec(bodies, dt=1e-3, dhat=1e-3, mu=mu, name="slide")])
     m0 = float(sc.mass[1])
     sc.mass[1:] = mass
-    sc.inertia.reshape(-1, 3, 3)[1:] *= mass / m0
+    sc.inertia.reshape(-1, 3, 3)[1:] *= mass / m0 * inertia_scale
     return sc
 
 

