@@ -318,8 +318,10 @@ __device__ void eval_elements(const Dev& d, const Work& w, const ScnCtx& sc, con
         }
         w.cactive[(long long)sc.s * kMaxContacts + i] = act;
     }
-    // hinge joint element blocks (mu J^T J) and gradients J^T (mu C + lambda)
-    for (int j = sc.j0 + threadIdx.x; j < sc.j1; j += blockDim.x) {
+    // hinge joint element blocks (mu J^T J) and gradients J^T (mu C + lambda); joints
+    // are taken from the last thread down, so that with fewer contacts than threads a
+    // joint is evaluated beside the contacts rather than after one
+    for (int j = sc.j0 + (blockDim.x - 1 - threadIdx.x); j < sc.j1; j += blockDim.x) {
         JEval e;
         joint_eval(d, j, X, e);
         const double mu = d.jmu[j];
@@ -1173,12 +1175,90 @@ __device__ bool ldlt6_solve(const double* A, const double* b, double* x) {
 // Returns 0 solved, 1 zero pivot, 2 a component exceeds kDenseBodies.
 constexpr int kSmallRows = 24;  // components of up to 4 bodies: one wavefront each
 constexpr int kSmallElems = kSceneScratch / 4;  // element list of each wavefront
+// Lower-triangle entries (i, j), j <= i, of an m x m matrix, m = 6, 12, 18, 24, as
+// (i << 8) | j ordered by j descending (then i ascending): the entries with j > k are the
+// first (m-k-1)(m-k)/2, the trailing triangle of elimination step k.
+struct SmallTri {
+    unsigned short v[21 + 78 + 171 + 300];
+};
+constexpr SmallTri make_small_tri() {
+    SmallTri t{};
+    int n = 0;
+    for (int mb = 1; mb <= 4; ++mb) {
+        const int m = 6 * mb;
+        for (int j = m - 1; j >= 1; --j)
+            for (int i = j; i < m; ++i) t.v[n++] = (unsigned short)((i << 8) | j);
+    }
+    return t;
+}
+__constant__ constexpr SmallTri kSmallTriTab = make_small_tri();
+__constant__ constexpr int kSmallTriOff[5] = {0, 0, 15, 15 + 66, 15 + 66 + 153};
+#define kSmallTri (kSmallTriTab.v)
 
 // Orders the wavefront's LDS accesses (no work-group barrier).
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lane l's double from lane `src` (a wavefront-uniform index), through two scalar reads.
+__device__ __forceinline__ double readlane_d(double v, int src) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// LDL^T factorization and solve of a 12 x 12 component (two bodies) with the matrix in
+// registers: lane i < 12 holds row i (its lower part, read from the LDS tile M), every
+// pivot and multiplier is broadcast from its lane by scalar reads, and each entry
+// receives the same subtractions in the same k-ascending order, with the same
+// expressions, as the LDS path (so the factor and the solution are bitwise those). x:
+// the right-hand side's row of the lane in, the solution's out. Returns false on a zero
+// or non-finite pivot.
+__device__ __noinline__ bool small12_ldlt_solve(const double* M, double& x) {
+    constexpr int m = 12;
+    const int lane = threadIdx.x & 63;
+    double a[m];
+#pragma unroll
+    for (int j = 0; j < m; ++j) a[j] = (lane < m && j <= lane) ? M[lane * m + j] : 0.0;
+#pragma unroll
+    for (int k = 0; k < m; ++k) {
+        const double Dk = readlane_d(a[k], k);
+        if (Dk == 0.0 || !isfinite(Dk)) return false;
+        const bool below = lane > k && lane < m;
+        const double l = below ? a[k] / Dk : 0.0;
+        if (below) a[k] = l;
+#pragma unroll
+        for (int j = k + 1; j < m; ++j) {
+            const double ljk = readlane_d(a[k], j);
+            if (lane == j) a[j] -= (l * l) * Dk;
+            else if (lane > j && lane < m) a[j] -= (l * Dk) * ljk;
+        }
+    }
+    // forward x_i -= L_ik x_k (k ascending), x_i /= D_i, backward x_k -= L_ik x_i
+    // (i descending)
+#pragma unroll
+    for (int k = 0; k < m; ++k) {
+        const double xk = readlane_d(x, k);
+        if (lane > k && lane < m) x -= a[k] * xk;
+    }
+    double dd = 1.0;
+#pragma unroll
+    for (int j = 0; j < m; ++j)
+        if (lane == j) dd = a[j];
+    if (lane < m) x /= dd;
+#pragma unroll
+    for (int i = m - 1; i >= 0; --i) {
+        const double xi = readlane_d(x, i);
+#pragma unroll
+        for (int c = 0; c < i; ++c) {
+            const double lic = readlane_d(a[c], i);
+            if (lane == c) x -= lic * xi;
+        }
+    }
+    return true;
 }
 
 // The component solve below for one component of at most 4 bodies (m <= kSmallRows),
@@ -1258,36 +1338,46 @@ __device__ bool small_component_solve(const Dev& d, const Work& w, const ScnCtx&
     }
     wave_sync();
     el += ne;
-    for (int e = 0; e < ne; ++e) {
-        const int code = el[e], kind = code >> 24, i = code & 0xffffff;
-        const int t = lane;
-        if (kind == 0) {
-            const int j = sc.j0 + i;
-            const int pa = lpos(d.ja[j] - sc.b0), pb = lpos(d.jb[j] - sc.b0);
-            const double* jb = w.jblk + 108 * j + 72;  // rows a, cols b
-            if (t < 36) {
-                const int r = t / 6, q = t % 6;
+    // Entry (r, q) of every 6x6 block is owned by lane 6 r + q, which adds the elements'
+    // terms to it in element order; the lanes share no entry, so the elements need no
+    // synchronisation between them and each lane's loads are issued back to back.
+    if (lane < 36) {
+        const int r = lane / 6, q = lane % 6;
+        for (int e = 0; e < ne; ++e) {
+            const int code = el[e], kind = code >> 24, i = code & 0xffffff;
+            if (kind == 0) {
+                const int j = sc.j0 + i;
+                const int pa = lpos(d.ja[j] - sc.b0), pb = lpos(d.jb[j] - sc.b0);
+                const double* jb = w.jblk + 108 * j + 72;  // rows a, cols b
                 if (pa > pb) M[(6 * pa + r) * m + 6 * pb + q] += jb[r * 6 + q];
                 else M[(6 * pb + r) * m + 6 * pa + q] += jb[q * 6 + r];
-            }
-        } else {
-            const long long cc = sc.cb + i, ce = (long long)sc.s * kMaxContacts + i;
-            const int pa = lpos(w.cint[5 * cc] - sc.b0), pb = lpos(w.cint[5 * cc + 1] - sc.b0);
-            const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
-            if (kind == 1) {
-                if (t < 9) {
-                    const int r = t / 3, q = t % 3;
-                    const double* cd = w.cdbl + 7 * cc;
-                    M[(6 * hi + r) * m + 6 * lo + q] += (-(b2arr[i] * (cd[3 + r] * cd[3 + q]))) * sc.h2;
+            } else {
+                const long long cc = sc.cb + i, ce = (long long)sc.s * kMaxContacts + i;
+                const int pa = lpos(w.cint[5 * cc] - sc.b0), pb = lpos(w.cint[5 * cc + 1] - sc.b0);
+                const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
+                if (kind == 1) {
+                    if (r < 3 && q < 3) {
+                        const double* cd = w.cdbl + 7 * cc;
+                        M[(6 * hi + r) * m + 6 * lo + q] += (-(b2arr[i] * (cd[3 + r] * cd[3 + q]))) * sc.h2;
+                    }
+                } else {
+                    const double* blk = w.cblk + 144 * ce + (pa > pb ? 72 : 108);
+                    M[(6 * hi + r) * m + 6 * lo + q] += blk[r * 6 + q];
                 }
-            } else if (t < 36) {
-                const int r = t / 6, q = t % 6;
-                const double* blk = w.cblk + 144 * ce + (pa > pb ? 72 : 108);
-                M[(6 * hi + r) * m + 6 * lo + q] += blk[r * 6 + q];
             }
         }
-        wave_sync();
     }
+    wave_sync();
+    if (m == 12) {  // two bodies: factor and solve in registers
+        double x = lane < m ? rhs[6 * L.list[c0 + lane / 6] + lane % 6] : 0.0;
+        if (!small12_ldlt_solve(M, x)) return false;
+        if (lane < m) xo[6 * L.list[c0 + lane / 6] + lane % 6] = x;
+        wave_sync();
+        return true;
+    }
+    // the trailing triangle of step k: the first (m-k-1)(m-k)/2 entries of the component
+    // size's table (entries ordered by column, descending)
+    const unsigned short* tri = kSmallTri + kSmallTriOff[mb];
     for (int k = 0; k < m; ++k) {
         const double Dk = M[k * m + k];
         if (Dk == 0.0 || !isfinite(Dk)) return false;
@@ -1298,10 +1388,9 @@ __device__ bool small_component_solve(const Dev& d, const Work& w, const ScnCtx&
             M[k * m + i] = l;
         }
         wave_sync();
-        const int r = m - k - 1;
-        for (int t = lane; t < r * r; t += 64) {
-            const int ii = k + 1 + t / r, j = k + 1 + t % r;
-            if (j > ii) continue;
+        const int cnt = (m - k - 1) * (m - k) / 2;
+        for (int t = lane; t < cnt; t += 64) {
+            const int ij = tri[t], ii = ij >> 8, j = ij & 255;
             const double lik = M[ii * m + k], ljk = M[j * m + k];
             if (ii == j) M[ii * m + ii] -= (lik * lik) * Dk;
             else M[ii * m + j] -= (lik * Dk) * ljk;
