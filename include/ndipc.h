@@ -297,13 +297,17 @@ int32_t ndipc_timing_enable(ndipc_ctx* ctx, int32_t on);
  * kernels one by one (on = 0); both run the same kernels with the same arguments. No
  * reference counterpart (host launch strategy). */
 int32_t ndipc_use_graph(ndipc_ctx* ctx, int32_t on);
-/* Adaptive ALM rounds (adaptive = 1, default): each step issues ALM rounds only until
- * every scene's ALM loop has ended (a round reports to pinned host memory), at most
- * `lookahead` rounds ahead of the last one reported; adaptive = 0 enqueues all
- * alm_max_iters rounds every step (scenes that finished skip them on the device). Same
- * results either way. The adjoint's recorded forward always runs the fixed rounds.
- * Environment at create: NDIPC_FIXED_ROUNDS=1, NDIPC_ROUND_LOOKAHEAD=n. No reference
- * counterpart (IPCSolver.cpp:965-1195 loops on the host). */
+/* Step modes. adaptive = 1: each step issues ALM rounds only until every scene's ALM
+ * loop has ended (a round reports to pinned host memory), at most `lookahead` rounds
+ * ahead of the last one reported. adaptive = 2 (default): as 1, and a call of several
+ * steps on a batch of several scenes steps the scenes asynchronously: a scene whose ALM
+ * loop has ended writes its step back and starts its next step in the next round instead
+ * of waiting for the slowest scene (a batch without line-search helpers; otherwise as 1).
+ * adaptive = 0 enqueues all alm_max_iters rounds every step (scenes that finished skip
+ * them on the device). Same results in every mode. The adjoint's recorded forward always
+ * runs the fixed rounds. Environment at create: NDIPC_FIXED_ROUNDS=1,
+ * NDIPC_ROUND_LOOKAHEAD=n, NDIPC_ASYNC_STEPS=0 (mode 1). No reference counterpart
+ * (IPCSolver.cpp:965-1195 loops on the host, one scene at a time). */
 int32_t ndipc_set_step_mode(ndipc_ctx* ctx, int32_t adaptive, int32_t lookahead);
 /* Autotuner of the launch shapes (work-group counts of the persistent top-K, narrow-phase,
  * scene-min-distance and CCD kernels) for the uploaded batch: from the current state,

@@ -315,9 +315,12 @@ class Engine:
         self._ck(self.L.ndipc_solver_counters(self.ctx, ptr(o, f64p)), "solver_counters")
         return o
 
-    def set_step_mode(self, adaptive=True, lookahead=1):
-        """Adaptive ALM rounds (issue only the rounds the scenes need) or fixed rounds."""
-        self._ck(self.L.ndipc_set_step_mode(self.ctx, 1 if adaptive else 0, int(lookahead)), "set_step_mode")
+    def set_step_mode(self, adaptive=True, lookahead=1, asynchronous=True):
+        """Adaptive ALM rounds (issue only the rounds the scenes need) or fixed rounds;
+        with asynchronous, a multi-step call on a batch lets each scene start its next
+        step as soon as its ALM loop ends."""
+        mode = (2 if asynchronous else 1) if adaptive else 0
+        self._ck(self.L.ndipc_set_step_mode(self.ctx, mode, int(lookahead)), "set_step_mode")
 
     def helper_counters(self):
         """Per scene: helpers on (0/1), and since upload the batches the helpers answered,

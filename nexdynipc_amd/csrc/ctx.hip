@@ -77,6 +77,12 @@ struct ndipc_ctx {
     bool adaptive = true;
     int round_lookahead = 1;          // rounds issued beyond the last one reported
     hipGraphExec_t pre_exec = nullptr, round_exec = nullptr;
+    // asynchronous stepping of a batch (adaptive mode, several scenes, several steps): each
+    // scene starts its next step as soon as its ALM loop ends; one captured iteration runs
+    // the write-back of the scenes that ended, the prefix for the scenes starting a step and
+    // one ALM round for every scene inside one
+    bool async_steps = true;
+    hipGraphExec_t async_exec = nullptr;
     unsigned int* hflag = nullptr;    // pinned host memory: [round number, all scenes done]
     unsigned int rounds_issued = 0;   // rounds with a flag issued since upload
     GrowBuf ipc_buf[8];               // ndipc_ipc_step's work space
@@ -173,7 +179,8 @@ void drop_graph(ndipc_ctx* ctx) {
     if (ctx->step_exec) (void)hipGraphExecDestroy(ctx->step_exec);
     if (ctx->pre_exec) (void)hipGraphExecDestroy(ctx->pre_exec);
     if (ctx->round_exec) (void)hipGraphExecDestroy(ctx->round_exec);
-    ctx->step_exec = ctx->pre_exec = ctx->round_exec = nullptr;
+    if (ctx->async_exec) (void)hipGraphExecDestroy(ctx->async_exec);
+    ctx->step_exec = ctx->pre_exec = ctx->round_exec = ctx->async_exec = nullptr;
     ctx->graph_alm = -1;
 }
 
@@ -354,6 +361,7 @@ int32_t ndipc_create(int32_t device, ndipc_ctx** out) {
     c->hflag[0] = c->hflag[1] = 0;
     if (const char* env = std::getenv("NDIPC_FIXED_ROUNDS")) c->adaptive = env[0] != '1';
     if (const char* env = std::getenv("NDIPC_ROUND_LOOKAHEAD")) c->round_lookahead = std::max(0, std::atoi(env));
+    if (const char* env = std::getenv("NDIPC_ASYNC_STEPS")) c->async_steps = env[0] != '0';
     *out = c;
     return 0;
 }
@@ -986,6 +994,53 @@ static int run_step_adaptive(ndipc_ctx* ctx, int alm_max, bool graph) {
     return 0;
 }
 
+// One iteration of asynchronous stepping: the write-back and state change of the scenes
+// whose ALM loop ended in the previous iteration, the step prefix (its per-step kernels act
+// only on the scenes starting a step; the candidate build, a function of each scene's
+// step-start state, runs for all) and one ALM round with its flag. A scene's kernels and
+// their inputs are those of the lockstep step, so its results are the same bits.
+static int enqueue_async_iter(ndipc_ctx* ctx) {
+    ctx->w.as_mode = 1;  // kernel arguments are copied at launch
+    nd::launch_async_advance(ctx->d, ctx->w, ctx->stream);
+    int rc = enqueue_prefix(ctx, nullptr, 0);
+    if (!rc) rc = enqueue_round(ctx, nullptr, 0, 0, true);
+    ctx->w.as_mode = 0;
+    return rc;
+}
+
+// n_steps steps of every scene, each scene advancing on its own: iterations are issued
+// ahead of the device by `round_lookahead` until one reports that every scene is in the
+// last round of its last step; then the final write-back. A step takes at most alm_max
+// rounds, so n_steps * alm_max iterations always suffice.
+static int run_steps_async(ndipc_ctx* ctx, int n_steps, int alm_max, bool graph) {
+    hipStream_t st = ctx->stream;
+    if (graph && !ctx->async_exec)
+        if (capture(ctx, &ctx->async_exec, [&] { return enqueue_async_iter(ctx); })) return -1;
+    nd::launch_async_begin(ctx->d, ctx->w, n_steps, st);
+    const long long cap = (long long)n_steps * alm_max;
+    const unsigned int base = ctx->rounds_issued;
+    long long issued = 0;
+    auto issue = [&]() -> int {
+        if (graph) HIPCHK(hipGraphLaunch(ctx->async_exec, st));
+        else if (enqueue_async_iter(ctx)) return -1;
+        ++issued;
+        ++ctx->rounds_issued;
+        return 0;
+    };
+    while (issued < std::min(cap, 1LL + ctx->round_lookahead))
+        if (issue()) return -1;
+    for (long long r = 0; r < cap; ++r) {
+        int done = 0;
+        if (wait_round(ctx, base + unsigned(r) + 1u, &done)) return -1;
+        if (done) break;
+        if (issued < cap && issue()) return -1;
+    }
+    ctx->w.as_mode = 1;
+    nd::launch_async_advance(ctx->d, ctx->w, st);
+    ctx->w.as_mode = 0;
+    return 0;
+}
+
 // The step sequence of IPCSolver::step for every scene, n_steps times. When recording
 // for the adjoint, or in fixed-round mode, every step enqueues all alm_max rounds (the
 // whole step replayed from one graph, or kernel by kernel with timing or graphs off);
@@ -995,6 +1050,14 @@ static int run_steps(ndipc_ctx* ctx, int n_steps, const nd::Adj* rec) {
     int alm_max = 1;
     for (auto& q : ctx->sp) alm_max = std::max(alm_max, q.alm_max_iters);
     const bool graph = !rec && ctx->use_graph && !ctx->timing;
+    // asynchronous stepping pays for the prefix every round, which only several scenes
+    // taking several steps recover; the line-search helpers keep per-step protocols
+    if (!rec && ctx->adaptive && ctx->async_steps && ctx->S > 1 && n_steps > 1 && !ctx->w.newton_helpers) {
+        g_timing_ctx = ctx;
+        if (run_steps_async(ctx, n_steps, alm_max, graph)) return -1;
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     const bool have = ctx->adaptive ? ctx->pre_exec && ctx->round_exec : ctx->step_exec && ctx->graph_alm == alm_max;
     if (graph && !have)
         if (capture_step(ctx, alm_max)) return -1;
@@ -1861,8 +1924,10 @@ int32_t ndipc_set_step_mode(ndipc_ctx* ctx, int32_t adaptive, int32_t lookahead)
     if (check_ctx(ctx)) return -1;
     if (lookahead < 0) return fail(ctx, "set_step_mode: lookahead must be >= 0");
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (adaptive < 0 || adaptive > 2) return fail(ctx, "set_step_mode: mode must be 0, 1 or 2");
     if ((adaptive != 0) != ctx->adaptive) drop_graph(ctx);
     ctx->adaptive = adaptive != 0;
+    ctx->async_steps = adaptive == 2;
     ctx->round_lookahead = lookahead;
     return 0;
 }

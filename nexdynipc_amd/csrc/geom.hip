@@ -1041,6 +1041,7 @@ __global__ void k_pose(Dev d, Work w, const double* x, int gate) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.N) return;
     if (gate && d.sp[d.body_scene[b]].alm_done) return;
+    if (!x && as_skip(w, d.sp[d.body_scene[b]])) return;
     if (!x) {
         st3(w.pose_p, b, ld3(d.p, b));
         st4(w.pose_q, b, ld4(d.q, b));
@@ -1229,7 +1230,7 @@ __global__ void __launch_bounds__(kBlock) k_np_count(Dev d, Work w, int vv, int 
     for (int p = blockIdx.x; p < total; p += gridDim.x) {
         const int s = w.pair_scene[p];
         const SceneParams& sp = d.sp[s];
-        if (gate && sp.alm_done) continue;
+        if (gate ? sp.alm_done : as_skip(w, sp)) continue;
         if (vv && !sp.vv_needed) continue;  // VV only when no FV/EE contact (IPCSolver.cpp:546)
         if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
         load_pair_geom(d, w, p, g);
@@ -1259,7 +1260,7 @@ __global__ void __launch_bounds__(kBlock) k_np_scan(Dev d, Work w, int vv, int g
     __shared__ double shd[kBlock];
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
-    if (gate && sp.alm_done) return;
+    if (gate ? sp.alm_done : as_skip(w, sp)) return;
     if (vv && !sp.vv_needed) return;
     const int p0 = w.pair_off[s], p1 = w.pair_off[s + 1];
     int run = 0;
@@ -1317,7 +1318,7 @@ __global__ void __launch_bounds__(kBlock) k_np_write(Dev d, Work w, int vv, int 
     const int total = *w.n_pairs_total;
     for (int p = blockIdx.x; p < total; p += gridDim.x) {
         const int s = w.pair_scene[p];
-        if (gate && d.sp[s].alm_done) continue;
+        if (gate ? d.sp[s].alm_done : as_skip(w, d.sp[s])) continue;
         if (vv && (!d.sp[s].vv_needed || w.hit_fv[p] == 0)) continue;
         if (!vv && w.hit_fv[p] + w.hit_ee[p] == 0) continue;
         load_pair_geom(d, w, p, g);
@@ -1399,7 +1400,7 @@ __global__ void __launch_bounds__(kTkLanes) k_np_cap_order(Dev d, Work w, int ga
     __shared__ int stk[3 * 64];
     const int s = blockIdx.x;
     const SceneParams& sp = d.sp[s];
-    if (gate && sp.alm_done) return;
+    if (gate ? sp.alm_done : as_skip(w, sp)) return;
     const int n = w.raw_cnt[s];
     if (n <= kMaxContacts) return;
     const int lane = threadIdx.x;
@@ -1440,7 +1441,7 @@ __global__ void __launch_bounds__(kTkLanes) k_np_cap_order(Dev d, Work w, int ga
 __global__ void __launch_bounds__(1024) k_np_cap(Dev d, Work w, int gate) {
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
-    if (gate && sp.alm_done) return;
+    if (gate ? sp.alm_done : as_skip(w, sp)) return;
     const int n = w.raw_cnt[s];
     if (n <= kMaxContacts) {
         if (threadIdx.x == 0) sp.n_contacts = n;
@@ -1532,7 +1533,7 @@ __global__ void __launch_bounds__(kBlock) k_scene_min_flag(Dev d, Work w, int un
         const int s = w.prox_scene[p];
         const SceneParams& sp = d.sp[s];
         if (!ungated && !(sp.min_pair_dist < sp.dhat * 2.0)) continue;
-        if (w.flag[s]) continue;
+        if (w.flag[s] || as_skip(w, sp)) continue;
         const int I = w.prox_pairs[2 * p], J = w.prox_pairs[2 * p + 1];
         const V3 pI = ld3(d.p, I), pJ = ld3(d.p, J);
         const Q qI = ld4(d.q, I), qJ = ld4(d.q, J);
@@ -1910,6 +1911,7 @@ __global__ void k_ccd_fv(Dev d, Work w, int gate_any) {
         const int s = w.pair_scene[p];
         SceneParams& sp = d.sp[s];
         if (gate_any && !(sp.enable_ccd && sp.ccd_any)) continue;
+        if (as_skip(w, sp)) continue;
         const int local = chunk * 64 + lane;
         if (local >= n1 + n2) continue;
         const int A = w.pairs[2 * p], B = w.pairs[2 * p + 1];

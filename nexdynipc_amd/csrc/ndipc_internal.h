@@ -95,7 +95,10 @@ struct SceneParams {
     double evals_total; // candidate evaluations since upload
     // Newton work counters of the last step
     int ls_trials, solves_direct, solves_pcg, pcg_iters;
-    int comp_max, pad2, pad3, pad4;
+    int comp_max;
+    // asynchronous stepping of a batch: 0 the scene starts a step, 1 it is inside one, 2 it
+    // has taken all its steps; the steps it has still to start or finish
+    int as_state, as_steps_left, pad4;
     double newton_total, trials_total;  // since upload
     double newton_tol;                  // NewtonSolver tolerance (NewtonSolver.cpp:10-95: 1e-6)
     int newton_max_iters, pad5;         // NewtonSolver iteration cap (20)
@@ -160,6 +163,10 @@ struct Work {
     // Newton line-search helpers: 1 when every scene gets kHelpers helper work-groups
     // beside its Newton work-group (S * (1 + kHelpers) fit the CUs at one per CU)
     int newton_helpers;
+    // asynchronous stepping (a copy of Work passed to the kernels of a batch iteration):
+    // the prefix's stateful kernels act only on scenes starting a step (as_state 0), the
+    // write-back only on scenes whose ALM loop ended (as_state 1, alm_done)
+    int as_mode;
     HelperBox* hbox;         // [S]
     HelperCfg* hcfg;         // [1] wait bounds of the helper protocol
     int* hsingle;            // [N] with helpers: 1 if the body is alone in its component
@@ -306,6 +313,11 @@ void launch_newton_helpers(const Dev& d, const Work& w, hipStream_t st);
 void launch_helpers_close(const Dev& d, const Work& w, hipStream_t st);
 void launch_alm_post(const Dev& d, const Work& w, hipStream_t st);
 void launch_finalize(const Dev& d, const Work& w, hipStream_t st);
+// asynchronous stepping: every scene has n_steps steps to take, starting one
+void launch_async_begin(const Dev& d, const Work& w, int n_steps, hipStream_t st);
+// asynchronous stepping: the write-back of the scenes whose ALM loop ended, then their
+// state (the next step, or done)
+void launch_async_advance(const Dev& d, const Work& w, hipStream_t st);
 void launch_round_flag(const Dev& d, const Work& w, unsigned int* hflag, hipStream_t st);
 void launch_barrier_init_ccd(const Dev& d, const Work& w, hipStream_t st);
 // which 0: DistanceBarrierForm, 1: FrictionForm, on the contacts stored for scene s
@@ -409,6 +421,10 @@ __device__ __forceinline__ double allowed_mu(const Dev& d, int s, int ida, int i
     return -1.0;
 }
 // World position of body-local vertex lv of global body b at pose (p, q).
+// Kernels of the step prefix that change a scene's per-step state skip, in asynchronous
+// stepping, the scenes that are inside a step or done.
+__device__ __forceinline__ bool as_skip(const Work& w, const SceneParams& sp) { return w.as_mode && sp.as_state != 0; }
+
 __device__ __forceinline__ V3 world_vertex(const Dev& d, int b, int lv, V3 p, Q q) {
     return p + rotate(q, ld3(d.verts, d.v_off[b] + lv));
 }

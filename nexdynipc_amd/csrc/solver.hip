@@ -2392,6 +2392,7 @@ __device__ void build_csr(const Work& w, const ScnCtx& sc) {
 __global__ void __launch_bounds__(kBlk) k_contacts_setup(Dev d, Work w) {
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
+    if (as_skip(w, sp)) return;
     const bool needs = sp.enable_adaptive || (sp.enable_friction && sp.mu > 0.0);
     if (threadIdx.x == 0) {
         if (!needs) sp.n_contacts = 0;
@@ -2426,6 +2427,7 @@ __global__ void __launch_bounds__(kBlk) k_contacts_setup(Dev d, Work w) {
     ScnCtx sc;
     load_scn(d, s, sc);
     build_csr(w, sc);
+    if (w.as_mode && threadIdx.x == 0) sp.as_state = 1;  // inside the step
 }
 
 void launch_contacts_setup(const Dev& d, const Work& w, hipStream_t st) {
@@ -2505,7 +2507,12 @@ __global__ void __launch_bounds__(256) k_round_flag(Dev d, Work w, unsigned int*
     if (threadIdx.x == 0) all = 1;
     __syncthreads();
     int mine = 1;
-    for (int s = threadIdx.x; s < d.S; s += blockDim.x) mine &= d.sp[s].alm_done;
+    for (int s = threadIdx.x; s < d.S; s += blockDim.x) {
+        const SceneParams& sp = d.sp[s];
+        // asynchronous stepping: every scene done, or in the ALM loop end of its last step
+        mine &= w.as_mode ? (sp.as_state == 2 || (sp.as_state == 1 && sp.alm_done && sp.as_steps_left <= 1))
+                          : sp.alm_done;
+    }
     if (!mine) atomicAnd(&all, 0);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -2525,6 +2532,7 @@ __global__ void k_step_begin(Dev d, Work w, const double* dt) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= d.S) return;
     SceneParams& sp = d.sp[s];
+    if (as_skip(w, sp)) return;
     sp.dt = dt[s];
     sp.dhat_cand = sp.dhat;  // candidates use the barrier's dhat before its first initialisation
     sp.min_pair_dist = 1.0 / 0.0;
@@ -2608,6 +2616,7 @@ __global__ void k_kappa_update(Dev d, Work w) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= d.S) return;
     SceneParams& sp = d.sp[s];
+    if (as_skip(w, sp)) return;
     if (!sp.enable_adaptive) return;
     if (!(sp.min_pair_dist < sp.dhat * 2.0)) return;
     if (w.flag[s] && sp.kappa < 1e20) {
@@ -2627,6 +2636,7 @@ void launch_kappa(const Dev& d, const Work& w, hipStream_t st) {
 __global__ void __launch_bounds__(kBlk) k_ccd_dt(Dev d, Work w) {
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
+    if (as_skip(w, sp)) return;
     long long fv = 0, ee = 0;
     for (int p = w.pair_off[s] + threadIdx.x; p < w.pair_off[s + 1]; p += blockDim.x) {
         const int* k = w.lcnt + 6 * p;
@@ -2667,6 +2677,7 @@ void launch_ccd_dt(const Dev& d, const Work& w, hipStream_t st) {
 __global__ void k_pack(Dev d, Work w) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.N) return;
+    if (as_skip(w, d.sp[d.body_scene[b]])) return;
     const double h = d.sp[d.body_scene[b]].adt;
     const double xv[6] = {d.p[3 * b], d.p[3 * b + 1], d.p[3 * b + 2], 0.0, 0.0, 0.0};
     const double vv[6] = {d.v[3 * b], d.v[3 * b + 1], d.v[3 * b + 2], d.w[3 * b], d.w[3 * b + 1], d.w[3 * b + 2]};
@@ -2687,6 +2698,10 @@ void launch_pack(const Dev& d, const Work& w, hipStream_t st) {
 __global__ void k_finalize(Dev d, Work w) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.N) return;
+    if (w.as_mode) {
+        const SceneParams& sp = d.sp[d.body_scene[b]];
+        if (!(sp.as_state == 1 && sp.alm_done)) return;
+    }
     const double h = d.sp[d.body_scene[b]].adt;
     double vn[6], an[6];
     for (int k = 0; k < 6; ++k) {
@@ -2704,6 +2719,34 @@ __global__ void k_finalize(Dev d, Work w) {
 
 void launch_finalize(const Dev& d, const Work& w, hipStream_t st) {
     hipLaunchKernelGGL(k_finalize, dim3((d.N + 255) / 256), dim3(256), 0, st, d, w);
+}
+
+__global__ void k_async_begin(Dev d, int n_steps) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.S) return;
+    d.sp[s].as_state = 0;
+    d.sp[s].as_steps_left = n_steps;
+}
+
+void launch_async_begin(const Dev& d, const Work& w, int n_steps, hipStream_t st) {
+    (void)w;
+    hipLaunchKernelGGL(k_async_begin, dim3((d.S + 255) / 256), dim3(256), 0, st, d, n_steps);
+}
+
+// After the write-back of a scene whose ALM loop ended: its next step, or done.
+__global__ void k_async_state(Dev d) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.S) return;
+    SceneParams& sp = d.sp[s];
+    if (sp.as_state == 1 && sp.alm_done) {
+        sp.as_steps_left -= 1;
+        sp.as_state = sp.as_steps_left > 0 ? 0 : 2;
+    }
+}
+
+void launch_async_advance(const Dev& d, const Work& w, hipStream_t st) {
+    launch_finalize(d, w, st);
+    hipLaunchKernelGGL(k_async_state, dim3((d.S + 255) / 256), dim3(256), 0, st, d);
 }
 
 void launch_barrier_init_ccd(const Dev& d, const Work& w, hipStream_t st) {

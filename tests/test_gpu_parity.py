@@ -366,6 +366,42 @@ def test_adaptive_rounds_match_fixed_rounds(engine_factory, name):
             assert np.array_equal(e.joint_state()[0], fixed.joint_state()[0])
 
 
+@pytest.mark.parametrize("name", ["c3x48", "c4x12", "c5x32"])
+def test_asynchronous_steps_match_lockstep_steps(engine_factory, name):
+    """A multi-step call on a batch without line-search helpers steps the scenes
+    asynchronously: a scene whose ALM loop has ended writes its step back and starts the
+    next one in the next round while slower scenes finish theirs. Against lockstep steps
+    (every scene waits for the slowest): same bits, statistics, solver counters and joint
+    multipliers after every call, with graphs and kernel by kernel, one iteration or none
+    issued ahead, and calls of 1 (lockstep), 2, 5 and 7 steps. c4x12 is twelve ramp
+    scenes with a hinged chain and a pinned link (ALM rounds on the joints)."""
+    if name == "c3x48":
+        sc = scenes.pack(scenes.config3(n_scenes=48))
+    elif name == "c4x12":
+        sc = scenes.pack([scenes.config4(scene=k)[0] for k in range(12)])
+    else:
+        sc = scenes.pack(scenes.config5(32))
+    lock = engine_factory(sc)
+    lock.set_step_mode(adaptive=True, asynchronous=False)
+    runs = []
+    for graph, look in ((True, 1), (True, 0), (False, 1)):
+        e = engine_factory(sc)
+        e.set_step_mode(adaptive=True, lookahead=look, asynchronous=True)
+        e.use_graph(graph)
+        runs.append(e)
+    assert int(runs[0].helper_counters()[:, 0].max()) == 0  # no helpers: asynchronous stepping applies
+    for chunk in (5, 1, 2, 7):
+        lock.step(chunk)
+        ref = lock.get_state()
+        for e in runs:
+            e.step(chunk)
+            for a, b in zip(e.get_state(), ref):
+                assert np.array_equal(a, b), chunk
+            assert np.array_equal(e.stats(), lock.stats()), chunk
+            assert np.array_equal(e.solver_counters(), lock.solver_counters()), chunk
+            assert np.array_equal(e.joint_state()[0], lock.joint_state()[0]), chunk
+
+
 @pytest.mark.parametrize("name", ["tie", "c3x8"])
 def test_broad_phase_tree_in_hbm_matches_tree_in_lds(engine_factory, name):
     """The step's broad phase traverses a copy of each scene's tree in LDS when the
