@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -419,7 +420,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     }
     // per-body scene, local AABB, max edge, unique edges (per distinct mesh)
     std::vector<int> body_scene(N);
-    std::vector<double> lbox(6 * size_t(N), 0.0), max_edge(N, 0.0);
+    std::vector<double> lbox(6 * size_t(N), 0.0), max_edge(N, 0.0), vbound(N, 0.0);
     ctx->e_off.assign(N, 0);
     ctx->ne.assign(N, 0);
     ctx->edges_host.clear();
@@ -447,6 +448,9 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
                 mx[a] = std::max(mx[a], V[3 * v + a]);
             }
         for (int a = 0; a < 3; ++a) { lbox[6 * b + a] = mn[a]; lbox[6 * b + 3 + a] = mx[a]; }
+        double vb = 0.0;
+        for (int v = 0; v < 3 * nvb; ++v) vb = std::isfinite(V[v]) ? std::max(vb, std::fabs(V[v])) : INFINITY;
+        vbound[b] = vb;
         const int* F = dsc->faces + 3 * size_t(ctx->f_off[b]);
         double me = 0.0;
         for (int f = 0; f < nfb; ++f)
@@ -543,6 +547,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     d.edges = dupload(ctx, ctx->edges_host.data(), ctx->edges_host.size());
     d.lbox = dupload(ctx, lbox.data(), lbox.size());
     d.max_edge = dupload(ctx, max_edge.data(), N);
+    d.vbound = dupload(ctx, vbound.data(), N);
     ctx->mass_dev = dupload(ctx, dsc->mass, N);
     d.mass = ctx->mass_dev;
     d.inertia = dupload(ctx, dsc->inertia, 9 * size_t(N));
@@ -626,6 +631,12 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     w.pair_off = dalloc<int>(ctx, S + 1);
     w.pairs = dalloc<int>(ctx, 2 * PC);
     w.pair_scene = dalloc<int>(ctx, PC);
+    w.ccd_list = dalloc<int>(ctx, PC);
+    w.ccd_nlist = dalloc<int>(ctx, 1);
+    {
+        const char* env = std::getenv("NDIPC_CCD_EVAL_ALL");
+        w.ccd_bound = (env && env[0] == '1') ? -1.0 : 1e50;
+    }
     w.n_pairs_total = dalloc<int>(ctx, 1);
     w.prox_cnt = dalloc<int>(ctx, S);
     w.prox_off = dalloc<int>(ctx, S + 1);

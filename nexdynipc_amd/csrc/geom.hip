@@ -1891,29 +1891,62 @@ void launch_ccd_batch(int kind, int n, const double* pa, const double* pb, const
 
 // CCDSystem::computeEarliestTOI (CCD.h:22-82) over the FV candidates of every close
 // pair: vertex body moves p -> p + v dt with frozen rotation, face body likewise.
-// Each candidate starts from earliest = 1; the scene result is the minimum.
-// Work units of one wavefront: (close pair, chunk of 64 of its at most 2048 face-vertex
-// candidates), persistent over the units. A unit loads the pair's two step poses once
-// and its lanes take the chunk's candidates (faces of A x vertices of B, then faces of
-// B x vertices of A, from the top-K lists); units beyond a pair's candidate count end
-// after one check.
-constexpr int kCcdChunks = 2 * kTopK * kTopK / 64;
-__global__ void k_ccd_fv(Dev d, Work w, int gate_any) {
+// Each candidate starts from earliest = 1; the scene result is the minimum. A
+// face-vertex query never reports an impact (ccd_toi_face_vertex), so a step's outcome
+// is only whether some query's pre-check evaluation meets a NaN.
+//
+// Screen (one thread per close pair): that evaluation is finite whenever every input
+// is finite and at most 1e50 in magnitude (the two bodies' p, q, v, the step's dt and
+// their mesh vertices). It is interval arithmetic (sums, and products with [-1, 1],
+// [0, 1] or [0, 1e-8]) over the slerp of q with itself, the rotation q v q* (no
+// division) and p + v dt: the slerp weights of a finite q are finite (linear weights
+// when |q|^2 >= 1 - 2^-52, else sin ratios with sin(acos|q|^2) >= 2e-8), and every
+// intermediate stays below ~1e160, far from overflow, so no inf or NaN can arise. Only
+// pairs with an input outside that bound are listed for the evaluation itself; the
+// outcome is that of evaluating every query.
+// (w.ccd_bound is that bound, or negative to list every pair: NDIPC_CCD_EVAL_ALL=1 at
+// upload.)
+__device__ __forceinline__ bool ccd_inputs_bounded(const Dev& d, int b, double dt, double bound) {
+    bool ok = d.vbound[b] <= bound && dt <= bound;
+    for (int k = 0; k < 3; ++k) ok = ok && fabs(d.p[3 * b + k]) <= bound && fabs(d.v[3 * b + k]) <= bound;
+    for (int k = 0; k < 4; ++k) ok = ok && fabs(d.q[4 * b + k]) <= bound;
+    return ok;  // false for NaN
+}
+
+__global__ void k_ccd_reset(Work w) { *w.ccd_nlist = 0; }
+
+__global__ void k_ccd_screen(Dev d, Work w, int gate_any) {
     const int total = *w.n_pairs_total;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+        const int* k = w.lcnt + p * 6;
+        if (k[4] * k[1] + k[5] * k[0] == 0) continue;
+        const SceneParams& sp = d.sp[w.pair_scene[p]];
+        if (gate_any && !(sp.enable_ccd && sp.ccd_any)) continue;
+        if (as_skip(w, sp)) continue;
+        if (ccd_inputs_bounded(d, w.pairs[2 * p], sp.dt, w.ccd_bound) &&
+            ccd_inputs_bounded(d, w.pairs[2 * p + 1], sp.dt, w.ccd_bound))
+            continue;
+        w.ccd_list[atomicAdd(w.ccd_nlist, 1)] = p;
+    }
+}
+
+// The evaluation over the listed pairs. Work units of one wavefront: (listed pair, chunk
+// of 64 of its at most 2048 face-vertex candidates), persistent over the units. A unit
+// loads the pair's two step poses once and its lanes take the chunk's candidates (faces
+// of A x vertices of B, then faces of B x vertices of A, from the top-K lists).
+constexpr int kCcdChunks = 2 * kTopK * kTopK / 64;
+__global__ void k_ccd_fv(Dev d, Work w) {
+    const int total = *w.ccd_nlist;
     const int lane = threadIdx.x & 63;
     const long long wid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
     for (long long u = wid; u < (long long)total * kCcdChunks; u += nw) {
-        const int p = int(u / kCcdChunks), chunk = int(u % kCcdChunks);
+        const int p = w.ccd_list[int(u / kCcdChunks)], chunk = int(u % kCcdChunks);
         const int* k = w.lcnt + p * 6;
         const int k0 = k[0], k1 = k[1], n1 = k[4] * k1, n2 = k[5] * k0;
-        if (chunk * 64 >= n1 + n2) continue;
-        const int s = w.pair_scene[p];
-        SceneParams& sp = d.sp[s];
-        if (gate_any && !(sp.enable_ccd && sp.ccd_any)) continue;
-        if (as_skip(w, sp)) continue;
         const int local = chunk * 64 + lane;
         if (local >= n1 + n2) continue;
+        SceneParams& sp = d.sp[w.pair_scene[p]];
         const int A = w.pairs[2 * p], B = w.pairs[2 * p + 1];
         const double dt = sp.dt;
         const bool first = local < n1;
@@ -1928,19 +1961,20 @@ __global__ void k_ccd_fv(Dev d, Work w, int gate_any) {
         q.a[0] = ld3(d.verts, d.v_off[vb] + vl);
         const int* f = d.faces + 3 * (d.f_off[fb] + fl);
         for (int m = 0; m < 3; ++m) q.b[m] = ld3(d.verts, d.v_off[fb] + f[m]);
-        // face-vertex queries never report an impact (ccd_toi_face_vertex); a NaN in
-        // the pre-check flags the step
         if (ccd_toi_face_vertex(q, 1e-8) < 0) atomicOr(&sp.solve_fail, 2);
     }
 }
 
 void launch_ccd_earliest(const Dev& d, const Work& w, int max_pairs, hipStream_t st) {
+    timing_begin("ccd_fv", st);
+    hipLaunchKernelGGL(k_ccd_reset, dim3(1), dim3(1), 0, st, w);
+    const int gs = std::max(1, std::min((max_pairs + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_ccd_screen, dim3(gs), dim3(256), 0, st, d, w, 1);
     long long g = ((long long)max_pairs * kCcdChunks * 64 + 255) / 256;  // a wavefront per unit
     const long long cap = w.grid_ccd > 0 ? w.grid_ccd : 4096;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
-    timing_begin("ccd_fv", st);
-    hipLaunchKernelGGL(k_ccd_fv, dim3((unsigned)g), dim3(256), 0, st, d, w, 1);
+    hipLaunchKernelGGL(k_ccd_fv, dim3((unsigned)g), dim3(256), 0, st, d, w);
     timing_end("ccd_fv", st);
 }
 

@@ -123,6 +123,7 @@ struct Dev {
     const int* edges;      // pool, 2 per edge
     const double* lbox;    // [N*6] local AABB (min, max)
     const double* max_edge;// [N]
+    const double* vbound;  // [N] largest |coordinate| of the body's mesh vertices (+inf if one is not finite)
     const double* mass;    // [N]
     const double* inertia; // [N*9]
     // joints
@@ -167,6 +168,10 @@ struct Work {
     // the prefix's stateful kernels act only on scenes starting a step (as_state 0), the
     // write-back only on scenes whose ALM loop ended (as_state 1, alm_done)
     int as_mode;
+    // face-vertex CCD: the close pairs whose queries are evaluated (the screen's list)
+    int* ccd_list;   // [pair capacity]
+    int* ccd_nlist;  // [1]
+    double ccd_bound;  // the screen's input bound (1e50; negative: every pair is evaluated)
     HelperBox* hbox;         // [S]
     HelperCfg* hcfg;         // [1] wait bounds of the helper protocol
     int* hsingle;            // [N] with helpers: 1 if the body is alone in its component

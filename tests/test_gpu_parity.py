@@ -422,6 +422,30 @@ def test_broad_phase_tree_in_hbm_matches_tree_in_lds(engine_factory, name):
     assert np.array_equal(ref[1], b.stats())
 
 
+@pytest.mark.parametrize("name", ["c3x48", "c5x16"])
+def test_face_vertex_ccd_screen_matches_full_evaluation(engine_factory, name):
+    """The face-vertex CCD evaluates the pre-check of the close pairs whose inputs are
+    not all finite and at most 1e50 in magnitude; below that bound the evaluation is
+    provably free of NaN. With NDIPC_CCD_EVAL_ALL=1 at upload every close pair's queries
+    are evaluated: same steps, same bits, same statistics."""
+    sc = scenes.pack(scenes.config3(n_scenes=48)) if name == "c3x48" else scenes.pack(scenes.config5(16))
+    a = engine_factory(sc)
+    os.environ["NDIPC_CCD_EVAL_ALL"] = "1"
+    try:
+        b = engine_factory(sc)
+    finally:
+        del os.environ["NDIPC_CCD_EVAL_ALL"]
+    n_fv = 0.0
+    for k in range(6):
+        a.step(5)
+        b.step(5)
+        for x, y in zip(a.get_state(), b.get_state()):
+            assert np.array_equal(x, y), k
+        assert np.array_equal(a.stats(), b.stats()), k
+        n_fv += a.stats()[:, 3].sum()
+    assert n_fv > 0  # face-vertex candidates were screened
+
+
 @pytest.mark.parametrize("name", ["tie", "c3x8"])
 def test_line_search_helpers_match_in_group_trials(engine_factory, name):
     """In small batches (1 and 8 scenes here) each scene's line-search trials, slopes,
