@@ -316,6 +316,11 @@ int32_t ndipc_set_step_mode(ndipc_ctx* ctx, int32_t adaptive, int32_t lookahead)
  * out_ms: ms per step with them). Results of a step do not depend on the shapes. No
  * reference counterpart. */
 int32_t ndipc_autotune(ndipc_ctx* ctx, const double* dt, int32_t trial_steps, int32_t* out4, double* out_ms);
+/* Sets the same four launch shapes directly (grids4: top-K, narrow phase, scene-min flag,
+ * CCD work-group counts; 0 keeps the default), e.g. shapes tuned on another context at
+ * a later state of the same workload. Kept until the next upload. No reference
+ * counterpart. */
+int32_t ndipc_set_launch_grids(ndipc_ctx* ctx, const int32_t* grids4);
 int32_t ndipc_timing_get(ndipc_ctx* ctx, const char* kernel, double* avg_ms, int32_t* count);
 /* Directed-rounding primitives of the interval CCD, evaluated on the device:
  * out6 = add_dn, add_up, sub_dn, sub_up, mul_dn, mul_up. */

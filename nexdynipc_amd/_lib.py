@@ -56,7 +56,7 @@ EXPORTS = [
     "ndipc_scene_min_distance_flag", "ndipc_barrier_eval", "ndipc_friction_eval", "ndipc_ccd_toi_batch",
     "ndipc_ccd_earliest_toi", "ndipc_pcg_solve", "ndipc_timing_enable", "ndipc_timing_get", "ndipc_round_ops", "ndipc_elemfn", "ndipc_solver_counters", "ndipc_set_joint_state",
     "ndipc_set_mass", "ndipc_adjoint_forward", "ndipc_adjoint_backward", "ndipc_use_graph",
-    "ndipc_autotune", "ndipc_helper_counters", "ndipc_helper_config",
+    "ndipc_autotune", "ndipc_set_launch_grids", "ndipc_helper_counters", "ndipc_helper_config",
     "ndipc_set_step_mode", "ndipc_pcg_solve_noalloc", "ndipc_last_residuals", "ndipc_ipc_barrier_eval",
     "ndipc_ipc_max_step", "ndipc_ipc_step",
 ]
@@ -124,6 +124,7 @@ def library():
         "ndipc_use_graph": ([vp, C.c_int32], C.c_int32),
         "ndipc_set_step_mode": ([vp, C.c_int32, C.c_int32], C.c_int32),
         "ndipc_autotune": ([vp, f64p, C.c_int32, i32p, f64p], C.c_int32),
+        "ndipc_set_launch_grids": ([vp, i32p], C.c_int32),
         "ndipc_timing_get": ([vp, C.c_char_p, f64p, i32p], C.c_int32),
         "ndipc_round_ops": ([vp, f64p, f64p, C.c_int32, f64p], C.c_int32),
         "ndipc_elemfn": ([vp, C.c_int32, f64p, f64p, C.c_int32, f64p], C.c_int32),
@@ -512,6 +513,12 @@ class Engine:
         self._ck(self.L.ndipc_autotune(self.ctx, ptr(dt, f64p), int(trial_steps), ptr(out, i32p), C.byref(ms)),
                  "autotune")
         return out, ms.value
+
+    def set_launch_grids(self, grids):
+        """Sets the four launch shapes of autotune directly (0 keeps a default)."""
+        g = np.ascontiguousarray(grids, dtype=np.int32)
+        assert g.shape == (4,)
+        self._ck(self.L.ndipc_set_launch_grids(self.ctx, ptr(g, i32p)), "set_launch_grids")
 
     def use_graph(self, on=True):
         """Replay the captured step graph (default) or enqueue kernels one by one."""

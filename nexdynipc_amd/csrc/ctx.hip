@@ -954,12 +954,17 @@ static int capture_step(ndipc_ctx* ctx, int alm_max) {
 }
 
 // Waits (host) until round `seq` has reported; *all_done: every scene's ALM loop ended.
+// Only the latest round's report is kept, and the device may be rounds ahead of the
+// host; every round issued since the step began belongs to it, and once a round reports
+// every loop ended so do all later rounds of the step (a loop that ended stays ended), so
+// the latest report answers for round `seq` too (if the latest says not ended, neither
+// had round seq).
 static int wait_round(ndipc_ctx* ctx, unsigned int seq, int* all_done) {
     volatile unsigned int* f = ctx->hflag;
     for (long long spin = 0;; ++spin) {
         const unsigned int v = __atomic_load_n(&f[0], __ATOMIC_ACQUIRE);
         if (int(v - seq) >= 0) {
-            *all_done = v == seq ? int(__atomic_load_n(&f[1], __ATOMIC_RELAXED)) : 0;
+            *all_done = int(__atomic_load_n(&f[1], __ATOMIC_ACQUIRE));
             return 0;
         }
         if ((spin & 0xffff) == 0xffff) {  // a stream that has ended or failed will not report
@@ -1922,6 +1927,20 @@ int32_t ndipc_autotune(ndipc_ctx* ctx, const double* dt, int32_t trial_steps, in
     for (int k = 0; k < 4; ++k)
         if (out4) out4[k] = *knob[k];
     if (out_ms) *out_ms = best_ms;
+    return 0;
+}
+
+int32_t ndipc_set_launch_grids(ndipc_ctx* ctx, const int32_t* grids4) {
+    if (check_ctx(ctx)) return -1;
+    if (!grids4) return fail(ctx, "set_launch_grids: null grids");
+    for (int k = 0; k < 4; ++k)
+        if (grids4[k] < 0 || grids4[k] > (1 << 20)) return fail(ctx, "set_launch_grids: work-group count out of range");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->w.grid_topk = grids4[0];
+    ctx->w.grid_np = grids4[1];
+    ctx->w.grid_smf = grids4[2];
+    ctx->w.grid_ccd = grids4[3];
+    drop_graph(ctx);  // recaptured with these shapes
     return 0;
 }
 
