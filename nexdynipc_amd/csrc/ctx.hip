@@ -684,6 +684,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
     w.flag = dalloc<int>(ctx, S);
     w.bcnt = dalloc<int>(ctx, 2 * size_t(N));
     w.boff = dalloc<int>(ctx, 2 * size_t(N));
+    w.bp_cache = dalloc<int>(ctx, size_t(N) * nd::kBpCache);
     // meshes above the LDS-resident size: per work-group HBM slabs of the top-K and
     // scene-min kernels (their grids are kept within the slab counts at launch)
     {

@@ -350,7 +350,12 @@ __device__ void lbvh_query_lds(const BvhLds& T, int m, int i, const Box3& bi, do
 // Grid (scene, chunk of kBlock bodies). With `lds` the work-group first copies its
 // scene's tree into dynamic LDS (kBvhLdsPerLeaf bytes per leaf, sized for the largest
 // scene at launch), followed by the lanes' stacks and hit buffers of nhits entries;
-// otherwise the traversal reads the tree in HBM.
+// otherwise the traversal reads the tree in HBM. The counting pass (pass 0) keeps each
+// body's candidate and prox hits in its w.bp_cache slot (candidates from the front, prox
+// from the back) when they fit in kBpCache entries; pass 1 then emits them (each list
+// by j descending, the traversal's order) without traversing again, and traverses only
+// for bodies whose hits did not fit (a work-group copies the tree only if one of its
+// bodies does).
 template <bool lds>
 __global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass, int nhits) {
     extern __shared__ __align__(16) unsigned char bp_lds[];
@@ -359,8 +364,12 @@ __global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass, int
     const int li = blockIdx.y * blockDim.x + threadIdx.x;
     if (blockIdx.y * blockDim.x >= nb) return;  // the whole work-group: no body of this chunk
     const int m = w.nleaves[s];
+    bool cached = false;  // pass 1: this body's hits are in its cache slot
+    if (pass && li < nb) cached = w.bcnt[b0 + li] + w.bcnt[d.N + b0 + li] <= kBpCache;
+    const bool trav = !pass || (li < nb && d.nv[b0 + li] > 0 && !cached);
+    const int any_trav = __syncthreads_or(trav);
     BvhLds T{};
-    if (lds) {
+    if (lds && any_trav) {
         T.lbox = reinterpret_cast<double*>(bp_lds);
         T.nbox = T.lbox + 6 * m;
         T.child = reinterpret_cast<int*>(T.nbox + 6 * (m > 1 ? m - 1 : 0));
@@ -384,6 +393,20 @@ __global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass, int
     if (li >= nb) return;
     const int i = b0 + li;
     SceneParams& sp = d.sp[s];
+    int* cache = w.bp_cache + size_t(i) * kBpCache;
+    if (cached) {
+        const int nc = w.bcnt[i], np = w.bcnt[d.N + i];
+        const int oc = w.pair_off[s] + w.boff[i], op = w.prox_off[s] + w.boff[d.N + i];
+        for (int k = 0; k < nc; ++k) {
+            bp_sort_insert(w.pairs, oc, k, i, cache[k]);
+            w.pair_scene[oc + k] = s;
+        }
+        for (int k = 0; k < np; ++k) {
+            bp_sort_insert(w.prox_pairs, op, k, i, cache[kBpCache - 1 - k]);
+            w.prox_scene[op + k] = s;
+        }
+        return;
+    }
     int nc = 0, np = 0;
     if (d.nv[i] > 0) {
         const double r0 = dmax(0.0, sp.dhat_cand);
@@ -416,6 +439,8 @@ __global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass, int
                         bp_sort_insert(w.pairs, o, nc, i, j);
                     }
                     w.pair_scene[o + nc] = s;
+                } else if (nc + np < kBpCache) {
+                    cache[nc] = j;
                 }
                 ++nc;
             }
@@ -432,6 +457,8 @@ __global__ void __launch_bounds__(kBlock) k_bp_step(Dev d, Work w, int pass, int
                             bp_sort_insert(w.prox_pairs, o, np, i, j);
                         }
                         w.prox_scene[o + np] = s;
+                    } else if (nc + np < kBpCache) {
+                        cache[kBpCache - 1 - np] = j;
                     }
                     ++np;
                 }

@@ -27,6 +27,7 @@ constexpr int kHitWords = 48;         // 3072 candidate bits per pair (2x1024 FV
 constexpr int kSceneScratch = 65536;  // per-scene scratch doubles (sort keys, barrier b'')
 constexpr int kDenseBodies = 64;      // largest coupled component factored directly
 constexpr int kDenseRows = 6 * kDenseBodies;
+constexpr int kBpCache = 128;         // broad-phase hits of a body kept by the counting pass
 constexpr int kLsBatch = 20;          // line-search trials evaluated together, at most
 constexpr int kLsBatchLarge = 8;      // ... for scenes whose 20 trials need several LDS rounds
 
@@ -228,6 +229,7 @@ struct Work {
     unsigned int* round_seq; // [1] ALM rounds ended by k_round_flag since upload
     int* bcnt;               // [2N] per-body pair counts (candidates, prox)
     int* boff;               // [2N] per-body offsets within the scene
+    int* bp_cache;           // [N * kBpCache] the counting pass's pair and prox hits of each body
 };
 
 // Launch wrappers (defined in the .hip files).
