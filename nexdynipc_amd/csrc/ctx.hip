@@ -626,6 +626,7 @@ int32_t ndipc_upload_scenes(ndipc_ctx* ctx, const ndipc_scene_desc* dsc) {
                                 nd::kHelperStraggleTicks};
         w.hcfg = dupload(ctx, &cfg, 1);
         w.round_seq = dalloc<unsigned int>(ctx, 1);
+        w.post_count = dalloc<unsigned int>(ctx, 1);
     }
     w.pair_cnt = dalloc<int>(ctx, S);
     w.pair_off = dalloc<int>(ctx, S + 1);
@@ -868,7 +869,7 @@ static int enqueue_prefix(ndipc_ctx* ctx, const nd::Adj* rec, int k) {
     return 0;
 }
 
-// One ALM round; with `flag`, k_round_flag reports to the host whether every scene's
+// One ALM round; with `flag`, the ALM post-pass reports to the host whether every scene's
 // ALM loop has ended.
 static int enqueue_round(ndipc_ctx* ctx, const nd::Adj* rec, int k, int a, bool flag) {
     const Dev& d = ctx->d;
@@ -877,10 +878,10 @@ static int enqueue_round(ndipc_ctx* ctx, const nd::Adj* rec, int k, int a, bool 
     if (rec) nd::launch_adj_snapshot(d, w, *rec, k * rec->A + a, st);
     nd::launch_newton(d, w, st);
     if (rec) nd::launch_adj_snapshot_post(d, w, *rec, k * rec->A + a, st);
-    nd::launch_pose_from_state(d, w, w.x, st);
-    nd::launch_narrow_phase(d, w, max_grid_pairs(ctx), true, st);
-    nd::launch_alm_post(d, w, st);
-    if (flag) nd::launch_round_flag(d, w, ctx->hflag, st);
+    // (k_newton leaves the poses of its iterate for the narrow phase; the ALM post-pass
+    // gathers the capped contacts first and reports the round)
+    nd::launch_narrow_phase(d, w, max_grid_pairs(ctx), true, st, false);
+    nd::launch_alm_post(d, w, st, true, flag ? ctx->hflag : nullptr);
     return 0;
 }
 

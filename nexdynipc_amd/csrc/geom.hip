@@ -1412,16 +1412,8 @@ __global__ void __launch_bounds__(kBlock) k_np_write(Dev d, Work w, int vv, int 
 // (distance, position) pairs by the wave, as the top-K lists do (tk_sort_blocks): the
 // introsort loop's partitions, then its final insertion sort as per-block rank sorts,
 // whose destinations below 2000 give kept[dst] = raw position. k_np_cap gathers them.
-// Per-scene scratch (at most kMaxRawContacts = 40000 entries): in sort_key the
-// distances (320,000 B) and the partition positions lpos, rpos (80,000 B each); in
-// sort_idx the 16-bit positions (80,000 B), the block starts (40,000 B) and kept.
-constexpr size_t kCapLposB = 8 * size_t(kMaxRawContacts);
-constexpr size_t kCapRposB = kCapLposB + 2 * size_t(kMaxRawContacts);
-constexpr size_t kCapStartB = 2 * size_t(kMaxRawContacts);
-constexpr size_t kCapKeptB = (kCapStartB + size_t(kMaxRawContacts) + 255) & ~size_t(255);
-static_assert(kCapRposB + 2 * size_t(kMaxRawContacts) <= sizeof(double) * size_t(kSceneScratch), "cap scratch");
-static_assert(kCapKeptB + sizeof(int) * kMaxContacts <= sizeof(int) * size_t(kSceneScratch), "cap scratch");
-static_assert(kMaxRawContacts <= 65535, "16-bit positions");
+// (The per-scene scratch layout, kCap*B, and the gather, np_cap_gather, are shared with
+// the ALM post-pass, which gathers in the ALM rounds.)
 
 __global__ void __launch_bounds__(kTkLanes) k_np_cap_order(Dev d, Work w, int gate) {
     __shared__ int stk[3 * 64];
@@ -1469,34 +1461,10 @@ __global__ void __launch_bounds__(1024) k_np_cap(Dev d, Work w, int gate) {
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
     if (gate ? sp.alm_done : as_skip(w, sp)) return;
-    const int n = w.raw_cnt[s];
-    if (n <= kMaxContacts) {
-        if (threadIdx.x == 0) sp.n_contacts = n;
-        return;
-    }
-    const int* kept = reinterpret_cast<const int*>(reinterpret_cast<const unsigned char*>(
-                                                       w.sort_idx + (long long)s * kSceneScratch) + kCapKeptB);
-    const long long cb = (long long)s * kMaxRawContacts;
-    // gather the kept contacts into sort_key (free once the order is known), then copy back
-    int* ti = reinterpret_cast<int*>(w.sort_key + (long long)s * kSceneScratch);  // 2000 * 5 ints
-    int* te = ti + 5 * kMaxContacts;                                                  // 2000 * 2 ints
-    double* td = w.sort_key + (long long)s * kSceneScratch + 8192;                   // 2000 * 7 doubles
-    for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
-        const int src = kept[i];
-        for (int k = 0; k < 7; ++k) td[7 * i + k] = w.cdbl[7 * (cb + src) + k];
-        for (int k = 0; k < 5; ++k) ti[5 * i + k] = w.cint[5 * (cb + src) + k];
-        for (int k = 0; k < 2; ++k) te[2 * i + k] = w.cext[2 * (cb + src) + k];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
-        for (int k = 0; k < 7; ++k) w.cdbl[7 * (cb + i) + k] = td[7 * i + k];
-        for (int k = 0; k < 5; ++k) w.cint[5 * (cb + i) + k] = ti[5 * i + k];
-        for (int k = 0; k < 2; ++k) w.cext[2 * (cb + i) + k] = te[2 * i + k];
-    }
-    if (threadIdx.x == 0) sp.n_contacts = kMaxContacts;
+    np_cap_gather(w, sp, s);
 }
 
-void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st) {
+void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st, bool gather) {
     const int cap = w.grid_np > 0 ? w.grid_np : 4096;
     int grid = max_pairs < cap ? max_pairs : cap;
     if (grid < 1) grid = 1;
@@ -1511,7 +1479,7 @@ void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_a
     hipLaunchKernelGGL(k_np_scan, dim3(d.S), dim3(kBlock), 0, st, d, w, 1, gate);
     hipLaunchKernelGGL(k_np_write, dim3(grid), dim3(kBlock), 0, st, d, w, 1, gate);
     hipLaunchKernelGGL(k_np_cap_order, dim3(d.S), dim3(kTkLanes), 0, st, d, w, gate);
-    hipLaunchKernelGGL(k_np_cap, dim3(d.S), dim3(1024), 0, st, d, w, gate);
+    if (gather) hipLaunchKernelGGL(k_np_cap, dim3(d.S), dim3(1024), 0, st, d, w, gate);
 }
 
 // ------------------------------------------------------------ scene min distance

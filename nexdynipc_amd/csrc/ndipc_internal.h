@@ -226,7 +226,8 @@ struct Work {
     unsigned char* smf_slab;
     size_t smf_slab_stride;
     int smf_slab_blocks;
-    unsigned int* round_seq; // [1] ALM rounds ended by k_round_flag since upload
+    unsigned int* round_seq; // [1] ALM rounds reported since upload
+    unsigned int* post_count; // [1] work-groups of the running ALM post-pass that finished
     int* bcnt;               // [2N] per-body pair counts (candidates, prox)
     int* boff;               // [2N] per-body offsets within the scene
     int* bp_cache;           // [N * kBpCache] the counting pass's pair and prox hits of each body
@@ -246,7 +247,8 @@ int geom_configure();  // kernel attributes, once per context creation (no strea
 int geom_configure_smf();  // the scene-min kernel's LDS limit; nonzero on failure
 void launch_topk(const Dev& d, const Work& w, int max_pairs, hipStream_t st);
 void launch_pose_from_state(const Dev& d, const Work& w, const double* x, hipStream_t st);
-void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st);
+void launch_narrow_phase(const Dev& d, const Work& w, int max_pairs, bool gate_alm, hipStream_t st,
+                         bool gather = true);
 void launch_scene_min_flag(const Dev& d, const Work& w, int max_prox, hipStream_t st, int ungated = 0);
 void launch_ccd_earliest(const Dev& d, const Work& w, int max_pairs, hipStream_t st);
 void launch_ccd_batch(int kind, int n, const double* pa, const double* pb, const double* a, const double* b,
@@ -318,14 +320,16 @@ void launch_newton(const Dev& d, const Work& w, hipStream_t st);
 // one step, to run beside the step's kernels
 void launch_newton_helpers(const Dev& d, const Work& w, hipStream_t st);
 void launch_helpers_close(const Dev& d, const Work& w, hipStream_t st);
-void launch_alm_post(const Dev& d, const Work& w, hipStream_t st);
+// The ALM post-pass of a round; with gather, the narrow phase's contact gather first
+// (np_cap_gather), and with hflag the round's report to the host (round number, every
+// scene's ALM loop ended) by the last work-group to finish.
+void launch_alm_post(const Dev& d, const Work& w, hipStream_t st, bool gather, unsigned int* hflag);
 void launch_finalize(const Dev& d, const Work& w, hipStream_t st);
 // asynchronous stepping: every scene has n_steps steps to take, starting one
 void launch_async_begin(const Dev& d, const Work& w, int n_steps, hipStream_t st);
 // asynchronous stepping: the write-back of the scenes whose ALM loop ended, then their
 // state (the next step, or done)
 void launch_async_advance(const Dev& d, const Work& w, hipStream_t st);
-void launch_round_flag(const Dev& d, const Work& w, unsigned int* hflag, hipStream_t st);
 void launch_barrier_init_ccd(const Dev& d, const Work& w, hipStream_t st);
 // which 0: DistanceBarrierForm, 1: FrictionForm, on the contacts stored for scene s
 void launch_forms_eval(const Dev& d, const Work& w, int s, int which, double dhat, double kappa, double mu,
@@ -428,6 +432,50 @@ __device__ __forceinline__ double allowed_mu(const Dev& d, int s, int ida, int i
     return -1.0;
 }
 // World position of body-local vertex lv of global body b at pose (p, q).
+// Truncation of a scene's raw contacts to the 2000 that std::sort by distance leaves in
+// front (IPCSolver.cpp:571-577). Per-scene scratch (at most kMaxRawContacts = 40000
+// entries): in sort_key the distances (320,000 B) and the partition positions lpos, rpos
+// (80,000 B each); in sort_idx the 16-bit positions (80,000 B), the block starts (40,000
+// B) and kept, the raw positions of the kept contacts in their sorted order (written by
+// k_np_cap_order when the raw count exceeds 2000).
+constexpr size_t kCapLposB = 8 * size_t(kMaxRawContacts);
+constexpr size_t kCapRposB = kCapLposB + 2 * size_t(kMaxRawContacts);
+constexpr size_t kCapStartB = 2 * size_t(kMaxRawContacts);
+constexpr size_t kCapKeptB = (kCapStartB + size_t(kMaxRawContacts) + 255) & ~size_t(255);
+static_assert(kCapRposB + 2 * size_t(kMaxRawContacts) <= sizeof(double) * size_t(kSceneScratch), "cap scratch");
+static_assert(kCapKeptB + sizeof(int) * kMaxContacts <= sizeof(int) * size_t(kSceneScratch), "cap scratch");
+static_assert(kMaxRawContacts <= 65535, "16-bit positions");
+
+// The scene's contact count after the narrow phase, and above 2000 the gather of the kept
+// contacts into the first 2000 slots (through sort_key, free once the order is known).
+// Called by every thread of one work-group per scene (any size).
+__device__ __forceinline__ void np_cap_gather(const Work& w, SceneParams& sp, int s) {
+    const int n = w.raw_cnt[s];
+    if (n <= kMaxContacts) {
+        if (threadIdx.x == 0) sp.n_contacts = n;
+        return;
+    }
+    const int* kept = reinterpret_cast<const int*>(reinterpret_cast<const unsigned char*>(
+                                                       w.sort_idx + (long long)s * kSceneScratch) + kCapKeptB);
+    const long long cb = (long long)s * kMaxRawContacts;
+    int* ti = reinterpret_cast<int*>(w.sort_key + (long long)s * kSceneScratch);  // 2000 * 5 ints
+    int* te = ti + 5 * kMaxContacts;                                                  // 2000 * 2 ints
+    double* td = w.sort_key + (long long)s * kSceneScratch + 8192;                   // 2000 * 7 doubles
+    for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
+        const int src = kept[i];
+        for (int k = 0; k < 7; ++k) td[7 * i + k] = w.cdbl[7 * (cb + src) + k];
+        for (int k = 0; k < 5; ++k) ti[5 * i + k] = w.cint[5 * (cb + src) + k];
+        for (int k = 0; k < 2; ++k) te[2 * i + k] = w.cext[2 * (cb + src) + k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMaxContacts; i += blockDim.x) {
+        for (int k = 0; k < 7; ++k) w.cdbl[7 * (cb + i) + k] = td[7 * i + k];
+        for (int k = 0; k < 5; ++k) w.cint[5 * (cb + i) + k] = ti[5 * i + k];
+        for (int k = 0; k < 2; ++k) w.cext[2 * (cb + i) + k] = te[2 * i + k];
+    }
+    if (threadIdx.x == 0) sp.n_contacts = kMaxContacts;
+}
+
 // Kernels of the step prefix that change a scene's per-step state skip, in asynchronous
 // stepping, the scenes that are inside a step or done.
 __device__ __forceinline__ bool as_skip(const Work& w, const SceneParams& sp) { return w.as_mode && sp.as_state != 0; }

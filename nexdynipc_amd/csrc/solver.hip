@@ -2246,6 +2246,13 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
         sp.viol = viol;
         sp.dual = dual;
     }
+    // the poses of the new iterate for the narrow phase that follows
+    // (detectContactsAtState(world, x_new), IPCSolver.cpp:1113)
+    for (int li = threadIdx.x; li < sc.n; li += blockDim.x) {
+        const int b = sc.b0 + li;
+        st3(w.pose_p, b, v3(X[6 * li], X[6 * li + 1], X[6 * li + 2]));
+        st4(w.pose_q, b, apply_theta(ld4(d.q, b), v3(X[6 * li + 3], X[6 * li + 4], X[6 * li + 5])));
+    }
 }
 
 void launch_newton(const Dev& d, const Work& w, hipStream_t st) {
@@ -2436,11 +2443,7 @@ void launch_contacts_setup(const Dev& d, const Work& w, hipStream_t st) {
 
 // After the narrow phase at x_new (IPCSolver.cpp:1112-1165): lagged normal forces from
 // the new contacts, contact residual, ALM break test.
-__global__ void __launch_bounds__(kBlk) k_alm_post(Dev d, Work w) {
-    __shared__ double red[kBlk];
-    const int s = blockIdx.x;
-    SceneParams& sp = d.sp[s];
-    if (sp.alm_done) return;
+__device__ void alm_post_scene(const Dev& d, const Work& w, SceneParams& sp, int s, double* red) {
     ScnCtx sc;
     load_scn(d, s, sc);
     const double* X = w.x;  // global body indexing
@@ -2492,20 +2495,23 @@ __global__ void __launch_bounds__(kBlk) k_alm_post(Dev d, Work w) {
     }
 }
 
-void launch_alm_post(const Dev& d, const Work& w, hipStream_t st) {
-    timing_begin("alm_post", st);
-    hipLaunchKernelGGL(k_alm_post, dim3(d.S), dim3(kBlk), 0, st, d, w);
-    timing_end("alm_post", st);
-}
-
 // End of one ALM round of a host-driven step: whether every scene's ALM loop has ended
 // (IPCSolver.cpp:1160-1165 break, or the iteration cap), written to host-visible memory
 // as (round number since upload, all ended), the number stored last with release, so
-// the host can stop issuing rounds once every scene is done. One work-group.
-__global__ void __launch_bounds__(256) k_round_flag(Dev d, Work w, unsigned int* hflag) {
-    __shared__ int all;
-    if (threadIdx.x == 0) all = 1;
+// the host can stop issuing rounds once every scene is done. Run by the last work-group
+// of k_alm_post to finish (an agent-scope counter, reset by that work-group): every
+// scene's flags are final by then.
+__device__ void round_report(const Dev& d, const Work& w, unsigned int* hflag) {
+    __shared__ int last, all;
+    __syncthreads();  // this work-group's writes are done
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(w.post_count, 1u) == gridDim.x - 1;
+        all = 1;
+    }
     __syncthreads();
+    if (!last) return;
+    __threadfence();
     int mine = 1;
     for (int s = threadIdx.x; s < d.S; s += blockDim.x) {
         const SceneParams& sp = d.sp[s];
@@ -2516,6 +2522,7 @@ __global__ void __launch_bounds__(256) k_round_flag(Dev d, Work w, unsigned int*
     if (!mine) atomicAnd(&all, 0);
     __syncthreads();
     if (threadIdx.x == 0) {
+        *w.post_count = 0u;  // for the next round (no work-group of this launch adds again)
         const unsigned int seq = w.round_seq[0] + 1u;
         w.round_seq[0] = seq;
         __hip_atomic_store(hflag + 1, (unsigned int)all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -2523,8 +2530,24 @@ __global__ void __launch_bounds__(256) k_round_flag(Dev d, Work w, unsigned int*
     }
 }
 
-void launch_round_flag(const Dev& d, const Work& w, unsigned int* hflag, hipStream_t st) {
-    hipLaunchKernelGGL(k_round_flag, dim3(1), dim3(256), 0, st, d, w, hflag);
+__global__ void __launch_bounds__(kBlk) k_alm_post(Dev d, Work w, int gather, unsigned int* hflag) {
+    __shared__ double red[kBlk];
+    const int s = blockIdx.x;
+    SceneParams& sp = d.sp[s];
+    if (!sp.alm_done) {
+        if (gather) {
+            np_cap_gather(w, sp, s);
+            __syncthreads();  // n_contacts and the kept contacts, for load_scn
+        }
+        alm_post_scene(d, w, sp, s, red);
+    }
+    if (hflag) round_report(d, w, hflag);
+}
+
+void launch_alm_post(const Dev& d, const Work& w, hipStream_t st, bool gather, unsigned int* hflag) {
+    timing_begin("alm_post", st);
+    hipLaunchKernelGGL(k_alm_post, dim3(d.S), dim3(kBlk), 0, st, d, w, gather ? 1 : 0, hflag);
+    timing_end("alm_post", st);
 }
 
 // ------------------------------------------------------------------ step prologue
