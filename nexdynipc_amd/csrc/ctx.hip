@@ -1017,11 +1017,33 @@ static int run_step_adaptive(ndipc_ctx* ctx, int alm_max, bool graph) {
 // only on the scenes starting a step; the candidate build, a function of each scene's
 // step-start state, runs for all) and one ALM round with its flag. A scene's kernels and
 // their inputs are those of the lockstep step, so its results are the same bits.
+//
+// The Newton solves of the scenes that were inside a step when the iteration began need
+// nothing from the prefix (the prefix's per-step kernels skip those scenes, and the
+// candidate build writes only pair lists, body boxes and the step's tree, which a Newton
+// solve does not read), so they run on the second stream beside the prefix; the scenes
+// that start a step take their Newton solve after it. The round's narrow phase and ALM
+// post-pass follow both.
+static int enqueue_async_kernels(ndipc_ctx* ctx) {
+    const Dev& d = ctx->d;
+    const Work& w = ctx->w;
+    hipStream_t st = ctx->stream;
+    nd::launch_async_advance(d, w, st);
+    HIPCHK(hipEventRecord(ctx->hfork, st));
+    HIPCHK(hipStreamWaitEvent(ctx->hstream, ctx->hfork, 0));
+    nd::launch_newton(d, w, ctx->hstream, 1);
+    HIPCHK(hipEventRecord(ctx->hjoin, ctx->hstream));
+    if (enqueue_prefix(ctx, nullptr, 0)) return -1;
+    nd::launch_newton(d, w, st, 2);
+    HIPCHK(hipStreamWaitEvent(st, ctx->hjoin, 0));
+    nd::launch_narrow_phase(d, w, max_grid_pairs(ctx), true, st, false);
+    nd::launch_alm_post(d, w, st, true, ctx->hflag);
+    return 0;
+}
+
 static int enqueue_async_iter(ndipc_ctx* ctx) {
     ctx->w.as_mode = 1;  // kernel arguments are copied at launch
-    nd::launch_async_advance(ctx->d, ctx->w, ctx->stream);
-    int rc = enqueue_prefix(ctx, nullptr, 0);
-    if (!rc) rc = enqueue_round(ctx, nullptr, 0, 0, true);
+    const int rc = enqueue_async_kernels(ctx);
     ctx->w.as_mode = 0;
     return rc;
 }

@@ -366,6 +366,24 @@ def test_adaptive_rounds_match_fixed_rounds(engine_factory, name):
             assert np.array_equal(e.joint_state()[0], fixed.joint_state()[0])
 
 
+def test_asynchronous_steps_match_lockstep_steps_c3(engine_factory):
+    """The 256-scene C3 batch through its contact phase in calls of 20 and 25 steps,
+    asynchronous against lockstep: same bits and counters after every call (the Newton
+    solves of scenes inside a step run beside the candidate build of the scenes starting
+    one)."""
+    sc = scenes.pack(scenes.config3())
+    lock = engine_factory(sc)
+    lock.set_step_mode(adaptive=True, asynchronous=False)
+    e = engine_factory(sc)
+    for chunk in (20, 25, 25):
+        lock.step(chunk)
+        e.step(chunk)
+        for a, b in zip(e.get_state(), lock.get_state()):
+            assert np.array_equal(a, b), chunk
+        assert np.array_equal(e.stats(), lock.stats()), chunk
+        assert np.array_equal(e.solver_counters(), lock.solver_counters()), chunk
+
+
 @pytest.mark.parametrize("name", ["c3x48", "c4x12", "c5x32"])
 def test_asynchronous_steps_match_lockstep_steps(engine_factory, name):
     """A multi-step call on a batch without line-search helpers steps the scenes
