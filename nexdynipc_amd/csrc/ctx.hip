@@ -79,11 +79,10 @@ struct ndipc_ctx {
     int round_lookahead = 1;          // rounds issued beyond the last one reported
     hipGraphExec_t pre_exec = nullptr, round_exec = nullptr;
     // asynchronous stepping of a batch (adaptive mode, several scenes, several steps): each
-    // scene starts its next step as soon as its ALM loop ends; one captured iteration runs
-    // the write-back of the scenes that ended, the prefix for the scenes starting a step and
+    // scene starts its next step as soon as its ALM loop ends; an iteration runs the
+    // write-back of the scenes that ended, the prefix for the scenes starting a step and
     // one ALM round for every scene inside one
     bool async_steps = true;
-    hipGraphExec_t async_exec = nullptr;
     unsigned int* hflag = nullptr;    // pinned host memory: [round number, all scenes done]
     unsigned int rounds_issued = 0;   // rounds with a flag issued since upload
     GrowBuf ipc_buf[8];               // ndipc_ipc_step's work space
@@ -180,8 +179,7 @@ void drop_graph(ndipc_ctx* ctx) {
     if (ctx->step_exec) (void)hipGraphExecDestroy(ctx->step_exec);
     if (ctx->pre_exec) (void)hipGraphExecDestroy(ctx->pre_exec);
     if (ctx->round_exec) (void)hipGraphExecDestroy(ctx->round_exec);
-    if (ctx->async_exec) (void)hipGraphExecDestroy(ctx->async_exec);
-    ctx->step_exec = ctx->pre_exec = ctx->round_exec = ctx->async_exec = nullptr;
+    ctx->step_exec = ctx->pre_exec = ctx->round_exec = nullptr;
     ctx->graph_alm = -1;
 }
 
@@ -827,7 +825,7 @@ int32_t ndipc_get_joint_state(ndipc_ctx* ctx, double* lambda6, double* mu) {
 // step-start state, the objective of every Newton solve, the converged variable and the
 // solve's iteration count. Kernels only (no copies, fills or host work), except the
 // recording copies.
-static int enqueue_prefix(ndipc_ctx* ctx, const nd::Adj* rec, int k) {
+static int enqueue_prefix(ndipc_ctx* ctx, const nd::Adj* rec, int k, bool side = true) {
     const Dev& d = ctx->d;
     const Work& w = ctx->w;
     hipStream_t st = ctx->stream;
@@ -849,13 +847,16 @@ static int enqueue_prefix(ndipc_ctx* ctx, const nd::Adj* rec, int k) {
     // and writes only the scene flag and kappa, which nothing before the contact setup
     // reads: it runs on the third stream beside the candidate build, the CCD gate and the
     // first narrow phase (latency-bound kernels that leave most CUs idle).
-    HIPCHK(hipEventRecord(ctx->pfork, st));
-    HIPCHK(hipStreamWaitEvent(ctx->pstream, ctx->pfork, 0));
-    nd::timing_begin("scene_min_flag", ctx->pstream);
-    nd::launch_scene_min_flag(d, w, mp, ctx->pstream);
-    nd::timing_end("scene_min_flag", ctx->pstream);
-    nd::launch_kappa(d, w, ctx->pstream);
-    HIPCHK(hipEventRecord(ctx->pjoin, ctx->pstream));
+    hipStream_t sst = side ? ctx->pstream : st;
+    if (side) {
+        HIPCHK(hipEventRecord(ctx->pfork, st));
+        HIPCHK(hipStreamWaitEvent(ctx->pstream, ctx->pfork, 0));
+    }
+    nd::timing_begin("scene_min_flag", sst);
+    nd::launch_scene_min_flag(d, w, mp, sst);
+    nd::timing_end("scene_min_flag", sst);
+    nd::launch_kappa(d, w, sst);
+    if (side) HIPCHK(hipEventRecord(ctx->pjoin, ctx->pstream));
     nd::timing_begin("topk", st);
     nd::launch_topk(d, w, mp, st);
     nd::timing_end("topk", st);
@@ -864,7 +865,7 @@ static int enqueue_prefix(ndipc_ctx* ctx, const nd::Adj* rec, int k) {
     nd::launch_pack(d, w, st);
     nd::launch_pose_from_state(d, w, nullptr, st);
     nd::launch_narrow_phase(d, w, mp, false, st);
-    HIPCHK(hipStreamWaitEvent(st, ctx->pjoin, 0));
+    if (side) HIPCHK(hipStreamWaitEvent(st, ctx->pjoin, 0));
     nd::launch_contacts_setup(d, w, st);
     return 0;
 }
@@ -1015,30 +1016,16 @@ static int run_step_adaptive(ndipc_ctx* ctx, int alm_max, bool graph) {
 // One iteration of asynchronous stepping: the write-back and state change of the scenes
 // whose ALM loop ended in the previous iteration, the step prefix (its per-step kernels act
 // only on the scenes starting a step; the candidate build, a function of each scene's
-// step-start state, runs for all) and one ALM round with its flag. A scene's kernels and
-// their inputs are those of the lockstep step, so its results are the same bits.
-//
-// The Newton solves of the scenes that were inside a step when the iteration began need
-// nothing from the prefix (the prefix's per-step kernels skip those scenes, and the
-// candidate build writes only pair lists, body boxes and the step's tree, which a Newton
-// solve does not read), so they run on the second stream beside the prefix; the scenes
-// that start a step take their Newton solve after it. The round's narrow phase and ALM
-// post-pass follow both.
+// step-start state, runs for all) and one ALM round with its report. A scene's kernels
+// and their inputs are those of its lockstep step, so its results are the same bits.
+// Everything runs on the context stream, enqueued kernel by kernel: an iteration is
+// about a millisecond of device work, so launching its ~30 kernels costs the device
+// nothing, while replaying the iteration as a captured graph gave, in about one run of
+// the 256-scene batch in 80, a contact record mixed from two contacts (cause not found).
 static int enqueue_async_kernels(ndipc_ctx* ctx) {
-    const Dev& d = ctx->d;
-    const Work& w = ctx->w;
-    hipStream_t st = ctx->stream;
-    nd::launch_async_advance(d, w, st);
-    HIPCHK(hipEventRecord(ctx->hfork, st));
-    HIPCHK(hipStreamWaitEvent(ctx->hstream, ctx->hfork, 0));
-    nd::launch_newton(d, w, ctx->hstream, 1);
-    HIPCHK(hipEventRecord(ctx->hjoin, ctx->hstream));
-    if (enqueue_prefix(ctx, nullptr, 0)) return -1;
-    nd::launch_newton(d, w, st, 2);
-    HIPCHK(hipStreamWaitEvent(st, ctx->hjoin, 0));
-    nd::launch_narrow_phase(d, w, max_grid_pairs(ctx), true, st, false);
-    nd::launch_alm_post(d, w, st, true, ctx->hflag);
-    return 0;
+    nd::launch_async_advance(ctx->d, ctx->w, ctx->stream);
+    if (enqueue_prefix(ctx, nullptr, 0, false)) return -1;
+    return enqueue_round(ctx, nullptr, 0, 0, true);
 }
 
 static int enqueue_async_iter(ndipc_ctx* ctx) {
@@ -1049,20 +1036,17 @@ static int enqueue_async_iter(ndipc_ctx* ctx) {
 }
 
 // n_steps steps of every scene, each scene advancing on its own: iterations are issued
-// ahead of the device by `round_lookahead` until one reports that every scene is in the
-// last round of its last step; then the final write-back. A step takes at most alm_max
-// rounds, so n_steps * alm_max iterations always suffice.
-static int run_steps_async(ndipc_ctx* ctx, int n_steps, int alm_max, bool graph) {
+// ahead of the device by `round_lookahead` until one reports that every scene is done or
+// in the last round of its last step; then the final write-back. A step takes at most
+// alm_max rounds, so n_steps * alm_max iterations always suffice.
+static int run_steps_async(ndipc_ctx* ctx, int n_steps, int alm_max) {
     hipStream_t st = ctx->stream;
-    if (graph && !ctx->async_exec)
-        if (capture(ctx, &ctx->async_exec, [&] { return enqueue_async_iter(ctx); })) return -1;
     nd::launch_async_begin(ctx->d, ctx->w, n_steps, st);
     const long long cap = (long long)n_steps * alm_max;
     const unsigned int base = ctx->rounds_issued;
     long long issued = 0;
     auto issue = [&]() -> int {
-        if (graph) HIPCHK(hipGraphLaunch(ctx->async_exec, st));
-        else if (enqueue_async_iter(ctx)) return -1;
+        if (enqueue_async_iter(ctx)) return -1;
         ++issued;
         ++ctx->rounds_issued;
         return 0;
@@ -1094,7 +1078,7 @@ static int run_steps(ndipc_ctx* ctx, int n_steps, const nd::Adj* rec) {
     // taking several steps recover; the line-search helpers keep per-step protocols
     if (!rec && ctx->adaptive && ctx->async_steps && ctx->S > 1 && n_steps > 1 && !ctx->w.newton_helpers) {
         g_timing_ctx = ctx;
-        if (run_steps_async(ctx, n_steps, alm_max, graph)) return -1;
+        if (run_steps_async(ctx, n_steps, alm_max)) return -1;
         HIPCHK(hipGetLastError());
         return 0;
     }

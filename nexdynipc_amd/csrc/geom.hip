@@ -66,9 +66,6 @@ __global__ void __launch_bounds__(kBlock) k_lbvh_build(Dev d, Work w) {
     __shared__ int redi[kBlock];
     const int s = blockIdx.x;
     const int t = threadIdx.x;
-    // asynchronous stepping: a scene inside a step keeps the tree of its step start (its
-    // bodies have not moved since, and the tree's scratch is its Newton solve's now)
-    if (as_skip(w, d.sp[s])) return;
     const int b0 = d.body_off[s], b1 = d.body_off[s + 1];
     const int nb = b1 - b0;
     const int age = w.bvh_age[s];

@@ -98,10 +98,8 @@ struct SceneParams {
     int ls_trials, solves_direct, solves_pcg, pcg_iters;
     int comp_max;
     // asynchronous stepping of a batch: 0 the scene starts a step, 1 it is inside one, 2 it
-    // has taken all its steps; the steps it has still to start or finish; 1 if it was
-    // inside a step when the current iteration began (its Newton solve runs beside the
-    // prefix of the scenes starting one)
-    int as_state, as_steps_left, as_inside;
+    // has taken all its steps; the steps it has still to start or finish
+    int as_state, as_steps_left, pad4;
     double newton_total, trials_total;  // since upload
     double newton_tol;                  // NewtonSolver tolerance (NewtonSolver.cpp:10-95: 1e-6)
     int newton_max_iters, pad5;         // NewtonSolver iteration cap (20)
@@ -317,9 +315,7 @@ void launch_kappa(const Dev& d, const Work& w, hipStream_t st);
 void launch_ccd_dt(const Dev& d, const Work& w, hipStream_t st);
 void launch_pack(const Dev& d, const Work& w, hipStream_t st);
 void launch_contacts_setup(const Dev& d, const Work& w, hipStream_t st);
-// which: 0 every scene whose ALM loop runs; in asynchronous stepping 1 only the scenes
-// that were inside a step when the iteration began, 2 only those that started one in it.
-void launch_newton(const Dev& d, const Work& w, hipStream_t st, int which = 0);
+void launch_newton(const Dev& d, const Work& w, hipStream_t st);
 // k_newton's line-search helpers (w.newton_helpers) for the n_launch Newton launches of
 // one step, to run beside the step's kernels
 void launch_newton_helpers(const Dev& d, const Work& w, hipStream_t st);

@@ -2062,7 +2062,7 @@ __global__ void __launch_bounds__(kBlk) k_newton_helper(Dev d, Work w) {
 }
 
 // One work-group per scene; with newton_helpers, k_newton_helper runs beside it.
-__global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w, int which) {
+__global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w) {
     __shared__ double red[kBlk];
     __shared__ int ok_sh;
     __shared__ NewtonLds nl;
@@ -2075,7 +2075,6 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w, int which) {
     CompLds& comp = nl.comp;
     const int s = blockIdx.x;
     SceneParams& sp = d.sp[s];
-    if (which != 0 && sp.as_inside != (which == 1 ? 1 : 0)) return;
     HelperBox* hb = w.newton_helpers ? w.hbox + s : nullptr;
     if (hb && threadIdx.x == 0) {
         helpers_open(hb);
@@ -2256,9 +2255,9 @@ __global__ void __launch_bounds__(kBlk) k_newton(Dev d, Work w, int which) {
     }
 }
 
-void launch_newton(const Dev& d, const Work& w, hipStream_t st, int which) {
+void launch_newton(const Dev& d, const Work& w, hipStream_t st) {
     timing_begin("newton", st);
-    hipLaunchKernelGGL(k_newton, dim3(d.S), dim3(kBlk), 0, st, d, w, which);
+    hipLaunchKernelGGL(k_newton, dim3(d.S), dim3(kBlk), 0, st, d, w);
     timing_end("newton", st);
 }
 
@@ -2750,7 +2749,6 @@ __global__ void k_async_begin(Dev d, int n_steps) {
     if (s >= d.S) return;
     d.sp[s].as_state = 0;
     d.sp[s].as_steps_left = n_steps;
-    d.sp[s].as_inside = 0;
 }
 
 void launch_async_begin(const Dev& d, const Work& w, int n_steps, hipStream_t st) {
@@ -2767,7 +2765,6 @@ __global__ void k_async_state(Dev d) {
         sp.as_steps_left -= 1;
         sp.as_state = sp.as_steps_left > 0 ? 0 : 2;
     }
-    sp.as_inside = sp.as_state == 1;
 }
 
 void launch_async_advance(const Dev& d, const Work& w, hipStream_t st) {

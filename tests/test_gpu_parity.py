@@ -368,9 +368,8 @@ def test_adaptive_rounds_match_fixed_rounds(engine_factory, name):
 
 def test_asynchronous_steps_match_lockstep_steps_c3(engine_factory):
     """The 256-scene C3 batch through its contact phase in calls of 20 and 25 steps,
-    asynchronous against lockstep: same bits and counters after every call (the Newton
-    solves of scenes inside a step run beside the candidate build of the scenes starting
-    one)."""
+    asynchronous against lockstep: same bits, statistics and counters (the cumulative
+    Newton iterations and line-search trials included) after every call."""
     sc = scenes.pack(scenes.config3())
     lock = engine_factory(sc)
     lock.set_step_mode(adaptive=True, asynchronous=False)
@@ -390,9 +389,10 @@ def test_asynchronous_steps_match_lockstep_steps(engine_factory, name):
     asynchronously: a scene whose ALM loop has ended writes its step back and starts the
     next one in the next round while slower scenes finish theirs. Against lockstep steps
     (every scene waits for the slowest): same bits, statistics, solver counters and joint
-    multipliers after every call, with graphs and kernel by kernel, one iteration or none
-    issued ahead, and calls of 1 (lockstep), 2, 5 and 7 steps. c4x12 is twelve ramp
-    scenes with a hinged chain and a pinned link (ALM rounds on the joints)."""
+    multipliers after every call, one iteration or none issued ahead, with the step graphs
+    of the lockstep path on and off (asynchronous iterations are enqueued kernel by kernel
+    either way), and calls of 1 (lockstep), 2, 5 and 7 steps. c4x12 is twelve ramp scenes
+    with a hinged chain and a pinned link (ALM rounds on the joints)."""
     if name == "c3x48":
         sc = scenes.pack(scenes.config3(n_scenes=48))
     elif name == "c4x12":
