@@ -208,7 +208,7 @@ struct Work {
     double* D;               // [N*36]
     double* Dinv;            // [N*36]
     double* nf;              // [S*2000] normal forces
-    double* cblk;            // [S*2000*3*36] contact blocks AA, BB, AB (scaled)
+    double* cblk;            // [S*2000*4*36] friction contact blocks AA, BB, AB, BA (scaled by h^2)
     double* cgrad;           // [S*2000*24] barrier gA gB, friction gA gB
     int* cactive;            // [S*2000]
     int* csr_off;            // [N+1 + S] body -> contact entries
